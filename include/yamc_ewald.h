@@ -1,0 +1,207 @@
+/*
+ * yamc_ewald.h -- C ABI of the MI355X Ewald Monte Carlo library
+ * (libyamc_ewald.so, HIP kernels for gfx950 inside).
+ *
+ * The ABI takes plain pointers and sizes only.  Coordinates are in box units
+ * (box = [-0.5, 0.5)^3), energies in kT, charges in e, exactly as in the YAMC
+ * Fortran modules it replaces.  Every entry point returns a status:
+ *   0 = ok, 1 = ERROR_IO, 2 = ERROR_LIMITS, 3 = ERROR_PARAMETER,
+ *   4 = ERROR_INITIALIZATION, 5 = ERROR_WRONG_FUNCTION (error.f90:47-51),
+ *   6 = YAMC_ERROR_DEVICE (HIP runtime failure or no usable device).
+ * A real-space overlap is not an error: it is reported through an int flag,
+ * as the logical `overlap` result of EwaldSumRealSpace_calc.
+ *
+ * Threading: a context is used by one host thread at a time; every call is
+ * ordered on the context's own HIP stream and returns after that stream has
+ * finished the call's work.
+ */
+#ifndef YAMC_EWALD_H
+#define YAMC_EWALD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    YAMC_OK = 0,
+    YAMC_ERROR_IO = 1,
+    YAMC_ERROR_LIMITS = 2,
+    YAMC_ERROR_PARAMETER = 3,
+    YAMC_ERROR_INITIALIZATION = 4,
+    YAMC_ERROR_WRONG_FUNCTION = 5,
+    YAMC_ERROR_DEVICE = 6
+};
+
+/* Arithmetic of the state sums S(k) and the per-move k-space sweep. */
+enum {
+    YAMC_FP64 = 0,  /* plain FP64 everywhere */
+    YAMC_FP32C = 1  /* S(k) kept as FP32 (hi, lo) two-sum pairs, swept in FP32,
+                       FP64 acceptance / constants / periodic resync */
+};
+
+/* Limits of this build. */
+#define YAMC_MAX_MOL_ATOM 4   /* atoms per molecule (SPC/E: 3, ions: 1) */
+#define YAMC_MAX_KMAX 40      /* largest integer |k| cutoff */
+#define YAMC_MAX_LJ_TYPE 20   /* LJTypes.f90:20 MAX_LJ_TYPE */
+
+/*
+ * System description.  The fields are the values the reference modules read
+ * from `parameters` (parameters.f90:355-413), `TAtomicData`
+ * (AtomicData.f90:9-22), `TLJTypes` (LJTypes.f90:31-63), `TMoleculeTable`
+ * (MoleculeTable.f90:9-19) and `TComposition` (composition.f90:10-17).
+ * Arrays are copied at creation and never retained.
+ */
+typedef struct yamc_system_desc {
+    int32_t natom;          /* TAtomicData%natom */
+    int32_t nmol;           /* TMoleculeTable%nmol */
+    int32_t ntype;          /* TLJTypes%NType */
+    int32_t kmax;           /* parameters: kmax */
+    int32_t precision;      /* YAMC_FP64 or YAMC_FP32C */
+    int32_t pad;
+    double box_length;      /* parameters: BoxLength (Angstrom) */
+    double alpha;           /* parameters: alpha (1/box length) */
+    double rmax2;           /* parameters: rmax2 (box units^2) */
+    double xclb;            /* parameters: xclb (Bjerrum length / L) */
+    double kext;            /* parameters: kext = 2 pi xclb / (2 eps_ext + 1) */
+    double dr;              /* parameters: dr (max displacement, box units) */
+    double d_angle;         /* parameters: d_angle (radians) */
+    double lambda_f;        /* parameters: xlambda_f */
+    double lambda_c;        /* parameters: xlambda_c */
+    const double *charge;            /* [natom] TAtomicData%charge */
+    const double *hard_core_angstr;  /* [natom] TAtomicData%hard_core_angstr */
+    const double *mass;              /* [natom] TMolecule%mass per atom */
+    const int32_t *molnum;           /* [natom] molnum_by_atomnum, 1-based */
+    const int32_t *lj_type;          /* [natom] type_by_index, 0 = no LJ */
+    const int32_t *mol_first;        /* [nmol] first_atom, 0-based */
+    const int32_t *mol_last;         /* [nmol] last_atom, 0-based inclusive */
+    const double *lj6;               /* [ntype*ntype] LJ6Tab, (t1-1)*ntype+(t2-1) */
+    const double *lj12;              /* [ntype*ntype] LJ12Tab */
+    const double *four_eps;          /* [ntype*ntype] four_epsilon_tab */
+    const double *sigma6;            /* [ntype*ntype] sigma6_tab */
+    const int32_t *tail_counts;      /* [ntype] comp%mol_numbers(t) (EwaldSumTails.f90:88) */
+} yamc_system_desc;
+
+typedef struct yamc_ctx yamc_ctx;
+
+/* Per-move record (the quantities movemc computes, MC.f90:831-1322). */
+typedef struct yamc_move_rec {
+    int32_t imol, accepted, overlap, pad;
+    double du_real, du_kc, du_klj, du_ext, du_total;
+    double f_old[3], t_old[3], f_new[3], t_new[3];
+    double delta[3];
+    double angle;
+} yamc_move_rec;
+
+/* ---- life cycle -------------------------------------------------------- */
+/* Replaces EwaldSumKSpace_alloc (EwaldSumKSpace.f90:88-149), FourierGrid_init
+ * (FourierGrid.f90:118-189), EwaldSumRealSpace_alloc_full/_partial
+ * (EwaldSumRealSpace.f90:70-131) and mc_alloc (MC.f90:257-379). */
+int yamc_ctx_create(const yamc_system_desc *desc, yamc_ctx **out);
+/* Replaces EwaldSumKSpace_dealloc (:151-194), EwaldSumRealSpace_dealloc_*
+ * (:187-230), clearmc (MC.f90:604-679). */
+int yamc_ctx_destroy(yamc_ctx *ctx);
+/* Device index used by the context, number of k vectors (TFourierGrid%nk). */
+int yamc_ctx_info(const yamc_ctx *ctx, int32_t *device, int32_t *nk);
+
+/* Copies TAtomicData%xx,yy,zz in/out (box units). */
+int yamc_set_positions(yamc_ctx *ctx, const double *xx, const double *yy, const double *zz);
+int yamc_get_positions(yamc_ctx *ctx, double *xx, double *yy, double *zz);
+
+/* FourierGrid_init (FourierGrid.f90:118-189): kx, ky, kz, ip, iq of nk. */
+int yamc_grid(yamc_ctx *ctx, int32_t *kx, int32_t *ky, int32_t *kz, int32_t *ip, int32_t *iq);
+
+/* ---- k space ----------------------------------------------------------- */
+/* EwaldSumKSpace_init (EwaldSumKSpace.f90:197-223): beta, beta6, beta12,
+ * beta_LJ unless scale_beta > 1e-9 (then beta *= scale_beta), then
+ * calc_total_energy (:225-260).  Outputs energy_coulomb and energy_LJ. */
+int yamc_kspace_init(yamc_ctx *ctx, double scale_beta, double *e_coulomb, double *e_lj);
+/* initBeta / initBetaLJ results (:263-386); beta_lj is [ntype*ntype*nk]. */
+int yamc_kspace_get_beta(yamc_ctx *ctx, double *beta, double *beta6, double *beta12, double *beta_lj);
+/* TRhoSquared sums (RhoSquared.f90:9-45) as [(1+ntype)][8][nk]: set 0 is
+ * sumsincos_coulomb, set t is sumsincos_LJ(t); the 8 arrays are cos_ppp,
+ * sin_ppp, cos_ppm, sin_ppm, cos_pmp, sin_pmp, cos_pmm, sin_pmm. */
+int yamc_kspace_get_sums(yamc_ctx *ctx, double *sums);
+/* calc_dU_coulomb + calc_dU_LJ (EwaldSumKSpace.f90:528-753) for molecule
+ * imol (0-based) moved to (x,y,z) [n_m atoms]; the state is not changed. */
+int yamc_kspace_delta(yamc_ctx *ctx, int32_t imol, const double *x, const double *y, const double *z,
+                      double *du_coulomb, double *du_lj);
+/* ForceKSpace_coulomb + ForceKSpace_LJ (ForceKSpace.f90:59-302) for the n
+ * atoms first..first+n-1 placed at (x,y,z), against the current sums. */
+int yamc_kspace_forces(yamc_ctx *ctx, int32_t first, int32_t n, const double *x, const double *y,
+                       const double *z, double *fx, double *fy, double *fz);
+
+/* ---- real space -------------------------------------------------------- */
+/* EwaldSumRealSpace_calc, full sum (EwaldSumRealSpace.f90:233-500, j > i).
+ * comps[6] = uu_ew, uu_lj6, uu_lj12, uu_ew_intra, uu_lj6_intra, uu_lj12_intra;
+ * uu, fx, fy, fz over natom (may be NULL). */
+int yamc_rspace_full(yamc_ctx *ctx, double *comps, double *uu, double *fx, double *fy, double *fz,
+                     int32_t *overlap);
+/* EwaldSumRealSpace_set_molecule + _calc, partial sum of molecule imol at
+ * (x,y,z): uu/F over its n_m atoms, *_back over all natom atoms (may be
+ * NULL), comps[6] as above. */
+int yamc_rspace_partial(yamc_ctx *ctx, int32_t imol, const double *x, const double *y, const double *z,
+                        double *comps, double *uu, double *fx, double *fy, double *fz, double *uu_back,
+                        double *fx_back, double *fy_back, double *fz_back, int32_t *overlap);
+
+/* ---- constant terms ---------------------------------------------------- */
+/* ewald_sum_coulomb_tail and ewald_sum_lj_tails (EwaldSumTails.f90:18-127). */
+int yamc_tails(yamc_ctx *ctx, double *self_coulomb, double *lj6_tail, double *lj12_tail);
+/* EwaldSumExternal_calc_mu (EwaldSumExternal.f90:165-183) over all atoms. */
+int yamc_external_mu(yamc_ctx *ctx, double *mu3);
+
+/* ---- Monte Carlo ------------------------------------------------------- */
+/* mc_calc_total_sums + mc_total_energy (MC.f90:477-585, 681-698) on the
+ * device.  e[10] = U_total, U_total_ew, U_total_lj, uu_ew, uu_lj6, uu_lj12,
+ * E_kC, E_kLJ, U_tail, U_ext. */
+int yamc_mc_init(yamc_ctx *ctx, double *e, int32_t *overlap);
+/* Energies of the current (incrementally updated) state, same layout. */
+int yamc_mc_energies(yamc_ctx *ctx, double *e);
+/* Per-atom real-space energy and force arrays of the state
+ * (RealSpace_TotalSum%uu, Fx, Fy, Fz). */
+int yamc_mc_real_arrays(yamc_ctx *ctx, double *uu, double *fx, double *fy, double *fz);
+/* RNG stream: Philox-4x32-10 keyed by seed, counter (high 64 bits) = stream,
+ * word position = counter; 24-bit uniforms in (0,1) as random.f90 gives. */
+int yamc_mc_set_rng(yamc_ctx *ctx, uint64_t seed, uint64_t stream, uint64_t counter);
+int yamc_mc_get_rng(yamc_ctx *ctx, uint64_t *counter);
+/* run_mc cycles (runmc.f90:204-268; NVT, no exchange), all on the device:
+ * per cycle one pressure-step draw, then nmol steps of (exchange draw,
+ * molecule pick, movemc).  recs (may be NULL) receives ncycles*nmol records.
+ * resync_every > 0 rebuilds S(k) from scratch in FP64 after that many moves
+ * (FP32C form); 0 = never. */
+int yamc_mc_run(yamc_ctx *ctx, int32_t ncycles, int32_t resync_every, yamc_move_rec *recs,
+                int64_t *n_accepted);
+/* movemc with a caller-given molecule and one constant uniform per move
+ * (random.f90:80-82 RND_CONSTANT semantics), for traces against the
+ * reference. */
+int yamc_mc_run_const(yamc_ctx *ctx, int32_t nmoves, const int32_t *imol, const double *u,
+                      yamc_move_rec *recs, int64_t *n_accepted);
+/* Rebuilds S(k) and the k-space energies from the current positions
+ * (calc_total_energy, EwaldSumKSpace.f90:225-260) in FP64. */
+int yamc_resync(yamc_ctx *ctx);
+
+/* ---- batched replicas -------------------------------------------------- */
+/* Advances n independent contexts (uneven sizes allowed) by ncycles each in
+ * one persistent launch, one workgroup per replica.  All contexts must live
+ * on the same device. */
+int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int64_t *n_accepted);
+
+/* ---- forces ------------------------------------------------------------ */
+/* Full-system forces.  f_grad = -dE_total/dr for all atoms (real + reciprocal
+ * + LJ-Ewald + external, intramolecular real-space pairs included), from the
+ * hand-written adjoint; f_ref = the reference force convention
+ * (mc_calc_forces.f90:230-316 with the movemc real-space pairs j > i).
+ * Either pointer triple may be NULL. */
+int yamc_forces(yamc_ctx *ctx, double *gx, double *gy, double *gz, double *rx, double *ry, double *rz);
+
+/* ---- diagnostics ------------------------------------------------------- */
+/* Time of the last yamc_mc_run / yamc_batch_run kernel launch in ms (HIP
+ * events on the context stream). */
+int yamc_last_kernel_ms(yamc_ctx *ctx, double *ms);
+const char *yamc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

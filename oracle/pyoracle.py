@@ -1,0 +1,261 @@
+"""ctypes loader of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests, the smoke check and the benchmark's CPU-baseline leg only;
+the product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+MOVE_DTYPE = np.dtype([
+    ("imol", "<i4"), ("accepted", "<i4"), ("overlap", "<i4"), ("pad", "<i4"),
+    ("du_real", "<f8"), ("du_kc", "<f8"), ("du_klj", "<f8"), ("du_ext", "<f8"), ("du_total", "<f8"),
+    ("f_old", "<f8", (3,)), ("t_old", "<f8", (3,)), ("f_new", "<f8", (3,)), ("t_new", "<f8", (3,)),
+    ("delta", "<f8", (3,)), ("angle", "<f8"),
+])
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("natom", C.c_int32), ("nmol", C.c_int32), ("ntype", C.c_int32), ("kmax", C.c_int32),
+        ("precision", C.c_int32), ("pad", C.c_int32),
+        ("box_length", C.c_double), ("alpha", C.c_double), ("rmax2", C.c_double), ("xclb", C.c_double),
+        ("kext", C.c_double), ("dr", C.c_double), ("d_angle", C.c_double), ("lambda_f", C.c_double),
+        ("lambda_c", C.c_double),
+        ("charge", _DP), ("hard_core_angstr", _DP), ("mass", _DP), ("molnum", _IP), ("lj_type", _IP),
+        ("mol_first", _IP), ("mol_last", _IP), ("lj6", _DP), ("lj12", _DP), ("four_eps", _DP), ("sigma6", _DP),
+        ("tail_counts", _IP),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        _lib.yo_create.restype = C.c_void_p
+        _lib.yo_shi_sans_exp.restype = C.c_double
+        _lib.yo_erfc_luc.restype = C.c_double
+        _lib.yo_erf_luc.restype = C.c_double
+        _lib.yo_rand.restype = C.c_double
+        _lib.yo_rng_counter.restype = C.c_uint64
+        for f in ("yo_erfc_luc", "yo_erf_luc"):
+            getattr(_lib, f).argtypes = [C.c_double, C.c_double, C.c_double]
+        _lib.yo_shi_sans_exp.argtypes = [C.c_double]
+        _lib.yo_erfc_bet6_bet12.argtypes = [C.c_double, C.c_double, C.c_double, _DP, _DP, _DP]
+        for f in ("yo_destroy", "yo_nk", "yo_rand", "yo_rng_counter"):
+            getattr(_lib, f).argtypes = [C.c_void_p]
+        _lib.yo_rng_philox.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64]
+        _lib.yo_rng_const.argtypes = [C.c_void_p, C.c_double]
+        _lib.yo_kspace_init.argtypes = [C.c_void_p, C.c_double]
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_DP)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_IP)
+
+
+class Oracle:
+    """FP64 CPU restatement of one replica (mirror of yamc_amd.ewald.EwaldMC)."""
+
+    def __init__(self, sys):
+        self.lib = load()
+        self.sys = sys
+        d = sys.derived
+        keep = []
+
+        def D(a):
+            arr, p = _d(a if len(a) else np.zeros(1))
+            keep.append(arr)
+            return p
+
+        def I(a):
+            arr, p = _i(a if len(a) else np.zeros(1, np.int32))
+            keep.append(arr)
+            return p
+
+        desc = Desc(natom=sys.natom, nmol=sys.nmol, ntype=sys.ntype, kmax=int(d["kmax"]), precision=0, pad=0,
+                    box_length=sys.params.box_length, alpha=d["alpha"], rmax2=d["rmax2"], xclb=d["xclb"],
+                    kext=d["kext"], dr=d["dr"], d_angle=d["d_angle"], lambda_f=sys.params.lambda_f,
+                    lambda_c=sys.params.lambda_c, charge=D(sys.charge), hard_core_angstr=D(sys.hard_core),
+                    mass=D(sys.mass), molnum=I(sys.molnum), lj_type=I(sys.lj_type), mol_first=I(sys.mol_first),
+                    mol_last=I(sys.mol_last), lj6=D(sys.lj6), lj12=D(sys.lj12), four_eps=D(sys.four_eps),
+                    sigma6=D(sys.sigma6), tail_counts=I(sys.tail_counts))
+        self.h = self.lib.yo_create(C.byref(desc))
+        if not self.h:
+            raise RuntimeError("oracle: yo_create rejected the system")
+        self.nk = self.lib.yo_nk(self.h)
+        self.set_positions(sys.x, sys.y, sys.z)
+
+    def __del__(self):
+        try:
+            self.lib.yo_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_positions(self, x, y, z):
+        (x, px), (y, py), (z, pz) = _d(x), _d(y), _d(z)
+        self.lib.yo_set_positions(C.c_void_p(self.h), px, py, pz)
+
+    def positions(self):
+        n = self.sys.natom
+        x, y, z = np.empty(n), np.empty(n), np.empty(n)
+        self.lib.yo_get_positions(C.c_void_p(self.h), _d(x)[1], _d(y)[1], _d(z)[1])
+        return x, y, z
+
+    def kspace_init(self, scale_beta=0.0):
+        self.lib.yo_kspace_init(C.c_void_p(self.h), C.c_double(scale_beta))
+        a, b = C.c_double(), C.c_double()
+        self.lib.yo_kspace_energy(C.c_void_p(self.h), C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def beta(self):
+        T, nk = self.sys.ntype, self.nk
+        b, b6, b12, bl = np.empty(nk), np.empty(nk), np.empty(nk), np.empty(max(T * T, 1) * nk)
+        self.lib.yo_get_beta(C.c_void_p(self.h), _d(b)[1], _d(b6)[1], _d(b12)[1], _d(bl)[1])
+        return b, b6, b12, bl[: T * T * nk].reshape(T, T, nk)
+
+    def sums(self):
+        s = np.empty((1 + self.sys.ntype) * 8 * self.nk)
+        self.lib.yo_get_sums(C.c_void_p(self.h), _d(s)[1])
+        return s.reshape(1 + self.sys.ntype, 8, self.nk)
+
+    def kspace_delta(self, imol, x, y, z):
+        a, b = C.c_double(), C.c_double()
+        self.lib.yo_kspace_delta(C.c_void_p(self.h), int(imol), _d(x)[1], _d(y)[1], _d(z)[1], C.byref(a),
+                                 C.byref(b))
+        return a.value, b.value
+
+    def kspace_forces(self, first, x, y, z):
+        n = len(x)
+        fx, fy, fz = np.empty(n), np.empty(n), np.empty(n)
+        xs, ys, zs = _d(x), _d(y), _d(z)
+        self.lib.yo_kspace_forces(C.c_void_p(self.h), int(first), n, xs[1], ys[1], zs[1], _d(fx)[1], _d(fy)[1],
+                                  _d(fz)[1])
+        return fx, fy, fz
+
+    def rspace_full(self):
+        n = self.sys.natom
+        comps, uu, fx, fy, fz = np.empty(6), np.empty(n), np.empty(n), np.empty(n), np.empty(n)
+        ov = self.lib.yo_rspace_full(C.c_void_p(self.h), _d(comps)[1], _d(uu)[1], _d(fx)[1], _d(fy)[1], _d(fz)[1])
+        return comps, uu, fx, fy, fz, bool(ov)
+
+    def rspace_partial(self, imol, x, y, z):
+        na, n = len(x), self.sys.natom
+        comps = np.empty(6)
+        uu, fx, fy, fz = (np.empty(na) for _ in range(4))
+        ub, fxb, fyb, fzb = (np.empty(n) for _ in range(4))
+        xs, ys, zs = _d(x), _d(y), _d(z)
+        ov = self.lib.yo_rspace_partial(C.c_void_p(self.h), int(imol), xs[1], ys[1], zs[1], _d(comps)[1],
+                                        _d(uu)[1], _d(fx)[1], _d(fy)[1], _d(fz)[1], _d(ub)[1], _d(fxb)[1],
+                                        _d(fyb)[1], _d(fzb)[1])
+        return dict(comps=comps, uu=uu, fx=fx, fy=fy, fz=fz, uu_back=ub, fx_back=fxb, fy_back=fyb, fz_back=fzb,
+                    overlap=bool(ov))
+
+    def tails(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self.lib.yo_tails(C.c_void_p(self.h), C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def mu(self):
+        m = np.empty(3)
+        self.lib.yo_mu(C.c_void_p(self.h), _d(m)[1])
+        return m
+
+    def mc_init(self):
+        e = np.empty(10)
+        ov = self.lib.yo_mc_init(C.c_void_p(self.h), _d(e)[1])
+        from yamc_amd.ewald import ENERGY_NAMES
+        return dict(zip(ENERGY_NAMES, e)), bool(ov)
+
+    def energies(self):
+        e = np.empty(10)
+        self.lib.yo_mc_energies(C.c_void_p(self.h), _d(e)[1])
+        from yamc_amd.ewald import ENERGY_NAMES
+        return dict(zip(ENERGY_NAMES, e))
+
+    def real_arrays(self):
+        n = self.sys.natom
+        uu, fx, fy, fz = (np.empty(n) for _ in range(4))
+        self.lib.yo_get_real_arrays(C.c_void_p(self.h), _d(uu)[1], _d(fx)[1], _d(fy)[1], _d(fz)[1])
+        return uu, fx, fy, fz
+
+    def set_rng(self, seed, stream=0, counter=0):
+        self.lib.yo_rng_philox(C.c_void_p(self.h), seed, stream, counter)
+
+    def rng_counter(self):
+        return self.lib.yo_rng_counter(C.c_void_p(self.h))
+
+    def run(self, ncycles, records=True):
+        recs = np.zeros(ncycles * self.sys.nmol, dtype=MOVE_DTYPE) if records else None
+        nacc = C.c_int64()
+        self.lib.yo_run_cycles(C.c_void_p(self.h), int(ncycles), recs.ctypes.data_as(C.c_void_p) if records else None,
+                               C.byref(nacc))
+        return nacc.value, recs
+
+    def run_const(self, imol, u):
+        im, pim = _i(imol)
+        uu, pu = _d(u)
+        recs = np.zeros(len(im), dtype=MOVE_DTYPE)
+        acc = self.lib.yo_run_const(C.c_void_p(self.h), len(im), pim, pu, recs.ctypes.data_as(C.c_void_p))
+        return acc, recs
+
+    def forces_refconv(self):
+        n = self.sys.natom
+        fx, fy, fz = np.empty(n), np.empty(n), np.empty(n)
+        self.lib.yo_forces_refconv(C.c_void_p(self.h), _d(fx)[1], _d(fy)[1], _d(fz)[1])
+        return np.stack([fx, fy, fz], 1)
+
+    def sincos_kr(self, x, y, z, q):
+        out = np.empty(8 * self.nk)
+        self.lib.yo_sincos_kr(C.c_void_p(self.h), C.c_double(x), C.c_double(y), C.c_double(z), C.c_double(q),
+                              _d(out)[1])
+        return out.reshape(8, self.nk)
+
+
+def grid(kmax):
+    lib = load()
+    nk = lib.yo_grid_nk(int(kmax))
+    arrs = [np.empty(nk, np.int32) for _ in range(5)]
+    lib.yo_grid_fill(int(kmax), *(a.ctypes.data_as(_IP) for a in arrs))
+    return dict(zip(("kx", "ky", "kz", "ip", "iq"), arrs))
+
+
+def philox(ctr, key):
+    lib = load()
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib.yo_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def erfc_bet6_bet12(b):
+    lib = load()
+    e, b6, b12 = C.c_double(), C.c_double(), C.c_double()
+    lib.yo_erfc_bet6_bet12(b, b * b, float(np.exp(-b * b)), C.byref(e), C.byref(b6), C.byref(b12))
+    return e.value, b6.value, b12.value

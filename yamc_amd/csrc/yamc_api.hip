@@ -1,0 +1,751 @@
+// yamc_api.hip -- C ABI of libyamc_ewald.so (declared in include/yamc_ewald.h).
+//
+// A context owns one replica's device state on its own HIP stream.  Host
+// arrays passed in are copied to the device and never retained; every entry
+// point synchronises its stream before returning, so outputs are ready.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "yamc_device.h"
+
+// kernels (yamc_kernels.hip)
+__global__ void yk_beta(YdChain c, const double *xk2, double *beta6, double *beta12);
+__global__ void yk_beta_lj(YdChain c, const double *lj6, const double *lj12, const double *beta6,
+                           const double *beta12);
+__global__ void yk_scale_beta(YdChain c, double s);
+__global__ void yk_atom_tables(YdChain c, const double *x, const double *y, const double *z, double *tab);
+__global__ void yk_full_sums(YdChain c, const double *tab);
+__global__ void yk_kenergy(YdChain c);
+__global__ void yk_mu(YdChain c, double *mu_out);
+__global__ void yk_tails(YdChain c, const double *lj6, const double *lj12, const int *counts);
+__global__ void yk_rspace_full(YdChain c, const double *lj_sig6, const double *lj_4eps, double *part, int *ov);
+__global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, double *comps_out);
+template <bool F32>
+__global__ void yk_chain(const YdChain *chains, const double *const *sig6s, const double *const *feps_s, int nsteps,
+                         int mode, int max_moves);
+template <bool F32>
+__global__ void yk_kspace_delta(YdChain c, int imol, const double *x, const double *y, const double *z,
+                                double *out);
+template <bool F32>
+__global__ void yk_kspace_forces(YdChain c, int first, int n, const double *x, const double *y, const double *z,
+                                 double *out);
+__global__ void yk_rspace_partial(YdChain c, int imol, const double *x, const double *y, const double *z,
+                                  const double *sig6, const double *feps, double *out, double *uub, double *fxb,
+                                  double *fyb, double *fzb);
+void yk_launch_forces(const YdChain &c, const double *sig6, const double *feps, const double *tab, double *g,
+                      double *r, hipStream_t s);
+
+#define YK_BS 512
+
+static thread_local char g_err[512] = "";
+
+#include <stdarg.h>
+static int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess) return fail(YAMC_ERROR_DEVICE, "HIP error %s (line %d)", hipGetErrorString(e_), __LINE__); \
+    } while (0)
+
+struct yamc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.f;
+    YdChain ch{};
+    int natom = 0, nmol = 0, ntype = 0, nk = 0, kmax = 0, precision = 0;
+    std::vector<int> kx, ky, kz, ip, iq;
+    // device buffers owned by the context
+    std::vector<void *> owned;
+    double *d_lj6 = nullptr, *d_lj12 = nullptr, *d_sig6 = nullptr, *d_4eps = nullptr;
+    int *d_counts = nullptr;
+    double *d_xk2 = nullptr, *d_beta6 = nullptr, *d_beta12 = nullptr, *d_tab = nullptr, *d_part = nullptr;
+    int *d_ov = nullptr;
+    double *d_io = nullptr;  // small staging buffer for single-call results (64 doubles)
+    double *d_mol = nullptr; // staged molecule coordinates (3 * YD_MAXA)
+    YdChain *d_chain = nullptr;
+    const double **d_sig6s = nullptr, **d_feps = nullptr;
+    bool beta_ready = false, state_ready = false;
+};
+
+template <typename T>
+static int dalloc(yamc_ctx *c, T **p, size_t n) {
+    void *q = nullptr;
+    HIPCHK(hipMalloc(&q, n * sizeof(T) + 64));
+    HIPCHK(hipMemsetAsync(q, 0, n * sizeof(T) + 64, c->stream));
+    c->owned.push_back(q);
+    *p = (T *)q;
+    return YAMC_OK;
+}
+
+#define TRY(x)                      \
+    do {                            \
+        int r_ = (x);               \
+        if (r_ != YAMC_OK) return r_; \
+    } while (0)
+
+// FourierGrid.f90:118-189: half-space k list, row flags ip, zero pattern iq.
+static void build_grid(int kmax, std::vector<int> &kx, std::vector<int> &ky, std::vector<int> &kz,
+                       std::vector<int> &ip, std::vector<int> &iq, std::vector<double> &xk2) {
+    double xkmax2 = (double)kmax * kmax;
+    for (int ix = 0; ix <= kmax; ++ix) {
+        double xkx2 = (double)(ix * ix);
+        int kymax = (int)sqrt(xkmax2 - xkx2);
+        for (int iy = 0; iy <= kymax; ++iy) {
+            double xky2 = (double)(iy * iy);
+            int kzmax = (int)sqrt(xkmax2 - xkx2 - xky2);
+            for (int iz = 0; iz <= kzmax; ++iz) {
+                double k2 = xkx2 + xky2 + (double)(iz * iz);
+                if (k2 < 0.1) continue;
+                kx.push_back(ix); ky.push_back(iy); kz.push_back(iz); xk2.push_back(k2);
+                iq.push_back(iy == 0 ? (iz == 0 ? 1 : 3) : (iz == 0 ? 2 : 4));
+                ip.push_back(0);
+            }
+            if (!ip.empty()) ip.back() = 1;
+        }
+        if (!ip.empty()) ip.back() = 2;
+    }
+    if (!ip.empty()) ip.back() = 3;
+}
+
+static int sync(yamc_ctx *c) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return YAMC_OK;
+}
+
+extern "C" const char *yamc_last_error(void) { return g_err; }
+
+extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
+    if (!d || !out) return fail(YAMC_ERROR_PARAMETER, "yamc_ctx_create: null argument");
+    *out = nullptr;
+    if (d->natom < 1 || d->nmol < 1 || d->ntype < 0 || d->kmax < 1)
+        return fail(YAMC_ERROR_PARAMETER, "yamc_ctx_create: bad sizes");
+    if (d->kmax > YAMC_MAX_KMAX) return fail(YAMC_ERROR_LIMITS, "kmax %d exceeds YAMC_MAX_KMAX", d->kmax);
+    if (d->ntype > YAMC_MAX_LJ_TYPE) return fail(YAMC_ERROR_LIMITS, "ntype %d exceeds MAX_LJ_TYPE", d->ntype);
+    if (d->precision != YAMC_FP64 && d->precision != YAMC_FP32C)
+        return fail(YAMC_ERROR_PARAMETER, "unknown precision %d", d->precision);
+    for (int m = 0; m < d->nmol; ++m) {
+        int f = d->mol_first[m], l = d->mol_last[m];
+        if (f < 0 || l < f || l >= d->natom) return fail(YAMC_ERROR_PARAMETER, "bad molecule range %d", m);
+        if (l - f + 1 > YAMC_MAX_MOL_ATOM)
+            return fail(YAMC_ERROR_LIMITS, "molecule %d has more than YAMC_MAX_MOL_ATOM atoms", m);
+        int seen[YAMC_MAX_MOL_ATOM], ns = 0;
+        for (int a = f; a <= l; ++a) {
+            if (d->molnum[a] != m + 1) return fail(YAMC_ERROR_PARAMETER, "molnum mismatch at atom %d", a);
+            int t = d->lj_type[a];
+            if (t < 0 || t > d->ntype) return fail(YAMC_ERROR_PARAMETER, "bad lj_type at atom %d", a);
+            bool dup = false;
+            for (int s = 0; s < ns; ++s) dup |= seen[s] == t;
+            if (t > 0 && !dup) seen[ns++] = t;
+        }
+        if (ns > 2) return fail(YAMC_ERROR_LIMITS, "molecule %d has more than 2 LJ types", m);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(YAMC_ERROR_DEVICE, "no HIP device available");
+    yamc_ctx *c = new yamc_ctx();
+    HIPCHK(hipGetDevice(&c->device));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&c->ev0));
+    HIPCHK(hipEventCreate(&c->ev1));
+    int N = d->natom, T = d->ntype, M = d->nmol;
+    c->natom = N; c->nmol = M; c->ntype = T; c->kmax = d->kmax; c->precision = d->precision;
+    std::vector<double> xk2;
+    build_grid(d->kmax, c->kx, c->ky, c->kz, c->ip, c->iq, xk2);
+    int nk = (int)c->kx.size();
+    c->nk = nk;
+    YdParams &p = c->ch.p;
+    p.natom = N; p.nmol = M; p.ntype = T; p.kmax = d->kmax; p.nk = nk; p.precision = d->precision;
+    p.L = d->box_length; p.alpha = d->alpha; p.rmax2 = d->rmax2; p.xclb = d->xclb; p.kext = d->kext;
+    p.m2kext = -2 * d->kext;  // parameters.f90:404
+    p.dr = d->dr; p.d_angle = d->d_angle; p.lambda_f = d->lambda_f; p.lambda_c = d->lambda_c;
+    p.alpha_over_sqrt_pi = d->alpha / sqrt(YD_PI);  // parameters.f90:400-401
+    p.two_alpha_over_sqrt_pi = 2.0 * p.alpha_over_sqrt_pi;
+    YdChain &h = c->ch;
+    int TT = T * T > 0 ? T * T : 1;
+    int rc = YAMC_OK;
+    auto go = [&]() -> int {
+        TRY(dalloc(c, &h.x, N)); TRY(dalloc(c, &h.y, N)); TRY(dalloc(c, &h.z, N));
+        TRY(dalloc(c, &h.q, N)); TRY(dalloc(c, &h.dh, N)); TRY(dalloc(c, &h.mass, N));
+        TRY(dalloc(c, &h.molnum, N)); TRY(dalloc(c, &h.type, N));
+        TRY(dalloc(c, &h.first, M)); TRY(dalloc(c, &h.last, M));
+        TRY(dalloc(c, &h.kxyz, nk)); TRY(dalloc(c, &h.beta, nk));
+        TRY(dalloc(c, &h.beta_lj, (size_t)TT * nk));
+        size_t nS = (size_t)(1 + T) * 8 * nk;
+        if (d->precision == YAMC_FP32C) {
+            TRY(dalloc(c, &h.Shi, nS)); TRY(dalloc(c, &h.Slo, nS));
+        } else {
+            TRY(dalloc(c, &h.S, nS));
+        }
+        TRY(dalloc(c, &h.uu, N)); TRY(dalloc(c, &h.Fx, N)); TRY(dalloc(c, &h.Fy, N)); TRY(dalloc(c, &h.Fz, N));
+        TRY(dalloc(c, &h.freq_end, M)); TRY(dalloc(c, &h.st, 1));
+        TRY(dalloc(c, &c->d_lj6, TT)); TRY(dalloc(c, &c->d_lj12, TT));
+        TRY(dalloc(c, &c->d_sig6, TT)); TRY(dalloc(c, &c->d_4eps, TT));
+        TRY(dalloc(c, &c->d_counts, T > 0 ? T : 1));
+        TRY(dalloc(c, &c->d_xk2, nk)); TRY(dalloc(c, &c->d_beta6, nk)); TRY(dalloc(c, &c->d_beta12, nk));
+        TRY(dalloc(c, &c->d_tab, (size_t)N * 6 * YD_KP));
+        TRY(dalloc(c, &c->d_part, (size_t)((N + 255) / 256 + 1) * 6));
+        TRY(dalloc(c, &c->d_ov, 1)); TRY(dalloc(c, &c->d_io, 64)); TRY(dalloc(c, &c->d_mol, 3 * YD_MAXA));
+        TRY(dalloc(c, &c->d_chain, 1)); TRY(dalloc(c, &c->d_sig6s, 1)); TRY(dalloc(c, &c->d_feps, 1));
+        // host staging
+        std::vector<double> dh(N);
+        for (int i = 0; i < N; ++i) dh[i] = d->hard_core_angstr[i] / d->box_length;  // EwaldSumRealSpace.f90:359,381
+        std::vector<int> kxyz(nk);
+        for (int k = 0; k < nk; ++k) kxyz[k] = c->kx[k] | (c->ky[k] << 8) | (c->kz[k] << 16);
+        // BiasedRandom.f90:155-180 with mv_num(:) = 1 (runmc.f90:43)
+        std::vector<double> fe(M);
+        double prob = 1.0 / (double)M, beg = 0.0;
+        for (int m = 0; m < M; ++m) { fe[m] = beg + prob; beg = fe[m]; }
+        hipStream_t s = c->stream;
+        HIPCHK(hipMemcpyAsync(h.q, d->charge, sizeof(double) * N, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.dh, dh.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.mass, d->mass, sizeof(double) * N, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.molnum, d->molnum, sizeof(int) * N, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.type, d->lj_type, sizeof(int) * N, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.first, d->mol_first, sizeof(int) * M, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.last, d->mol_last, sizeof(int) * M, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.kxyz, kxyz.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->d_xk2, xk2.data(), sizeof(double) * nk, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.freq_end, fe.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
+        if (T > 0) {
+            HIPCHK(hipMemcpyAsync(c->d_lj6, d->lj6, sizeof(double) * T * T, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(c->d_lj12, d->lj12, sizeof(double) * T * T, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(c->d_sig6, d->sigma6, sizeof(double) * T * T, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(c->d_4eps, d->four_eps, sizeof(double) * T * T, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(c->d_counts, d->tail_counts, sizeof(int) * T, hipMemcpyHostToDevice, s));
+        }
+        HIPCHK(hipMemcpyAsync(c->d_chain, &c->ch, sizeof(YdChain), hipMemcpyHostToDevice, s));
+        const double *sp = c->d_sig6, *fp = c->d_4eps;
+        HIPCHK(hipMemcpyAsync(c->d_sig6s, &sp, sizeof(sp), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->d_feps, &fp, sizeof(fp), hipMemcpyHostToDevice, s));
+        return sync(c);
+    };
+    rc = go();
+    if (rc != YAMC_OK) {
+        yamc_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_ctx_destroy(yamc_ctx *c) {
+    if (!c) return YAMC_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void *p : c->owned) hipFree(p);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_ctx_info(const yamc_ctx *c, int32_t *device, int32_t *nk) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    if (device) *device = c->device;
+    if (nk) *nk = c->nk;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_grid(yamc_ctx *c, int32_t *kx, int32_t *ky, int32_t *kz, int32_t *ip, int32_t *iq) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    for (int k = 0; k < c->nk; ++k) {
+        if (kx) kx[k] = c->kx[k];
+        if (ky) ky[k] = c->ky[k];
+        if (kz) kz[k] = c->kz[k];
+        if (ip) ip[k] = c->ip[k];
+        if (iq) iq[k] = c->iq[k];
+    }
+    return YAMC_OK;
+}
+
+extern "C" int yamc_set_positions(yamc_ctx *c, const double *x, const double *y, const double *z) {
+    if (!c || !x || !y || !z) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    size_t n = sizeof(double) * c->natom;
+    HIPCHK(hipMemcpyAsync(c->ch.x, x, n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->ch.y, y, n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->ch.z, z, n, hipMemcpyHostToDevice, c->stream));
+    c->state_ready = false;
+    return sync(c);
+}
+
+extern "C" int yamc_get_positions(yamc_ctx *c, double *x, double *y, double *z) {
+    if (!c || !x || !y || !z) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    size_t n = sizeof(double) * c->natom;
+    HIPCHK(hipMemcpyAsync(x, c->ch.x, n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(y, c->ch.y, n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(z, c->ch.z, n, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+// S(k) from scratch + k-space energies (EwaldSumKSpace.f90:225-260).
+static int total_energy(yamc_ctx *c) {
+    hipStream_t s = c->stream;
+    int N = c->natom, nk = c->nk, T = c->ntype;
+    yk_atom_tables<<<(N + 255) / 256, 256, 0, s>>>(c->ch, c->ch.x, c->ch.y, c->ch.z, c->d_tab);
+    yk_full_sums<<<((1 + T) * nk + 127) / 128, 128, 0, s>>>(c->ch, c->d_tab);
+    yk_kenergy<<<1, 1024, 0, s>>>(c->ch);
+    HIPCHK(hipGetLastError());
+    return YAMC_OK;
+}
+
+extern "C" int yamc_kspace_init(yamc_ctx *c, double scale_beta, double *e_c, double *e_lj) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    hipStream_t s = c->stream;
+    int nk = c->nk, T = c->ntype;
+    if (scale_beta > 1e-9) {
+        if (!c->beta_ready) return fail(YAMC_ERROR_INITIALIZATION, "scale_beta before beta is initialised");
+        yk_scale_beta<<<(nk + 255) / 256, 256, 0, s>>>(c->ch, scale_beta);
+    } else {
+        yk_beta<<<(nk + 63) / 64, 64, 0, s>>>(c->ch, c->d_xk2, c->d_beta6, c->d_beta12);
+        if (T > 0) {
+            long long n = (long long)T * T * nk;
+            yk_beta_lj<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(c->ch, c->d_lj6, c->d_lj12, c->d_beta6, c->d_beta12);
+        }
+        c->beta_ready = true;
+    }
+    HIPCHK(hipGetLastError());
+    TRY(total_energy(c));
+    TRY(sync(c));
+    if (e_c || e_lj) {
+        YdState st;
+        HIPCHK(hipMemcpy(&st, c->ch.st, sizeof st, hipMemcpyDeviceToHost));
+        if (e_c) *e_c = st.e_kc;
+        if (e_lj) *e_lj = st.e_klj;
+    }
+    return YAMC_OK;
+}
+
+extern "C" int yamc_kspace_get_beta(yamc_ctx *c, double *beta, double *beta6, double *beta12, double *beta_lj) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    if (!c->beta_ready) return fail(YAMC_ERROR_INITIALIZATION, "beta not initialised");
+    size_t nk = c->nk;
+    if (beta) HIPCHK(hipMemcpy(beta, c->ch.beta, sizeof(double) * nk, hipMemcpyDeviceToHost));
+    if (beta6) HIPCHK(hipMemcpy(beta6, c->d_beta6, sizeof(double) * nk, hipMemcpyDeviceToHost));
+    if (beta12) HIPCHK(hipMemcpy(beta12, c->d_beta12, sizeof(double) * nk, hipMemcpyDeviceToHost));
+    if (beta_lj && c->ntype > 0)
+        HIPCHK(hipMemcpy(beta_lj, c->ch.beta_lj, sizeof(double) * nk * c->ntype * c->ntype, hipMemcpyDeviceToHost));
+    return YAMC_OK;
+}
+
+extern "C" int yamc_kspace_get_sums(yamc_ctx *c, double *sums) {
+    if (!c || !sums) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    size_t n = (size_t)(1 + c->ntype) * 8 * c->nk;
+    if (c->precision == YAMC_FP32C) {
+        std::vector<float> hi(n), lo(n);
+        HIPCHK(hipMemcpy(hi.data(), c->ch.Shi, sizeof(float) * n, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(lo.data(), c->ch.Slo, sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) sums[i] = (double)hi[i] + (double)lo[i];
+    } else {
+        HIPCHK(hipMemcpy(sums, c->ch.S, sizeof(double) * n, hipMemcpyDeviceToHost));
+    }
+    return YAMC_OK;
+}
+
+static int stage_mol(yamc_ctx *c, int n, const double *x, const double *y, const double *z) {
+    double buf[3 * YD_MAXA];
+    for (int a = 0; a < n; ++a) { buf[a] = x[a]; buf[YD_MAXA + a] = y[a]; buf[2 * YD_MAXA + a] = z[a]; }
+    HIPCHK(hipMemcpyAsync(c->d_mol, buf, sizeof buf, hipMemcpyHostToDevice, c->stream));
+    return YAMC_OK;
+}
+
+static int mol_size(yamc_ctx *c, int imol, int *na) {
+    if (imol < 0 || imol >= c->nmol) return fail(YAMC_ERROR_PARAMETER, "molecule index %d out of range", imol);
+    int f, l;
+    HIPCHK(hipMemcpy(&f, c->ch.first + imol, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&l, c->ch.last + imol, sizeof(int), hipMemcpyDeviceToHost));
+    *na = l - f + 1;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_kspace_delta(yamc_ctx *c, int32_t imol, const double *x, const double *y, const double *z,
+                                 double *du_c, double *du_lj) {
+    if (!c || !x || !y || !z) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    if (!c->beta_ready) return fail(YAMC_ERROR_INITIALIZATION, "kspace not initialised");
+    int na;
+    TRY(mol_size(c, imol, &na));
+    TRY(stage_mol(c, na, x, y, z));
+    const double *dm = c->d_mol;
+    if (c->precision == YAMC_FP32C)
+        yk_kspace_delta<true><<<1, YK_BS, 0, c->stream>>>(c->ch, imol, dm, dm + YD_MAXA, dm + 2 * YD_MAXA, c->d_io);
+    else
+        yk_kspace_delta<false><<<1, YK_BS, 0, c->stream>>>(c->ch, imol, dm, dm + YD_MAXA, dm + 2 * YD_MAXA, c->d_io);
+    HIPCHK(hipGetLastError());
+    double o[2];
+    HIPCHK(hipMemcpyAsync(o, c->d_io, sizeof o, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    if (du_c) *du_c = o[0];
+    if (du_lj) *du_lj = o[1];
+    return YAMC_OK;
+}
+
+extern "C" int yamc_kspace_forces(yamc_ctx *c, int32_t first, int32_t n, const double *x, const double *y,
+                                  const double *z, double *fx, double *fy, double *fz) {
+    if (!c || !x || !y || !z || !fx || !fy || !fz) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    if (!c->beta_ready) return fail(YAMC_ERROR_INITIALIZATION, "kspace not initialised");
+    if (first < 0 || n < 0 || first + n > c->natom) return fail(YAMC_ERROR_PARAMETER, "atom range out of bounds");
+    for (int b = 0; b < n; b += YD_MAXA) {
+        int m = n - b < YD_MAXA ? n - b : YD_MAXA;
+        TRY(stage_mol(c, m, x + b, y + b, z + b));
+        const double *dm = c->d_mol;
+        if (c->precision == YAMC_FP32C)
+            yk_kspace_forces<true><<<1, YK_BS, 0, c->stream>>>(c->ch, first + b, m, dm, dm + YD_MAXA,
+                                                                dm + 2 * YD_MAXA, c->d_io);
+        else
+            yk_kspace_forces<false><<<1, YK_BS, 0, c->stream>>>(c->ch, first + b, m, dm, dm + YD_MAXA,
+                                                                 dm + 2 * YD_MAXA, c->d_io);
+        HIPCHK(hipGetLastError());
+        double o[3 * YD_MAXA];
+        HIPCHK(hipMemcpyAsync(o, c->d_io, sizeof(double) * 3 * m, hipMemcpyDeviceToHost, c->stream));
+        TRY(sync(c));
+        for (int a = 0; a < m; ++a) {
+            fx[b + a] = o[3 * a];
+            fy[b + a] = o[3 * a + 1];
+            fz[b + a] = o[3 * a + 2];
+        }
+    }
+    return YAMC_OK;
+}
+
+static int rspace_full(yamc_ctx *c, double *comps, int32_t *overlap) {
+    hipStream_t s = c->stream;
+    int N = c->natom, nb = (N + 255) / 256;
+    HIPCHK(hipMemsetAsync(c->d_ov, 0, sizeof(int), s));
+    yk_rspace_full<<<nb, 256, 0, s>>>(c->ch, c->d_sig6, c->d_4eps, c->d_part, c->d_ov);
+    yk_rspace_full_final<<<1, 256, 0, s>>>(c->ch, c->d_part, nb, c->d_io);
+    HIPCHK(hipGetLastError());
+    double o[6];
+    int ov = 0;
+    HIPCHK(hipMemcpyAsync(o, c->d_io, sizeof o, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&ov, c->d_ov, sizeof ov, hipMemcpyDeviceToHost, s));
+    TRY(sync(c));
+    if (comps)
+        for (int m = 0; m < 6; ++m) comps[m] = o[m];
+    if (overlap) *overlap = ov;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_rspace_full(yamc_ctx *c, double *comps, double *uu, double *fx, double *fy, double *fz,
+                                int32_t *overlap) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    TRY(rspace_full(c, comps, overlap));
+    c->state_ready = false;  // uu/F/components were overwritten; k-space and tails still need mc_init
+    return yamc_mc_real_arrays(c, uu, fx, fy, fz);
+}
+
+extern "C" int yamc_rspace_partial(yamc_ctx *c, int32_t imol, const double *x, const double *y, const double *z,
+                                   double *comps, double *uu, double *fx, double *fy, double *fz, double *uu_back,
+                                   double *fx_back, double *fy_back, double *fz_back, int32_t *overlap) {
+    if (!c || !x || !y || !z) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    int na;
+    TRY(mol_size(c, imol, &na));
+    TRY(stage_mol(c, na, x, y, z));
+    int N = c->natom;
+    double *back = nullptr;
+    bool want_back = uu_back || fx_back || fy_back || fz_back;
+    if (want_back) HIPCHK(hipMallocAsync((void **)&back, sizeof(double) * 4 * N, c->stream));
+    const double *dm = c->d_mol;
+    yk_rspace_partial<<<1, YK_BS, 0, c->stream>>>(c->ch, imol, dm, dm + YD_MAXA, dm + 2 * YD_MAXA, c->d_sig6,
+                                                  c->d_4eps, c->d_io, back, back ? back + N : nullptr,
+                                                  back ? back + 2 * N : nullptr, back ? back + 3 * N : nullptr);
+    HIPCHK(hipGetLastError());
+    double o[4 * YD_MAXA + 4];
+    HIPCHK(hipMemcpyAsync(o, c->d_io, sizeof o, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> hb;
+    if (want_back) {
+        hb.resize(4 * (size_t)N);
+        HIPCHK(hipMemcpyAsync(hb.data(), back, sizeof(double) * 4 * N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipFreeAsync(back, c->stream));
+    }
+    TRY(sync(c));
+    for (int a = 0; a < na; ++a) {
+        if (uu) uu[a] = o[a];
+        if (fx) fx[a] = o[YD_MAXA + 3 * a];
+        if (fy) fy[a] = o[YD_MAXA + 3 * a + 1];
+        if (fz) fz[a] = o[YD_MAXA + 3 * a + 2];
+    }
+    if (comps) {
+        comps[0] = o[4 * YD_MAXA]; comps[1] = o[4 * YD_MAXA + 1]; comps[2] = o[4 * YD_MAXA + 2];
+        comps[3] = comps[4] = comps[5] = 0.0;  // same-molecule pairs are skipped in the partial sum
+    }
+    if (overlap) *overlap = o[4 * YD_MAXA + 3] != 0.0;
+    if (want_back) {
+        if (uu_back) memcpy(uu_back, hb.data(), sizeof(double) * N);
+        if (fx_back) memcpy(fx_back, hb.data() + N, sizeof(double) * N);
+        if (fy_back) memcpy(fy_back, hb.data() + 2 * N, sizeof(double) * N);
+        if (fz_back) memcpy(fz_back, hb.data() + 3 * N, sizeof(double) * N);
+    }
+    return YAMC_OK;
+}
+
+extern "C" int yamc_tails(yamc_ctx *c, double *self_c, double *lj6, double *lj12) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    yk_tails<<<1, 64, 0, c->stream>>>(c->ch, c->d_lj6, c->d_lj12, c->d_counts);
+    HIPCHK(hipGetLastError());
+    YdState st;
+    HIPCHK(hipMemcpyAsync(&st, c->ch.st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    if (self_c) *self_c = st.tail_c;
+    if (lj6) *lj6 = st.tail6;
+    if (lj12) *lj12 = st.tail12;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_external_mu(yamc_ctx *c, double *mu3) {
+    if (!c || !mu3) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    yk_mu<<<1, 1024, 0, c->stream>>>(c->ch, c->d_io);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(mu3, c->d_io, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+// MC.f90:681-698 on the host from the device scalars.
+static void energies_from_state(const YdState &st, double *e) {
+    double U_tail = st.tail_c + st.tail12 - st.tail6;
+    double U_ext = 0.0;  // filled by caller (needs kext)
+    double ew = st.uu_ew + st.e_kc + st.tail_c;
+    double lj = st.uu_lj12 - st.uu_lj6 + st.e_klj + st.tail12 - st.tail6;
+    e[1] = ew; e[2] = lj; e[3] = st.uu_ew; e[4] = st.uu_lj6; e[5] = st.uu_lj12;
+    e[6] = st.e_kc; e[7] = st.e_klj; e[8] = U_tail; e[9] = U_ext;
+}
+
+extern "C" int yamc_mc_energies(yamc_ctx *c, double *e) {
+    if (!c || !e) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    YdState st;
+    HIPCHK(hipMemcpyAsync(&st, c->ch.st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    energies_from_state(st, e);
+    double U_ext = c->ch.p.kext * (st.mu[0] * st.mu[0] + st.mu[1] * st.mu[1] + st.mu[2] * st.mu[2]);
+    e[9] = U_ext;
+    e[1] = e[1] + U_ext;  // U_total_ew = uu_ew + E_kC + U_tail_C + U_ext
+    e[0] = e[1] + e[2];
+    return YAMC_OK;
+}
+
+extern "C" int yamc_mc_init(yamc_ctx *c, double *e, int32_t *overlap) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    int32_t ov = 0;
+    TRY(rspace_full(c, nullptr, &ov));
+    TRY(yamc_kspace_init(c, 0.0, nullptr, nullptr));
+    double mu[3];
+    TRY(yamc_external_mu(c, mu));
+    TRY(yamc_tails(c, nullptr, nullptr, nullptr));
+    c->state_ready = true;
+    if (overlap) *overlap = ov;
+    if (e) TRY(yamc_mc_energies(c, e));
+    return YAMC_OK;
+}
+
+extern "C" int yamc_mc_real_arrays(yamc_ctx *c, double *uu, double *fx, double *fy, double *fz) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    size_t n = sizeof(double) * c->natom;
+    if (uu) HIPCHK(hipMemcpyAsync(uu, c->ch.uu, n, hipMemcpyDeviceToHost, c->stream));
+    if (fx) HIPCHK(hipMemcpyAsync(fx, c->ch.Fx, n, hipMemcpyDeviceToHost, c->stream));
+    if (fy) HIPCHK(hipMemcpyAsync(fy, c->ch.Fy, n, hipMemcpyDeviceToHost, c->stream));
+    if (fz) HIPCHK(hipMemcpyAsync(fz, c->ch.Fz, n, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+extern "C" int yamc_mc_set_rng(yamc_ctx *c, uint64_t seed, uint64_t stream, uint64_t counter) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    YdState st;
+    HIPCHK(hipMemcpyAsync(&st, c->ch.st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    st.key[0] = (uint32_t)seed; st.key[1] = (uint32_t)(seed >> 32);
+    st.stream[0] = (uint32_t)stream; st.stream[1] = (uint32_t)(stream >> 32);
+    st.ctr = counter;
+    HIPCHK(hipMemcpyAsync(c->ch.st, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+
+extern "C" int yamc_mc_get_rng(yamc_ctx *c, uint64_t *counter) {
+    if (!c || !counter) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    YdState st;
+    HIPCHK(hipMemcpyAsync(&st, c->ch.st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    *counter = st.ctr;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_resync(yamc_ctx *c) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    if (!c->beta_ready) return fail(YAMC_ERROR_INITIALIZATION, "kspace not initialised");
+    TRY(total_energy(c));
+    return sync(c);
+}
+
+static int launch_chain(yamc_ctx *c, const YdChain *d_chain, const double *const *sig6s,
+                        const double *const *feps, int nchains, int nsteps, int mode) {
+    if (c->precision == YAMC_FP32C)
+        yk_chain<true><<<nchains, YK_BS, 0, c->stream>>>(d_chain, sig6s, feps, nsteps, mode, 0);
+    else
+        yk_chain<false><<<nchains, YK_BS, 0, c->stream>>>(d_chain, sig6s, feps, nsteps, mode, 0);
+    HIPCHK(hipGetLastError());
+    return YAMC_OK;
+}
+
+extern "C" int yamc_mc_run(yamc_ctx *c, int32_t ncycles, int32_t resync_every, yamc_move_rec *recs,
+                           int64_t *n_accepted) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    if (!c->state_ready) return fail(YAMC_ERROR_INITIALIZATION, "yamc_mc_init has not run on this state");
+    if (ncycles < 0) return fail(YAMC_ERROR_PARAMETER, "negative cycle count");
+    YdState st0;
+    HIPCHK(hipMemcpyAsync(&st0, c->ch.st, sizeof st0, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    // Segments of whole cycles between FP64 resyncs of S(k).
+    int seg = ncycles;
+    if (resync_every > 0) {
+        seg = (resync_every + c->nmol - 1) / c->nmol;
+        if (seg < 1) seg = 1;
+    }
+    yamc_move_rec *d_rec = nullptr;
+    size_t nrec = (size_t)ncycles * c->nmol;
+    if (recs && nrec) HIPCHK(hipMallocAsync((void **)&d_rec, sizeof(yamc_move_rec) * nrec, c->stream));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int done = 0; done < ncycles; done += seg) {
+        int n = ncycles - done < seg ? ncycles - done : seg;
+        YdChain ch = c->ch;
+        ch.rec = d_rec ? d_rec + (size_t)done * c->nmol : nullptr;
+        HIPCHK(hipMemcpyAsync(c->d_chain, &ch, sizeof ch, hipMemcpyHostToDevice, c->stream));
+        TRY(launch_chain(c, c->d_chain, c->d_sig6s, c->d_feps, 1, n, 0));
+        if (resync_every > 0 && done + n < ncycles) TRY(total_energy(c));
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    if (d_rec) {
+        HIPCHK(hipMemcpyAsync(recs, d_rec, sizeof(yamc_move_rec) * nrec, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipFreeAsync(d_rec, c->stream));
+    }
+    YdState st1;
+    HIPCHK(hipMemcpyAsync(&st1, c->ch.st, sizeof st1, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    HIPCHK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    if (n_accepted) *n_accepted = st1.n_accepted - st0.n_accepted;
+    HIPCHK(hipMemcpyAsync(c->d_chain, &c->ch, sizeof(YdChain), hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+
+extern "C" int yamc_mc_run_const(yamc_ctx *c, int32_t nmoves, const int32_t *imol, const double *u,
+                                 yamc_move_rec *recs, int64_t *n_accepted) {
+    if (!c || !imol || !u) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    if (!c->state_ready) return fail(YAMC_ERROR_INITIALIZATION, "yamc_mc_init has not run on this state");
+    for (int k = 0; k < nmoves; ++k)
+        if (imol[k] < 0 || imol[k] >= c->nmol) return fail(YAMC_ERROR_PARAMETER, "molecule index out of range at move %d", k);
+    YdState st0;
+    HIPCHK(hipMemcpyAsync(&st0, c->ch.st, sizeof st0, hipMemcpyDeviceToHost, c->stream));
+    int *d_imol = nullptr;
+    double *d_u = nullptr;
+    yamc_move_rec *d_rec = nullptr;
+    HIPCHK(hipMallocAsync((void **)&d_imol, sizeof(int) * (nmoves + 1), c->stream));
+    HIPCHK(hipMallocAsync((void **)&d_u, sizeof(double) * (nmoves + 1), c->stream));
+    HIPCHK(hipMemcpyAsync(d_imol, imol, sizeof(int) * nmoves, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_u, u, sizeof(double) * nmoves, hipMemcpyHostToDevice, c->stream));
+    if (recs) HIPCHK(hipMallocAsync((void **)&d_rec, sizeof(yamc_move_rec) * (nmoves + 1), c->stream));
+    YdChain ch = c->ch;
+    ch.rec = d_rec; ch.cimol = d_imol; ch.cu = d_u;
+    HIPCHK(hipMemcpyAsync(c->d_chain, &ch, sizeof ch, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    TRY(launch_chain(c, c->d_chain, c->d_sig6s, c->d_feps, 1, nmoves, 1));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    if (recs) HIPCHK(hipMemcpyAsync(recs, d_rec, sizeof(yamc_move_rec) * nmoves, hipMemcpyDeviceToHost, c->stream));
+    YdState st1;
+    HIPCHK(hipMemcpyAsync(&st1, c->ch.st, sizeof st1, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_chain, &c->ch, sizeof(YdChain), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipFreeAsync(d_imol, c->stream));
+    HIPCHK(hipFreeAsync(d_u, c->stream));
+    if (d_rec) HIPCHK(hipFreeAsync(d_rec, c->stream));
+    TRY(sync(c));
+    HIPCHK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    if (n_accepted) *n_accepted = st1.n_accepted - st0.n_accepted;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int64_t *n_accepted) {
+    if (!ctxs || n < 1) return fail(YAMC_ERROR_PARAMETER, "empty batch");
+    yamc_ctx *c0 = ctxs[0];
+    std::vector<YdChain> chains(n);
+    std::vector<const double *> s6(n), fe(n);
+    std::vector<long long> acc0(n);
+    for (int i = 0; i < n; ++i) {
+        yamc_ctx *c = ctxs[i];
+        if (!c) return fail(YAMC_ERROR_PARAMETER, "null context in batch");
+        if (c->device != c0->device || c->precision != c0->precision)
+            return fail(YAMC_ERROR_PARAMETER, "batch contexts differ in device or precision");
+        if (!c->state_ready) return fail(YAMC_ERROR_INITIALIZATION, "yamc_mc_init has not run on a batch member");
+        HIPCHK(hipStreamSynchronize(c->stream));
+        chains[i] = c->ch;
+        s6[i] = c->d_sig6;
+        fe[i] = c->d_4eps;
+        YdState st;
+        HIPCHK(hipMemcpy(&st, c->ch.st, sizeof st, hipMemcpyDeviceToHost));
+        acc0[i] = st.n_accepted;
+    }
+    hipStream_t s = c0->stream;
+    YdChain *d_ch = nullptr;
+    const double **d_s6 = nullptr, **d_fe = nullptr;
+    HIPCHK(hipMallocAsync((void **)&d_ch, sizeof(YdChain) * n, s));
+    HIPCHK(hipMallocAsync((void **)&d_s6, sizeof(void *) * n, s));
+    HIPCHK(hipMallocAsync((void **)&d_fe, sizeof(void *) * n, s));
+    HIPCHK(hipMemcpyAsync(d_ch, chains.data(), sizeof(YdChain) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_s6, s6.data(), sizeof(void *) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_fe, fe.data(), sizeof(void *) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(c0->ev0, s));
+    TRY(launch_chain(c0, d_ch, d_s6, d_fe, n, ncycles, 0));
+    HIPCHK(hipEventRecord(c0->ev1, s));
+    HIPCHK(hipFreeAsync(d_ch, s));
+    HIPCHK(hipFreeAsync(d_s6, s));
+    HIPCHK(hipFreeAsync(d_fe, s));
+    TRY(sync(c0));
+    HIPCHK(hipEventElapsedTime(&c0->last_ms, c0->ev0, c0->ev1));
+    if (n_accepted) {
+        for (int i = 0; i < n; ++i) {
+            YdState st;
+            HIPCHK(hipMemcpy(&st, ctxs[i]->ch.st, sizeof st, hipMemcpyDeviceToHost));
+            n_accepted[i] = st.n_accepted - acc0[i];
+        }
+    }
+    return YAMC_OK;
+}
+
+extern "C" int yamc_forces(yamc_ctx *c, double *gx, double *gy, double *gz, double *rx, double *ry, double *rz) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    if (!c->state_ready) return fail(YAMC_ERROR_INITIALIZATION, "yamc_mc_init has not run on this state");
+    int N = c->natom;
+    double *dg = nullptr, *dr = nullptr;
+    HIPCHK(hipMallocAsync((void **)&dg, sizeof(double) * 3 * N, c->stream));
+    HIPCHK(hipMallocAsync((void **)&dr, sizeof(double) * 3 * N, c->stream));
+    hipStream_t s = c->stream;
+    yk_atom_tables<<<(N + 255) / 256, 256, 0, s>>>(c->ch, c->ch.x, c->ch.y, c->ch.z, c->d_tab);
+    HIPCHK(hipGetLastError());
+    double mu[3];
+    TRY(yamc_external_mu(c, mu));
+    yk_launch_forces(c->ch, c->d_sig6, c->d_4eps, c->d_tab, dg, dr, s);
+    HIPCHK(hipGetLastError());
+    std::vector<double> hg(3 * (size_t)N), hr(3 * (size_t)N);
+    HIPCHK(hipMemcpyAsync(hg.data(), dg, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hr.data(), dr, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipFreeAsync(dg, s));
+    HIPCHK(hipFreeAsync(dr, s));
+    TRY(sync(c));
+    for (int i = 0; i < N; ++i) {
+        if (gx) { gx[i] = hg[i]; gy[i] = hg[N + i]; gz[i] = hg[2 * N + i]; }
+        if (rx) { rx[i] = hr[i]; ry[i] = hr[N + i]; rz[i] = hr[2 * N + i]; }
+    }
+    return YAMC_OK;
+}
+
+extern "C" int yamc_last_kernel_ms(yamc_ctx *c, double *ms) {
+    if (!c || !ms) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    *ms = c->last_ms;
+    return YAMC_OK;
+}

@@ -1,0 +1,315 @@
+// yamc_device.h -- device-side data layout and arithmetic of the Ewald
+// Monte Carlo path (gfx950).  Included by the kernel translation units only.
+//
+// Layout in HBM (one replica):
+//   atoms       SoA x, y, z, q, dh (= hard core / L), mass : double[natom]
+//               molnum (1-based), type (0 = no LJ)          : int[natom]
+//   molecules   first, last (0-based inclusive)            : int[nmol]
+//   k grid      kxyz packed kx | ky<<8 | kz<<16            : int[nk]
+//               beta                                       : double[nk]
+//               beta_lj, full symmetric T x T rows         : double[T*T*nk]
+//   S(k)        [(1+T) sets][8 components][nk]  (set 0 = Coulomb,
+//               set t = LJ type t; components cos/sin of ppp, ppm, pmp, pmm)
+//               FP64: double; FP32C: float hi + float lo two-sum pairs
+//   real space  uu, Fx, Fy, Fz : double[natom]
+// Every k-space sweep walks k fastest, so a wavefront reads 64 consecutive
+// doubles (512 B) of one component array per load.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/yamc_ewald.h"
+
+#define YD_PI 3.14159265358979323846
+#define YD_TWO_PI (2.0 * YD_PI)
+#define YD_FOUR_PI (4.0 * YD_PI)
+#define YD_MAXA YAMC_MAX_MOL_ATOM
+#define YD_KP (YAMC_MAX_KMAX + 1)
+#define YD_MAXT YAMC_MAX_LJ_TYPE
+
+struct YdParams {
+    int natom, nmol, ntype, kmax, nk, precision;
+    double L, alpha, rmax2, xclb, kext, m2kext, dr, d_angle, lambda_f, lambda_c;
+    double alpha_over_sqrt_pi, two_alpha_over_sqrt_pi;
+};
+
+// Scalars of the Monte Carlo state kept on the device.
+struct YdState {
+    double e_kc, e_klj, uu_ew, uu_lj6, uu_lj12, uu_ew_intra, uu_lj6_intra, uu_lj12_intra;
+    double mu[3];
+    double tail_c, tail6, tail12;
+    uint32_t key[2], stream[2];
+    unsigned long long ctr;
+    long long n_moves, n_accepted, n_overlap, since_resync;
+};
+
+struct YdChain {
+    YdParams p;
+    double *x, *y, *z, *q, *dh, *mass;
+    int *molnum, *type, *first, *last;
+    int *kxyz;
+    double *beta, *beta_lj;
+    double *S;            // FP64 sums
+    float *Shi, *Slo;     // FP32C sums
+    double *uu, *Fx, *Fy, *Fz;
+    double *freq_end;     // cumulative move intervals (BiasedRandom.f90:172-180)
+    YdState *st;
+    yamc_move_rec *rec;   // optional per-move records
+    const int *cimol;     // constant-uniform mode: molecule per move
+    const double *cu;     // constant-uniform mode: uniform per move
+};
+
+// ---------------------------------------------------------------- math ----
+// Functions.f90:116-142 erfc_Luc
+__device__ __forceinline__ double yd_erfc_luc(double x, double x2, double ee2) {
+    const double P = 0.3275911, A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741, A4 = -1.453152027,
+                 A5 = 1.061405429, pi05 = 1.7724538509055;
+    if (x < 0.455) return 1.0 - 2.0 / pi05 * x * (1.0 - x2 * (1.0 / 3.0 - x2 * (0.1 - x2 * (1.0 / 42.0 - x2 / 216.0))));
+    if (x > 3.0) return (1.0 - 0.5 * (1.0 - 1.5 / x2) / x2) / (pi05 * x) * ee2;
+    double T = 1.0 / (1.0 + P * x);
+    return T * (A1 + T * (A2 + T * (A3 + T * (A4 + T * A5)))) * ee2;
+}
+
+// Functions.f90:145-171 erf_Luc
+__device__ __forceinline__ double yd_erf_luc(double x, double x2, double ee2) {
+    const double P = 0.3275911, A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741, A4 = -1.453152027,
+                 A5 = 1.061405429, pi05 = 1.7724538509055;
+    if (x < 0.455) return 2.0 / pi05 * x * (1.0 - x2 * (1.0 / 3.0 - x2 * (0.1 - x2 * (1.0 / 42.0 - x2 / 216.0))));
+    if (x > 3.0) return 1.0 - (1.0 - 0.5 * (1.0 - 1.5 / x2) / x2) / (pi05 * x) * ee2;
+    double T = 1.0 / (1.0 + P * x);
+    return 1.0 - T * (A1 + T * (A2 + T * (A3 + T * (A4 + T * A5)))) * ee2;
+}
+
+__device__ __forceinline__ double yd_p3(double x) { return x * x * x; }
+__device__ __forceinline__ double yd_p9(double x) { double x2 = x * x, x4 = x2 * x2; return x4 * x4 * x; }
+
+// Functions.f90:225-289 shi_sans_exp_f (Shi(x) e^-x)
+__device__ double yd_shi_sans_exp(double x) {
+    const double dx0 = 0.2;
+    double x1, shi_f;
+    if (x <= 1.0) {
+        double x2 = x * x, fac = 1.0, xn = x;
+        shi_f = x;
+        for (int i = 1; i <= 4; ++i) {
+            double deuxi = 2.0 * i;
+            xn = xn * x2;
+            fac = fac * deuxi * (deuxi + 1.0);
+            shi_f = shi_f + xn / (deuxi + 1.0) / fac;
+        }
+        return exp(-x) * shi_f;
+    } else if (x > 20.0) {
+        double som = 1.0, term = 1.0;
+        for (int i = 1; i <= 6; ++i) {
+            term = term * i / x;
+            som = som + term;
+        }
+        return 1.0 / 2.0 / x * som;
+    } else if (x < 5.0) {
+        x1 = 1.0; shi_f = 1.057250875;
+    } else if (x < 10.0) {
+        x1 = 5.0; shi_f = 20.093211826;
+    } else if (x < 15.0) {
+        x1 = 10.0; shi_f = 1246.11449;
+    } else {
+        x1 = 15.0; shi_f = 117477.926;
+    }
+    int n = (int)((x - x1) / dx0 + 2);
+    double dx = (x - x1) / (n - 1);
+    shi_f = shi_f + dx * sinh(x1) / x1 / 2.0 + dx * dx / 12.0 * (cosh(x1) / x1 - sinh(x1) / (x1 * x1));
+    double t = x1, s = sinh(x1);
+    for (int i = 2; i <= n; ++i) {
+        t = x1 + (double)(i - 1) * dx;
+        s = sinh(t);
+        shi_f = shi_f + dx * s / t;
+    }
+    shi_f = shi_f - dx * s / t / 2.0 - dx * dx / 12.0 * (cosh(t) / t - s / (t * t));
+    return exp(-x) * shi_f;
+}
+
+// ---------------------------------------------------------------- RNG -----
+// Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ void yd_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                          uint32_t k1, uint32_t *o) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
+    }
+    o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3;
+}
+
+// Uniform source of one chain (thread 0 only).  Either the Philox stream of
+// 24-bit uniforms (random.f90:65-93 draws a default real, 0 excluded) or a
+// constant per move (random.f90:80-82).
+struct YdRng {
+    uint32_t key0, key1, s0, s1;
+    unsigned long long ctr;
+    int constant;
+    double cu;
+    __device__ double next() {
+        if (constant) return cu;
+        for (;;) {
+            unsigned long long n = ctr++;
+            uint32_t o[4];
+            yd_philox((uint32_t)(n >> 2), (uint32_t)(n >> 34), s0, s1, key0, key1, o);
+            uint32_t k = o[n & 3] >> 8;
+            if (k != 0) return (double)k * (1.0 / 16777216.0);
+        }
+    }
+};
+
+// ----------------------------------------------------------- reductions ---
+// Block-wide sum of NV doubles; result valid in out[] for every thread after
+// the call.  red must hold (blockDim/64) * NV doubles, out NV doubles.
+template <int NV>
+__device__ __forceinline__ void yd_block_sum(double (&v)[NV], double *red, double *out) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double a = v[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        v[i] = a;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) red[w * NV + i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double a = 0.0;
+        for (int j = 0; j < nw; ++j) a += red[j * NV + threadIdx.x];
+        out[threadIdx.x] = a;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------ k-space helpers ---
+// SumSinCosKR.f90:181-223 SinCosXYZ_fill for one coordinate: tab[0..kmax]
+// cos, tab[KP..] sin of 2 pi k x, angle-addition recurrence.
+__device__ __forceinline__ void yd_sincos_axis(double x, int kmax, double *c, double *s) {
+    double cx = cos(YD_TWO_PI * x), sx = sin(YD_TWO_PI * x);
+    c[0] = 1.0; s[0] = 0.0;
+    for (int k = 1; k <= kmax; ++k) {
+        c[k] = c[k - 1] * cx - s[k - 1] * sx;
+        s[k] = s[k - 1] * cx + c[k - 1] * sx;
+    }
+}
+
+// SumSinCosKR.f90:297-455 SinCosKR_fill evaluated at one k.  tab holds the
+// six axis tables of one atom (cx, sx, cy, sy, cz, sz) with stride YD_KP.
+// The xy factor is rebuilt from the tables at each k, which is the value the
+// reference recurrence carries along a (kx, ky) row.
+__device__ __forceinline__ void yd_kr(const double *tab, double q, int kx, int ky, int kz, double *o) {
+    double cxk = tab[kx], sxk = tab[YD_KP + kx], cyk = tab[2 * YD_KP + ky], syk = tab[3 * YD_KP + ky];
+    double czk = tab[4 * YD_KP + kz], szk = tab[5 * YD_KP + kz];
+    double qcc = q * cxk * cyk, qss = q * sxk * syk, qsc = q * sxk * cyk, qcs = q * cxk * syk;
+    double qcpy = qcc - qss, qspy = qsc + qcs, qcmy = qcc + qss, qsmy = qsc - qcs;
+    double a = qcpy * czk, b = qspy * szk, c = qspy * czk, d = qcpy * szk;
+    o[0] = a - b;
+    o[1] = c + d;
+    o[2] = kz > 0 ? a + b : 0.0;
+    o[3] = kz > 0 ? c - d : 0.0;
+    double e = qcmy * czk, f = qsmy * szk, g = qsmy * czk, h = qcmy * szk;
+    o[4] = ky > 0 ? e - f : 0.0;
+    o[5] = ky > 0 ? g + h : 0.0;
+    o[6] = (ky > 0 && kz > 0) ? e + f : 0.0;
+    o[7] = (ky > 0 && kz > 0) ? g - h : 0.0;
+}
+
+// Coulomb and LJ parts of one atom's sin/cos at k (RhoSquared.f90:160-183):
+// the LJ part of a charged LJ atom is the Coulomb part times 1/q.
+__device__ __forceinline__ void yd_kr_atom(const double *tab, double q, int typ, int kx, int ky, int kz,
+                                           double *krc, double *krl) {
+    if (q != 0.0) {
+        yd_kr(tab, q, kx, ky, kz, krc);
+        if (typ > 0) {
+            double inv = 1.0 / q;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) krl[i] = krc[i] * inv;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) krc[i] = 0.0;
+        if (typ > 0) yd_kr(tab, 1.0, kx, ky, kz, krl);
+    }
+}
+
+// One k of ForceKSpace_coulomb/_LJ (ForceKSpace.f90:116-158): C, S of the
+// four sign slots against the atom's q sin/cos, k-vector component signs
+// folded per slot, times kx, ky, kz and beta.
+__device__ __forceinline__ void yd_force_k(const double *C, const double *kr, int kx, int ky, int kz, double bt,
+                                           double *f) {
+    double A = C[0] * kr[1] - C[1] * kr[0];
+    double fx = A, fy = A, fz = A;
+    if (ky > 0) {
+        A = C[4] * kr[5] - C[5] * kr[4];
+        fx = fx + A; fy = fy - A; fz = fz + A;
+    }
+    if (kz > 0) {
+        A = C[2] * kr[3] - C[3] * kr[2];
+        fx = fx + A; fy = fy + A; fz = fz - A;
+    }
+    if (ky > 0 && kz > 0) {
+        A = C[6] * kr[7] - C[7] * kr[6];
+        fx = fx + A; fy = fy - A; fz = fz - A;
+    }
+    f[0] += fx * (double)kx * bt;
+    f[1] += fy * (double)ky * bt;
+    f[2] += fz * (double)kz * bt;
+}
+
+// One k of calc_dU_coulomb (EwaldSumKSpace.f90:581-614) without beta.
+__device__ __forceinline__ double yd_du_k(const double *C, const double *D, int ky, int kz) {
+    double loc = (C[0] + C[0] + D[0]) * D[0] + (C[1] + C[1] + D[1]) * D[1];
+    if (ky > 0) loc = loc + (C[4] + C[4] + D[4]) * D[4] + (C[5] + C[5] + D[5]) * D[5];
+    if (kz > 0) loc = loc + (C[2] + C[2] + D[2]) * D[2] + (C[3] + C[3] + D[3]) * D[3];
+    if (ky > 0 && kz > 0) loc = loc + (C[6] + C[6] + D[6]) * D[6] + (C[7] + C[7] + D[7]) * D[7];
+    return loc;
+}
+
+// One k of calc_dU_LJ (EwaldSumKSpace.f90:725-741): tw = 2 C^t1 + dC^t1.
+__device__ __forceinline__ double yd_dulj_k(const double *tw, const double *D2, int ky, int kz) {
+    double loc = tw[0] * D2[0] + tw[1] * D2[1];
+    if (ky > 0) loc = loc + tw[4] * D2[4] + tw[5] * D2[5];
+    if (kz > 0) loc = loc + tw[2] * D2[2] + tw[3] * D2[3];
+    if (ky > 0 && kz > 0) loc = loc + tw[6] * D2[6] + tw[7] * D2[7];
+    return loc;
+}
+
+// --------------------------------------------------- real-space helpers ---
+// RealSumLocal.f90:63-168 for one pair.
+struct YdPair {
+    double potew, potlj6, potlj12, pot, FR;
+};
+
+__device__ __forceinline__ void yd_pair(const YdParams &p, double r2, bool same_mol, double qi, double qj, bool lj,
+                                        double sigma6, double four_eps, YdPair &o) {
+    double r = sqrt(r2), a = p.alpha * r, a2 = a * a, e = exp(-a2);
+    double erfc_a = same_mol ? -yd_erf_luc(a, a2, e) : yd_erfc_luc(a, a2, e);
+    double qq = p.xclb * qi * qj;
+    o.potew = qq * erfc_a / r;
+    o.pot = o.potew;
+    o.FR = qq * (erfc_a / r + p.two_alpha_over_sqrt_pi * e) / r2;
+    o.potlj6 = 0.0;
+    o.potlj12 = 0.0;
+    if (lj) {
+        double r6 = r2 * r2 * r2, sor6 = sigma6 / r6;
+        double C6 = 1.0 + a2 * (1.0 + a2 / 2.0);
+        double a6 = a2 * a2 * a2;
+        double C12 = C6 + a6 * (1.0 + a2 * (1.0 + a2 / 5.0) / 4.0) / 6.0;
+        o.potlj6 = four_eps * sor6 * e * C6;
+        o.potlj12 = four_eps * (sor6 * sor6) * e * C12;
+        o.pot = o.pot + o.potlj12 - o.potlj6;
+        o.FR = o.FR + four_eps * 6.0 * sor6 * e * (2.0 * sor6 * (C12 + (a6 * a6) / 720.0) - (C6 + a6 / 6.0)) / r2;
+    }
+}
+
+// Single-wrap minimum image (EwaldSumRealSpace.f90:394-401).
+__device__ __forceinline__ double yd_wrap(double d) {
+    if (d > 0.5) d = d - 1.0;
+    if (d < -0.5) d = d + 1.0;
+    return d;
+}

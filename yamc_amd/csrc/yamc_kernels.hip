@@ -1,0 +1,1208 @@
+// yamc_kernels.hip -- HIP kernels of the Ewald Monte Carlo path for gfx950.
+//
+// Hot path: yk_chain, one workgroup per Monte Carlo chain (replica), looping
+// over trial moves on the device.  Per move it runs the phases of movemc
+// (MC.f90:831-1322): old k-space forces of the moved molecule (one sweep over
+// S(k)), the force-biased proposal and the new real-space partial sum, the new
+// sweep (delta S, dU_k, new k-space forces), the Metropolis test, and on
+// acceptance the S(k) commit and the O(N) update of the per-atom arrays.
+// Scalar work (RNG, proposal, acceptance) runs on thread 0 in FP64; sweeps are
+// spread over the workgroup with k (or atom j) fastest for coalesced loads.
+// Nothing here is a dense contraction, so no MFMA is used.
+#include "yamc_device.h"
+
+#define YK_BS 512            // workgroup size of the chain kernel
+#define YK_NW (YK_BS / 64)
+#define YK_MAXP 2            // distinct LJ types inside one molecule
+// Bound on every rejection-sampling loop of the proposal.  The reference
+// loops until success; with a constant uniform per move (RND_CONSTANT) some
+// loops never succeed, so a move that exceeds the bound is rejected without
+// an acceptance draw and recorded with overlap = 2.
+#define YK_TRIES 65536
+
+// ------------------------------------------------------------------------
+// Functions.f90:26-111 erfc_Luc_bet6_bet12
+__device__ void yk_erfc_bet6_bet12(double x, double x2, double e2, double *erfk, double *bet6, double *bet12) {
+    double pi = 4.0 * atan(1.0);
+    double pi12 = sqrt(pi);
+    double pi15 = pow(pi, 1.5);
+    const double xmin = 0.5, xmax = 8.0;
+    if (x <= xmin) {
+        *erfk = yd_erfc_luc(x, x2, e2);
+        *bet6 = pi15 / 3.0 * yd_p3(x) * (pi12 * *erfk + (0.5 / x2 - 1.0) / x * e2);
+        *bet12 = pi15 / 1080.0 * yd_p9(x) *
+                 (-16.0 * pi12 / 105.0 * *erfk +
+                  (((((1.0 / x2 - 2.0 / 7.0) / x2 + 4.0 / 35.0) / x2 - 8.0 / 105.0) / x2 + 16.0 / 105.0)) / x * e2);
+    } else if (x >= xmax) {
+        *erfk = e2 / pi12 / x * (1.0 - 0.5 / x2 * (1.0 - 1.5 / x2 * (1.0 - 2.5 / x2)));
+        *bet6 = pi15 * e2 / 4.0 / (x * x) * (1.0 - 2.5 / x2 * (1.0 - 3.5 / x2 * (1.0 - 4.5 / x2)));
+        *bet12 = pi15 * e2 / 240.0 / (x * x) * (1.0 - 5.5 / x2 * (1.0 - 6.5 / x2 * (1.0 - 7.5 / x2)));
+    } else {
+        double dx = 1.0e-3;
+        int n = (int)((xmax - x) / dx + 2);
+        dx = (xmax - x) / (n - 1);
+        double x4 = (x * x) * (x * x), x3 = x * x * x, x5 = x4 * x, x8 = x4 * x4, x9 = x8 * x, x10 = x8 * x * x,
+               x11 = x10 * x;
+        double som = dx * e2 / 2.0 - dx * dx / 12.0 * 2.0 * x * e2;
+        double som6 = dx * e2 / x4 / 2.0 - dx * dx / 12.0 * (2.0 / x3 + 4.0 / x5) * e2;
+        double som12 = dx * e2 / x10 / 2.0 - dx * dx / 12.0 * (2.0 / x9 + 10.0 / x11) * e2;
+        double y = x, e = e2;
+        for (int i = 2; i <= n; ++i) {
+            y = x + (double)(i - 1) * dx;
+            e = exp(-(y * y));
+            double y4 = (y * y) * (y * y), y10 = y4 * y4 * (y * y);
+            som = som + dx * e;
+            som6 = som6 + dx * e / y4;
+            som12 = som12 + dx * e / y10;
+        }
+        double y2 = y * y, y3 = y2 * y, y4 = y2 * y2, y5 = y4 * y, y8 = y4 * y4, y9 = y8 * y, y10 = y8 * y2,
+               y11 = y10 * y;
+        som = som - dx * e / 2.0 + dx * dx / 12.0 * 2.0 * y * e;
+        som = som + e / 2.0 / y * (1.0 - 0.5 / y2 * (1.0 - 1.5 / y2 * (1.0 - 2.5 / y2)));
+        *erfk = 2.0 / pi12 * som;
+        som6 = som6 - dx * e / y4 / 2.0 + dx * dx / 12.0 * (2.0 / y3 + 4.0 / y5) * e;
+        som6 = som6 + e / 2.0 / y5 * (1.0 - 2.5 / y2 * (1.0 - 3.5 / y2 * (1.0 - 4.5 / y2)));
+        *bet6 = pi15 * x3 / 2.0 * som6;
+        som12 = som12 - dx * e / y10 / 2.0 + dx * dx / 12.0 * (2.0 / y9 + 10.0 / y11) * e;
+        som12 = som12 + e / 2.0 / y11 * (1.0 - 5.5 / y2 * (1.0 - 6.5 / y2 * (1.0 - 7.5 / y2)));
+        *bet12 = pi15 * x9 / 120.0 * som12;
+    }
+}
+
+// EwaldSumKSpace.f90:263-350 initBeta: one thread per k.
+__global__ void yk_beta(YdChain c, const double *xk2, double *beta6, double *beta12) {
+    int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (kk >= c.p.nk) return;
+    double poa = YD_PI / c.p.alpha, poa2 = poa * poa;
+    double k2 = xk2[kk], b2 = poa2 * k2, e = exp(-b2);
+    double bt = c.p.xclb * e / k2 / YD_TWO_PI;
+    double b = poa * sqrt(k2), erfc_b, B6, B12;
+    yk_erfc_bet6_bet12(b, b2, e, &erfc_b, &B6, &B12);
+    double b6 = yd_p3(c.p.alpha) * B6, b12 = yd_p9(c.p.alpha) * B12;
+    if ((c.kxyz[kk] & 255) > 0) { bt = 2.0 * bt; b6 = 2.0 * b6; b12 = 2.0 * b12; }
+    c.beta[kk] = bt;
+    beta6[kk] = b6;
+    beta12[kk] = b12;
+}
+
+// EwaldSumKSpace.f90:352-386 initBetaLJ, full symmetric rows.
+__global__ void yk_beta_lj(YdChain c, const double *lj6, const double *lj12, const double *beta6,
+                           const double *beta12) {
+    int T = c.p.ntype, nk = c.p.nk;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)T * T * nk) return;
+    int kk = (int)(i % nk), pr = (int)(i / nk), t1 = pr / T, t2 = pr % T;
+    int a = t1 <= t2 ? t1 : t2, b = t1 <= t2 ? t2 : t1;
+    c.beta_lj[i] = lj12[a * T + b] * beta12[kk] - lj6[a * T + b] * beta6[kk];
+}
+
+// EwaldSumKSpace.f90:210-211: beta(:) = beta(:) * scale_beta.
+__global__ void yk_scale_beta(YdChain c, double s) {
+    int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (kk < c.p.nk) c.beta[kk] = c.beta[kk] * s;
+}
+
+// SumSinCosKR.f90:181-223 for every atom: tab[i][6][YD_KP].
+__global__ void yk_atom_tables(YdChain c, const double *x, const double *y, const double *z, double *tab) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.p.natom) return;
+    double *t = tab + (size_t)i * 6 * YD_KP;
+    yd_sincos_axis(x[i], c.p.kmax, t, t + YD_KP);
+    yd_sincos_axis(y[i], c.p.kmax, t + 2 * YD_KP, t + 3 * YD_KP);
+    yd_sincos_axis(z[i], c.p.kmax, t + 4 * YD_KP, t + 5 * YD_KP);
+}
+
+// RhoSquared_addAtoms (RhoSquared.f90:188-243) from scratch: one thread per
+// (set, k), atoms added in index order, as the reference accumulates them.
+// FP32C stores the FP64 result as a (hi, lo) pair.
+__global__ void yk_full_sums(YdChain c, const double *tab) {
+    int nk = c.p.nk, T = c.p.ntype;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (1 + T) * nk) return;
+    int set = i / nk, kk = i % nk;
+    int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < c.p.natom; ++a) {
+        double q = c.q[a];
+        int typ = c.type[a];
+        if (set == 0 ? q == 0.0 : typ != set) continue;
+        double krc[8], krl[8];
+        yd_kr_atom(tab + (size_t)a * 6 * YD_KP, q, typ, kx, ky, kz, krc, krl);
+        const double *kr = set == 0 ? krc : krl;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) s[m] = s[m] + kr[m];
+    }
+    size_t base = (size_t)set * 8 * nk + kk;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        if (c.p.precision == YAMC_FP32C) {
+            float hi = (float)s[m];
+            c.Shi[base + (size_t)m * nk] = hi;
+            c.Slo[base + (size_t)m * nk] = (float)(s[m] - (double)hi);
+        } else {
+            c.S[base + (size_t)m * nk] = s[m];
+        }
+    }
+}
+
+__device__ __forceinline__ double yk_ldS(const YdChain &c, size_t idx) {
+    if (c.p.precision == YAMC_FP32C) return (double)c.Shi[idx] + (double)c.Slo[idx];
+    return c.S[idx];
+}
+
+// calc_coulomb_energy + calc_LJ_energy (EwaldSumKSpace.f90:389-526), one
+// workgroup; result into st->e_kc, st->e_klj.
+__global__ void yk_kenergy(YdChain c) {
+    __shared__ double red[16 * 2], out[2];
+    int nk = c.p.nk, T = c.p.ntype;
+    double v[2] = {0.0, 0.0};
+    for (int kk = threadIdx.x; kk < nk; kk += blockDim.x) {
+        int kxyz = c.kxyz[kk], ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        double C[8];
+        for (int m = 0; m < 8; ++m) C[m] = yk_ldS(c, (size_t)m * nk + kk);
+        double r2 = C[0] * C[0] + C[1] * C[1];
+        if (ky > 0) r2 = r2 + C[4] * C[4] + C[5] * C[5];
+        if (kz > 0) r2 = r2 + C[2] * C[2] + C[3] * C[3];
+        if (ky > 0 && kz > 0) r2 = r2 + C[6] * C[6] + C[7] * C[7];
+        v[0] += c.beta[kk] * r2;
+        for (int t1 = 0; t1 < T; ++t1) {
+            double A[8];
+            for (int m = 0; m < 8; ++m) A[m] = yk_ldS(c, ((size_t)(1 + t1) * 8 + m) * nk + kk);
+            for (int t2 = t1; t2 < T; ++t2) {
+                double B[8];
+                for (int m = 0; m < 8; ++m) B[m] = yk_ldS(c, ((size_t)(1 + t2) * 8 + m) * nk + kk);
+                double loc = A[0] * B[0] + A[1] * B[1];
+                if (ky > 0) loc = loc + A[4] * B[4] + A[5] * B[5];
+                if (kz > 0) loc = loc + A[2] * B[2] + A[3] * B[3];
+                if (ky > 0 && kz > 0) loc = loc + A[6] * B[6] + A[7] * B[7];
+                double term = c.beta_lj[((size_t)t1 * T + t2) * nk + kk] * loc;
+                v[1] += (t1 != t2) ? term + term : term;
+            }
+        }
+    }
+    yd_block_sum<2>(v, red, out);
+    if (threadIdx.x == 0) {
+        c.st->e_kc = out[0];
+        c.st->e_klj = out[1];
+    }
+}
+
+// EwaldSumExternal_calc_mu (EwaldSumExternal.f90:165-183), one workgroup.
+__global__ void yk_mu(YdChain c, double *mu_out) {
+    __shared__ double red[16 * 3], out[3];
+    double v[3] = {0, 0, 0};
+    for (int i = threadIdx.x; i < c.p.natom; i += blockDim.x) {
+        double q = c.q[i];
+        v[0] += q * c.x[i];
+        v[1] += q * c.y[i];
+        v[2] += q * c.z[i];
+    }
+    yd_block_sum<3>(v, red, out);
+    if (threadIdx.x == 0) {
+        for (int d = 0; d < 3; ++d) c.st->mu[d] = out[d];
+        if (mu_out)
+            for (int d = 0; d < 3; ++d) mu_out[d] = out[d];
+    }
+}
+
+// EwaldSumTails.f90:18-127 (constant terms), single thread.
+__global__ void yk_tails(YdChain c, const double *lj6, const double *lj12, const int *counts) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double S = 0.0;
+    for (int i = 0; i < c.p.natom; ++i) S = S + c.q[i] * c.q[i];
+    c.st->tail_c = -c.p.xclb * c.p.alpha_over_sqrt_pi * S;
+    int T = c.p.ntype;
+    double a3 = yd_p3(c.p.alpha), a6 = a3 * a3, a9 = a3 * a6, a12 = a6 * a6, pspi = pow(YD_PI, 1.5);
+    double sij6 = 0.0, sij12 = 0.0, sj6 = 0.0, sj12 = 0.0;
+    for (int t1 = 0; t1 < T; ++t1)
+        for (int t2 = t1; t2 < T; ++t2) {
+            int n12 = counts[t1] * counts[t2];
+            double l6 = lj6[t1 * T + t2] * (double)n12, l12 = lj12[t1 * T + t2] * (double)n12;
+            if (t1 != t2) { l6 = l6 + l6; l12 = l12 + l12; }
+            sij6 = sij6 + l6;
+            sij12 = sij12 + l12;
+        }
+    for (int t = 0; t < T; ++t) {
+        sj6 = sj6 + lj6[t * T + t] * (double)counts[t];
+        sj12 = sj12 + lj12[t * T + t] * (double)counts[t];
+    }
+    c.st->tail6 = pspi * a3 / 6.0 * sij6 - a6 / 12.0 * sj6;
+    c.st->tail12 = pspi * a9 / 1080.0 * sij12 - a12 / 1440.0 * sj12;
+}
+
+// Full real-space sum (EwaldSumRealSpace.f90:338-498, full mode): one thread
+// per atom i visits every j != i, so uu(i) and F(i) are complete without
+// atomics; the six energy components count pairs j > i only and are reduced
+// per workgroup into part[block][6].  ov is set on an LJ hard-core overlap.
+__global__ void __launch_bounds__(256) yk_rspace_full(YdChain c, const double *lj_sig6, const double *lj_4eps,
+                                                      double *part, int *ov) {
+    __shared__ double red[4 * 6], out[6];
+    int N = c.p.natom, T = c.p.ntype;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (i < N) {
+        int im = c.molnum[i], it = c.type[i];
+        double qi = c.q[i], dhi = c.dh[i], xi = c.x[i], yi = c.y[i], zi = c.z[i];
+        double uu = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int j = 0; j < N; ++j) {
+            if (j == i) continue;
+            int jm = c.molnum[j], jt = c.type[j];
+            double xij = yd_wrap(xi - c.x[j]), yij = yd_wrap(yi - c.y[j]), zij = yd_wrap(zi - c.z[j]);
+            double r2 = xij * xij + yij * yij + zij * zij;
+            double d12 = (dhi + c.dh[j]) / 2.0;
+            if (it > 0 && jt > 0 && im != jm && r2 < d12 * d12) *ov = 1;
+            if (r2 >= c.p.rmax2) continue;
+            bool lj = it != 0 && jt != 0;
+            int off = lj ? (it - 1) * T + (jt - 1) : 0;
+            YdPair pr;
+            yd_pair(c.p, r2, im == jm, qi, c.q[j], lj, lj_sig6[off], lj_4eps[off], pr);
+            if (j > i) {
+                v[0] += pr.potew;
+                v[1] += pr.potlj6;
+                v[2] += pr.potlj12;
+                if (im == jm) {
+                    v[3] += pr.potew;
+                    v[4] += pr.potlj6;
+                    v[5] += pr.potlj12;
+                }
+            }
+            if (im != jm) {
+                uu += pr.pot;
+                fx += xij * pr.FR;
+                fy += yij * pr.FR;
+                fz += zij * pr.FR;
+            }
+        }
+        c.uu[i] = uu;
+        c.Fx[i] = fx;
+        c.Fy[i] = fy;
+        c.Fz[i] = fz;
+    }
+    yd_block_sum<6>(v, red, out);
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = out[threadIdx.x];
+}
+
+__global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, double *comps_out) {
+    __shared__ double red[16 * 6], out[6];
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+        for (int m = 0; m < 6; ++m) v[m] += part[b * 6 + m];
+    yd_block_sum<6>(v, red, out);
+    if (threadIdx.x == 0) {
+        c.st->uu_ew = out[0]; c.st->uu_lj6 = out[1]; c.st->uu_lj12 = out[2];
+        c.st->uu_ew_intra = out[3]; c.st->uu_lj6_intra = out[4]; c.st->uu_lj12_intra = out[5];
+        if (comps_out)
+            for (int m = 0; m < 6; ++m) comps_out[m] = out[m];
+    }
+}
+
+// ========================================================================
+// Per-move machinery shared by the chain kernel and the single-call entry
+// points.  Molecule data of the move lives in LDS.
+struct YkMol {
+    int imol, first, na, np;
+    int typ[YD_MAXA], lslot[YD_MAXA], ptype[YK_MAXP];
+    double q[YD_MAXA], mass[YD_MAXA];
+    double xo[YD_MAXA], yo[YD_MAXA], zo[YD_MAXA], xn[YD_MAXA], yn[YD_MAXA], zn[YD_MAXA];
+    double tab_o[YD_MAXA * 6 * YD_KP], tab_n[YD_MAXA * 6 * YD_KP];
+};
+
+// forward declarations of the thread-0 state types (defined below)
+struct YkMcm {
+    int stall;  // a rejection-sampling loop exceeded YK_TRIES draws
+    double fo[3], ftotold, co[3], couplold, umaxold, vmaxold, cold, qold;
+    double v[3], d[3], angle;
+    double fn[3], ftotnew, cn[3], couplnew, umaxnew, vmaxnew, cnew, qnew;
+    double cphi, sphi, ct, st, ca, sa;
+};
+
+// Scalar state of the move, written by thread 0 only; kept in LDS so that
+// the wide sweeps do not carry it in registers.
+struct YkScalar {
+    YkMcm mcm;
+    YdRng rng;
+    double U_old, frn[YD_MAXA][3], uun[YD_MAXA], cmp_new[3], duc, dulj, du_real, du_ext;
+    double mu[3], mu_new[3], muo[3];
+    long long acc_count, nmv, nov;
+};
+
+struct YkShared {
+    YkMol m;
+    YkScalar t0;
+    double red[YK_NW * 32];
+    double out[32];
+    int ov, accepted, stall;
+};
+
+// Loads molecule imol (old positions from the state) into LDS; thread 0.
+__device__ void yk_load_mol(const YdChain &c, YkMol &m, int imol) {
+    m.imol = imol;
+    m.first = c.first[imol];
+    m.na = c.last[imol] - m.first + 1;
+    m.np = 0;
+    for (int a = 0; a < m.na; ++a) {
+        int g = m.first + a;
+        m.q[a] = c.q[g];
+        m.typ[a] = c.type[g];
+        m.mass[a] = c.mass[g];
+        m.xo[a] = c.x[g];
+        m.yo[a] = c.y[g];
+        m.zo[a] = c.z[g];
+        m.lslot[a] = -1;
+        if (m.typ[a] > 0) {
+            int s = -1;
+            for (int p = 0; p < m.np; ++p)
+                if (m.ptype[p] == m.typ[a]) s = p;
+            if (s < 0 && m.np < YK_MAXP) { s = m.np; m.ptype[m.np++] = m.typ[a]; }
+            m.lslot[a] = s;
+        }
+    }
+}
+
+// Axis tables of the molecule's atoms (old or new positions): 3*na threads.
+__device__ void yk_mol_tables(const YdChain &c, YkMol &m, bool newpos) {
+    int t = threadIdx.x;
+    if (t < 3 * m.na) {
+        int a = t / 3, d = t % 3;
+        double v = newpos ? (d == 0 ? m.xn[a] : d == 1 ? m.yn[a] : m.zn[a])
+                          : (d == 0 ? m.xo[a] : d == 1 ? m.yo[a] : m.zo[a]);
+        double *tab = (newpos ? m.tab_n : m.tab_o) + a * 6 * YD_KP + 2 * d * YD_KP;
+        yd_sincos_axis(v, c.p.kmax, tab, tab + YD_KP);
+    }
+}
+
+// Delta of the Coulomb sums at k: (sum of new) - (sum of old) over the
+// molecule's charged atoms in index order (RhoSquared_addAtom into nulified
+// sums, then RhoSquared_sub: MC.f90:946-958, 1096-1125).
+__device__ __forceinline__ void yk_delta_coul(const YkMol &m, int kx, int ky, int kz, double *dc) {
+    double ns[8] = {0, 0, 0, 0, 0, 0, 0, 0}, os[8] = {0, 0, 0, 0, 0, 0, 0, 0}, kr[8];
+    for (int a = 0; a < m.na; ++a) {
+        double q = m.q[a];
+        if (q == 0.0) continue;
+        yd_kr(m.tab_n + a * 6 * YD_KP, q, kx, ky, kz, kr);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) ns[i] = ns[i] + kr[i];
+        yd_kr(m.tab_o + a * 6 * YD_KP, q, kx, ky, kz, kr);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) os[i] = os[i] + kr[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dc[i] = ns[i] - os[i];
+}
+
+// Delta of the LJ sums of present slot p at k.
+__device__ __forceinline__ void yk_delta_lj(const YkMol &m, int p, int kx, int ky, int kz, double *dl) {
+    double ns[8] = {0, 0, 0, 0, 0, 0, 0, 0}, os[8] = {0, 0, 0, 0, 0, 0, 0, 0}, krc[8], krl[8];
+    for (int a = 0; a < m.na; ++a) {
+        if (m.lslot[a] != p) continue;
+        yd_kr_atom(m.tab_n + a * 6 * YD_KP, m.q[a], m.typ[a], kx, ky, kz, krc, krl);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) ns[i] = ns[i] + krl[i];
+        yd_kr_atom(m.tab_o + a * 6 * YD_KP, m.q[a], m.typ[a], kx, ky, kz, krc, krl);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) os[i] = os[i] + krl[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dl[i] = ns[i] - os[i];
+}
+
+// Loads the 8 components of set at k.  FP32C reads the hi word only: the
+// sweep's bandwidth is what the FP32 form saves.
+template <bool F32>
+__device__ __forceinline__ void yk_ld8(const YdChain &c, int set, int kk, double *C) {
+    size_t base = (size_t)set * 8 * c.p.nk + kk;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        if (F32) C[m] = (double)c.Shi[base + (size_t)m * c.p.nk];
+        else C[m] = c.S[base + (size_t)m * c.p.nk];
+    }
+}
+
+// Accumulator layout of the sweeps: [0] dU_C, [1] dU_LJ, [2 + 3a + d]
+// Coulomb force on atom a, [2 + 3*MAXA + 3a + d] LJ force on atom a.
+#define YK_NACC (2 + 6 * YD_MAXA)
+#define YK_FC(a, d) (2 + 3 * (a) + (d))
+#define YK_FL(a, d) (2 + 3 * YD_MAXA + 3 * (a) + (d))
+
+// One sweep over k.  DELTA = false: k-space forces of the molecule at its
+// old positions against S (MC.f90:963-968).  DELTA = true: dU_C, dU_LJ
+// against S and forces at new positions against S + delta
+// (MC.f90:1136-1173).
+template <bool F32, bool DELTA>
+__device__ void yk_sweep(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC]) {
+    const int nk = c.p.nk, T = c.p.ntype;
+    const double *tabs = DELTA ? m.tab_n : m.tab_o;
+    for (int kk = threadIdx.x; kk < nk; kk += YK_BS) {
+        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        double bt = c.beta[kk];
+        double C[8], dc[8], kr[8], krl[8];
+        yk_ld8<F32>(c, 0, kk, C);
+        if (DELTA) {
+            yk_delta_coul(m, kx, ky, kz, dc);
+            acc[0] += yd_du_k(C, dc, ky, kz) * bt;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) C[i] = C[i] + dc[i];
+        }
+        for (int a = 0; a < m.na; ++a) {
+            if (m.q[a] == 0.0) continue;
+            yd_kr(tabs + a * 6 * YD_KP, m.q[a], kx, ky, kz, kr);
+            double f[3] = {0, 0, 0};
+            yd_force_k(C, kr, kx, ky, kz, bt, f);
+            acc[YK_FC(a, 0)] += f[0];
+            acc[YK_FC(a, 1)] += f[1];
+            acc[YK_FC(a, 2)] += f[2];
+        }
+        if (T == 0 || m.np == 0) continue;
+        double dl[YK_MAXP][8];
+        if (DELTA) {
+#pragma unroll
+            for (int p = 0; p < YK_MAXP; ++p)
+                if (p < m.np) yk_delta_lj(m, p, kx, ky, kz, dl[p]);
+        }
+        for (int t1 = 0; t1 < T; ++t1) {
+            double C1[8];
+            yk_ld8<F32>(c, 1 + t1, kk, C1);
+            if (DELTA) {
+                int s1 = -1;
+#pragma unroll
+                for (int p = 0; p < YK_MAXP; ++p)
+                    if (p < m.np && m.ptype[p] == t1 + 1) s1 = p;
+                double tw[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    double d1 = 0.0;
+#pragma unroll
+                    for (int p = 0; p < YK_MAXP; ++p)
+                        if (p == s1) d1 = dl[p][i];
+                    tw[i] = C1[i] + C1[i] + d1;
+                    C1[i] = C1[i] + d1;
+                }
+#pragma unroll
+                for (int p = 0; p < YK_MAXP; ++p)
+                    if (p < m.np)
+                        acc[1] += yd_dulj_k(tw, dl[p], ky, kz) *
+                                  c.beta_lj[((size_t)t1 * T + (m.ptype[p] - 1)) * nk + kk];
+            }
+            for (int a = 0; a < m.na; ++a) {
+                int ta = m.typ[a];
+                if (ta <= 0) continue;
+                yd_kr_atom(tabs + a * 6 * YD_KP, m.q[a], ta, kx, ky, kz, kr, krl);
+                double f[3] = {0, 0, 0};
+                yd_force_k(C1, krl, kx, ky, kz, c.beta_lj[((size_t)(ta - 1) * T + t1) * nk + kk], f);
+                acc[YK_FL(a, 0)] += f[0];
+                acc[YK_FL(a, 1)] += f[1];
+                acc[YK_FL(a, 2)] += f[2];
+            }
+        }
+    }
+}
+
+// RhoSquared_add on acceptance (MC.f90:1245): S += delta for the Coulomb set
+// and the molecule's LJ sets.  FP32C keeps (hi, lo) as a two-sum pair.
+template <bool F32>
+__device__ void yk_commit(const YdChain &c, const YkMol &m) {
+    const int nk = c.p.nk;
+    for (int kk = threadIdx.x; kk < nk; kk += YK_BS) {
+        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        double d[8];
+        for (int set = 0; set <= m.np; ++set) {
+            if (set == 0) yk_delta_coul(m, kx, ky, kz, d);
+            else yk_delta_lj(m, set - 1, kx, ky, kz, d);
+            int sidx = set == 0 ? 0 : m.ptype[set - 1];
+            size_t base = (size_t)sidx * 8 * nk + kk;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                size_t idx = base + (size_t)i * nk;
+                if (F32) {
+                    double v = ((double)c.Shi[idx] + (double)c.Slo[idx]) + d[i];
+                    float hi = (float)v;
+                    c.Shi[idx] = hi;
+                    c.Slo[idx] = (float)(v - (double)hi);
+                } else {
+                    c.S[idx] = c.S[idx] + d[i];
+                }
+            }
+        }
+    }
+}
+
+// Real-space partial sum of the molecule at new positions against all
+// other atoms (EwaldSumRealSpace.f90:338-498, partial mode).
+// Accumulators: [a] uu, [4 + 3a + d] F, [16..18] uu_ew, uu_lj6, uu_lj12.
+#define YK_NRS (4 * YD_MAXA + 3)
+__device__ void yk_rs_new(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
+                          double (&acc)[YK_NRS], int &ov) {
+    const int N = c.p.natom, T = c.p.ntype, me = m.imol + 1;
+    for (int j = threadIdx.x; j < N; j += YK_BS) {
+        int jm = c.molnum[j];
+        if (jm == me) continue;
+        int jt = c.type[j];
+        double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+        for (int a = 0; a < m.na; ++a) {
+            int it = m.typ[a];
+            double xij = yd_wrap(m.xn[a] - xj), yij = yd_wrap(m.yn[a] - yj), zij = yd_wrap(m.zn[a] - zj);
+            double r2 = xij * xij + yij * yij + zij * zij;
+            double d12 = (c.dh[m.first + a] + dhj) / 2.0;
+            if (it > 0 && jt > 0 && r2 < d12 * d12) ov = 1;
+            if (r2 >= c.p.rmax2) continue;
+            bool lj = it != 0 && jt != 0;
+            int off = lj ? (it - 1) * T + (jt - 1) : 0;
+            YdPair pr;
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            acc[a] += pr.pot;
+            acc[YD_MAXA + 3 * a + 0] += xij * pr.FR;
+            acc[YD_MAXA + 3 * a + 1] += yij * pr.FR;
+            acc[YD_MAXA + 3 * a + 2] += zij * pr.FR;
+            acc[4 * YD_MAXA + 0] += pr.potew;
+            acc[4 * YD_MAXA + 1] += pr.potlj6;
+            acc[4 * YD_MAXA + 2] += pr.potlj12;
+        }
+    }
+}
+
+// Accepted move: per-atom arrays of every other atom take the new minus the
+// old contribution of the molecule (MC.f90:1251-1275), computed per j in the
+// reference's atom order; acc[0..2] = old uu_ew, uu_lj6, uu_lj12.
+__device__ void yk_rs_update(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
+                             double (&acc)[3]) {
+    const int N = c.p.natom, T = c.p.ntype, me = m.imol + 1;
+    for (int j = threadIdx.x; j < N; j += YK_BS) {
+        int jm = c.molnum[j];
+        if (jm == me) continue;
+        int jt = c.type[j];
+        double qj = c.q[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+        double nb[4] = {0, 0, 0, 0}, ob[4] = {0, 0, 0, 0};
+        for (int pass = 0; pass < 2; ++pass) {
+            double *b = pass == 0 ? nb : ob;
+            for (int a = 0; a < m.na; ++a) {
+                int it = m.typ[a];
+                double xa = pass == 0 ? m.xn[a] : m.xo[a], ya = pass == 0 ? m.yn[a] : m.yo[a],
+                       za = pass == 0 ? m.zn[a] : m.zo[a];
+                double xij = yd_wrap(xa - xj), yij = yd_wrap(ya - yj), zij = yd_wrap(za - zj);
+                double r2 = xij * xij + yij * yij + zij * zij;
+                if (r2 >= c.p.rmax2) continue;
+                bool lj = it != 0 && jt != 0;
+                int off = lj ? (it - 1) * T + (jt - 1) : 0;
+                YdPair pr;
+                yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+                b[0] = b[0] + pr.pot;
+                b[1] = b[1] - xij * pr.FR;
+                b[2] = b[2] - yij * pr.FR;
+                b[3] = b[3] - zij * pr.FR;
+                if (pass == 1) {
+                    acc[0] += pr.potew;
+                    acc[1] += pr.potlj6;
+                    acc[2] += pr.potlj12;
+                }
+            }
+        }
+        if (nb[0] != 0.0 || ob[0] != 0.0 || nb[1] != 0.0 || ob[1] != 0.0 || nb[2] != 0.0 || ob[2] != 0.0 ||
+            nb[3] != 0.0 || ob[3] != 0.0) {
+            c.uu[j] = c.uu[j] + nb[0] - ob[0];
+            c.Fx[j] = c.Fx[j] + nb[1] - ob[1];
+            c.Fy[j] = c.Fy[j] + nb[2] - ob[2];
+            c.Fz[j] = c.Fz[j] + nb[3] - ob[3];
+        }
+    }
+}
+
+// MonteCarloMove.f90 on thread 0 ------------------------------------------
+
+__device__ __forceinline__ void yk_prod(double x1, double y1, double z1, double x2, double y2, double z2, double *o) {
+    o[0] = y1 * z2 - z1 * y2;
+    o[1] = z1 * x2 - x1 * z2;
+    o[2] = x1 * y2 - y1 * x2;
+}
+
+// MonteCarloMove.f90:109-266 chooseDisplacement
+__device__ void yk_displacement(const YdParams &p, YkMcm &m, YdRng &rng) {
+    double fxo = m.fo[0], fyo = m.fo[1], fzo = m.fo[2], ftot = m.ftotold;
+    double umax = p.lambda_f * ftot * p.dr, cold;
+    if (umax > 1e-2) {
+        double e = exp(-2.0 * umax);
+        double t = (umax * (1.0 + e) - (1.0 - e)) / 2.0;
+        cold = 3.0 * t / yd_p3(umax);
+        double xksi = 0.0;
+        for (int g = 0; xksi < 1e-10; ++g) {
+            if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+            xksi = rng.next();
+        }
+        double z = umax + log(xksi * t), u;
+        if (z <= log(4.0)) {
+            double sv = exp(z);
+            u = pow(3.0 * sv, 1.0 / 3.0) - sv / 10.0;
+        } else {
+            u = z - log(z / 2.0);
+        }
+        for (int k = 1; k <= 10; ++k) {
+            e = exp(-2.0 * u);
+            double ch = (1.0 + e) / 2.0, sh = (1.0 - e) / 2.0;
+            if (u < 1e-5) sh = u - u * u;
+            double g = u * ch - sh;
+            if (u < 1e-3) g = yd_p3(u) / 3.0;
+            double h = u + log(g), h1 = u * sh / g;
+            if (k < 10) u = u + (z - h) / h1;
+        }
+        double deltar = u / (p.lambda_f * ftot);
+        xksi = 1.0;
+        for (int g = 0; xksi > (1.0 - 1e-10) || xksi < 1e-10; ++g) {
+            if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+            xksi = rng.next();
+        }
+        double xcostet = 1.0 + log(xksi + (1.0 - xksi) * e) / u;
+        if (u < 1e-7) xcostet = 1.0 - 2.0 * (1.0 - xksi);
+        if (xcostet > 1.0 - 1e-14) xcostet = 1.0 - 1e-14;
+        if (xcostet < -1.0 + 1e-14) xcostet = -1.0 + 1e-14;
+        double xsintet = sqrt(1.0 - xcostet * xcostet);
+        double w = -1.0, wv[3] = {0, 0, 0};
+        for (int g = 0; w <= 1e-10; ++g) {
+                if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+            double v2 = 2.0, vx = 0, vy = 0, vz = 0;
+            for (int g = 0; v2 > 1.0; ++g) {
+                if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+                vx = 2.0 * rng.next() - 1.0;
+                vy = 2.0 * rng.next() - 1.0;
+                vz = 2.0 * rng.next() - 1.0;
+                v2 = vx * vx + vy * vy + vz * vz;
+            }
+            yk_prod(vx, vy, vz, fxo, fyo, fzo, wv);
+            w = sqrt(wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2]);
+        }
+        m.d[0] = deltar * (xcostet * fxo / ftot + xsintet * wv[0] / w);
+        m.d[1] = deltar * (xcostet * fyo / ftot + xsintet * wv[1] / w);
+        m.d[2] = deltar * (xcostet * fzo / ftot + xsintet * wv[2] / w);
+    } else {
+        double v2 = 2.0, vx = 0, vy = 0, vz = 0;
+        for (int g = 0; v2 > 1.0; ++g) {
+                if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+            vx = 2.0 * rng.next() - 1.0;
+            vy = 2.0 * rng.next() - 1.0;
+            vz = 2.0 * rng.next() - 1.0;
+            v2 = vx * vx + vy * vy + vz * vz;
+        }
+        m.d[0] = p.dr * vx; m.d[1] = p.dr * vy; m.d[2] = p.dr * vz;
+        umax = 0.0; cold = 1.0;
+    }
+    m.cold = cold;
+    m.umaxold = umax;
+}
+
+// MonteCarloMove.f90:269-444 chooseRotation
+__device__ void yk_rotation(const YdParams &p, YkMcm &m, YdRng &rng) {
+    double cx = m.co[0], cy = m.co[1], cz = m.co[2], cabs = m.couplold;
+    double vmax = p.lambda_c * cabs * p.d_angle, qold, angle, vx, vy, vz;
+    if (vmax > 1e-2) {
+        double emax = exp(-2.0 * vmax), shmax = (1.0 - emax) / 2.0;
+        double xksi = 1.0, proba = 0.0, v = 0.0, e = 1.0;
+        for (int g = 0; xksi >= proba; ++g) {
+                if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+            v = 0.0;
+            for (int g = 0; fabs(v) < 1e-10; ++g) {
+                if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+                xksi = rng.next();
+                double xs = xksi * shmax;
+                v = vmax + log(xs + sqrt(xs * xs + emax));
+                if (xs * xs < 1e-14 * emax) v = xs / sqrt(emax);
+            }
+            e = exp(-2.0 * v);
+            proba = (1.0 - e) / v / (1.0 + e);
+            if (v < 1e-5) proba = (2.0 * v - 2.0 * v * v) / v / (1.0 + e);
+            xksi = rng.next();
+        }
+        double cosx = v / vmax;
+        qold = yd_shi_sans_exp(vmax) / vmax;
+        xksi = 1.0;
+        for (int g = 0; xksi > (1.0 - 1e-10); ++g) {
+            if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+            xksi = rng.next();
+        }
+        angle = p.d_angle * (1.0 + log(xksi + (1.0 - xksi) * e) / v);
+        if (v < 1e-7) angle = p.d_angle * (1.0 - 2.0 * (1.0 - xksi));
+        double w = -1.0, wv[3] = {0, 0, 0};
+        vx = vy = vz = 0;
+        for (int g = 0; w <= 1e-10; ++g) {
+                if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+            double v2 = 2.0;
+            for (int g = 0; v2 > 1.0 || v2 < 1e-10; ++g) {
+                if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+                vx = 2.0 * rng.next() - 1.0;
+                vy = 2.0 * rng.next() - 1.0;
+                vz = 2.0 * rng.next() - 1.0;
+                v2 = vx * vx + vy * vy + vz * vz;
+            }
+            yk_prod(vx, vy, vz, cx, cy, cz, wv);
+            w = sqrt(wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2]);
+        }
+        double sinx = sqrt(1.0 - cosx * cosx);
+        vx = cx / cabs * cosx + wv[0] / w * sinx;
+        vy = cy / cabs * cosx + wv[1] / w * sinx;
+        vz = cz / cabs * cosx + wv[2] / w * sinx;
+    } else {
+        double v2 = 2.0;
+        vx = vy = vz = 0;
+        for (int g = 0; v2 > 1.0 || v2 < 1e-10; ++g) {
+                if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+            vx = 2.0 * rng.next() - 1.0;
+            vy = 2.0 * rng.next() - 1.0;
+            vz = 2.0 * rng.next() - 1.0;
+            v2 = vx * vx + vy * vy + vz * vz;
+        }
+        double v = sqrt(v2);
+        vx = vx / v; vy = vy / v; vz = vz / v;
+        angle = p.d_angle * (2.0 * rng.next() - 1.0);
+        vmax = 0.0; qold = 1.0;
+    }
+    m.ca = cos(angle); m.sa = sin(angle); m.ct = vz; m.st = sqrt(1.0 - vz * vz);
+    m.cphi = 1.0; m.sphi = 0.0;
+    if (m.st > 0.0) { m.cphi = vx / m.st; m.sphi = vy / m.st; }
+    m.v[0] = vx; m.v[1] = vy; m.v[2] = vz;
+    m.vmaxold = vmax; m.angle = angle; m.qold = qold;
+}
+
+// MonteCarloMove.f90:56-106 init_new
+__device__ void yk_init_new(const YdParams &p, YkMcm &m) {
+    m.ftotnew = sqrt(m.fn[0] * m.fn[0] + m.fn[1] * m.fn[1] + m.fn[2] * m.fn[2]);
+    m.couplnew = sqrt(m.cn[0] * m.cn[0] + m.cn[1] * m.cn[1] + m.cn[2] * m.cn[2]);
+    double umaxnew = p.lambda_f * m.ftotnew * p.dr, cnew;
+    if (umaxnew > 1e-2) {
+        double e = exp(-2.0 * umaxnew);
+        double t = (umaxnew * (1.0 + e) - (1.0 - e)) / 2.0;
+        cnew = 3.0 * t / yd_p3(umaxnew);
+    } else {
+        umaxnew = 0.0; cnew = 1.0;
+    }
+    m.umaxnew = umaxnew; m.cnew = cnew;
+    double vmaxnew = p.lambda_c * m.couplnew * p.d_angle, qnew;
+    if (vmaxnew > 1e-2) qnew = yd_shi_sans_exp(vmaxnew) / vmaxnew;
+    else { vmaxnew = 0.0; qnew = 1.0; }
+    m.vmaxnew = vmaxnew; m.qnew = qnew;
+}
+
+// MonteCarloMove.f90:447-570 accept_or_decline
+__device__ int yk_accept(const YdParams &p, const YkMcm &m, double dutot, YdRng &rng) {
+    double exposu = -dutot;
+    double exposf = -p.lambda_f * ((m.fo[0] + m.fn[0]) * m.d[0] + (m.fo[1] + m.fn[1]) * m.d[1] +
+                                   (m.fo[2] + m.fn[2]) * m.d[2]);
+    double exposfcorr = m.umaxold - m.umaxnew;
+    double exposc = -p.lambda_c * m.angle *
+                    ((m.co[0] + m.cn[0]) * m.v[0] + (m.co[1] + m.cn[1]) * m.v[1] + (m.co[2] + m.cn[2]) * m.v[2]);
+    double exposccorr = m.vmaxold - m.vmaxnew;
+    double factor = m.cold * m.qold / (m.cnew * m.qnew) * exp(exposu + exposf + exposfcorr + exposc + exposccorr);
+    double xksi = rng.next();
+    return xksi > factor ? 0 : 1;
+}
+
+// BiasedRandom.f90:192-220 with flat frequencies: first i with
+// beg(i) < rnd < end(i), else 1 (index 0 here).
+__device__ int yk_choose(const YdChain &c, double rnd) {
+    int lo = 0, hi = c.p.nmol - 1;
+    while (lo < hi) {  // first i with rnd < end(i)
+        int mid = (lo + hi) >> 1;
+        if (rnd < c.freq_end[mid]) hi = mid;
+        else lo = mid + 1;
+    }
+    double beg = lo > 0 ? c.freq_end[lo - 1] : 0.0;
+    if (rnd > beg && rnd < c.freq_end[lo]) return lo;
+    return 0;
+}
+
+// ========================================================================
+// The chain kernel.  mode 0: run_mc cycles (runmc.f90:204-268) with the
+// Philox stream, nsteps = cycles; mode 1: nsteps moves with caller-given
+// molecule and constant uniform per move.
+template <bool F32>
+__global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const double *const *sig6s,
+                                                  const double *const *feps_s, int nsteps, int mode,
+                                                  int max_moves) {
+    __shared__ YkShared sh;
+    const YdChain &c = chains[blockIdx.x];
+    const double *sig6 = sig6s[blockIdx.x], *feps = feps_s[blockIdx.x];
+    const int tid = threadIdx.x;
+    const YdParams &p = c.p;
+    YkScalar &t0 = sh.t0;
+    YkMcm &mcm = t0.mcm;
+    YdRng &rng = t0.rng;
+    double &U_old = t0.U_old, (&frn)[YD_MAXA][3] = t0.frn, (&uun)[YD_MAXA] = t0.uun, (&cmp_new)[3] = t0.cmp_new;
+    double &duc = t0.duc, &dulj = t0.dulj, &du_real = t0.du_real, &du_ext = t0.du_ext;
+    double(&mu)[3] = t0.mu, (&mu_new)[3] = t0.mu_new, (&muo)[3] = t0.muo;
+    long long &acc_count = t0.acc_count, &nmv = t0.nmv, &nov = t0.nov;
+    if (tid == 0) {
+        acc_count = 0; nmv = 0; nov = 0;
+        rng.key0 = c.st->key[0]; rng.key1 = c.st->key[1];
+        rng.s0 = c.st->stream[0]; rng.s1 = c.st->stream[1];
+        rng.ctr = c.st->ctr; rng.constant = mode == 1; rng.cu = 0.0;
+    }
+    int nmol = p.nmol;
+    int total = mode == 1 ? nsteps : nsteps * nmol;
+    if (max_moves > 0 && total > max_moves) total = max_moves;
+    for (int mv = 0; mv < total; ++mv) {
+        // ---- molecule pick (runmc.f90:206, 232, 258) -------------------
+        if (tid == 0) {
+            int imol;
+            if (mode == 1) {
+                imol = c.cimol[mv];
+                rng.cu = c.cu[mv];
+            } else {
+                if (mv % nmol == 0) (void)rng.next();  // pressure-step draw per cycle
+                (void)rng.next();                       // exchange draw per step
+                imol = yk_choose(c, rng.next());
+            }
+            yk_load_mol(c, sh.m, imol);
+            sh.ov = 0;
+        }
+        __syncthreads();
+        // ---- mu from scratch (MC.f90:916) + old axis tables -------------
+        {
+            double v[3] = {0, 0, 0};
+            for (int i = tid; i < p.natom; i += YK_BS) {
+                double q = c.q[i];
+                v[0] += q * c.x[i]; v[1] += q * c.y[i]; v[2] += q * c.z[i];
+            }
+            yk_mol_tables(c, sh.m, false);
+            yd_block_sum<3>(v, sh.red, sh.out);
+            if (tid == 0) { mu[0] = sh.out[0]; mu[1] = sh.out[1]; mu[2] = sh.out[2]; }
+        }
+        // ---- old k-space forces (MC.f90:946-982) ------------------------
+        {
+            double acc[YK_NACC];
+#pragma unroll
+            for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
+            yk_sweep<F32, false>(c, sh.m, acc);
+            yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
+        }
+        // ---- proposal on thread 0 (MC.f90:984-1067) ---------------------
+        if (tid == 0) {
+            YkMol &m = sh.m;
+            int f = m.first, na = m.na;
+            double fx[YD_MAXA], fy[YD_MAXA], fz[YD_MAXA];
+            U_old = 0.0;
+            for (int a = 0; a < na; ++a) U_old = U_old + c.uu[f + a];
+            for (int a = 0; a < na; ++a) {
+                fx[a] = c.Fx[f + a] + YD_FOUR_PI * sh.out[YK_FC(a, 0)] + YD_FOUR_PI * sh.out[YK_FL(a, 0)];
+                fy[a] = c.Fy[f + a] + YD_FOUR_PI * sh.out[YK_FC(a, 1)] + YD_FOUR_PI * sh.out[YK_FL(a, 1)];
+                fz[a] = c.Fz[f + a] + YD_FOUR_PI * sh.out[YK_FC(a, 2)] + YD_FOUR_PI * sh.out[YK_FL(a, 2)];
+                fx[a] = fx[a] + p.m2kext * m.q[a] * mu[0];
+                fy[a] = fy[a] + p.m2kext * m.q[a] * mu[1];
+                fz[a] = fz[a] + p.m2kext * m.q[a] * mu[2];
+            }
+            muo[0] = muo[1] = muo[2] = 0.0;
+            for (int a = 0; a < na; ++a) {
+                muo[0] = muo[0] + m.q[a] * m.xo[a];
+                muo[1] = muo[1] + m.q[a] * m.yo[a];
+                muo[2] = muo[2] + m.q[a] * m.zo[a];
+            }
+            double Fo[3] = {0, 0, 0}, To[3] = {0, 0, 0}, com[3] = {0, 0, 0}, tm = 0.0, tq[3];
+            for (int a = 0; a < na; ++a) { Fo[0] = Fo[0] + fx[a]; Fo[1] = Fo[1] + fy[a]; Fo[2] = Fo[2] + fz[a]; }
+            for (int a = 0; a < na; ++a) {
+                com[0] = com[0] + m.mass[a] * m.xo[a];
+                com[1] = com[1] + m.mass[a] * m.yo[a];
+                com[2] = com[2] + m.mass[a] * m.zo[a];
+                tm = tm + m.mass[a];
+            }
+            com[0] = com[0] / tm; com[1] = com[1] / tm; com[2] = com[2] / tm;
+            for (int a = 0; a < na; ++a) {
+                yk_prod(m.xo[a] - com[0], m.yo[a] - com[1], m.zo[a] - com[2], fx[a], fy[a], fz[a], tq);
+                To[0] = To[0] + tq[0]; To[1] = To[1] + tq[1]; To[2] = To[2] + tq[2];
+            }
+            for (int d = 0; d < 3; ++d) { mcm.fo[d] = Fo[d]; mcm.co[d] = To[d]; }
+            mcm.ftotold = sqrt(Fo[0] * Fo[0] + Fo[1] * Fo[1] + Fo[2] * Fo[2]);
+            mcm.couplold = sqrt(To[0] * To[0] + To[1] * To[1] + To[2] * To[2]);
+            mcm.stall = 0;
+            yk_displacement(p, mcm, rng);
+            yk_rotation(p, mcm, rng);
+            sh.stall = mcm.stall;
+            double cn[3];
+            for (int d = 0; d < 3; ++d) cn[d] = com[d] + mcm.d[d] - round(com[d] + mcm.d[d]);
+            if (na > 1) {
+                double cphi = mcm.cphi, sphi = mcm.sphi, st = mcm.st, ct = mcm.ct, ca = mcm.ca, sa = mcm.sa;
+                double ca1 = 1.0 - ca, R[9];
+                R[0] = (cphi * cphi * (st * st) * ca1 + ca);
+                R[1] = (sphi * cphi * (st * st) * ca1 - ct * sa);
+                R[2] = st * (ct * ca1 * cphi + sa * sphi);
+                R[3] = (cphi * sphi * (st * st) * ca1 + ct * sa);
+                R[4] = (sphi * sphi * (st * st) * ca1 + ca);
+                R[5] = st * (ct * ca1 * sphi - sa * cphi);
+                R[6] = st * (ct * ca1 * cphi - sa * sphi);
+                R[7] = st * (ct * ca1 * sphi + sa * cphi);
+                R[8] = (st * st * ca + ct * ct);
+                for (int a = 0; a < na; ++a) {
+                    double xr = m.xo[a] - com[0], yr = m.yo[a] - com[1], zr = m.zo[a] - com[2];
+                    m.xn[a] = (R[0] * xr + R[1] * yr + R[2] * zr) + cn[0];
+                    m.yn[a] = (R[3] * xr + R[4] * yr + R[5] * zr) + cn[1];
+                    m.zn[a] = (R[6] * xr + R[7] * yr + R[8] * zr) + cn[2];
+                }
+            } else {
+                m.xn[0] = cn[0]; m.yn[0] = cn[1]; m.zn[0] = cn[2];
+            }
+        }
+        __syncthreads();
+        if (sh.stall) {  // proposal loop over its bound: rejected, no acceptance draw
+            if (tid == 0) {
+                ++nmv;
+                if (c.rec) {
+                    yamc_move_rec r = {};
+                    r.imol = sh.m.imol; r.overlap = 2;
+                    c.rec[mv] = r;
+                }
+            }
+            __syncthreads();
+            continue;
+        }
+        // ---- new real-space partial sum (MC.f90:1077-1090) --------------
+        {
+            double acc[YK_NRS];
+#pragma unroll
+            for (int i = 0; i < YK_NRS; ++i) acc[i] = 0.0;
+            int ov = 0;
+            yk_mol_tables(c, sh.m, true);
+            yk_rs_new(c, sh.m, sig6, feps, acc, ov);
+            if (ov) sh.ov = 1;
+            yd_block_sum<YK_NRS>(acc, sh.red, sh.out);
+            if (tid == 0) {
+                for (int a = 0; a < sh.m.na; ++a) {
+                    uun[a] = sh.out[a];
+                    frn[a][0] = sh.out[YD_MAXA + 3 * a + 0];
+                    frn[a][1] = sh.out[YD_MAXA + 3 * a + 1];
+                    frn[a][2] = sh.out[YD_MAXA + 3 * a + 2];
+                }
+                for (int i = 0; i < 3; ++i) cmp_new[i] = sh.out[4 * YD_MAXA + i];
+            }
+        }
+        if (sh.ov) {  // overlap: rejected, no acceptance draw (MC.f90:1080-1084)
+            if (tid == 0) {
+                ++nmv; ++nov;
+                if (c.rec) {
+                    yamc_move_rec r = {};
+                    r.imol = sh.m.imol; r.overlap = 1;
+                    for (int d = 0; d < 3; ++d) { r.f_old[d] = mcm.fo[d]; r.t_old[d] = mcm.co[d]; r.delta[d] = mcm.d[d]; }
+                    r.angle = mcm.angle;
+                    c.rec[mv] = r;
+                }
+            }
+            __syncthreads();
+            continue;
+        }
+        // ---- new k-space sweep: dU and new forces (MC.f90:1096-1173) ----
+        {
+            double acc[YK_NACC];
+#pragma unroll
+            for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
+            yk_sweep<F32, true>(c, sh.m, acc);
+            yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
+        }
+        // ---- acceptance on thread 0 (MC.f90:1176-1233) ------------------
+        if (tid == 0) {
+            YkMol &m = sh.m;
+            int na = m.na;
+            double U_new = 0.0;
+            for (int a = 0; a < na; ++a) U_new = U_new + uun[a];
+            du_real = U_new - U_old;
+            duc = sh.out[0];
+            dulj = sh.out[1];
+            double fx[YD_MAXA], fy[YD_MAXA], fz[YD_MAXA];
+            for (int a = 0; a < na; ++a) {
+                fx[a] = frn[a][0] + YD_FOUR_PI * sh.out[YK_FC(a, 0)] + YD_FOUR_PI * sh.out[YK_FL(a, 0)];
+                fy[a] = frn[a][1] + YD_FOUR_PI * sh.out[YK_FC(a, 1)] + YD_FOUR_PI * sh.out[YK_FL(a, 1)];
+                fz[a] = frn[a][2] + YD_FOUR_PI * sh.out[YK_FC(a, 2)] + YD_FOUR_PI * sh.out[YK_FL(a, 2)];
+            }
+            double mun[3] = {0, 0, 0};
+            for (int a = 0; a < na; ++a) {
+                mun[0] = mun[0] + m.q[a] * m.xn[a];
+                mun[1] = mun[1] + m.q[a] * m.yn[a];
+                mun[2] = mun[2] + m.q[a] * m.zn[a];
+            }
+            double dmu[3] = {mun[0] - muo[0], mun[1] - muo[1], mun[2] - muo[2]};
+            du_ext = p.kext * ((2 * mu[0] + dmu[0]) * dmu[0] + (2 * mu[1] + dmu[1]) * dmu[1] +
+                               (2 * mu[2] + dmu[2]) * dmu[2]);
+            for (int d = 0; d < 3; ++d) mu_new[d] = mu[d] + dmu[d];
+            for (int a = 0; a < na; ++a) {
+                fx[a] = fx[a] + p.m2kext * m.q[a] * mu_new[0];
+                fy[a] = fy[a] + p.m2kext * m.q[a] * mu_new[1];
+                fz[a] = fz[a] + p.m2kext * m.q[a] * mu_new[2];
+            }
+            double Fn[3] = {0, 0, 0}, Tn[3] = {0, 0, 0}, com[3] = {0, 0, 0}, tm = 0.0, tq[3];
+            for (int a = 0; a < na; ++a) { Fn[0] = Fn[0] + fx[a]; Fn[1] = Fn[1] + fy[a]; Fn[2] = Fn[2] + fz[a]; }
+            for (int a = 0; a < na; ++a) {
+                com[0] = com[0] + m.mass[a] * m.xn[a];
+                com[1] = com[1] + m.mass[a] * m.yn[a];
+                com[2] = com[2] + m.mass[a] * m.zn[a];
+                tm = tm + m.mass[a];
+            }
+            com[0] = com[0] / tm; com[1] = com[1] / tm; com[2] = com[2] / tm;
+            for (int a = 0; a < na; ++a) {
+                yk_prod(m.xn[a] - com[0], m.yn[a] - com[1], m.zn[a] - com[2], fx[a], fy[a], fz[a], tq);
+                Tn[0] = Tn[0] + tq[0]; Tn[1] = Tn[1] + tq[1]; Tn[2] = Tn[2] + tq[2];
+            }
+            for (int d = 0; d < 3; ++d) { mcm.fn[d] = Fn[d]; mcm.cn[d] = Tn[d]; }
+            double du_total = du_real + (duc + dulj) + du_ext;
+            yk_init_new(p, mcm);
+            int accd = yk_accept(p, mcm, du_total, rng);
+            sh.accepted = accd;
+            ++nmv;
+            acc_count += accd;
+            if (c.rec) {
+                yamc_move_rec r = {};
+                r.imol = m.imol; r.accepted = accd; r.overlap = 0;
+                r.du_real = du_real; r.du_kc = duc; r.du_klj = dulj; r.du_ext = du_ext; r.du_total = du_total;
+                for (int d = 0; d < 3; ++d) {
+                    r.f_old[d] = mcm.fo[d]; r.t_old[d] = mcm.co[d]; r.f_new[d] = Fn[d]; r.t_new[d] = Tn[d];
+                    r.delta[d] = mcm.d[d];
+                }
+                r.angle = mcm.angle;
+                c.rec[mv] = r;
+            }
+        }
+        __syncthreads();
+        if (!sh.accepted) continue;
+        // ---- commit (MC.f90:1242-1318) ----------------------------------
+        yk_commit<F32>(c, sh.m);
+        {
+            double acc[3] = {0, 0, 0};
+            yk_rs_update(c, sh.m, sig6, feps, acc);
+            yd_block_sum<3>(acc, sh.red, sh.out);
+        }
+        if (tid == 0) {
+            YkMol &m = sh.m;
+            int f = m.first;
+            for (int a = 0; a < m.na; ++a) {
+                c.uu[f + a] = uun[a];
+                c.Fx[f + a] = frn[a][0];
+                c.Fy[f + a] = frn[a][1];
+                c.Fz[f + a] = frn[a][2];
+                c.x[f + a] = m.xn[a];
+                c.y[f + a] = m.yn[a];
+                c.z[f + a] = m.zn[a];
+            }
+            YdState *st = c.st;
+            st->e_kc = st->e_kc + duc;
+            st->e_klj = st->e_klj + dulj;
+            st->uu_ew = st->uu_ew + cmp_new[0] - sh.out[0];
+            st->uu_lj6 = st->uu_lj6 + cmp_new[1] - sh.out[1];
+            st->uu_lj12 = st->uu_lj12 + cmp_new[2] - sh.out[2];
+            for (int d = 0; d < 3; ++d) st->mu[d] = mu_new[d];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (mode != 1) c.st->ctr = rng.ctr;
+        c.st->n_moves += nmv;
+        c.st->n_accepted += acc_count;
+        c.st->n_overlap += nov;
+        c.st->since_resync += nmv;
+    }
+}
+
+template __global__ void yk_chain<false>(const YdChain *, const double *const *, const double *const *, int, int,
+                                         int);
+template __global__ void yk_chain<true>(const YdChain *, const double *const *, const double *const *, int, int,
+                                        int);
+
+// ========================================================================
+// Single-call entry points (one workgroup): the per-function parity surface.
+
+// calc_dU_coulomb + calc_dU_LJ for molecule imol moved to (x,y,z).
+template <bool F32>
+__global__ void __launch_bounds__(YK_BS) yk_kspace_delta(YdChain c, int imol, const double *x, const double *y,
+                                                         const double *z, double *out) {
+    __shared__ YkShared sh;
+    if (threadIdx.x == 0) {
+        yk_load_mol(c, sh.m, imol);
+        for (int a = 0; a < sh.m.na; ++a) { sh.m.xn[a] = x[a]; sh.m.yn[a] = y[a]; sh.m.zn[a] = z[a]; }
+    }
+    __syncthreads();
+    yk_mol_tables(c, sh.m, false);
+    __syncthreads();
+    yk_mol_tables(c, sh.m, true);
+    __syncthreads();
+    double acc[YK_NACC];
+    for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
+    yk_sweep<F32, true>(c, sh.m, acc);
+    yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
+    if (threadIdx.x == 0) { out[0] = sh.out[0]; out[1] = sh.out[1]; }
+}
+template __global__ void yk_kspace_delta<false>(YdChain, int, const double *, const double *, const double *,
+                                                double *);
+template __global__ void yk_kspace_delta<true>(YdChain, int, const double *, const double *, const double *,
+                                               double *);
+
+// ForceKSpace_coulomb + _LJ for n <= YD_MAXA atoms first..first+n-1 at (x,y,z).
+template <bool F32>
+__global__ void __launch_bounds__(YK_BS) yk_kspace_forces(YdChain c, int first, int n, const double *x,
+                                                          const double *y, const double *z, double *out) {
+    __shared__ YkShared sh;
+    if (threadIdx.x == 0) {
+        YkMol &m = sh.m;
+        m.imol = -1; m.first = first; m.na = n; m.np = 0;
+        for (int a = 0; a < n; ++a) {
+            m.q[a] = c.q[first + a];
+            m.typ[a] = c.type[first + a];
+            m.xo[a] = x[a]; m.yo[a] = y[a]; m.zo[a] = z[a];
+            m.lslot[a] = -1;
+            if (m.typ[a] > 0 && m.np == 0) { m.ptype[0] = m.typ[a]; m.np = 1; }
+        }
+    }
+    __syncthreads();
+    yk_mol_tables(c, sh.m, false);
+    __syncthreads();
+    double acc[YK_NACC];
+    for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
+    yk_sweep<F32, false>(c, sh.m, acc);
+    yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
+    if (threadIdx.x < 3 * n) {
+        int a = threadIdx.x / 3, d = threadIdx.x % 3;
+        out[threadIdx.x] = YD_FOUR_PI * sh.out[YK_FC(a, d)] + YD_FOUR_PI * sh.out[YK_FL(a, d)];
+    }
+}
+template __global__ void yk_kspace_forces<false>(YdChain, int, int, const double *, const double *, const double *,
+                                                 double *);
+template __global__ void yk_kspace_forces<true>(YdChain, int, int, const double *, const double *, const double *,
+                                                double *);
+
+// Real-space partial sum of molecule imol at (x,y,z), with back arrays.
+// out: [0..na) uu, [YD_MAXA + 3a + d] F, [16..18] comps, [19] overlap.
+__global__ void __launch_bounds__(YK_BS) yk_rspace_partial(YdChain c, int imol, const double *x, const double *y,
+                                                           const double *z, const double *sig6, const double *feps,
+                                                           double *out, double *uub, double *fxb, double *fyb,
+                                                           double *fzb) {
+    __shared__ YkShared sh;
+    if (threadIdx.x == 0) {
+        yk_load_mol(c, sh.m, imol);
+        for (int a = 0; a < sh.m.na; ++a) { sh.m.xn[a] = x[a]; sh.m.yn[a] = y[a]; sh.m.zn[a] = z[a]; }
+        sh.ov = 0;
+    }
+    __syncthreads();
+    double acc[YK_NRS];
+    for (int i = 0; i < YK_NRS; ++i) acc[i] = 0.0;
+    int ov = 0;
+    yk_rs_new(c, sh.m, sig6, feps, acc, ov);
+    if (ov) sh.ov = 1;
+    yd_block_sum<YK_NRS>(acc, sh.red, sh.out);
+    if (threadIdx.x < YK_NRS) out[threadIdx.x] = sh.out[threadIdx.x];
+    if (threadIdx.x == 0) out[YK_NRS] = sh.ov;
+    if (uub) {
+        const int N = c.p.natom, T = c.p.ntype, me = imol + 1;
+        const YkMol &m = sh.m;
+        for (int j = threadIdx.x; j < N; j += YK_BS) {
+            double b[4] = {0, 0, 0, 0};
+            if (c.molnum[j] != me) {
+                int jt = c.type[j];
+                for (int a = 0; a < m.na; ++a) {
+                    int it = m.typ[a];
+                    double xij = yd_wrap(m.xn[a] - c.x[j]), yij = yd_wrap(m.yn[a] - c.y[j]),
+                           zij = yd_wrap(m.zn[a] - c.z[j]);
+                    double r2 = xij * xij + yij * yij + zij * zij;
+                    if (r2 >= c.p.rmax2) continue;
+                    bool lj = it != 0 && jt != 0;
+                    int off = lj ? (it - 1) * T + (jt - 1) : 0;
+                    YdPair pr;
+                    yd_pair(c.p, r2, false, m.q[a], c.q[j], lj, sig6[off], feps[off], pr);
+                    b[0] = b[0] + pr.pot;
+                    b[1] = b[1] - xij * pr.FR;
+                    b[2] = b[2] - yij * pr.FR;
+                    b[3] = b[3] - zij * pr.FR;
+                }
+            }
+            uub[j] = b[0]; fxb[j] = b[1]; fyb[j] = b[2]; fzb[j] = b[3];
+        }
+    }
+}
