@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark: Ewald Monte Carlo trial moves/s on one MI355X.
+
+Workload (the metric's configuration): 2048 SPC/E + 40 Na+ + 40 Cl- (6224
+atoms) in a 39.4 A box, T = 298 K, rc = 12 A, kmax = 12 (nk = 1068), Coulomb
++ LJ-Ewald, force-biased single-molecule moves (lambda_f = lambda_c = 0.5).
+A step is one MC sweep: nmol = 2128 trial moves of every chain, run on the
+device (molecule pick, proposal, dE, Metropolis test and commit all inside
+the chain kernel).
+
+Forms timed:
+  fp64      plain FP64, one chain (the reference algorithm on one replica)
+  fp32c     FP32-compensated S(k), one chain
+  batch     FP32-compensated, R independent replicas of the same system in
+            one persistent launch (one workgroup per replica) -> `value`
+The per-move dE error of fp32c against fp64 is measured on one stream
+(moves up to the first differing decision).  The CPU baseline is the FP64
+oracle (a plain-C restatement of the reference) timed on this host.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+KT_KJMOL_298 = None  # filled from the reference constants below
+
+
+def algorithmic_bytes_per_move(sysm, nk_info, w):
+    """k-space bytes per trial move: w * [2 (1+T) n_s + (1 + T |t_M|) nk]
+    (the S(k) cos/sin of the Coulomb set and all T LJ sets over the nonzero
+    sign slots, plus beta_C and the beta_LJ rows of the moved molecule's type;
+    |t_M| = 1 for every species here)."""
+    ky, kz = nk_info["ky"], nk_info["kz"]
+    n_s = int(sum(1 + (a > 0) + (b > 0) + ((a > 0) and (b > 0)) for a, b in zip(ky, kz)))
+    T = sysm.ntype
+    nk = len(ky)
+    return w * (2 * (1 + T) * n_s + (1 + T * 1) * nk), n_s
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--cpu-moves", type=int, default=0, help="moves for the CPU baseline (0 = auto ~15 s)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp64,fp32c,batch")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(replicas would shard across GPUs with no communication). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from yamc_amd import YAMC_FP32C, YAMC_FP64, config, system
+    from yamc_amd.ewald import EwaldMC, batch_run
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    kT_kjmol = system.BOLTZ * 298.0 / system.KCAL_MOL * 4.184
+
+    forms = args.forms.split(",")
+    base = config(args.workload, seed=20261203)
+    nmol = base.nmol
+    res = {}
+
+    def sync():
+        torch.cuda.synchronize()
+
+    # ---------------------------------------------------------- single chains
+    singles = {}
+    for form, prec in (("fp64", YAMC_FP64), ("fp32c", YAMC_FP32C)):
+        if form not in forms and not (form == "fp64" and "err" in forms):
+            continue
+        g = EwaldMC(base, precision=prec)
+        g.mc_init()
+        g.set_rng(20261203, 0)
+        for _ in range(args.warmup):
+            g.run(1)
+        sync()
+        t0 = time.perf_counter()
+        kms = 0.0
+        nacc = 0
+        for _ in range(args.steps):
+            n, _ = g.run(1, resync_every=1000 if prec == YAMC_FP32C else 0)
+            kms += g.last_kernel_ms()
+            nacc += n
+        sync()
+        dt = time.perf_counter() - t0
+        moves = args.steps * nmol
+        singles[form] = g
+        res[form] = dict(moves_per_s=moves / dt, ms_per_step=1e3 * dt / args.steps, kernel_ms=kms,
+                         acceptance=nacc / moves)
+
+    # ------------------------------------------------------- dE error, fp32c
+    err = None
+    if "fp64" in forms and "fp32c" in forms:
+        a, b = EwaldMC(base, precision=YAMC_FP64), EwaldMC(base, precision=YAMC_FP32C)
+        a.mc_init(); b.mc_init()
+        a.set_rng(7, 0); b.set_rng(7, 0)
+        _, ra = a.run(1, records=True)
+        _, rb = b.run(1, records=True)
+        same = ra["accepted"] == rb["accepted"]
+        n = int(np.argmin(same)) if not same.all() else len(ra)
+        ok = (ra["overlap"][:n] == 0)
+        d = np.abs(ra["du_total"][:n] - rb["du_total"][:n])[ok]
+        fo = np.abs(ra["f_old"][:n] - rb["f_old"][:n])[ok] / np.maximum(np.abs(ra["f_old"][:n])[ok], 1e-30)
+        err = dict(moves_compared=int(n), max_dE_kT=float(d.max()) if d.size else 0.0,
+                   max_dE_kJmol=float(d.max() * kT_kjmol) if d.size else 0.0,
+                   rms_dE_kT=float(np.sqrt((d ** 2).mean())) if d.size else 0.0,
+                   max_rel_force_err=float(fo.max()) if fo.size else 0.0,
+                   decision_agreement_first_sweep=float(same.mean()))
+
+    # ------------------------------------------------------ batched replicas
+    if "batch" in forms:
+        R = args.replicas
+        reps = []
+        for i in range(R):
+            s = config(args.workload, seed=20261203 + i)
+            r = EwaldMC(s, precision=YAMC_FP32C)
+            r.mc_init()
+            r.set_rng(20261203, i)
+            reps.append(r)
+        for _ in range(args.warmup):
+            batch_run(reps, 1)
+        sync()
+        t0 = time.perf_counter()
+        kms = 0.0
+        nacc = 0
+        for _ in range(args.steps):
+            acc, ms = batch_run(reps, 1)
+            kms += ms
+            nacc += int(acc.sum())
+        sync()
+        dt = time.perf_counter() - t0
+        moves = args.steps * nmol * R
+        res["batch"] = dict(replicas=R, moves_per_s=moves / dt, ms_per_step=1e3 * dt / args.steps, kernel_ms=kms,
+                            acceptance=nacc / moves)
+
+    # ------------------------------------------------------------ roofline
+    g0 = next(iter(singles.values())) if singles else reps[0]
+    grid = g0.grid()
+    main_form = "batch" if "batch" in res else ("fp32c" if "fp32c" in res else "fp64")
+    w = 4 if main_form in ("batch", "fp32c") else 8
+    bpm, n_s = algorithmic_bytes_per_move(base, grid, w)
+    r0 = res[main_form]
+    moves_main = args.steps * nmol * (r0.get("replicas", 1))
+    launch_s = r0["kernel_ms"] / 1e3 / args.steps
+    achieved = bpm * moves_main / args.steps / launch_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_c2.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get(main_form)
+        except Exception:
+            traffic = None
+
+    # --------------------------------------------------------- CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle
+        o = pyoracle.Oracle(base)
+        o.mc_init()
+        o.set_rng(20261203, 0)
+        im = np.random.default_rng(1).integers(0, nmol, 64).astype(np.int32)
+        t0 = time.perf_counter()
+        o.run_const(im, np.full(64, 0.4))
+        per = (time.perf_counter() - t0) / 64
+        nmv = args.cpu_moves or max(64, min(20000, int(15.0 / per)))
+        o2 = pyoracle.Oracle(base)
+        o2.mc_init()
+        o2.set_rng(20261203, 0)
+        ncyc = max(1, nmv // nmol)
+        t0 = time.perf_counter()
+        if nmv >= nmol:
+            o2.run(ncyc, records=False)
+            done = ncyc * nmol
+        else:
+            im = np.random.default_rng(2).integers(0, nmol, nmv).astype(np.int32)
+            o2.run_const(im, np.full(nmv, 0.4))
+            done = nmv
+        dtc = time.perf_counter() - t0
+        cpu = dict(value=done / dtc, unit="trial moves/s", cores=1, kind="port",
+                   sample=f"{done} FP64 trial moves of the same 6224-atom workload, plain-C oracle, 1 thread")
+
+    value = res[main_form]["moves_per_s"]
+    out = {
+        "metric": "MC trial moves/s at 2048 SPC/E+NaCl, FP32-comp vs FP64 GPU; dE err vs FP64",
+        "value": value,
+        "unit": "trial moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res[main_form]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 S(k) sweep + fp64 accept" if main_form != "fp64" else "f64",
+        "data": "synthetic: seeded lattice start, seeded orientations",
+        "config": {
+            "workload": f"{args.workload}: 2048 SPC/E + 40 Na + 40 Cl, L=39.4 A, rc=12 A, kmax=12, "
+                        f"force-biased moves; step = 1 sweep (2128 moves) per chain",
+            "form": main_form,
+            "replicas": res[main_form].get("replicas", 1),
+            "nk": len(grid["kx"]),
+            "n_s": n_s,
+        },
+        "forms": res,
+        "dE_error_fp32c_vs_fp64": err,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_move": bpm,
+            "kernel": "yk_chain",
+        },
+        "cpu_baseline": cpu,
+    }
+    if "fp64" in res and "batch" in res:
+        out["batch_vs_fp64_single"] = res["batch"]["moves_per_s"] / res["fp64"]["moves_per_s"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
