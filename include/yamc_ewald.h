@@ -199,6 +199,10 @@ int yamc_forces(yamc_ctx *ctx, double *gx, double *gy, double *gz, double *rx, d
 /* Time of the last yamc_mc_run / yamc_batch_run kernel launch in ms (HIP
  * events on the context stream). */
 int yamc_last_kernel_ms(yamc_ctx *ctx, double *ms);
+/* Cumulative wall-clock microseconds spent by the chain kernel in its nine
+ * phases (pick, mu + tables, old k sweep, proposal, real-space new sum, new
+ * k sweep, acceptance, commit, finalize); us[9]. */
+int yamc_phase_times(yamc_ctx *ctx, double *us);
 const char *yamc_last_error(void);
 
 #ifdef __cplusplus

@@ -39,7 +39,9 @@ __global__ void yk_rspace_partial(YdChain c, int imol, const double *x, const do
 void yk_launch_forces(const YdChain &c, const double *sig6, const double *feps, const double *tab, double *g,
                       double *r, hipStream_t s);
 
+#ifndef YK_BS
 #define YK_BS 512
+#endif
 
 static thread_local char g_err[512] = "";
 
@@ -741,6 +743,17 @@ extern "C" int yamc_forces(yamc_ctx *c, double *gx, double *gy, double *gz, doub
         if (gx) { gx[i] = hg[i]; gy[i] = hg[N + i]; gz[i] = hg[2 * N + i]; }
         if (rx) { rx[i] = hr[i]; ry[i] = hr[N + i]; rz[i] = hr[2 * N + i]; }
     }
+    return YAMC_OK;
+}
+
+extern "C" int yamc_phase_times(yamc_ctx *c, double *us) {
+    if (!c || !us) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    YdState st;
+    HIPCHK(hipMemcpyAsync(&st, c->ch.st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+    for (int i = 0; i < 9; ++i) us[i] = khz > 0 ? (double)st.phase[i] * 1e3 / khz : 0.0;
     return YAMC_OK;
 }
 

@@ -41,6 +41,7 @@ struct YdState {
     uint32_t key[2], stream[2];
     unsigned long long ctr;
     long long n_moves, n_accepted, n_overlap, since_resync;
+    unsigned long long phase[10];  // wall-clock ticks per chain-kernel phase (thread 0)
 };
 
 struct YdChain {

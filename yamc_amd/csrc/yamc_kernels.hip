@@ -11,7 +11,11 @@
 // Nothing here is a dense contraction, so no MFMA is used.
 #include "yamc_device.h"
 
-#define YK_BS 512            // workgroup size of the chain kernel
+#ifndef YK_BS
+#define YK_BS 512            // workgroup size of the chain kernel (8 waves)
+
+
+#endif
 #define YK_NW (YK_BS / 64)
 #define YK_MAXP 2            // distinct LJ types inside one molecule
 // Bound on every rejection-sampling loop of the proposal.  The reference
@@ -19,6 +23,9 @@
 // loops never succeed, so a move that exceeds the bound is rejected without
 // an acceptance draw and recorded with overlap = 2.
 #define YK_TRIES 65536
+// Candidate neighbours of one move kept in LDS: atoms within the cut-off of
+// a new or an old site of the moved molecule.
+#define YK_NBCAP 2048
 
 // ------------------------------------------------------------------------
 // Functions.f90:26-111 erfc_Luc_bet6_bet12
@@ -302,7 +309,7 @@ __global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, do
 struct YkMol {
     int imol, first, na, np;
     int typ[YD_MAXA], lslot[YD_MAXA], ptype[YK_MAXP];
-    double q[YD_MAXA], mass[YD_MAXA];
+    double q[YD_MAXA], mass[YD_MAXA], dh[YD_MAXA];
     double xo[YD_MAXA], yo[YD_MAXA], zo[YD_MAXA], xn[YD_MAXA], yn[YD_MAXA], zn[YD_MAXA];
     double tab_o[YD_MAXA * 6 * YD_KP], tab_n[YD_MAXA * 6 * YD_KP];
 };
@@ -331,7 +338,9 @@ struct YkShared {
     YkScalar t0;
     double red[YK_NW * 32];
     double out[32];
-    int ov, accepted, stall;
+    double fr_old[YD_MAXA][3], uu_old[YD_MAXA];
+    int ov, accepted, stall, nbcount;
+    int nblist[YK_NBCAP];
 };
 
 // Loads molecule imol (old positions from the state) into LDS; thread 0.
@@ -345,6 +354,7 @@ __device__ void yk_load_mol(const YdChain &c, YkMol &m, int imol) {
         m.q[a] = c.q[g];
         m.typ[a] = c.type[g];
         m.mass[a] = c.mass[g];
+        m.dh[a] = c.dh[g];
         m.xo[a] = c.x[g];
         m.yo[a] = c.y[g];
         m.zo[a] = c.z[g];
@@ -418,17 +428,30 @@ __device__ __forceinline__ void yk_ld8(const YdChain &c, int set, int kk, double
     }
 }
 
-// Accumulator layout of the sweeps: [0] dU_C, [1] dU_LJ, [2 + 3a + d]
-// Coulomb force on atom a, [2 + 3*MAXA + 3a + d] LJ force on atom a.
-#define YK_NACC (2 + 6 * YD_MAXA)
-#define YK_FC(a, d) (2 + 3 * (a) + (d))
-#define YK_FL(a, d) (2 + 3 * YD_MAXA + 3 * (a) + (d))
+// Accumulator layout of the sweeps: [0] dU_C, [1] dU_LJ, [2 + 3a + d] the
+// k-space force on atom a (Coulomb and LJ parts summed per k; the total over
+// k differs from the reference's separate sums by rounding only).
+#define YK_NACC (2 + 3 * YD_MAXA)
+#define YK_FK(a, d) (2 + 3 * (a) + (d))
+
+// acc[base + 3a + d] += f[d] for a runtime site index a, written so that acc
+// stays in registers (every index is a compile-time constant).
+template <int N>
+__device__ __forceinline__ void yk_scatter3(double (&acc)[N], int base, int a, const double *f) {
+#pragma unroll
+    for (int b = 0; b < YD_MAXA; ++b)
+        if (b == a) {
+            acc[base + 3 * b + 0] += f[0];
+            acc[base + 3 * b + 1] += f[1];
+            acc[base + 3 * b + 2] += f[2];
+        }
+}
 
 // One sweep over k.  DELTA = false: k-space forces of the molecule at its
 // old positions against S (MC.f90:963-968).  DELTA = true: dU_C, dU_LJ
 // against S and forces at new positions against S + delta
 // (MC.f90:1136-1173).
-template <bool F32, bool DELTA>
+template <bool F32, bool DELTA, int NA>
 __device__ void yk_sweep(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC]) {
     const int nk = c.p.nk, T = c.p.ntype;
     const double *tabs = DELTA ? m.tab_n : m.tab_o;
@@ -443,14 +466,15 @@ __device__ void yk_sweep(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC
 #pragma unroll
             for (int i = 0; i < 8; ++i) C[i] = C[i] + dc[i];
         }
-        for (int a = 0; a < m.na; ++a) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
             if (m.q[a] == 0.0) continue;
             yd_kr(tabs + a * 6 * YD_KP, m.q[a], kx, ky, kz, kr);
             double f[3] = {0, 0, 0};
             yd_force_k(C, kr, kx, ky, kz, bt, f);
-            acc[YK_FC(a, 0)] += f[0];
-            acc[YK_FC(a, 1)] += f[1];
-            acc[YK_FC(a, 2)] += f[2];
+            acc[YK_FK(a, 0)] += f[0];
+            acc[YK_FK(a, 1)] += f[1];
+            acc[YK_FK(a, 2)] += f[2];
         }
         if (T == 0 || m.np == 0) continue;
         double dl[YK_MAXP][8];
@@ -483,15 +507,16 @@ __device__ void yk_sweep(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC
                         acc[1] += yd_dulj_k(tw, dl[p], ky, kz) *
                                   c.beta_lj[((size_t)t1 * T + (m.ptype[p] - 1)) * nk + kk];
             }
-            for (int a = 0; a < m.na; ++a) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
                 int ta = m.typ[a];
                 if (ta <= 0) continue;
                 yd_kr_atom(tabs + a * 6 * YD_KP, m.q[a], ta, kx, ky, kz, kr, krl);
                 double f[3] = {0, 0, 0};
                 yd_force_k(C1, krl, kx, ky, kz, c.beta_lj[((size_t)(ta - 1) * T + t1) * nk + kk], f);
-                acc[YK_FL(a, 0)] += f[0];
-                acc[YK_FL(a, 1)] += f[1];
-                acc[YK_FL(a, 2)] += f[2];
+                acc[YK_FK(a, 0)] += f[0];
+                acc[YK_FK(a, 1)] += f[1];
+                acc[YK_FK(a, 2)] += f[2];
             }
         }
     }
@@ -530,6 +555,24 @@ __device__ void yk_commit(const YdChain &c, const YkMol &m) {
 // other atoms (EwaldSumRealSpace.f90:338-498, partial mode).
 // Accumulators: [a] uu, [4 + 3a + d] F, [16..18] uu_ew, uu_lj6, uu_lj12.
 #define YK_NRS (4 * YD_MAXA + 3)
+
+// Adds one pair of site a into the partial-sum accumulators (constant
+// indices after unrolling, so acc stays in registers).
+__device__ __forceinline__ void yk_rs_acc(double (&acc)[YK_NRS], int a, const YdPair &pr, double xij, double yij,
+                                          double zij) {
+#pragma unroll
+    for (int b = 0; b < YD_MAXA; ++b)
+        if (b == a) {
+            acc[b] += pr.pot;
+            acc[YD_MAXA + 3 * b + 0] += xij * pr.FR;
+            acc[YD_MAXA + 3 * b + 1] += yij * pr.FR;
+            acc[YD_MAXA + 3 * b + 2] += zij * pr.FR;
+        }
+    acc[4 * YD_MAXA + 0] += pr.potew;
+    acc[4 * YD_MAXA + 1] += pr.potlj6;
+    acc[4 * YD_MAXA + 2] += pr.potlj12;
+}
+template <int NA>
 __device__ void yk_rs_new(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
                           double (&acc)[YK_NRS], int &ov) {
     const int N = c.p.natom, T = c.p.ntype, me = m.imol + 1;
@@ -538,24 +581,19 @@ __device__ void yk_rs_new(const YdChain &c, const YkMol &m, const double *sig6, 
         if (jm == me) continue;
         int jt = c.type[j];
         double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
-        for (int a = 0; a < m.na; ++a) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
             int it = m.typ[a];
             double xij = yd_wrap(m.xn[a] - xj), yij = yd_wrap(m.yn[a] - yj), zij = yd_wrap(m.zn[a] - zj);
             double r2 = xij * xij + yij * yij + zij * zij;
-            double d12 = (c.dh[m.first + a] + dhj) / 2.0;
+            double d12 = (m.dh[a] + dhj) / 2.0;
             if (it > 0 && jt > 0 && r2 < d12 * d12) ov = 1;
             if (r2 >= c.p.rmax2) continue;
             bool lj = it != 0 && jt != 0;
             int off = lj ? (it - 1) * T + (jt - 1) : 0;
             YdPair pr;
             yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
-            acc[a] += pr.pot;
-            acc[YD_MAXA + 3 * a + 0] += xij * pr.FR;
-            acc[YD_MAXA + 3 * a + 1] += yij * pr.FR;
-            acc[YD_MAXA + 3 * a + 2] += zij * pr.FR;
-            acc[4 * YD_MAXA + 0] += pr.potew;
-            acc[4 * YD_MAXA + 1] += pr.potlj6;
-            acc[4 * YD_MAXA + 2] += pr.potlj12;
+            yk_rs_acc(acc, a, pr, xij, yij, zij);
         }
     }
 }
@@ -603,6 +641,129 @@ __device__ void yk_rs_update(const YdChain &c, const YkMol &m, const double *sig
             c.Fy[j] = c.Fy[j] + nb[2] - ob[2];
             c.Fz[j] = c.Fz[j] + nb[3] - ob[3];
         }
+    }
+}
+
+
+// Fused real-space pass of a trial move over every other atom j: the new
+// partial sum (EwaldSumRealSpace.f90:338-498 for the molecule at its new
+// sites) and the list of atoms within the cut-off of a new or an old site,
+// which are the only atoms whose per-atom arrays change on acceptance
+// (MC.f90:1251-1275).  *count may exceed YK_NBCAP; the list is then partial.
+template <int NA>
+__device__ void yk_rs_move(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
+                           double (&acc)[YK_NRS], int &ov, int *list, int *count) {
+    const int N = c.p.natom, T = c.p.ntype, me = m.imol + 1;
+    for (int j = threadIdx.x; j < N; j += YK_BS) {
+        int jm = c.molnum[j];
+        if (jm == me) continue;
+        int jt = c.type[j];
+        double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+        bool cand = false;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            int it = m.typ[a];
+            double xo = yd_wrap(m.xo[a] - xj), yo = yd_wrap(m.yo[a] - yj), zo = yd_wrap(m.zo[a] - zj);
+            if (xo * xo + yo * yo + zo * zo < c.p.rmax2) cand = true;
+            double xij = yd_wrap(m.xn[a] - xj), yij = yd_wrap(m.yn[a] - yj), zij = yd_wrap(m.zn[a] - zj);
+            double r2 = xij * xij + yij * yij + zij * zij;
+            double d12 = (m.dh[a] + dhj) / 2.0;
+            if (it > 0 && jt > 0 && r2 < d12 * d12) ov = 1;
+            if (r2 >= c.p.rmax2) continue;
+            cand = true;
+            bool lj = it != 0 && jt != 0;
+            int off = lj ? (it - 1) * T + (jt - 1) : 0;
+            YdPair pr;
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            yk_rs_acc(acc, a, pr, xij, yij, zij);
+        }
+        if (cand) {
+            int slot = atomicAdd(count, 1);
+            if (slot < YK_NBCAP) list[slot] = j;
+        }
+    }
+}
+
+// Accepted move, one atom j: its per-atom arrays take the molecule's new
+// minus old contribution, summed over the molecule's sites in index order as
+// the back arrays of the two partial sums hold them; acc[0..2] gather the
+// old partial sum's uu_ew, uu_lj6, uu_lj12.
+__device__ __forceinline__ void yk_rs_apply(const YdChain &c, const YkMol &m, const double *sig6,
+                                            const double *feps, int j, double (&acc)[3]) {
+    const int T = c.p.ntype;
+    int jt = c.type[j];
+    double qj = c.q[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+    double nb[4] = {0, 0, 0, 0}, ob[4] = {0, 0, 0, 0};
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int a = 0; a < m.na; ++a) {
+            int it = m.typ[a];
+            double xa = pass == 0 ? m.xn[a] : m.xo[a], ya = pass == 0 ? m.yn[a] : m.yo[a],
+                   za = pass == 0 ? m.zn[a] : m.zo[a];
+            double xij = yd_wrap(xa - xj), yij = yd_wrap(ya - yj), zij = yd_wrap(za - zj);
+            double r2 = xij * xij + yij * yij + zij * zij;
+            if (r2 >= c.p.rmax2) continue;
+            bool lj = it != 0 && jt != 0;
+            int off = lj ? (it - 1) * T + (jt - 1) : 0;
+            YdPair pr;
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            double fx = xij * pr.FR, fy = yij * pr.FR, fz = zij * pr.FR;
+            if (pass == 0) {
+                nb[0] = nb[0] + pr.pot; nb[1] = nb[1] - fx; nb[2] = nb[2] - fy; nb[3] = nb[3] - fz;
+            } else {
+                ob[0] = ob[0] + pr.pot; ob[1] = ob[1] - fx; ob[2] = ob[2] - fy; ob[3] = ob[3] - fz;
+                acc[0] += pr.potew;
+                acc[1] += pr.potlj6;
+                acc[2] += pr.potlj12;
+            }
+        }
+    }
+    c.uu[j] = c.uu[j] + nb[0] - ob[0];
+    c.Fx[j] = c.Fx[j] + nb[1] - ob[1];
+    c.Fy[j] = c.Fy[j] + nb[2] - ob[2];
+    c.Fz[j] = c.Fz[j] + nb[3] - ob[3];
+}
+
+// Commit of the real-space arrays: over the candidate list, or over every
+// atom when the list overflowed.
+__device__ void yk_rs_commit(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
+                             const int *list, int count, double (&acc)[3]) {
+    if (count <= YK_NBCAP) {
+        for (int i = threadIdx.x; i < count; i += YK_BS) yk_rs_apply(c, m, sig6, feps, list[i], acc);
+    } else {
+        const int me = m.imol + 1;
+        for (int j = threadIdx.x; j < c.p.natom; j += YK_BS)
+            if (c.molnum[j] != me) yk_rs_apply(c, m, sig6, feps, j, acc);
+    }
+}
+
+// Dispatch on the molecule size so every per-site loop has a compile-time
+// trip count (keeps the per-site accumulators in registers).
+template <bool F32, bool DELTA>
+__device__ __forceinline__ void yk_sweep_n(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC]) {
+    switch (m.na) {
+        case 1: yk_sweep<F32, DELTA, 1>(c, m, acc); break;
+        case 2: yk_sweep<F32, DELTA, 2>(c, m, acc); break;
+        case 3: yk_sweep<F32, DELTA, 3>(c, m, acc); break;
+        default: yk_sweep<F32, DELTA, 4>(c, m, acc); break;
+    }
+}
+__device__ __forceinline__ void yk_rs_move_n(const YdChain &c, const YkMol &m, const double *sig6,
+                                             const double *feps, double (&acc)[YK_NRS], int &ov, int *list,
+                                             int *count) {
+    switch (m.na) {
+        case 1: yk_rs_move<1>(c, m, sig6, feps, acc, ov, list, count); break;
+        case 2: yk_rs_move<2>(c, m, sig6, feps, acc, ov, list, count); break;
+        case 3: yk_rs_move<3>(c, m, sig6, feps, acc, ov, list, count); break;
+        default: yk_rs_move<4>(c, m, sig6, feps, acc, ov, list, count); break;
+    }
+}
+__device__ __forceinline__ void yk_rs_new_n(const YdChain &c, const YkMol &m, const double *sig6,
+                                            const double *feps, double (&acc)[YK_NRS], int &ov) {
+    switch (m.na) {
+        case 1: yk_rs_new<1>(c, m, sig6, feps, acc, ov); break;
+        case 2: yk_rs_new<2>(c, m, sig6, feps, acc, ov); break;
+        case 3: yk_rs_new<3>(c, m, sig6, feps, acc, ov); break;
+        default: yk_rs_new<4>(c, m, sig6, feps, acc, ov); break;
     }
 }
 
@@ -793,9 +954,18 @@ __device__ int yk_accept(const YdParams &p, const YkMcm &m, double dutot, YdRng 
 }
 
 // BiasedRandom.f90:192-220 with flat frequencies: first i with
-// beg(i) < rnd < end(i), else 1 (index 0 here).
+// beg(i) < rnd < end(i), else 1 (index 0 here).  The cumulative ends are
+// i/nmol to rounding, so the interval is looked up next to floor(rnd*nmol);
+// a binary search covers any case the guess misses.
 __device__ int yk_choose(const YdChain &c, double rnd) {
-    int lo = 0, hi = c.p.nmol - 1;
+    const int n = c.p.nmol;
+    int g = (int)(rnd * n);
+    if (g > n - 1) g = n - 1;
+    for (int i = (g > 0 ? g - 1 : 0); i <= (g + 1 < n ? g + 1 : n - 1); ++i) {
+        double beg = i > 0 ? c.freq_end[i - 1] : 0.0;
+        if (rnd > beg && rnd < c.freq_end[i]) return i;
+    }
+    int lo = 0, hi = n - 1;
     while (lo < hi) {  // first i with rnd < end(i)
         int mid = (lo + hi) >> 1;
         if (rnd < c.freq_end[mid]) hi = mid;
@@ -805,6 +975,18 @@ __device__ int yk_choose(const YdChain &c, double rnd) {
     if (rnd > beg && rnd < c.freq_end[lo]) return lo;
     return 0;
 }
+
+// Phase timers of the chain kernel: thread 0 adds wall-clock ticks between
+// workgroup barriers into st->phase[] (pick, tables, old sweep, proposal,
+// real-space pass, new sweep, acceptance, commit, finalize).
+#define YK_TICK(i)                                                  \
+    do {                                                            \
+        if (tid == 0) {                                             \
+            unsigned long long t_ = wall_clock64();                 \
+            ph[i] += t_ - t_last;                                   \
+            t_last = t_;                                            \
+        }                                                           \
+    } while (0)
 
 // ========================================================================
 // The chain kernel.  mode 0: run_mc cycles (runmc.f90:204-268) with the
@@ -826,7 +1008,21 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
     double &duc = t0.duc, &dulj = t0.dulj, &du_real = t0.du_real, &du_ext = t0.du_ext;
     double(&mu)[3] = t0.mu, (&mu_new)[3] = t0.mu_new, (&muo)[3] = t0.muo;
     long long &acc_count = t0.acc_count, &nmv = t0.nmv, &nov = t0.nov;
+    unsigned long long ph[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, t_last = 0;
+    // mu = sum_j q_j r_j from scratch once per launch (EwaldSumExternal.f90:
+    // 165-183); afterwards it is carried incrementally, mu + dmu on each
+    // accepted move, which is the value movemc recomputes (MC.f90:916).
+    {
+        double v[3] = {0, 0, 0};
+        for (int i = tid; i < p.natom; i += YK_BS) {
+            double q = c.q[i];
+            v[0] += q * c.x[i]; v[1] += q * c.y[i]; v[2] += q * c.z[i];
+        }
+        yd_block_sum<3>(v, sh.red, sh.out);
+    }
     if (tid == 0) {
+        mu[0] = sh.out[0]; mu[1] = sh.out[1]; mu[2] = sh.out[2];
+        t_last = wall_clock64();
         acc_count = 0; nmv = 0; nov = 0;
         rng.key0 = c.st->key[0]; rng.key1 = c.st->key[1];
         rng.s0 = c.st->stream[0]; rng.s1 = c.st->stream[1];
@@ -847,40 +1043,62 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
                 (void)rng.next();                       // exchange draw per step
                 imol = yk_choose(c, rng.next());
             }
-            yk_load_mol(c, sh.m, imol);
+            YkMol &m = sh.m;
+            m.imol = imol;
+            m.first = c.first[imol];
+            m.na = c.last[imol] - m.first + 1;
             sh.ov = 0;
+            sh.nbcount = 0;
         }
         __syncthreads();
-        // ---- mu from scratch (MC.f90:916) + old axis tables -------------
-        {
-            double v[3] = {0, 0, 0};
-            for (int i = tid; i < p.natom; i += YK_BS) {
-                double q = c.q[i];
-                v[0] += q * c.x[i]; v[1] += q * c.y[i]; v[2] += q * c.z[i];
-            }
-            yk_mol_tables(c, sh.m, false);
-            yd_block_sum<3>(v, sh.red, sh.out);
-            if (tid == 0) { mu[0] = sh.out[0]; mu[1] = sh.out[1]; mu[2] = sh.out[2]; }
+        if (tid < sh.m.na) {  // one lane per site loads the molecule
+            YkMol &m = sh.m;
+            int a = tid, g = m.first + a;
+            m.q[a] = c.q[g]; m.typ[a] = c.type[g]; m.mass[a] = c.mass[g]; m.dh[a] = c.dh[g];
+            m.xo[a] = c.x[g]; m.yo[a] = c.y[g]; m.zo[a] = c.z[g];
+            sh.uu_old[a] = c.uu[g];
+            sh.fr_old[a][0] = c.Fx[g]; sh.fr_old[a][1] = c.Fy[g]; sh.fr_old[a][2] = c.Fz[g];
         }
+        __syncthreads();
+        if (tid == 0) {  // LJ types present in the molecule
+            YkMol &m = sh.m;
+            m.np = 0;
+            for (int a = 0; a < m.na; ++a) {
+                m.lslot[a] = -1;
+                if (m.typ[a] > 0) {
+                    int sl = -1;
+                    for (int q = 0; q < m.np; ++q)
+                        if (m.ptype[q] == m.typ[a]) sl = q;
+                    if (sl < 0 && m.np < YK_MAXP) { sl = m.np; m.ptype[m.np++] = m.typ[a]; }
+                    m.lslot[a] = sl;
+                }
+            }
+        }
+        YK_TICK(0);
+        // ---- old axis tables --------------------------------------------
+        yk_mol_tables(c, sh.m, false);
+        __syncthreads();
+        YK_TICK(1);
         // ---- old k-space forces (MC.f90:946-982) ------------------------
         {
             double acc[YK_NACC];
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-            yk_sweep<F32, false>(c, sh.m, acc);
+            yk_sweep_n<F32, false>(c, sh.m, acc);
             yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
         }
+        YK_TICK(2);
         // ---- proposal on thread 0 (MC.f90:984-1067) ---------------------
         if (tid == 0) {
             YkMol &m = sh.m;
-            int f = m.first, na = m.na;
+            int na = m.na;
             double fx[YD_MAXA], fy[YD_MAXA], fz[YD_MAXA];
             U_old = 0.0;
-            for (int a = 0; a < na; ++a) U_old = U_old + c.uu[f + a];
+            for (int a = 0; a < na; ++a) U_old = U_old + sh.uu_old[a];
             for (int a = 0; a < na; ++a) {
-                fx[a] = c.Fx[f + a] + YD_FOUR_PI * sh.out[YK_FC(a, 0)] + YD_FOUR_PI * sh.out[YK_FL(a, 0)];
-                fy[a] = c.Fy[f + a] + YD_FOUR_PI * sh.out[YK_FC(a, 1)] + YD_FOUR_PI * sh.out[YK_FL(a, 1)];
-                fz[a] = c.Fz[f + a] + YD_FOUR_PI * sh.out[YK_FC(a, 2)] + YD_FOUR_PI * sh.out[YK_FL(a, 2)];
+                fx[a] = sh.fr_old[a][0] + YD_FOUR_PI * sh.out[YK_FK(a, 0)];
+                fy[a] = sh.fr_old[a][1] + YD_FOUR_PI * sh.out[YK_FK(a, 1)];
+                fz[a] = sh.fr_old[a][2] + YD_FOUR_PI * sh.out[YK_FK(a, 2)];
                 fx[a] = fx[a] + p.m2kext * m.q[a] * mu[0];
                 fy[a] = fy[a] + p.m2kext * m.q[a] * mu[1];
                 fz[a] = fz[a] + p.m2kext * m.q[a] * mu[2];
@@ -936,6 +1154,7 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
             }
         }
         __syncthreads();
+        YK_TICK(3);
         if (sh.stall) {  // proposal loop over its bound: rejected, no acceptance draw
             if (tid == 0) {
                 ++nmv;
@@ -955,7 +1174,7 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
             for (int i = 0; i < YK_NRS; ++i) acc[i] = 0.0;
             int ov = 0;
             yk_mol_tables(c, sh.m, true);
-            yk_rs_new(c, sh.m, sig6, feps, acc, ov);
+            yk_rs_move_n(c, sh.m, sig6, feps, acc, ov, sh.nblist, &sh.nbcount);
             if (ov) sh.ov = 1;
             yd_block_sum<YK_NRS>(acc, sh.red, sh.out);
             if (tid == 0) {
@@ -968,6 +1187,7 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
                 for (int i = 0; i < 3; ++i) cmp_new[i] = sh.out[4 * YD_MAXA + i];
             }
         }
+        YK_TICK(4);
         if (sh.ov) {  // overlap: rejected, no acceptance draw (MC.f90:1080-1084)
             if (tid == 0) {
                 ++nmv; ++nov;
@@ -987,9 +1207,10 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
             double acc[YK_NACC];
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-            yk_sweep<F32, true>(c, sh.m, acc);
+            yk_sweep_n<F32, true>(c, sh.m, acc);
             yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
         }
+        YK_TICK(5);
         // ---- acceptance on thread 0 (MC.f90:1176-1233) ------------------
         if (tid == 0) {
             YkMol &m = sh.m;
@@ -1001,9 +1222,9 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
             dulj = sh.out[1];
             double fx[YD_MAXA], fy[YD_MAXA], fz[YD_MAXA];
             for (int a = 0; a < na; ++a) {
-                fx[a] = frn[a][0] + YD_FOUR_PI * sh.out[YK_FC(a, 0)] + YD_FOUR_PI * sh.out[YK_FL(a, 0)];
-                fy[a] = frn[a][1] + YD_FOUR_PI * sh.out[YK_FC(a, 1)] + YD_FOUR_PI * sh.out[YK_FL(a, 1)];
-                fz[a] = frn[a][2] + YD_FOUR_PI * sh.out[YK_FC(a, 2)] + YD_FOUR_PI * sh.out[YK_FL(a, 2)];
+                fx[a] = frn[a][0] + YD_FOUR_PI * sh.out[YK_FK(a, 0)];
+                fy[a] = frn[a][1] + YD_FOUR_PI * sh.out[YK_FK(a, 1)];
+                fz[a] = frn[a][2] + YD_FOUR_PI * sh.out[YK_FK(a, 2)];
             }
             double mun[3] = {0, 0, 0};
             for (int a = 0; a < na; ++a) {
@@ -1053,14 +1274,16 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
             }
         }
         __syncthreads();
+        YK_TICK(6);
         if (!sh.accepted) continue;
         // ---- commit (MC.f90:1242-1318) ----------------------------------
         yk_commit<F32>(c, sh.m);
         {
             double acc[3] = {0, 0, 0};
-            yk_rs_update(c, sh.m, sig6, feps, acc);
+            yk_rs_commit(c, sh.m, sig6, feps, sh.nblist, sh.nbcount, acc);
             yd_block_sum<3>(acc, sh.red, sh.out);
         }
+        YK_TICK(7);
         if (tid == 0) {
             YkMol &m = sh.m;
             int f = m.first;
@@ -1079,9 +1302,13 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
             st->uu_ew = st->uu_ew + cmp_new[0] - sh.out[0];
             st->uu_lj6 = st->uu_lj6 + cmp_new[1] - sh.out[1];
             st->uu_lj12 = st->uu_lj12 + cmp_new[2] - sh.out[2];
-            for (int d = 0; d < 3; ++d) st->mu[d] = mu_new[d];
+            for (int d = 0; d < 3; ++d) {
+                st->mu[d] = mu_new[d];
+                mu[d] = mu_new[d];
+            }
         }
         __syncthreads();
+        YK_TICK(8);
     }
     if (tid == 0) {
         if (mode != 1) c.st->ctr = rng.ctr;
@@ -1089,6 +1316,7 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
         c.st->n_accepted += acc_count;
         c.st->n_overlap += nov;
         c.st->since_resync += nmv;
+        for (int i = 0; i < 9; ++i) c.st->phase[i] += ph[i];
     }
 }
 
@@ -1116,7 +1344,7 @@ __global__ void __launch_bounds__(YK_BS) yk_kspace_delta(YdChain c, int imol, co
     __syncthreads();
     double acc[YK_NACC];
     for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-    yk_sweep<F32, true>(c, sh.m, acc);
+    yk_sweep_n<F32, true>(c, sh.m, acc);
     yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
     if (threadIdx.x == 0) { out[0] = sh.out[0]; out[1] = sh.out[1]; }
 }
@@ -1146,11 +1374,11 @@ __global__ void __launch_bounds__(YK_BS) yk_kspace_forces(YdChain c, int first, 
     __syncthreads();
     double acc[YK_NACC];
     for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-    yk_sweep<F32, false>(c, sh.m, acc);
+    yk_sweep_n<F32, false>(c, sh.m, acc);
     yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
     if (threadIdx.x < 3 * n) {
         int a = threadIdx.x / 3, d = threadIdx.x % 3;
-        out[threadIdx.x] = YD_FOUR_PI * sh.out[YK_FC(a, d)] + YD_FOUR_PI * sh.out[YK_FL(a, d)];
+        out[threadIdx.x] = YD_FOUR_PI * sh.out[YK_FK(a, d)];
     }
 }
 template __global__ void yk_kspace_forces<false>(YdChain, int, int, const double *, const double *, const double *,
@@ -1174,7 +1402,7 @@ __global__ void __launch_bounds__(YK_BS) yk_rspace_partial(YdChain c, int imol, 
     double acc[YK_NRS];
     for (int i = 0; i < YK_NRS; ++i) acc[i] = 0.0;
     int ov = 0;
-    yk_rs_new(c, sh.m, sig6, feps, acc, ov);
+    yk_rs_new_n(c, sh.m, sig6, feps, acc, ov);
     if (ov) sh.ov = 1;
     yd_block_sum<YK_NRS>(acc, sh.red, sh.out);
     if (threadIdx.x < YK_NRS) out[threadIdx.x] = sh.out[threadIdx.x];

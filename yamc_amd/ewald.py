@@ -221,6 +221,15 @@ class EwaldMC:
         check(self.lib.yamc_forces(self.ctx, *(dptr(a) for a in g), *(dptr(a) for a in r)))
         return np.stack(g, 1), np.stack(r, 1)
 
+    PHASES = ("pick", "mu_tables", "old_sweep", "proposal", "rs_new", "new_sweep", "accept", "commit",
+              "finalize")
+
+    def phase_times(self):
+        """Cumulative device microseconds per chain-kernel phase."""
+        us = np.empty(9)
+        check(self.lib.yamc_phase_times(self.ctx, dptr(us)))
+        return dict(zip(self.PHASES, us))
+
     def last_kernel_ms(self) -> float:
         v = C.c_double()
         check(self.lib.yamc_last_kernel_ms(self.ctx, C.byref(v)))
