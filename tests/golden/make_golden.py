@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures tests/golden/<case>.npz from the compiled
+reference (oracle/_ref/ref_driver, built by oracle/build_ref.sh from the YAMC
+Fortran sources).  Run in the development container; the fixtures are data
+(inputs and reference outputs) and are committed.
+
+Each case stages a small system with yamc_amd.system, writes the .mol and
+composition files and a case file, runs the reference driver, and stores
+the inputs (molecule table, parameters, move list) with the reference's
+outputs: staged atoms, LJ tables, k grid, beta tables, S(k) sums, energy
+components, per-atom real-space uu/F, per-atom k-space forces, and the state
+after every move of a constant-uniform move trace (movemc with RND_CONSTANT).
+
+The constant uniform of each move is chosen so that no rejection-sampling
+loop of the proposal spins forever (checked with the oracle first).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import pyoracle  # noqa: E402
+from yamc_amd import system as ys  # noqa: E402
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def write_mol(path, mol):
+    with open(path, "w") as f:
+        for r in mol.atoms:
+            f.write("%s %.15f %.15f %.15f %.6f %.6f %.6f %.4f %.4f\n" % r)
+
+
+def parse(text):
+    lines = text.split("\n")
+    i = 0
+    out = {"BETALJ": [], "SUMS": [], "MOVES": []}
+    while i < len(lines):
+        ln = lines[i].strip()
+        if not ln:
+            i += 1
+            continue
+        tok = ln.split()
+        if tok[0] == "SIZES":
+            out["SIZES"] = [int(v) for v in tok[1:]]
+        elif tok[0] == "KMAX":
+            out["KMAX"] = int(tok[1])
+        elif tok[0] in ("PARAMS", "PARAMS2"):
+            out[tok[0]] = [float(v) for v in tok[1:]]
+        elif tok[0] in ("R", "I"):
+            tag, n = tok[1], int(tok[2])
+            vals = [float(v) if tok[0] == "R" else int(v) for v in lines[i + 1:i + 1 + n]]
+            i += n
+            if tag in ("BETALJ", "SUMS"):
+                out[tag].append(vals)
+            else:
+                out[tag] = vals
+        elif tok[0] == "LJ12UPPER":
+            n = int(tok[1])
+            out["LJ12UPPER"] = [float(v) for v in lines[i + 1:i + 1 + n]]
+            i += n
+        elif tok[0] == "E":
+            vals = " ".join(lines[i + 1:i + 4]).split()
+            e = [float(v) for v in vals]
+            if tok[1] == "INIT":
+                out["E_INIT"] = e
+            else:
+                out["MOVES"][-1]["E"] = e
+            i += 3
+        elif tok[0] == "FK":
+            n = int(tok[1])
+            out["FK"] = [[float(v) for v in lines[i + 1 + k].split()] for k in range(n)]
+            i += n
+        elif tok[0] == "MOVE":
+            out["MOVES"].append({"imol": int(tok[2]), "accepted": tok[3] == "T"})
+        elif tok[0] == "MOVES":
+            pass
+        elif tok[0] == "END":
+            break
+        elif len(tok) == 1 and out["MOVES"] and "E" in out["MOVES"][-1] and "pos" not in out["MOVES"][-1]:
+            na = int(tok[0])
+            out["MOVES"][-1]["pos"] = [[float(v) for v in lines[i + 1 + k].split()] for k in range(na)]
+            i += na
+        i += 1
+    return out
+
+
+def pick_moves(sysm, nmoves, seed):
+    """Molecules and constant uniforms such that the oracle finishes every
+    proposal (a stalled proposal would make the reference loop forever)."""
+    rng = np.random.default_rng(seed)
+    imol = rng.integers(0, sysm.nmol, nmoves).astype(np.int32)
+    u = rng.uniform(0.25, 0.75, nmoves)
+    for _ in range(200):
+        o = pyoracle.Oracle(sysm)
+        o.mc_init()
+        _, recs = o.run_const(imol, u)
+        bad = np.nonzero(recs["overlap"] == 2)[0]
+        if not len(bad):
+            return imol, u
+        u[bad[0]] = rng.uniform(0.25, 0.75)
+    raise RuntimeError("could not find non-stalling uniforms")
+
+
+def make_case(name, sysm, nmoves, seed):
+    imol, u = pick_moves(sysm, nmoves, seed)
+    p = sysm.params
+    with tempfile.TemporaryDirectory() as td:
+        names = []
+        for k, mol in enumerate(sysm.species):
+            fn = f"species{k}.mol"
+            write_mol(os.path.join(td, fn), mol)
+            names.append(fn)
+        with open(os.path.join(td, "case.composition"), "w") as f:
+            f.write(f"{len(sysm.species)}\n")
+            for fn, cnt in zip(names, sysm.counts):
+                f.write(f"{fn} {cnt}\n")
+        with open(os.path.join(td, "case.in"), "w") as f:
+            f.write("case.composition\n")
+            f.write(" ".join("%.17g" % v for v in (p.box_length, p.temp, p.sr, p.sk, p.alpha, p.external_permutivity,
+                                                  p.dr_a, p.d_angle_degree, p.lambda_f, p.lambda_c)) + "\n")
+            f.write(f"{sysm.nmol}\n")
+            for i in range(sysm.nmol):
+                f.write("%d %.17g %.17g %.17g %.17g %.17g %.17g\n" % (
+                    sysm.mol_type[i] + 1, *sysm.mol_pos[i], *sysm.mol_angles[i]))
+            f.write(f"{nmoves}\n")
+            for a, b in zip(imol, u):
+                f.write("%d %.17g\n" % (a + 1, b))
+        r = subprocess.run([DRIVER, "case.in"], cwd=td, capture_output=True, text=True, timeout=120)
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr)
+    g = parse(r.stdout)
+    natom, nk, nt = g["SIZES"]
+    moves = g["MOVES"]
+    assert len(moves) == nmoves
+    arrays = dict(
+        # inputs
+        params=np.array([p.box_length, p.temp, p.sr, p.sk, p.alpha, p.external_permutivity, p.dr_a,
+                         p.d_angle_degree, p.lambda_f, p.lambda_c]),
+        species=np.array([m.name for m in sysm.species]), counts=np.array(sysm.counts, np.int32),
+        mol_type=sysm.mol_type, mol_pos=sysm.mol_pos, mol_angles=sysm.mol_angles,
+        move_imol=imol, move_u=u,
+        # reference outputs
+        kmax=np.int32(g["KMAX"]), derived=np.array(g["PARAMS"] + g["PARAMS2"]),
+        xx=np.array(g["XX"]), yy=np.array(g["YY"]), zz=np.array(g["ZZ"]), charge=np.array(g["CHARGE"]),
+        sigma=np.array(g["SIGMA"]), epsilon=np.array(g["EPSILON"]), hard_core=np.array(g["HARDCORE"]),
+        molnum=np.array(g["MOLNUM"], np.int32), lj_type=np.array(g["TYPE"], np.int32),
+        first=np.array(g["FIRST"], np.int32) - 1, last=np.array(g["LAST"], np.int32) - 1,
+        lj6=np.array(g["LJ6"]), lj12_upper=np.array(g["LJ12UPPER"]), four_eps=np.array(g["FOUREPS"]),
+        sigma6=np.array(g["SIGMA6"]),
+        kx=np.array(g["KX"], np.int32), ky=np.array(g["KY"], np.int32), kz=np.array(g["KZ"], np.int32),
+        ip=np.array(g["IP"], np.int32), iq=np.array(g["IQ"], np.int32),
+        beta=np.array(g["BETA"]), beta6=np.array(g["BETA6"]), beta12=np.array(g["BETA12"]),
+        beta_lj=np.array(g["BETALJ"]).reshape(nt, nt, nk),
+        sums=np.array(g["SUMS"][:(1 + nt) * 8]).reshape(1 + nt, 8, nk),
+        e_init=np.array(g["E_INIT"]),
+        uu=np.array(g["UU"]), fx=np.array(g["FX"]), fy=np.array(g["FY"]), fz=np.array(g["FZ"]),
+        fk=np.array(g["FK"]),
+        move_accepted=np.array([m["accepted"] for m in moves], np.int32),
+        move_e=np.array([m["E"] for m in moves]),
+        move_pos=np.array([np.pad(np.array(m["pos"]), ((0, 4 - len(m["pos"])), (0, 0))) for m in moves]),
+        uu_end=np.array(g["UUEND"]), fx_end=np.array(g["FXEND"]), fy_end=np.array(g["FYEND"]),
+        fz_end=np.array(g["FZEND"]),
+        sums_coulomb_end=np.array(g["SUMS"][(1 + nt) * 8:]).reshape(8, nk),
+    )
+    path = os.path.join(OUT, f"{name}.npz")
+    np.savez_compressed(path, **arrays)
+    print(f"{path}: natom={natom} nk={nk} ntype={nt} moves={nmoves} accepted={arrays['move_accepted'].sum()}")
+
+
+def main():
+    if not os.path.exists(DRIVER):
+        raise SystemExit("oracle/_ref/ref_driver is not built (oracle/build_ref.sh)")
+    make_case("ref_small_bias", ys.config("small", seed=31), 40, 1)
+    make_case("ref_small_nobias", ys.config("small_nobias", seed=32), 40, 2)
+    make_case("ref_water64", ys.lattice_system(
+        ys.Params(box_length=12.42, temp=298.0, sr=3.2, sk=3.0, alpha=6.4, external_permutivity=70.0, dr_a=0.3,
+                  d_angle_degree=15.0, lambda_f=0.0, lambda_c=0.0), [ys.SPCE], [64], 33), 30, 3)
+
+
+if __name__ == "__main__":
+    main()

@@ -6,8 +6,13 @@ accept/reject decisions on the same RNG stream.  Relative errors of dE use
 the scale sum |terms| (dE_real, dE_kC, dE_kLJ, dE_ext) because dE itself can
 vanish.
 """
+import os
+import sys
+
 import numpy as np
 import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 pytestmark = pytest.mark.gpu
 
@@ -241,3 +246,20 @@ def test_fp32c_error_is_bounded(lib):
     assert n > 10
     d = np.abs(ra["du_total"][:n] - rb["du_total"][:n])
     assert np.max(d) < 1e-3, np.max(d)
+
+
+@pytest.mark.parametrize("case", __import__("golden_cases").cases(), ids=lambda p: p.split("/")[-1])
+def test_gpu_matches_reference_golden(lib, case):
+    """The HIP path against the compiled reference's own outputs."""
+    import golden_cases as gc
+    from yamc_amd.ewald import EwaldMC
+    sysm, g = gc.load(case)
+    gpu = EwaldMC(sysm)
+    gpu.mc_init()
+    gc.check_init(gpu, sysm, g)
+
+    def step(imol, u):
+        n, recs = gpu.run_const([imol], [u])
+        return n
+
+    gc.check_trace(gpu, sysm, g, step)
