@@ -26,7 +26,7 @@ __global__ void yk_rspace_full(YdChain c, const double *lj_sig6, const double *l
 __global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, double *comps_out);
 template <bool F32>
 __global__ void yk_chain(const YdChain *chains, const double *const *sig6s, const double *const *feps_s, int nsteps,
-                         int mode, int max_moves);
+                         int mode, int max_moves, int lds_atoms);
 template <bool F32>
 __global__ void yk_kspace_delta(YdChain c, int imol, const double *x, const double *y, const double *z,
                                 double *out);
@@ -588,11 +588,28 @@ extern "C" int yamc_resync(yamc_ctx *c) {
 }
 
 static int launch_chain(yamc_ctx *c, const YdChain *d_chain, const double *const *sig6s,
-                        const double *const *feps, int nchains, int nsteps, int mode) {
-    if (c->precision == YAMC_FP32C)
-        yk_chain<true><<<nchains, YK_BS, 0, c->stream>>>(d_chain, sig6s, feps, nsteps, mode, 0);
-    else
-        yk_chain<false><<<nchains, YK_BS, 0, c->stream>>>(d_chain, sig6s, feps, nsteps, mode, 0);
+                        const double *const *feps, int nchains, int nsteps, int mode, int max_natom) {
+    // The FP32 position copy of the real-space scan goes into dynamic LDS when
+    // it fits next to the kernel's static LDS.
+    int lds_max = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    hipFuncAttributes fa;
+    if (c->precision == YAMC_FP32C) HIPCHK(hipFuncGetAttributes(&fa, (const void *)yk_chain<true>));
+    else HIPCHK(hipFuncGetAttributes(&fa, (const void *)yk_chain<false>));
+    size_t want = (size_t)max_natom * 16;
+    int lds_atoms = 0;
+    size_t dyn = 0;
+    if (want + fa.sharedSizeBytes + 1024 <= (size_t)lds_max) {
+        lds_atoms = max_natom;
+        dyn = want;
+    }
+    if (c->precision == YAMC_FP32C) {
+        if (dyn > 0) HIPCHK(hipFuncSetAttribute((const void *)yk_chain<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+        yk_chain<true><<<nchains, YK_BS, dyn, c->stream>>>(d_chain, sig6s, feps, nsteps, mode, 0, lds_atoms);
+    } else {
+        if (dyn > 0) HIPCHK(hipFuncSetAttribute((const void *)yk_chain<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+        yk_chain<false><<<nchains, YK_BS, dyn, c->stream>>>(d_chain, sig6s, feps, nsteps, mode, 0, lds_atoms);
+    }
     HIPCHK(hipGetLastError());
     return YAMC_OK;
 }
@@ -620,7 +637,7 @@ extern "C" int yamc_mc_run(yamc_ctx *c, int32_t ncycles, int32_t resync_every, y
         YdChain ch = c->ch;
         ch.rec = d_rec ? d_rec + (size_t)done * c->nmol : nullptr;
         HIPCHK(hipMemcpyAsync(c->d_chain, &ch, sizeof ch, hipMemcpyHostToDevice, c->stream));
-        TRY(launch_chain(c, c->d_chain, c->d_sig6s, c->d_feps, 1, n, 0));
+        TRY(launch_chain(c, c->d_chain, c->d_sig6s, c->d_feps, 1, n, 0, c->natom));
         if (resync_every > 0 && done + n < ncycles) TRY(total_energy(c));
     }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -657,7 +674,7 @@ extern "C" int yamc_mc_run_const(yamc_ctx *c, int32_t nmoves, const int32_t *imo
     ch.rec = d_rec; ch.cimol = d_imol; ch.cu = d_u;
     HIPCHK(hipMemcpyAsync(c->d_chain, &ch, sizeof ch, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    TRY(launch_chain(c, c->d_chain, c->d_sig6s, c->d_feps, 1, nmoves, 1));
+    TRY(launch_chain(c, c->d_chain, c->d_sig6s, c->d_feps, 1, nmoves, 1, c->natom));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     if (recs) HIPCHK(hipMemcpyAsync(recs, d_rec, sizeof(yamc_move_rec) * nmoves, hipMemcpyDeviceToHost, c->stream));
     YdState st1;
@@ -701,8 +718,10 @@ extern "C" int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles,
     HIPCHK(hipMemcpyAsync(d_ch, chains.data(), sizeof(YdChain) * n, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d_s6, s6.data(), sizeof(void *) * n, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d_fe, fe.data(), sizeof(void *) * n, hipMemcpyHostToDevice, s));
+    int max_natom = 0;
+    for (int i = 0; i < n; ++i) max_natom = ctxs[i]->natom > max_natom ? ctxs[i]->natom : max_natom;
     HIPCHK(hipEventRecord(c0->ev0, s));
-    TRY(launch_chain(c0, d_ch, d_s6, d_fe, n, ncycles, 0));
+    TRY(launch_chain(c0, d_ch, d_s6, d_fe, n, ncycles, 0, max_natom));
     HIPCHK(hipEventRecord(c0->ev1, s));
     HIPCHK(hipFreeAsync(d_ch, s));
     HIPCHK(hipFreeAsync(d_s6, s));

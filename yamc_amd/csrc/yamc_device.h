@@ -288,23 +288,26 @@ struct YdPair {
 
 __device__ __forceinline__ void yd_pair(const YdParams &p, double r2, bool same_mol, double qi, double qj, bool lj,
                                         double sigma6, double four_eps, YdPair &o) {
-    double r = sqrt(r2), a = p.alpha * r, a2 = a * a, e = exp(-a2);
+    // 1/r and 1/r^2 once; the divisions of the reference expressions become
+    // multiplications (each within an ulp of the reference's quotient).
+    double r = sqrt(r2), ir = 1.0 / r, ir2 = ir * ir, a = p.alpha * r, a2 = a * a, e = exp(-a2);
     double erfc_a = same_mol ? -yd_erf_luc(a, a2, e) : yd_erfc_luc(a, a2, e);
     double qq = p.xclb * qi * qj;
-    o.potew = qq * erfc_a / r;
+    o.potew = qq * erfc_a * ir;
     o.pot = o.potew;
-    o.FR = qq * (erfc_a / r + p.two_alpha_over_sqrt_pi * e) / r2;
+    o.FR = qq * (erfc_a * ir + p.two_alpha_over_sqrt_pi * e) * ir2;
     o.potlj6 = 0.0;
     o.potlj12 = 0.0;
     if (lj) {
-        double r6 = r2 * r2 * r2, sor6 = sigma6 / r6;
-        double C6 = 1.0 + a2 * (1.0 + a2 / 2.0);
+        double sor6 = sigma6 * (ir2 * ir2 * ir2);
+        double C6 = 1.0 + a2 * (1.0 + a2 * 0.5);
         double a6 = a2 * a2 * a2;
-        double C12 = C6 + a6 * (1.0 + a2 * (1.0 + a2 / 5.0) / 4.0) / 6.0;
+        double C12 = C6 + a6 * (1.0 + a2 * (1.0 + a2 * 0.2) * 0.25) * (1.0 / 6.0);
         o.potlj6 = four_eps * sor6 * e * C6;
         o.potlj12 = four_eps * (sor6 * sor6) * e * C12;
         o.pot = o.pot + o.potlj12 - o.potlj6;
-        o.FR = o.FR + four_eps * 6.0 * sor6 * e * (2.0 * sor6 * (C12 + (a6 * a6) / 720.0) - (C6 + a6 / 6.0)) / r2;
+        o.FR = o.FR + four_eps * 6.0 * sor6 * e *
+                          (2.0 * sor6 * (C12 + (a6 * a6) * (1.0 / 720.0)) - (C6 + a6 * (1.0 / 6.0))) * ir2;
     }
 }
 

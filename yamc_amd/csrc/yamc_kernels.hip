@@ -26,6 +26,9 @@
 // Candidate neighbours of one move kept in LDS: atoms within the cut-off of
 // a new or an old site of the moved molecule.
 #define YK_NBCAP 2048
+// Candidate bitmap over atom indices (order-independent marking, then a
+// deterministic ascending compaction into the list).
+#define YK_BMWORDS 4096  // 131072 atoms
 
 // ------------------------------------------------------------------------
 // Functions.f90:26-111 erfc_Luc_bet6_bet12
@@ -341,6 +344,8 @@ struct YkShared {
     double fr_old[YD_MAXA][3], uu_old[YD_MAXA];
     int ov, accepted, stall, nbcount;
     int nblist[YK_NBCAP];
+    unsigned int bm[YK_BMWORDS];
+    int wsum[YK_NW];
 };
 
 // Loads molecule imol (old positions from the state) into LDS; thread 0.
@@ -736,6 +741,148 @@ __device__ void yk_rs_commit(const YdChain &c, const YkMol &m, const double *sig
     }
 }
 
+// Real-space pass of a trial move in two stages.
+//  1. Scan: every other atom j is tested against the molecule's old and new
+//     sites; survivors are appended to the LDS candidate list.  With an
+//     LDS-resident FP32 copy of the positions (x, y, z, molecule number as
+//     bits in w) the test is in FP32 with a margin of 1e-5 box units^2 on
+//     r^2, far above the FP32 rounding of r^2 (about 1e-6 for |d| <= 1.5), so
+//     no atom the FP64 test accepts is lost; without it the test is FP64.
+//  2. Exact pass over the compacted list: the FP64 pair terms of the new
+//     sites (EwaldSumRealSpace.f90:338-498, partial mode) and the overlap
+//     test, with every lane busy.  The same list then drives the commit of
+//     the per-atom arrays (only atoms within the cut-off of an old or new
+//     site change, MC.f90:1251-1275).
+// Returns false when the list overflowed (the caller then uses the full
+// FP64 pass yk_rs_move_n and the full commit).
+template <int NA>
+__device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const float4 *pos, unsigned int *bm) {
+    const int N = c.p.natom, me = m.imol + 1;
+    if (pos) {
+        const float r2f = (float)c.p.rmax2 + 1e-5f;
+        float xof[NA], yof[NA], zof[NA], xnf[NA], ynf[NA], znf[NA];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            xof[a] = (float)m.xo[a]; yof[a] = (float)m.yo[a]; zof[a] = (float)m.zo[a];
+            xnf[a] = (float)m.xn[a]; ynf[a] = (float)m.yn[a]; znf[a] = (float)m.zn[a];
+        }
+        for (int j = threadIdx.x; j < N; j += YK_BS) {
+            float4 pj = pos[j];
+            bool near = false;
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                float dx = xnf[a] - pj.x, dy = ynf[a] - pj.y, dz = znf[a] - pj.z;
+                dx = dx > 0.5f ? dx - 1.0f : dx; dx = dx < -0.5f ? dx + 1.0f : dx;
+                dy = dy > 0.5f ? dy - 1.0f : dy; dy = dy < -0.5f ? dy + 1.0f : dy;
+                dz = dz > 0.5f ? dz - 1.0f : dz; dz = dz < -0.5f ? dz + 1.0f : dz;
+                near |= dx * dx + dy * dy + dz * dz < r2f;
+                float ex = xof[a] - pj.x, ey = yof[a] - pj.y, ez = zof[a] - pj.z;
+                ex = ex > 0.5f ? ex - 1.0f : ex; ex = ex < -0.5f ? ex + 1.0f : ex;
+                ey = ey > 0.5f ? ey - 1.0f : ey; ey = ey < -0.5f ? ey + 1.0f : ey;
+                ez = ez > 0.5f ? ez - 1.0f : ez; ez = ez < -0.5f ? ez + 1.0f : ez;
+                near |= ex * ex + ey * ey + ez * ez < r2f;
+            }
+            if (near && __float_as_int(pj.w) != me) atomicOr(&bm[j >> 5], 1u << (j & 31));
+        }
+    } else {
+        for (int j = threadIdx.x; j < N; j += YK_BS) {
+            double xj = c.x[j], yj = c.y[j], zj = c.z[j];
+            int jm = c.molnum[j];
+            bool near = false;
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                double xo = yd_wrap(m.xo[a] - xj), yo = yd_wrap(m.yo[a] - yj), zo = yd_wrap(m.zo[a] - zj);
+                near |= xo * xo + yo * yo + zo * zo < c.p.rmax2;
+                double xn = yd_wrap(m.xn[a] - xj), yn = yd_wrap(m.yn[a] - yj), zn = yd_wrap(m.zn[a] - zj);
+                near |= xn * xn + yn * yn + zn * zn < c.p.rmax2;
+            }
+            if (near && jm != me) atomicOr(&bm[j >> 5], 1u << (j & 31));
+        }
+    }
+}
+
+template <int NA>
+__device__ void yk_rs_exact(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
+                            const int *list, int count, double (&acc)[YK_NRS], int &ov) {
+    const int T = c.p.ntype;
+    for (int i = threadIdx.x; i < count; i += YK_BS) {
+        int j = list[i];
+        int jt = c.type[j];
+        double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            int it = m.typ[a];
+            double xij = yd_wrap(m.xn[a] - xj), yij = yd_wrap(m.yn[a] - yj), zij = yd_wrap(m.zn[a] - zj);
+            double r2 = xij * xij + yij * yij + zij * zij;
+            double d12 = (m.dh[a] + dhj) / 2.0;
+            if (it > 0 && jt > 0 && r2 < d12 * d12) ov = 1;
+            if (r2 >= c.p.rmax2) continue;
+            bool lj = it != 0 && jt != 0;
+            int off = lj ? (it - 1) * T + (jt - 1) : 0;
+            YdPair pr;
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            yk_rs_acc(acc, a, pr, xij, yij, zij);
+        }
+    }
+}
+
+__device__ __forceinline__ void yk_rs_scan_n(const YdChain &c, const YkMol &m, const float4 *pos,
+                                             unsigned int *bm) {
+    switch (m.na) {
+        case 1: yk_rs_scan<1>(c, m, pos, bm); break;
+        case 2: yk_rs_scan<2>(c, m, pos, bm); break;
+        case 3: yk_rs_scan<3>(c, m, pos, bm); break;
+        default: yk_rs_scan<4>(c, m, pos, bm); break;
+    }
+}
+
+// Ascending compaction of the candidate bitmap into list[] (deterministic:
+// thread t owns a contiguous run of words, offsets from a block prefix sum).
+// *count gets the number of candidates (may exceed YK_NBCAP: list partial).
+__device__ void yk_compact(const unsigned int *bm, int nwords, int *list, int *count, int *wsum) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int per = (nwords + YK_BS - 1) / YK_BS, w0 = tid * per;
+    int cnt = 0;
+    for (int k = 0; k < per; ++k)
+        if (w0 + k < nwords) cnt += __popc(bm[w0 + k]);
+    int incl = cnt;  // inclusive scan inside the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int i = 0; i < YK_NW; ++i) {
+        if (i < w) base += wsum[i];
+        total += wsum[i];
+    }
+    int pos = base + incl - cnt;
+    for (int k = 0; k < per; ++k) {
+        if (w0 + k >= nwords) break;
+        unsigned int b = bm[w0 + k];
+        while (b) {
+            int bit = __ffs(b) - 1;
+            b &= b - 1;
+            if (pos < YK_NBCAP) list[pos] = ((w0 + k) << 5) + bit;
+            ++pos;
+        }
+    }
+    if (tid == 0) *count = total;
+}
+
+__device__ __forceinline__ void yk_rs_exact_n(const YdChain &c, const YkMol &m, const double *sig6,
+                                              const double *feps, const int *list, int count,
+                                              double (&acc)[YK_NRS], int &ov) {
+    switch (m.na) {
+        case 1: yk_rs_exact<1>(c, m, sig6, feps, list, count, acc, ov); break;
+        case 2: yk_rs_exact<2>(c, m, sig6, feps, list, count, acc, ov); break;
+        case 3: yk_rs_exact<3>(c, m, sig6, feps, list, count, acc, ov); break;
+        default: yk_rs_exact<4>(c, m, sig6, feps, list, count, acc, ov); break;
+    }
+}
+
 // Dispatch on the molecule size so every per-site loop has a compile-time
 // trip count (keeps the per-site accumulators in registers).
 template <bool F32, bool DELTA>
@@ -995,12 +1142,17 @@ __device__ int yk_choose(const YdChain &c, double rnd) {
 template <bool F32>
 __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const double *const *sig6s,
                                                   const double *const *feps_s, int nsteps, int mode,
-                                                  int max_moves) {
+                                                  int max_moves, int lds_atoms) {
     __shared__ YkShared sh;
+    extern __shared__ float4 lds_pos[];  // FP32 positions when the launch provides LDS for them
     const YdChain &c = chains[blockIdx.x];
     const double *sig6 = sig6s[blockIdx.x], *feps = feps_s[blockIdx.x];
     const int tid = threadIdx.x;
     const YdParams &p = c.p;
+    const bool use_lds = lds_atoms >= p.natom;
+    if (use_lds)
+        for (int i = tid; i < p.natom; i += YK_BS)
+            lds_pos[i] = make_float4((float)c.x[i], (float)c.y[i], (float)c.z[i], __int_as_float(c.molnum[i]));
     YkScalar &t0 = sh.t0;
     YkMcm &mcm = t0.mcm;
     YdRng &rng = t0.rng;
@@ -1174,7 +1326,23 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
             for (int i = 0; i < YK_NRS; ++i) acc[i] = 0.0;
             int ov = 0;
             yk_mol_tables(c, sh.m, true);
-            yk_rs_move_n(c, sh.m, sig6, feps, acc, ov, sh.nblist, &sh.nbcount);
+            const int nwords = (p.natom + 31) >> 5;
+            if (nwords <= YK_BMWORDS) {
+                for (int i = tid; i < nwords; i += YK_BS) sh.bm[i] = 0u;
+                __syncthreads();
+                yk_rs_scan_n(c, sh.m, use_lds ? lds_pos : nullptr, sh.bm);
+                __syncthreads();
+                yk_compact(sh.bm, nwords, sh.nblist, &sh.nbcount, sh.wsum);
+            } else if (tid == 0) {
+                sh.nbcount = YK_NBCAP + 1;  // too many atoms for the bitmap: full passes
+            }
+            __syncthreads();
+            if (sh.nbcount <= YK_NBCAP) {
+                yk_rs_exact_n(c, sh.m, sig6, feps, sh.nblist, sh.nbcount, acc, ov);
+            } else {  // list overflow: full FP64 pass (the commit then visits every atom)
+                int junk = 0;
+                yk_rs_move_n(c, sh.m, sig6, feps, acc, ov, sh.nblist, &junk);
+            }
             if (ov) sh.ov = 1;
             yd_block_sum<YK_NRS>(acc, sh.red, sh.out);
             if (tid == 0) {
@@ -1295,6 +1463,9 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
                 c.x[f + a] = m.xn[a];
                 c.y[f + a] = m.yn[a];
                 c.z[f + a] = m.zn[a];
+                if (use_lds)
+                    lds_pos[f + a] = make_float4((float)m.xn[a], (float)m.yn[a], (float)m.zn[a],
+                                                 __int_as_float(m.imol + 1));
             }
             YdState *st = c.st;
             st->e_kc = st->e_kc + duc;
@@ -1321,9 +1492,9 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
 }
 
 template __global__ void yk_chain<false>(const YdChain *, const double *const *, const double *const *, int, int,
-                                         int);
+                                         int, int);
 template __global__ void yk_chain<true>(const YdChain *, const double *const *, const double *const *, int, int,
-                                        int);
+                                        int, int);
 
 // ========================================================================
 // Single-call entry points (one workgroup): the per-function parity surface.
