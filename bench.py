@@ -57,6 +57,9 @@ def main():
     ap.add_argument("--cpu-moves", type=int, default=0, help="moves for the CPU baseline (0 = auto ~15 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32c,batch")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain fp32 streaming copy of MIB MiB (3 launches) and print its byte count; "
+                         "used to scale the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -65,6 +68,17 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.rand(n, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)  # one elementwise kernel: reads 4n bytes, writes 4n bytes
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"kernel": "elementwise mul by 1", "launches": 3,
+                                               "read_bytes": 4 * n, "write_bytes": 4 * n}}))
+        return
 
     from yamc_amd import YAMC_FP32C, YAMC_FP64, config, system
     from yamc_amd.ewald import EwaldMC, batch_run

@@ -195,6 +195,14 @@ int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int64_t *n
  * Either pointer triple may be NULL. */
 int yamc_forces(yamc_ctx *ctx, double *gx, double *gy, double *gz, double *rx, double *ry, double *rz);
 
+/* ---- launch shape ------------------------------------------------------ */
+/* Launch shape of the chain kernel for this context (and for a batch whose
+ * first context it is): 256 or 512 = one workgroup of that size per chain;
+ * 64 = one wavefront per chain, four chains per workgroup (run_mc cycles
+ * only; constant-uniform traces use 256).  No reference counterpart (the
+ * reference is serial); the autotuner (yamc_amd.tune) measures them. */
+int yamc_set_block_size(yamc_ctx *ctx, int32_t block);
+
 /* ---- diagnostics ------------------------------------------------------- */
 /* Time of the last yamc_mc_run / yamc_batch_run kernel launch in ms (HIP
  * events on the context stream). */

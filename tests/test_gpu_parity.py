@@ -111,9 +111,13 @@ def test_kspace_delta_and_forces(lib, name):
         assert rel(fg, fo, np.abs(fo).max()) < RTOL
 
 
+@pytest.mark.parametrize("block", [256, 512, 64])
 @pytest.mark.parametrize("name,ncyc", [("small", 20), ("small_nobias", 20), ("c1", 1)])
-def test_mc_trace_matches_oracle(lib, name, ncyc):
+def test_mc_trace_matches_oracle(lib, name, ncyc, block):
+    """Every launch shape of the chain kernel (workgroup of 256 or 512 per
+    chain, or one wave per chain) against the oracle on one RNG stream."""
     s, g, o = make(name, seed=21)
+    g.set_block_size(block)
     g.mc_init()
     o.mc_init()
     g.set_rng(99, 1)
@@ -165,13 +169,19 @@ def test_mc_incremental_state_matches_fresh_sums(lib):
         assert abs(e_inc[k] - e_new[k]) <= 1e-9 * max(1.0, abs(e_new[k])), (k, e_inc[k], e_new[k])
 
 
-def test_batch_equals_single(lib):
+@pytest.mark.parametrize("block", [256, 64])
+def test_batch_equals_single(lib, block):
+    """Uneven replicas (sizes, compositions, temperatures) advanced in one
+    launch equal the same replicas run one by one, bit for bit."""
     from yamc_amd import replica_system
     from yamc_amd.ewald import EwaldMC, batch_run
     systems = [replica_system(n, c, t, seed=i) for i, (n, c, t) in enumerate([(40, 0.5, 300.0), (64, 0.0, 290.0),
-                                                                             (27, 1.0, 340.0)])]
+                                                                             (27, 1.0, 340.0), (50, 0.3, 320.0),
+                                                                             (33, 0.8, 285.0)])]
     a = [EwaldMC(s) for s in systems]
     b = [EwaldMC(s) for s in systems]
+    for x in a + b:
+        x.set_block_size(block)
     for i, (x, y) in enumerate(zip(a, b)):
         x.mc_init(); y.mc_init()
         x.set_rng(17, i); y.set_rng(17, i)

@@ -24,9 +24,13 @@ __global__ void yk_mu(YdChain c, double *mu_out);
 __global__ void yk_tails(YdChain c, const double *lj6, const double *lj12, const int *counts);
 __global__ void yk_rspace_full(YdChain c, const double *lj_sig6, const double *lj_4eps, double *part, int *ov);
 __global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, double *comps_out);
-template <bool F32>
+template <bool F32, int BS>
 __global__ void yk_chain(const YdChain *chains, const double *const *sig6s, const double *const *feps_s, int nsteps,
                          int mode, int max_moves, int lds_atoms);
+template <bool F32>
+__global__ void yk_wchain(const YdChain *chains, int nchains, const double *const *sig6s, const double *const *feps_s,
+                          int ncycles);
+#define YW_WPB 4  // replicas (waves) per workgroup of yk_wchain
 template <bool F32>
 __global__ void yk_kspace_delta(YdChain c, int imol, const double *x, const double *y, const double *z,
                                 double *out);
@@ -79,6 +83,7 @@ struct yamc_ctx {
     YdChain *d_chain = nullptr;
     const double **d_sig6s = nullptr, **d_feps = nullptr;
     bool beta_ready = false, state_ready = false;
+    int chain_bs = 256;  // workgroup size of the chain kernel (256 or 512)
 };
 
 template <typename T>
@@ -191,6 +196,8 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
             TRY(dalloc(c, &h.S, nS));
         }
         TRY(dalloc(c, &h.uu, N)); TRY(dalloc(c, &h.Fx, N)); TRY(dalloc(c, &h.Fy, N)); TRY(dalloc(c, &h.Fz, N));
+        TRY(dalloc(c, &h.rsb, (size_t)YD_NBCAP * 8));
+        TRY(dalloc(c, &h.nbl, (size_t)YD_NBCAP)); TRY(dalloc(c, &h.pos4, (size_t)N));
         TRY(dalloc(c, &h.freq_end, M)); TRY(dalloc(c, &h.st, 1));
         TRY(dalloc(c, &c->d_lj6, TT)); TRY(dalloc(c, &c->d_lj12, TT));
         TRY(dalloc(c, &c->d_sig6, TT)); TRY(dalloc(c, &c->d_4eps, TT));
@@ -587,30 +594,51 @@ extern "C" int yamc_resync(yamc_ctx *c) {
     return sync(c);
 }
 
-static int launch_chain(yamc_ctx *c, const YdChain *d_chain, const double *const *sig6s,
-                        const double *const *feps, int nchains, int nsteps, int mode, int max_natom) {
+template <bool F32, int BS>
+static int launch_chain_t(yamc_ctx *c, const YdChain *d_chain, const double *const *sig6s,
+                          const double *const *feps, int nchains, int nsteps, int mode, int max_natom) {
     // The FP32 position copy of the real-space scan goes into dynamic LDS when
     // it fits next to the kernel's static LDS.
     int lds_max = 0;
     HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    const void *fn = (const void *)yk_chain<F32, BS>;
     hipFuncAttributes fa;
-    if (c->precision == YAMC_FP32C) HIPCHK(hipFuncGetAttributes(&fa, (const void *)yk_chain<true>));
-    else HIPCHK(hipFuncGetAttributes(&fa, (const void *)yk_chain<false>));
+    HIPCHK(hipFuncGetAttributes(&fa, fn));
     size_t want = (size_t)max_natom * 16;
     int lds_atoms = 0;
     size_t dyn = 0;
     if (want + fa.sharedSizeBytes + 1024 <= (size_t)lds_max) {
         lds_atoms = max_natom;
         dyn = want;
+        HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
     }
-    if (c->precision == YAMC_FP32C) {
-        if (dyn > 0) HIPCHK(hipFuncSetAttribute((const void *)yk_chain<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
-        yk_chain<true><<<nchains, YK_BS, dyn, c->stream>>>(d_chain, sig6s, feps, nsteps, mode, 0, lds_atoms);
-    } else {
-        if (dyn > 0) HIPCHK(hipFuncSetAttribute((const void *)yk_chain<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
-        yk_chain<false><<<nchains, YK_BS, dyn, c->stream>>>(d_chain, sig6s, feps, nsteps, mode, 0, lds_atoms);
-    }
+    yk_chain<F32, BS><<<nchains, BS, dyn, c->stream>>>(d_chain, sig6s, feps, nsteps, mode, 0, lds_atoms);
     HIPCHK(hipGetLastError());
+    return YAMC_OK;
+}
+
+static int launch_chain(yamc_ctx *c, const YdChain *d_chain, const double *const *sig6s,
+                        const double *const *feps, int nchains, int nsteps, int mode, int max_natom) {
+    const bool f32 = c->precision == YAMC_FP32C;
+    if (c->chain_bs == 64 && mode == 0) {  // one wave per replica
+        int nb = (nchains + YW_WPB - 1) / YW_WPB;
+        if (f32) yk_wchain<true><<<nb, YW_WPB * 64, 0, c->stream>>>(d_chain, nchains, sig6s, feps, nsteps);
+        else yk_wchain<false><<<nb, YW_WPB * 64, 0, c->stream>>>(d_chain, nchains, sig6s, feps, nsteps);
+        HIPCHK(hipGetLastError());
+        return YAMC_OK;
+    }
+    if (c->chain_bs == 512)
+        return f32 ? launch_chain_t<true, 512>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom)
+                   : launch_chain_t<false, 512>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom);
+    return f32 ? launch_chain_t<true, 256>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom)
+               : launch_chain_t<false, 256>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom);
+}
+
+extern "C" int yamc_set_block_size(yamc_ctx *c, int32_t block) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    if (block != 64 && block != 256 && block != 512)
+        return fail(YAMC_ERROR_PARAMETER, "block size %d: must be 64 (one wave per replica), 256 or 512", block);
+    c->chain_bs = block;
     return YAMC_OK;
 }
 

@@ -24,6 +24,9 @@
 #define YD_TWO_PI (2.0 * YD_PI)
 #define YD_FOUR_PI (4.0 * YD_PI)
 #define YD_MAXA YAMC_MAX_MOL_ATOM
+// Capacity of the per-move real-space candidate list (atoms within the
+// cut-off of an old or a new site of the moved molecule).
+#define YD_NBCAP 2048
 #define YD_KP (YAMC_MAX_KMAX + 1)
 #define YD_MAXT YAMC_MAX_LJ_TYPE
 
@@ -53,6 +56,9 @@ struct YdChain {
     double *S;            // FP64 sums
     float *Shi, *Slo;     // FP32C sums
     double *uu, *Fx, *Fy, *Fz;
+    double *rsb;          // per candidate of the current move: new and old back terms (YD_NBCAP x 8)
+    int *nbl;             // candidate list of the current move (YD_NBCAP), wave kernel
+    float4 *pos4;         // FP32 positions + molecule number bits, wave kernel's scan
     double *freq_end;     // cumulative move intervals (BiasedRandom.f90:172-180)
     YdState *st;
     yamc_move_rec *rec;   // optional per-move records
@@ -123,6 +129,36 @@ __device__ double yd_shi_sans_exp(double x) {
         s = sinh(t);
         shi_f = shi_f + dx * s / t;
     }
+    shi_f = shi_f - dx * s / t / 2.0 - dx * dx / 12.0 * (cosh(t) / t - s / (t * t));
+    return exp(-x) * shi_f;
+}
+
+// shi_sans_exp_f evaluated by a whole wave (every lane passes the same x and
+// gets the same result).  The trapezoid terms dx sinh(t_i)/t_i, i = 2..n
+// (n <= 27), are computed one per lane and then added in index order, which
+// is the scalar loop's summation order, so the result is the scalar
+// function's to the bit.
+__device__ double yd_shi_sans_exp_wave(double x) {
+    if (x <= 1.0 || x > 20.0) return yd_shi_sans_exp(x);
+    const double dx0 = 0.2;
+    double x1, shi_f;
+    if (x < 5.0) {
+        x1 = 1.0; shi_f = 1.057250875;
+    } else if (x < 10.0) {
+        x1 = 5.0; shi_f = 20.093211826;
+    } else if (x < 15.0) {
+        x1 = 10.0; shi_f = 1246.11449;
+    } else {
+        x1 = 15.0; shi_f = 117477.926;
+    }
+    int n = (int)((x - x1) / dx0 + 2);
+    double dx = (x - x1) / (n - 1);
+    shi_f = shi_f + dx * sinh(x1) / x1 / 2.0 + dx * dx / 12.0 * (cosh(x1) / x1 - sinh(x1) / (x1 * x1));
+    const int lane = threadIdx.x & 63;
+    double t = x1 + (double)(lane + 1) * dx, s = sinh(t), term = dx * s / t;  // i = lane + 2
+    for (int l = 0; l < n - 1; ++l) shi_f = shi_f + __shfl(term, l, 64);
+    t = __shfl(t, n - 2, 64);
+    s = __shfl(s, n - 2, 64);
     shi_f = shi_f - dx * s / t / 2.0 - dx * dx / 12.0 * (cosh(t) / t - s / (t * t));
     return exp(-x) * shi_f;
 }

@@ -11,11 +11,10 @@
 // Nothing here is a dense contraction, so no MFMA is used.
 #include "yamc_device.h"
 
-#ifndef YK_BS
-#define YK_BS 512            // workgroup size of the chain kernel (8 waves)
-
-
-#endif
+// Largest workgroup of the chain kernel (8 waves) and the size of the
+// single-call kernels; the chain kernel is instantiated for 256 and 512 and
+// the launch picks one (yamc_tune measures both).
+#define YK_BS 512
 #define YK_NW (YK_BS / 64)
 #define YK_MAXP 2            // distinct LJ types inside one molecule
 // Bound on every rejection-sampling loop of the proposal.  The reference
@@ -25,7 +24,7 @@
 #define YK_TRIES 65536
 // Candidate neighbours of one move kept in LDS: atoms within the cut-off of
 // a new or an old site of the moved molecule.
-#define YK_NBCAP 2048
+#define YK_NBCAP YD_NBCAP
 // Candidate bitmap over atom indices (order-independent marking, then a
 // deterministic ascending compaction into the list).
 #define YK_BMWORDS 4096  // 131072 atoms
@@ -331,7 +330,7 @@ struct YkMcm {
 struct YkScalar {
     YkMcm mcm;
     YdRng rng;
-    double U_old, frn[YD_MAXA][3], uun[YD_MAXA], cmp_new[3], duc, dulj, du_real, du_ext;
+    double U_old, frn[YD_MAXA][3], uun[YD_MAXA], cmp_new[3], cmp_old[3], duc, dulj, du_real, du_ext;
     double mu[3], mu_new[3], muo[3];
     long long acc_count, nmv, nov;
 };
@@ -375,8 +374,7 @@ __device__ void yk_load_mol(const YdChain &c, YkMol &m, int imol) {
 }
 
 // Axis tables of the molecule's atoms (old or new positions): 3*na threads.
-__device__ void yk_mol_tables(const YdChain &c, YkMol &m, bool newpos) {
-    int t = threadIdx.x;
+__device__ void yk_mol_tables(const YdChain &c, YkMol &m, bool newpos, int t) {
     if (t < 3 * m.na) {
         int a = t / 3, d = t % 3;
         double v = newpos ? (d == 0 ? m.xn[a] : d == 1 ? m.yn[a] : m.zn[a])
@@ -421,11 +419,19 @@ __device__ __forceinline__ void yk_delta_lj(const YkMol &m, int p, int kx, int k
     for (int i = 0; i < 8; ++i) dl[i] = ns[i] - os[i];
 }
 
-// Loads the 8 components of set at k.  FP32C reads the hi word only: the
-// sweep's bandwidth is what the FP32 form saves.
+// Loads the 8 components of set at k.  FP32C reads the hi word of the LJ
+// sets (their k-space terms are small, the hi word's 2^-25 relative error
+// stays far below the dE bound) and hi + lo of the Coulomb set, whose
+// 2 beta C dC terms carry the dE error of a hi-only read.
 template <bool F32>
 __device__ __forceinline__ void yk_ld8(const YdChain &c, int set, int kk, double *C) {
     size_t base = (size_t)set * 8 * c.p.nk + kk;
+    if (F32 && set == 0) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+            C[m] = (double)c.Shi[base + (size_t)m * c.p.nk] + (double)c.Slo[base + (size_t)m * c.p.nk];
+        return;
+    }
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         if (F32) C[m] = (double)c.Shi[base + (size_t)m * c.p.nk];
@@ -455,18 +461,95 @@ __device__ __forceinline__ void yk_scatter3(double (&acc)[N], int base, int a, c
 // One sweep over k.  DELTA = false: k-space forces of the molecule at its
 // old positions against S (MC.f90:963-968).  DELTA = true: dU_C, dU_LJ
 // against S and forces at new positions against S + delta
-// (MC.f90:1136-1173).
+// (MC.f90:1136-1173).  Per k, each site's q sin/cos (SinCosKR_fill) is
+// evaluated once and reused by the delta sums, the Coulomb force and the
+// LJ force of every type (LJ part = Coulomb part / q, RhoSquared.f90:
+// 173-176).  Systems with up to YK_TPF LJ types load every S(k) value and
+// beta of a k before any arithmetic; larger T streams the LJ sets.
+#define YK_TPF 3
 template <bool F32, bool DELTA, int NA>
-__device__ void yk_sweep(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC]) {
+__device__ void yk_sweep(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size) {
     const int nk = c.p.nk, T = c.p.ntype;
     const double *tabs = DELTA ? m.tab_n : m.tab_o;
-    for (int kk = threadIdx.x; kk < nk; kk += YK_BS) {
+    const bool pf = T <= YK_TPF;
+    for (int kk = rank; kk < nk; kk += size) {
         int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
         double bt = c.beta[kk];
-        double C[8], dc[8], kr[8], krl[8];
+        double C[8];
+        double CP[YK_TPF][8], BL[YK_TPF][YK_MAXP];
         yk_ld8<F32>(c, 0, kk, C);
+        if (pf) {
+#pragma unroll
+            for (int t1 = 0; t1 < YK_TPF; ++t1) {
+                if (t1 < T) {
+                    yk_ld8<F32>(c, 1 + t1, kk, CP[t1]);
+#pragma unroll
+                    for (int q = 0; q < YK_MAXP; ++q)
+                        BL[t1][q] = q < m.np ? c.beta_lj[((size_t)t1 * T + (m.ptype[q] - 1)) * nk + kk] : 0.0;
+                }
+            }
+        }
+        // q sin/cos of every site at the sweep's positions; with DELTA also
+        // the new-minus-old sums of the Coulomb set and of each LJ slot, in
+        // site order (RhoSquared_addAtom into nulified sums, then
+        // RhoSquared_sub: MC.f90:946-958, 1096-1125)
+        double kn[NA][8], dc[8], dl[YK_MAXP][8];
+        {
+            double ns[8], os[8], nl[YK_MAXP][8], ol[YK_MAXP][8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                ns[i] = 0.0; os[i] = 0.0;
+#pragma unroll
+                for (int q = 0; q < YK_MAXP; ++q) { nl[q][i] = 0.0; ol[q][i] = 0.0; }
+            }
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const double qa = m.q[a];
+                const int ta = m.typ[a];
+                const double *tn = tabs + a * 6 * YD_KP;
+                if (qa != 0.0) {
+                    yd_kr(tn, qa, kx, ky, kz, kn[a]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) kn[a][i] = 0.0;
+                }
+                if (DELTA) {
+                    const double *to = m.tab_o + a * 6 * YD_KP;
+                    double ko[8];
+                    if (qa != 0.0) {
+                        yd_kr(to, qa, kx, ky, kz, ko);
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) { ns[i] = ns[i] + kn[a][i]; os[i] = os[i] + ko[i]; }
+                    }
+                    if (ta > 0) {
+                        double ln[8], lo[8];
+                        if (qa != 0.0) {
+                            double inv = 1.0 / qa;
+#pragma unroll
+                            for (int i = 0; i < 8; ++i) { ln[i] = kn[a][i] * inv; lo[i] = ko[i] * inv; }
+                        } else {
+                            yd_kr(tn, 1.0, kx, ky, kz, ln);
+                            yd_kr(to, 1.0, kx, ky, kz, lo);
+                        }
+#pragma unroll
+                        for (int q = 0; q < YK_MAXP; ++q)
+                            if (q == m.lslot[a]) {
+#pragma unroll
+                                for (int i = 0; i < 8; ++i) { nl[q][i] = nl[q][i] + ln[i]; ol[q][i] = ol[q][i] + lo[i]; }
+                            }
+                    }
+                }
+            }
+            if (DELTA) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    dc[i] = ns[i] - os[i];
+#pragma unroll
+                    for (int q = 0; q < YK_MAXP; ++q) dl[q][i] = nl[q][i] - ol[q][i];
+                }
+            }
+        }
         if (DELTA) {
-            yk_delta_coul(m, kx, ky, kz, dc);
             acc[0] += yd_du_k(C, dc, ky, kz) * bt;
 #pragma unroll
             for (int i = 0; i < 8; ++i) C[i] = C[i] + dc[i];
@@ -474,68 +557,102 @@ __device__ void yk_sweep(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             if (m.q[a] == 0.0) continue;
-            yd_kr(tabs + a * 6 * YD_KP, m.q[a], kx, ky, kz, kr);
             double f[3] = {0, 0, 0};
-            yd_force_k(C, kr, kx, ky, kz, bt, f);
+            yd_force_k(C, kn[a], kx, ky, kz, bt, f);
             acc[YK_FK(a, 0)] += f[0];
             acc[YK_FK(a, 1)] += f[1];
             acc[YK_FK(a, 2)] += f[2];
         }
         if (T == 0 || m.np == 0) continue;
-        double dl[YK_MAXP][8];
-        if (DELTA) {
-#pragma unroll
-            for (int p = 0; p < YK_MAXP; ++p)
-                if (p < m.np) yk_delta_lj(m, p, kx, ky, kz, dl[p]);
-        }
-        for (int t1 = 0; t1 < T; ++t1) {
-            double C1[8];
-            yk_ld8<F32>(c, 1 + t1, kk, C1);
+        // one LJ set t1 against the molecule (the body of calc_dU_LJ's t1
+        // loop and of ForceKSpace_LJ's type loop)
+        auto lj_set = [&](int t1, double *C1, const double *bl) {
             if (DELTA) {
                 int s1 = -1;
 #pragma unroll
-                for (int p = 0; p < YK_MAXP; ++p)
-                    if (p < m.np && m.ptype[p] == t1 + 1) s1 = p;
+                for (int q = 0; q < YK_MAXP; ++q)
+                    if (q < m.np && m.ptype[q] == t1 + 1) s1 = q;
                 double tw[8];
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     double d1 = 0.0;
 #pragma unroll
-                    for (int p = 0; p < YK_MAXP; ++p)
-                        if (p == s1) d1 = dl[p][i];
+                    for (int q = 0; q < YK_MAXP; ++q)
+                        if (q == s1) d1 = dl[q][i];
                     tw[i] = C1[i] + C1[i] + d1;
                     C1[i] = C1[i] + d1;
                 }
 #pragma unroll
-                for (int p = 0; p < YK_MAXP; ++p)
-                    if (p < m.np)
-                        acc[1] += yd_dulj_k(tw, dl[p], ky, kz) *
-                                  c.beta_lj[((size_t)t1 * T + (m.ptype[p] - 1)) * nk + kk];
+                for (int q = 0; q < YK_MAXP; ++q)
+                    if (q < m.np) acc[1] += yd_dulj_k(tw, dl[q], ky, kz) * bl[q];
             }
 #pragma unroll
             for (int a = 0; a < NA; ++a) {
-                int ta = m.typ[a];
+                const int ta = m.typ[a];
                 if (ta <= 0) continue;
-                yd_kr_atom(tabs + a * 6 * YD_KP, m.q[a], ta, kx, ky, kz, kr, krl);
+                const double qa = m.q[a];
+                double krl[8];
+                if (qa != 0.0) {
+                    double inv = 1.0 / qa;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) krl[i] = kn[a][i] * inv;
+                } else {
+                    yd_kr(tabs + a * 6 * YD_KP, 1.0, kx, ky, kz, krl);
+                }
                 double f[3] = {0, 0, 0};
-                yd_force_k(C1, krl, kx, ky, kz, c.beta_lj[((size_t)(ta - 1) * T + t1) * nk + kk], f);
+                // beta_LJ is symmetric in its type pair: row (ta-1, t1) = (t1, ta-1)
+                double b = 0.0;
+#pragma unroll
+                for (int q = 0; q < YK_MAXP; ++q)
+                    if (q == m.lslot[a]) b = bl[q];
+                if (m.lslot[a] < 0) b = c.beta_lj[((size_t)(ta - 1) * T + t1) * nk + kk];
+                yd_force_k(C1, krl, kx, ky, kz, b, f);
                 acc[YK_FK(a, 0)] += f[0];
                 acc[YK_FK(a, 1)] += f[1];
                 acc[YK_FK(a, 2)] += f[2];
+            }
+        };
+        if (pf) {
+#pragma unroll
+            for (int t1 = 0; t1 < YK_TPF; ++t1)
+                if (t1 < T) lj_set(t1, CP[t1], BL[t1]);
+        } else {
+            for (int t1 = 0; t1 < T; ++t1) {
+                double C1[8], bl[YK_MAXP];
+                yk_ld8<F32>(c, 1 + t1, kk, C1);
+#pragma unroll
+                for (int q = 0; q < YK_MAXP; ++q)
+                    bl[q] = q < m.np ? c.beta_lj[((size_t)t1 * T + (m.ptype[q] - 1)) * nk + kk] : 0.0;
+                lj_set(t1, C1, bl);
             }
         }
     }
 }
 
 // RhoSquared_add on acceptance (MC.f90:1245): S += delta for the Coulomb set
-// and the molecule's LJ sets.  FP32C keeps (hi, lo) as a two-sum pair.
+// and the molecule's LJ sets.  FP32C keeps (hi, lo) as a two-sum pair.  All
+// touched values of a k are loaded before the first store.
 template <bool F32>
-__device__ void yk_commit(const YdChain &c, const YkMol &m) {
+__device__ void yk_commit(const YdChain &c, const YkMol &m, int rank, int size) {
     const int nk = c.p.nk;
-    for (int kk = threadIdx.x; kk < nk; kk += YK_BS) {
+    for (int kk = rank; kk < nk; kk += size) {
         int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
-        double d[8];
-        for (int set = 0; set <= m.np; ++set) {
+        double v[1 + YK_MAXP][8];
+#pragma unroll
+        for (int set = 0; set <= YK_MAXP; ++set) {
+            if (set > m.np) break;
+            int sidx = set == 0 ? 0 : m.ptype[set - 1];
+            size_t base = (size_t)sidx * 8 * nk + kk;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                size_t idx = base + (size_t)i * nk;
+                v[set][i] = F32 ? (double)c.Shi[idx] + (double)c.Slo[idx] : c.S[idx];
+            }
+        }
+#pragma unroll
+        for (int set = 0; set <= YK_MAXP; ++set) {
+            if (set > m.np) break;
+            double d[8];
             if (set == 0) yk_delta_coul(m, kx, ky, kz, d);
             else yk_delta_lj(m, set - 1, kx, ky, kz, d);
             int sidx = set == 0 ? 0 : m.ptype[set - 1];
@@ -543,13 +660,13 @@ __device__ void yk_commit(const YdChain &c, const YkMol &m) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 size_t idx = base + (size_t)i * nk;
+                double w = v[set][i] + d[i];
                 if (F32) {
-                    double v = ((double)c.Shi[idx] + (double)c.Slo[idx]) + d[i];
-                    float hi = (float)v;
+                    float hi = (float)w;
                     c.Shi[idx] = hi;
-                    c.Slo[idx] = (float)(v - (double)hi);
+                    c.Slo[idx] = (float)(w - (double)hi);
                 } else {
-                    c.S[idx] = c.S[idx] + d[i];
+                    c.S[idx] = w;
                 }
             }
         }
@@ -563,7 +680,8 @@ __device__ void yk_commit(const YdChain &c, const YkMol &m) {
 
 // Adds one pair of site a into the partial-sum accumulators (constant
 // indices after unrolling, so acc stays in registers).
-__device__ __forceinline__ void yk_rs_acc(double (&acc)[YK_NRS], int a, const YdPair &pr, double xij, double yij,
+template <int N>
+__device__ __forceinline__ void yk_rs_acc(double (&acc)[N], int a, const YdPair &pr, double xij, double yij,
                                           double zij) {
 #pragma unroll
     for (int b = 0; b < YD_MAXA; ++b)
@@ -581,7 +699,7 @@ template <int NA>
 __device__ void yk_rs_new(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
                           double (&acc)[YK_NRS], int &ov) {
     const int N = c.p.natom, T = c.p.ntype, me = m.imol + 1;
-    for (int j = threadIdx.x; j < N; j += YK_BS) {
+    for (int j = threadIdx.x; j < N; j += (int)blockDim.x) {
         int jm = c.molnum[j];
         if (jm == me) continue;
         int jt = c.type[j];
@@ -603,53 +721,6 @@ __device__ void yk_rs_new(const YdChain &c, const YkMol &m, const double *sig6, 
     }
 }
 
-// Accepted move: per-atom arrays of every other atom take the new minus the
-// old contribution of the molecule (MC.f90:1251-1275), computed per j in the
-// reference's atom order; acc[0..2] = old uu_ew, uu_lj6, uu_lj12.
-__device__ void yk_rs_update(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
-                             double (&acc)[3]) {
-    const int N = c.p.natom, T = c.p.ntype, me = m.imol + 1;
-    for (int j = threadIdx.x; j < N; j += YK_BS) {
-        int jm = c.molnum[j];
-        if (jm == me) continue;
-        int jt = c.type[j];
-        double qj = c.q[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
-        double nb[4] = {0, 0, 0, 0}, ob[4] = {0, 0, 0, 0};
-        for (int pass = 0; pass < 2; ++pass) {
-            double *b = pass == 0 ? nb : ob;
-            for (int a = 0; a < m.na; ++a) {
-                int it = m.typ[a];
-                double xa = pass == 0 ? m.xn[a] : m.xo[a], ya = pass == 0 ? m.yn[a] : m.yo[a],
-                       za = pass == 0 ? m.zn[a] : m.zo[a];
-                double xij = yd_wrap(xa - xj), yij = yd_wrap(ya - yj), zij = yd_wrap(za - zj);
-                double r2 = xij * xij + yij * yij + zij * zij;
-                if (r2 >= c.p.rmax2) continue;
-                bool lj = it != 0 && jt != 0;
-                int off = lj ? (it - 1) * T + (jt - 1) : 0;
-                YdPair pr;
-                yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
-                b[0] = b[0] + pr.pot;
-                b[1] = b[1] - xij * pr.FR;
-                b[2] = b[2] - yij * pr.FR;
-                b[3] = b[3] - zij * pr.FR;
-                if (pass == 1) {
-                    acc[0] += pr.potew;
-                    acc[1] += pr.potlj6;
-                    acc[2] += pr.potlj12;
-                }
-            }
-        }
-        if (nb[0] != 0.0 || ob[0] != 0.0 || nb[1] != 0.0 || ob[1] != 0.0 || nb[2] != 0.0 || ob[2] != 0.0 ||
-            nb[3] != 0.0 || ob[3] != 0.0) {
-            c.uu[j] = c.uu[j] + nb[0] - ob[0];
-            c.Fx[j] = c.Fx[j] + nb[1] - ob[1];
-            c.Fy[j] = c.Fy[j] + nb[2] - ob[2];
-            c.Fz[j] = c.Fz[j] + nb[3] - ob[3];
-        }
-    }
-}
-
-
 // Fused real-space pass of a trial move over every other atom j: the new
 // partial sum (EwaldSumRealSpace.f90:338-498 for the molecule at its new
 // sites) and the list of atoms within the cut-off of a new or an old site,
@@ -657,9 +728,9 @@ __device__ void yk_rs_update(const YdChain &c, const YkMol &m, const double *sig
 // (MC.f90:1251-1275).  *count may exceed YK_NBCAP; the list is then partial.
 template <int NA>
 __device__ void yk_rs_move(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
-                           double (&acc)[YK_NRS], int &ov, int *list, int *count) {
+                           double (&acc)[YK_NRS], int &ov, int *list, int *count, int rank, int size) {
     const int N = c.p.natom, T = c.p.ntype, me = m.imol + 1;
-    for (int j = threadIdx.x; j < N; j += YK_BS) {
+    for (int j = rank; j < N; j += size) {
         int jm = c.molnum[j];
         if (jm == me) continue;
         int jt = c.type[j];
@@ -728,17 +799,13 @@ __device__ __forceinline__ void yk_rs_apply(const YdChain &c, const YkMol &m, co
     c.Fz[j] = c.Fz[j] + nb[3] - ob[3];
 }
 
-// Commit of the real-space arrays: over the candidate list, or over every
-// atom when the list overflowed.
-__device__ void yk_rs_commit(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
-                             const int *list, int count, double (&acc)[3]) {
-    if (count <= YK_NBCAP) {
-        for (int i = threadIdx.x; i < count; i += YK_BS) yk_rs_apply(c, m, sig6, feps, list[i], acc);
-    } else {
-        const int me = m.imol + 1;
-        for (int j = threadIdx.x; j < c.p.natom; j += YK_BS)
-            if (c.molnum[j] != me) yk_rs_apply(c, m, sig6, feps, j, acc);
-    }
+// Commit of the real-space arrays over every other atom (the candidate list
+// overflowed); acc[0..2] gather the old partial sum's uu_ew, uu_lj6, uu_lj12.
+__device__ void yk_rs_commit_all(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
+                                 double (&acc)[3], int rank, int size) {
+    const int me = m.imol + 1;
+    for (int j = rank; j < c.p.natom; j += size)
+        if (c.molnum[j] != me) yk_rs_apply(c, m, sig6, feps, j, acc);
 }
 
 // Real-space pass of a trial move in two stages.
@@ -766,7 +833,7 @@ __device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const float4 *pos, 
             xof[a] = (float)m.xo[a]; yof[a] = (float)m.yo[a]; zof[a] = (float)m.zo[a];
             xnf[a] = (float)m.xn[a]; ynf[a] = (float)m.yn[a]; znf[a] = (float)m.zn[a];
         }
-        for (int j = threadIdx.x; j < N; j += YK_BS) {
+        for (int j = threadIdx.x; j < N; j += (int)blockDim.x) {
             float4 pj = pos[j];
             bool near = false;
 #pragma unroll
@@ -785,7 +852,7 @@ __device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const float4 *pos, 
             if (near && __float_as_int(pj.w) != me) atomicOr(&bm[j >> 5], 1u << (j & 31));
         }
     } else {
-        for (int j = threadIdx.x; j < N; j += YK_BS) {
+        for (int j = threadIdx.x; j < N; j += (int)blockDim.x) {
             double xj = c.x[j], yj = c.y[j], zj = c.z[j];
             int jm = c.molnum[j];
             bool near = false;
@@ -801,14 +868,25 @@ __device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const float4 *pos, 
     }
 }
 
+// Accumulators of the exact pass: the partial-sum layout (YK_NRS) followed
+// by the old sites' uu_ew, uu_lj6, uu_lj12.
+#define YK_NRX (YK_NRS + 3)
+
+// Exact FP64 pass over the compacted candidate list.  For every candidate j
+// it evaluates the molecule's new sites (the partial sum and the overlap
+// test, EwaldSumRealSpace.f90:338-498) and its old sites, and stores j's
+// back terms of both (uu, -F summed over the sites in index order, as the
+// back arrays of the two partial sums hold them, MC.f90:1251-1275) in
+// c.rsb, so that an accepted move commits them without a second pair pass.
 template <int NA>
 __device__ void yk_rs_exact(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
-                            const int *list, int count, double (&acc)[YK_NRS], int &ov) {
+                            const int *list, int count, double (&acc)[YK_NRX], int &ov, int rank, int size) {
     const int T = c.p.ntype;
-    for (int i = threadIdx.x; i < count; i += YK_BS) {
+    for (int i = rank; i < count; i += size) {
         int j = list[i];
         int jt = c.type[j];
         double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+        double nb[4] = {0, 0, 0, 0}, ob[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             int it = m.typ[a];
@@ -822,7 +900,44 @@ __device__ void yk_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
             YdPair pr;
             yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
             yk_rs_acc(acc, a, pr, xij, yij, zij);
+            nb[0] = nb[0] + pr.pot;
+            nb[1] = nb[1] - xij * pr.FR;
+            nb[2] = nb[2] - yij * pr.FR;
+            nb[3] = nb[3] - zij * pr.FR;
         }
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            int it = m.typ[a];
+            double xij = yd_wrap(m.xo[a] - xj), yij = yd_wrap(m.yo[a] - yj), zij = yd_wrap(m.zo[a] - zj);
+            double r2 = xij * xij + yij * yij + zij * zij;
+            if (r2 >= c.p.rmax2) continue;
+            bool lj = it != 0 && jt != 0;
+            int off = lj ? (it - 1) * T + (jt - 1) : 0;
+            YdPair pr;
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            ob[0] = ob[0] + pr.pot;
+            ob[1] = ob[1] - xij * pr.FR;
+            ob[2] = ob[2] - yij * pr.FR;
+            ob[3] = ob[3] - zij * pr.FR;
+            acc[YK_NRS + 0] += pr.potew;
+            acc[YK_NRS + 1] += pr.potlj6;
+            acc[YK_NRS + 2] += pr.potlj12;
+        }
+        double *b = c.rsb + (size_t)i * 8;
+        b[0] = nb[0]; b[1] = nb[1]; b[2] = nb[2]; b[3] = nb[3];
+        b[4] = ob[0]; b[5] = ob[1]; b[6] = ob[2]; b[7] = ob[3];
+    }
+}
+
+// Commit of the per-atom arrays from the stored back terms of the list.
+__device__ void yk_rs_commit_list(const YdChain &c, const int *list, int count, int rank, int size) {
+    for (int i = rank; i < count; i += size) {
+        int j = list[i];
+        const double *b = c.rsb + (size_t)i * 8;
+        c.uu[j] = c.uu[j] + b[0] - b[4];
+        c.Fx[j] = c.Fx[j] + b[1] - b[5];
+        c.Fy[j] = c.Fy[j] + b[2] - b[6];
+        c.Fz[j] = c.Fz[j] + b[3] - b[7];
     }
 }
 
@@ -841,7 +956,7 @@ __device__ __forceinline__ void yk_rs_scan_n(const YdChain &c, const YkMol &m, c
 // *count gets the number of candidates (may exceed YK_NBCAP: list partial).
 __device__ void yk_compact(const unsigned int *bm, int nwords, int *list, int *count, int *wsum) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int per = (nwords + YK_BS - 1) / YK_BS, w0 = tid * per;
+    const int bs = blockDim.x, per = (nwords + bs - 1) / bs, w0 = tid * per;
     int cnt = 0;
     for (int k = 0; k < per; ++k)
         if (w0 + k < nwords) cnt += __popc(bm[w0 + k]);
@@ -854,7 +969,7 @@ __device__ void yk_compact(const unsigned int *bm, int nwords, int *list, int *c
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
     int base = 0, total = 0;
-    for (int i = 0; i < YK_NW; ++i) {
+    for (int i = 0; i < (bs >> 6); ++i) {
         if (i < w) base += wsum[i];
         total += wsum[i];
     }
@@ -874,34 +989,35 @@ __device__ void yk_compact(const unsigned int *bm, int nwords, int *list, int *c
 
 __device__ __forceinline__ void yk_rs_exact_n(const YdChain &c, const YkMol &m, const double *sig6,
                                               const double *feps, const int *list, int count,
-                                              double (&acc)[YK_NRS], int &ov) {
+                                              double (&acc)[YK_NRX], int &ov, int rank, int size) {
     switch (m.na) {
-        case 1: yk_rs_exact<1>(c, m, sig6, feps, list, count, acc, ov); break;
-        case 2: yk_rs_exact<2>(c, m, sig6, feps, list, count, acc, ov); break;
-        case 3: yk_rs_exact<3>(c, m, sig6, feps, list, count, acc, ov); break;
-        default: yk_rs_exact<4>(c, m, sig6, feps, list, count, acc, ov); break;
+        case 1: yk_rs_exact<1>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+        case 2: yk_rs_exact<2>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+        case 3: yk_rs_exact<3>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+        default: yk_rs_exact<4>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
     }
 }
 
 // Dispatch on the molecule size so every per-site loop has a compile-time
 // trip count (keeps the per-site accumulators in registers).
 template <bool F32, bool DELTA>
-__device__ __forceinline__ void yk_sweep_n(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC]) {
+__device__ __forceinline__ void yk_sweep_n(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank,
+                                           int size) {
     switch (m.na) {
-        case 1: yk_sweep<F32, DELTA, 1>(c, m, acc); break;
-        case 2: yk_sweep<F32, DELTA, 2>(c, m, acc); break;
-        case 3: yk_sweep<F32, DELTA, 3>(c, m, acc); break;
-        default: yk_sweep<F32, DELTA, 4>(c, m, acc); break;
+        case 1: yk_sweep<F32, DELTA, 1>(c, m, acc, rank, size); break;
+        case 2: yk_sweep<F32, DELTA, 2>(c, m, acc, rank, size); break;
+        case 3: yk_sweep<F32, DELTA, 3>(c, m, acc, rank, size); break;
+        default: yk_sweep<F32, DELTA, 4>(c, m, acc, rank, size); break;
     }
 }
 __device__ __forceinline__ void yk_rs_move_n(const YdChain &c, const YkMol &m, const double *sig6,
                                              const double *feps, double (&acc)[YK_NRS], int &ov, int *list,
-                                             int *count) {
+                                             int *count, int rank, int size) {
     switch (m.na) {
-        case 1: yk_rs_move<1>(c, m, sig6, feps, acc, ov, list, count); break;
-        case 2: yk_rs_move<2>(c, m, sig6, feps, acc, ov, list, count); break;
-        case 3: yk_rs_move<3>(c, m, sig6, feps, acc, ov, list, count); break;
-        default: yk_rs_move<4>(c, m, sig6, feps, acc, ov, list, count); break;
+        case 1: yk_rs_move<1>(c, m, sig6, feps, acc, ov, list, count, rank, size); break;
+        case 2: yk_rs_move<2>(c, m, sig6, feps, acc, ov, list, count, rank, size); break;
+        case 3: yk_rs_move<3>(c, m, sig6, feps, acc, ov, list, count, rank, size); break;
+        default: yk_rs_move<4>(c, m, sig6, feps, acc, ov, list, count, rank, size); break;
     }
 }
 __device__ __forceinline__ void yk_rs_new_n(const YdChain &c, const YkMol &m, const double *sig6,
@@ -995,7 +1111,9 @@ __device__ void yk_displacement(const YdParams &p, YkMcm &m, YdRng &rng) {
     m.umaxold = umax;
 }
 
-// MonteCarloMove.f90:269-444 chooseRotation
+// MonteCarloMove.f90:269-444 chooseRotation (WAVE: called by a whole wave
+// with identical arguments in every lane).
+template <bool WAVE>
 __device__ void yk_rotation(const YdParams &p, YkMcm &m, YdRng &rng) {
     double cx = m.co[0], cy = m.co[1], cz = m.co[2], cabs = m.couplold;
     double vmax = p.lambda_c * cabs * p.d_angle, qold, angle, vx, vy, vz;
@@ -1018,7 +1136,7 @@ __device__ void yk_rotation(const YdParams &p, YkMcm &m, YdRng &rng) {
             xksi = rng.next();
         }
         double cosx = v / vmax;
-        qold = yd_shi_sans_exp(vmax) / vmax;
+        qold = (WAVE ? yd_shi_sans_exp_wave(vmax) : yd_shi_sans_exp(vmax)) / vmax;
         xksi = 1.0;
         for (int g = 0; xksi > (1.0 - 1e-10); ++g) {
             if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
@@ -1068,6 +1186,7 @@ __device__ void yk_rotation(const YdParams &p, YkMcm &m, YdRng &rng) {
 }
 
 // MonteCarloMove.f90:56-106 init_new
+template <bool WAVE>
 __device__ void yk_init_new(const YdParams &p, YkMcm &m) {
     m.ftotnew = sqrt(m.fn[0] * m.fn[0] + m.fn[1] * m.fn[1] + m.fn[2] * m.fn[2]);
     m.couplnew = sqrt(m.cn[0] * m.cn[0] + m.cn[1] * m.cn[1] + m.cn[2] * m.cn[2]);
@@ -1081,7 +1200,7 @@ __device__ void yk_init_new(const YdParams &p, YkMcm &m) {
     }
     m.umaxnew = umaxnew; m.cnew = cnew;
     double vmaxnew = p.lambda_c * m.couplnew * p.d_angle, qnew;
-    if (vmaxnew > 1e-2) qnew = yd_shi_sans_exp(vmaxnew) / vmaxnew;
+    if (vmaxnew > 1e-2) qnew = (WAVE ? yd_shi_sans_exp_wave(vmaxnew) : yd_shi_sans_exp(vmaxnew)) / vmaxnew;
     else { vmaxnew = 0.0; qnew = 1.0; }
     m.vmaxnew = vmaxnew; m.qnew = qnew;
 }
@@ -1135,14 +1254,230 @@ __device__ int yk_choose(const YdChain &c, double rnd) {
         }                                                           \
     } while (0)
 
+// ------------------------------------------------------------------------
+// Scalar sections of a move, shared by the two chain kernels.  The
+// workgroup kernel runs them on thread 0; the wave kernel (WAVE = true) runs
+// them on all 64 lanes of the replica's wave with identical values in every
+// lane (LDS stores of one value from all lanes; global stores from lane 0),
+// which lets shi_sans_exp spread its trapezoid terms over the lanes.
+
+// Molecule pick (runmc.f90:206, 232, 258) and the molecule's scalar data.
+__device__ __forceinline__ void yk_sec_pick(const YdChain &c, YkMol &m, YdRng &rng, int mode, int mv) {
+    int imol;
+    if (mode == 1) {
+        imol = c.cimol[mv];
+        rng.cu = c.cu[mv];
+    } else {
+        if (mv % c.p.nmol == 0) (void)rng.next();  // pressure-step draw per cycle
+        (void)rng.next();                           // exchange draw per step
+        imol = yk_choose(c, rng.next());
+    }
+    m.imol = imol;
+    m.first = c.first[imol];
+    m.na = c.last[imol] - m.first + 1;
+}
+
+// LJ types present in the molecule (after the per-site load).
+__device__ __forceinline__ void yk_sec_slots(YkMol &m) {
+    m.np = 0;
+    for (int a = 0; a < m.na; ++a) {
+        m.lslot[a] = -1;
+        if (m.typ[a] > 0) {
+            int sl = -1;
+            for (int q = 0; q < m.np; ++q)
+                if (m.ptype[q] == m.typ[a]) sl = q;
+            if (sl < 0 && m.np < YK_MAXP) { sl = m.np; m.ptype[m.np++] = m.typ[a]; }
+            m.lslot[a] = sl;
+        }
+    }
+}
+
+// Proposal (MC.f90:984-1067): old total force and torque from the old
+// k-space sweep (out[YK_FK]) plus the per-atom real-space and external
+// parts, then the force/torque-biased displacement and rotation.  Returns
+// the stall flag.
+template <bool WAVE>
+__device__ int yk_sec_propose(const YdChain &c, YkMol &m, YkScalar &t0, const double *out,
+                              const double (*fr_old)[3], const double *uu_old) {
+    const YdParams &p = c.p;
+    YkMcm &mcm = t0.mcm;
+    int na = m.na;
+    double fx[YD_MAXA], fy[YD_MAXA], fz[YD_MAXA];
+    double U_old = 0.0;
+    for (int a = 0; a < na; ++a) U_old = U_old + uu_old[a];
+    t0.U_old = U_old;
+    for (int a = 0; a < na; ++a) {
+        fx[a] = fr_old[a][0] + YD_FOUR_PI * out[YK_FK(a, 0)];
+        fy[a] = fr_old[a][1] + YD_FOUR_PI * out[YK_FK(a, 1)];
+        fz[a] = fr_old[a][2] + YD_FOUR_PI * out[YK_FK(a, 2)];
+        fx[a] = fx[a] + p.m2kext * m.q[a] * t0.mu[0];
+        fy[a] = fy[a] + p.m2kext * m.q[a] * t0.mu[1];
+        fz[a] = fz[a] + p.m2kext * m.q[a] * t0.mu[2];
+    }
+    double mo[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < na; ++a) {
+        mo[0] = mo[0] + m.q[a] * m.xo[a];
+        mo[1] = mo[1] + m.q[a] * m.yo[a];
+        mo[2] = mo[2] + m.q[a] * m.zo[a];
+    }
+    t0.muo[0] = mo[0]; t0.muo[1] = mo[1]; t0.muo[2] = mo[2];
+    double Fo[3] = {0, 0, 0}, To[3] = {0, 0, 0}, com[3] = {0, 0, 0}, tm = 0.0, tq[3];
+    for (int a = 0; a < na; ++a) { Fo[0] = Fo[0] + fx[a]; Fo[1] = Fo[1] + fy[a]; Fo[2] = Fo[2] + fz[a]; }
+    for (int a = 0; a < na; ++a) {
+        com[0] = com[0] + m.mass[a] * m.xo[a];
+        com[1] = com[1] + m.mass[a] * m.yo[a];
+        com[2] = com[2] + m.mass[a] * m.zo[a];
+        tm = tm + m.mass[a];
+    }
+    com[0] = com[0] / tm; com[1] = com[1] / tm; com[2] = com[2] / tm;
+    for (int a = 0; a < na; ++a) {
+        yk_prod(m.xo[a] - com[0], m.yo[a] - com[1], m.zo[a] - com[2], fx[a], fy[a], fz[a], tq);
+        To[0] = To[0] + tq[0]; To[1] = To[1] + tq[1]; To[2] = To[2] + tq[2];
+    }
+    for (int d = 0; d < 3; ++d) { mcm.fo[d] = Fo[d]; mcm.co[d] = To[d]; }
+    mcm.ftotold = sqrt(Fo[0] * Fo[0] + Fo[1] * Fo[1] + Fo[2] * Fo[2]);
+    mcm.couplold = sqrt(To[0] * To[0] + To[1] * To[1] + To[2] * To[2]);
+    mcm.stall = 0;
+    yk_displacement(p, mcm, t0.rng);
+    yk_rotation<WAVE>(p, mcm, t0.rng);
+    double cn[3];
+    for (int d = 0; d < 3; ++d) cn[d] = com[d] + mcm.d[d] - round(com[d] + mcm.d[d]);
+    if (na > 1) {
+        double cphi = mcm.cphi, sphi = mcm.sphi, st = mcm.st, ct = mcm.ct, ca = mcm.ca, sa = mcm.sa;
+        double ca1 = 1.0 - ca, R[9];
+        R[0] = (cphi * cphi * (st * st) * ca1 + ca);
+        R[1] = (sphi * cphi * (st * st) * ca1 - ct * sa);
+        R[2] = st * (ct * ca1 * cphi + sa * sphi);
+        R[3] = (cphi * sphi * (st * st) * ca1 + ct * sa);
+        R[4] = (sphi * sphi * (st * st) * ca1 + ca);
+        R[5] = st * (ct * ca1 * sphi - sa * cphi);
+        R[6] = st * (ct * ca1 * cphi - sa * sphi);
+        R[7] = st * (ct * ca1 * sphi + sa * cphi);
+        R[8] = (st * st * ca + ct * ct);
+        for (int a = 0; a < na; ++a) {
+            double xr = m.xo[a] - com[0], yr = m.yo[a] - com[1], zr = m.zo[a] - com[2];
+            m.xn[a] = (R[0] * xr + R[1] * yr + R[2] * zr) + cn[0];
+            m.yn[a] = (R[3] * xr + R[4] * yr + R[5] * zr) + cn[1];
+            m.zn[a] = (R[6] * xr + R[7] * yr + R[8] * zr) + cn[2];
+        }
+    } else {
+        m.xn[0] = cn[0]; m.yn[0] = cn[1]; m.zn[0] = cn[2];
+    }
+    return mcm.stall;
+}
+
+// Acceptance (MC.f90:1176-1233): new forces and torque, dU_ext, the
+// acceptance factor and the draw.  out holds the new k-space sweep's sums.
+// Returns 1 if accepted; the move record goes to c.rec[mv] when present.
+template <bool WAVE>
+__device__ int yk_sec_accept(const YdChain &c, const YkMol &m, YkScalar &t0, const double *out, int mv) {
+    const YdParams &p = c.p;
+    YkMcm &mcm = t0.mcm;
+    int na = m.na;
+    double U_new = 0.0;
+    for (int a = 0; a < na; ++a) U_new = U_new + t0.uun[a];
+    double du_real = U_new - t0.U_old, duc = out[0], dulj = out[1];
+    double fx[YD_MAXA], fy[YD_MAXA], fz[YD_MAXA];
+    for (int a = 0; a < na; ++a) {
+        fx[a] = t0.frn[a][0] + YD_FOUR_PI * out[YK_FK(a, 0)];
+        fy[a] = t0.frn[a][1] + YD_FOUR_PI * out[YK_FK(a, 1)];
+        fz[a] = t0.frn[a][2] + YD_FOUR_PI * out[YK_FK(a, 2)];
+    }
+    double mun[3] = {0, 0, 0};
+    for (int a = 0; a < na; ++a) {
+        mun[0] = mun[0] + m.q[a] * m.xn[a];
+        mun[1] = mun[1] + m.q[a] * m.yn[a];
+        mun[2] = mun[2] + m.q[a] * m.zn[a];
+    }
+    double dmu[3] = {mun[0] - t0.muo[0], mun[1] - t0.muo[1], mun[2] - t0.muo[2]};
+    double du_ext = p.kext * ((2 * t0.mu[0] + dmu[0]) * dmu[0] + (2 * t0.mu[1] + dmu[1]) * dmu[1] +
+                              (2 * t0.mu[2] + dmu[2]) * dmu[2]);
+    double mn[3];
+    for (int d = 0; d < 3; ++d) mn[d] = t0.mu[d] + dmu[d];
+    for (int a = 0; a < na; ++a) {
+        fx[a] = fx[a] + p.m2kext * m.q[a] * mn[0];
+        fy[a] = fy[a] + p.m2kext * m.q[a] * mn[1];
+        fz[a] = fz[a] + p.m2kext * m.q[a] * mn[2];
+    }
+    double Fn[3] = {0, 0, 0}, Tn[3] = {0, 0, 0}, com[3] = {0, 0, 0}, tm = 0.0, tq[3];
+    for (int a = 0; a < na; ++a) { Fn[0] = Fn[0] + fx[a]; Fn[1] = Fn[1] + fy[a]; Fn[2] = Fn[2] + fz[a]; }
+    for (int a = 0; a < na; ++a) {
+        com[0] = com[0] + m.mass[a] * m.xn[a];
+        com[1] = com[1] + m.mass[a] * m.yn[a];
+        com[2] = com[2] + m.mass[a] * m.zn[a];
+        tm = tm + m.mass[a];
+    }
+    com[0] = com[0] / tm; com[1] = com[1] / tm; com[2] = com[2] / tm;
+    for (int a = 0; a < na; ++a) {
+        yk_prod(m.xn[a] - com[0], m.yn[a] - com[1], m.zn[a] - com[2], fx[a], fy[a], fz[a], tq);
+        Tn[0] = Tn[0] + tq[0]; Tn[1] = Tn[1] + tq[1]; Tn[2] = Tn[2] + tq[2];
+    }
+    for (int d = 0; d < 3; ++d) { mcm.fn[d] = Fn[d]; mcm.cn[d] = Tn[d]; }
+    double du_total = du_real + (duc + dulj) + du_ext;
+    yk_init_new<WAVE>(p, mcm);
+    int accd = yk_accept(p, mcm, du_total, t0.rng);
+    t0.du_real = du_real; t0.duc = duc; t0.dulj = dulj; t0.du_ext = du_ext;
+    for (int d = 0; d < 3; ++d) t0.mu_new[d] = mn[d];
+    if (c.rec && (threadIdx.x & 63) == 0) {
+        yamc_move_rec r = {};
+        r.imol = m.imol; r.accepted = accd; r.overlap = 0;
+        r.du_real = du_real; r.du_kc = duc; r.du_klj = dulj; r.du_ext = du_ext; r.du_total = du_total;
+        for (int d = 0; d < 3; ++d) {
+            r.f_old[d] = mcm.fo[d]; r.t_old[d] = mcm.co[d]; r.f_new[d] = Fn[d]; r.t_new[d] = Tn[d];
+            r.delta[d] = mcm.d[d];
+        }
+        r.angle = mcm.angle;
+        c.rec[mv] = r;
+    }
+    return accd;
+}
+
+// Record of a move rejected before the acceptance test: overlap = 1 (hard
+// core, MC.f90:1080-1084) or 2 (a proposal loop over its bound).
+__device__ __forceinline__ void yk_sec_reject_rec(const YdChain &c, const YkMol &m, const YkMcm &mcm, int mv,
+                                                  int overlap) {
+    if (!c.rec) return;
+    yamc_move_rec r = {};
+    r.imol = m.imol; r.overlap = overlap;
+    if (overlap == 1) {
+        for (int d = 0; d < 3; ++d) { r.f_old[d] = mcm.fo[d]; r.t_old[d] = mcm.co[d]; r.delta[d] = mcm.d[d]; }
+        r.angle = mcm.angle;
+    }
+    c.rec[mv] = r;
+}
+
+// Commit of the moved molecule's own per-atom arrays, positions and the
+// energy components (MC.f90:1242-1318); single lane.
+__device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m, YkScalar &t0, float4 *pos) {
+    int f = m.first;
+    for (int a = 0; a < m.na; ++a) {
+        c.uu[f + a] = t0.uun[a];
+        c.Fx[f + a] = t0.frn[a][0];
+        c.Fy[f + a] = t0.frn[a][1];
+        c.Fz[f + a] = t0.frn[a][2];
+        c.x[f + a] = m.xn[a];
+        c.y[f + a] = m.yn[a];
+        c.z[f + a] = m.zn[a];
+        if (pos) pos[f + a] = make_float4((float)m.xn[a], (float)m.yn[a], (float)m.zn[a], __int_as_float(m.imol + 1));
+    }
+    YdState *st = c.st;
+    st->e_kc = st->e_kc + t0.duc;
+    st->e_klj = st->e_klj + t0.dulj;
+    st->uu_ew = st->uu_ew + t0.cmp_new[0] - t0.cmp_old[0];
+    st->uu_lj6 = st->uu_lj6 + t0.cmp_new[1] - t0.cmp_old[1];
+    st->uu_lj12 = st->uu_lj12 + t0.cmp_new[2] - t0.cmp_old[2];
+    for (int d = 0; d < 3; ++d) st->mu[d] = t0.mu_new[d];
+}
+
 // ========================================================================
-// The chain kernel.  mode 0: run_mc cycles (runmc.f90:204-268) with the
-// Philox stream, nsteps = cycles; mode 1: nsteps moves with caller-given
-// molecule and constant uniform per move.
-template <bool F32>
-__global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const double *const *sig6s,
-                                                  const double *const *feps_s, int nsteps, int mode,
-                                                  int max_moves, int lds_atoms) {
+// The workgroup chain kernel: one workgroup per chain, every per-move sum
+// spread over the workgroup.  mode 0: run_mc cycles (runmc.f90:204-268)
+// with the Philox stream, nsteps = cycles; mode 1: nsteps moves with
+// caller-given molecule and constant uniform per move.
+template <bool F32, int BS>
+__global__ void __launch_bounds__(BS) yk_chain(const YdChain *chains, const double *const *sig6s,
+                                               const double *const *feps_s, int nsteps, int mode, int max_moves,
+                                               int lds_atoms) {
     __shared__ YkShared sh;
     extern __shared__ float4 lds_pos[];  // FP32 positions when the launch provides LDS for them
     const YdChain &c = chains[blockIdx.x];
@@ -1151,54 +1486,35 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
     const YdParams &p = c.p;
     const bool use_lds = lds_atoms >= p.natom;
     if (use_lds)
-        for (int i = tid; i < p.natom; i += YK_BS)
+        for (int i = tid; i < p.natom; i += (int)blockDim.x)
             lds_pos[i] = make_float4((float)c.x[i], (float)c.y[i], (float)c.z[i], __int_as_float(c.molnum[i]));
     YkScalar &t0 = sh.t0;
-    YkMcm &mcm = t0.mcm;
-    YdRng &rng = t0.rng;
-    double &U_old = t0.U_old, (&frn)[YD_MAXA][3] = t0.frn, (&uun)[YD_MAXA] = t0.uun, (&cmp_new)[3] = t0.cmp_new;
-    double &duc = t0.duc, &dulj = t0.dulj, &du_real = t0.du_real, &du_ext = t0.du_ext;
-    double(&mu)[3] = t0.mu, (&mu_new)[3] = t0.mu_new, (&muo)[3] = t0.muo;
-    long long &acc_count = t0.acc_count, &nmv = t0.nmv, &nov = t0.nov;
     unsigned long long ph[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, t_last = 0;
     // mu = sum_j q_j r_j from scratch once per launch (EwaldSumExternal.f90:
     // 165-183); afterwards it is carried incrementally, mu + dmu on each
     // accepted move, which is the value movemc recomputes (MC.f90:916).
     {
         double v[3] = {0, 0, 0};
-        for (int i = tid; i < p.natom; i += YK_BS) {
+        for (int i = tid; i < p.natom; i += (int)blockDim.x) {
             double q = c.q[i];
             v[0] += q * c.x[i]; v[1] += q * c.y[i]; v[2] += q * c.z[i];
         }
         yd_block_sum<3>(v, sh.red, sh.out);
     }
     if (tid == 0) {
-        mu[0] = sh.out[0]; mu[1] = sh.out[1]; mu[2] = sh.out[2];
+        t0.mu[0] = sh.out[0]; t0.mu[1] = sh.out[1]; t0.mu[2] = sh.out[2];
         t_last = wall_clock64();
-        acc_count = 0; nmv = 0; nov = 0;
+        t0.acc_count = 0; t0.nmv = 0; t0.nov = 0;
+        YdRng &rng = t0.rng;
         rng.key0 = c.st->key[0]; rng.key1 = c.st->key[1];
         rng.s0 = c.st->stream[0]; rng.s1 = c.st->stream[1];
         rng.ctr = c.st->ctr; rng.constant = mode == 1; rng.cu = 0.0;
     }
-    int nmol = p.nmol;
-    int total = mode == 1 ? nsteps : nsteps * nmol;
+    int total = mode == 1 ? nsteps : nsteps * p.nmol;
     if (max_moves > 0 && total > max_moves) total = max_moves;
     for (int mv = 0; mv < total; ++mv) {
-        // ---- molecule pick (runmc.f90:206, 232, 258) -------------------
         if (tid == 0) {
-            int imol;
-            if (mode == 1) {
-                imol = c.cimol[mv];
-                rng.cu = c.cu[mv];
-            } else {
-                if (mv % nmol == 0) (void)rng.next();  // pressure-step draw per cycle
-                (void)rng.next();                       // exchange draw per step
-                imol = yk_choose(c, rng.next());
-            }
-            YkMol &m = sh.m;
-            m.imol = imol;
-            m.first = c.first[imol];
-            m.na = c.last[imol] - m.first + 1;
+            yk_sec_pick(c, sh.m, t0.rng, mode, mv);
             sh.ov = 0;
             sh.nbcount = 0;
         }
@@ -1212,23 +1528,10 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
             sh.fr_old[a][0] = c.Fx[g]; sh.fr_old[a][1] = c.Fy[g]; sh.fr_old[a][2] = c.Fz[g];
         }
         __syncthreads();
-        if (tid == 0) {  // LJ types present in the molecule
-            YkMol &m = sh.m;
-            m.np = 0;
-            for (int a = 0; a < m.na; ++a) {
-                m.lslot[a] = -1;
-                if (m.typ[a] > 0) {
-                    int sl = -1;
-                    for (int q = 0; q < m.np; ++q)
-                        if (m.ptype[q] == m.typ[a]) sl = q;
-                    if (sl < 0 && m.np < YK_MAXP) { sl = m.np; m.ptype[m.np++] = m.typ[a]; }
-                    m.lslot[a] = sl;
-                }
-            }
-        }
+        if (tid == 0) yk_sec_slots(sh.m);
         YK_TICK(0);
         // ---- old axis tables --------------------------------------------
-        yk_mol_tables(c, sh.m, false);
+        yk_mol_tables(c, sh.m, false, tid);
         __syncthreads();
         YK_TICK(1);
         // ---- old k-space forces (MC.f90:946-982) ------------------------
@@ -1236,99 +1539,35 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
             double acc[YK_NACC];
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-            yk_sweep_n<F32, false>(c, sh.m, acc);
+            yk_sweep_n<F32, false>(c, sh.m, acc, tid, blockDim.x);
             yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
         }
         YK_TICK(2);
-        // ---- proposal on thread 0 (MC.f90:984-1067) ---------------------
-        if (tid == 0) {
-            YkMol &m = sh.m;
-            int na = m.na;
-            double fx[YD_MAXA], fy[YD_MAXA], fz[YD_MAXA];
-            U_old = 0.0;
-            for (int a = 0; a < na; ++a) U_old = U_old + sh.uu_old[a];
-            for (int a = 0; a < na; ++a) {
-                fx[a] = sh.fr_old[a][0] + YD_FOUR_PI * sh.out[YK_FK(a, 0)];
-                fy[a] = sh.fr_old[a][1] + YD_FOUR_PI * sh.out[YK_FK(a, 1)];
-                fz[a] = sh.fr_old[a][2] + YD_FOUR_PI * sh.out[YK_FK(a, 2)];
-                fx[a] = fx[a] + p.m2kext * m.q[a] * mu[0];
-                fy[a] = fy[a] + p.m2kext * m.q[a] * mu[1];
-                fz[a] = fz[a] + p.m2kext * m.q[a] * mu[2];
-            }
-            muo[0] = muo[1] = muo[2] = 0.0;
-            for (int a = 0; a < na; ++a) {
-                muo[0] = muo[0] + m.q[a] * m.xo[a];
-                muo[1] = muo[1] + m.q[a] * m.yo[a];
-                muo[2] = muo[2] + m.q[a] * m.zo[a];
-            }
-            double Fo[3] = {0, 0, 0}, To[3] = {0, 0, 0}, com[3] = {0, 0, 0}, tm = 0.0, tq[3];
-            for (int a = 0; a < na; ++a) { Fo[0] = Fo[0] + fx[a]; Fo[1] = Fo[1] + fy[a]; Fo[2] = Fo[2] + fz[a]; }
-            for (int a = 0; a < na; ++a) {
-                com[0] = com[0] + m.mass[a] * m.xo[a];
-                com[1] = com[1] + m.mass[a] * m.yo[a];
-                com[2] = com[2] + m.mass[a] * m.zo[a];
-                tm = tm + m.mass[a];
-            }
-            com[0] = com[0] / tm; com[1] = com[1] / tm; com[2] = com[2] / tm;
-            for (int a = 0; a < na; ++a) {
-                yk_prod(m.xo[a] - com[0], m.yo[a] - com[1], m.zo[a] - com[2], fx[a], fy[a], fz[a], tq);
-                To[0] = To[0] + tq[0]; To[1] = To[1] + tq[1]; To[2] = To[2] + tq[2];
-            }
-            for (int d = 0; d < 3; ++d) { mcm.fo[d] = Fo[d]; mcm.co[d] = To[d]; }
-            mcm.ftotold = sqrt(Fo[0] * Fo[0] + Fo[1] * Fo[1] + Fo[2] * Fo[2]);
-            mcm.couplold = sqrt(To[0] * To[0] + To[1] * To[1] + To[2] * To[2]);
-            mcm.stall = 0;
-            yk_displacement(p, mcm, rng);
-            yk_rotation(p, mcm, rng);
-            sh.stall = mcm.stall;
-            double cn[3];
-            for (int d = 0; d < 3; ++d) cn[d] = com[d] + mcm.d[d] - round(com[d] + mcm.d[d]);
-            if (na > 1) {
-                double cphi = mcm.cphi, sphi = mcm.sphi, st = mcm.st, ct = mcm.ct, ca = mcm.ca, sa = mcm.sa;
-                double ca1 = 1.0 - ca, R[9];
-                R[0] = (cphi * cphi * (st * st) * ca1 + ca);
-                R[1] = (sphi * cphi * (st * st) * ca1 - ct * sa);
-                R[2] = st * (ct * ca1 * cphi + sa * sphi);
-                R[3] = (cphi * sphi * (st * st) * ca1 + ct * sa);
-                R[4] = (sphi * sphi * (st * st) * ca1 + ca);
-                R[5] = st * (ct * ca1 * sphi - sa * cphi);
-                R[6] = st * (ct * ca1 * cphi - sa * sphi);
-                R[7] = st * (ct * ca1 * sphi + sa * cphi);
-                R[8] = (st * st * ca + ct * ct);
-                for (int a = 0; a < na; ++a) {
-                    double xr = m.xo[a] - com[0], yr = m.yo[a] - com[1], zr = m.zo[a] - com[2];
-                    m.xn[a] = (R[0] * xr + R[1] * yr + R[2] * zr) + cn[0];
-                    m.yn[a] = (R[3] * xr + R[4] * yr + R[5] * zr) + cn[1];
-                    m.zn[a] = (R[6] * xr + R[7] * yr + R[8] * zr) + cn[2];
-                }
-            } else {
-                m.xn[0] = cn[0]; m.yn[0] = cn[1]; m.zn[0] = cn[2];
-            }
+        // ---- proposal on wave 0 (MC.f90:984-1067) -----------------------
+        if (tid < 64) {
+            int stall = yk_sec_propose<true>(c, sh.m, t0, sh.out, sh.fr_old, sh.uu_old);
+            if (tid == 0) sh.stall = stall;
         }
         __syncthreads();
         YK_TICK(3);
         if (sh.stall) {  // proposal loop over its bound: rejected, no acceptance draw
             if (tid == 0) {
-                ++nmv;
-                if (c.rec) {
-                    yamc_move_rec r = {};
-                    r.imol = sh.m.imol; r.overlap = 2;
-                    c.rec[mv] = r;
-                }
+                ++t0.nmv;
+                yk_sec_reject_rec(c, sh.m, t0.mcm, mv, 2);
             }
             __syncthreads();
             continue;
         }
         // ---- new real-space partial sum (MC.f90:1077-1090) --------------
         {
-            double acc[YK_NRS];
+            double acc[YK_NRX];
 #pragma unroll
-            for (int i = 0; i < YK_NRS; ++i) acc[i] = 0.0;
+            for (int i = 0; i < YK_NRX; ++i) acc[i] = 0.0;
             int ov = 0;
-            yk_mol_tables(c, sh.m, true);
+            yk_mol_tables(c, sh.m, true, tid);
             const int nwords = (p.natom + 31) >> 5;
             if (nwords <= YK_BMWORDS) {
-                for (int i = tid; i < nwords; i += YK_BS) sh.bm[i] = 0u;
+                for (int i = tid; i < nwords; i += (int)blockDim.x) sh.bm[i] = 0u;
                 __syncthreads();
                 yk_rs_scan_n(c, sh.m, use_lds ? lds_pos : nullptr, sh.bm);
                 __syncthreads();
@@ -1338,34 +1577,36 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
             }
             __syncthreads();
             if (sh.nbcount <= YK_NBCAP) {
-                yk_rs_exact_n(c, sh.m, sig6, feps, sh.nblist, sh.nbcount, acc, ov);
+                yk_rs_exact_n(c, sh.m, sig6, feps, sh.nblist, sh.nbcount, acc, ov, tid, blockDim.x);
             } else {  // list overflow: full FP64 pass (the commit then visits every atom)
                 int junk = 0;
-                yk_rs_move_n(c, sh.m, sig6, feps, acc, ov, sh.nblist, &junk);
+                double a2[YK_NRS];
+#pragma unroll
+                for (int i = 0; i < YK_NRS; ++i) a2[i] = 0.0;
+                yk_rs_move_n(c, sh.m, sig6, feps, a2, ov, sh.nblist, &junk, tid, blockDim.x);
+#pragma unroll
+                for (int i = 0; i < YK_NRS; ++i) acc[i] = a2[i];
             }
             if (ov) sh.ov = 1;
-            yd_block_sum<YK_NRS>(acc, sh.red, sh.out);
+            yd_block_sum<YK_NRX>(acc, sh.red, sh.out);
             if (tid == 0) {
                 for (int a = 0; a < sh.m.na; ++a) {
-                    uun[a] = sh.out[a];
-                    frn[a][0] = sh.out[YD_MAXA + 3 * a + 0];
-                    frn[a][1] = sh.out[YD_MAXA + 3 * a + 1];
-                    frn[a][2] = sh.out[YD_MAXA + 3 * a + 2];
+                    t0.uun[a] = sh.out[a];
+                    t0.frn[a][0] = sh.out[YD_MAXA + 3 * a + 0];
+                    t0.frn[a][1] = sh.out[YD_MAXA + 3 * a + 1];
+                    t0.frn[a][2] = sh.out[YD_MAXA + 3 * a + 2];
                 }
-                for (int i = 0; i < 3; ++i) cmp_new[i] = sh.out[4 * YD_MAXA + i];
+                for (int i = 0; i < 3; ++i) {
+                    t0.cmp_new[i] = sh.out[4 * YD_MAXA + i];
+                    t0.cmp_old[i] = sh.out[YK_NRS + i];
+                }
             }
         }
         YK_TICK(4);
         if (sh.ov) {  // overlap: rejected, no acceptance draw (MC.f90:1080-1084)
             if (tid == 0) {
-                ++nmv; ++nov;
-                if (c.rec) {
-                    yamc_move_rec r = {};
-                    r.imol = sh.m.imol; r.overlap = 1;
-                    for (int d = 0; d < 3; ++d) { r.f_old[d] = mcm.fo[d]; r.t_old[d] = mcm.co[d]; r.delta[d] = mcm.d[d]; }
-                    r.angle = mcm.angle;
-                    c.rec[mv] = r;
-                }
+                ++t0.nmv; ++t0.nov;
+                yk_sec_reject_rec(c, sh.m, t0.mcm, mv, 1);
             }
             __syncthreads();
             continue;
@@ -1375,126 +1616,304 @@ __global__ void __launch_bounds__(YK_BS) yk_chain(const YdChain *chains, const d
             double acc[YK_NACC];
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-            yk_sweep_n<F32, true>(c, sh.m, acc);
+            yk_sweep_n<F32, true>(c, sh.m, acc, tid, blockDim.x);
             yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
         }
         YK_TICK(5);
-        // ---- acceptance on thread 0 (MC.f90:1176-1233) ------------------
-        if (tid == 0) {
-            YkMol &m = sh.m;
-            int na = m.na;
-            double U_new = 0.0;
-            for (int a = 0; a < na; ++a) U_new = U_new + uun[a];
-            du_real = U_new - U_old;
-            duc = sh.out[0];
-            dulj = sh.out[1];
-            double fx[YD_MAXA], fy[YD_MAXA], fz[YD_MAXA];
-            for (int a = 0; a < na; ++a) {
-                fx[a] = frn[a][0] + YD_FOUR_PI * sh.out[YK_FK(a, 0)];
-                fy[a] = frn[a][1] + YD_FOUR_PI * sh.out[YK_FK(a, 1)];
-                fz[a] = frn[a][2] + YD_FOUR_PI * sh.out[YK_FK(a, 2)];
-            }
-            double mun[3] = {0, 0, 0};
-            for (int a = 0; a < na; ++a) {
-                mun[0] = mun[0] + m.q[a] * m.xn[a];
-                mun[1] = mun[1] + m.q[a] * m.yn[a];
-                mun[2] = mun[2] + m.q[a] * m.zn[a];
-            }
-            double dmu[3] = {mun[0] - muo[0], mun[1] - muo[1], mun[2] - muo[2]};
-            du_ext = p.kext * ((2 * mu[0] + dmu[0]) * dmu[0] + (2 * mu[1] + dmu[1]) * dmu[1] +
-                               (2 * mu[2] + dmu[2]) * dmu[2]);
-            for (int d = 0; d < 3; ++d) mu_new[d] = mu[d] + dmu[d];
-            for (int a = 0; a < na; ++a) {
-                fx[a] = fx[a] + p.m2kext * m.q[a] * mu_new[0];
-                fy[a] = fy[a] + p.m2kext * m.q[a] * mu_new[1];
-                fz[a] = fz[a] + p.m2kext * m.q[a] * mu_new[2];
-            }
-            double Fn[3] = {0, 0, 0}, Tn[3] = {0, 0, 0}, com[3] = {0, 0, 0}, tm = 0.0, tq[3];
-            for (int a = 0; a < na; ++a) { Fn[0] = Fn[0] + fx[a]; Fn[1] = Fn[1] + fy[a]; Fn[2] = Fn[2] + fz[a]; }
-            for (int a = 0; a < na; ++a) {
-                com[0] = com[0] + m.mass[a] * m.xn[a];
-                com[1] = com[1] + m.mass[a] * m.yn[a];
-                com[2] = com[2] + m.mass[a] * m.zn[a];
-                tm = tm + m.mass[a];
-            }
-            com[0] = com[0] / tm; com[1] = com[1] / tm; com[2] = com[2] / tm;
-            for (int a = 0; a < na; ++a) {
-                yk_prod(m.xn[a] - com[0], m.yn[a] - com[1], m.zn[a] - com[2], fx[a], fy[a], fz[a], tq);
-                Tn[0] = Tn[0] + tq[0]; Tn[1] = Tn[1] + tq[1]; Tn[2] = Tn[2] + tq[2];
-            }
-            for (int d = 0; d < 3; ++d) { mcm.fn[d] = Fn[d]; mcm.cn[d] = Tn[d]; }
-            double du_total = du_real + (duc + dulj) + du_ext;
-            yk_init_new(p, mcm);
-            int accd = yk_accept(p, mcm, du_total, rng);
-            sh.accepted = accd;
-            ++nmv;
-            acc_count += accd;
-            if (c.rec) {
-                yamc_move_rec r = {};
-                r.imol = m.imol; r.accepted = accd; r.overlap = 0;
-                r.du_real = du_real; r.du_kc = duc; r.du_klj = dulj; r.du_ext = du_ext; r.du_total = du_total;
-                for (int d = 0; d < 3; ++d) {
-                    r.f_old[d] = mcm.fo[d]; r.t_old[d] = mcm.co[d]; r.f_new[d] = Fn[d]; r.t_new[d] = Tn[d];
-                    r.delta[d] = mcm.d[d];
-                }
-                r.angle = mcm.angle;
-                c.rec[mv] = r;
+        // ---- acceptance on wave 0 (MC.f90:1176-1233) --------------------
+        if (tid < 64) {
+            int accd = yk_sec_accept<true>(c, sh.m, t0, sh.out, mv);
+            if (tid == 0) {
+                sh.accepted = accd;
+                ++t0.nmv;
+                t0.acc_count += accd;
             }
         }
         __syncthreads();
         YK_TICK(6);
         if (!sh.accepted) continue;
         // ---- commit (MC.f90:1242-1318) ----------------------------------
-        yk_commit<F32>(c, sh.m);
-        {
+        yk_commit<F32>(c, sh.m, tid, blockDim.x);
+        if (sh.nbcount <= YK_NBCAP) {
+            yk_rs_commit_list(c, sh.nblist, sh.nbcount, tid, blockDim.x);
+        } else {
             double acc[3] = {0, 0, 0};
-            yk_rs_commit(c, sh.m, sig6, feps, sh.nblist, sh.nbcount, acc);
+            yk_rs_commit_all(c, sh.m, sig6, feps, acc, tid, blockDim.x);
             yd_block_sum<3>(acc, sh.red, sh.out);
+            if (tid == 0)
+                for (int i = 0; i < 3; ++i) t0.cmp_old[i] = sh.out[i];
         }
+        __syncthreads();
         YK_TICK(7);
         if (tid == 0) {
-            YkMol &m = sh.m;
-            int f = m.first;
-            for (int a = 0; a < m.na; ++a) {
-                c.uu[f + a] = uun[a];
-                c.Fx[f + a] = frn[a][0];
-                c.Fy[f + a] = frn[a][1];
-                c.Fz[f + a] = frn[a][2];
-                c.x[f + a] = m.xn[a];
-                c.y[f + a] = m.yn[a];
-                c.z[f + a] = m.zn[a];
-                if (use_lds)
-                    lds_pos[f + a] = make_float4((float)m.xn[a], (float)m.yn[a], (float)m.zn[a],
-                                                 __int_as_float(m.imol + 1));
-            }
-            YdState *st = c.st;
-            st->e_kc = st->e_kc + duc;
-            st->e_klj = st->e_klj + dulj;
-            st->uu_ew = st->uu_ew + cmp_new[0] - sh.out[0];
-            st->uu_lj6 = st->uu_lj6 + cmp_new[1] - sh.out[1];
-            st->uu_lj12 = st->uu_lj12 + cmp_new[2] - sh.out[2];
-            for (int d = 0; d < 3; ++d) {
-                st->mu[d] = mu_new[d];
-                mu[d] = mu_new[d];
-            }
+            yk_sec_finalize(c, sh.m, t0, use_lds ? lds_pos : nullptr);
+            for (int d = 0; d < 3; ++d) t0.mu[d] = t0.mu_new[d];
         }
         __syncthreads();
         YK_TICK(8);
     }
     if (tid == 0) {
-        if (mode != 1) c.st->ctr = rng.ctr;
-        c.st->n_moves += nmv;
-        c.st->n_accepted += acc_count;
-        c.st->n_overlap += nov;
-        c.st->since_resync += nmv;
+        if (mode != 1) c.st->ctr = t0.rng.ctr;
+        c.st->n_moves += t0.nmv;
+        c.st->n_accepted += t0.acc_count;
+        c.st->n_overlap += t0.nov;
+        c.st->since_resync += t0.nmv;
         for (int i = 0; i < 9; ++i) c.st->phase[i] += ph[i];
     }
 }
 
-template __global__ void yk_chain<false>(const YdChain *, const double *const *, const double *const *, int, int,
-                                         int, int);
-template __global__ void yk_chain<true>(const YdChain *, const double *const *, const double *const *, int, int,
-                                        int, int);
+#define YK_CHAIN_INST(F, B)                                                                                    \
+    template __global__ void yk_chain<F, B>(const YdChain *, const double *const *, const double *const *, int, int, \
+                                            int, int);
+YK_CHAIN_INST(false, 256)
+YK_CHAIN_INST(false, 512)
+YK_CHAIN_INST(true, 256)
+YK_CHAIN_INST(true, 512)
+
+// ========================================================================
+// The wave chain kernel (batched replicas): one wavefront per replica,
+// YW_WPB replicas per workgroup, no workgroup barriers.  Each wave runs
+// run_mc cycles of its own chain exactly as the workgroup kernel does; its
+// sums are wave-shuffle reductions, its real-space candidate list is built
+// in ascending atom order from wave ballots, and several replicas share
+// every SIMD so that one replica's latency is hidden by the others'.
+#define YW_WPB 4
+
+// Per-replica LDS of the wave kernel.
+struct YwShared {
+    YkMol m;
+    YkScalar t0;
+    double out[YK_NRX > YK_NACC ? YK_NRX : YK_NACC];
+    double fr_old[YD_MAXA][3], uu_old[YD_MAXA];
+};
+
+// Orders the wave's LDS and global accesses before and after the call for
+// every lane of the wave (the lanes of one wave run in lockstep; this keeps
+// the compiler from moving accesses across and waits for them to land).
+__device__ __forceinline__ void yw_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Sum of NV doubles over the wave; every lane gets the same sums (each
+// butterfly step adds the same two operands in both partner lanes).
+template <int NV>
+__device__ __forceinline__ void yw_sum(double (&v)[NV]) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double a = v[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        v[i] = a;
+    }
+}
+
+// Candidate scan of the wave kernel: atoms of other molecules within the
+// cut-off (plus the FP32 margin) of an old or a new site, appended in
+// ascending order through ballots.  Returns the count (the list holds the
+// first YK_NBCAP).
+template <int NA>
+__device__ int yw_scan(const YdChain &c, const YkMol &m, int lane) {
+    const int N = c.p.natom, me = m.imol + 1;
+    const float r2f = (float)c.p.rmax2 + 1e-5f;
+    float xof[NA], yof[NA], zof[NA], xnf[NA], ynf[NA], znf[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        xof[a] = (float)m.xo[a]; yof[a] = (float)m.yo[a]; zof[a] = (float)m.zo[a];
+        xnf[a] = (float)m.xn[a]; ynf[a] = (float)m.yn[a]; znf[a] = (float)m.zn[a];
+    }
+    int cnt = 0;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int base = 0; base < N; base += 64) {
+        int j = base + lane;
+        bool near = false;
+        if (j < N) {
+            float4 pj = c.pos4[j];
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                float dx = xnf[a] - pj.x, dy = ynf[a] - pj.y, dz = znf[a] - pj.z;
+                dx = dx > 0.5f ? dx - 1.0f : dx; dx = dx < -0.5f ? dx + 1.0f : dx;
+                dy = dy > 0.5f ? dy - 1.0f : dy; dy = dy < -0.5f ? dy + 1.0f : dy;
+                dz = dz > 0.5f ? dz - 1.0f : dz; dz = dz < -0.5f ? dz + 1.0f : dz;
+                near |= dx * dx + dy * dy + dz * dz < r2f;
+                float ex = xof[a] - pj.x, ey = yof[a] - pj.y, ez = zof[a] - pj.z;
+                ex = ex > 0.5f ? ex - 1.0f : ex; ex = ex < -0.5f ? ex + 1.0f : ex;
+                ey = ey > 0.5f ? ey - 1.0f : ey; ey = ey < -0.5f ? ey + 1.0f : ey;
+                ez = ez > 0.5f ? ez - 1.0f : ez; ez = ez < -0.5f ? ez + 1.0f : ez;
+                near |= ex * ex + ey * ey + ez * ez < r2f;
+            }
+            near = near && __float_as_int(pj.w) != me;
+        }
+        unsigned long long mask = __ballot(near);
+        if (near) {
+            int pos = cnt + __popcll(mask & below);
+            if (pos < YK_NBCAP) c.nbl[pos] = j;
+        }
+        cnt += __popcll(mask);
+    }
+    return cnt;
+}
+
+__device__ __forceinline__ int yw_scan_n(const YdChain &c, const YkMol &m, int lane) {
+    switch (m.na) {
+        case 1: return yw_scan<1>(c, m, lane);
+        case 2: return yw_scan<2>(c, m, lane);
+        case 3: return yw_scan<3>(c, m, lane);
+        default: return yw_scan<4>(c, m, lane);
+    }
+}
+
+template <bool F32>
+__global__ void __launch_bounds__(YW_WPB * 64) yk_wchain(const YdChain *chains, int nchains,
+                                                         const double *const *sig6s, const double *const *feps_s,
+                                                         int ncycles) {
+    __shared__ YwShared shw[YW_WPB];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ci = blockIdx.x * YW_WPB + wv;
+    if (ci >= nchains) return;  // the whole wave leaves; nothing waits for it
+    YwShared &sh = shw[wv];
+    const YdChain &c = chains[ci];
+    const double *sig6 = sig6s[ci], *feps = feps_s[ci];
+    const YdParams &p = c.p;
+    YkScalar &t0 = sh.t0;
+    YkMol &m = sh.m;
+    // FP32 copy of the positions for the scan, and mu from scratch
+    // (EwaldSumExternal.f90:165-183), carried incrementally afterwards.
+    {
+        double v[3] = {0, 0, 0};
+        for (int i = lane; i < p.natom; i += 64) {
+            double x = c.x[i], y = c.y[i], z = c.z[i], q = c.q[i];
+            c.pos4[i] = make_float4((float)x, (float)y, (float)z, __int_as_float(c.molnum[i]));
+            v[0] += q * x; v[1] += q * y; v[2] += q * z;
+        }
+        yw_sum<3>(v);
+        t0.mu[0] = v[0]; t0.mu[1] = v[1]; t0.mu[2] = v[2];
+        t0.acc_count = 0; t0.nmv = 0; t0.nov = 0;
+        YdRng &rng = t0.rng;
+        rng.key0 = c.st->key[0]; rng.key1 = c.st->key[1];
+        rng.s0 = c.st->stream[0]; rng.s1 = c.st->stream[1];
+        rng.ctr = c.st->ctr; rng.constant = 0; rng.cu = 0.0;
+    }
+    yw_sync();
+    const int total = ncycles * p.nmol;
+    for (int mv = 0; mv < total; ++mv) {
+        yk_sec_pick(c, m, t0.rng, 0, mv);
+        yw_sync();
+        if (lane < m.na) {
+            int a = lane, g = m.first + a;
+            m.q[a] = c.q[g]; m.typ[a] = c.type[g]; m.mass[a] = c.mass[g]; m.dh[a] = c.dh[g];
+            m.xo[a] = c.x[g]; m.yo[a] = c.y[g]; m.zo[a] = c.z[g];
+            sh.uu_old[a] = c.uu[g];
+            sh.fr_old[a][0] = c.Fx[g]; sh.fr_old[a][1] = c.Fy[g]; sh.fr_old[a][2] = c.Fz[g];
+        }
+        yw_sync();
+        yk_sec_slots(m);
+        yk_mol_tables(c, m, false, lane);
+        yw_sync();
+        // ---- old k-space forces (MC.f90:946-982) ------------------------
+        {
+            double acc[YK_NACC];
+#pragma unroll
+            for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
+            yk_sweep_n<F32, false>(c, m, acc, lane, 64);
+            yw_sum<YK_NACC>(acc);
+#pragma unroll
+            for (int i = 0; i < YK_NACC; ++i) sh.out[i] = acc[i];
+        }
+        yw_sync();
+        // ---- proposal (MC.f90:984-1067) ---------------------------------
+        int stall = yk_sec_propose<true>(c, m, t0, sh.out, sh.fr_old, sh.uu_old);
+        yw_sync();
+        if (stall) {
+            ++t0.nmv;
+            if (lane == 0) yk_sec_reject_rec(c, m, t0.mcm, mv, 2);
+            yw_sync();
+            continue;
+        }
+        // ---- new real-space partial sum (MC.f90:1077-1090) --------------
+        yk_mol_tables(c, m, true, lane);
+        int nb = yw_scan_n(c, m, lane);
+        yw_sync();
+        int ov = 0;
+        {
+            double acc[YK_NRX];
+#pragma unroll
+            for (int i = 0; i < YK_NRX; ++i) acc[i] = 0.0;
+            if (nb <= YK_NBCAP) {
+                yk_rs_exact_n(c, m, sig6, feps, c.nbl, nb, acc, ov, lane, 64);
+            } else {
+                int junk = 0;
+                double a2[YK_NRS];
+#pragma unroll
+                for (int i = 0; i < YK_NRS; ++i) a2[i] = 0.0;
+                yk_rs_move_n(c, m, sig6, feps, a2, ov, c.nbl, &junk, lane, 64);
+#pragma unroll
+                for (int i = 0; i < YK_NRS; ++i) acc[i] = a2[i];
+            }
+            yw_sum<YK_NRX>(acc);
+            for (int a = 0; a < m.na; ++a) {
+                t0.uun[a] = acc[a];
+                t0.frn[a][0] = acc[YD_MAXA + 3 * a + 0];
+                t0.frn[a][1] = acc[YD_MAXA + 3 * a + 1];
+                t0.frn[a][2] = acc[YD_MAXA + 3 * a + 2];
+            }
+            for (int i = 0; i < 3; ++i) {
+                t0.cmp_new[i] = acc[4 * YD_MAXA + i];
+                t0.cmp_old[i] = acc[YK_NRS + i];
+            }
+        }
+        ov = __any(ov);
+        yw_sync();
+        if (ov) {  // overlap: rejected, no acceptance draw (MC.f90:1080-1084)
+            ++t0.nmv; ++t0.nov;
+            if (lane == 0) yk_sec_reject_rec(c, m, t0.mcm, mv, 1);
+            yw_sync();
+            continue;
+        }
+        // ---- new k-space sweep: dU and new forces (MC.f90:1096-1173) ----
+        {
+            double acc[YK_NACC];
+#pragma unroll
+            for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
+            yk_sweep_n<F32, true>(c, m, acc, lane, 64);
+            yw_sum<YK_NACC>(acc);
+#pragma unroll
+            for (int i = 0; i < YK_NACC; ++i) sh.out[i] = acc[i];
+        }
+        yw_sync();
+        // ---- acceptance (MC.f90:1176-1233) ------------------------------
+        int accd = yk_sec_accept<true>(c, m, t0, sh.out, mv);
+        ++t0.nmv;
+        t0.acc_count += accd;
+        yw_sync();
+        if (!accd) continue;
+        // ---- commit (MC.f90:1242-1318) ----------------------------------
+        yk_commit<F32>(c, m, lane, 64);
+        if (nb <= YK_NBCAP) {
+            yk_rs_commit_list(c, c.nbl, nb, lane, 64);
+        } else {
+            double acc[3] = {0, 0, 0};
+            yk_rs_commit_all(c, m, sig6, feps, acc, lane, 64);
+            yw_sum<3>(acc);
+            for (int i = 0; i < 3; ++i) t0.cmp_old[i] = acc[i];
+        }
+        yw_sync();
+        if (lane == 0) yk_sec_finalize(c, m, t0, c.pos4);
+        for (int d = 0; d < 3; ++d) t0.mu[d] = t0.mu_new[d];
+        yw_sync();
+    }
+    if (lane == 0) {
+        c.st->ctr = t0.rng.ctr;
+        c.st->n_moves += t0.nmv;
+        c.st->n_accepted += t0.acc_count;
+        c.st->n_overlap += t0.nov;
+        c.st->since_resync += t0.nmv;
+    }
+}
+
+template __global__ void yk_wchain<false>(const YdChain *, int, const double *const *, const double *const *, int);
+template __global__ void yk_wchain<true>(const YdChain *, int, const double *const *, const double *const *, int);
 
 // ========================================================================
 // Single-call entry points (one workgroup): the per-function parity surface.
@@ -1509,13 +1928,13 @@ __global__ void __launch_bounds__(YK_BS) yk_kspace_delta(YdChain c, int imol, co
         for (int a = 0; a < sh.m.na; ++a) { sh.m.xn[a] = x[a]; sh.m.yn[a] = y[a]; sh.m.zn[a] = z[a]; }
     }
     __syncthreads();
-    yk_mol_tables(c, sh.m, false);
+    yk_mol_tables(c, sh.m, false, threadIdx.x);
     __syncthreads();
-    yk_mol_tables(c, sh.m, true);
+    yk_mol_tables(c, sh.m, true, threadIdx.x);
     __syncthreads();
     double acc[YK_NACC];
     for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-    yk_sweep_n<F32, true>(c, sh.m, acc);
+    yk_sweep_n<F32, true>(c, sh.m, acc, threadIdx.x, blockDim.x);
     yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
     if (threadIdx.x == 0) { out[0] = sh.out[0]; out[1] = sh.out[1]; }
 }
@@ -1531,21 +1950,20 @@ __global__ void __launch_bounds__(YK_BS) yk_kspace_forces(YdChain c, int first, 
     __shared__ YkShared sh;
     if (threadIdx.x == 0) {
         YkMol &m = sh.m;
-        m.imol = -1; m.first = first; m.na = n; m.np = 0;
+        m.imol = -1; m.first = first; m.na = n;
         for (int a = 0; a < n; ++a) {
             m.q[a] = c.q[first + a];
             m.typ[a] = c.type[first + a];
             m.xo[a] = x[a]; m.yo[a] = y[a]; m.zo[a] = z[a];
-            m.lslot[a] = -1;
-            if (m.typ[a] > 0 && m.np == 0) { m.ptype[0] = m.typ[a]; m.np = 1; }
         }
+        yk_sec_slots(m);  // atoms beyond YK_MAXP distinct LJ types keep slot -1
     }
     __syncthreads();
-    yk_mol_tables(c, sh.m, false);
+    yk_mol_tables(c, sh.m, false, threadIdx.x);
     __syncthreads();
     double acc[YK_NACC];
     for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-    yk_sweep_n<F32, false>(c, sh.m, acc);
+    yk_sweep_n<F32, false>(c, sh.m, acc, threadIdx.x, blockDim.x);
     yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
     if (threadIdx.x < 3 * n) {
         int a = threadIdx.x / 3, d = threadIdx.x % 3;
@@ -1581,7 +1999,7 @@ __global__ void __launch_bounds__(YK_BS) yk_rspace_partial(YdChain c, int imol, 
     if (uub) {
         const int N = c.p.natom, T = c.p.ntype, me = imol + 1;
         const YkMol &m = sh.m;
-        for (int j = threadIdx.x; j < N; j += YK_BS) {
+        for (int j = threadIdx.x; j < N; j += (int)blockDim.x) {
             double b[4] = {0, 0, 0, 0};
             if (c.molnum[j] != me) {
                 int jt = c.type[j];

@@ -230,6 +230,10 @@ class EwaldMC:
         check(self.lib.yamc_phase_times(self.ctx, dptr(us)))
         return dict(zip(self.PHASES, us))
 
+    def set_block_size(self, block: int) -> None:
+        """Workgroup size of the chain kernel (256 or 512)."""
+        check(self.lib.yamc_set_block_size(self.ctx, int(block)))
+
     def last_kernel_ms(self) -> float:
         v = C.c_double()
         check(self.lib.yamc_last_kernel_ms(self.ctx, C.byref(v)))
