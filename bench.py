@@ -11,11 +11,12 @@ the chain kernel).
 Forms timed:
   fp64      plain FP64, one chain (the reference algorithm on one replica)
   fp32c     FP32-compensated S(k), one chain
-  batch     FP32-compensated, R independent replicas of the same system in
-            one persistent launch (one workgroup per replica) -> `value`
-The per-move dE error of fp32c against fp64 is measured on one stream
-(moves up to the first differing decision).  The CPU baseline is the FP64
-oracle (a plain-C restatement of the reference) timed on this host.
+  batch     FP32-compensated, R independent replicas of the same system (own
+            Philox streams) advanced by one captured launch per sweep, launch
+            shape chosen by yamc_amd.tune -> `value`
+The per-move dE and force errors of fp32c against fp64 are measured on one
+stream (moves up to the first differing decision).  The CPU baseline is the
+FP64 oracle (a plain-C restatement of the reference) timed on this host.
 
 Prints one JSON line.
 """
@@ -23,7 +24,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -32,19 +32,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KT_KJMOL_298 = None  # filled from the reference constants below
 
 
-def algorithmic_bytes_per_move(sysm, nk_info, w):
-    """k-space bytes per trial move: w * [2 (1+T) n_s + (1 + T |t_M|) nk]
+def algorithmic_bytes_per_move(sysm, nk_info, w, p_acc):
+    """k-space bytes per trial move:
+        w [2 (1+T) n_s + (1 + T |t_M|) nk] + 2 w' 2 (1 + |t_M|) n_s P_acc
     (the S(k) cos/sin of the Coulomb set and all T LJ sets over the nonzero
-    sign slots, plus beta_C and the beta_LJ rows of the moved molecule's type;
-    |t_M| = 1 for every species here)."""
+    sign slots, plus beta_C and the beta_LJ rows of the moved molecule's type,
+    plus the read-modify-write of the touched sets on acceptance; w = 8 (FP64)
+    or 4 (FP32 hi read), w' = 8 (FP64 or the FP32 hi+lo pair); |t_M| = 1 for
+    every species here)."""
     ky, kz = nk_info["ky"], nk_info["kz"]
     n_s = int(sum(1 + (a > 0) + (b > 0) + ((a > 0) and (b > 0)) for a, b in zip(ky, kz)))
     T = sysm.ntype
     nk = len(ky)
-    return w * (2 * (1 + T) * n_s + (1 + T * 1) * nk), n_s
+    sweep = w * (2 * (1 + T) * n_s + (1 + T * 1) * nk)
+    commit = 2 * 8 * 2 * (1 + 1) * n_s * p_acc
+    return sweep + commit, n_s
 
 
 def main():
@@ -52,8 +56,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--replicas", type=int, default=1024)
     ap.add_argument("--workload", default="c2")
+    ap.add_argument("--block", type=int, default=0, help="batch launch shape (64, 256, 512); 0 = autotune")
     ap.add_argument("--cpu-moves", type=int, default=0, help="moves for the CPU baseline (0 = auto ~15 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32c,batch")
@@ -81,7 +86,8 @@ def main():
         return
 
     from yamc_amd import YAMC_FP32C, YAMC_FP64, config, system
-    from yamc_amd.ewald import EwaldMC, batch_run
+    from yamc_amd.ewald import Batch, EwaldMC
+    from yamc_amd.tune import tune_batch
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     kT_kjmol = system.BOLTZ * 298.0 / system.KCAL_MOL * 4.184
@@ -95,11 +101,11 @@ def main():
         torch.cuda.synchronize()
 
     # ---------------------------------------------------------- single chains
-    singles = {}
     for form, prec in (("fp64", YAMC_FP64), ("fp32c", YAMC_FP32C)):
-        if form not in forms and not (form == "fp64" and "err" in forms):
+        if form not in forms:
             continue
         g = EwaldMC(base, precision=prec)
+        g.set_block_size(256)
         g.mc_init()
         g.set_rng(20261203, 0)
         for _ in range(args.warmup):
@@ -115,11 +121,11 @@ def main():
         sync()
         dt = time.perf_counter() - t0
         moves = args.steps * nmol
-        singles[form] = g
         res[form] = dict(moves_per_s=moves / dt, ms_per_step=1e3 * dt / args.steps, kernel_ms=kms,
-                         acceptance=nacc / moves)
+                         acceptance=nacc / moves, launch_shape=256)
+        g.close()
 
-    # ------------------------------------------------------- dE error, fp32c
+    # ------------------------------------------------- fp32c error vs fp64
     err = None
     if "fp64" in forms and "fp32c" in forms:
         a, b = EwaldMC(base, precision=YAMC_FP64), EwaldMC(base, precision=YAMC_FP32C)
@@ -129,56 +135,79 @@ def main():
         _, rb = b.run(1, records=True)
         same = ra["accepted"] == rb["accepted"]
         n = int(np.argmin(same)) if not same.all() else len(ra)
-        ok = (ra["overlap"][:n] == 0)
+        ok = ra["overlap"][:n] == 0
         d = np.abs(ra["du_total"][:n] - rb["du_total"][:n])[ok]
-        fo = np.abs(ra["f_old"][:n] - rb["f_old"][:n])[ok] / np.maximum(np.abs(ra["f_old"][:n])[ok], 1e-30)
+        fa = np.concatenate([ra["f_old"][:n][ok], ra["f_new"][:n][ok]])
+        fb = np.concatenate([rb["f_old"][:n][ok], rb["f_new"][:n][ok]])
+        f_norm = np.linalg.norm(fa - fb, axis=1) / np.maximum(np.linalg.norm(fa, axis=1), 1e-300)
+        f_comp = np.abs(fa - fb) / np.maximum(np.abs(fa), 1e-300)
         err = dict(moves_compared=int(n), max_dE_kT=float(d.max()) if d.size else 0.0,
                    max_dE_kJmol=float(d.max() * kT_kjmol) if d.size else 0.0,
                    rms_dE_kT=float(np.sqrt((d ** 2).mean())) if d.size else 0.0,
-                   max_rel_force_err=float(fo.max()) if fo.size else 0.0,
+                   max_rel_force_err_norm=float(f_norm.max()) if f_norm.size else 0.0,
+                   max_rel_force_err_component=float(f_comp.max()) if f_comp.size else 0.0,
                    decision_agreement_first_sweep=float(same.mean()))
+        a.close(); b.close()
 
     # ------------------------------------------------------ batched replicas
+    grid = None
+    tuned = None
     if "batch" in forms:
         R = args.replicas
         reps = []
         for i in range(R):
-            s = config(args.workload, seed=20261203 + i)
-            r = EwaldMC(s, precision=YAMC_FP32C)
+            r = EwaldMC(base, precision=YAMC_FP32C)
             r.mc_init()
             r.set_rng(20261203, i)
             reps.append(r)
+        grid = reps[0].grid()
+        if args.block:
+            for r in reps:
+                r.set_block_size(args.block)
+            shape, tuned = args.block, None
+        else:
+            shape, tuned = tune_batch(reps)  # three ordinary sweeps, one per launch shape
+        bat = Batch(reps, 1)
         for _ in range(args.warmup):
-            batch_run(reps, 1)
+            bat.launch()
         sync()
         t0 = time.perf_counter()
         kms = 0.0
         nacc = 0
         for _ in range(args.steps):
-            acc, ms = batch_run(reps, 1)
+            acc, ms = bat.launch()
             kms += ms
             nacc += int(acc.sum())
         sync()
         dt = time.perf_counter() - t0
         moves = args.steps * nmol * R
         res["batch"] = dict(replicas=R, moves_per_s=moves / dt, ms_per_step=1e3 * dt / args.steps, kernel_ms=kms,
-                            acceptance=nacc / moves)
+                            acceptance=nacc / moves, launch_shape=shape,
+                            tuner_moves_per_s={str(k): v for k, v in tuned.items()} if tuned else None)
+        bat.close()
+        for r in reps:
+            r.close()
 
     # ------------------------------------------------------------ roofline
-    g0 = next(iter(singles.values())) if singles else reps[0]
-    grid = g0.grid()
+    if grid is None:
+        g0 = EwaldMC(base)
+        grid = g0.grid()
+        g0.close()
     main_form = "batch" if "batch" in res else ("fp32c" if "fp32c" in res else "fp64")
     w = 4 if main_form in ("batch", "fp32c") else 8
-    bpm, n_s = algorithmic_bytes_per_move(base, grid, w)
     r0 = res[main_form]
-    moves_main = args.steps * nmol * (r0.get("replicas", 1))
+    bpm, n_s = algorithmic_bytes_per_move(base, grid, w, r0["acceptance"])
+    moves_per_launch = nmol * r0.get("replicas", 1)
     launch_s = r0["kernel_ms"] / 1e3 / args.steps
-    achieved = bpm * moves_main / args.steps / launch_s / 1e9
+    achieved = bpm * moves_per_launch / launch_s / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_c2.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get(main_form)
+            t = json.load(open(tpath))
+            key = f"{main_form}_{r0.get('launch_shape')}_{r0.get('replicas', 1)}"
+            if key in t:  # HBM bytes per launch from the PMC runs of this exact launch
+                traffic = t[key]["bytes_per_launch"]
         except Exception:
             traffic = None
 
@@ -222,13 +251,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 S(k) sweep + fp64 accept" if main_form != "fp64" else "f64",
+        "dtype": "f64 arithmetic, f32 hi/lo S(k) storage" if main_form != "fp64" else "f64",
         "data": "synthetic: seeded lattice start, seeded orientations",
         "config": {
             "workload": f"{args.workload}: 2048 SPC/E + 40 Na + 40 Cl, L=39.4 A, rc=12 A, kmax=12, "
                         f"force-biased moves; step = 1 sweep (2128 moves) per chain",
             "form": main_form,
             "replicas": res[main_form].get("replicas", 1),
+            "launch_shape": res[main_form].get("launch_shape"),
             "nk": len(grid["kx"]),
             "n_s": n_s,
         },
@@ -242,7 +272,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "bytes_per_move": bpm,
-            "kernel": "yk_chain",
+            "kernel": "yk_wchain" if res[main_form].get("launch_shape") == 64 else "yk_chain",
         },
         "cpu_baseline": cpu,
     }

@@ -37,8 +37,9 @@ enum {
 /* Arithmetic of the state sums S(k) and the per-move k-space sweep. */
 enum {
     YAMC_FP64 = 0,  /* plain FP64 everywhere */
-    YAMC_FP32C = 1  /* S(k) kept as FP32 (hi, lo) two-sum pairs, swept in FP32,
-                       FP64 acceptance / constants / periodic resync */
+    YAMC_FP32C = 1  /* S(k) kept as FP32 (hi, lo) two-sum pairs; sweeps read hi
+                       (LJ sets) or hi + lo (Coulomb set); FP64 arithmetic,
+                       acceptance, constants and periodic resync */
 };
 
 /* Limits of this build. */
@@ -182,10 +183,21 @@ int yamc_mc_run_const(yamc_ctx *ctx, int32_t nmoves, const int32_t *imol, const 
 int yamc_resync(yamc_ctx *ctx);
 
 /* ---- batched replicas -------------------------------------------------- */
-/* Advances n independent contexts (uneven sizes allowed) by ncycles each in
- * one persistent launch, one workgroup per replica.  All contexts must live
- * on the same device. */
+/* Advances n independent contexts (uneven sizes, compositions and
+ * temperatures allowed) by ncycles run_mc cycles each (runmc.f90:204-268)
+ * in one persistent launch; the launch shape is the first context's
+ * (yamc_set_block_size).  All contexts must live on the same device.
+ * n_accepted[n] (may be NULL) receives each replica's accepted moves. */
 int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int64_t *n_accepted);
+/* The same sweep captured once in a HIP graph: yamc_batch_create stages the
+ * replicas and captures one launch of ncycles cycles, yamc_batch_launch
+ * replays it (each replay continues every replica's chain and RNG stream),
+ * yamc_batch_destroy frees the graph.  The contexts must outlive the batch;
+ * no reference counterpart (the reference runs one system serially). */
+typedef struct yamc_batch yamc_batch;
+int yamc_batch_create(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, yamc_batch **out);
+int yamc_batch_launch(yamc_batch *batch, int64_t *n_accepted);
+int yamc_batch_destroy(yamc_batch *batch);
 
 /* ---- forces ------------------------------------------------------------ */
 /* Full-system forces.  f_grad = -dE_total/dr for all atoms (real + reciprocal

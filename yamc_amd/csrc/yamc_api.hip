@@ -717,12 +717,23 @@ extern "C" int yamc_mc_run_const(yamc_ctx *c, int32_t nmoves, const int32_t *imo
     return YAMC_OK;
 }
 
-extern "C" int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int64_t *n_accepted) {
+// Staged batch: the chain descriptors of n contexts on the device.
+struct yamc_batch {
+    yamc_ctx *c0 = nullptr;
+    int n = 0, ncycles = 0, max_natom = 0;
+    std::vector<yamc_ctx *> ctxs;
+    YdChain *d_ch = nullptr;
+    const double **d_s6 = nullptr, **d_fe = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static int batch_stage(yamc_ctx *const *ctxs, int32_t n, yamc_batch *b) {
     if (!ctxs || n < 1) return fail(YAMC_ERROR_PARAMETER, "empty batch");
     yamc_ctx *c0 = ctxs[0];
+    if (!c0) return fail(YAMC_ERROR_PARAMETER, "null context in batch");
     std::vector<YdChain> chains(n);
     std::vector<const double *> s6(n), fe(n);
-    std::vector<long long> acc0(n);
     for (int i = 0; i < n; ++i) {
         yamc_ctx *c = ctxs[i];
         if (!c) return fail(YAMC_ERROR_PARAMETER, "null context in batch");
@@ -733,36 +744,119 @@ extern "C" int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles,
         chains[i] = c->ch;
         s6[i] = c->d_sig6;
         fe[i] = c->d_4eps;
-        YdState st;
-        HIPCHK(hipMemcpy(&st, c->ch.st, sizeof st, hipMemcpyDeviceToHost));
-        acc0[i] = st.n_accepted;
+        b->max_natom = c->natom > b->max_natom ? c->natom : b->max_natom;
     }
+    b->c0 = c0;
+    b->n = n;
+    b->ctxs.assign(ctxs, ctxs + n);
+    HIPCHK(hipMalloc((void **)&b->d_ch, sizeof(YdChain) * n));
+    HIPCHK(hipMalloc((void **)&b->d_s6, sizeof(void *) * n));
+    HIPCHK(hipMalloc((void **)&b->d_fe, sizeof(void *) * n));
+    HIPCHK(hipMemcpy(b->d_ch, chains.data(), sizeof(YdChain) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_s6, s6.data(), sizeof(void *) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_fe, fe.data(), sizeof(void *) * n, hipMemcpyHostToDevice));
+    return YAMC_OK;
+}
+
+static void batch_free(yamc_batch *b) {
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->d_ch) hipFree(b->d_ch);
+    if (b->d_s6) hipFree(b->d_s6);
+    if (b->d_fe) hipFree(b->d_fe);
+    delete b;
+}
+
+static int batch_counts(const yamc_batch *b, std::vector<long long> &acc) {
+    acc.resize(b->n);
+    for (int i = 0; i < b->n; ++i) {
+        YdState st;
+        HIPCHK(hipMemcpy(&st, b->ctxs[i]->ch.st, sizeof st, hipMemcpyDeviceToHost));
+        acc[i] = st.n_accepted;
+    }
+    return YAMC_OK;
+}
+
+extern "C" int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int64_t *n_accepted) {
+    yamc_batch *b = new yamc_batch;
+    int rc = batch_stage(ctxs, n, b);
+    std::vector<long long> acc0, acc1;
+    if (rc == YAMC_OK && n_accepted) rc = batch_counts(b, acc0);
+    if (rc == YAMC_OK) {
+        yamc_ctx *c0 = b->c0;
+        hipStream_t s = c0->stream;
+        auto go = [&]() -> int {
+            HIPCHK(hipEventRecord(c0->ev0, s));
+            TRY(launch_chain(c0, b->d_ch, b->d_s6, b->d_fe, n, ncycles, 0, b->max_natom));
+            HIPCHK(hipEventRecord(c0->ev1, s));
+            TRY(sync(c0));
+            HIPCHK(hipEventElapsedTime(&c0->last_ms, c0->ev0, c0->ev1));
+            return YAMC_OK;
+        };
+        rc = go();
+    }
+    if (rc == YAMC_OK && n_accepted) {
+        rc = batch_counts(b, acc1);
+        for (int i = 0; rc == YAMC_OK && i < n; ++i) n_accepted[i] = acc1[i] - acc0[i];
+    }
+    batch_free(b);
+    return rc;
+}
+
+extern "C" int yamc_batch_create(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, yamc_batch **out) {
+    if (!out) return fail(YAMC_ERROR_PARAMETER, "null output");
+    *out = nullptr;
+    if (ncycles < 1) return fail(YAMC_ERROR_PARAMETER, "ncycles must be positive");
+    yamc_batch *b = new yamc_batch;
+    int rc = batch_stage(ctxs, n, b);
+    if (rc == YAMC_OK) {
+        b->ncycles = ncycles;
+        yamc_ctx *c0 = b->c0;
+        auto go = [&]() -> int {
+            // The capture records one kernel node (no memset or copy nodes).
+            HIPCHK(hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
+            int lrc = launch_chain(c0, b->d_ch, b->d_s6, b->d_fe, n, ncycles, 0, b->max_natom);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c0->stream, &g);
+            if (lrc != YAMC_OK) {
+                if (g) hipGraphDestroy(g);
+                return lrc;
+            }
+            HIPCHK(e);
+            b->graph = g;
+            HIPCHK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+            return YAMC_OK;
+        };
+        rc = go();
+    }
+    if (rc != YAMC_OK) {
+        batch_free(b);
+        return rc;
+    }
+    *out = b;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_batch_launch(yamc_batch *b, int64_t *n_accepted) {
+    if (!b) return fail(YAMC_ERROR_PARAMETER, "null batch");
+    std::vector<long long> acc0, acc1;
+    if (n_accepted) TRY(batch_counts(b, acc0));
+    yamc_ctx *c0 = b->c0;
     hipStream_t s = c0->stream;
-    YdChain *d_ch = nullptr;
-    const double **d_s6 = nullptr, **d_fe = nullptr;
-    HIPCHK(hipMallocAsync((void **)&d_ch, sizeof(YdChain) * n, s));
-    HIPCHK(hipMallocAsync((void **)&d_s6, sizeof(void *) * n, s));
-    HIPCHK(hipMallocAsync((void **)&d_fe, sizeof(void *) * n, s));
-    HIPCHK(hipMemcpyAsync(d_ch, chains.data(), sizeof(YdChain) * n, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d_s6, s6.data(), sizeof(void *) * n, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d_fe, fe.data(), sizeof(void *) * n, hipMemcpyHostToDevice, s));
-    int max_natom = 0;
-    for (int i = 0; i < n; ++i) max_natom = ctxs[i]->natom > max_natom ? ctxs[i]->natom : max_natom;
     HIPCHK(hipEventRecord(c0->ev0, s));
-    TRY(launch_chain(c0, d_ch, d_s6, d_fe, n, ncycles, 0, max_natom));
+    HIPCHK(hipGraphLaunch(b->exec, s));
     HIPCHK(hipEventRecord(c0->ev1, s));
-    HIPCHK(hipFreeAsync(d_ch, s));
-    HIPCHK(hipFreeAsync(d_s6, s));
-    HIPCHK(hipFreeAsync(d_fe, s));
     TRY(sync(c0));
     HIPCHK(hipEventElapsedTime(&c0->last_ms, c0->ev0, c0->ev1));
     if (n_accepted) {
-        for (int i = 0; i < n; ++i) {
-            YdState st;
-            HIPCHK(hipMemcpy(&st, ctxs[i]->ch.st, sizeof st, hipMemcpyDeviceToHost));
-            n_accepted[i] = st.n_accepted - acc0[i];
-        }
+        TRY(batch_counts(b, acc1));
+        for (int i = 0; i < b->n; ++i) n_accepted[i] = acc1[i] - acc0[i];
     }
+    return YAMC_OK;
+}
+
+extern "C" int yamc_batch_destroy(yamc_batch *b) {
+    if (b) batch_free(b);
     return YAMC_OK;
 }
 

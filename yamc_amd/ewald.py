@@ -248,3 +248,33 @@ def batch_run(reps: Sequence[EwaldMC], ncycles: int):
     check(lib.yamc_batch_run(arr, len(reps), int(ncycles), nacc.ctypes.data_as(C.POINTER(C.c_int64))))
     ms = reps[0].last_kernel_ms()
     return nacc, ms
+
+
+class Batch:
+    """yamc_batch_create/launch/destroy: one sweep of ncycles cycles of all
+    replicas captured in a HIP graph and replayed per launch()."""
+
+    def __init__(self, reps: Sequence[EwaldMC], ncycles: int = 1):
+        self.lib = _lib.load()
+        self.reps = list(reps)
+        arr = (C.c_void_p * len(self.reps))(*[r.ctx.value for r in self.reps])
+        h = C.c_void_p()
+        check(self.lib.yamc_batch_create(arr, len(self.reps), int(ncycles), C.byref(h)))
+        self.h = h
+
+    def launch(self):
+        """Replays the graph; returns (accepted per replica, launch ms)."""
+        nacc = np.zeros(len(self.reps), dtype=np.int64)
+        check(self.lib.yamc_batch_launch(self.h, nacc.ctypes.data_as(C.POINTER(C.c_int64))))
+        return nacc, self.reps[0].last_kernel_ms()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.yamc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -322,3 +322,27 @@ def replica_system(n_water: int, conc_molar: float, temp: float, seed: int, rc_a
     counts = [n_water, n_pair, n_pair] if n_pair > 0 else [n_water]
     species = [SPCE, NA_DANG, CL_DANG] if n_pair > 0 else [SPCE]
     return lattice_system(p, species, counts, seed)
+
+
+def write_host_case(sysm: System, path: str, precision: int = 0, seed: int = 1, stream: int = 0,
+                    ncycles: int = 1) -> None:
+    """Writes the text case file read by the Fortran host fortran/mc_host
+    (layout documented at the top of fortran/mc_host.f90)."""
+    d = sysm.derived
+    T = sysm.ntype
+    with open(path, "w") as f:
+        f.write(f"{sysm.natom} {sysm.nmol} {T} {int(d['kmax'])} {precision}\n")
+        f.write(" ".join("%.17g" % v for v in (sysm.params.box_length, d["alpha"], d["rmax2"], d["xclb"],
+                                              d["kext"], d["dr"], d["d_angle"], sysm.params.lambda_f,
+                                              sysm.params.lambda_c)) + "\n")
+        for i in range(sysm.natom):
+            f.write("%.17g %.17g %.17g %d %d %.17g %.17g %.17g\n" % (
+                sysm.charge[i], sysm.hard_core[i], sysm.mass[i], sysm.molnum[i], sysm.lj_type[i], sysm.x[i],
+                sysm.y[i], sysm.z[i]))
+        for m in range(sysm.nmol):
+            f.write(f"{sysm.mol_first[m]} {sysm.mol_last[m]}\n")
+        for k in range(T * T):
+            f.write("%.17g %.17g %.17g %.17g\n" % (sysm.lj6[k], sysm.lj12[k], sysm.four_eps[k], sysm.sigma6[k]))
+        for t in range(T):
+            f.write(f"{sysm.tail_counts[t]}\n")
+        f.write(f"{seed} {stream} {ncycles}\n")

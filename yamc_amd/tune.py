@@ -1,0 +1,49 @@
+"""Launch-shape autotuner of the chain kernels.
+
+The library has three launch shapes for a batch of replicas (and for a single
+chain): a workgroup of 256 or 512 threads per chain, or one wavefront per
+replica with four replicas per workgroup (yamc_set_block_size 256, 512, 64).
+Which one is fastest depends on the replica count and size: one chain per
+workgroup spreads every sum over the workgroup (short moves, few chains per
+CU), one wave per replica trades a longer move for more chains in flight.
+
+tune_batch() times one sweep of the batch itself with each shape and keeps
+the fastest.  The timed sweeps are ordinary Monte Carlo sweeps of the chains
+(every shape runs the same algorithm on the same RNG streams), so tuning
+doubles as warm-up; the choice is applied to every context of the batch.
+"""
+from __future__ import annotations
+
+from typing import Dict, Sequence, Tuple
+
+from .ewald import EwaldMC, batch_run
+
+SHAPES = (256, 64, 512)
+
+
+def tune_batch(reps: Sequence[EwaldMC], shapes: Sequence[int] = SHAPES, cycles: int = 1) -> Tuple[int, Dict[int, float]]:
+    """Returns (best shape, {shape: trial moves/s}) and sets the best shape on
+    every replica.  Rates come from the HIP-event time of each launch."""
+    moves = sum(r.sys.nmol for r in reps) * cycles
+    rates = {}
+    for bs in shapes:
+        for r in reps:
+            r.set_block_size(bs)
+        _, ms = batch_run(reps, cycles)
+        rates[bs] = moves / (ms * 1e-3)
+    best = max(rates, key=rates.get)
+    for r in reps:
+        r.set_block_size(best)
+    return best, rates
+
+
+def tune_single(rep: EwaldMC, shapes: Sequence[int] = SHAPES, cycles: int = 1) -> Tuple[int, Dict[int, float]]:
+    """Same for one chain (yamc_mc_run)."""
+    rates = {}
+    for bs in shapes:
+        rep.set_block_size(bs)
+        rep.run(cycles)
+        rates[bs] = rep.sys.nmol * cycles / (rep.last_kernel_ms() * 1e-3)
+    best = max(rates, key=rates.get)
+    rep.set_block_size(best)
+    return best, rates
