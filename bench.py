@@ -62,6 +62,14 @@ def main():
     ap.add_argument("--cpu-moves", type=int, default=0, help="moves for the CPU baseline (0 = auto ~15 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32c,batch")
+    ap.add_argument("--forces-check", action="store_true",
+                    help="instead of the headline line: forward energy + adjoint forces of the 24,896-atom "
+                         "workload (FP32C), timed, with a central-difference check on 64 seeded atoms and the "
+                         "FP64 forces as reference")
+    ap.add_argument("--large-check", type=int, default=0, metavar="CYCLES",
+                    help="instead of the headline line: CYCLES run_mc cycles (33,968 moves each) of the "
+                         "99,504-atom workload in FP64 and in FP32C (FP64 resync every 1000 moves) on one "
+                         "RNG stream; rates, per-move dE error until the first differing decision, drift")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain fp32 streaming copy of MIB MiB (3 launches) and print its byte count; "
                          "used to scale the FETCH_SIZE / WRITE_SIZE counters")
@@ -90,6 +98,12 @@ def main():
     from yamc_amd.tune import tune_batch
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.forces_check:
+        forces_check(args)
+        return
+    if args.large_check:
+        large_check(args)
+        return
     kT_kjmol = system.BOLTZ * 298.0 / system.KCAL_MOL * 4.184
 
     forms = args.forms.split(",")
@@ -278,6 +292,137 @@ def main():
     }
     if "fp64" in res and "batch" in res:
         out["batch_vs_fp64_single"] = res["batch"]["moves_per_s"] / res["fp64"]["moves_per_s"]
+    print(json.dumps(out))
+
+
+def forces_check(args):
+    """8192 SPC/E + 160 Na + 160 Cl (24,896 atoms), L = 62.6 A, rc = 12 A,
+    kmax = 16, FP32C: time forward U_total (full recompute) + backward
+    (adjoint forces, sin/cos recomputed) through the torch.autograd harness,
+    check the gradient on 64 seeded atoms against central differences of
+    U_total (h = 1e-4 A), and compare all forces with the FP64 form."""
+    import numpy as np
+    import torch
+
+    from yamc_amd import YAMC_FP32C, YAMC_FP64, config
+    from yamc_amd.autograd import ewald_energy
+    from yamc_amd.ewald import EwaldMC
+
+    s = config("c3", seed=20261205)
+    f32 = EwaldMC(s, precision=YAMC_FP32C)
+    x, y, z = (torch.tensor(a, dtype=torch.float64, requires_grad=True) for a in s.positions())
+    ewald_energy(f32, x, y, z).backward()  # warm-up
+    fw, bw = [], []
+    for _ in range(max(1, args.steps)):
+        for t in (x, y, z):
+            t.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e = ewald_energy(f32, x, y, z)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        e.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fw.append(t1 - t0)
+        bw.append(t2 - t1)
+    grad = torch.stack([x.grad, y.grad, z.grad], 1).numpy()  # dU/dr, box units
+    f64 = EwaldMC(s, precision=YAMC_FP64)
+    f64.mc_init()
+    g64, r64 = f64.forces()
+    g32 = -grad
+    norm_err = np.linalg.norm(g32 - g64, axis=1) / np.maximum(np.linalg.norm(g64, axis=1), 1e-300)
+    h = 1e-4 / s.params.box_length
+    rng = np.random.default_rng(20261205)
+    atoms = rng.choice(s.natom, 64, replace=False)
+    x0, y0, z0 = s.positions()
+    probe = EwaldMC(s, precision=YAMC_FP32C)
+    # The reference energy is truncated at rc (EwaldSumRealSpace.f90:437):
+    # a pair whose distance is within h of rc jumps across the cut-off inside
+    # the difference stencil, which no gradient reproduces.  Such probes are
+    # counted and reported separately.
+    rc = float(np.sqrt(s.derived["rmax2"]))
+    errs, jumps = [], []
+    scale = np.abs(g64).max()
+    for k, i in enumerate(atoms):
+        dx = [a - a[i] for a in (x0, y0, z0)]
+        dx = [v - np.round(v) for v in dx]
+        r = np.sqrt(dx[0] ** 2 + dx[1] ** 2 + dx[2] ** 2)
+        r[i] = 0.0
+        crosses = bool(np.any(np.abs(r - rc) < 2 * h))
+        for d in range(3):
+            vals = []
+            for sgn in (1, -1):
+                xs = [x0.copy(), y0.copy(), z0.copy()]
+                xs[d][i] += sgn * h
+                probe.set_positions(*xs)
+                ev, _ = probe.mc_init()
+                vals.append(ev["U_total"])
+            fd = (vals[0] - vals[1]) / (2 * h)  # dU/dr
+            (jumps if crosses else errs).append(abs(fd - grad[i, d]) / scale)
+        if k % 8 == 7:
+            print(f"forces-check: {k + 1}/64 atoms, worst |fd - grad| / max|grad| = {max(errs or [0]):.3e}",
+                  file=sys.stderr, flush=True)
+    worst = max(errs) if errs else 0.0
+    out = {
+        "metric": "forward U_total + adjoint forces, 24,896 atoms, FP32C",
+        "workload": "c3: 8192 SPC/E + 160 Na + 160 Cl, L=62.6 A, rc=12 A, kmax=16",
+        "forward_ms": 1e3 * float(np.median(fw)),
+        "backward_ms": 1e3 * float(np.median(bw)),
+        "forward_backward_ms": 1e3 * float(np.median(np.array(fw) + np.array(bw))),
+        "fd_atoms": 64, "fd_h_angstrom": 1e-4,
+        "fd_max_err_rel_to_max_force": worst,
+        "fd_median_err_rel_to_max_force": float(np.median(errs)) if errs else 0.0,
+        "fd_probes": len(errs),
+        "fd_probes_with_a_pair_at_the_cutoff": len(jumps),
+        "fd_max_err_cutoff_probes": max(jumps) if jumps else None,
+        "fp32c_vs_fp64_force_max_rel_err_norm": float(norm_err.max()),
+        "fp32c_vs_fp64_force_median_rel_err_norm": float(np.median(norm_err)),
+        "dtype": "f64 arithmetic, f32 hi/lo S(k) storage",
+        "data": "synthetic: seeded lattice start, seeded orientations",
+    }
+    print(json.dumps(out))
+
+
+def large_check(args):
+    """32768 SPC/E + 600 Na + 600 Cl (99,504 atoms), L = 99.4 A, rc = 12 A,
+    kmax = 24, force-biased moves: the FP32C form (S(k) resync in FP64 every
+    1000 moves) against the plain FP64 form on one Philox stream."""
+    import numpy as np
+
+    from yamc_amd import YAMC_FP32C, YAMC_FP64, config, system
+    from yamc_amd.ewald import EwaldMC
+
+    kT_kjmol = system.BOLTZ * 298.0 / system.KCAL_MOL * 4.184
+    s = config("c5", seed=20261207)
+    out = {"metric": "large system: FP32C (resync every 1000 moves) vs FP64, one RNG stream",
+           "workload": "c5: 32768 SPC/E + 600 Na + 600 Cl, L=99.4 A, rc=12 A, kmax=24",
+           "cycles": args.large_check, "moves": args.large_check * s.nmol}
+    recs = {}
+    for form, prec, rs in (("fp64", YAMC_FP64, 0), ("fp32c", YAMC_FP32C, 1000)):
+        g = EwaldMC(s, precision=prec)
+        g.mc_init()
+        g.set_rng(20261207, 0)
+        t0 = time.perf_counter()
+        n, r = g.run(args.large_check, resync_every=rs, records=True)
+        dt = time.perf_counter() - t0
+        inc = g.energies()
+        g.resync()
+        fresh = g.energies()
+        recs[form] = r
+        out[form] = dict(moves_per_s=len(r) / dt, seconds=dt, acceptance=n / len(r),
+                         kspace_drift_rel=max(abs(inc[k] - fresh[k]) / max(1.0, abs(fresh[k]))
+                                              for k in ("E_kC", "E_kLJ")))
+        print(f"large-check: {form} done, {len(r)} moves in {dt:.1f} s", file=sys.stderr, flush=True)
+        g.close()
+    a, b = recs["fp64"], recs["fp32c"]
+    same = (a["accepted"] == b["accepted"]) & (a["imol"] == b["imol"])
+    n = int(np.argmin(same)) if not same.all() else len(a)
+    ok = a["overlap"][:n] == 0
+    d = np.abs(a["du_total"][:n] - b["du_total"][:n])[ok]
+    out["moves_before_first_differing_decision"] = n
+    out["max_dE_kT"] = float(d.max()) if d.size else 0.0
+    out["max_dE_kJmol"] = float(d.max() * kT_kjmol) if d.size else 0.0
     print(json.dumps(out))
 
 

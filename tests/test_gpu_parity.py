@@ -310,8 +310,48 @@ def test_fp32c_error_is_bounded(lib):
     _, rb = b.run(1, records=True)
     n = np.argmax(ra["accepted"] != rb["accepted"]) if np.any(ra["accepted"] != rb["accepted"]) else len(ra)
     assert n > 10
-    d = np.abs(ra["du_total"][:n] - rb["du_total"][:n])
-    assert np.max(d) < 1e-3, np.max(d)
+    ok = ra["overlap"][:n] == 0
+    d = np.abs(ra["du_total"][:n] - rb["du_total"][:n])[ok]
+    # per-move dE within 1e-6 kJ/mol = 4.04e-7 kT at 298 K
+    assert np.max(d) < 4.04e-7, np.max(d)
+    fa = np.concatenate([ra["f_old"][:n][ok], ra["f_new"][:n][ok]])
+    fb = np.concatenate([rb["f_old"][:n][ok], rb["f_new"][:n][ok]])
+    fe = np.linalg.norm(fa - fb, axis=1) / np.linalg.norm(fa, axis=1)
+    assert fe.max() < 1e-5, fe.max()
+
+
+def test_autograd_harness_gradient(lib):
+    """torch.autograd over the adjoint: dU_total/dr from backward equals
+    central differences of U_total (h = 1e-4 A) and minus the library's
+    gradient forces."""
+    import torch
+    from yamc_amd import config
+    from yamc_amd.autograd import ewald_energy
+    from yamc_amd.ewald import EwaldMC
+    s = config("c1", seed=9)
+    mc = EwaldMC(s)
+    x, y, z = (torch.tensor(a, dtype=torch.float64, requires_grad=True) for a in s.positions())
+    e = ewald_energy(mc, x, y, z)
+    e.backward()
+    grad = torch.stack([x.grad, y.grad, z.grad], 1).numpy()
+    g, _ = mc.forces()
+    assert np.array_equal(grad, -g)
+    probe = EwaldMC(s)
+    h = 1e-4 / s.params.box_length
+    x0, y0, z0 = s.positions()
+    rng = np.random.default_rng(9)
+    worst = 0.0
+    for i in rng.choice(s.natom, 8, replace=False):
+        for d in range(3):
+            vals = []
+            for sgn in (1, -1):
+                xs = [x0.copy(), y0.copy(), z0.copy()]
+                xs[d][i] += sgn * h
+                probe.set_positions(*xs)
+                ev, _ = probe.mc_init()
+                vals.append(ev["U_total"])
+            worst = max(worst, abs((vals[0] - vals[1]) / (2 * h) - grad[i, d]) / np.abs(grad).max())
+    assert worst < 1e-5, worst
 
 
 @pytest.mark.parametrize("case", __import__("golden_cases").cases(), ids=lambda p: p.split("/")[-1])
