@@ -191,6 +191,15 @@ module yamc_ewald
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx, gx, gy, gz, rx, ry, rz
       end function
+      integer(c_int) function yamc_rdf(ctx, mol_species, nspecies, dr_angstr, rmax_angstr, nbins, counts) &
+            bind(C, name="yamc_rdf")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), intent(in) :: mol_species(*)
+         integer(c_int32_t), value :: nspecies, nbins
+         real(c_double), value :: dr_angstr, rmax_angstr
+         integer(c_int64_t), intent(inout) :: counts(*)
+      end function
       integer(c_int) function yamc_set_block_size(ctx, block) bind(C, name="yamc_set_block_size")
          import :: c_int, c_ptr, c_int32_t
          type(c_ptr), value :: ctx

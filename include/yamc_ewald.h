@@ -207,6 +207,17 @@ int yamc_batch_destroy(yamc_batch *batch);
  * Either pointer triple may be NULL. */
 int yamc_forces(yamc_ctx *ctx, double *gx, double *gy, double *gz, double *rx, double *ry, double *rz);
 
+/* ---- analysis ----------------------------------------------------------- */
+/* Centre-of-mass radial distribution counts of the current configuration
+ * (mcrdf2.f90:262-336 count_distances, one frame, cnt4 column): molecule
+ * COMs (mass-weighted, wrapped into the box), every ordered pair i != j with
+ * the single minimum-image shift, r < rmax (rmax <= L/2), bin floor(r/dr) of
+ * the (min, max) species pair.  counts[nspecies][nspecies][nbins] is
+ * accumulated (+=), so successive frames add up; mol_species[nmol] is
+ * 0-based, nspecies <= 8, nbins <= 4096. */
+int yamc_rdf(yamc_ctx *ctx, const int32_t *mol_species, int32_t nspecies, double dr_angstr, double rmax_angstr,
+             int32_t nbins, uint64_t *counts);
+
 /* ---- launch shape ------------------------------------------------------ */
 /* Launch shape of the chain kernel for this context (and for a batch whose
  * first context it is): 256 or 512 = one workgroup of that size per chain;

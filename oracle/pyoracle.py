@@ -259,3 +259,45 @@ def erfc_bet6_bet12(b):
     e, b6, b12 = C.c_double(), C.c_double(), C.c_double()
     lib.yo_erfc_bet6_bet12(b, b * b, float(np.exp(-b * b)), C.byref(e), C.byref(b6), C.byref(b12))
     return e.value, b6.value, b12.value
+
+
+def rdf_counts(sysm, mol_species, nspecies, dr, rmax, nbins):
+    """mcrdf2.f90:262-336 count_distances for one frame (cnt4 column), as the
+    CPU check of yamc_rdf: molecule COMs (mass-weighted, products and sums
+    in atom order, wrapped with round-half-away), ordered pairs i != j, the
+    single minimum-image shift of :276-283, r^2 > rmax^2 - 1e-10 skipped,
+    bin floor(r / dr) of the (min, max) species pair."""
+    import numpy as _np
+    L = sysm.params.box_length
+    com = _np.zeros((sysm.nmol, 3))
+    for m in range(sysm.nmol):
+        f, l = int(sysm.mol_first[m]), int(sysm.mol_last[m])
+        s = [0.0, 0.0, 0.0]
+        sm = 0.0
+        for a in range(f, l + 1):
+            w = float(sysm.mass[a])
+            s[0] = s[0] + w * float(sysm.x[a])
+            s[1] = s[1] + w * float(sysm.y[a])
+            s[2] = s[2] + w * float(sysm.z[a])
+            sm = sm + w
+        for d in range(3):
+            v = s[d] / sm
+            r = _np.sign(v) * _np.floor(abs(v) + 0.5)
+            com[m, d] = v - r
+    sp = _np.asarray(mol_species)
+    counts = _np.zeros((nspecies, nspecies, nbins), dtype=_np.uint64)
+    x1 = com[:, None, :]
+    x2 = _np.broadcast_to(com[None, :, :], (sysm.nmol, sysm.nmol, 3)).copy()
+    x2 = _np.where(x1 - x2 > 0.5, x2 + 1.0, x2)
+    x2 = _np.where(x1 - x2 < -0.5, x2 - 1.0, x2)
+    d = (x1 - x2) * L
+    r2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    ii, jj = _np.nonzero(~(r2 > rmax * rmax - 1e-10))
+    keep = ii != jj
+    ii, jj = ii[keep], jj[keep]
+    b = (_np.sqrt(r2[ii, jj]) / dr).astype(_np.int64)
+    lo = _np.minimum(sp[ii], sp[jj])
+    hi = _np.maximum(sp[ii], sp[jj])
+    ok = b < nbins
+    _np.add.at(counts, (lo[ok], hi[ok], b[ok]), 1)
+    return counts

@@ -42,6 +42,8 @@ __global__ void yk_rspace_partial(YdChain c, int imol, const double *x, const do
                                   double *fyb, double *fzb);
 void yk_launch_forces(const YdChain &c, const double *sig6, const double *feps, const double *tab, double *g,
                       double *r, hipStream_t s);
+void yk_launch_rdf(const YdChain &c, double *com, const int *species, int nspec, double dr, double rmax2, int nbins,
+                   unsigned long long *hist, hipStream_t s);
 
 #ifndef YK_BS
 #define YK_BS 512
@@ -901,5 +903,38 @@ extern "C" int yamc_phase_times(yamc_ctx *c, double *us) {
 extern "C" int yamc_last_kernel_ms(yamc_ctx *c, double *ms) {
     if (!c || !ms) return fail(YAMC_ERROR_PARAMETER, "null argument");
     *ms = c->last_ms;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_rdf(yamc_ctx *c, const int32_t *mol_species, int32_t nspecies, double dr_angstr,
+                        double rmax_angstr, int32_t nbins, uint64_t *counts) {
+    if (!c || !mol_species || !counts) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    if (nspecies < 1 || nspecies > 8) return fail(YAMC_ERROR_PARAMETER, "nspecies %d outside 1..8", nspecies);
+    if (nbins < 1 || nbins > 4096) return fail(YAMC_ERROR_PARAMETER, "nbins %d outside 1..4096", nbins);
+    if (!(dr_angstr > 0.0)) return fail(YAMC_ERROR_PARAMETER, "dr must be positive");
+    if (rmax_angstr > 0.5 * c->ch.p.L)
+        return fail(YAMC_ERROR_PARAMETER, "rmax %g A exceeds half the box (%g A)", rmax_angstr, 0.5 * c->ch.p.L);
+    for (int m = 0; m < c->nmol; ++m)
+        if (mol_species[m] < 0 || mol_species[m] >= nspecies)
+            return fail(YAMC_ERROR_PARAMETER, "species of molecule %d outside 0..%d", m, nspecies - 1);
+    const size_t nh = (size_t)nspecies * nspecies * nbins;
+    hipStream_t s = c->stream;
+    double *com = nullptr;
+    int *sp = nullptr;
+    unsigned long long *hist = nullptr;
+    HIPCHK(hipMallocAsync((void **)&com, sizeof(double) * 3 * c->nmol, s));
+    HIPCHK(hipMallocAsync((void **)&sp, sizeof(int) * c->nmol, s));
+    HIPCHK(hipMallocAsync((void **)&hist, sizeof(unsigned long long) * nh, s));
+    HIPCHK(hipMemcpyAsync(sp, mol_species, sizeof(int) * c->nmol, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(hist, 0, sizeof(unsigned long long) * nh, s));
+    yk_launch_rdf(c->ch, com, sp, nspecies, dr_angstr, rmax_angstr * rmax_angstr, nbins, hist, s);
+    HIPCHK(hipGetLastError());
+    std::vector<unsigned long long> h(nh);
+    HIPCHK(hipMemcpyAsync(h.data(), hist, sizeof(unsigned long long) * nh, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipFreeAsync(com, s));
+    HIPCHK(hipFreeAsync(sp, s));
+    HIPCHK(hipFreeAsync(hist, s));
+    TRY(sync(c));
+    for (size_t k = 0; k < nh; ++k) counts[k] += h[k];
     return YAMC_OK;
 }

@@ -230,6 +230,20 @@ class EwaldMC:
         check(self.lib.yamc_phase_times(self.ctx, dptr(us)))
         return dict(zip(self.PHASES, us))
 
+    def rdf(self, mol_species, nspecies: int, dr: float = 0.1, rmax: Optional[float] = None,
+            counts: Optional[np.ndarray] = None) -> np.ndarray:
+        """yamc_rdf: adds this configuration's COM pair counts (mcrdf2.f90
+        count_distances, cnt4) into counts[nspecies, nspecies, nbins]."""
+        L = self.sys.params.box_length
+        rmax = 0.5 * L if rmax is None else rmax
+        nbins = int((rmax - 1e-10) / dr) + 1
+        if counts is None:
+            counts = np.zeros((nspecies, nspecies, nbins), dtype=np.uint64)
+        sp = np.ascontiguousarray(mol_species, dtype=np.int32)
+        check(self.lib.yamc_rdf(self.ctx, iptr(sp), int(nspecies), C.c_double(dr), C.c_double(rmax),
+                                int(counts.shape[2]), counts.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return counts
+
     def set_block_size(self, block: int) -> None:
         """Workgroup size of the chain kernel (256 or 512)."""
         check(self.lib.yamc_set_block_size(self.ctx, int(block)))
@@ -278,3 +292,19 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def rdf_normalize(counts: np.ndarray, nframes: int, n_per_species: Sequence[int], volume: float,
+                  dr: float) -> np.ndarray:
+    """mcrdf2.f90:440-470 (rdf1 column) at constant volume: g_ij(r) =
+    cnt / N_frame / <1/V> / N_i / N_j / dV(r) / (1 if i == j else 2), with
+    dV = 4 pi dr (r^2 + r dr) + 4/3 pi dr^3 and r the bin's lower edge."""
+    ns, _, nb = counts.shape
+    r = np.arange(nb) * dr
+    dV = 4.0 * np.pi * dr * (r ** 2 + r * dr) + 4.0 / 3.0 * np.pi * dr ** 3
+    g = np.zeros(counts.shape)
+    for i in range(ns):
+        for j in range(i, ns):
+            f = 1.0 if i == j else 2.0
+            g[i, j] = counts[i, j] / nframes * volume / n_per_species[i] / n_per_species[j] / dV / f
+    return g
