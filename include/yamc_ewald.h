@@ -221,9 +221,12 @@ int yamc_rdf(yamc_ctx *ctx, const int32_t *mol_species, int32_t nspecies, double
 /* ---- launch shape ------------------------------------------------------ */
 /* Launch shape of the chain kernel for this context (and for a batch whose
  * first context it is): 256 or 512 = one workgroup of that size per chain;
- * 64 = one wavefront per chain, four chains per workgroup (run_mc cycles
- * only; constant-uniform traces use 256).  No reference counterpart (the
- * reference is serial); the autotuner (yamc_amd.tune) measures them. */
+ * 64 or 128 = one wavefront per chain, four chains per workgroup, registers
+ * sized for one (64) or two (128) waves per SIMD, i.e. 4 or 8 chains in
+ * flight per CU (FP64 run_mc cycles; FP32C contexts and constant-uniform
+ * traces run these shapes as workgroups of 256).  No
+ * reference counterpart (the reference is serial); the autotuner
+ * (yamc_amd.tune) measures them. */
 int yamc_set_block_size(yamc_ctx *ctx, int32_t block);
 
 /* ---- diagnostics ------------------------------------------------------- */

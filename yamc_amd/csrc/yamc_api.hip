@@ -27,9 +27,9 @@ __global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, do
 template <bool F32, int BS>
 __global__ void yk_chain(const YdChain *chains, const double *const *sig6s, const double *const *feps_s, int nsteps,
                          int mode, int max_moves, int lds_atoms);
-template <bool F32>
+template <bool F32, int OCC>
 __global__ void yk_wchain(const YdChain *chains, int nchains, const double *const *sig6s, const double *const *feps_s,
-                          int ncycles);
+                          int ncycles);  // FP64 instantiations only
 #define YW_WPB 4  // replicas (waves) per workgroup of yk_wchain
 template <bool F32>
 __global__ void yk_kspace_delta(YdChain c, int imol, const double *x, const double *y, const double *z,
@@ -85,7 +85,7 @@ struct yamc_ctx {
     YdChain *d_chain = nullptr;
     const double **d_sig6s = nullptr, **d_feps = nullptr;
     bool beta_ready = false, state_ready = false;
-    int chain_bs = 256;  // workgroup size of the chain kernel (256 or 512)
+    int chain_bs = 256;  // launch shape of the chain kernel (yamc_set_block_size)
 };
 
 template <typename T>
@@ -619,13 +619,17 @@ static int launch_chain_t(yamc_ctx *c, const YdChain *d_chain, const double *con
     return YAMC_OK;
 }
 
+// The wave shapes (64, 128) run the wave kernel for FP64 run_mc cycles;
+// FP32C chains and constant-uniform traces run them as workgroups of 256
+// (the FP32C wave kernel is switched off: it made an illegal memory access
+// on its first GPU launch and its cause is not yet found).
 static int launch_chain(yamc_ctx *c, const YdChain *d_chain, const double *const *sig6s,
                         const double *const *feps, int nchains, int nsteps, int mode, int max_natom) {
     const bool f32 = c->precision == YAMC_FP32C;
-    if (c->chain_bs == 64 && mode == 0) {  // one wave per replica
+    if ((c->chain_bs == 64 || c->chain_bs == 128) && mode == 0 && !f32) {  // one wave per replica
         int nb = (nchains + YW_WPB - 1) / YW_WPB;
-        if (f32) yk_wchain<true><<<nb, YW_WPB * 64, 0, c->stream>>>(d_chain, nchains, sig6s, feps, nsteps);
-        else yk_wchain<false><<<nb, YW_WPB * 64, 0, c->stream>>>(d_chain, nchains, sig6s, feps, nsteps);
+        if (c->chain_bs == 64) yk_wchain<false, 1><<<nb, YW_WPB * 64, 0, c->stream>>>(d_chain, nchains, sig6s, feps, nsteps);
+        else yk_wchain<false, 2><<<nb, YW_WPB * 64, 0, c->stream>>>(d_chain, nchains, sig6s, feps, nsteps);
         HIPCHK(hipGetLastError());
         return YAMC_OK;
     }
@@ -638,8 +642,9 @@ static int launch_chain(yamc_ctx *c, const YdChain *d_chain, const double *const
 
 extern "C" int yamc_set_block_size(yamc_ctx *c, int32_t block) {
     if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
-    if (block != 64 && block != 256 && block != 512)
-        return fail(YAMC_ERROR_PARAMETER, "block size %d: must be 64 (one wave per replica), 256 or 512", block);
+    if (block != 64 && block != 128 && block != 256 && block != 512)
+        return fail(YAMC_ERROR_PARAMETER, "launch shape %d: must be 64 or 128 (one wave per replica), 256 or 512",
+                    block);
     c->chain_bs = block;
     return YAMC_OK;
 }

@@ -1304,9 +1304,11 @@ __device__ int yk_sec_propose(const YdChain &c, YkMol &m, YkScalar &t0, const do
     int na = m.na;
     double fx[YD_MAXA], fy[YD_MAXA], fz[YD_MAXA];
     double U_old = 0.0;
-    for (int a = 0; a < na; ++a) U_old = U_old + uu_old[a];
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < na) U_old = U_old + uu_old[a];
     t0.U_old = U_old;
-    for (int a = 0; a < na; ++a) {
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < na) {
         fx[a] = fr_old[a][0] + YD_FOUR_PI * out[YK_FK(a, 0)];
         fy[a] = fr_old[a][1] + YD_FOUR_PI * out[YK_FK(a, 1)];
         fz[a] = fr_old[a][2] + YD_FOUR_PI * out[YK_FK(a, 2)];
@@ -1315,22 +1317,26 @@ __device__ int yk_sec_propose(const YdChain &c, YkMol &m, YkScalar &t0, const do
         fz[a] = fz[a] + p.m2kext * m.q[a] * t0.mu[2];
     }
     double mo[3] = {0.0, 0.0, 0.0};
-    for (int a = 0; a < na; ++a) {
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < na) {
         mo[0] = mo[0] + m.q[a] * m.xo[a];
         mo[1] = mo[1] + m.q[a] * m.yo[a];
         mo[2] = mo[2] + m.q[a] * m.zo[a];
     }
     t0.muo[0] = mo[0]; t0.muo[1] = mo[1]; t0.muo[2] = mo[2];
     double Fo[3] = {0, 0, 0}, To[3] = {0, 0, 0}, com[3] = {0, 0, 0}, tm = 0.0, tq[3];
-    for (int a = 0; a < na; ++a) { Fo[0] = Fo[0] + fx[a]; Fo[1] = Fo[1] + fy[a]; Fo[2] = Fo[2] + fz[a]; }
-    for (int a = 0; a < na; ++a) {
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < na) { Fo[0] = Fo[0] + fx[a]; Fo[1] = Fo[1] + fy[a]; Fo[2] = Fo[2] + fz[a]; }
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < na) {
         com[0] = com[0] + m.mass[a] * m.xo[a];
         com[1] = com[1] + m.mass[a] * m.yo[a];
         com[2] = com[2] + m.mass[a] * m.zo[a];
         tm = tm + m.mass[a];
     }
     com[0] = com[0] / tm; com[1] = com[1] / tm; com[2] = com[2] / tm;
-    for (int a = 0; a < na; ++a) {
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < na) {
         yk_prod(m.xo[a] - com[0], m.yo[a] - com[1], m.zo[a] - com[2], fx[a], fy[a], fz[a], tq);
         To[0] = To[0] + tq[0]; To[1] = To[1] + tq[1]; To[2] = To[2] + tq[2];
     }
@@ -1354,7 +1360,8 @@ __device__ int yk_sec_propose(const YdChain &c, YkMol &m, YkScalar &t0, const do
         R[6] = st * (ct * ca1 * cphi - sa * sphi);
         R[7] = st * (ct * ca1 * sphi + sa * cphi);
         R[8] = (st * st * ca + ct * ct);
-        for (int a = 0; a < na; ++a) {
+#pragma unroll
+        for (int a = 0; a < YD_MAXA; ++a) if (a < na) {
             double xr = m.xo[a] - com[0], yr = m.yo[a] - com[1], zr = m.zo[a] - com[2];
             m.xn[a] = (R[0] * xr + R[1] * yr + R[2] * zr) + cn[0];
             m.yn[a] = (R[3] * xr + R[4] * yr + R[5] * zr) + cn[1];
@@ -1375,16 +1382,19 @@ __device__ int yk_sec_accept(const YdChain &c, const YkMol &m, YkScalar &t0, con
     YkMcm &mcm = t0.mcm;
     int na = m.na;
     double U_new = 0.0;
-    for (int a = 0; a < na; ++a) U_new = U_new + t0.uun[a];
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < na) U_new = U_new + t0.uun[a];
     double du_real = U_new - t0.U_old, duc = out[0], dulj = out[1];
     double fx[YD_MAXA], fy[YD_MAXA], fz[YD_MAXA];
-    for (int a = 0; a < na; ++a) {
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < na) {
         fx[a] = t0.frn[a][0] + YD_FOUR_PI * out[YK_FK(a, 0)];
         fy[a] = t0.frn[a][1] + YD_FOUR_PI * out[YK_FK(a, 1)];
         fz[a] = t0.frn[a][2] + YD_FOUR_PI * out[YK_FK(a, 2)];
     }
     double mun[3] = {0, 0, 0};
-    for (int a = 0; a < na; ++a) {
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < na) {
         mun[0] = mun[0] + m.q[a] * m.xn[a];
         mun[1] = mun[1] + m.q[a] * m.yn[a];
         mun[2] = mun[2] + m.q[a] * m.zn[a];
@@ -1394,21 +1404,25 @@ __device__ int yk_sec_accept(const YdChain &c, const YkMol &m, YkScalar &t0, con
                               (2 * t0.mu[2] + dmu[2]) * dmu[2]);
     double mn[3];
     for (int d = 0; d < 3; ++d) mn[d] = t0.mu[d] + dmu[d];
-    for (int a = 0; a < na; ++a) {
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < na) {
         fx[a] = fx[a] + p.m2kext * m.q[a] * mn[0];
         fy[a] = fy[a] + p.m2kext * m.q[a] * mn[1];
         fz[a] = fz[a] + p.m2kext * m.q[a] * mn[2];
     }
     double Fn[3] = {0, 0, 0}, Tn[3] = {0, 0, 0}, com[3] = {0, 0, 0}, tm = 0.0, tq[3];
-    for (int a = 0; a < na; ++a) { Fn[0] = Fn[0] + fx[a]; Fn[1] = Fn[1] + fy[a]; Fn[2] = Fn[2] + fz[a]; }
-    for (int a = 0; a < na; ++a) {
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < na) { Fn[0] = Fn[0] + fx[a]; Fn[1] = Fn[1] + fy[a]; Fn[2] = Fn[2] + fz[a]; }
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < na) {
         com[0] = com[0] + m.mass[a] * m.xn[a];
         com[1] = com[1] + m.mass[a] * m.yn[a];
         com[2] = com[2] + m.mass[a] * m.zn[a];
         tm = tm + m.mass[a];
     }
     com[0] = com[0] / tm; com[1] = com[1] / tm; com[2] = com[2] / tm;
-    for (int a = 0; a < na; ++a) {
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < na) {
         yk_prod(m.xn[a] - com[0], m.yn[a] - com[1], m.zn[a] - com[2], fx[a], fy[a], fz[a], tq);
         Tn[0] = Tn[0] + tq[0]; Tn[1] = Tn[1] + tq[1]; Tn[2] = Tn[2] + tq[2];
     }
@@ -1670,6 +1684,235 @@ YK_CHAIN_INST(false, 512)
 YK_CHAIN_INST(true, 256)
 YK_CHAIN_INST(true, 512)
 
+// ------------------------------------------------------------------------
+// Lean per-move passes of the wave kernel.  The same arithmetic as
+// yk_sweep / yk_rs_exact, split so that fewer values are live at once: the
+// Coulomb set and the LJ sets are swept in separate k loops, and the exact
+// real-space pass visits the molecule's sites one at a time.  A wave then
+// needs fewer registers, so more replicas share a SIMD.
+
+// Coulomb set of a sweep (MC.f90:963-968 / 1136-1173, Coulomb parts):
+// DELTA adds dU_C (acc[0]) and evaluates the forces against S + dS.
+template <bool F32, bool DELTA, int NA>
+__device__ void yl_sweep_coul(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size) {
+    const int nk = c.p.nk;
+    const double *tabs = DELTA ? m.tab_n : m.tab_o;
+    for (int kk = rank; kk < nk; kk += size) {
+        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        double bt = c.beta[kk];
+        double C[8];
+        yk_ld8<F32>(c, 0, kk, C);
+        double kn[NA][8];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            if (m.q[a] != 0.0) yd_kr(tabs + a * 6 * YD_KP, m.q[a], kx, ky, kz, kn[a]);
+        }
+        if (DELTA) {
+            double ns[8], os[8], ko[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) { ns[i] = 0.0; os[i] = 0.0; }
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                if (m.q[a] == 0.0) continue;
+                yd_kr(m.tab_o + a * 6 * YD_KP, m.q[a], kx, ky, kz, ko);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) { ns[i] = ns[i] + kn[a][i]; os[i] = os[i] + ko[i]; }
+            }
+            double dc[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) dc[i] = ns[i] - os[i];
+            acc[0] += yd_du_k(C, dc, ky, kz) * bt;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) C[i] = C[i] + dc[i];
+        }
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            if (m.q[a] == 0.0) continue;
+            double f[3] = {0, 0, 0};
+            yd_force_k(C, kn[a], kx, ky, kz, bt, f);
+            acc[YK_FK(a, 0)] += f[0];
+            acc[YK_FK(a, 1)] += f[1];
+            acc[YK_FK(a, 2)] += f[2];
+        }
+    }
+}
+
+// LJ sets of a sweep: DELTA adds dU_LJ (acc[1]); forces of the LJ sites
+// against every LJ set (+ its delta).  The LJ q-weighted sin/cos of a
+// charged site is its Coulomb one times 1/q (RhoSquared.f90:173-176).
+template <bool F32, bool DELTA, int NA>
+__device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size) {
+    const int nk = c.p.nk, T = c.p.ntype;
+    if (T == 0 || m.np == 0) return;
+    const double *tabs = DELTA ? m.tab_n : m.tab_o;
+    for (int kk = rank; kk < nk; kk += size) {
+        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        double kl[NA][8];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            if (m.typ[a] <= 0) continue;
+            double qa = m.q[a];
+            if (qa != 0.0) {
+                yd_kr(tabs + a * 6 * YD_KP, qa, kx, ky, kz, kl[a]);
+                double inv = 1.0 / qa;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) kl[a][i] = kl[a][i] * inv;
+            } else {
+                yd_kr(tabs + a * 6 * YD_KP, 1.0, kx, ky, kz, kl[a]);
+            }
+        }
+        double dl[YK_MAXP][8];
+        if (DELTA) {
+            double nl[YK_MAXP][8], ol[YK_MAXP][8];
+#pragma unroll
+            for (int q = 0; q < YK_MAXP; ++q)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) { nl[q][i] = 0.0; ol[q][i] = 0.0; }
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                if (m.typ[a] <= 0) continue;
+                double qa = m.q[a], lo[8];
+                if (qa != 0.0) {
+                    yd_kr(m.tab_o + a * 6 * YD_KP, qa, kx, ky, kz, lo);
+                    double inv = 1.0 / qa;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) lo[i] = lo[i] * inv;
+                } else {
+                    yd_kr(m.tab_o + a * 6 * YD_KP, 1.0, kx, ky, kz, lo);
+                }
+#pragma unroll
+                for (int q = 0; q < YK_MAXP; ++q)
+                    if (q == m.lslot[a]) {
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) { nl[q][i] = nl[q][i] + kl[a][i]; ol[q][i] = ol[q][i] + lo[i]; }
+                    }
+            }
+#pragma unroll
+            for (int q = 0; q < YK_MAXP; ++q)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) dl[q][i] = nl[q][i] - ol[q][i];
+        }
+        for (int t1 = 0; t1 < T; ++t1) {
+            double C1[8], bl[YK_MAXP];
+            yk_ld8<F32>(c, 1 + t1, kk, C1);
+#pragma unroll
+            for (int q = 0; q < YK_MAXP; ++q)
+                bl[q] = q < m.np ? c.beta_lj[((size_t)t1 * T + (m.ptype[q] - 1)) * nk + kk] : 0.0;
+            if (DELTA) {
+                int s1 = -1;
+#pragma unroll
+                for (int q = 0; q < YK_MAXP; ++q)
+                    if (q < m.np && m.ptype[q] == t1 + 1) s1 = q;
+                double tw[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    double d1 = 0.0;
+#pragma unroll
+                    for (int q = 0; q < YK_MAXP; ++q)
+                        if (q == s1) d1 = dl[q][i];
+                    tw[i] = C1[i] + C1[i] + d1;
+                    C1[i] = C1[i] + d1;
+                }
+#pragma unroll
+                for (int q = 0; q < YK_MAXP; ++q)
+                    if (q < m.np) acc[1] += yd_dulj_k(tw, dl[q], ky, kz) * bl[q];
+            }
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const int ta = m.typ[a];
+                if (ta <= 0) continue;
+                double b = 0.0;
+#pragma unroll
+                for (int q = 0; q < YK_MAXP; ++q)
+                    if (q == m.lslot[a]) b = bl[q];
+                if (m.lslot[a] < 0) b = c.beta_lj[((size_t)(ta - 1) * T + t1) * nk + kk];
+                double f[3] = {0, 0, 0};
+                yd_force_k(C1, kl[a], kx, ky, kz, b, f);
+                acc[YK_FK(a, 0)] += f[0];
+                acc[YK_FK(a, 1)] += f[1];
+                acc[YK_FK(a, 2)] += f[2];
+            }
+        }
+    }
+}
+
+template <bool F32, bool DELTA>
+__device__ __forceinline__ void yl_sweep_n(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank,
+                                           int size) {
+    switch (__builtin_amdgcn_readfirstlane(m.na)) {
+        case 1: yl_sweep_coul<F32, DELTA, 1>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 1>(c, m, acc, rank, size); break;
+        case 2: yl_sweep_coul<F32, DELTA, 2>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 2>(c, m, acc, rank, size); break;
+        case 3: yl_sweep_coul<F32, DELTA, 3>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 3>(c, m, acc, rank, size); break;
+        default: yl_sweep_coul<F32, DELTA, 4>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 4>(c, m, acc, rank, size); break;
+    }
+}
+
+// Exact real-space pass (same terms and stored back terms as yk_rs_exact):
+// the new sites, whose per-site sums need compile-time accumulator indices,
+// are unrolled; the old sites, which only feed site-independent sums, are
+// visited one at a time.
+template <int NA>
+__device__ void yl_rs_exact(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
+                            const int *list, int count, double (&acc)[YK_NRX], int &ov, int rank, int size) {
+    const int T = c.p.ntype;
+    for (int i = rank; i < count; i += size) {
+        int j = list[i];
+        int jt = c.type[j];
+        double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+        double nb[4] = {0, 0, 0, 0}, ob[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            int it = m.typ[a];
+            double xij = yd_wrap(m.xn[a] - xj), yij = yd_wrap(m.yn[a] - yj), zij = yd_wrap(m.zn[a] - zj);
+            double r2 = xij * xij + yij * yij + zij * zij;
+            double d12 = (m.dh[a] + dhj) / 2.0;
+            if (it > 0 && jt > 0 && r2 < d12 * d12) ov = 1;
+            if (r2 >= c.p.rmax2) continue;
+            bool lj = it != 0 && jt != 0;
+            int off = lj ? (it - 1) * T + (jt - 1) : 0;
+            YdPair pr;
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            yk_rs_acc(acc, a, pr, xij, yij, zij);
+            nb[0] = nb[0] + pr.pot;
+            nb[1] = nb[1] - xij * pr.FR;
+            nb[2] = nb[2] - yij * pr.FR;
+            nb[3] = nb[3] - zij * pr.FR;
+        }
+#pragma unroll 1
+        for (int a = 0; a < NA; ++a) {
+            int it = m.typ[a];
+            double xij = yd_wrap(m.xo[a] - xj), yij = yd_wrap(m.yo[a] - yj), zij = yd_wrap(m.zo[a] - zj);
+            double r2 = xij * xij + yij * yij + zij * zij;
+            if (r2 >= c.p.rmax2) continue;
+            bool lj = it != 0 && jt != 0;
+            int off = lj ? (it - 1) * T + (jt - 1) : 0;
+            YdPair pr;
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            ob[0] = ob[0] + pr.pot;
+            ob[1] = ob[1] - xij * pr.FR;
+            ob[2] = ob[2] - yij * pr.FR;
+            ob[3] = ob[3] - zij * pr.FR;
+            acc[YK_NRS + 0] += pr.potew;
+            acc[YK_NRS + 1] += pr.potlj6;
+            acc[YK_NRS + 2] += pr.potlj12;
+        }
+        double *b = c.rsb + (size_t)i * 8;
+        b[0] = nb[0]; b[1] = nb[1]; b[2] = nb[2]; b[3] = nb[3];
+        b[4] = ob[0]; b[5] = ob[1]; b[6] = ob[2]; b[7] = ob[3];
+    }
+}
+
+__device__ __forceinline__ void yl_rs_exact_n(const YdChain &c, const YkMol &m, const double *sig6,
+                                              const double *feps, const int *list, int count,
+                                              double (&acc)[YK_NRX], int &ov, int rank, int size) {
+    switch (__builtin_amdgcn_readfirstlane(m.na)) {
+        case 1: yl_rs_exact<1>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+        case 2: yl_rs_exact<2>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+        case 3: yl_rs_exact<3>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+        default: yl_rs_exact<4>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+    }
+}
+
 // ========================================================================
 // The wave chain kernel (batched replicas): one wavefront per replica,
 // YW_WPB replicas per workgroup, no workgroup barriers.  Each wave runs
@@ -1756,7 +1999,7 @@ __device__ int yw_scan(const YdChain &c, const YkMol &m, int lane) {
 }
 
 __device__ __forceinline__ int yw_scan_n(const YdChain &c, const YkMol &m, int lane) {
-    switch (m.na) {
+    switch (__builtin_amdgcn_readfirstlane(m.na)) {
         case 1: return yw_scan<1>(c, m, lane);
         case 2: return yw_scan<2>(c, m, lane);
         case 3: return yw_scan<3>(c, m, lane);
@@ -1764,13 +2007,18 @@ __device__ __forceinline__ int yw_scan_n(const YdChain &c, const YkMol &m, int l
     }
 }
 
-template <bool F32>
-__global__ void __launch_bounds__(YW_WPB * 64) yk_wchain(const YdChain *chains, int nchains,
+// OCC: waves per SIMD the register allocation is sized for (1: four
+// replicas per CU; 2: eight, at the cost of some register spills).
+template <bool F32, int OCC>
+__global__ void __launch_bounds__(YW_WPB * 64) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
+yk_wchain(const YdChain *chains, int nchains,
                                                          const double *const *sig6s, const double *const *feps_s,
                                                          int ncycles) {
     __shared__ YwShared shw[YW_WPB];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int ci = blockIdx.x * YW_WPB + wv;
+    // the replica index is uniform over the wave: keep it (and so the chain
+    // descriptor's fields) in scalar registers
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ci = __builtin_amdgcn_readfirstlane(blockIdx.x * YW_WPB + wv);
     if (ci >= nchains) return;  // the whole wave leaves; nothing waits for it
     YwShared &sh = shw[wv];
     const YdChain &c = chains[ci];
@@ -1816,7 +2064,7 @@ __global__ void __launch_bounds__(YW_WPB * 64) yk_wchain(const YdChain *chains, 
             double acc[YK_NACC];
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-            yk_sweep_n<F32, false>(c, m, acc, lane, 64);
+            yl_sweep_n<F32, false>(c, m, acc, lane, 64);
             yw_sum<YK_NACC>(acc);
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) sh.out[i] = acc[i];
@@ -1841,7 +2089,7 @@ __global__ void __launch_bounds__(YW_WPB * 64) yk_wchain(const YdChain *chains, 
 #pragma unroll
             for (int i = 0; i < YK_NRX; ++i) acc[i] = 0.0;
             if (nb <= YK_NBCAP) {
-                yk_rs_exact_n(c, m, sig6, feps, c.nbl, nb, acc, ov, lane, 64);
+                yl_rs_exact_n(c, m, sig6, feps, c.nbl, nb, acc, ov, lane, 64);
             } else {
                 int junk = 0;
                 double a2[YK_NRS];
@@ -1876,7 +2124,7 @@ __global__ void __launch_bounds__(YW_WPB * 64) yk_wchain(const YdChain *chains, 
             double acc[YK_NACC];
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-            yk_sweep_n<F32, true>(c, m, acc, lane, 64);
+            yl_sweep_n<F32, true>(c, m, acc, lane, 64);
             yw_sum<YK_NACC>(acc);
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) sh.out[i] = acc[i];
@@ -1912,8 +2160,11 @@ __global__ void __launch_bounds__(YW_WPB * 64) yk_wchain(const YdChain *chains, 
     }
 }
 
-template __global__ void yk_wchain<false>(const YdChain *, int, const double *const *, const double *const *, int);
-template __global__ void yk_wchain<true>(const YdChain *, int, const double *const *, const double *const *, int);
+// Only the FP64 form of the wave kernel is built: the FP32C form made an
+// illegal memory access on its first GPU launch and stays switched off until
+// the cause is found (FP32C batches run the workgroup kernel).
+template __global__ void yk_wchain<false, 1>(const YdChain *, int, const double *const *, const double *const *, int);
+template __global__ void yk_wchain<false, 2>(const YdChain *, int, const double *const *, const double *const *, int);
 
 // ========================================================================
 // Single-call entry points (one workgroup): the per-function parity surface.

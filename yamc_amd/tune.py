@@ -1,8 +1,9 @@
 """Launch-shape autotuner of the chain kernels.
 
-The library has three launch shapes for a batch of replicas (and for a single
+The library has four launch shapes for a batch of replicas (and for a single
 chain): a workgroup of 256 or 512 threads per chain, or one wavefront per
-replica with four replicas per workgroup (yamc_set_block_size 256, 512, 64).
+replica with four replicas per workgroup and registers sized for four or
+eight replicas per CU (yamc_set_block_size 256, 512, 64, 128).
 Which one is fastest depends on the replica count and size: one chain per
 workgroup spreads every sum over the workgroup (short moves, few chains per
 CU), one wave per replica trades a longer move for more chains in flight.
@@ -18,7 +19,7 @@ from typing import Dict, Sequence, Tuple
 
 from .ewald import EwaldMC, batch_run
 
-SHAPES = (256, 64, 512)
+SHAPES = (256, 64, 128, 512)
 
 
 def tune_batch(reps: Sequence[EwaldMC], shapes: Sequence[int] = SHAPES, cycles: int = 1) -> Tuple[int, Dict[int, float]]:
