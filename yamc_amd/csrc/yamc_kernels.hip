@@ -345,6 +345,7 @@ struct YkShared {
     int nblist[YK_NBCAP];
     unsigned int bm[YK_BMWORDS];
     int wsum[YK_NW];
+    unsigned long long ph[9], t_last;  // phase timers (thread 0)
 };
 
 // Loads molecule imol (old positions from the state) into LDS; thread 0.
@@ -1242,6 +1243,235 @@ __device__ int yk_choose(const YdChain &c, double rnd) {
     return 0;
 }
 
+// ------------------------------------------------------------------------
+// Lean per-move passes of the wave kernel.  The same arithmetic as
+// yk_sweep / yk_rs_exact, split so that fewer values are live at once: the
+// Coulomb set and the LJ sets are swept in separate k loops, and the exact
+// real-space pass visits the molecule's sites one at a time.  A wave then
+// needs fewer registers, so more replicas share a SIMD.
+
+// Coulomb set of a sweep (MC.f90:963-968 / 1136-1173, Coulomb parts):
+// DELTA adds dU_C (acc[0]) and evaluates the forces against S + dS.
+template <bool F32, bool DELTA, int NA>
+__device__ void yl_sweep_coul(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size) {
+    const int nk = c.p.nk;
+    const double *tabs = DELTA ? m.tab_n : m.tab_o;
+    for (int kk = rank; kk < nk; kk += size) {
+        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        double bt = c.beta[kk];
+        double C[8];
+        yk_ld8<F32>(c, 0, kk, C);
+        double kn[NA][8];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            if (m.q[a] != 0.0) yd_kr(tabs + a * 6 * YD_KP, m.q[a], kx, ky, kz, kn[a]);
+        }
+        if (DELTA) {
+            double ns[8], os[8], ko[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) { ns[i] = 0.0; os[i] = 0.0; }
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                if (m.q[a] == 0.0) continue;
+                yd_kr(m.tab_o + a * 6 * YD_KP, m.q[a], kx, ky, kz, ko);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) { ns[i] = ns[i] + kn[a][i]; os[i] = os[i] + ko[i]; }
+            }
+            double dc[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) dc[i] = ns[i] - os[i];
+            acc[0] += yd_du_k(C, dc, ky, kz) * bt;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) C[i] = C[i] + dc[i];
+        }
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            if (m.q[a] == 0.0) continue;
+            double f[3] = {0, 0, 0};
+            yd_force_k(C, kn[a], kx, ky, kz, bt, f);
+            acc[YK_FK(a, 0)] += f[0];
+            acc[YK_FK(a, 1)] += f[1];
+            acc[YK_FK(a, 2)] += f[2];
+        }
+    }
+}
+
+// LJ sets of a sweep: DELTA adds dU_LJ (acc[1]); forces of the LJ sites
+// against every LJ set (+ its delta).  The LJ q-weighted sin/cos of a
+// charged site is its Coulomb one times 1/q (RhoSquared.f90:173-176).
+template <bool F32, bool DELTA, int NA>
+__device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size) {
+    const int nk = c.p.nk, T = c.p.ntype;
+    if (T == 0 || m.np == 0) return;
+    const double *tabs = DELTA ? m.tab_n : m.tab_o;
+    for (int kk = rank; kk < nk; kk += size) {
+        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        double kl[NA][8];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            if (m.typ[a] <= 0) continue;
+            double qa = m.q[a];
+            if (qa != 0.0) {
+                yd_kr(tabs + a * 6 * YD_KP, qa, kx, ky, kz, kl[a]);
+                double inv = 1.0 / qa;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) kl[a][i] = kl[a][i] * inv;
+            } else {
+                yd_kr(tabs + a * 6 * YD_KP, 1.0, kx, ky, kz, kl[a]);
+            }
+        }
+        double dl[YK_MAXP][8];
+        if (DELTA) {
+            double nl[YK_MAXP][8], ol[YK_MAXP][8];
+#pragma unroll
+            for (int q = 0; q < YK_MAXP; ++q)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) { nl[q][i] = 0.0; ol[q][i] = 0.0; }
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                if (m.typ[a] <= 0) continue;
+                double qa = m.q[a], lo[8];
+                if (qa != 0.0) {
+                    yd_kr(m.tab_o + a * 6 * YD_KP, qa, kx, ky, kz, lo);
+                    double inv = 1.0 / qa;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) lo[i] = lo[i] * inv;
+                } else {
+                    yd_kr(m.tab_o + a * 6 * YD_KP, 1.0, kx, ky, kz, lo);
+                }
+#pragma unroll
+                for (int q = 0; q < YK_MAXP; ++q)
+                    if (q == m.lslot[a]) {
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) { nl[q][i] = nl[q][i] + kl[a][i]; ol[q][i] = ol[q][i] + lo[i]; }
+                    }
+            }
+#pragma unroll
+            for (int q = 0; q < YK_MAXP; ++q)
+#pragma unroll
+                for (int i = 0; i < 8; ++i) dl[q][i] = nl[q][i] - ol[q][i];
+        }
+        for (int t1 = 0; t1 < T; ++t1) {
+            double C1[8], bl[YK_MAXP];
+            yk_ld8<F32>(c, 1 + t1, kk, C1);
+#pragma unroll
+            for (int q = 0; q < YK_MAXP; ++q)
+                bl[q] = q < m.np ? c.beta_lj[((size_t)t1 * T + (m.ptype[q] - 1)) * nk + kk] : 0.0;
+            if (DELTA) {
+                int s1 = -1;
+#pragma unroll
+                for (int q = 0; q < YK_MAXP; ++q)
+                    if (q < m.np && m.ptype[q] == t1 + 1) s1 = q;
+                double tw[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    double d1 = 0.0;
+#pragma unroll
+                    for (int q = 0; q < YK_MAXP; ++q)
+                        if (q == s1) d1 = dl[q][i];
+                    tw[i] = C1[i] + C1[i] + d1;
+                    C1[i] = C1[i] + d1;
+                }
+#pragma unroll
+                for (int q = 0; q < YK_MAXP; ++q)
+                    if (q < m.np) acc[1] += yd_dulj_k(tw, dl[q], ky, kz) * bl[q];
+            }
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const int ta = m.typ[a];
+                if (ta <= 0) continue;
+                double b = 0.0;
+#pragma unroll
+                for (int q = 0; q < YK_MAXP; ++q)
+                    if (q == m.lslot[a]) b = bl[q];
+                if (m.lslot[a] < 0) b = c.beta_lj[((size_t)(ta - 1) * T + t1) * nk + kk];
+                double f[3] = {0, 0, 0};
+                yd_force_k(C1, kl[a], kx, ky, kz, b, f);
+                acc[YK_FK(a, 0)] += f[0];
+                acc[YK_FK(a, 1)] += f[1];
+                acc[YK_FK(a, 2)] += f[2];
+            }
+        }
+    }
+}
+
+template <bool F32, bool DELTA>
+__device__ __forceinline__ void yl_sweep_n(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank,
+                                           int size) {
+    switch (__builtin_amdgcn_readfirstlane(m.na)) {
+        case 1: yl_sweep_coul<F32, DELTA, 1>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 1>(c, m, acc, rank, size); break;
+        case 2: yl_sweep_coul<F32, DELTA, 2>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 2>(c, m, acc, rank, size); break;
+        case 3: yl_sweep_coul<F32, DELTA, 3>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 3>(c, m, acc, rank, size); break;
+        default: yl_sweep_coul<F32, DELTA, 4>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 4>(c, m, acc, rank, size); break;
+    }
+}
+
+// Exact real-space pass (same terms and stored back terms as yk_rs_exact):
+// the new sites, whose per-site sums need compile-time accumulator indices,
+// are unrolled; the old sites, which only feed site-independent sums, are
+// visited one at a time.
+template <int NA>
+__device__ void yl_rs_exact(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
+                            const int *list, int count, double (&acc)[YK_NRX], int &ov, int rank, int size) {
+    const int T = c.p.ntype;
+    for (int i = rank; i < count; i += size) {
+        int j = list[i];
+        int jt = c.type[j];
+        double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+        double nb[4] = {0, 0, 0, 0}, ob[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            int it = m.typ[a];
+            double xij = yd_wrap(m.xn[a] - xj), yij = yd_wrap(m.yn[a] - yj), zij = yd_wrap(m.zn[a] - zj);
+            double r2 = xij * xij + yij * yij + zij * zij;
+            double d12 = (m.dh[a] + dhj) / 2.0;
+            if (it > 0 && jt > 0 && r2 < d12 * d12) ov = 1;
+            if (r2 >= c.p.rmax2) continue;
+            bool lj = it != 0 && jt != 0;
+            int off = lj ? (it - 1) * T + (jt - 1) : 0;
+            YdPair pr;
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            yk_rs_acc(acc, a, pr, xij, yij, zij);
+            nb[0] = nb[0] + pr.pot;
+            nb[1] = nb[1] - xij * pr.FR;
+            nb[2] = nb[2] - yij * pr.FR;
+            nb[3] = nb[3] - zij * pr.FR;
+        }
+#pragma unroll 1
+        for (int a = 0; a < NA; ++a) {
+            int it = m.typ[a];
+            double xij = yd_wrap(m.xo[a] - xj), yij = yd_wrap(m.yo[a] - yj), zij = yd_wrap(m.zo[a] - zj);
+            double r2 = xij * xij + yij * yij + zij * zij;
+            if (r2 >= c.p.rmax2) continue;
+            bool lj = it != 0 && jt != 0;
+            int off = lj ? (it - 1) * T + (jt - 1) : 0;
+            YdPair pr;
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            ob[0] = ob[0] + pr.pot;
+            ob[1] = ob[1] - xij * pr.FR;
+            ob[2] = ob[2] - yij * pr.FR;
+            ob[3] = ob[3] - zij * pr.FR;
+            acc[YK_NRS + 0] += pr.potew;
+            acc[YK_NRS + 1] += pr.potlj6;
+            acc[YK_NRS + 2] += pr.potlj12;
+        }
+        double *b = c.rsb + (size_t)i * 8;
+        b[0] = nb[0]; b[1] = nb[1]; b[2] = nb[2]; b[3] = nb[3];
+        b[4] = ob[0]; b[5] = ob[1]; b[6] = ob[2]; b[7] = ob[3];
+    }
+}
+
+__device__ __forceinline__ void yl_rs_exact_n(const YdChain &c, const YkMol &m, const double *sig6,
+                                              const double *feps, const int *list, int count,
+                                              double (&acc)[YK_NRX], int &ov, int rank, int size) {
+    switch (__builtin_amdgcn_readfirstlane(m.na)) {
+        case 1: yl_rs_exact<1>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+        case 2: yl_rs_exact<2>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+        case 3: yl_rs_exact<3>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+        default: yl_rs_exact<4>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+    }
+}
+
 // Phase timers of the chain kernel: thread 0 adds wall-clock ticks between
 // workgroup barriers into st->phase[] (pick, tables, old sweep, proposal,
 // real-space pass, new sweep, acceptance, commit, finalize).
@@ -1249,8 +1479,8 @@ __device__ int yk_choose(const YdChain &c, double rnd) {
     do {                                                            \
         if (tid == 0) {                                             \
             unsigned long long t_ = wall_clock64();                 \
-            ph[i] += t_ - t_last;                                   \
-            t_last = t_;                                            \
+            sh.ph[i] += t_ - sh.t_last;                             \
+            sh.t_last = t_;                                         \
         }                                                           \
     } while (0)
 
@@ -1503,7 +1733,6 @@ __global__ void __launch_bounds__(BS) yk_chain(const YdChain *chains, const doub
         for (int i = tid; i < p.natom; i += (int)blockDim.x)
             lds_pos[i] = make_float4((float)c.x[i], (float)c.y[i], (float)c.z[i], __int_as_float(c.molnum[i]));
     YkScalar &t0 = sh.t0;
-    unsigned long long ph[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, t_last = 0;
     // mu = sum_j q_j r_j from scratch once per launch (EwaldSumExternal.f90:
     // 165-183); afterwards it is carried incrementally, mu + dmu on each
     // accepted move, which is the value movemc recomputes (MC.f90:916).
@@ -1517,7 +1746,8 @@ __global__ void __launch_bounds__(BS) yk_chain(const YdChain *chains, const doub
     }
     if (tid == 0) {
         t0.mu[0] = sh.out[0]; t0.mu[1] = sh.out[1]; t0.mu[2] = sh.out[2];
-        t_last = wall_clock64();
+        for (int i = 0; i < 9; ++i) sh.ph[i] = 0;
+        sh.t_last = wall_clock64();
         t0.acc_count = 0; t0.nmv = 0; t0.nov = 0;
         YdRng &rng = t0.rng;
         rng.key0 = c.st->key[0]; rng.key1 = c.st->key[1];
@@ -1553,7 +1783,7 @@ __global__ void __launch_bounds__(BS) yk_chain(const YdChain *chains, const doub
             double acc[YK_NACC];
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-            yk_sweep_n<F32, false>(c, sh.m, acc, tid, blockDim.x);
+            yl_sweep_n<F32, false>(c, sh.m, acc, tid, blockDim.x);
             yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
         }
         YK_TICK(2);
@@ -1591,7 +1821,7 @@ __global__ void __launch_bounds__(BS) yk_chain(const YdChain *chains, const doub
             }
             __syncthreads();
             if (sh.nbcount <= YK_NBCAP) {
-                yk_rs_exact_n(c, sh.m, sig6, feps, sh.nblist, sh.nbcount, acc, ov, tid, blockDim.x);
+                yl_rs_exact_n(c, sh.m, sig6, feps, sh.nblist, sh.nbcount, acc, ov, tid, blockDim.x);
             } else {  // list overflow: full FP64 pass (the commit then visits every atom)
                 int junk = 0;
                 double a2[YK_NRS];
@@ -1630,7 +1860,7 @@ __global__ void __launch_bounds__(BS) yk_chain(const YdChain *chains, const doub
             double acc[YK_NACC];
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-            yk_sweep_n<F32, true>(c, sh.m, acc, tid, blockDim.x);
+            yl_sweep_n<F32, true>(c, sh.m, acc, tid, blockDim.x);
             yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
         }
         YK_TICK(5);
@@ -1672,7 +1902,7 @@ __global__ void __launch_bounds__(BS) yk_chain(const YdChain *chains, const doub
         c.st->n_accepted += t0.acc_count;
         c.st->n_overlap += t0.nov;
         c.st->since_resync += t0.nmv;
-        for (int i = 0; i < 9; ++i) c.st->phase[i] += ph[i];
+        for (int i = 0; i < 9; ++i) c.st->phase[i] += sh.ph[i];
     }
 }
 
@@ -1683,235 +1913,6 @@ YK_CHAIN_INST(false, 256)
 YK_CHAIN_INST(false, 512)
 YK_CHAIN_INST(true, 256)
 YK_CHAIN_INST(true, 512)
-
-// ------------------------------------------------------------------------
-// Lean per-move passes of the wave kernel.  The same arithmetic as
-// yk_sweep / yk_rs_exact, split so that fewer values are live at once: the
-// Coulomb set and the LJ sets are swept in separate k loops, and the exact
-// real-space pass visits the molecule's sites one at a time.  A wave then
-// needs fewer registers, so more replicas share a SIMD.
-
-// Coulomb set of a sweep (MC.f90:963-968 / 1136-1173, Coulomb parts):
-// DELTA adds dU_C (acc[0]) and evaluates the forces against S + dS.
-template <bool F32, bool DELTA, int NA>
-__device__ void yl_sweep_coul(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size) {
-    const int nk = c.p.nk;
-    const double *tabs = DELTA ? m.tab_n : m.tab_o;
-    for (int kk = rank; kk < nk; kk += size) {
-        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
-        double bt = c.beta[kk];
-        double C[8];
-        yk_ld8<F32>(c, 0, kk, C);
-        double kn[NA][8];
-#pragma unroll
-        for (int a = 0; a < NA; ++a) {
-            if (m.q[a] != 0.0) yd_kr(tabs + a * 6 * YD_KP, m.q[a], kx, ky, kz, kn[a]);
-        }
-        if (DELTA) {
-            double ns[8], os[8], ko[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) { ns[i] = 0.0; os[i] = 0.0; }
-#pragma unroll
-            for (int a = 0; a < NA; ++a) {
-                if (m.q[a] == 0.0) continue;
-                yd_kr(m.tab_o + a * 6 * YD_KP, m.q[a], kx, ky, kz, ko);
-#pragma unroll
-                for (int i = 0; i < 8; ++i) { ns[i] = ns[i] + kn[a][i]; os[i] = os[i] + ko[i]; }
-            }
-            double dc[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) dc[i] = ns[i] - os[i];
-            acc[0] += yd_du_k(C, dc, ky, kz) * bt;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) C[i] = C[i] + dc[i];
-        }
-#pragma unroll
-        for (int a = 0; a < NA; ++a) {
-            if (m.q[a] == 0.0) continue;
-            double f[3] = {0, 0, 0};
-            yd_force_k(C, kn[a], kx, ky, kz, bt, f);
-            acc[YK_FK(a, 0)] += f[0];
-            acc[YK_FK(a, 1)] += f[1];
-            acc[YK_FK(a, 2)] += f[2];
-        }
-    }
-}
-
-// LJ sets of a sweep: DELTA adds dU_LJ (acc[1]); forces of the LJ sites
-// against every LJ set (+ its delta).  The LJ q-weighted sin/cos of a
-// charged site is its Coulomb one times 1/q (RhoSquared.f90:173-176).
-template <bool F32, bool DELTA, int NA>
-__device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size) {
-    const int nk = c.p.nk, T = c.p.ntype;
-    if (T == 0 || m.np == 0) return;
-    const double *tabs = DELTA ? m.tab_n : m.tab_o;
-    for (int kk = rank; kk < nk; kk += size) {
-        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
-        double kl[NA][8];
-#pragma unroll
-        for (int a = 0; a < NA; ++a) {
-            if (m.typ[a] <= 0) continue;
-            double qa = m.q[a];
-            if (qa != 0.0) {
-                yd_kr(tabs + a * 6 * YD_KP, qa, kx, ky, kz, kl[a]);
-                double inv = 1.0 / qa;
-#pragma unroll
-                for (int i = 0; i < 8; ++i) kl[a][i] = kl[a][i] * inv;
-            } else {
-                yd_kr(tabs + a * 6 * YD_KP, 1.0, kx, ky, kz, kl[a]);
-            }
-        }
-        double dl[YK_MAXP][8];
-        if (DELTA) {
-            double nl[YK_MAXP][8], ol[YK_MAXP][8];
-#pragma unroll
-            for (int q = 0; q < YK_MAXP; ++q)
-#pragma unroll
-                for (int i = 0; i < 8; ++i) { nl[q][i] = 0.0; ol[q][i] = 0.0; }
-#pragma unroll
-            for (int a = 0; a < NA; ++a) {
-                if (m.typ[a] <= 0) continue;
-                double qa = m.q[a], lo[8];
-                if (qa != 0.0) {
-                    yd_kr(m.tab_o + a * 6 * YD_KP, qa, kx, ky, kz, lo);
-                    double inv = 1.0 / qa;
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) lo[i] = lo[i] * inv;
-                } else {
-                    yd_kr(m.tab_o + a * 6 * YD_KP, 1.0, kx, ky, kz, lo);
-                }
-#pragma unroll
-                for (int q = 0; q < YK_MAXP; ++q)
-                    if (q == m.lslot[a]) {
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) { nl[q][i] = nl[q][i] + kl[a][i]; ol[q][i] = ol[q][i] + lo[i]; }
-                    }
-            }
-#pragma unroll
-            for (int q = 0; q < YK_MAXP; ++q)
-#pragma unroll
-                for (int i = 0; i < 8; ++i) dl[q][i] = nl[q][i] - ol[q][i];
-        }
-        for (int t1 = 0; t1 < T; ++t1) {
-            double C1[8], bl[YK_MAXP];
-            yk_ld8<F32>(c, 1 + t1, kk, C1);
-#pragma unroll
-            for (int q = 0; q < YK_MAXP; ++q)
-                bl[q] = q < m.np ? c.beta_lj[((size_t)t1 * T + (m.ptype[q] - 1)) * nk + kk] : 0.0;
-            if (DELTA) {
-                int s1 = -1;
-#pragma unroll
-                for (int q = 0; q < YK_MAXP; ++q)
-                    if (q < m.np && m.ptype[q] == t1 + 1) s1 = q;
-                double tw[8];
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    double d1 = 0.0;
-#pragma unroll
-                    for (int q = 0; q < YK_MAXP; ++q)
-                        if (q == s1) d1 = dl[q][i];
-                    tw[i] = C1[i] + C1[i] + d1;
-                    C1[i] = C1[i] + d1;
-                }
-#pragma unroll
-                for (int q = 0; q < YK_MAXP; ++q)
-                    if (q < m.np) acc[1] += yd_dulj_k(tw, dl[q], ky, kz) * bl[q];
-            }
-#pragma unroll
-            for (int a = 0; a < NA; ++a) {
-                const int ta = m.typ[a];
-                if (ta <= 0) continue;
-                double b = 0.0;
-#pragma unroll
-                for (int q = 0; q < YK_MAXP; ++q)
-                    if (q == m.lslot[a]) b = bl[q];
-                if (m.lslot[a] < 0) b = c.beta_lj[((size_t)(ta - 1) * T + t1) * nk + kk];
-                double f[3] = {0, 0, 0};
-                yd_force_k(C1, kl[a], kx, ky, kz, b, f);
-                acc[YK_FK(a, 0)] += f[0];
-                acc[YK_FK(a, 1)] += f[1];
-                acc[YK_FK(a, 2)] += f[2];
-            }
-        }
-    }
-}
-
-template <bool F32, bool DELTA>
-__device__ __forceinline__ void yl_sweep_n(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank,
-                                           int size) {
-    switch (__builtin_amdgcn_readfirstlane(m.na)) {
-        case 1: yl_sweep_coul<F32, DELTA, 1>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 1>(c, m, acc, rank, size); break;
-        case 2: yl_sweep_coul<F32, DELTA, 2>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 2>(c, m, acc, rank, size); break;
-        case 3: yl_sweep_coul<F32, DELTA, 3>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 3>(c, m, acc, rank, size); break;
-        default: yl_sweep_coul<F32, DELTA, 4>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 4>(c, m, acc, rank, size); break;
-    }
-}
-
-// Exact real-space pass (same terms and stored back terms as yk_rs_exact):
-// the new sites, whose per-site sums need compile-time accumulator indices,
-// are unrolled; the old sites, which only feed site-independent sums, are
-// visited one at a time.
-template <int NA>
-__device__ void yl_rs_exact(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
-                            const int *list, int count, double (&acc)[YK_NRX], int &ov, int rank, int size) {
-    const int T = c.p.ntype;
-    for (int i = rank; i < count; i += size) {
-        int j = list[i];
-        int jt = c.type[j];
-        double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
-        double nb[4] = {0, 0, 0, 0}, ob[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int a = 0; a < NA; ++a) {
-            int it = m.typ[a];
-            double xij = yd_wrap(m.xn[a] - xj), yij = yd_wrap(m.yn[a] - yj), zij = yd_wrap(m.zn[a] - zj);
-            double r2 = xij * xij + yij * yij + zij * zij;
-            double d12 = (m.dh[a] + dhj) / 2.0;
-            if (it > 0 && jt > 0 && r2 < d12 * d12) ov = 1;
-            if (r2 >= c.p.rmax2) continue;
-            bool lj = it != 0 && jt != 0;
-            int off = lj ? (it - 1) * T + (jt - 1) : 0;
-            YdPair pr;
-            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
-            yk_rs_acc(acc, a, pr, xij, yij, zij);
-            nb[0] = nb[0] + pr.pot;
-            nb[1] = nb[1] - xij * pr.FR;
-            nb[2] = nb[2] - yij * pr.FR;
-            nb[3] = nb[3] - zij * pr.FR;
-        }
-#pragma unroll 1
-        for (int a = 0; a < NA; ++a) {
-            int it = m.typ[a];
-            double xij = yd_wrap(m.xo[a] - xj), yij = yd_wrap(m.yo[a] - yj), zij = yd_wrap(m.zo[a] - zj);
-            double r2 = xij * xij + yij * yij + zij * zij;
-            if (r2 >= c.p.rmax2) continue;
-            bool lj = it != 0 && jt != 0;
-            int off = lj ? (it - 1) * T + (jt - 1) : 0;
-            YdPair pr;
-            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
-            ob[0] = ob[0] + pr.pot;
-            ob[1] = ob[1] - xij * pr.FR;
-            ob[2] = ob[2] - yij * pr.FR;
-            ob[3] = ob[3] - zij * pr.FR;
-            acc[YK_NRS + 0] += pr.potew;
-            acc[YK_NRS + 1] += pr.potlj6;
-            acc[YK_NRS + 2] += pr.potlj12;
-        }
-        double *b = c.rsb + (size_t)i * 8;
-        b[0] = nb[0]; b[1] = nb[1]; b[2] = nb[2]; b[3] = nb[3];
-        b[4] = ob[0]; b[5] = ob[1]; b[6] = ob[2]; b[7] = ob[3];
-    }
-}
-
-__device__ __forceinline__ void yl_rs_exact_n(const YdChain &c, const YkMol &m, const double *sig6,
-                                              const double *feps, const int *list, int count,
-                                              double (&acc)[YK_NRX], int &ov, int rank, int size) {
-    switch (__builtin_amdgcn_readfirstlane(m.na)) {
-        case 1: yl_rs_exact<1>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
-        case 2: yl_rs_exact<2>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
-        case 3: yl_rs_exact<3>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
-        default: yl_rs_exact<4>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
-    }
-}
 
 // ========================================================================
 // The wave chain kernel (batched replicas): one wavefront per replica,
