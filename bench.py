@@ -56,9 +56,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--replicas", type=int, default=1024)
+    ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--block", type=int, default=0, help="batch launch shape (64, 256, 512); 0 = autotune")
+    ap.add_argument("--block", type=int, default=0, help="batch launch shape (64, 128, 256, 512); 0 = autotune")
     ap.add_argument("--cpu-moves", type=int, default=0, help="moves for the CPU baseline (0 = auto ~15 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32c,batch")
@@ -95,7 +95,7 @@ def main():
 
     from yamc_amd import YAMC_FP32C, YAMC_FP64, config, system
     from yamc_amd.ewald import Batch, EwaldMC
-    from yamc_amd.tune import tune_batch
+    from yamc_amd.tune import tune_batch, tune_single
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.forces_check:
@@ -119,9 +119,9 @@ def main():
         if form not in forms:
             continue
         g = EwaldMC(base, precision=prec)
-        g.set_block_size(256)
         g.mc_init()
         g.set_rng(20261203, 0)
+        shape1, _ = tune_single(g, shapes=(256, 512))  # two ordinary sweeps, one per shape
         for _ in range(args.warmup):
             g.run(1)
         sync()
@@ -136,7 +136,7 @@ def main():
         dt = time.perf_counter() - t0
         moves = args.steps * nmol
         res[form] = dict(moves_per_s=moves / dt, ms_per_step=1e3 * dt / args.steps, kernel_ms=kms,
-                         acceptance=nacc / moves, launch_shape=256)
+                         acceptance=nacc / moves, launch_shape=shape1)
         g.close()
 
     # ------------------------------------------------- fp32c error vs fp64
@@ -286,7 +286,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "bytes_per_move": bpm,
-            "kernel": "yk_wchain" if res[main_form].get("launch_shape") == 64 else "yk_chain",
+            "kernel": "yk_wchain" if (res[main_form].get("launch_shape") in (64, 128) and main_form == "fp64") else "yk_chain",
         },
         "cpu_baseline": cpu,
     }

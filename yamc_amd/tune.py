@@ -22,9 +22,17 @@ from .ewald import EwaldMC, batch_run
 SHAPES = (256, 64, 128, 512)
 
 
-def tune_batch(reps: Sequence[EwaldMC], shapes: Sequence[int] = SHAPES, cycles: int = 1) -> Tuple[int, Dict[int, float]]:
+def shapes_for(precision: int) -> Tuple[int, ...]:
+    """Launch shapes that run distinct kernels for this precision (FP32C
+    chains run the wave shapes as workgroups of 256)."""
+    return (256, 512) if precision == 1 else SHAPES
+
+
+def tune_batch(reps: Sequence[EwaldMC], shapes: Sequence[int] = None, cycles: int = 1) -> Tuple[int, Dict[int, float]]:
     """Returns (best shape, {shape: trial moves/s}) and sets the best shape on
     every replica.  Rates come from the HIP-event time of each launch."""
+    if shapes is None:
+        shapes = shapes_for(reps[0].precision)
     moves = sum(r.sys.nmol for r in reps) * cycles
     rates = {}
     for bs in shapes:
@@ -38,8 +46,10 @@ def tune_batch(reps: Sequence[EwaldMC], shapes: Sequence[int] = SHAPES, cycles: 
     return best, rates
 
 
-def tune_single(rep: EwaldMC, shapes: Sequence[int] = SHAPES, cycles: int = 1) -> Tuple[int, Dict[int, float]]:
+def tune_single(rep: EwaldMC, shapes: Sequence[int] = None, cycles: int = 1) -> Tuple[int, Dict[int, float]]:
     """Same for one chain (yamc_mc_run)."""
+    if shapes is None:
+        shapes = shapes_for(rep.precision)
     rates = {}
     for bs in shapes:
         rep.set_block_size(bs)
