@@ -70,6 +70,9 @@ def main():
                     help="instead of the headline line: CYCLES run_mc cycles (33,968 moves each) of the "
                          "99,504-atom workload in FP64 and in FP32C (FP64 resync every 1000 moves) on one "
                          "RNG stream; rates, per-move dE error until the first differing decision, drift")
+    ap.add_argument("--replica-check", action="store_true",
+                    help="instead of the headline line: 64 uneven FP32C replicas (128-2048 waters, 0-1 M NaCl, "
+                         "280-360 K, seeded) advanced by one captured launch per sweep, launch shape autotuned")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain fp32 streaming copy of MIB MiB (3 launches) and print its byte count; "
                          "used to scale the FETCH_SIZE / WRITE_SIZE counters")
@@ -103,6 +106,9 @@ def main():
         return
     if args.large_check:
         large_check(args)
+        return
+    if args.replica_check:
+        replica_check(args)
         return
     kT_kjmol = system.BOLTZ * 298.0 / system.KCAL_MOL * 4.184
 
@@ -382,6 +388,69 @@ def forces_check(args):
         "data": "synthetic: seeded lattice start, seeded orientations",
     }
     print(json.dumps(out))
+
+
+def replica_check(args):
+    """64 independent replicas of uneven size, composition and temperature
+    (seeded: N_w in [128, 2048], NaCl uniform 0-1 M, T uniform 280-360 K,
+    density 0.0334 /A^3, alpha = 0.2667 /A, kmax from sk = 3.2), FP32C,
+    one captured launch per sweep."""
+    import numpy as np
+
+    from yamc_amd import YAMC_FP32C, replica_system
+    from yamc_amd.ewald import Batch, EwaldMC
+    from yamc_amd.tune import tune_batch
+
+    rng = np.random.default_rng(20261206)
+    reps, sizes, kmaxes, temps = [], [], [], []
+    for i in range(64):
+        nw, conc, T = int(rng.integers(128, 2049)), float(rng.uniform(0.0, 1.0)), float(rng.uniform(280.0, 360.0))
+        s = replica_system(nw, conc, T, seed=1000 + i)
+        r = EwaldMC(s, precision=YAMC_FP32C)
+        r.mc_init()
+        r.set_rng(20261206, i)
+        reps.append(r)
+        sizes.append(s.natom)
+        kmaxes.append(int(s.derived["kmax"]))
+        temps.append(T)
+    shape, tuned = tune_batch(reps)
+    bat = Batch(reps, 1)
+    bat.launch()
+    torch_sync()
+    moves_per_sweep = sum(r.sys.nmol for r in reps)
+    t0 = time.perf_counter()
+    kms = 0.0
+    nacc = 0
+    for _ in range(args.steps):
+        acc, ms = bat.launch()
+        kms += ms
+        nacc += int(acc.sum())
+    torch_sync()
+    dt = time.perf_counter() - t0
+    bat.close()
+    drift = 0.0
+    for r in reps[:8]:
+        inc = r.energies()
+        r.resync()
+        fresh = r.energies()
+        drift = max(drift, max(abs(inc[k] - fresh[k]) / max(1.0, abs(fresh[k])) for k in ("E_kC", "E_kLJ")))
+    out = {
+        "metric": "batched uneven replicas: trial moves/s, FP32C, one captured launch per sweep",
+        "workload": "64 replicas, seeded N_w 128-2048, NaCl 0-1 M, T 280-360 K, rho 0.0334/A^3, rc 12 A",
+        "replicas": 64, "atoms_min": min(sizes), "atoms_max": max(sizes), "atoms_total": sum(sizes),
+        "kmax_min": min(kmaxes), "kmax_max": max(kmaxes), "T_min": min(temps), "T_max": max(temps),
+        "launch_shape": shape, "tuner_moves_per_s": {str(k): v for k, v in tuned.items()},
+        "moves_per_sweep": moves_per_sweep, "steps": args.steps,
+        "moves_per_s": moves_per_sweep * args.steps / dt, "ms_per_sweep": 1e3 * dt / args.steps,
+        "kernel_ms_per_sweep": kms / args.steps, "acceptance": nacc / (moves_per_sweep * args.steps),
+        "kspace_drift_rel_after_run": drift,
+    }
+    print(json.dumps(out))
+
+
+def torch_sync():
+    import torch
+    torch.cuda.synchronize()
 
 
 def large_check(args):

@@ -166,6 +166,28 @@ module yamc_ewald
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx
       end function
+      integer(c_int) function yamc_mc_volume_move(ctx, pressure_kT_A3, log_max_volume_scaling, u, accepted, &
+            box_length) bind(C, name="yamc_mc_volume_move")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: pressure_kT_A3, log_max_volume_scaling
+         type(c_ptr), value :: u
+         integer(c_int32_t), intent(out) :: accepted
+         real(c_double), intent(out) :: box_length
+      end function
+      integer(c_int) function yamc_mc_run_npt(ctx, ncycles, pressure_step_multiplier, pressure_kT_A3, &
+            log_max_volume_scaling, n_accepted, n_vol_steps, n_vol_accepted) bind(C, name="yamc_mc_run_npt")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: ncycles
+         real(c_double), value :: pressure_step_multiplier, pressure_kT_A3, log_max_volume_scaling
+         integer(c_int64_t), intent(out) :: n_accepted, n_vol_steps, n_vol_accepted
+      end function
+      integer(c_int) function yamc_box_length(ctx, box_length) bind(C, name="yamc_box_length")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: box_length
+      end function
       integer(c_int) function yamc_batch_run(ctxs, n, ncycles, n_accepted) bind(C, name="yamc_batch_run")
          import :: c_int, c_ptr, c_int32_t, c_int64_t
          type(c_ptr), intent(in) :: ctxs(*)

@@ -182,6 +182,31 @@ int yamc_mc_run_const(yamc_ctx *ctx, int32_t nmoves, const int32_t *imol, const 
  * (calc_total_energy, EwaldSumKSpace.f90:225-260) in FP64. */
 int yamc_resync(yamc_ctx *ctx);
 
+/* ---- volume steps (NPT) ------------------------------------------------ */
+/* mcvol (MC.f90:701-825): V_new = exp(ln V_old + (2u-1) ln max_volume_scaling),
+ * molecules keep their centres of mass while their atoms' offsets scale
+ * (scale_box.f90:6-62), the box-unit parameters, hard cores, LJ tables
+ * (LJTypes_scale_sigma) and beta_LJ follow the new box, the whole state is
+ * recomputed (mc_calc_total_sums with scale_beta = xclb_new / xclb_old) and
+ * the step is accepted with min(1, exp(-dU - P dV + (nmol+1) ln(V_new/V_old)))
+ * (no acceptance draw after an overlap or when that exceeds 1); a rejected
+ * step restores the old state.  pressure_kT_A3 = P / (k_B T) in kT per
+ * cubic Angstrom (pressure_angstr_kT, MC.f90:721).  u (may be NULL) is a
+ * constant uniform for both draws (RND_CONSTANT); NULL draws them from the
+ * context's stream.  box_length (may be NULL) receives the box after the step
+ * (Angstrom).  Needs yamc_mc_init first. */
+int yamc_mc_volume_move(yamc_ctx *ctx, double pressure_kT_A3, double log_max_volume_scaling, const double *u,
+                        int32_t *accepted, double *box_length);
+/* run_mc cycles with volume steps (runmc.f90:204-268): per cycle one draw
+ * rnd = u (psm + 1); psm > 0 and rnd > psm gives a volume step, otherwise a
+ * cycle of nmol moves (yamc_mc_run).  Counts of accepted moves, volume
+ * steps and accepted volume steps are returned (each may be NULL). */
+int yamc_mc_run_npt(yamc_ctx *ctx, int32_t ncycles, double pressure_step_multiplier, double pressure_kT_A3,
+                    double log_max_volume_scaling, int64_t *n_accepted, int64_t *n_vol_steps,
+                    int64_t *n_vol_accepted);
+/* Current box length in Angstrom (atomic_data%BoxLength). */
+int yamc_box_length(yamc_ctx *ctx, double *box_length);
+
 /* ---- batched replicas -------------------------------------------------- */
 /* Advances n independent contexts (uneven sizes, compositions and
  * temperatures allowed) by ncycles run_mc cycles each (runmc.f90:204-268)

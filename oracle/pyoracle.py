@@ -66,6 +66,11 @@ def load():
         _lib.yo_rng_philox.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64]
         _lib.yo_rng_const.argtypes = [C.c_void_p, C.c_double]
         _lib.yo_kspace_init.argtypes = [C.c_void_p, C.c_double]
+        _lib.yo_volume_move.argtypes = [C.c_void_p, C.c_double, C.c_double, _DP, C.POINTER(C.c_int)]
+        _lib.yo_run_npt.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        _lib.yo_box_length.restype = C.c_double
+        _lib.yo_box_length.argtypes = [C.c_void_p]
     return _lib
 
 
@@ -223,6 +228,28 @@ class Oracle:
         recs = np.zeros(len(im), dtype=MOVE_DTYPE)
         acc = self.lib.yo_run_const(C.c_void_p(self.h), len(im), pim, pu, recs.ctypes.data_as(C.c_void_p))
         return acc, recs
+
+    def volume_move(self, pressure_kT_A3, log_max_volume_scaling, u=None):
+        """MC.f90:701-825 mcvol; u: constant uniform (RND_CONSTANT) or None
+        for the Philox stream.  Returns (accepted, new box length, overlap)."""
+        if u is not None:
+            self.lib.yo_rng_const(C.c_void_p(self.h), float(u))
+        L = C.c_double()
+        ov = C.c_int()
+        acc = self.lib.yo_volume_move(C.c_void_p(self.h), pressure_kT_A3, log_max_volume_scaling, C.byref(L),
+                                      C.byref(ov))
+        return bool(acc), L.value, bool(ov.value)
+
+    def run_npt(self, ncycles, pressure_step_multiplier, pressure_kT_A3, log_max_volume_scaling):
+        """runmc.f90:204-268 with volume steps; returns (accepted moves,
+        volume steps, accepted volume steps)."""
+        a, nv, nva = C.c_int64(), C.c_int64(), C.c_int64()
+        self.lib.yo_run_npt(C.c_void_p(self.h), int(ncycles), float(pressure_step_multiplier), pressure_kT_A3,
+                            log_max_volume_scaling, C.byref(a), C.byref(nv), C.byref(nva))
+        return a.value, nv.value, nva.value
+
+    def box_length(self):
+        return self.lib.yo_box_length(C.c_void_p(self.h))
 
     def forces_refconv(self):
         n = self.sys.natom

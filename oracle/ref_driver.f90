@@ -8,7 +8,9 @@
 !     components, per-atom real-space uu and F, mu, tails
 !   * per-atom k-space forces: RhoSquared_addAtom + ForceKSpace_coulomb/_LJ
 !   * a move trace: movemc with the reference's RND_CONSTANT random mode and a
-!     caller-chosen molecule and constant uniform per move.
+!     caller-chosen molecule and constant uniform per move;
+!   * volume steps: mcvol with a constant uniform, pressure and maximum
+!     volume scaling per step.
 !
 ! Input (free format, file given as the first argument):
 !   composition_file
@@ -17,6 +19,8 @@
 !   nmol lines: species(1-based) x y z theta phi psi
 !   nmoves
 !   nmoves lines: imol(1-based) u
+!   nvol (optional)
+!   nvol lines: u pressure[Pa] max_volume_scaling   (mcvol steps)
 program ref_driver
    use SystemSettings, only : SYSTEM_STRING_LENGTH
    use parameters
@@ -38,8 +42,8 @@ program ref_driver
    Type(TLJTypes), target :: lj_types
    Type(TRhoSquared) :: scratch
    Type(TSinCosKR) :: krc, krl
-   integer :: nmol, i, j, natom, nk, t1, t2, nt, nmoves, imol, species, f, l, uin
-   real(8) :: bl, u, fxc, fyc, fzc, fxl, fyl, fzl
+   integer :: nmol, i, j, natom, nk, t1, t2, nt, nmoves, imol, species, f, l, uin, nvol, ios
+   real(8) :: bl, u, fxc, fyc, fzc, fxl, fyl, fzl, pvol, mvs
    logical :: accepted
    Type(TMolecule), pointer :: mol_ptr
 
@@ -165,6 +169,28 @@ program ref_driver
    call dump_real('FYEND', RealSpace_TotalSum % Fy, natom)
    call dump_real('FZEND', RealSpace_TotalSum % Fz, natom)
    call dump_sums(KSpace_TotalSum % rho_squared_total % sumsincos_coulomb, nk)
+
+   ! volume steps (mcvol) with one constant uniform each: lines of
+   ! u  pressure[Pa]  max_volume_scaling
+   read(uin, *, iostat=ios) nvol
+   if (ios /= 0) nvol = 0
+   write(*, '(A,I10)') 'VOLS', nvol
+   do j = 1, nvol
+      read(uin, *) u, pvol, mvs
+      pressure = pvol
+      max_volume_scaling = mvs
+      call Parameters_recalc(BoxLength)
+      rnd_constant_const = u
+      call mcvol(mol_table, accepted)
+      write(*, '(A,I8,L3,ES26.17)') 'VOL', j, accepted, BoxLength
+      call dump_energies('AFTER')
+      f = mol_table % first_atom(1)
+      l = mol_table % last_atom(nmol)
+      write(*, '(I6)') l - f + 1
+      do i = f, l
+         write(*, '(3ES26.17)') atomic_data % xx(i), atomic_data % yy(i), atomic_data % zz(i)
+      end do
+   end do
    write(*, '(A)') 'END'
 
 contains

@@ -1252,6 +1252,153 @@ int yo_run_const(yo_sys *s, int nmoves, const int32_t *imol, const double *u, yo
     return acc;
 }
 
+/* ------------------------------------------------------------------------ */
+/* MC.f90:701-825 mcvol, with scale_box (scale_box.f90:6-62),
+ * Parameters_recalc at the new length (parameters.f90:355-413) and
+ * LJTypes_scale_sigma (LJTypes.f90:238-252).  The box-unit parameters are
+ * rescaled from the current ones: xclb, kext and dr scale as 1/L, the LJ
+ * tables as sigma^6 ~ L^-6 and sigma^12 ~ L^-12 (four_eps unchanged); alpha,
+ * rmax and kmax are constant in box units.  pressure is in kT/A^3.
+ * rand() order: lambda, then the acceptance draw only when proba <= 1
+ * (the second operand of the reference's .or., MC.f90:804). */
+typedef struct {
+    double L, xclb, kext, minus_two_kext, dr;
+    double *xx, *yy, *zz, *S, *uu, *Fx, *Fy, *Fz, *beta, *lj6, *lj12, *sigma6;
+    double uu_ew, uu_lj6, uu_lj12, uu_ew_intra, uu_lj6_intra, uu_lj12_intra, e_kc, e_klj, mu[3], tail_c, tail6,
+        tail12;
+} yo_saved;
+
+static double *dupd(const double *a, size_t n) {
+    double *b = malloc(sizeof(double) * (n ? n : 1));
+    if (n) memcpy(b, a, sizeof(double) * n);
+    return b;
+}
+
+static void save_state(const yo_sys *s, yo_saved *v) {
+    size_t N = s->natom, T2 = (size_t)s->ntype * s->ntype, nS = (size_t)(1 + s->ntype) * 8 * s->nk;
+    v->L = s->L; v->xclb = s->xclb; v->kext = s->kext; v->minus_two_kext = s->minus_two_kext; v->dr = s->dr;
+    v->xx = dupd(s->xx, N); v->yy = dupd(s->yy, N); v->zz = dupd(s->zz, N);
+    v->S = dupd(s->S, nS); v->uu = dupd(s->uu, N); v->Fx = dupd(s->Fx, N); v->Fy = dupd(s->Fy, N);
+    v->Fz = dupd(s->Fz, N); v->beta = dupd(s->beta, s->nk);
+    v->lj6 = dupd(s->lj6, T2); v->lj12 = dupd(s->lj12, T2); v->sigma6 = dupd(s->sigma6, T2);
+    v->uu_ew = s->uu_ew; v->uu_lj6 = s->uu_lj6; v->uu_lj12 = s->uu_lj12;
+    v->uu_ew_intra = s->uu_ew_intra; v->uu_lj6_intra = s->uu_lj6_intra; v->uu_lj12_intra = s->uu_lj12_intra;
+    v->e_kc = s->e_kc; v->e_klj = s->e_klj;
+    for (int d = 0; d < 3; ++d) v->mu[d] = s->mu[d];
+    v->tail_c = s->tail_c; v->tail6 = s->tail6; v->tail12 = s->tail12;
+}
+
+static void restore_free(yo_sys *s, yo_saved *v, int restore) {
+    size_t N = s->natom, T2 = (size_t)s->ntype * s->ntype, nS = (size_t)(1 + s->ntype) * 8 * s->nk;
+    if (restore) {
+        s->L = v->L; s->xclb = v->xclb; s->kext = v->kext; s->minus_two_kext = v->minus_two_kext; s->dr = v->dr;
+        memcpy(s->xx, v->xx, sizeof(double) * N); memcpy(s->yy, v->yy, sizeof(double) * N);
+        memcpy(s->zz, v->zz, sizeof(double) * N); memcpy(s->S, v->S, sizeof(double) * nS);
+        memcpy(s->uu, v->uu, sizeof(double) * N); memcpy(s->Fx, v->Fx, sizeof(double) * N);
+        memcpy(s->Fy, v->Fy, sizeof(double) * N); memcpy(s->Fz, v->Fz, sizeof(double) * N);
+        memcpy(s->beta, v->beta, sizeof(double) * s->nk);
+        if (T2) {
+            memcpy(s->lj6, v->lj6, sizeof(double) * T2); memcpy(s->lj12, v->lj12, sizeof(double) * T2);
+            memcpy(s->sigma6, v->sigma6, sizeof(double) * T2);
+        }
+        init_beta_lj(s);
+        s->uu_ew = v->uu_ew; s->uu_lj6 = v->uu_lj6; s->uu_lj12 = v->uu_lj12;
+        s->uu_ew_intra = v->uu_ew_intra; s->uu_lj6_intra = v->uu_lj6_intra; s->uu_lj12_intra = v->uu_lj12_intra;
+        s->e_kc = v->e_kc; s->e_klj = v->e_klj;
+        for (int d = 0; d < 3; ++d) s->mu[d] = v->mu[d];
+        s->tail_c = v->tail_c; s->tail6 = v->tail6; s->tail12 = v->tail12;
+    }
+    double *p[] = {v->xx, v->yy, v->zz, v->S, v->uu, v->Fx, v->Fy, v->Fz, v->beta, v->lj6, v->lj12, v->sigma6};
+    for (size_t i = 0; i < sizeof p / sizeof p[0]; ++i) free(p[i]);
+}
+
+int yo_volume_move(yo_sys *s, double pressure, double log_max_volume_scaling, double *new_box_length,
+                   int *overlap_out) {
+    const double L_old = s->L, V_old = L_old * L_old * L_old;
+    double lambda = 2.0 * yo_rand(s) - 1.0;
+    double V_new = exp(log(V_old) + lambda * log_max_volume_scaling);
+    double L_new = pow(V_new, 1.0 / 3.0);
+    double e[10];
+    yo_mc_energies(s, e);
+    double U_old = e[0];
+    yo_saved v;
+    save_state(s, &v);
+    /* scale_box: COMs stay, intramolecular offsets shrink or grow */
+    double sc = L_old / L_new;
+    for (int m = 0; m < s->nmol; ++m) {
+        int f = s->first[m], n = s->last[m] - f + 1;
+        if (n == 1) continue;
+        double c[3];
+        center_of_mass(s->xx + f, s->yy + f, s->zz + f, s->mass + f, n, c);
+        for (int j = f; j < f + n; ++j) {
+            s->xx[j] = (s->xx[j] - c[0]) * sc + c[0];
+            s->yy[j] = (s->yy[j] - c[1]) * sc + c[1];
+            s->zz[j] = (s->zz[j] - c[2]) * sc + c[2];
+        }
+    }
+    /* Parameters_recalc(L_new) */
+    double xclb_old = s->xclb, xclb_new = (xclb_old * L_old) / L_new;
+    s->L = L_new;
+    s->xclb = xclb_new;
+    s->kext = s->kext * (xclb_new / xclb_old);
+    s->minus_two_kext = -2 * s->kext;
+    s->dr = (s->dr * L_old) / L_new;
+    /* LJTypes_scale_sigma + initBetaLJ */
+    double sc2 = sc * sc, sc6 = sc2 * sc2 * sc2;
+    for (int t = 0; t < s->ntype * s->ntype; ++t) {
+        s->sigma6[t] = s->sigma6[t] * sc6;
+        s->lj6[t] = s->lj6[t] * sc6;
+        s->lj12[t] = s->lj12[t] * sc6 * sc6;
+    }
+    init_beta_lj(s);
+    /* mc_calc_total_sums(scale_beta = xclb_new / xclb_old) */
+    double comps[6];
+    int ov = rspace_calc(s, -1, s->xx, s->yy, s->zz, comps, s->uu, s->Fx, s->Fy, s->Fz, 0, 0, 0, 0);
+    s->uu_ew = comps[0]; s->uu_lj6 = comps[1]; s->uu_lj12 = comps[2];
+    s->uu_ew_intra = comps[3]; s->uu_lj6_intra = comps[4]; s->uu_lj12_intra = comps[5];
+    yo_kspace_init(s, xclb_new / xclb_old);
+    yo_mu(s, s->mu);
+    yo_tails(s, &s->tail_c, &s->tail6, &s->tail12);
+    double proba = -1.0;
+    if (!ov) {
+        yo_mc_energies(s, e);
+        double dU = e[0] - U_old, dV = V_new - V_old;
+        double arg = -dU - pressure * dV + (double)(s->nmol + 1) * log(V_new / V_old);
+        proba = exp(arg);
+    }
+    int accepted = proba > 1.0 || yo_rand(s) < proba;
+    restore_free(s, &v, !accepted);
+    if (new_box_length) *new_box_length = s->L;
+    if (overlap_out) *overlap_out = ov;
+    return accepted;
+}
+
+/* runmc.f90:204-268 with volume steps: per cycle one draw decides between
+ * mcvol (rnd > pressure_step_multiplier, psm > 0) and nmol single moves. */
+int yo_run_npt(yo_sys *s, int ncycles, double pressure_step_multiplier, double pressure,
+               double log_max_volume_scaling, int64_t *n_accepted, int64_t *n_vol_steps, int64_t *n_vol_accepted) {
+    int64_t acc = 0, nv = 0, nva = 0;
+    for (int c = 0; c < ncycles; ++c) {
+        double rnd = yo_rand(s) * (pressure_step_multiplier + 1);
+        if (pressure_step_multiplier > 0 && rnd > pressure_step_multiplier) {
+            ++nv;
+            nva += yo_volume_move(s, pressure, log_max_volume_scaling, NULL, NULL);
+        } else {
+            for (int j = 0; j < s->nmol; ++j) {
+                (void)yo_rand(s);
+                int imol = choose_molecule(s);
+                acc += yo_movemc(s, imol, NULL);
+            }
+        }
+    }
+    if (n_accepted) *n_accepted = acc;
+    if (n_vol_steps) *n_vol_steps = nv;
+    if (n_vol_accepted) *n_vol_accepted = nva;
+    return 0;
+}
+
+double yo_box_length(const yo_sys *s) { return s->L; }
+
 /* mc_calc_forces.f90:230-316, with the movemc real-space convention (j > i) */
 void yo_forces_refconv(const yo_sys *s, double *fx, double *fy, double *fz) {
     double mu[3];

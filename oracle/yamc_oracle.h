@@ -116,6 +116,14 @@ int yo_run_const(yo_sys *s, int nmoves, const int32_t *imol, const double *u, yo
 /* Reference-convention total force (mc_calc_forces.f90:230-316 with the
  * movemc real-space convention j>i, quirk Q1 avoided): real + k + external. */
 void yo_forces_refconv(const yo_sys *s, double *fx, double *fy, double *fz);
+/* MC.f90:701-825 mcvol (NPT volume step); pressure in kT/A^3.  Returns 1 if
+ * accepted; *new_box_length = box length after the step. */
+int yo_volume_move(yo_sys *s, double pressure, double log_max_volume_scaling, double *new_box_length,
+                   int *overlap);
+/* runmc.f90:204-268 with volume steps (no exchange). */
+int yo_run_npt(yo_sys *s, int ncycles, double pressure_step_multiplier, double pressure,
+               double log_max_volume_scaling, int64_t *n_accepted, int64_t *n_vol_steps, int64_t *n_vol_accepted);
+double yo_box_length(const yo_sys *s);
 
 #ifdef __cplusplus
 }

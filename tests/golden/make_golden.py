@@ -42,7 +42,8 @@ def write_mol(path, mol):
 def parse(text):
     lines = text.split("\n")
     i = 0
-    out = {"BETALJ": [], "SUMS": [], "MOVES": []}
+    out = {"BETALJ": [], "SUMS": [], "MOVES": [], "VOLS": []}
+    cur = None  # record (move or volume step) that E/positions attach to
     while i < len(lines):
         ln = lines[i].strip()
         if not ln:
@@ -73,21 +74,25 @@ def parse(text):
             if tok[1] == "INIT":
                 out["E_INIT"] = e
             else:
-                out["MOVES"][-1]["E"] = e
+                cur["E"] = e
             i += 3
         elif tok[0] == "FK":
             n = int(tok[1])
             out["FK"] = [[float(v) for v in lines[i + 1 + k].split()] for k in range(n)]
             i += n
         elif tok[0] == "MOVE":
-            out["MOVES"].append({"imol": int(tok[2]), "accepted": tok[3] == "T"})
-        elif tok[0] == "MOVES":
+            cur = {"imol": int(tok[2]), "accepted": tok[3] == "T"}
+            out["MOVES"].append(cur)
+        elif tok[0] == "VOL":
+            cur = {"accepted": tok[2] == "T", "box": float(tok[3])}
+            out["VOLS"].append(cur)
+        elif tok[0] in ("MOVES", "VOLS"):
             pass
         elif tok[0] == "END":
             break
-        elif len(tok) == 1 and out["MOVES"] and "E" in out["MOVES"][-1] and "pos" not in out["MOVES"][-1]:
+        elif len(tok) == 1 and cur is not None and "E" in cur and "pos" not in cur:
             na = int(tok[0])
-            out["MOVES"][-1]["pos"] = [[float(v) for v in lines[i + 1 + k].split()] for k in range(na)]
+            cur["pos"] = [[float(v) for v in lines[i + 1 + k].split()] for k in range(na)]
             i += na
         i += 1
     return out
@@ -110,7 +115,9 @@ def pick_moves(sysm, nmoves, seed):
     raise RuntimeError("could not find non-stalling uniforms")
 
 
-def make_case(name, sysm, nmoves, seed):
+def make_case(name, sysm, nmoves, seed, vols=()):
+    """vols: (u, pressure [Pa], max_volume_scaling) of mcvol steps run after
+    the move trace."""
     imol, u = pick_moves(sysm, nmoves, seed)
     p = sysm.params
     with tempfile.TemporaryDirectory() as td:
@@ -134,6 +141,9 @@ def make_case(name, sysm, nmoves, seed):
             f.write(f"{nmoves}\n")
             for a, b in zip(imol, u):
                 f.write("%d %.17g\n" % (a + 1, b))
+            f.write(f"{len(vols)}\n")
+            for vu, vp, vm in vols:
+                f.write("%.17g %.17g %.17g\n" % (vu, vp, vm))
         r = subprocess.run([DRIVER, "case.in"], cwd=td, capture_output=True, text=True, timeout=120)
         if r.returncode != 0:
             raise RuntimeError(r.stderr)
@@ -171,6 +181,15 @@ def make_case(name, sysm, nmoves, seed):
         fz_end=np.array(g["FZEND"]),
         sums_coulomb_end=np.array(g["SUMS"][(1 + nt) * 8:]).reshape(8, nk),
     )
+    if vols:
+        vs = g["VOLS"]
+        assert len(vs) == len(vols)
+        arrays.update(
+            vol_u=np.array([v[0] for v in vols]), vol_pressure=np.array([v[1] for v in vols]),
+            vol_mvs=np.array([v[2] for v in vols]),
+            vol_accepted=np.array([v["accepted"] for v in vs], np.int32),
+            vol_box=np.array([v["box"] for v in vs]), vol_e=np.array([v["E"] for v in vs]),
+            vol_pos=np.array([v["pos"] for v in vs]))
     path = os.path.join(OUT, f"{name}.npz")
     np.savez_compressed(path, **arrays)
     print(f"{path}: natom={natom} nk={nk} ntype={nt} moves={nmoves} accepted={arrays['move_accepted'].sum()}")
@@ -179,11 +198,13 @@ def make_case(name, sysm, nmoves, seed):
 def main():
     if not os.path.exists(DRIVER):
         raise SystemExit("oracle/_ref/ref_driver is not built (oracle/build_ref.sh)")
-    make_case("ref_small_bias", ys.config("small", seed=31), 40, 1)
-    make_case("ref_small_nobias", ys.config("small_nobias", seed=32), 40, 2)
+    # volume steps: shrink and grow at 1 bar and at 5 kbar, 2 % maximum scaling
+    vols = [(0.30, 1.0e5, 1.02), (0.85, 1.0e5, 1.02), (0.10, 5.0e8, 1.02), (0.65, 1.0e5, 1.05)]
+    make_case("ref_small_bias", ys.config("small", seed=31), 40, 1, vols)
+    make_case("ref_small_nobias", ys.config("small_nobias", seed=32), 40, 2, vols)
     make_case("ref_water64", ys.lattice_system(
         ys.Params(box_length=12.42, temp=298.0, sr=3.2, sk=3.0, alpha=6.4, external_permutivity=70.0, dr_a=0.3,
-                  d_angle_degree=15.0, lambda_f=0.0, lambda_c=0.0), [ys.SPCE], [64], 33), 30, 3)
+                  d_angle_degree=15.0, lambda_f=0.0, lambda_c=0.0), [ys.SPCE], [64], 33), 30, 3, vols)
 
 
 if __name__ == "__main__":

@@ -105,3 +105,30 @@ def check_trace(impl, sysm, g, step):
     fe = np.stack([g["fx_end"], g["fy_end"], g["fz_end"]])
     assert rel(np.stack([fx, fy, fz]), fe, fe) < RTOL
     assert rel(impl.sums()[0], g["sums_coulomb_end"], g["sums_coulomb_end"]) < RTOL
+
+
+def pressure_kT_A3(pressure_pa, temp):
+    """parameters.f90:411: pressure [J/m^3] / kT [J] * 1e-30."""
+    return pressure_pa / (ys.BOLTZ * temp) * 1e-30
+
+
+def check_volume_steps(impl, sysm, g, vstep):
+    """vstep(u, pressure_kT_A3, log_max_volume_scaling) runs one mcvol step
+    with a constant uniform and returns (accepted, box length); compared
+    with the reference after each step (run after the move trace)."""
+    if "vol_u" not in g:
+        return 0
+    T = sysm.params.temp
+    for k in range(len(g["vol_u"])):
+        acc, L = vstep(float(g["vol_u"][k]), pressure_kT_A3(float(g["vol_pressure"][k]), T),
+                       float(np.log(g["vol_mvs"][k])))
+        assert int(acc) == int(g["vol_accepted"][k]), f"volume decision differs at step {k}"
+        assert abs(L - g["vol_box"][k]) <= 1e-12 * g["vol_box"][k], (k, L, g["vol_box"][k])
+        e = impl.energies()
+        ev = energies_vec(e, impl.mu(), impl.tails())
+        ref = g["vol_e"][k]
+        assert rel(ev[:13], ref[:13], ref[:13]) < RTOL, (k, ev[:13] - ref[:13])
+        x, y, z = impl.positions()
+        pos = np.stack([x, y, z], 1)
+        assert rel(pos, g["vol_pos"][k], 1.0) < 1e-13, k
+    return len(g["vol_u"])

@@ -1,8 +1,9 @@
 """The FP64 oracle against golden vectors of the compiled reference Fortran
 (tests/golden/*.npz, written by tests/golden/make_golden.py from
 oracle/_ref/ref_driver).  Pins staging, beta tables, S(k), every energy
-component, per-atom real-space and k-space forces and a movemc trace with
-identical decisions, all within 1e-10 relative (staging within 1e-14)."""
+component, per-atom real-space and k-space forces, a movemc trace and mcvol
+volume steps with identical decisions, all within 1e-10 relative (staging
+within 1e-14)."""
 import numpy as np
 import pytest
 
@@ -39,3 +40,9 @@ def test_oracle_matches_reference(oracle_mod, path):
         return acc
 
     gc.check_trace(v, sysm, g, step)
+
+    def vstep(u, p, lmvs):
+        acc, L, ov = o.volume_move(p, lmvs, u=u)
+        return acc, L
+
+    assert gc.check_volume_steps(v, sysm, g, vstep) == 4

@@ -377,6 +377,11 @@ def test_gpu_matches_reference_golden(lib, case):
 
     gc.check_trace(gpu, sysm, g, step)
 
+    def vstep(u, p, lmvs):
+        return gpu.volume_move(p, lmvs, u=u)
+
+    assert gc.check_volume_steps(gpu, sysm, g, vstep) == 4
+
 
 def test_rdf_counts_match_oracle(lib):
     """yamc_rdf (COM pair counts, mcrdf2.f90 count_distances) equals the CPU
@@ -433,3 +438,82 @@ def test_fp32c_rdf_within_statistical_error(lib):
         z = (a.mean(0) - b.mean(0))[m] / se[m]
         assert np.abs(z).max() < 6.0, (i, j, np.abs(z).max())
         assert (z ** 2).mean() < 2.5, (i, j, (z ** 2).mean())
+
+
+@pytest.mark.parametrize("name,ncyc,psm", [("small", 24, 1.0), ("c1", 4, 1.0)])
+def test_npt_cycles_match_oracle(lib, name, ncyc, psm):
+    """run_mc cycles with volume steps (runmc.f90:204-268, mcvol
+    MC.f90:701-825) on one RNG stream: identical move and volume decisions,
+    box length, energies, positions and sums after the run."""
+    import golden_cases as gc
+    s, g, o = make(name, seed=31)
+    g.mc_init()
+    o.mc_init()
+    g.set_rng(123, 2)
+    o.set_rng(123, 2)
+    p = gc.pressure_kT_A3(1e5, s.params.temp)
+    lmvs = float(np.log(1.03))
+    rg = g.run_npt(ncyc, psm, p, lmvs)
+    ro = o.run_npt(ncyc, psm, p, lmvs)
+    assert rg == ro, (rg, ro)
+    assert ro[1] > 0, "no volume step drawn"
+    assert g.rng_counter() == o.rng_counter()
+    assert abs(g.box_length() - o.box_length()) <= 1e-13 * o.box_length()
+    eg, eo = g.energies(), o.energies()
+    for k in eo:
+        assert rel(eg[k], eo[k], max(1.0, abs(eo[k]))) < RTOL, k
+    for a, b in zip(g.positions(), o.positions()):
+        assert rel(a, b, 1.0) < 1e-12
+    assert rel(g.sums(), o.sums(), np.abs(o.sums()).max()) < RTOL
+
+
+def test_volume_step_rejected_restores_state(lib):
+    """A volume step that cannot be accepted (huge pressure, u near 1 so the
+    box grows) leaves positions, box, energies and sums as they were."""
+    from yamc_amd import config
+    from yamc_amd.ewald import EwaldMC
+    s = config("small", seed=5)
+    g = EwaldMC(s)
+    g.mc_init()
+    e0, x0, S0, L0 = g.energies(), g.positions(), g.sums(), g.box_length()
+    acc, L = g.volume_move(1e3, float(np.log(1.05)), u=0.99)
+    assert not acc and L == L0
+    e1 = g.energies()
+    for k in e0:
+        assert e1[k] == e0[k], k
+    for a, b in zip(g.positions(), x0):
+        assert np.array_equal(a, b)
+    assert np.array_equal(g.sums(), S0)
+    # the chain continues from the restored state like a fresh context
+    g.set_rng(3)
+    h = EwaldMC(s)
+    h.mc_init()
+    h.set_rng(3)
+    assert g.run(2)[0] == h.run(2)[0]
+    assert g.energies() == h.energies()
+
+
+def test_fp32c_npt_state_matches_fresh_sums(lib):
+    """FP32C form: after cycles with volume steps the carried state (box,
+    scaled beta, energies) equals a from-scratch rebuild of the same
+    configuration in the same context (mc_init: beta from the box's own
+    xclb, sums and energies anew)."""
+    import golden_cases as gc
+    from yamc_amd import config
+    from yamc_amd.ewald import EwaldMC
+    s = config("c1", seed=8)
+    g = EwaldMC(s, precision=1)
+    g.mc_init()
+    g.set_rng(77)
+    # 8 cycles on this stream draw 3 volume steps, one accepted (as the oracle runs them in FP64)
+    _, nv, nva = g.run_npt(8, 1.0, gc.pressure_kT_A3(1e5, s.params.temp), float(np.log(1.005)))
+    assert nv > 0 and nva > 0, (nv, nva)
+    assert g.box_length() != s.params.box_length
+    e_carried, b_carried = g.energies(), g.beta()
+    e_fresh, ov = g.mc_init()
+    assert not ov
+    b_fresh = g.beta()
+    for a, b in zip(b_carried, b_fresh):
+        assert rel(a, b, np.abs(b).max()) < 1e-12
+    # carried FP32C energies drift by at most the per-move bound times the moves since the last rebuild
+    assert abs(e_carried["U_total"] - e_fresh["U_total"]) < 8 * 256 * 4.04e-7

@@ -27,7 +27,7 @@ EXPORTS = [
     "yamc_grid", "yamc_kspace_init", "yamc_kspace_get_beta", "yamc_kspace_get_sums", "yamc_kspace_delta",
     "yamc_kspace_forces", "yamc_rspace_full", "yamc_rspace_partial", "yamc_tails", "yamc_external_mu",
     "yamc_mc_init", "yamc_mc_energies", "yamc_mc_real_arrays", "yamc_mc_set_rng", "yamc_mc_get_rng",
-    "yamc_mc_run", "yamc_mc_run_const", "yamc_resync", "yamc_batch_run", "yamc_batch_create", "yamc_batch_launch",
+    "yamc_mc_run", "yamc_mc_run_const", "yamc_resync", "yamc_mc_volume_move", "yamc_mc_run_npt", "yamc_box_length", "yamc_batch_run", "yamc_batch_create", "yamc_batch_launch",
     "yamc_batch_destroy", "yamc_forces", "yamc_rdf",
     "yamc_set_block_size", "yamc_last_kernel_ms", "yamc_phase_times", "yamc_last_error",
 ]

@@ -22,6 +22,9 @@ __global__ void yk_full_sums(YdChain c, const double *tab);
 __global__ void yk_kenergy(YdChain c);
 __global__ void yk_mu(YdChain c, double *mu_out);
 __global__ void yk_tails(YdChain c, const double *lj6, const double *lj12, const int *counts);
+__global__ void yk_scale_box(YdChain c, double sc);
+__global__ void yk_beta_lj(YdChain c, const double *lj6, const double *lj12, const double *beta6,
+                           const double *beta12);
 __global__ void yk_rspace_full(YdChain c, const double *lj_sig6, const double *lj_4eps, double *part, int *ov);
 __global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, double *comps_out);
 template <bool F32, int BS>
@@ -86,6 +89,10 @@ struct yamc_ctx {
     const double **d_sig6s = nullptr, **d_feps = nullptr;
     bool beta_ready = false, state_ready = false;
     int chain_bs = 256;  // launch shape of the chain kernel (yamc_set_block_size)
+    // host copies of the box-dependent inputs, scaled to the current box
+    std::vector<double> hard_core_angstr, lj6, lj12, sigma6;
+    // device copies of the state kept across a volume step (allocated on first use)
+    std::vector<void *> vol_save;
 };
 
 template <typename T>
@@ -210,6 +217,12 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
         TRY(dalloc(c, &c->d_ov, 1)); TRY(dalloc(c, &c->d_io, 64)); TRY(dalloc(c, &c->d_mol, 3 * YD_MAXA));
         TRY(dalloc(c, &c->d_chain, 1)); TRY(dalloc(c, &c->d_sig6s, 1)); TRY(dalloc(c, &c->d_feps, 1));
         // host staging
+        c->hard_core_angstr.assign(d->hard_core_angstr, d->hard_core_angstr + N);
+        if (T > 0) {
+            c->lj6.assign(d->lj6, d->lj6 + T * T);
+            c->lj12.assign(d->lj12, d->lj12 + T * T);
+            c->sigma6.assign(d->sigma6, d->sigma6 + T * T);
+        }
         std::vector<double> dh(N);
         for (int i = 0; i < N; ++i) dh[i] = d->hard_core_angstr[i] / d->box_length;  // EwaldSumRealSpace.f90:359,381
         std::vector<int> kxyz(nk);
@@ -255,6 +268,7 @@ extern "C" int yamc_ctx_destroy(yamc_ctx *c) {
     if (!c) return YAMC_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void *p : c->owned) hipFree(p);
+    for (void *p : c->vol_save) hipFree(p);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -544,16 +558,23 @@ extern "C" int yamc_mc_energies(yamc_ctx *c, double *e) {
     return YAMC_OK;
 }
 
-extern "C" int yamc_mc_init(yamc_ctx *c, double *e, int32_t *overlap) {
-    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+// mc_calc_total_sums (MC.f90:477-585): real space, k space (beta scaled by
+// scale_beta when it is positive, else rebuilt), mu, tails.
+static int calc_total_sums(yamc_ctx *c, double scale_beta, int32_t *overlap) {
     int32_t ov = 0;
     TRY(rspace_full(c, nullptr, &ov));
-    TRY(yamc_kspace_init(c, 0.0, nullptr, nullptr));
+    TRY(yamc_kspace_init(c, scale_beta, nullptr, nullptr));
     double mu[3];
     TRY(yamc_external_mu(c, mu));
     TRY(yamc_tails(c, nullptr, nullptr, nullptr));
     c->state_ready = true;
     if (overlap) *overlap = ov;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_mc_init(yamc_ctx *c, double *e, int32_t *overlap) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    TRY(calc_total_sums(c, 0.0, overlap));
     if (e) TRY(yamc_mc_energies(c, e));
     return YAMC_OK;
 }
@@ -941,5 +962,211 @@ extern "C" int yamc_rdf(yamc_ctx *c, const int32_t *mol_species, int32_t nspecie
     HIPCHK(hipFreeAsync(hist, s));
     TRY(sync(c));
     for (size_t k = 0; k < nh; ++k) counts[k] += h[k];
+    return YAMC_OK;
+}
+
+// ---- volume steps ---------------------------------------------------------
+static int get_state(yamc_ctx *c, YdState *st) {
+    HIPCHK(hipMemcpyAsync(st, c->ch.st, sizeof *st, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+static int put_state(yamc_ctx *c, const YdState *st) {
+    HIPCHK(hipMemcpyAsync(c->ch.st, st, sizeof *st, hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+// Host Philox-4x32-10 on the context's stream (the device generator's word
+// order: word n = lane n & 3 of block n >> 2; 24-bit uniforms, zero skipped).
+static void host_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t *o) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
+        uint32_t h0 = (uint32_t)(p0 >> 32), l0 = (uint32_t)p0, h1 = (uint32_t)(p1 >> 32), l1 = (uint32_t)p1;
+        uint32_t n0 = h1 ^ c1 ^ k0, n1 = l1, n2 = h0 ^ c3 ^ k1, n3 = l0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += W0; k1 += W1;
+    }
+    o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3;
+}
+
+static double host_uniform(const YdState &st, uint64_t *ctr) {
+    for (;;) {
+        uint64_t n = (*ctr)++;
+        uint32_t o[4];
+        host_philox((uint32_t)(n >> 2), (uint32_t)(n >> 34), st.stream[0], st.stream[1], st.key[0], st.key[1], o);
+        uint32_t k = o[n & 3] >> 8;
+        if (k != 0) return (double)k * (1.0 / 16777216.0);
+    }
+}
+
+// Draws one uniform from the context's stream (advancing it) or returns *u.
+static int draw(yamc_ctx *c, const double *u, double *out) {
+    if (u) {
+        *out = *u;
+        return YAMC_OK;
+    }
+    YdState st;
+    TRY(get_state(c, &st));
+    uint64_t ctr = st.ctr;
+    *out = host_uniform(st, &ctr);
+    st.ctr = ctr;
+    return put_state(c, &st);
+}
+
+// Uploads the box-dependent inputs for box length L: hard cores in box units
+// and the LJ tables (host copies, kept scaled to the current box), then
+// rebuilds beta_LJ (EwaldSumKSpace_initBetaLJ, EwaldSumKSpace.f90:357-386).
+static int upload_box_tables(yamc_ctx *c, double L) {
+    const int N = c->natom, T = c->ntype;
+    std::vector<double> dh(N);
+    for (int i = 0; i < N; ++i) dh[i] = c->hard_core_angstr[i] / L;  // EwaldSumRealSpace.f90:359,381
+    HIPCHK(hipMemcpyAsync(c->ch.dh, dh.data(), sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+    if (T > 0) {
+        HIPCHK(hipMemcpyAsync(c->d_sig6, c->sigma6.data(), sizeof(double) * T * T, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_lj6, c->lj6.data(), sizeof(double) * T * T, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_lj12, c->lj12.data(), sizeof(double) * T * T, hipMemcpyHostToDevice, c->stream));
+        long long n = (long long)T * T * c->nk;
+        yk_beta_lj<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->ch, c->d_lj6, c->d_lj12, c->d_beta6,
+                                                                        c->d_beta12);
+        HIPCHK(hipGetLastError());
+    }
+    return sync(c);  // the host vectors are staging for the copies above
+}
+
+// Device buffers saved across a volume step: positions, S(k), per-atom
+// real-space arrays, beta, chain state.
+static int vol_buffers(yamc_ctx *c, std::vector<std::pair<void *, size_t>> &bufs) {
+    const size_t N = c->natom, nS = (size_t)(1 + c->ntype) * 8 * c->nk;
+    bufs = {{c->ch.x, N * 8}, {c->ch.y, N * 8}, {c->ch.z, N * 8}, {c->ch.uu, N * 8}, {c->ch.Fx, N * 8},
+            {c->ch.Fy, N * 8}, {c->ch.Fz, N * 8}, {c->ch.beta, (size_t)c->nk * 8}, {c->ch.st, sizeof(YdState)}};
+    if (c->precision == YAMC_FP32C) {
+        bufs.push_back({c->ch.Shi, nS * 4});
+        bufs.push_back({c->ch.Slo, nS * 4});
+    } else {
+        bufs.push_back({c->ch.S, nS * 8});
+    }
+    if (c->vol_save.empty())
+        for (auto &b : bufs) {
+            void *q = nullptr;
+            HIPCHK(hipMalloc(&q, b.second));
+            c->vol_save.push_back(q);
+        }
+    return YAMC_OK;
+}
+
+extern "C" int yamc_mc_volume_move(yamc_ctx *c, double pressure_kT_A3, double log_max_volume_scaling,
+                                   const double *u, int32_t *accepted, double *box_length) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    if (!c->state_ready) return fail(YAMC_ERROR_INITIALIZATION, "yamc_mc_init has not run on this state");
+    TRY(sync(c));
+    YdParams &p = c->ch.p;
+    const double L_old = p.L, V_old = L_old * L_old * L_old;
+    double r1;
+    TRY(draw(c, u, &r1));
+    const double lambda = 2.0 * r1 - 1.0;  // MC.f90:742-748
+    const double V_new = exp(log(V_old) + lambda * log_max_volume_scaling);
+    const double L_new = pow(V_new, 1.0 / 3.0);
+    double e[10];
+    TRY(yamc_mc_energies(c, e));
+    const double U_old = e[0];
+    // mc_save_state
+    std::vector<std::pair<void *, size_t>> bufs;
+    TRY(vol_buffers(c, bufs));
+    for (size_t i = 0; i < bufs.size(); ++i)
+        HIPCHK(hipMemcpyAsync(c->vol_save[i], bufs[i].first, bufs[i].second, hipMemcpyDeviceToDevice, c->stream));
+    const YdParams p_old = p;
+    const std::vector<double> lj6_old = c->lj6, lj12_old = c->lj12, sig6_old = c->sigma6;
+    // scale_box + Parameters_recalc(L_new) + LJTypes_scale_sigma + initBetaLJ
+    // (MC.f90:760-770): sigma scales by sc, so sigma^6 and LJ6 by sc^6, LJ12 by sc^12
+    const double sc = L_old / L_new;
+    yk_scale_box<<<(c->nmol + 255) / 256, 256, 0, c->stream>>>(c->ch, sc);
+    HIPCHK(hipGetLastError());
+    const double xclb_new = (p.xclb * L_old) / L_new;
+    p.L = L_new;
+    p.kext = p.kext * (xclb_new / p.xclb);
+    p.m2kext = -2 * p.kext;
+    p.dr = (p.dr * L_old) / L_new;
+    const double scale_beta = xclb_new / p.xclb;
+    p.xclb = xclb_new;
+    const double sc2 = sc * sc, sc6 = sc2 * sc2 * sc2;
+    for (size_t t = 0; t < c->lj6.size(); ++t) {
+        c->sigma6[t] = c->sigma6[t] * sc6;
+        c->lj6[t] = c->lj6[t] * sc6;
+        c->lj12[t] = c->lj12[t] * sc6 * sc6;
+    }
+    TRY(upload_box_tables(c, L_new));
+    // mc_calc_total_sums(scale_beta = xclb_new / xclb_old)
+    int32_t ov = 0;
+    TRY(calc_total_sums(c, scale_beta, &ov));
+    double proba = -1.0;
+    if (!ov) {
+        TRY(yamc_mc_energies(c, e));
+        const double dU = e[0] - U_old, dV = V_new - V_old;
+        const double arg = -dU - pressure_kT_A3 * dV + (double)(c->nmol + 1) * log(V_new / V_old);
+        proba = exp(arg);
+    }
+    int acc = proba > 1.0;
+    if (!acc) {  // the acceptance draw is taken only when proba <= 1 (MC.f90:804)
+        double r2;
+        TRY(draw(c, u, &r2));
+        acc = r2 < proba;
+    }
+    if (!acc) {  // mc_restore_state and the old box's parameters (MC.f90:806-812)
+        YdState st, sv;
+        TRY(get_state(c, &st));  // the generator keeps its position
+        for (size_t i = 0; i < bufs.size(); ++i)
+            HIPCHK(hipMemcpyAsync(bufs[i].first, c->vol_save[i], bufs[i].second, hipMemcpyDeviceToDevice, c->stream));
+        TRY(get_state(c, &sv));
+        sv.ctr = st.ctr;
+        TRY(put_state(c, &sv));
+        p = p_old;
+        c->lj6 = lj6_old;
+        c->lj12 = lj12_old;
+        c->sigma6 = sig6_old;
+        TRY(upload_box_tables(c, L_old));
+    }
+    HIPCHK(hipMemcpyAsync(c->d_chain, &c->ch, sizeof(YdChain), hipMemcpyHostToDevice, c->stream));
+    TRY(sync(c));
+    if (accepted) *accepted = acc;
+    if (box_length) *box_length = p.L;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_mc_run_npt(yamc_ctx *c, int32_t ncycles, double pressure_step_multiplier,
+                               double pressure_kT_A3, double log_max_volume_scaling, int64_t *n_accepted,
+                               int64_t *n_vol_steps, int64_t *n_vol_accepted) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    if (!c->state_ready) return fail(YAMC_ERROR_INITIALIZATION, "yamc_mc_init has not run on this state");
+    int64_t nacc = 0, nv = 0, nva = 0;
+    for (int cyc = 0; cyc < ncycles; ++cyc) {
+        // runmc.f90:206-208: one draw per cycle chooses a volume step or a
+        // cycle of moves; the move kernel takes that draw itself, so it is
+        // only looked at here
+        YdState st;
+        TRY(get_state(c, &st));
+        uint64_t ctr = st.ctr;
+        const double rnd = host_uniform(st, &ctr) * (pressure_step_multiplier + 1);
+        if (pressure_step_multiplier > 0 && rnd > pressure_step_multiplier) {
+            st.ctr = ctr;
+            TRY(put_state(c, &st));
+            int32_t a = 0;
+            TRY(yamc_mc_volume_move(c, pressure_kT_A3, log_max_volume_scaling, nullptr, &a, nullptr));
+            ++nv;
+            nva += a;
+        } else {
+            int64_t n = 0;
+            TRY(yamc_mc_run(c, 1, 0, nullptr, &n));
+            nacc += n;
+        }
+    }
+    if (n_accepted) *n_accepted = nacc;
+    if (n_vol_steps) *n_vol_steps = nv;
+    if (n_vol_accepted) *n_vol_accepted = nva;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_box_length(yamc_ctx *c, double *box_length) {
+    if (!c || !box_length) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    *box_length = c->ch.p.L;
     return YAMC_OK;
 }

@@ -111,6 +111,28 @@ __global__ void yk_scale_beta(YdChain c, double s) {
     if (kk < c.p.nk) c.beta[kk] = c.beta[kk] * s;
 }
 
+// scale_box.f90:6-62: every molecule of more than one atom keeps its centre
+// of mass (box units) while its atoms' offsets from it scale by sc = L_old /
+// L_new, so that bond lengths in Angstrom are preserved.  Thread per molecule.
+__global__ void yk_scale_box(YdChain c, double sc) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= c.p.nmol) return;
+    int f = c.first[m], l = c.last[m];
+    if (l == f) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0, tm = 0.0;  // geometry.f90 center_of_mass
+    for (int a = f; a <= l; ++a) {
+        double w = c.mass[a];
+        sx = sx + w * c.x[a]; sy = sy + w * c.y[a]; sz = sz + w * c.z[a];
+        tm = tm + w;
+    }
+    double cx = sx / tm, cy = sy / tm, cz = sz / tm;
+    for (int a = f; a <= l; ++a) {
+        c.x[a] = (c.x[a] - cx) * sc + cx;
+        c.y[a] = (c.y[a] - cy) * sc + cy;
+        c.z[a] = (c.z[a] - cz) * sc + cz;
+    }
+}
+
 // SumSinCosKR.f90:181-223 for every atom: tab[i][6][YD_KP].
 __global__ void yk_atom_tables(YdChain c, const double *x, const double *y, const double *z, double *tab) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -213,6 +213,30 @@ class EwaldMC:
     def resync(self):
         check(self.lib.yamc_resync(self.ctx))
 
+    def volume_move(self, pressure_kT_A3: float, log_max_volume_scaling: float, u: Optional[float] = None):
+        """mcvol (MC.f90:701-825): returns (accepted, box length in Angstrom)."""
+        acc = C.c_int32()
+        L = C.c_double()
+        up = C.byref(C.c_double(u)) if u is not None else None
+        check(self.lib.yamc_mc_volume_move(self.ctx, C.c_double(pressure_kT_A3), C.c_double(log_max_volume_scaling),
+                                           up, C.byref(acc), C.byref(L)))
+        return bool(acc.value), L.value
+
+    def run_npt(self, ncycles: int, pressure_step_multiplier: float, pressure_kT_A3: float,
+                log_max_volume_scaling: float):
+        """run_mc cycles with volume steps (runmc.f90:204-268): returns
+        (accepted moves, volume steps, accepted volume steps)."""
+        a, v, va = C.c_int64(), C.c_int64(), C.c_int64()
+        check(self.lib.yamc_mc_run_npt(self.ctx, int(ncycles), C.c_double(pressure_step_multiplier),
+                                       C.c_double(pressure_kT_A3), C.c_double(log_max_volume_scaling),
+                                       C.byref(a), C.byref(v), C.byref(va)))
+        return a.value, v.value, va.value
+
+    def box_length(self) -> float:
+        L = C.c_double()
+        check(self.lib.yamc_box_length(self.ctx, C.byref(L)))
+        return L.value
+
     def forces(self):
         """-dE/dr of the full energy and the reference force convention."""
         n = self.sys.natom
@@ -234,8 +258,7 @@ class EwaldMC:
             counts: Optional[np.ndarray] = None) -> np.ndarray:
         """yamc_rdf: adds this configuration's COM pair counts (mcrdf2.f90
         count_distances, cnt4) into counts[nspecies, nspecies, nbins]."""
-        L = self.sys.params.box_length
-        rmax = 0.5 * L if rmax is None else rmax
+        rmax = 0.5 * self.box_length() if rmax is None else rmax
         nbins = int((rmax - 1e-10) / dr) + 1
         if counts is None:
             counts = np.zeros((nspecies, nspecies, nbins), dtype=np.uint64)
