@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
 MOVE_DTYPE = np.dtype([
-    ("imol", "<i4"), ("accepted", "<i4"), ("overlap", "<i4"), ("pad", "<i4"),
+    ("imol", "<i4"), ("accepted", "<i4"), ("overlap", "<i4"), ("imol2", "<i4"),
     ("du_real", "<f8"), ("du_kc", "<f8"), ("du_klj", "<f8"), ("du_ext", "<f8"), ("du_total", "<f8"),
     ("f_old", "<f8", (3,)), ("t_old", "<f8", (3,)), ("f_new", "<f8", (3,)), ("t_new", "<f8", (3,)),
     ("delta", "<f8", (3,)), ("angle", "<f8"),
@@ -71,6 +71,8 @@ def load():
                                     C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         _lib.yo_box_length.restype = C.c_double
         _lib.yo_box_length.argtypes = [C.c_void_p]
+        _lib.yo_set_frequencies.argtypes = [C.c_void_p, _DP, _DP]
+        _lib.yo_xchange.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     return _lib
 
 
@@ -247,6 +249,24 @@ class Oracle:
         self.lib.yo_run_npt(C.c_void_p(self.h), int(ncycles), float(pressure_step_multiplier), pressure_kT_A3,
                             log_max_volume_scaling, C.byref(a), C.byref(nv), C.byref(nva))
         return a.value, nv.value, nva.value
+
+    def set_frequencies(self, move=None, xchange=None):
+        """runmc.f90:26-66 init_freq with per-molecule weights (None = the
+        'none' file: moves 1, exchanges 0)."""
+        mv = None if move is None else np.ascontiguousarray(move, dtype=np.float64)
+        xc = None if xchange is None else np.ascontiguousarray(xchange, dtype=np.float64)
+        self._freq = (mv, xc)  # keep alive for the call
+        self.lib.yo_set_frequencies(C.c_void_p(self.h), None if mv is None else _d(mv)[1],
+                                    None if xc is None else _d(xc)[1])
+
+    def xchange(self, imol1, imol2, u=None):
+        """MC.f90:1325-1954 mc_xchange; u: constant uniform or None for the
+        Philox stream.  Returns (accepted, record)."""
+        if u is not None:
+            self.lib.yo_rng_const(C.c_void_p(self.h), float(u))
+        rec = np.zeros(1, dtype=MOVE_DTYPE)
+        acc = self.lib.yo_xchange(C.c_void_p(self.h), int(imol1), int(imol2), rec.ctypes.data_as(C.c_void_p))
+        return bool(acc), rec[0]
 
     def box_length(self):
         return self.lib.yo_box_length(C.c_void_p(self.h))

@@ -9,6 +9,8 @@
 !   * per-atom k-space forces: RhoSquared_addAtom + ForceKSpace_coulomb/_LJ
 !   * a move trace: movemc with the reference's RND_CONSTANT random mode and a
 !     caller-chosen molecule and constant uniform per move;
+!   * exchange steps: mc_xchange of given molecule pairs with a constant
+!     uniform each, after the volume steps;
 !   * volume steps: mcvol with a constant uniform, pressure and maximum
 !     volume scaling per step.
 !
@@ -21,6 +23,8 @@
 !   nmoves lines: imol(1-based) u
 !   nvol (optional)
 !   nvol lines: u pressure[Pa] max_volume_scaling   (mcvol steps)
+!   nxchg (optional)
+!   nxchg lines: imol1 imol2 u   (mc_xchange steps, 1-based molecules)
 program ref_driver
    use SystemSettings, only : SYSTEM_STRING_LENGTH
    use parameters
@@ -42,7 +46,7 @@ program ref_driver
    Type(TLJTypes), target :: lj_types
    Type(TRhoSquared) :: scratch
    Type(TSinCosKR) :: krc, krl
-   integer :: nmol, i, j, natom, nk, t1, t2, nt, nmoves, imol, species, f, l, uin, nvol, ios
+   integer :: nmol, i, j, natom, nk, t1, t2, nt, nmoves, imol, species, f, l, uin, nvol, ios, imol2, nxchg
    real(8) :: bl, u, fxc, fyc, fzc, fxl, fyl, fzl, pvol, mvs
    logical :: accepted
    Type(TMolecule), pointer :: mol_ptr
@@ -191,6 +195,32 @@ program ref_driver
          write(*, '(3ES26.17)') atomic_data % xx(i), atomic_data % yy(i), atomic_data % zz(i)
       end do
    end do
+
+   ! exchange steps (mc_xchange) with one constant uniform each: lines of
+   ! imol1 imol2 u (1-based molecule numbers)
+   read(uin, *, iostat=ios) nxchg
+   if (ios /= 0) nxchg = 0
+   write(*, '(/,A,I10)') 'XCHGS', nxchg
+   do j = 1, nxchg
+      read(uin, *) imol, imol2, u
+      rnd_constant_const = u
+      call mc_xchange(imol, imol2, mol_table, accepted)
+      write(*, '(/,A,3I8,L3)') 'XCHG', j, imol, imol2, accepted
+      call dump_energies('AFTER')
+      f = mol_table % first_atom(1)
+      l = mol_table % last_atom(nmol)
+      write(*, '(I6)') l - f + 1
+      do i = f, l
+         write(*, '(3ES26.17)') atomic_data % xx(i), atomic_data % yy(i), atomic_data % zz(i)
+      end do
+   end do
+   if (nxchg > 0) then
+      call dump_real('UUX', RealSpace_TotalSum % uu, natom)
+      call dump_real('FXX', RealSpace_TotalSum % Fx, natom)
+      call dump_real('FYX', RealSpace_TotalSum % Fy, natom)
+      call dump_real('FZX', RealSpace_TotalSum % Fz, natom)
+      call dump_sums(KSpace_TotalSum % rho_squared_total % sumsincos_coulomb, nk)
+   end if
    write(*, '(A)') 'END'
 
 contains

@@ -39,7 +39,9 @@ struct yo_sys {
     double *uu, *Fx, *Fy, *Fz;
     double uu_ew, uu_lj6, uu_lj12, uu_ew_intra, uu_lj6_intra, uu_lj12_intra;
     double e_kc, e_klj, mu[3], tail_c, tail6, tail12;
-    double *freq_end; /* BiasedRandom.f90:172-180 cumulative ends, flat mv_num */
+    /* BiasedRandom.f90:41-75 intervals of the move and exchange pickers
+     * (runmc.f90:26-66), and xchange_probability */
+    double *freq_beg, *freq_end, *xfreq_beg, *xfreq_end, pxch;
     /* rng */
     int rng_mode;
     uint32_t key[2], stream[2];
@@ -619,9 +621,11 @@ static void pair_terms(const yo_sys *s, double r2, int same_mol, double qi, doub
 }
 
 /* EwaldSumRealSpace.f90:233-500; full when imol < 0.  comps[6] out. */
-static int rspace_calc(const yo_sys *s, int imol, const double *xn, const double *yn, const double *zn,
-                       double *comps, double *uu, double *fx, double *fy, double *fz, double *uub, double *fxb,
-                       double *fyb, double *fzb) {
+/* jlo >= 0 restricts j to jlo..jhi (the optional first/last arguments,
+ * EwaldSumRealSpace.f90:338-339, used for molecule-molecule terms). */
+static int rspace_calc_range(const yo_sys *s, int imol, const double *xn, const double *yn, const double *zn,
+                             int jlo, int jhi, double *comps, double *uu, double *fx, double *fy, double *fz,
+                             double *uub, double *fxb, double *fyb, double *fzb) {
     int full = imol < 0, N = s->natom, T = s->ntype;
     int first = full ? 0 : s->first[imol];
     int natom = full ? N : s->last[imol] - first + 1;
@@ -640,11 +644,12 @@ static int rspace_calc(const yo_sys *s, int imol, const double *xn, const double
     }
     for (int c = 0; c < 6; ++c) comps[c] = 0.0;
     for (int i = 0; i < natom; ++i) {
-        int j0 = full ? i + 1 : 0;
+        int j0 = full ? i + 1 : 0, j1 = N - 1;
+        if (jlo >= 0) { j0 = jlo; j1 = jhi; }
         int ii = full ? i : first + i;
         int im = s->molnum[ii], it = s->type[ii];
         double qi = s->charge[ii], dhi = s->hard_core[ii] / s->L;
-        for (int j = j0; j < N; ++j) {
+        for (int j = j0; j <= j1; ++j) {
             int jm = s->molnum[j], jt = s->type[j];
             if (!full && im == jm) continue;
             double qj = s->charge[j], dhj = s->hard_core[j] / s->L;
@@ -684,6 +689,12 @@ static int rspace_calc(const yo_sys *s, int imol, const double *xn, const double
         }
     }
     return 0;
+}
+
+static int rspace_calc(const yo_sys *s, int imol, const double *xn, const double *yn, const double *zn,
+                       double *comps, double *uu, double *fx, double *fy, double *fz, double *uub, double *fxb,
+                       double *fyb, double *fzb) {
+    return rspace_calc_range(s, imol, xn, yn, zn, -1, -1, comps, uu, fx, fy, fz, uub, fxb, fyb, fzb);
 }
 
 int yo_rspace_full(yo_sys *s, double *comps, double *uu, double *fx, double *fy, double *fz) {
@@ -1054,6 +1065,7 @@ int yo_movemc(yo_sys *s, int imol, yo_move_rec *rec) {
     yo_move_rec r;
     memset(&r, 0, sizeof r);
     r.imol = imol;
+    r.imol2 = -1;
     yo_mu(s, s->mu); /* MC.f90:916 */
     int f = s->first[imol], last = s->last[imol], na = last - f + 1;
     const double *xo = s->xx + f, *yo = s->yy + f, *zo = s->zz + f;
@@ -1213,28 +1225,217 @@ int yo_movemc(yo_sys *s, int imol, yo_move_rec *rec) {
     return acc;
 }
 
-/* BiasedRandom.f90:192-220 with the flat move frequencies of runmc.f90:41-56 */
-static int choose_molecule(yo_sys *s) {
-    double rnd = yo_rand(s);
-    double beg = 0.0;
-    for (int i = 0; i < s->nmol; ++i) {
-        double end = s->freq_end[i];
-        if (rnd > beg && rnd < end) return i;
-        beg = end;
+/* BiasedRandom.f90:41-75 BiasedRandom_init: cumulative intervals of
+ * num/sum(num) (flat when the sum is 0); a zero entry gets end = -1 so that
+ * it is never chosen. */
+static void biased_init(int n, const double *num, double *beg, double *end) {
+    double S = 0.0;
+    for (int i = 0; i < n; ++i) S = S + num[i];
+    for (int i = 0; i < n; ++i) {
+        double prob = S == 0 ? 1.0 / (double)n : num[i] / S;
+        beg[i] = i == 0 ? 0.0 : end[i - 1];
+        end[i] = beg[i] + prob;
     }
+    if (S > 0)
+        for (int i = 0; i < n; ++i)
+            if (num[i] == 0) end[i] = -1.0;
+}
+
+/* BiasedRandom.f90:80-109 BiasedRandom_choose: first interval with
+ * beg < rnd < end, else the first one. */
+static int biased_choose(yo_sys *s, const double *beg, const double *end) {
+    double rnd = yo_rand(s);
+    for (int i = 0; i < s->nmol; ++i)
+        if (rnd > beg[i] && rnd < end[i]) return i;
     return 0;
 }
 
-/* runmc.f90:204-268 (NVT, no exchange: pressure_step_multiplier = 0) */
+static int choose_molecule(yo_sys *s) { return biased_choose(s, s->freq_beg, s->freq_end); }
+
+/* runmc.f90:26-66 init_freq with per-molecule move and exchange weights
+ * (read_freq_file columns 2 and 3); NULL = all ones / all zeros. */
+int yo_set_frequencies(yo_sys *s, const double *mv_num, const double *xchange_num) {
+    int n = s->nmol;
+    double *mv = malloc(sizeof(double) * n), *xc = malloc(sizeof(double) * n);
+    double smv = 0.0, sxc = 0.0;
+    for (int i = 0; i < n; ++i) {
+        mv[i] = mv_num ? mv_num[i] : 1.0;
+        xc[i] = xchange_num ? xchange_num[i] : 0.0;
+    }
+    for (int i = 0; i < n; ++i) smv = smv + mv[i];
+    for (int i = 0; i < n; ++i) sxc = sxc + xc[i];
+    biased_init(n, mv, s->freq_beg, s->freq_end);
+    biased_init(n, xc, s->xfreq_beg, s->xfreq_end);
+    s->pxch = sxc / (smv + sxc);
+    free(mv);
+    free(xc);
+    return 0;
+}
+
+/* MC.f90:1325-1954 mc_xchange: molecules i1 and i2 swap centres of mass
+ * (each keeps its orientation); three partial real sums with both at the
+ * new places (new1 over j not in i1, new2 over j not in i2, new12 over the
+ * pair only), k-space dU of the two molecules' joint delta, external dU,
+ * accepted iff rand() <= exp(-dU) (MonteCarloMove.f90:572-597); no
+ * acceptance draw after an overlap of new1 or new2.  The acceptance dU_real
+ * is sum(uu new) - sum(uu old) over both molecules, which counts the pair
+ * term twice; the stored components subtract it once (MC.f90:1829-1859). */
+int yo_xchange(yo_sys *s, int i1, int i2, yo_move_rec *rec) {
+    const int N = s->natom, nk = s->nk;
+    const size_t n8 = 8 * (size_t)nk, nS = (size_t)(1 + s->ntype) * n8;
+    yo_move_rec r;
+    memset(&r, 0, sizeof r);
+    r.imol = i1;
+    r.imol2 = i2;
+    yo_mu(s, s->mu); /* MC.f90:1429 */
+    const int f1 = s->first[i1], n1 = s->last[i1] - f1 + 1, f2 = s->first[i2], n2 = s->last[i2] - f2 + 1;
+    double xo1[8], yo1[8], zo1[8], xo2[8], yo2[8], zo2[8];
+    for (int a = 0; a < n1; ++a) { xo1[a] = s->xx[f1 + a]; yo1[a] = s->yy[f1 + a]; zo1[a] = s->zz[f1 + a]; }
+    for (int a = 0; a < n2; ++a) { xo2[a] = s->xx[f2 + a]; yo2[a] = s->yy[f2 + a]; zo2[a] = s->zz[f2 + a]; }
+    double u1 = 0.0, u2 = 0.0;
+    for (int a = 0; a < n1; ++a) u1 = u1 + s->uu[f1 + a];
+    for (int a = 0; a < n2; ++a) u2 = u2 + s->uu[f2 + a];
+    const double U_old = u1 + u2;
+    /* old joint sums, atoms of i1 then of i2 (MC.f90:1464-1488) */
+    memset(s->mol_old, 0, sizeof(double) * nS);
+    for (int a = 0; a < n1; ++a)
+        add_atom(s, s->mol_old, NULL, xo1[a], yo1[a], zo1[a], s->charge[f1 + a], s->type[f1 + a], s->krc, s->krl);
+    for (int a = 0; a < n2; ++a)
+        add_atom(s, s->mol_old, NULL, xo2[a], yo2[a], zo2[a], s->charge[f2 + a], s->type[f2 + a], s->krc, s->krl);
+    double muo1[3] = {0, 0, 0}, muo2[3] = {0, 0, 0};
+    for (int a = 0; a < n1; ++a) {
+        double q = s->charge[f1 + a];
+        muo1[0] = muo1[0] + q * xo1[a]; muo1[1] = muo1[1] + q * yo1[a]; muo1[2] = muo1[2] + q * zo1[a];
+    }
+    for (int a = 0; a < n2; ++a) {
+        double q = s->charge[f2 + a];
+        muo2[0] = muo2[0] + q * xo2[a]; muo2[1] = muo2[1] + q * yo2[a]; muo2[2] = muo2[2] + q * zo2[a];
+    }
+    /* swap the centres of mass (MC.f90:1514-1531) */
+    double c1[3], c2[3];
+    center_of_mass(xo1, yo1, zo1, s->mass + f1, n1, c1);
+    center_of_mass(xo2, yo2, zo2, s->mass + f2, n2, c2);
+    double xn1[8], yn1[8], zn1[8], xn2[8], yn2[8], zn2[8];
+    for (int a = 0; a < n1; ++a) {
+        xn1[a] = xo1[a] - c1[0] + c2[0]; yn1[a] = yo1[a] - c1[1] + c2[1]; zn1[a] = zo1[a] - c1[2] + c2[2];
+    }
+    for (int a = 0; a < n2; ++a) {
+        xn2[a] = xo2[a] - c2[0] + c1[0]; yn2[a] = yo2[a] - c2[1] + c1[1]; zn2[a] = zo2[a] - c2[2] + c1[2];
+    }
+    /* new partial sums with both molecules at their new places (MC.f90:1547-1584) */
+    double *bn1 = malloc(sizeof(double) * 4 * N), *bn2 = malloc(sizeof(double) * 4 * N);
+    double *bo1 = malloc(sizeof(double) * 4 * N), *bo2 = malloc(sizeof(double) * 4 * N);
+    double *b12 = malloc(sizeof(double) * 4 * N);
+    for (int a = 0; a < n1; ++a) { s->xx[f1 + a] = xn1[a]; s->yy[f1 + a] = yn1[a]; s->zz[f1 + a] = zn1[a]; }
+    for (int a = 0; a < n2; ++a) { s->xx[f2 + a] = xn2[a]; s->yy[f2 + a] = yn2[a]; s->zz[f2 + a] = zn2[a]; }
+    double cn1[6], cn2[6], cn12[6], uun1[8], uun2[8], fn1[3][8], fn2[3][8], t12[4][8];
+    int ov1 = rspace_calc(s, i1, xn1, yn1, zn1, cn1, uun1, fn1[0], fn1[1], fn1[2], bn1, bn1 + N, bn1 + 2 * N,
+                          bn1 + 3 * N);
+    int ov2 = rspace_calc(s, i2, xn2, yn2, zn2, cn2, uun2, fn2[0], fn2[1], fn2[2], bn2, bn2 + N, bn2 + 2 * N,
+                          bn2 + 3 * N);
+    (void)rspace_calc_range(s, i1, xn1, yn1, zn1, f2, f2 + n2 - 1, cn12, t12[0], t12[1], t12[2], t12[3], b12,
+                            b12 + N, b12 + 2 * N, b12 + 3 * N);
+    for (int a = 0; a < n1; ++a) { s->xx[f1 + a] = xo1[a]; s->yy[f1 + a] = yo1[a]; s->zz[f1 + a] = zo1[a]; }
+    for (int a = 0; a < n2; ++a) { s->xx[f2 + a] = xo2[a]; s->yy[f2 + a] = yo2[a]; s->zz[f2 + a] = zo2[a]; }
+    int acc = 0;
+    if (ov1 || ov2) {
+        r.overlap = 1;
+    } else {
+        double v1 = 0.0, v2 = 0.0;
+        for (int a = 0; a < n1; ++a) v1 = v1 + uun1[a];
+        for (int a = 0; a < n2; ++a) v2 = v2 + uun2[a];
+        const double du_real = (v1 + v2) - U_old;
+        /* new joint sums and the delta (MC.f90:1602-1641) */
+        int pres[64];
+        memset(s->mol_new, 0, sizeof(double) * nS);
+        memset(pres, 0, sizeof pres);
+        for (int a = 0; a < n1; ++a)
+            add_atom(s, s->mol_new, pres, xn1[a], yn1[a], zn1[a], s->charge[f1 + a], s->type[f1 + a], s->krc, s->krl);
+        for (int a = 0; a < n2; ++a)
+            add_atom(s, s->mol_new, pres, xn2[a], yn2[a], zn2[a], s->charge[f2 + a], s->type[f2 + a], s->krc, s->krl);
+        for (size_t i = 0; i < nS; ++i) s->mol_new[i] = s->mol_new[i] - s->mol_old[i];
+        const double du_kc = du_coulomb(s, s->S, s->mol_new);
+        const double du_klj = du_lj_calc(s, s->S, s->mol_new, pres);
+        const double du_f = du_kc + du_klj;
+        /* external term (MC.f90:1692-1716) */
+        double mun1[3] = {0, 0, 0}, mun2[3] = {0, 0, 0}, dmu[3], mu_new[3];
+        for (int a = 0; a < n1; ++a) {
+            double q = s->charge[f1 + a];
+            mun1[0] = mun1[0] + q * xn1[a]; mun1[1] = mun1[1] + q * yn1[a]; mun1[2] = mun1[2] + q * zn1[a];
+        }
+        for (int a = 0; a < n2; ++a) {
+            double q = s->charge[f2 + a];
+            mun2[0] = mun2[0] + q * xn2[a]; mun2[1] = mun2[1] + q * yn2[a]; mun2[2] = mun2[2] + q * zn2[a];
+        }
+        for (int d = 0; d < 3; ++d) dmu[d] = (mun1[d] - muo1[d]) + (mun2[d] - muo2[d]);
+        const double du_ext = s->kext * ((2 * s->mu[0] + dmu[0]) * dmu[0] + (2 * s->mu[1] + dmu[1]) * dmu[1] +
+                                         (2 * s->mu[2] + dmu[2]) * dmu[2]);
+        for (int d = 0; d < 3; ++d) mu_new[d] = s->mu[d] + dmu[d];
+        const double du_total = du_real + du_f + du_ext;
+        /* MonteCarloMove.f90:572-597 */
+        const double factor = exp(-du_total);
+        acc = !(yo_rand(s) > factor);
+        r.du_real = du_real; r.du_kc = du_kc; r.du_klj = du_klj; r.du_ext = du_ext; r.du_total = du_total;
+        for (int d = 0; d < 3; ++d) r.delta[d] = c2[d] - c1[d];
+        if (acc) { /* MC.f90:1754-1872 */
+            for (size_t i = 0; i < nS; ++i) s->S[i] = s->S[i] + s->mol_new[i];
+            double co1[6], co2[6], co12[6], uuo[8], fo[3][8];
+            rspace_calc(s, i1, xo1, yo1, zo1, co1, uuo, fo[0], fo[1], fo[2], bo1, bo1 + N, bo1 + 2 * N, bo1 + 3 * N);
+            rspace_calc(s, i2, xo2, yo2, zo2, co2, uuo, fo[0], fo[1], fo[2], bo2, bo2 + N, bo2 + 2 * N, bo2 + 3 * N);
+            (void)rspace_calc_range(s, i1, xo1, yo1, zo1, f2, f2 + n2 - 1, co12, t12[0], t12[1], t12[2], t12[3], b12,
+                                    b12 + N, b12 + 2 * N, b12 + 3 * N);
+            for (int j = 0; j < N; ++j) {
+                s->uu[j] = s->uu[j] + bn1[j] - bo1[j] + bn2[j] - bo2[j];
+                s->Fx[j] = s->Fx[j] + bn1[N + j] - bo1[N + j] + bn2[N + j] - bo2[N + j];
+                s->Fy[j] = s->Fy[j] + bn1[2 * N + j] - bo1[2 * N + j] + bn2[2 * N + j] - bo2[2 * N + j];
+                s->Fz[j] = s->Fz[j] + bn1[3 * N + j] - bo1[3 * N + j] + bn2[3 * N + j] - bo2[3 * N + j];
+            }
+            for (int a = 0; a < n1; ++a) {
+                s->uu[f1 + a] = uun1[a];
+                s->Fx[f1 + a] = fn1[0][a]; s->Fy[f1 + a] = fn1[1][a]; s->Fz[f1 + a] = fn1[2][a];
+            }
+            for (int a = 0; a < n2; ++a) {
+                s->uu[f2 + a] = uun2[a];
+                s->Fx[f2 + a] = fn2[0][a]; s->Fy[f2 + a] = fn2[1][a]; s->Fz[f2 + a] = fn2[2][a];
+            }
+            s->e_kc = s->e_kc + du_kc;
+            s->e_klj = s->e_klj + du_klj;
+            s->uu_ew = s->uu_ew + cn1[0] - co1[0] + cn2[0] - co2[0] - (cn12[0] - co12[0]);
+            s->uu_lj6 = s->uu_lj6 + cn1[1] - co1[1] + cn2[1] - co2[1] - (cn12[1] - co12[1]);
+            s->uu_lj12 = s->uu_lj12 + cn1[2] - co1[2] + cn2[2] - co2[2] - (cn12[2] - co12[2]);
+            for (int d = 0; d < 3; ++d) s->mu[d] = mu_new[d];
+            for (int a = 0; a < n1; ++a) { s->xx[f1 + a] = xn1[a]; s->yy[f1 + a] = yn1[a]; s->zz[f1 + a] = zn1[a]; }
+            for (int a = 0; a < n2; ++a) { s->xx[f2 + a] = xn2[a]; s->yy[f2 + a] = yn2[a]; s->zz[f2 + a] = zn2[a]; }
+        }
+    }
+    r.accepted = acc;
+    free(bn1); free(bn2); free(bo1); free(bo2); free(b12);
+    if (rec) *rec = r;
+    return acc;
+}
+
+/* runmc.f90:223-266: one step of a move cycle: the exchange draw, then an
+ * exchange of two distinct molecules picked with the exchange weights, or a
+ * movemc of one picked with the move weights. */
+static int run_step(yo_sys *s, yo_move_rec *rec) {
+    double rnd = yo_rand(s);
+    if (rnd < s->pxch) {
+        int i1 = biased_choose(s, s->xfreq_beg, s->xfreq_end), i2 = i1;
+        while (i2 == i1) i2 = biased_choose(s, s->xfreq_beg, s->xfreq_end);
+        return yo_xchange(s, i1, i2, rec);
+    }
+    return yo_movemc(s, choose_molecule(s), rec);
+}
+
+/* runmc.f90:204-268 without volume steps: per cycle one NPT draw, then
+ * nmol steps. */
 int yo_run_cycles(yo_sys *s, int ncycles, yo_move_rec *recs, int64_t *n_accepted) {
     int64_t acc = 0, k = 0;
     for (int c = 0; c < ncycles; ++c) {
         (void)yo_rand(s); /* runmc.f90:206 */
         for (int j = 0; j < s->nmol; ++j) {
-            (void)yo_rand(s); /* runmc.f90:232 */
-            int imol = choose_molecule(s);
             yo_move_rec r;
-            acc += yo_movemc(s, imol, &r);
+            acc += run_step(s, &r);
             if (recs) recs[k] = r;
             ++k;
         }
@@ -1384,11 +1585,7 @@ int yo_run_npt(yo_sys *s, int ncycles, double pressure_step_multiplier, double p
             ++nv;
             nva += yo_volume_move(s, pressure, log_max_volume_scaling, NULL, NULL);
         } else {
-            for (int j = 0; j < s->nmol; ++j) {
-                (void)yo_rand(s);
-                int imol = choose_molecule(s);
-                acc += yo_movemc(s, imol, NULL);
-            }
+            for (int j = 0; j < s->nmol; ++j) acc += run_step(s, NULL);
         }
     }
     if (n_accepted) *n_accepted = acc;
@@ -1472,13 +1669,9 @@ yo_sys *yo_create(const yo_desc *d) {
     s->krl = calloc((size_t)mx * 8 * nk, sizeof(double));
     s->back_new = calloc(4 * (size_t)N, sizeof(double));
     s->back_old = calloc(4 * (size_t)N, sizeof(double));
-    /* BiasedRandom.f90:155-180 with mv_num(:) = 1 (runmc.f90:43) */
-    s->freq_end = malloc(sizeof(double) * d->nmol);
-    double prob = 1.0 / (double)d->nmol, beg = 0.0;
-    for (int m = 0; m < d->nmol; ++m) {
-        s->freq_end[m] = beg + prob;
-        beg = s->freq_end[m];
-    }
+    s->freq_beg = malloc(sizeof(double) * d->nmol); s->freq_end = malloc(sizeof(double) * d->nmol);
+    s->xfreq_beg = malloc(sizeof(double) * d->nmol); s->xfreq_end = malloc(sizeof(double) * d->nmol);
+    yo_set_frequencies(s, NULL, NULL); /* freq_file 'none' (runmc.f90:41-45) */
     yo_rng_philox(s, 0, 0, 0);
     return s;
 }
@@ -1489,7 +1682,7 @@ void yo_destroy(yo_sys *s) {
                  s->four_eps, s->sigma6, s->tail_counts, s->xx, s->yy, s->zz, s->uu, s->Fx, s->Fy, s->Fz,
                  s->g.kx, s->g.ky, s->g.kz, s->g.ip, s->g.iq, s->g.xk, s->g.xk2, s->beta, s->beta6,
                  s->beta12, s->beta_lj, s->S, s->mol_old, s->mol_new, s->tot_new, s->krc, s->krl,
-                 s->back_new, s->back_old, s->freq_end};
+                 s->back_new, s->back_old, s->freq_beg, s->freq_end, s->xfreq_beg, s->xfreq_end};
     for (size_t i = 0; i < sizeof p / sizeof p[0]; ++i) free(p[i]);
     free(s);
 }

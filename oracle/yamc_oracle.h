@@ -31,9 +31,10 @@ typedef struct yo_desc {
 
 typedef struct yo_sys yo_sys;
 
-/* Per-move record produced by yo_movemc (MC.f90:831-1322). */
+/* Per-step record produced by yo_movemc (MC.f90:831-1322) or yo_xchange
+ * (MC.f90:1325-1954; imol2 = second molecule, -1 for a single move). */
 typedef struct yo_move_rec {
-    int32_t imol, accepted, overlap, pad;
+    int32_t imol, accepted, overlap, imol2;
     double du_real, du_kc, du_klj, du_ext, du_total;
     double f_old[3], t_old[3], f_new[3], t_new[3];
     double delta[3];
@@ -106,8 +107,14 @@ void yo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 
 /* MC.f90:831-1322 (one trial move of molecule imol, 0-based). */
 int yo_movemc(yo_sys *s, int imol, yo_move_rec *rec);
-/* runmc.f90:204-268, NVT without exchange: per cycle one NPT draw, then
- * nmol steps of (exchange draw, molecule pick, movemc).  recs may be NULL. */
+/* runmc.f90:26-66: per-molecule move and exchange weights (NULL = ones /
+ * zeros); xchange_probability = sum(xchange) / (sum(move) + sum(xchange)). */
+int yo_set_frequencies(yo_sys *s, const double *mv_num, const double *xchange_num);
+/* MC.f90:1325-1954 mc_xchange of molecules i1 != i2 (0-based). */
+int yo_xchange(yo_sys *s, int i1, int i2, yo_move_rec *rec);
+/* runmc.f90:204-268 without volume steps: per cycle one NPT draw, then nmol
+ * steps of (exchange draw, then mc_xchange or molecule pick + movemc).
+ * recs may be NULL. */
 int yo_run_cycles(yo_sys *s, int ncycles, yo_move_rec *recs, int64_t *n_accepted);
 /* Same move loop with a caller-given molecule per move and a constant
  * uniform per move (for traces against the compiled reference). */
@@ -120,7 +127,7 @@ void yo_forces_refconv(const yo_sys *s, double *fx, double *fy, double *fz);
  * accepted; *new_box_length = box length after the step. */
 int yo_volume_move(yo_sys *s, double pressure, double log_max_volume_scaling, double *new_box_length,
                    int *overlap);
-/* runmc.f90:204-268 with volume steps (no exchange). */
+/* runmc.f90:204-268 with volume steps. */
 int yo_run_npt(yo_sys *s, int ncycles, double pressure_step_multiplier, double pressure,
                double log_max_volume_scaling, int64_t *n_accepted, int64_t *n_vol_steps, int64_t *n_vol_accepted);
 double yo_box_length(const yo_sys *s);

@@ -42,7 +42,7 @@ def write_mol(path, mol):
 def parse(text):
     lines = text.split("\n")
     i = 0
-    out = {"BETALJ": [], "SUMS": [], "MOVES": [], "VOLS": []}
+    out = {"BETALJ": [], "SUMS": [], "MOVES": [], "VOLS": [], "XCHGS": []}
     cur = None  # record (move or volume step) that E/positions attach to
     while i < len(lines):
         ln = lines[i].strip()
@@ -86,7 +86,10 @@ def parse(text):
         elif tok[0] == "VOL":
             cur = {"accepted": tok[2] == "T", "box": float(tok[3])}
             out["VOLS"].append(cur)
-        elif tok[0] in ("MOVES", "VOLS"):
+        elif "XCHG" in tok[0] and tok[0] != "XCHGS":  # after an overlap the reference prints OX first
+            cur = {"imol1": int(tok[2]) - 1, "imol2": int(tok[3]) - 1, "accepted": tok[4] == "T"}
+            out["XCHGS"].append(cur)
+        elif tok[0] in ("MOVES", "VOLS", "XCHGS"):
             pass
         elif tok[0] == "END":
             break
@@ -115,9 +118,10 @@ def pick_moves(sysm, nmoves, seed):
     raise RuntimeError("could not find non-stalling uniforms")
 
 
-def make_case(name, sysm, nmoves, seed, vols=()):
+def make_case(name, sysm, nmoves, seed, vols=(), xchgs=()):
     """vols: (u, pressure [Pa], max_volume_scaling) of mcvol steps run after
-    the move trace."""
+    the move trace; xchgs: (imol1, imol2, u) of mc_xchange steps (0-based
+    molecules) run after the volume steps."""
     imol, u = pick_moves(sysm, nmoves, seed)
     p = sysm.params
     with tempfile.TemporaryDirectory() as td:
@@ -144,6 +148,9 @@ def make_case(name, sysm, nmoves, seed, vols=()):
             f.write(f"{len(vols)}\n")
             for vu, vp, vm in vols:
                 f.write("%.17g %.17g %.17g\n" % (vu, vp, vm))
+            f.write(f"{len(xchgs)}\n")
+            for a, b, xu in xchgs:
+                f.write("%d %d %.17g\n" % (a + 1, b + 1, xu))
         r = subprocess.run([DRIVER, "case.in"], cwd=td, capture_output=True, text=True, timeout=120)
         if r.returncode != 0:
             raise RuntimeError(r.stderr)
@@ -179,7 +186,7 @@ def make_case(name, sysm, nmoves, seed, vols=()):
         move_pos=np.array([np.pad(np.array(m["pos"]), ((0, 4 - len(m["pos"])), (0, 0))) for m in moves]),
         uu_end=np.array(g["UUEND"]), fx_end=np.array(g["FXEND"]), fy_end=np.array(g["FYEND"]),
         fz_end=np.array(g["FZEND"]),
-        sums_coulomb_end=np.array(g["SUMS"][(1 + nt) * 8:]).reshape(8, nk),
+        sums_coulomb_end=np.array(g["SUMS"][(1 + nt) * 8:(1 + nt) * 8 + 8]).reshape(8, nk),
     )
     if vols:
         vs = g["VOLS"]
@@ -190,6 +197,17 @@ def make_case(name, sysm, nmoves, seed, vols=()):
             vol_accepted=np.array([v["accepted"] for v in vs], np.int32),
             vol_box=np.array([v["box"] for v in vs]), vol_e=np.array([v["E"] for v in vs]),
             vol_pos=np.array([v["pos"] for v in vs]))
+    if xchgs:
+        xs = g["XCHGS"]
+        assert len(xs) == len(xchgs)
+        ns = (1 + nt) * 8 + 8
+        arrays.update(
+            xchg_imol1=np.array([x[0] for x in xchgs], np.int32), xchg_imol2=np.array([x[1] for x in xchgs], np.int32),
+            xchg_u=np.array([x[2] for x in xchgs]),
+            xchg_accepted=np.array([x["accepted"] for x in xs], np.int32),
+            xchg_e=np.array([x["E"] for x in xs]), xchg_pos=np.array([x["pos"] for x in xs]),
+            uu_xchg=np.array(g["UUX"]), fx_xchg=np.array(g["FXX"]), fy_xchg=np.array(g["FYX"]),
+            fz_xchg=np.array(g["FZX"]), sums_coulomb_xchg=np.array(g["SUMS"][ns:ns + 8]).reshape(8, nk))
     path = os.path.join(OUT, f"{name}.npz")
     np.savez_compressed(path, **arrays)
     print(f"{path}: natom={natom} nk={nk} ntype={nt} moves={nmoves} accepted={arrays['move_accepted'].sum()}")
@@ -200,11 +218,31 @@ def main():
         raise SystemExit("oracle/_ref/ref_driver is not built (oracle/build_ref.sh)")
     # volume steps: shrink and grow at 1 bar and at 5 kbar, 2 % maximum scaling
     vols = [(0.30, 1.0e5, 1.02), (0.85, 1.0e5, 1.02), (0.10, 5.0e8, 1.02), (0.65, 1.0e5, 1.05)]
-    make_case("ref_small_bias", ys.config("small", seed=31), 40, 1, vols)
-    make_case("ref_small_nobias", ys.config("small_nobias", seed=32), 40, 2, vols)
-    make_case("ref_water64", ys.lattice_system(
+    small = ys.config("small", seed=31)
+    make_case("ref_small_bias", small, 40, 1, vols, xchange_pairs(small, 11))
+    small_nb = ys.config("small_nobias", seed=32)
+    make_case("ref_small_nobias", small_nb, 40, 2, vols, xchange_pairs(small_nb, 12))
+    w64 = ys.lattice_system(
         ys.Params(box_length=12.42, temp=298.0, sr=3.2, sk=3.0, alpha=6.4, external_permutivity=70.0, dr_a=0.3,
-                  d_angle_degree=15.0, lambda_f=0.0, lambda_c=0.0), [ys.SPCE], [64], 33), 30, 3, vols)
+                  d_angle_degree=15.0, lambda_f=0.0, lambda_c=0.0), [ys.SPCE], [64], 33)
+    make_case("ref_water64", w64, 30, 3, vols, xchange_pairs(w64, 13))
+
+
+def xchange_pairs(sysm, seed, n=8):
+    """Exchange steps over every species pairing present (ion-ion,
+    water-ion, water-water), with small and large constant uniforms."""
+    rng = np.random.default_rng(seed)
+    t = np.asarray(sysm.mol_type)
+    by = {k: np.nonzero(t == k)[0] for k in sorted(set(t.tolist()))}
+    kinds = sorted(by)
+    combos = [(a, b) for a in kinds for b in kinds if a <= b and (a != b or len(by[a]) > 1)]
+    out = []
+    for k in range(n):
+        a, b = combos[k % len(combos)]
+        i1 = int(rng.choice(by[a]))
+        i2 = int(rng.choice([m for m in by[b] if m != i1]))
+        out.append((i1, i2, 0.05 if k % 2 == 0 else 0.9))
+    return out
 
 
 if __name__ == "__main__":

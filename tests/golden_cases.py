@@ -132,3 +132,27 @@ def check_volume_steps(impl, sysm, g, vstep):
         pos = np.stack([x, y, z], 1)
         assert rel(pos, g["vol_pos"][k], 1.0) < 1e-13, k
     return len(g["vol_u"])
+
+
+def check_xchange_steps(impl, sysm, g, xstep):
+    """xstep(imol1, imol2, u) runs one mc_xchange step with a constant uniform
+    and returns accepted; compared with the reference after each step (run
+    after the move trace and the volume steps), then the per-atom real-space
+    arrays and the Coulomb sums after the last step."""
+    if "xchg_u" not in g:
+        return 0
+    for k in range(len(g["xchg_u"])):
+        acc = xstep(int(g["xchg_imol1"][k]), int(g["xchg_imol2"][k]), float(g["xchg_u"][k]))
+        assert int(acc) == int(g["xchg_accepted"][k]), f"exchange decision differs at step {k}"
+        e = impl.energies()
+        ev = energies_vec(e, impl.mu(), impl.tails())
+        ref = g["xchg_e"][k]
+        assert rel(ev[:13], ref[:13], ref[:13]) < RTOL, (k, ev[:13] - ref[:13])
+        x, y, z = impl.positions()
+        assert rel(np.stack([x, y, z], 1), g["xchg_pos"][k], 1.0) < 1e-13, k
+    uu, fx, fy, fz = impl.real_arrays()
+    assert rel(uu, g["uu_xchg"], g["uu_xchg"]) < RTOL
+    fe = np.stack([g["fx_xchg"], g["fy_xchg"], g["fz_xchg"]])
+    assert rel(np.stack([fx, fy, fz]), fe, fe) < RTOL
+    assert rel(impl.sums()[0], g["sums_coulomb_xchg"], g["sums_coulomb_xchg"]) < RTOL
+    return len(g["xchg_u"])

@@ -1,9 +1,9 @@
 """The FP64 oracle against golden vectors of the compiled reference Fortran
 (tests/golden/*.npz, written by tests/golden/make_golden.py from
 oracle/_ref/ref_driver).  Pins staging, beta tables, S(k), every energy
-component, per-atom real-space and k-space forces, a movemc trace and mcvol
-volume steps with identical decisions, all within 1e-10 relative (staging
-within 1e-14)."""
+component, per-atom real-space and k-space forces, a movemc trace, mcvol
+volume steps and mc_xchange exchange steps with identical decisions, all
+within 1e-10 relative (staging within 1e-14)."""
 import numpy as np
 import pytest
 
@@ -46,3 +46,9 @@ def test_oracle_matches_reference(oracle_mod, path):
         return acc, L
 
     assert gc.check_volume_steps(v, sysm, g, vstep) == 4
+
+    def xstep(i1, i2, u):
+        acc, rec = o.xchange(i1, i2, u=u)
+        return acc
+
+    assert gc.check_xchange_steps(v, sysm, g, xstep) == 8
