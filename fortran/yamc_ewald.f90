@@ -25,7 +25,7 @@ module yamc_ewald
    end type yamc_system_desc
 
    type, bind(C) :: yamc_move_rec
-      integer(c_int32_t) :: imol, accepted, overlap, pad
+      integer(c_int32_t) :: imol, accepted, overlap, imol2
       real(c_double) :: du_real, du_kc, du_klj, du_ext, du_total
       real(c_double) :: f_old(3), t_old(3), f_new(3), t_new(3), delta(3)
       real(c_double) :: angle
@@ -182,6 +182,19 @@ module yamc_ewald
          integer(c_int32_t), value :: ncycles
          real(c_double), value :: pressure_step_multiplier, pressure_kT_A3, log_max_volume_scaling
          integer(c_int64_t), intent(out) :: n_accepted, n_vol_steps, n_vol_accepted
+      end function
+      integer(c_int) function yamc_mc_set_frequencies(ctx, move_weights, xchange_weights) &
+            bind(C, name="yamc_mc_set_frequencies")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, move_weights, xchange_weights
+      end function
+      integer(c_int) function yamc_mc_xchange(ctx, imol1, imol2, u, accepted, rec) bind(C, name="yamc_mc_xchange")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: imol1, imol2
+         type(c_ptr), value :: u
+         integer(c_int32_t), intent(out) :: accepted
+         type(c_ptr), value :: rec
       end function
       integer(c_int) function yamc_box_length(ctx, box_length) bind(C, name="yamc_box_length")
          import :: c_int, c_ptr, c_double

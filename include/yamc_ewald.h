@@ -86,9 +86,12 @@ typedef struct yamc_system_desc {
 
 typedef struct yamc_ctx yamc_ctx;
 
-/* Per-move record (the quantities movemc computes, MC.f90:831-1322). */
+/* Per-step record: the quantities movemc computes (MC.f90:831-1322), or
+ * for an exchange step (mc_xchange, MC.f90:1325-1954) the two molecules
+ * (imol, imol2), the decision, overlap, the dU terms and delta = the
+ * centre-of-mass shift of imol.  imol2 = -1 for a single-molecule move. */
 typedef struct yamc_move_rec {
-    int32_t imol, accepted, overlap, pad;
+    int32_t imol, accepted, overlap, imol2;
     double du_real, du_kc, du_klj, du_ext, du_total;
     double f_old[3], t_old[3], f_new[3], t_new[3];
     double delta[3];
@@ -166,9 +169,11 @@ int yamc_mc_real_arrays(yamc_ctx *ctx, double *uu, double *fx, double *fy, doubl
  * word position = counter; 24-bit uniforms in (0,1) as random.f90 gives. */
 int yamc_mc_set_rng(yamc_ctx *ctx, uint64_t seed, uint64_t stream, uint64_t counter);
 int yamc_mc_get_rng(yamc_ctx *ctx, uint64_t *counter);
-/* run_mc cycles (runmc.f90:204-268; NVT, no exchange), all on the device:
- * per cycle one pressure-step draw, then nmol steps of (exchange draw,
- * molecule pick, movemc).  recs (may be NULL) receives ncycles*nmol records.
+/* run_mc cycles (runmc.f90:204-268, no volume steps): per cycle one
+ * pressure-step draw, then nmol steps of (exchange draw, then molecule pick
+ * + movemc on the device, or an exchange step when exchange weights are set,
+ * see yamc_mc_set_frequencies).  recs (may be NULL) receives ncycles*nmol
+ * records.
  * resync_every > 0 rebuilds S(k) from scratch in FP64 after that many moves
  * (FP32C form); 0 = never. */
 int yamc_mc_run(yamc_ctx *ctx, int32_t ncycles, int32_t resync_every, yamc_move_rec *recs,
@@ -181,6 +186,25 @@ int yamc_mc_run_const(yamc_ctx *ctx, int32_t nmoves, const int32_t *imol, const 
 /* Rebuilds S(k) and the k-space energies from the current positions
  * (calc_total_energy, EwaldSumKSpace.f90:225-260) in FP64. */
 int yamc_resync(yamc_ctx *ctx);
+
+/* ---- exchange steps ---------------------------------------------------- */
+/* init_freq (runmc.f90:26-66) with the per-molecule weights of a frequency
+ * file (read_freq_file columns 2 and 3): move_weights[nmol] (NULL = all 1),
+ * xchange_weights[nmol] (NULL = all 0).  The move picker and the exchange
+ * picker take intervals weight / sum (BiasedRandom.f90:41-75; a zero weight
+ * is never picked) and a step is an exchange with probability
+ * sum(xchange) / (sum(move) + sum(xchange)).  Nonzero exchange weights must
+ * cover at least two molecules.  yamc_mc_run then runs exchange steps where
+ * the reference would; batches refuse contexts with exchange weights. */
+int yamc_mc_set_frequencies(yamc_ctx *ctx, const double *move_weights, const double *xchange_weights);
+/* mc_xchange (MC.f90:1325-1954) of molecules imol1 != imol2 (0-based): the
+ * two swap centres of mass, keeping their orientations; accepted with
+ * min(1, exp(-dU)) (MonteCarloMove.f90:572-597), no acceptance draw after an
+ * overlap.  u (may be NULL) is a constant uniform (RND_CONSTANT), NULL draws
+ * from the context's stream.  rec (may be NULL) receives the step's record
+ * (imol, imol2, decision, overlap, dU terms). */
+int yamc_mc_xchange(yamc_ctx *ctx, int32_t imol1, int32_t imol2, const double *u, int32_t *accepted,
+                    yamc_move_rec *rec);
 
 /* ---- volume steps (NPT) ------------------------------------------------ */
 /* mcvol (MC.f90:701-825): V_new = exp(ln V_old + (2u-1) ln max_volume_scaling),

@@ -382,6 +382,12 @@ def test_gpu_matches_reference_golden(lib, case):
 
     assert gc.check_volume_steps(gpu, sysm, g, vstep) == 4
 
+    def xstep(i1, i2, u):
+        acc, rec = gpu.xchange(i1, i2, u=u)
+        return acc
+
+    assert gc.check_xchange_steps(gpu, sysm, g, xstep) == 8
+
 
 def test_rdf_counts_match_oracle(lib):
     """yamc_rdf (COM pair counts, mcrdf2.f90 count_distances) equals the CPU
@@ -517,3 +523,104 @@ def test_fp32c_npt_state_matches_fresh_sums(lib):
         assert rel(a, b, np.abs(b).max()) < 1e-12
     # carried FP32C energies drift by at most the per-move bound times the moves since the last rebuild
     assert abs(e_carried["U_total"] - e_fresh["U_total"]) < 8 * 256 * 4.04e-7
+
+
+def _exchange_weights(s, kind):
+    mt = np.asarray(s.mol_type)
+    if kind == "ions":
+        return (mt > 0).astype(np.float64)
+    if kind == "all":
+        return np.ones(s.nmol)
+    w = np.zeros(s.nmol)
+    w[::16] = 1.0
+    return w
+
+
+@pytest.mark.parametrize("name,ncyc,kind", [("small", 20, "ions"), ("small", 10, "all"), ("c1", 2, "every16")])
+def test_exchange_cycles_match_oracle(lib, name, ncyc, kind):
+    """run_mc cycles with exchange steps (runmc.f90:223-266, mc_xchange
+    MC.f90:1325-1954) on one RNG stream: the device runs the moves and hands
+    each exchange step to the host; identical step kinds, molecules and
+    decisions, and the same state after the run."""
+    s, g, o = make(name, seed=41)
+    xc = _exchange_weights(s, kind)
+    g.set_frequencies(None, xc)
+    o.set_frequencies(None, xc)
+    g.mc_init()
+    o.mc_init()
+    g.set_rng(55, 3)
+    o.set_rng(55, 3)
+    ng, rg = g.run(ncyc, records=True)
+    no, ro = o.run(ncyc)
+    for k in ("imol", "imol2", "overlap", "accepted"):
+        assert np.array_equal(rg[k], ro[k]), k
+    xs = ro["imol2"] >= 0
+    assert xs.sum() > 0, "no exchange step drawn"
+    if kind != "ions":  # Na/Cl swaps in the 8-water box are all rejected on this stream
+        assert ro["accepted"][xs].sum() > 0, "no accepted exchange step"
+    assert ng == no
+    ok = ro["overlap"] == 0
+    scale = (np.abs(ro["du_real"]) + np.abs(ro["du_kc"]) + np.abs(ro["du_klj"]) + np.abs(ro["du_ext"]))[ok]
+    for k in ("du_total", "du_real", "du_kc", "du_klj", "du_ext"):
+        assert rel(rg[k][ok], ro[k][ok], scale) < RTOL, k
+    assert g.rng_counter() == o.rng_counter()
+    eg, eo = g.energies(), o.energies()
+    for k in eo:
+        assert rel(eg[k], eo[k], max(1.0, abs(eo[k]))) < RTOL, k
+    for a, b in zip(g.positions(), o.positions()):
+        assert rel(a, b, 1.0) < 1e-12
+    for a, b in zip(g.real_arrays(), o.real_arrays()):
+        assert rel(a, b, np.abs(b).max()) < RTOL
+    assert rel(g.sums(), o.sums(), np.abs(o.sums()).max()) < RTOL
+
+
+@pytest.mark.parametrize("block", [256, 64])
+def test_weighted_move_picker_matches_oracle(lib, block):
+    """Non-uniform move weights (BiasedRandom intervals with a zero weight)
+    pick the same molecules on the device as in the oracle."""
+    s, g, o = make("small", seed=43)
+    mv = np.where(np.asarray(s.mol_type) > 0, 5.0, 1.0)
+    mv[2] = 0.0
+    g.set_block_size(block)
+    g.set_frequencies(mv, None)
+    o.set_frequencies(mv, None)
+    g.mc_init()
+    o.mc_init()
+    g.set_rng(9, 1)
+    o.set_rng(9, 1)
+    ng, rg = g.run(10, records=True)
+    no, ro = o.run(10)
+    assert np.array_equal(rg["imol"], ro["imol"])
+    assert not np.any(ro["imol"] == 2)
+    assert np.array_equal(rg["accepted"], ro["accepted"]) and ng == no
+    assert g.rng_counter() == o.rng_counter()
+
+
+def test_fp32c_exchange_state_matches_fresh_sums(lib):
+    """FP32C form with exchange steps: the carried energies equal a rebuild
+    of the final configuration within the FP32C bound."""
+    from yamc_amd import config
+    from yamc_amd.ewald import EwaldMC
+    s = config("c1", seed=8)
+    g = EwaldMC(s, precision=1)
+    g.set_frequencies(None, _exchange_weights(s, "every16"))
+    g.mc_init()
+    g.set_rng(78)
+    _, recs = g.run(2, records=True)
+    assert int((recs["imol2"] >= 0).sum()) > 0
+    e_carried = g.energies()
+    e_fresh, ov = g.mc_init()
+    assert not ov
+    assert abs(e_carried["U_total"] - e_fresh["U_total"]) < 2 * 256 * 4.04e-7
+
+
+def test_batch_refuses_exchange_contexts(lib):
+    from yamc_amd import _lib as L
+    from yamc_amd import config
+    from yamc_amd.ewald import EwaldMC, batch_run
+    s = config("small", seed=5)
+    g = EwaldMC(s)
+    g.set_frequencies(None, _exchange_weights(s, "ions"))
+    g.mc_init()
+    with pytest.raises(L.YamcError):
+        batch_run([g], 1)

@@ -27,7 +27,8 @@ EXPORTS = [
     "yamc_grid", "yamc_kspace_init", "yamc_kspace_get_beta", "yamc_kspace_get_sums", "yamc_kspace_delta",
     "yamc_kspace_forces", "yamc_rspace_full", "yamc_rspace_partial", "yamc_tails", "yamc_external_mu",
     "yamc_mc_init", "yamc_mc_energies", "yamc_mc_real_arrays", "yamc_mc_set_rng", "yamc_mc_get_rng",
-    "yamc_mc_run", "yamc_mc_run_const", "yamc_resync", "yamc_mc_volume_move", "yamc_mc_run_npt", "yamc_box_length", "yamc_batch_run", "yamc_batch_create", "yamc_batch_launch",
+    "yamc_mc_run", "yamc_mc_run_const", "yamc_resync", "yamc_mc_volume_move", "yamc_mc_run_npt", "yamc_box_length",
+    "yamc_mc_set_frequencies", "yamc_mc_xchange", "yamc_batch_run", "yamc_batch_create", "yamc_batch_launch",
     "yamc_batch_destroy", "yamc_forces", "yamc_rdf",
     "yamc_set_block_size", "yamc_last_kernel_ms", "yamc_phase_times", "yamc_last_error",
 ]
@@ -61,7 +62,7 @@ class SystemDesc(C.Structure):
 
 class MoveRec(C.Structure):
     _fields_ = [
-        ("imol", C.c_int32), ("accepted", C.c_int32), ("overlap", C.c_int32), ("pad", C.c_int32),
+        ("imol", C.c_int32), ("accepted", C.c_int32), ("overlap", C.c_int32), ("imol2", C.c_int32),
         ("du_real", C.c_double), ("du_kc", C.c_double), ("du_klj", C.c_double), ("du_ext", C.c_double),
         ("du_total", C.c_double),
         ("f_old", C.c_double * 3), ("t_old", C.c_double * 3), ("f_new", C.c_double * 3), ("t_new", C.c_double * 3),
@@ -70,7 +71,7 @@ class MoveRec(C.Structure):
 
 
 MOVE_DTYPE = np.dtype([
-    ("imol", "<i4"), ("accepted", "<i4"), ("overlap", "<i4"), ("pad", "<i4"),
+    ("imol", "<i4"), ("accepted", "<i4"), ("overlap", "<i4"), ("imol2", "<i4"),
     ("du_real", "<f8"), ("du_kc", "<f8"), ("du_klj", "<f8"), ("du_ext", "<f8"), ("du_total", "<f8"),
     ("f_old", "<f8", (3,)), ("t_old", "<f8", (3,)), ("f_new", "<f8", (3,)), ("t_new", "<f8", (3,)),
     ("delta", "<f8", (3,)), ("angle", "<f8"),

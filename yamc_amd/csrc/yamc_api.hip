@@ -4,6 +4,7 @@
 // arrays passed in are copied to the device and never retained; every entry
 // point synchronises its stream before returning, so outputs are ready.
 #include <math.h>
+#include <cmath>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -23,8 +24,6 @@ __global__ void yk_kenergy(YdChain c);
 __global__ void yk_mu(YdChain c, double *mu_out);
 __global__ void yk_tails(YdChain c, const double *lj6, const double *lj12, const int *counts);
 __global__ void yk_scale_box(YdChain c, double sc);
-__global__ void yk_beta_lj(YdChain c, const double *lj6, const double *lj12, const double *beta6,
-                           const double *beta12);
 __global__ void yk_rspace_full(YdChain c, const double *lj_sig6, const double *lj_4eps, double *part, int *ov);
 __global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, double *comps_out);
 template <bool F32, int BS>
@@ -43,6 +42,16 @@ __global__ void yk_kspace_forces(YdChain c, int first, int n, const double *x, c
 __global__ void yk_rspace_partial(YdChain c, int imol, const double *x, const double *y, const double *z,
                                   const double *sig6, const double *feps, double *out, double *uub, double *fxb,
                                   double *fyb, double *fzb);
+__global__ void yk_xchg_delta(YdChain c, YxAtoms at, double *D);
+template <bool F32>
+__global__ void yk_xchg_du(YdChain c, const double *D, unsigned pmask, double *out);
+template <bool F32>
+__global__ void yk_xchg_commit_sums(YdChain c, const double *D, unsigned pmask);
+__global__ void yk_rspace_pair(YdChain c, int imol1, int imol2, const double *sig6, const double *feps,
+                               double *out);
+__global__ void yk_xchg_commit_rs(YdChain c, const double *bn1, const double *bo1, const double *bn2,
+                                  const double *bo2, const double *mv, int f1, int n1, int f2, int n2);
+#define YK_NRS (4 * YD_MAXA + 3)  // yk_rspace_partial output: uu, F, 3 components; overlap flag after
 void yk_launch_forces(const YdChain &c, const double *sig6, const double *feps, const double *tab, double *g,
                       double *r, hipStream_t s);
 void yk_launch_rdf(const YdChain &c, double *com, const int *species, int nspec, double dr, double rmax2, int nbins,
@@ -93,6 +102,14 @@ struct yamc_ctx {
     std::vector<double> hard_core_angstr, lj6, lj12, sigma6;
     // device copies of the state kept across a volume step (allocated on first use)
     std::vector<void *> vol_save;
+    // host copies of the per-atom and per-molecule inputs read by exchange steps
+    std::vector<double> h_q, h_mass;
+    std::vector<int> h_type, h_first, h_last;
+    // exchange steps: move and exchange weights (runmc.f90:26-66) and device
+    // scratch (joint delta, four back arrays, the molecules' new values)
+    std::vector<double> xfreq_beg, xfreq_end;
+    double pxch = 0.0;
+    double *d_xd = nullptr, *d_xb = nullptr, *d_xmv = nullptr;
 };
 
 template <typename T>
@@ -133,6 +150,22 @@ static void build_grid(int kmax, std::vector<int> &kx, std::vector<int> &ky, std
         if (!ip.empty()) ip.back() = 2;
     }
     if (!ip.empty()) ip.back() = 3;
+}
+
+// BiasedRandom.f90:41-75 BiasedRandom_init: cumulative intervals of
+// num / sum(num) (flat when the sum is 0); a zero weight gets end = -1 so
+// that its interval is never chosen.
+static void biased_init(int n, const double *num, double *beg, double *end) {
+    double S = 0.0;
+    for (int i = 0; i < n; ++i) S = S + num[i];
+    for (int i = 0; i < n; ++i) {
+        double prob = S == 0 ? 1.0 / (double)n : num[i] / S;
+        beg[i] = i == 0 ? 0.0 : end[i - 1];
+        end[i] = beg[i] + prob;
+    }
+    if (S > 0)
+        for (int i = 0; i < n; ++i)
+            if (num[i] == 0) end[i] = -1.0;
 }
 
 static int sync(yamc_ctx *c) {
@@ -207,7 +240,7 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
         TRY(dalloc(c, &h.uu, N)); TRY(dalloc(c, &h.Fx, N)); TRY(dalloc(c, &h.Fy, N)); TRY(dalloc(c, &h.Fz, N));
         TRY(dalloc(c, &h.rsb, (size_t)YD_NBCAP * 8));
         TRY(dalloc(c, &h.nbl, (size_t)YD_NBCAP)); TRY(dalloc(c, &h.pos4, (size_t)N));
-        TRY(dalloc(c, &h.freq_end, M)); TRY(dalloc(c, &h.st, 1));
+        TRY(dalloc(c, &h.freq_end, M)); TRY(dalloc(c, &h.freq_beg, M)); TRY(dalloc(c, &h.st, 1));
         TRY(dalloc(c, &c->d_lj6, TT)); TRY(dalloc(c, &c->d_lj12, TT));
         TRY(dalloc(c, &c->d_sig6, TT)); TRY(dalloc(c, &c->d_4eps, TT));
         TRY(dalloc(c, &c->d_counts, T > 0 ? T : 1));
@@ -218,6 +251,11 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
         TRY(dalloc(c, &c->d_chain, 1)); TRY(dalloc(c, &c->d_sig6s, 1)); TRY(dalloc(c, &c->d_feps, 1));
         // host staging
         c->hard_core_angstr.assign(d->hard_core_angstr, d->hard_core_angstr + N);
+        c->h_q.assign(d->charge, d->charge + N);
+        c->h_mass.assign(d->mass, d->mass + N);
+        c->h_type.assign(d->lj_type, d->lj_type + N);
+        c->h_first.assign(d->mol_first, d->mol_first + M);
+        c->h_last.assign(d->mol_last, d->mol_last + M);
         if (T > 0) {
             c->lj6.assign(d->lj6, d->lj6 + T * T);
             c->lj12.assign(d->lj12, d->lj12 + T * T);
@@ -227,10 +265,11 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
         for (int i = 0; i < N; ++i) dh[i] = d->hard_core_angstr[i] / d->box_length;  // EwaldSumRealSpace.f90:359,381
         std::vector<int> kxyz(nk);
         for (int k = 0; k < nk; ++k) kxyz[k] = c->kx[k] | (c->ky[k] << 8) | (c->kz[k] << 16);
-        // BiasedRandom.f90:155-180 with mv_num(:) = 1 (runmc.f90:43)
-        std::vector<double> fe(M);
-        double prob = 1.0 / (double)M, beg = 0.0;
-        for (int m = 0; m < M; ++m) { fe[m] = beg + prob; beg = fe[m]; }
+        // BiasedRandom.f90:41-75 with mv_num(:) = 1 (runmc.f90:43)
+        std::vector<double> fb(M), fe(M), ones(M, 1.0);
+        biased_init(M, ones.data(), fb.data(), fe.data());
+        c->xfreq_beg.assign(M, 0.0);
+        c->xfreq_end.assign(M, 0.0);
         hipStream_t s = c->stream;
         HIPCHK(hipMemcpyAsync(h.q, d->charge, sizeof(double) * N, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(h.dh, dh.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
@@ -242,6 +281,7 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
         HIPCHK(hipMemcpyAsync(h.kxyz, kxyz.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(c->d_xk2, xk2.data(), sizeof(double) * nk, hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(h.freq_end, fe.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h.freq_beg, fb.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
         if (T > 0) {
             HIPCHK(hipMemcpyAsync(c->d_lj6, d->lj6, sizeof(double) * T * T, hipMemcpyHostToDevice, s));
             HIPCHK(hipMemcpyAsync(c->d_lj12, d->lj12, sizeof(double) * T * T, hipMemcpyHostToDevice, s));
@@ -269,6 +309,9 @@ extern "C" int yamc_ctx_destroy(yamc_ctx *c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void *p : c->owned) hipFree(p);
     for (void *p : c->vol_save) hipFree(p);
+    if (c->d_xd) hipFree(c->d_xd);
+    if (c->d_xb) hipFree(c->d_xb);
+    if (c->d_xmv) hipFree(c->d_xmv);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -619,7 +662,8 @@ extern "C" int yamc_resync(yamc_ctx *c) {
 
 template <bool F32, int BS>
 static int launch_chain_t(yamc_ctx *c, const YdChain *d_chain, const double *const *sig6s,
-                          const double *const *feps, int nchains, int nsteps, int mode, int max_natom) {
+                          const double *const *feps, int nchains, int nsteps, int mode, int max_natom,
+                          int max_moves) {
     // The FP32 position copy of the real-space scan goes into dynamic LDS when
     // it fits next to the kernel's static LDS.
     int lds_max = 0;
@@ -635,19 +679,21 @@ static int launch_chain_t(yamc_ctx *c, const YdChain *d_chain, const double *con
         dyn = want;
         HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
     }
-    yk_chain<F32, BS><<<nchains, BS, dyn, c->stream>>>(d_chain, sig6s, feps, nsteps, mode, 0, lds_atoms);
+    yk_chain<F32, BS><<<nchains, BS, dyn, c->stream>>>(d_chain, sig6s, feps, nsteps, mode, max_moves, lds_atoms);
     HIPCHK(hipGetLastError());
     return YAMC_OK;
 }
 
 // The wave shapes (64, 128) run the wave kernel for FP64 run_mc cycles;
-// FP32C chains and constant-uniform traces run them as workgroups of 256
+// FP32C chains, constant-uniform traces and runs with exchange steps (which
+// launch step ranges, max_moves > 0) run them as workgroups of 256
 // (the FP32C wave kernel is switched off: it made an illegal memory access
 // on its first GPU launch and its cause is not yet found).
 static int launch_chain(yamc_ctx *c, const YdChain *d_chain, const double *const *sig6s,
-                        const double *const *feps, int nchains, int nsteps, int mode, int max_natom) {
+                        const double *const *feps, int nchains, int nsteps, int mode, int max_natom,
+                        int max_moves = 0) {
     const bool f32 = c->precision == YAMC_FP32C;
-    if ((c->chain_bs == 64 || c->chain_bs == 128) && mode == 0 && !f32) {  // one wave per replica
+    if ((c->chain_bs == 64 || c->chain_bs == 128) && mode == 0 && !f32 && max_moves == 0) {  // one wave per replica
         int nb = (nchains + YW_WPB - 1) / YW_WPB;
         if (c->chain_bs == 64) yk_wchain<false, 1><<<nb, YW_WPB * 64, 0, c->stream>>>(d_chain, nchains, sig6s, feps, nsteps);
         else yk_wchain<false, 2><<<nb, YW_WPB * 64, 0, c->stream>>>(d_chain, nchains, sig6s, feps, nsteps);
@@ -655,10 +701,10 @@ static int launch_chain(yamc_ctx *c, const YdChain *d_chain, const double *const
         return YAMC_OK;
     }
     if (c->chain_bs == 512)
-        return f32 ? launch_chain_t<true, 512>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom)
-                   : launch_chain_t<false, 512>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom);
-    return f32 ? launch_chain_t<true, 256>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom)
-               : launch_chain_t<false, 256>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom);
+        return f32 ? launch_chain_t<true, 512>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom, max_moves)
+                   : launch_chain_t<false, 512>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom, max_moves);
+    return f32 ? launch_chain_t<true, 256>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom, max_moves)
+               : launch_chain_t<false, 256>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom, max_moves);
 }
 
 extern "C" int yamc_set_block_size(yamc_ctx *c, int32_t block) {
@@ -669,6 +715,8 @@ extern "C" int yamc_set_block_size(yamc_ctx *c, int32_t block) {
     c->chain_bs = block;
     return YAMC_OK;
 }
+
+static int run_with_exchanges(yamc_ctx *c, int cyc0, int ncyc, yamc_move_rec *d_rec);
 
 extern "C" int yamc_mc_run(yamc_ctx *c, int32_t ncycles, int32_t resync_every, yamc_move_rec *recs,
                            int64_t *n_accepted) {
@@ -690,10 +738,14 @@ extern "C" int yamc_mc_run(yamc_ctx *c, int32_t ncycles, int32_t resync_every, y
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     for (int done = 0; done < ncycles; done += seg) {
         int n = ncycles - done < seg ? ncycles - done : seg;
-        YdChain ch = c->ch;
-        ch.rec = d_rec ? d_rec + (size_t)done * c->nmol : nullptr;
-        HIPCHK(hipMemcpyAsync(c->d_chain, &ch, sizeof ch, hipMemcpyHostToDevice, c->stream));
-        TRY(launch_chain(c, c->d_chain, c->d_sig6s, c->d_feps, 1, n, 0, c->natom));
+        if (c->pxch > 0) {
+            TRY(run_with_exchanges(c, done, n, d_rec));
+        } else {
+            YdChain ch = c->ch;
+            ch.rec = d_rec ? d_rec + (size_t)done * c->nmol : nullptr;
+            HIPCHK(hipMemcpyAsync(c->d_chain, &ch, sizeof ch, hipMemcpyHostToDevice, c->stream));
+            TRY(launch_chain(c, c->d_chain, c->d_sig6s, c->d_feps, 1, n, 0, c->natom));
+        }
         if (resync_every > 0 && done + n < ncycles) TRY(total_energy(c));
     }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -768,6 +820,8 @@ static int batch_stage(yamc_ctx *const *ctxs, int32_t n, yamc_batch *b) {
         if (c->device != c0->device || c->precision != c0->precision)
             return fail(YAMC_ERROR_PARAMETER, "batch contexts differ in device or precision");
         if (!c->state_ready) return fail(YAMC_ERROR_INITIALIZATION, "yamc_mc_init has not run on a batch member");
+        if (c->pxch > 0)
+            return fail(YAMC_ERROR_PARAMETER, "batch member %d has exchange steps: run it with yamc_mc_run", i);
         HIPCHK(hipStreamSynchronize(c->stream));
         chains[i] = c->ch;
         s6[i] = c->d_sig6;
@@ -1168,5 +1222,308 @@ extern "C" int yamc_mc_run_npt(yamc_ctx *c, int32_t ncycles, double pressure_ste
 extern "C" int yamc_box_length(yamc_ctx *c, double *box_length) {
     if (!c || !box_length) return fail(YAMC_ERROR_PARAMETER, "null argument");
     *box_length = c->ch.p.L;
+    return YAMC_OK;
+}
+
+// ---- exchange steps ---------------------------------------------------------
+static int xchg_buffers(yamc_ctx *c) {
+    if (c->d_xd) return YAMC_OK;
+    const size_t nS = (size_t)(1 + c->ntype) * 8 * c->nk, N = c->natom;
+    HIPCHK(hipMalloc((void **)&c->d_xd, sizeof(double) * nS));
+    HIPCHK(hipMalloc((void **)&c->d_xb, sizeof(double) * 16 * N));
+    HIPCHK(hipMalloc((void **)&c->d_xmv, sizeof(double) * 8 * YD_MAXA));
+    return YAMC_OK;
+}
+
+static int write_pos(yamc_ctx *c, int f, int n, const double *x, const double *y, const double *z) {
+    HIPCHK(hipMemcpyAsync(c->ch.x + f, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->ch.y + f, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->ch.z + f, z, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+
+// EwaldSumRealSpace_calc of molecule imol at (x,y,z), partial mode, with the
+// other atoms where c.x/y/z hold them: o[a] uu, o[YD_MAXA + 3a + d] F,
+// o[4 YD_MAXA + 0..2] uu_ew, uu_lj6, uu_lj12, o[YK_NRS] overlap; back
+// arrays into back[4N] on the device.
+static int partial_dev(yamc_ctx *c, int imol, const double *x, const double *y, const double *z, double *back,
+                       double *o) {
+    const int na = c->h_last[imol] - c->h_first[imol] + 1, N = c->natom;
+    TRY(stage_mol(c, na, x, y, z));
+    const double *dm = c->d_mol;
+    yk_rspace_partial<<<1, YK_BS, 0, c->stream>>>(c->ch, imol, dm, dm + YD_MAXA, dm + 2 * YD_MAXA, c->d_sig6,
+                                                  c->d_4eps, c->d_io, back, back + N, back + 2 * N, back + 3 * N);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(o, c->d_io, sizeof(double) * (YK_NRS + 1), hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+static int pair_term(yamc_ctx *c, int i1, int i2, double *o3) {
+    yk_rspace_pair<<<1, 64, 0, c->stream>>>(c->ch, i1, i2, c->d_sig6, c->d_4eps, c->d_io);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(o3, c->d_io, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+    return sync(c);
+}
+
+// geometry.f90:170-201 center_of_mass
+static void com(const double *x, const double *y, const double *z, const double *m, int n, double *o) {
+    double sx = 0, sy = 0, sz = 0, tm = 0;
+    for (int i = 0; i < n; ++i) {
+        sx = sx + m[i] * x[i];
+        sy = sy + m[i] * y[i];
+        sz = sz + m[i] * z[i];
+        tm = tm + m[i];
+    }
+    o[0] = sx / tm; o[1] = sy / tm; o[2] = sz / tm;
+}
+
+// mc_xchange (MC.f90:1325-1954) of molecules i1 != i2: they swap centres of
+// mass, each keeping its orientation.  Real space: partial sums of each
+// molecule with both at the new places and their pair term; k space: dU of
+// the joint delta of both molecules' sums; external dU; accepted iff
+// rand() <= exp(-dU) (MonteCarloMove.f90:572-597), no draw after an overlap.
+// The acceptance dU_real = sum(uu new) - sum(uu old) over both molecules
+// counts the pair term twice, as in the reference; the stored components
+// subtract it once (MC.f90:1829-1859).
+static int xchange_step(yamc_ctx *c, int i1, int i2, const double *u, yamc_move_rec *rec, int *accepted) {
+    TRY(xchg_buffers(c));
+    const int N = c->natom, f1 = c->h_first[i1], n1 = c->h_last[i1] - f1 + 1, f2 = c->h_first[i2],
+              n2 = c->h_last[i2] - f2 + 1;
+    double mu[3];
+    TRY(yamc_external_mu(c, mu));  // from scratch (MC.f90:1429)
+    double xo1[YD_MAXA], yo1[YD_MAXA], zo1[YD_MAXA], xo2[YD_MAXA], yo2[YD_MAXA], zo2[YD_MAXA], uo1[YD_MAXA],
+        uo2[YD_MAXA];
+    const std::pair<double *, int> reads[8] = {{xo1, 0}, {yo1, 1}, {zo1, 2}, {uo1, 3},
+                                               {xo2, 0}, {yo2, 1}, {zo2, 2}, {uo2, 3}};
+    const double *src[4] = {c->ch.x, c->ch.y, c->ch.z, c->ch.uu};
+    for (int k = 0; k < 8; ++k) {
+        const int f = k < 4 ? f1 : f2, n = k < 4 ? n1 : n2;
+        HIPCHK(hipMemcpyAsync(reads[k].first, src[reads[k].second] + f, sizeof(double) * n, hipMemcpyDeviceToHost,
+                              c->stream));
+    }
+    TRY(sync(c));
+    double s1 = 0.0, s2 = 0.0;
+    for (int a = 0; a < n1; ++a) s1 = s1 + uo1[a];
+    for (int a = 0; a < n2; ++a) s2 = s2 + uo2[a];
+    const double U_old = s1 + s2;
+    const double *q1 = c->h_q.data() + f1, *q2 = c->h_q.data() + f2;
+    double muo1[3] = {0, 0, 0}, muo2[3] = {0, 0, 0};
+    for (int a = 0; a < n1; ++a) {
+        muo1[0] = muo1[0] + q1[a] * xo1[a]; muo1[1] = muo1[1] + q1[a] * yo1[a]; muo1[2] = muo1[2] + q1[a] * zo1[a];
+    }
+    for (int a = 0; a < n2; ++a) {
+        muo2[0] = muo2[0] + q2[a] * xo2[a]; muo2[1] = muo2[1] + q2[a] * yo2[a]; muo2[2] = muo2[2] + q2[a] * zo2[a];
+    }
+    double c1[3], c2[3];  // MC.f90:1514-1531
+    com(xo1, yo1, zo1, c->h_mass.data() + f1, n1, c1);
+    com(xo2, yo2, zo2, c->h_mass.data() + f2, n2, c2);
+    double xn1[YD_MAXA], yn1[YD_MAXA], zn1[YD_MAXA], xn2[YD_MAXA], yn2[YD_MAXA], zn2[YD_MAXA];
+    for (int a = 0; a < n1; ++a) {
+        xn1[a] = xo1[a] - c1[0] + c2[0]; yn1[a] = yo1[a] - c1[1] + c2[1]; zn1[a] = zo1[a] - c1[2] + c2[2];
+    }
+    for (int a = 0; a < n2; ++a) {
+        xn2[a] = xo2[a] - c2[0] + c1[0]; yn2[a] = yo2[a] - c2[1] + c1[1]; zn2[a] = zo2[a] - c2[2] + c1[2];
+    }
+    // new partial sums with both molecules at their new places (MC.f90:1547-1584)
+    double *bn1 = c->d_xb, *bn2 = bn1 + 4 * (size_t)N, *bo1 = bn2 + 4 * (size_t)N, *bo2 = bo1 + 4 * (size_t)N;
+    double on1[YK_NRS + 1], on2[YK_NRS + 1], cn12[3];
+    TRY(write_pos(c, f1, n1, xn1, yn1, zn1));
+    TRY(write_pos(c, f2, n2, xn2, yn2, zn2));
+    TRY(partial_dev(c, i1, xn1, yn1, zn1, bn1, on1));
+    TRY(partial_dev(c, i2, xn2, yn2, zn2, bn2, on2));
+    TRY(pair_term(c, i1, i2, cn12));
+    TRY(write_pos(c, f1, n1, xo1, yo1, zo1));
+    TRY(write_pos(c, f2, n2, xo2, yo2, zo2));
+    yamc_move_rec r = {};
+    r.imol = i1;
+    r.imol2 = i2;
+    int acc = 0;
+    YdState st;
+    if (on1[YK_NRS] != 0.0 || on2[YK_NRS] != 0.0) {
+        r.overlap = 1;
+        TRY(get_state(c, &st));
+        ++st.n_moves;
+        ++st.n_overlap;
+        TRY(put_state(c, &st));
+    } else {
+        double v1 = 0.0, v2 = 0.0;
+        for (int a = 0; a < n1; ++a) v1 = v1 + on1[a];
+        for (int a = 0; a < n2; ++a) v2 = v2 + on2[a];
+        const double du_real = (v1 + v2) - U_old;
+        // joint delta of the two molecules' sums and its dU (MC.f90:1602-1688)
+        YxAtoms at = {};
+        at.n = n1 + n2;
+        for (int a = 0; a < at.n; ++a) {
+            const bool m1 = a < n1;
+            const int g = m1 ? f1 + a : f2 + a - n1, b = m1 ? a : a - n1;
+            at.typ[a] = c->h_type[g];
+            at.q[a] = c->h_q[g];
+            if (at.typ[a] > 0) at.pmask |= 1u << (at.typ[a] - 1);
+            at.pos[0][0][a] = m1 ? xo1[b] : xo2[b]; at.pos[0][1][a] = m1 ? yo1[b] : yo2[b];
+            at.pos[0][2][a] = m1 ? zo1[b] : zo2[b];
+            at.pos[1][0][a] = m1 ? xn1[b] : xn2[b]; at.pos[1][1][a] = m1 ? yn1[b] : yn2[b];
+            at.pos[1][2][a] = m1 ? zn1[b] : zn2[b];
+        }
+        yk_xchg_delta<<<(c->nk + 255) / 256, 256, 0, c->stream>>>(c->ch, at, c->d_xd);
+        if (c->precision == YAMC_FP32C) yk_xchg_du<true><<<1, 256, 0, c->stream>>>(c->ch, c->d_xd, at.pmask, c->d_io);
+        else yk_xchg_du<false><<<1, 256, 0, c->stream>>>(c->ch, c->d_xd, at.pmask, c->d_io);
+        HIPCHK(hipGetLastError());
+        double du2[2];
+        HIPCHK(hipMemcpyAsync(du2, c->d_io, sizeof du2, hipMemcpyDeviceToHost, c->stream));
+        TRY(sync(c));
+        const double du_kc = du2[0], du_klj = du2[1];
+        // external term (MC.f90:1692-1716)
+        double mun1[3] = {0, 0, 0}, mun2[3] = {0, 0, 0}, dmu[3], mu_new[3];
+        for (int a = 0; a < n1; ++a) {
+            mun1[0] = mun1[0] + q1[a] * xn1[a]; mun1[1] = mun1[1] + q1[a] * yn1[a]; mun1[2] = mun1[2] + q1[a] * zn1[a];
+        }
+        for (int a = 0; a < n2; ++a) {
+            mun2[0] = mun2[0] + q2[a] * xn2[a]; mun2[1] = mun2[1] + q2[a] * yn2[a]; mun2[2] = mun2[2] + q2[a] * zn2[a];
+        }
+        for (int d = 0; d < 3; ++d) dmu[d] = (mun1[d] - muo1[d]) + (mun2[d] - muo2[d]);
+        const double kext = c->ch.p.kext;
+        const double du_ext =
+            kext * ((2 * mu[0] + dmu[0]) * dmu[0] + (2 * mu[1] + dmu[1]) * dmu[1] + (2 * mu[2] + dmu[2]) * dmu[2]);
+        for (int d = 0; d < 3; ++d) mu_new[d] = mu[d] + dmu[d];
+        const double du_total = du_real + (du_kc + du_klj) + du_ext;
+        double xksi;
+        TRY(draw(c, u, &xksi));
+        acc = !(xksi > exp(-du_total));
+        r.du_real = du_real; r.du_kc = du_kc; r.du_klj = du_klj; r.du_ext = du_ext; r.du_total = du_total;
+        for (int d = 0; d < 3; ++d) r.delta[d] = c2[d] - c1[d];
+        double co12[3] = {0, 0, 0}, oo1[YK_NRS + 1], oo2[YK_NRS + 1];
+        if (acc) {  // MC.f90:1754-1872
+            if (c->precision == YAMC_FP32C)
+                yk_xchg_commit_sums<true><<<(unsigned)((((size_t)(1 + c->ntype) * 8 * c->nk) + 255) / 256), 256, 0,
+                                            c->stream>>>(c->ch, c->d_xd, at.pmask);
+            else
+                yk_xchg_commit_sums<false><<<(unsigned)((((size_t)(1 + c->ntype) * 8 * c->nk) + 255) / 256), 256, 0,
+                                             c->stream>>>(c->ch, c->d_xd, at.pmask);
+            HIPCHK(hipGetLastError());
+            TRY(partial_dev(c, i1, xo1, yo1, zo1, bo1, oo1));
+            TRY(partial_dev(c, i2, xo2, yo2, zo2, bo2, oo2));
+            TRY(pair_term(c, i1, i2, co12));
+            double mv[8 * YD_MAXA] = {};
+            for (int a = 0; a < n1; ++a)
+                for (int k = 0; k < 4; ++k) mv[4 * a + k] = k == 0 ? on1[a] : on1[YD_MAXA + 3 * a + k - 1];
+            for (int a = 0; a < n2; ++a)
+                for (int k = 0; k < 4; ++k) mv[4 * (YD_MAXA + a) + k] = k == 0 ? on2[a] : on2[YD_MAXA + 3 * a + k - 1];
+            HIPCHK(hipMemcpyAsync(c->d_xmv, mv, sizeof mv, hipMemcpyHostToDevice, c->stream));
+            yk_xchg_commit_rs<<<(N + 255) / 256, 256, 0, c->stream>>>(c->ch, bn1, bo1, bn2, bo2, c->d_xmv, f1, n1, f2,
+                                                                      n2);
+            HIPCHK(hipGetLastError());
+            TRY(write_pos(c, f1, n1, xn1, yn1, zn1));
+            TRY(write_pos(c, f2, n2, xn2, yn2, zn2));
+        }
+        TRY(get_state(c, &st));
+        ++st.n_moves;
+        if (acc) {
+            const int E = 4 * YD_MAXA;
+            ++st.n_accepted;
+            st.e_kc = st.e_kc + du_kc;
+            st.e_klj = st.e_klj + du_klj;
+            st.uu_ew = st.uu_ew + on1[E] - oo1[E] + on2[E] - oo2[E] - (cn12[0] - co12[0]);
+            st.uu_lj6 = st.uu_lj6 + on1[E + 1] - oo1[E + 1] + on2[E + 1] - oo2[E + 1] - (cn12[1] - co12[1]);
+            st.uu_lj12 = st.uu_lj12 + on1[E + 2] - oo1[E + 2] + on2[E + 2] - oo2[E + 2] - (cn12[2] - co12[2]);
+            for (int d = 0; d < 3; ++d) st.mu[d] = mu_new[d];
+        }
+        TRY(put_state(c, &st));
+    }
+    r.accepted = acc;
+    if (rec) *rec = r;
+    if (accepted) *accepted = acc;
+    return YAMC_OK;
+}
+
+// BiasedRandom.f90:80-109 on the host, for the exchange picker.
+static int host_choose(yamc_ctx *c, const std::vector<double> &beg, const std::vector<double> &end, int *out) {
+    double rnd;
+    TRY(draw(c, nullptr, &rnd));
+    *out = 0;
+    for (int i = 0; i < c->nmol; ++i)
+        if (rnd > beg[i] && rnd < end[i]) {
+            *out = i;
+            break;
+        }
+    return YAMC_OK;
+}
+
+// Cycles cyc0 .. cyc0+ncyc-1 of a run with exchange steps (runmc.f90:223-266):
+// the chain kernel runs the moves and ends its launch at an exchange step
+// (its exchange draw taken); the host picks the two molecules from the
+// exchange weights and runs the step, then the next launch continues.
+static int run_with_exchanges(yamc_ctx *c, int cyc0, int ncyc, yamc_move_rec *d_rec) {
+    const long long first = (long long)cyc0 * c->nmol, total = (long long)ncyc * c->nmol;
+    long long done = 0;
+    while (done < total) {
+        YdState st0, st1;
+        TRY(get_state(c, &st0));
+        YdChain ch = c->ch;
+        ch.rec = d_rec ? d_rec + first + done : nullptr;
+        ch.mv_begin = (int)((first + done) % c->nmol);
+        HIPCHK(hipMemcpyAsync(c->d_chain, &ch, sizeof ch, hipMemcpyHostToDevice, c->stream));
+        const long long rem = total - done;
+        const int nsteps = (int)((rem + c->nmol - 1) / c->nmol) + 1;
+        TRY(launch_chain(c, c->d_chain, c->d_sig6s, c->d_feps, 1, nsteps, 0, c->natom, (int)rem));
+        TRY(get_state(c, &st1));
+        done += st1.n_moves - st0.n_moves;
+        if (done >= total) break;
+        int i1, i2;  // runmc.f90:236-247
+        TRY(host_choose(c, c->xfreq_beg, c->xfreq_end, &i1));
+        i2 = i1;
+        for (int t = 0; i2 == i1; ++t) {
+            if (t > 1000000) return fail(YAMC_ERROR_PARAMETER, "exchange weights pick one molecule only");
+            TRY(host_choose(c, c->xfreq_beg, c->xfreq_end, &i2));
+        }
+        yamc_move_rec r;
+        TRY(xchange_step(c, i1, i2, nullptr, &r, nullptr));
+        if (d_rec) HIPCHK(hipMemcpyAsync(d_rec + first + done, &r, sizeof r, hipMemcpyHostToDevice, c->stream));
+        TRY(sync(c));
+        ++done;
+    }
+    HIPCHK(hipMemcpyAsync(c->d_chain, &c->ch, sizeof(YdChain), hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+
+extern "C" int yamc_mc_set_frequencies(yamc_ctx *c, const double *move_weights, const double *xchange_weights) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    const int M = c->nmol;
+    std::vector<double> mv(M), xc(M);
+    double smv = 0.0, sxc = 0.0;
+    int nx = 0;
+    for (int i = 0; i < M; ++i) {
+        mv[i] = move_weights ? move_weights[i] : 1.0;
+        xc[i] = xchange_weights ? xchange_weights[i] : 0.0;
+        if (!(mv[i] >= 0.0) || !(xc[i] >= 0.0) || !std::isfinite(mv[i]) || !std::isfinite(xc[i]))
+            return fail(YAMC_ERROR_PARAMETER, "weights of molecule %d must be finite and >= 0", i);
+        nx += xc[i] > 0.0;
+    }
+    for (int i = 0; i < M; ++i) smv = smv + mv[i];
+    for (int i = 0; i < M; ++i) sxc = sxc + xc[i];
+    if (smv + sxc <= 0.0) return fail(YAMC_ERROR_PARAMETER, "all weights are zero");
+    if (sxc > 0.0 && nx < 2) return fail(YAMC_ERROR_PARAMETER, "exchange weights must cover at least two molecules");
+    std::vector<double> fb(M), fe(M);
+    biased_init(M, mv.data(), fb.data(), fe.data());
+    c->xfreq_beg.assign(M, 0.0);
+    c->xfreq_end.assign(M, 0.0);
+    biased_init(M, xc.data(), c->xfreq_beg.data(), c->xfreq_end.data());
+    c->pxch = sxc / (smv + sxc);  // runmc.f90:61
+    c->ch.p.pxch = c->pxch;
+    HIPCHK(hipMemcpyAsync(c->ch.freq_beg, fb.data(), sizeof(double) * M, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->ch.freq_end, fe.data(), sizeof(double) * M, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_chain, &c->ch, sizeof(YdChain), hipMemcpyHostToDevice, c->stream));
+    return sync(c);
+}
+
+extern "C" int yamc_mc_xchange(yamc_ctx *c, int32_t imol1, int32_t imol2, const double *u, int32_t *accepted,
+                               yamc_move_rec *rec) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    if (!c->state_ready) return fail(YAMC_ERROR_INITIALIZATION, "yamc_mc_init has not run on this state");
+    if (imol1 < 0 || imol1 >= c->nmol || imol2 < 0 || imol2 >= c->nmol || imol1 == imol2)
+        return fail(YAMC_ERROR_PARAMETER, "exchange needs two distinct molecules in range (got %d, %d)", imol1, imol2);
+    TRY(sync(c));
+    int acc = 0;
+    TRY(xchange_step(c, imol1, imol2, u, rec, &acc));
+    if (accepted) *accepted = acc;
     return YAMC_OK;
 }

@@ -34,6 +34,7 @@ struct YdParams {
     int natom, nmol, ntype, kmax, nk, precision;
     double L, alpha, rmax2, xclb, kext, m2kext, dr, d_angle, lambda_f, lambda_c;
     double alpha_over_sqrt_pi, two_alpha_over_sqrt_pi;
+    double pxch;  // xchange_probability (runmc.f90:61); 0 = no exchange steps
 };
 
 // Scalars of the Monte Carlo state kept on the device.
@@ -45,6 +46,17 @@ struct YdState {
     unsigned long long ctr;
     long long n_moves, n_accepted, n_overlap, since_resync;
     unsigned long long phase[10];  // wall-clock ticks per chain-kernel phase (thread 0)
+};
+
+// The two molecules of an exchange step (mc_xchange, MC.f90:1325-1954):
+// their atoms, molecule 1 first, at the old and the new places.
+#define YX_MAXA (2 * YD_MAXA)
+struct YxAtoms {
+    int n;                      // atoms of both molecules
+    unsigned pmask;             // bit t-1: LJ type t is present among them
+    int typ[YX_MAXA];
+    double q[YX_MAXA];
+    double pos[2][3][YX_MAXA];  // [old, new][x, y, z][atom]
 };
 
 struct YdChain {
@@ -59,11 +71,12 @@ struct YdChain {
     double *rsb;          // per candidate of the current move: new and old back terms (YD_NBCAP x 8)
     int *nbl;             // candidate list of the current move (YD_NBCAP), wave kernel
     float4 *pos4;         // FP32 positions + molecule number bits, wave kernel's scan
-    double *freq_end;     // cumulative move intervals (BiasedRandom.f90:172-180)
+    double *freq_beg, *freq_end;  // move intervals (BiasedRandom.f90:41-75)
     YdState *st;
     yamc_move_rec *rec;   // optional per-move records
     const int *cimol;     // constant-uniform mode: molecule per move
     const double *cu;     // constant-uniform mode: uniform per move
+    int mv_begin;         // step number of the launch's first step in its run (cycle starts: multiples of nmol)
 };
 
 // ---------------------------------------------------------------- math ----

@@ -1242,26 +1242,25 @@ __device__ int yk_accept(const YdParams &p, const YkMcm &m, double dutot, YdRng 
     return xksi > factor ? 0 : 1;
 }
 
-// BiasedRandom.f90:192-220 with flat frequencies: first i with
-// beg(i) < rnd < end(i), else 1 (index 0 here).  The cumulative ends are
-// i/nmol to rounding, so the interval is looked up next to floor(rnd*nmol);
-// a binary search covers any case the guess misses.
+// BiasedRandom.f90:80-109 BiasedRandom_choose: first i with
+// beg(i) < rnd < end(i), else 1 (index 0 here).  With flat weights the
+// interval bounds are i/nmol to rounding, so the interval is looked up next
+// to floor(rnd*nmol) first; a binary search over the non-decreasing begins
+// covers any other weights.
 __device__ int yk_choose(const YdChain &c, double rnd) {
     const int n = c.p.nmol;
     int g = (int)(rnd * n);
     if (g > n - 1) g = n - 1;
-    for (int i = (g > 0 ? g - 1 : 0); i <= (g + 1 < n ? g + 1 : n - 1); ++i) {
-        double beg = i > 0 ? c.freq_end[i - 1] : 0.0;
-        if (rnd > beg && rnd < c.freq_end[i]) return i;
-    }
+    for (int i = (g > 0 ? g - 1 : 0); i <= (g + 1 < n ? g + 1 : n - 1); ++i)
+        if (rnd > c.freq_beg[i] && rnd < c.freq_end[i]) return i;
+    // the intervals are disjoint: only the last one with beg < rnd can hold rnd
     int lo = 0, hi = n - 1;
-    while (lo < hi) {  // first i with rnd < end(i)
-        int mid = (lo + hi) >> 1;
-        if (rnd < c.freq_end[mid]) hi = mid;
-        else lo = mid + 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (c.freq_beg[mid] < rnd) lo = mid;
+        else hi = mid - 1;
     }
-    double beg = lo > 0 ? c.freq_end[lo - 1] : 0.0;
-    if (rnd > beg && rnd < c.freq_end[lo]) return lo;
+    if (rnd > c.freq_beg[lo] && rnd < c.freq_end[lo]) return lo;
     return 0;
 }
 
@@ -1520,8 +1519,11 @@ __device__ __forceinline__ void yk_sec_pick(const YdChain &c, YkMol &m, YdRng &r
         imol = c.cimol[mv];
         rng.cu = c.cu[mv];
     } else {
-        if (mv % c.p.nmol == 0) (void)rng.next();  // pressure-step draw per cycle
-        (void)rng.next();                           // exchange draw per step
+        if ((mv + c.mv_begin) % c.p.nmol == 0) (void)rng.next();  // pressure-step draw per cycle
+        if (rng.next() < c.p.pxch) {  // exchange step (runmc.f90:232-254): handed to the host
+            m.imol = -1;
+            return;
+        }
         imol = yk_choose(c, rng.next());
     }
     m.imol = imol;
@@ -1686,7 +1688,7 @@ __device__ int yk_sec_accept(const YdChain &c, const YkMol &m, YkScalar &t0, con
     for (int d = 0; d < 3; ++d) t0.mu_new[d] = mn[d];
     if (c.rec && (threadIdx.x & 63) == 0) {
         yamc_move_rec r = {};
-        r.imol = m.imol; r.accepted = accd; r.overlap = 0;
+        r.imol = m.imol; r.accepted = accd; r.overlap = 0; r.imol2 = -1;
         r.du_real = du_real; r.du_kc = duc; r.du_klj = dulj; r.du_ext = du_ext; r.du_total = du_total;
         for (int d = 0; d < 3; ++d) {
             r.f_old[d] = mcm.fo[d]; r.t_old[d] = mcm.co[d]; r.f_new[d] = Fn[d]; r.t_new[d] = Tn[d];
@@ -1704,7 +1706,7 @@ __device__ __forceinline__ void yk_sec_reject_rec(const YdChain &c, const YkMol 
                                                   int overlap) {
     if (!c.rec) return;
     yamc_move_rec r = {};
-    r.imol = m.imol; r.overlap = overlap;
+    r.imol = m.imol; r.overlap = overlap; r.imol2 = -1;
     if (overlap == 1) {
         for (int d = 0; d < 3; ++d) { r.f_old[d] = mcm.fo[d]; r.t_old[d] = mcm.co[d]; r.delta[d] = mcm.d[d]; }
         r.angle = mcm.angle;
@@ -1785,6 +1787,7 @@ __global__ void __launch_bounds__(BS) yk_chain(const YdChain *chains, const doub
             sh.nbcount = 0;
         }
         __syncthreads();
+        if (sh.m.imol < 0) break;  // exchange step: the launch ends here and the host runs it
         if (tid < sh.m.na) {  // one lane per site loads the molecule
             YkMol &m = sh.m;
             int a = tid, g = m.first + a;
@@ -2071,6 +2074,7 @@ yk_wchain(const YdChain *chains, int nchains,
     for (int mv = 0; mv < total; ++mv) {
         yk_sec_pick(c, m, t0.rng, 0, mv);
         yw_sync();
+        if (m.imol < 0) break;  // exchange step: the host takes over (launched only without them)
         if (lane < m.na) {
             int a = lane, g = m.first + a;
             m.q[a] = c.q[g]; m.typ[a] = c.type[g]; m.mass[a] = c.mass[g]; m.dh[a] = c.dh[g];
@@ -2294,6 +2298,166 @@ __global__ void __launch_bounds__(YK_BS) yk_rspace_partial(YdChain c, int imol, 
                 }
             }
             uub[j] = b[0]; fxb[j] = b[1]; fyb[j] = b[2]; fzb[j] = b[3];
+        }
+    }
+}
+
+// ========================================================================
+// Exchange steps (mc_xchange, MC.f90:1325-1954).
+
+// Joint delta of the sums: for every set (Coulomb, then the present LJ
+// types), new minus old, each summed over the atoms in order into nulified
+// sums (RhoSquared_addAtom + RhoSquared_sub, MC.f90:1464-1488, 1602-1641).
+// D[(1+T)][8][nk] FP64; sets of absent types are not written.
+__global__ void __launch_bounds__(256) yk_xchg_delta(YdChain c, YxAtoms at, double *D) {
+    __shared__ double tab[2][YX_MAXA * 6 * YD_KP];
+    const int tid = threadIdx.x, n = at.n, nk = c.p.nk, T = c.p.ntype;
+    for (int w = tid; w < 2 * 3 * n; w += (int)blockDim.x) {
+        int s = w / (3 * n), r = w % (3 * n), a = r / 3, d = r % 3;
+        double *t = tab[s] + a * 6 * YD_KP + 2 * d * YD_KP;
+        yd_sincos_axis(at.pos[s][d][a], c.p.kmax, t, t + YD_KP);
+    }
+    __syncthreads();
+    for (int kk = blockIdx.x * blockDim.x + tid; kk < nk; kk += gridDim.x * blockDim.x) {
+        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        for (int set = 0; set <= T; ++set) {
+            if (set > 0 && !((at.pmask >> (set - 1)) & 1u)) continue;
+            double ns[8] = {0, 0, 0, 0, 0, 0, 0, 0}, os[8] = {0, 0, 0, 0, 0, 0, 0, 0}, krc[8], krl[8];
+            for (int a = 0; a < n; ++a) {
+                if (set == 0 ? at.q[a] == 0.0 : at.typ[a] != set) continue;
+                yd_kr_atom(tab[1] + a * 6 * YD_KP, at.q[a], at.typ[a], kx, ky, kz, krc, krl);
+                for (int i = 0; i < 8; ++i) ns[i] = ns[i] + (set == 0 ? krc[i] : krl[i]);
+                yd_kr_atom(tab[0] + a * 6 * YD_KP, at.q[a], at.typ[a], kx, ky, kz, krc, krl);
+                for (int i = 0; i < 8; ++i) os[i] = os[i] + (set == 0 ? krc[i] : krl[i]);
+            }
+            size_t base = (size_t)set * 8 * nk + kk;
+            for (int i = 0; i < 8; ++i) D[base + (size_t)i * nk] = ns[i] - os[i];
+        }
+    }
+}
+
+// Sum of the four sign slots of (2C + D1) D2 (EwaldSumKSpace.f90:528-753
+// slot order: ppp, pmp, ppm, pmm).
+__device__ __forceinline__ double yx_slots(const double *tw, const double *d2, int ky, int kz) {
+    double loc = tw[0] * d2[0] + tw[1] * d2[1];
+    if (ky > 0) loc = loc + tw[4] * d2[4] + tw[5] * d2[5];
+    if (kz > 0) loc = loc + tw[2] * d2[2] + tw[3] * d2[3];
+    if (ky > 0 && kz > 0) loc = loc + tw[6] * d2[6] + tw[7] * d2[7];
+    return loc;
+}
+
+// calc_dU_coulomb and calc_dU_LJ of the joint delta (EwaldSumKSpace.f90:
+// 528-753) against the current sums (FP32C: hi + lo); one workgroup.
+template <bool F32>
+__global__ void __launch_bounds__(256) yk_xchg_du(YdChain c, const double *D, unsigned pmask, double *out) {
+    __shared__ double red[4 * 2], res[2];
+    const int nk = c.p.nk, T = c.p.ntype;
+    double acc[2] = {0.0, 0.0};
+    for (int kk = threadIdx.x; kk < nk; kk += (int)blockDim.x) {
+        int kxyz = c.kxyz[kk], ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        double C[8], d[8], tw[8], d2[8];
+        for (int set = 0; set <= T; ++set) {
+            const bool pres = set == 0 || ((pmask >> (set - 1)) & 1u);
+            size_t base = (size_t)set * 8 * nk + kk;
+            for (int i = 0; i < 8; ++i) {
+                size_t idx = base + (size_t)i * nk;
+                C[i] = F32 ? (double)c.Shi[idx] + (double)c.Slo[idx] : c.S[idx];
+                d[i] = pres ? D[idx] : 0.0;
+                tw[i] = C[i] + C[i] + d[i];
+            }
+            if (set == 0) {
+                acc[0] += yx_slots(tw, d, ky, kz) * c.beta[kk];
+                continue;
+            }
+            for (int t2 = 1; t2 <= T; ++t2) {
+                if (!((pmask >> (t2 - 1)) & 1u)) continue;
+                size_t b2 = (size_t)t2 * 8 * nk + kk;
+                for (int i = 0; i < 8; ++i) d2[i] = D[b2 + (size_t)i * nk];
+                acc[1] += yx_slots(tw, d2, ky, kz) * c.beta_lj[((size_t)(set - 1) * T + (t2 - 1)) * nk + kk];
+            }
+        }
+    }
+    yd_block_sum<2>(acc, red, res);
+    if (threadIdx.x == 0) { out[0] = res[0]; out[1] = res[1]; }
+}
+template __global__ void yk_xchg_du<false>(YdChain, const double *, unsigned, double *);
+template __global__ void yk_xchg_du<true>(YdChain, const double *, unsigned, double *);
+
+// S += D for the Coulomb set and the present LJ sets (FP32C: hi/lo split of
+// the FP64 sum, as the move commit does).
+template <bool F32>
+__global__ void yk_xchg_commit_sums(YdChain c, const double *D, unsigned pmask) {
+    const int nk = c.p.nk, T = c.p.ntype;
+    const size_t n = (size_t)(1 + T) * 8 * nk;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        int set = (int)(i / ((size_t)8 * nk));
+        if (set > 0 && !((pmask >> (set - 1)) & 1u)) continue;
+        if (F32) {
+            double w = (double)c.Shi[i] + (double)c.Slo[i] + D[i];
+            float hi = (float)w;
+            c.Shi[i] = hi;
+            c.Slo[i] = (float)(w - (double)hi);
+        } else {
+            c.S[i] = c.S[i] + D[i];
+        }
+    }
+}
+template __global__ void yk_xchg_commit_sums<false>(YdChain, const double *, unsigned);
+template __global__ void yk_xchg_commit_sums<true>(YdChain, const double *, unsigned);
+
+// Molecule-molecule real-space term: EwaldSumRealSpace_calc of molecule 1
+// with j restricted to molecule 2's atoms (MC.f90:1573-1574, 1784-1785),
+// positions as they stand in c.x/y/z.  out: uu_ew, uu_lj6, uu_lj12; the
+// early return on an overlap is kept (EwaldSumRealSpace.f90:409-433).
+__global__ void yk_rspace_pair(YdChain c, int imol1, int imol2, const double *sig6, const double *feps,
+                               double *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int T = c.p.ntype, f1 = c.first[imol1], l1 = c.last[imol1], f2 = c.first[imol2], l2 = c.last[imol2];
+    double e0 = 0.0, e6 = 0.0, e12 = 0.0;
+    for (int i = f1; i <= l1; ++i) {
+        const int it = c.type[i];
+        for (int j = f2; j <= l2; ++j) {
+            const int jt = c.type[j];
+            double xij = yd_wrap(c.x[i] - c.x[j]), yij = yd_wrap(c.y[i] - c.y[j]), zij = yd_wrap(c.z[i] - c.z[j]);
+            double r2 = xij * xij + yij * yij + zij * zij;
+            double d12 = (c.dh[i] + c.dh[j]) / 2.0;
+            if (it > 0 && jt > 0 && r2 < d12 * d12) {
+                out[0] = e0; out[1] = e6; out[2] = e12;
+                return;
+            }
+            if (r2 >= c.p.rmax2) continue;
+            bool lj = it != 0 && jt != 0;
+            int off = lj ? (it - 1) * T + (jt - 1) : 0;
+            YdPair pr;
+            yd_pair(c.p, r2, false, c.q[i], c.q[j], lj, sig6[off], feps[off], pr);
+            e0 = e0 + pr.potew;
+            if (lj) {
+                e6 = e6 + pr.potlj6;
+                e12 = e12 + pr.potlj12;
+            }
+        }
+    }
+    out[0] = e0; out[1] = e6; out[2] = e12;
+}
+
+// Per-atom real-space arrays after an accepted exchange (MC.f90:1786-1810):
+// every atom takes new1 - old1 + new2 - old2 of the back arrays, then the
+// two molecules' atoms take their new partial sums (mv[m][a] = uu, fx, fy, fz).
+__global__ void yk_xchg_commit_rs(YdChain c, const double *bn1, const double *bo1, const double *bn2,
+                                  const double *bo2, const double *mv, int f1, int n1, int f2, int n2) {
+    const int N = c.p.natom;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
+        if (j >= f1 && j < f1 + n1) {
+            const double *v = mv + 4 * (j - f1);
+            c.uu[j] = v[0]; c.Fx[j] = v[1]; c.Fy[j] = v[2]; c.Fz[j] = v[3];
+        } else if (j >= f2 && j < f2 + n2) {
+            const double *v = mv + 4 * (YD_MAXA + j - f2);
+            c.uu[j] = v[0]; c.Fx[j] = v[1]; c.Fy[j] = v[2]; c.Fz[j] = v[3];
+        } else {
+            c.uu[j] = c.uu[j] + bn1[j] - bo1[j] + bn2[j] - bo2[j];
+            c.Fx[j] = c.Fx[j] + bn1[N + j] - bo1[N + j] + bn2[N + j] - bo2[N + j];
+            c.Fy[j] = c.Fy[j] + bn1[2 * N + j] - bo1[2 * N + j] + bn2[2 * N + j] - bo2[2 * N + j];
+            c.Fz[j] = c.Fz[j] + bn1[3 * N + j] - bo1[3 * N + j] + bn2[3 * N + j] - bo2[3 * N + j];
         }
     }
 }

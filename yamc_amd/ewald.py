@@ -213,6 +213,26 @@ class EwaldMC:
     def resync(self):
         check(self.lib.yamc_resync(self.ctx))
 
+    def set_frequencies(self, move: Optional[Sequence[float]] = None, xchange: Optional[Sequence[float]] = None):
+        """init_freq (runmc.f90:26-66) with per-molecule move and exchange
+        weights (None: the reference's 'none' file, moves 1, exchanges 0)."""
+        mv = None if move is None else np.ascontiguousarray(move, dtype=np.float64)
+        xc = None if xchange is None else np.ascontiguousarray(xchange, dtype=np.float64)
+        for a in (mv, xc):
+            if a is not None and a.shape != (self.sys.nmol,):
+                raise ValueError("one weight per molecule")
+        check(self.lib.yamc_mc_set_frequencies(self.ctx, None if mv is None else dptr(mv),
+                                               None if xc is None else dptr(xc)))
+
+    def xchange(self, imol1: int, imol2: int, u: Optional[float] = None):
+        """mc_xchange (MC.f90:1325-1954): returns (accepted, record)."""
+        acc = C.c_int32()
+        rec = np.zeros(1, dtype=MOVE_DTYPE)
+        up = C.byref(C.c_double(u)) if u is not None else None
+        check(self.lib.yamc_mc_xchange(self.ctx, int(imol1), int(imol2), up, C.byref(acc),
+                                       rec.ctypes.data_as(C.c_void_p)))
+        return bool(acc.value), rec[0]
+
     def volume_move(self, pressure_kT_A3: float, log_max_volume_scaling: float, u: Optional[float] = None):
         """mcvol (MC.f90:701-825): returns (accepted, box length in Angstrom)."""
         acc = C.c_int32()
