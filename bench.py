@@ -73,6 +73,11 @@ def main():
     ap.add_argument("--replica-check", action="store_true",
                     help="instead of the headline line: 64 uneven FP32C replicas (128-2048 waters, 0-1 M NaCl, "
                          "280-360 K, seeded) advanced by one captured launch per sweep, launch shape autotuned")
+    ap.add_argument("--npt-check", type=int, default=0, metavar="CYCLES",
+                    help="instead of the headline line: the reference's own NPT run (100 SPC/E + 5 Na + 5 Cl, "
+                         "its parameters.prm) for CYCLES run_mc cycles on the device, compared with the "
+                         "reference's energy.dat, boxlength.dat and RDF_*.dat (means from cycle 2000 on)")
+    ap.add_argument("--npt-precision", choices=("fp64", "fp32c"), default="fp64")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain fp32 streaming copy of MIB MiB (3 launches) and print its byte count; "
                          "used to scale the FETCH_SIZE / WRITE_SIZE counters")
@@ -109,6 +114,9 @@ def main():
         return
     if args.replica_check:
         replica_check(args)
+        return
+    if args.npt_check:
+        npt_check(args)
         return
     kT_kjmol = system.BOLTZ * 298.0 / system.KCAL_MOL * 4.184
 
@@ -444,6 +452,78 @@ def replica_check(args):
         "moves_per_s": moves_per_sweep * args.steps / dt, "ms_per_sweep": 1e3 * dt / args.steps,
         "kernel_ms_per_sweep": kms / args.steps, "acceptance": nacc / (moves_per_sweep * args.steps),
         "kspace_drift_rel_after_run": drift,
+    }
+    print(json.dumps(out))
+
+
+def npt_check(args):
+    """The reference's ions100 NPT run (tests/golden/ions100: parameters.prm,
+    ions100.composition and its output files) repeated on the device from a
+    seeded lattice start: mean energy and box length from cycle 2000 on with
+    block standard errors, and the centre-of-mass RDFs (frames every 100
+    cycles, r < 7 A), against the reference's files."""
+    import numpy as np
+
+    from yamc_amd import formats as fm
+    from yamc_amd import system as ys
+    from yamc_amd.ewald import rdf_normalize
+    from yamc_amd.runmc import run
+
+    d = os.path.join(ROOT, "tests", "golden", "ions100")
+    prm = fm.read_parameters(os.path.join(d, "parameters.prm"))
+    species, counts = fm.read_composition(os.path.join(d, "ions100.composition"))
+    params = fm.params_from(prm, sum(counts))
+    sysm = ys.lattice_system(params, species, counts, 20261207)
+    ncyc, start, dr, rmax = args.npt_check, 2000, 0.1, 7.0
+    res = run(sysm, prm, ncyc, 1 if args.npt_precision == "fp32c" else 0, seed=20261207,
+              rdf=(sysm.mol_type, 3, dr, rmax, 100, start), progress=max(1, ncyc // 20))
+
+    def block_mean(v, nb=8):
+        v = np.asarray(v, dtype=np.float64)
+        m = v[: len(v) // nb * nb].reshape(nb, -1).mean(1)
+        return float(v.mean()), float(m.std(ddof=1) / np.sqrt(nb))
+
+    e_ref = fm.read_table(os.path.join(d, "energy.dat"))
+    b_ref = fm.read_table(os.path.join(d, "boxlength.dat"))
+    u_ref = e_ref[e_ref[:, 0] >= start, 1]
+    grid = np.arange(1, int(e_ref[-1, 0]) + 1)
+    idx = np.searchsorted(b_ref[:, 0], grid, side="right") - 1
+    L_ref = np.where(idx >= 0, b_ref[np.maximum(idx, 0), 1], params.box_length)[start - 1:]
+    e = res["energy"]
+    u_mine = e[e[:, 0] >= start, 1]
+    L_mine = res["box"][start - 1:]
+    (ur, ur_e), (um, um_e) = block_mean(u_ref, 4), block_mean(u_mine, 4)
+    (lr, lr_e), (lm, lm_e) = block_mean(L_ref), block_mean(L_mine)
+    g = rdf_normalize(res["rdf_counts"].astype(np.float64), res["rdf_frames"], counts, 1.0 / res["rdf_inv_vol"], dr)
+    names = ["water", "Na", "Cl"]
+    rdfs = {}
+    for i in range(3):
+        for j in range(i, 3):
+            ref = fm.read_table(os.path.join(d, f"RDF_{names[i]}_{names[j]}.dat"))
+            nb = min(g.shape[2], len(ref))
+            r = ref[:nb, 0]
+            sel = (r > 2.0) & (r < rmax)
+            cnt = res["rdf_counts"][i, j, :nb].astype(np.float64)
+            # Poisson error of this run's bins, doubled for the reference's own noise
+            err = np.where(cnt > 0, g[i, j, :nb] / np.sqrt(np.maximum(cnt, 1.0)), np.nan) * np.sqrt(2.0)
+            dg = g[i, j, :nb] - ref[:nb, 1]
+            z = np.abs(dg[sel]) / err[sel]
+            pk = int(np.argmax(ref[:nb, 1] * sel))
+            rdfs[f"{names[i]}_{names[j]}"] = {
+                "rms_diff_2_7A": float(np.sqrt(np.mean(dg[sel] ** 2))),
+                "median_z": float(np.nanmedian(z)),
+                "peak_r_ref": float(r[pk]), "peak_g_ref": float(ref[pk, 1]),
+                "peak_r": float(r[int(np.argmax(g[i, j, :nb] * sel))]), "peak_g": float(np.max(g[i, j, :nb] * sel))}
+    out = {
+        "check": "reference ions100 NPT run repeated on the device (its parameters.prm and composition)",
+        "precision": args.npt_precision, "cycles": ncyc, "seconds": res["seconds"],
+        "moves": res["n_mv"], "moves_accepted": res["n_mv_acc"], "volume_steps": res["n_vol"],
+        "volume_steps_accepted": res["n_vol_acc"],
+        "U_total_mean_from_2000": {"this": um, "this_err": um_e, "reference": ur, "reference_err": ur_e,
+                                   "z": (um - ur) / float(np.hypot(um_e, ur_e))},
+        "box_length_mean_from_2000": {"this": lm, "this_err": lm_e, "reference": lr, "reference_err": lr_e,
+                                      "z": (lm - lr) / float(np.hypot(lm_e, lr_e))},
+        "rdf_frames": res["rdf_frames"], "rdf": rdfs,
     }
     print(json.dumps(out))
 

@@ -37,6 +37,8 @@ program ref_driver
    use RhoSquared
    use ForceKSpace
    use SumSinCosKR
+   use FloatingPoint, only : write_float
+   use io, only : io_open, io_close
    implicit none
 
    character(SYSTEM_STRING_LENGTH) :: case_file, comp_file
@@ -47,6 +49,10 @@ program ref_driver
    Type(TRhoSquared) :: scratch
    Type(TSinCosKR) :: krc, krl
    integer :: nmol, i, j, natom, nk, t1, t2, nt, nmoves, imol, species, f, l, uin, nvol, ios, imol2, nxchg
+   integer :: nb, hfile
+   character(SYSTEM_STRING_LENGTH) :: fname
+   real(8), dimension(10) :: specials = (/ 0.d0, 1.d0, -1.d0, 0.75d0, -3.5d0, 2.d0, 1.d-15, -1.d15, 123456.789d0, &
+                                           -2.5d-7 /)
    real(8) :: bl, u, fxc, fyc, fzc, fxl, fyl, fzl, pvol, mvs
    logical :: accepted
    Type(TMolecule), pointer :: mol_ptr
@@ -221,6 +227,32 @@ program ref_driver
       call dump_real('FZX', RealSpace_TotalSum % Fz, natom)
       call dump_sums(KSpace_TotalSum % rho_squared_total % sumsincos_coulomb, nk)
    end if
+   ! formats: molecule table of the final configuration (positions and
+   ! orientations from the atoms), its binary moltab at 1, 2 and 3 bytes per
+   ! sample, and 24+8-bit floats of the real-space forces / BoxLength (the
+   ! mc_calc_forces output) followed by a few special values
+   call MoleculeTable_calcPositionsOrientations(mol_table, atomic_data % xx, atomic_data % yy, &
+                                                atomic_data % zz, BoxLength)
+   write(*, '(A,I8,ES26.17)') 'MOLTAB', nmol, BoxLength
+   do i = 1, nmol
+      write(*, '(6ES26.17)') mol_table % x(i), mol_table % y(i), mol_table % z(i), mol_table % theta(i), &
+                             mol_table % phi(i), mol_table % psi(i)
+   end do
+   do nb = 1, 3
+      write(fname, '(A,I1,A)') 'moltab', nb, '.bin'
+      hfile = io_open(fname, 'b')
+      call MoleculeTable_save_binary(mol_table, hfile, nb, nb)
+      call io_close(hfile)
+   end do
+   fname = 'floats.bin'
+   hfile = io_open(fname, 'b')
+   do i = 1, natom
+      call write_float(hfile, RealSpace_TotalSum % Fx(i) / BoxLength)
+   end do
+   do i = 1, 10
+      call write_float(hfile, specials(i))
+   end do
+   call io_close(hfile)
    write(*, '(A)') 'END'
 
 contains

@@ -91,6 +91,11 @@ def parse(text):
             out["XCHGS"].append(cur)
         elif tok[0] in ("MOVES", "VOLS", "XCHGS"):
             pass
+        elif tok[0] == "MOLTAB":
+            n = int(tok[1])
+            out["MOLTAB_L"] = float(tok[2])
+            out["MOLTAB"] = [[float(v) for v in lines[i + 1 + k].split()] for k in range(n)]
+            i += n
         elif tok[0] == "END":
             break
         elif len(tok) == 1 and cur is not None and "E" in cur and "pos" not in cur:
@@ -154,6 +159,10 @@ def make_case(name, sysm, nmoves, seed, vols=(), xchgs=()):
         r = subprocess.run([DRIVER, "case.in"], cwd=td, capture_output=True, text=True, timeout=120)
         if r.returncode != 0:
             raise RuntimeError(r.stderr)
+        blobs = {}
+        for fn in ("moltab1.bin", "moltab2.bin", "moltab3.bin", "floats.bin"):
+            with open(os.path.join(td, fn), "rb") as fb:
+                blobs[fn] = np.frombuffer(fb.read(), dtype=np.uint8).copy()
     g = parse(r.stdout)
     natom, nk, nt = g["SIZES"]
     moves = g["MOVES"]
@@ -208,6 +217,12 @@ def make_case(name, sysm, nmoves, seed, vols=(), xchgs=()):
             xchg_e=np.array([x["E"] for x in xs]), xchg_pos=np.array([x["pos"] for x in xs]),
             uu_xchg=np.array(g["UUX"]), fx_xchg=np.array(g["FXX"]), fy_xchg=np.array(g["FYX"]),
             fz_xchg=np.array(g["FZX"]), sums_coulomb_xchg=np.array(g["SUMS"][ns:ns + 8]).reshape(8, nk))
+    # formats of the final configuration (after the exchange steps)
+    arrays.update(
+        moltab_final=np.array(g["MOLTAB"]), moltab_box=np.float64(g["MOLTAB_L"]),
+        moltab1=blobs["moltab1.bin"], moltab2=blobs["moltab2.bin"], moltab3=blobs["moltab3.bin"],
+        floats=blobs["floats.bin"],
+        float_specials=np.array([0.0, 1.0, -1.0, 0.75, -3.5, 2.0, 1e-15, -1e15, 123456.789, -2.5e-7]))
     path = os.path.join(OUT, f"{name}.npz")
     np.savez_compressed(path, **arrays)
     print(f"{path}: natom={natom} nk={nk} ntype={nt} moves={nmoves} accepted={arrays['move_accepted'].sum()}")

@@ -624,3 +624,33 @@ def test_batch_refuses_exchange_contexts(lib):
     g.mc_init()
     with pytest.raises(L.YamcError):
         batch_run([g], 1)
+
+
+def test_runmc_driver_npt_files(lib, tmp_path):
+    """The run_mc host driver on the reference's ions100 parameter and
+    composition files (300 NPT cycles): output files consistent with the
+    run's counters and final state."""
+    from yamc_amd import formats as fm
+    from yamc_amd import system as ys
+    from yamc_amd.runmc import run
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ions100")
+    prm = fm.read_parameters(os.path.join(d, "parameters.prm"))
+    prm.update(n_store_energy_interval=50.0, n_store_traj_interval=100.0)
+    species, counts = fm.read_composition(os.path.join(d, "ions100.composition"))
+    sysm = ys.lattice_system(fm.params_from(prm, sum(counts)), species, counts, 3)
+    res = run(sysm, prm, 300, 0, seed=3, out_dir=str(tmp_path))
+    assert res["n_vol"] > 0 and res["n_vol_acc"] > 0
+    assert res["n_mv"] + res["n_vol"] * sysm.nmol == 300 * sysm.nmol
+    e = fm.read_table(str(tmp_path / "energy.dat"))
+    assert e.shape == (6, 9) and np.array_equal(e[:, 0], np.arange(50, 301, 50))
+    b = fm.read_table(str(tmp_path / "boxlength.dat"))
+    assert len(b) == res["n_vol_acc"] and abs(b[-1, 1] - res["box"][-1]) < 1e-12 * b[-1, 1]
+    raw = open(tmp_path / "traj.moltraj", "rb").read()
+    assert len(raw) == 3 * sysm.nmol * 12
+    fr = fm.read_table(str(tmp_path / "frames.dat"))
+    assert fr.shape[0] == 3 and fr[-1, 4] == res["n_vol"]
+    pos, _ = fm.moltab_decode(raw, sysm.nmol, 2, 2, offset=2 * sysm.nmol * 12)
+    assert np.all(np.abs(pos) <= 0.5 + 1e-4)
+    # the last energy row is the final state
+    assert abs(e[-1, 1] - res["final_energies"]["U_total"]) < 1e-9 * abs(e[-1, 1])
+    assert abs(e[-1, 1] - e[-1, 2] - e[-1, 3]) < 1e-9 * abs(e[-1, 1])
