@@ -56,7 +56,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--block", type=int, default=0, help="batch launch shape (64, 128, 256, 512); 0 = autotune")
     ap.add_argument("--cpu-moves", type=int, default=0, help="moves for the CPU baseline (0 = auto ~15 s)")

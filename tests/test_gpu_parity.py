@@ -654,3 +654,34 @@ def test_runmc_driver_npt_files(lib, tmp_path):
     # the last energy row is the final state
     assert abs(e[-1, 1] - res["final_energies"]["U_total"]) < 1e-9 * abs(e[-1, 1])
     assert abs(e[-1, 1] - e[-1, 2] - e[-1, 3]) < 1e-9 * abs(e[-1, 1])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_two_chains_per_cu_equals_single(lib, precision):
+    """More 256-thread chains than CUs: the launch keeps the scan's position
+    copy in HBM so that two chains share a CU; every replica still equals its
+    own run alone."""
+    import torch
+    from yamc_amd import replica_system
+    from yamc_amd.ewald import EwaldMC, batch_run
+    ncu = torch.cuda.get_device_properties(0).multi_processor_count
+    n = ncu + 8
+    systems = [replica_system(27 + (i % 5) * 4, 0.5 * (i % 3), 290.0 + i % 40, seed=100 + i) for i in range(n)]
+    a = [EwaldMC(s, precision=precision) for s in systems]
+    for i, x in enumerate(a):
+        x.set_block_size(256)
+        x.mc_init()
+        x.set_rng(23, i)
+    nacc, _ = batch_run(a, 2)
+    for i in (0, 1, n // 2, n - 1):
+        y = EwaldMC(systems[i], precision=precision)
+        y.set_block_size(256)
+        y.mc_init()
+        y.set_rng(23, i)
+        m, _ = y.run(2)
+        assert m == nacc[i]
+        for p, q in zip(a[i].positions(), y.positions()):
+            assert np.array_equal(p, q)
+        y.close()
+    for x in a:
+        x.close()

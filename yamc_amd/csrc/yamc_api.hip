@@ -238,7 +238,7 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
             TRY(dalloc(c, &h.S, nS));
         }
         TRY(dalloc(c, &h.uu, N)); TRY(dalloc(c, &h.Fx, N)); TRY(dalloc(c, &h.Fy, N)); TRY(dalloc(c, &h.Fz, N));
-        TRY(dalloc(c, &h.rsb, (size_t)YD_NBCAP * 8));
+        TRY(dalloc(c, &h.rsb, (size_t)YD_NBCAP * 4));
         TRY(dalloc(c, &h.nbl, (size_t)YD_NBCAP)); TRY(dalloc(c, &h.pos4, (size_t)N));
         TRY(dalloc(c, &h.freq_end, M)); TRY(dalloc(c, &h.freq_beg, M)); TRY(dalloc(c, &h.st, 1));
         TRY(dalloc(c, &c->d_lj6, TT)); TRY(dalloc(c, &c->d_lj12, TT));
@@ -306,15 +306,16 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
 
 extern "C" int yamc_ctx_destroy(yamc_ctx *c) {
     if (!c) return YAMC_OK;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (void *p : c->owned) hipFree(p);
-    for (void *p : c->vol_save) hipFree(p);
-    if (c->d_xd) hipFree(c->d_xd);
-    if (c->d_xb) hipFree(c->d_xb);
-    if (c->d_xmv) hipFree(c->d_xmv);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    // teardown: errors here have no one to report to
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->owned) (void)hipFree(p);
+    for (void *p : c->vol_save) (void)hipFree(p);
+    if (c->d_xd) (void)hipFree(c->d_xd);
+    if (c->d_xb) (void)hipFree(c->d_xb);
+    if (c->d_xmv) (void)hipFree(c->d_xmv);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return YAMC_OK;
 }
@@ -674,7 +675,12 @@ static int launch_chain_t(yamc_ctx *c, const YdChain *d_chain, const double *con
     size_t want = (size_t)max_natom * 16;
     int lds_atoms = 0;
     size_t dyn = 0;
-    if (want + fa.sharedSizeBytes + 1024 <= (size_t)lds_max) {
+    // More chains than CUs at 256 threads: leave the position copy in HBM so
+    // that two chains fit on a CU.
+    int ncu = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    const bool two_per_cu = BS == 256 && nchains > ncu;
+    if (!two_per_cu && want + fa.sharedSizeBytes + 1024 <= (size_t)lds_max) {
         lds_atoms = max_natom;
         dyn = want;
         HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
@@ -841,11 +847,11 @@ static int batch_stage(yamc_ctx *const *ctxs, int32_t n, yamc_batch *b) {
 }
 
 static void batch_free(yamc_batch *b) {
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    if (b->d_ch) hipFree(b->d_ch);
-    if (b->d_s6) hipFree(b->d_s6);
-    if (b->d_fe) hipFree(b->d_fe);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->d_ch) (void)hipFree(b->d_ch);
+    if (b->d_s6) (void)hipFree(b->d_s6);
+    if (b->d_fe) (void)hipFree(b->d_fe);
     delete b;
 }
 
@@ -901,7 +907,7 @@ extern "C" int yamc_batch_create(yamc_ctx *const *ctxs, int32_t n, int32_t ncycl
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(c0->stream, &g);
             if (lrc != YAMC_OK) {
-                if (g) hipGraphDestroy(g);
+                if (g) (void)hipGraphDestroy(g);
                 return lrc;
             }
             HIPCHK(e);

@@ -68,7 +68,7 @@ struct YdChain {
     double *S;            // FP64 sums
     float *Shi, *Slo;     // FP32C sums
     double *uu, *Fx, *Fy, *Fz;
-    double *rsb;          // per candidate of the current move: new and old back terms (YD_NBCAP x 8)
+    double *rsb;          // per candidate of the current move: new - old back terms (YD_NBCAP x 4)
     int *nbl;             // candidate list of the current move (YD_NBCAP), wave kernel
     float4 *pos4;         // FP32 positions + molecule number bits, wave kernel's scan
     double *freq_beg, *freq_end;  // move intervals (BiasedRandom.f90:41-75)

@@ -899,8 +899,9 @@ __device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const float4 *pos, 
 // it evaluates the molecule's new sites (the partial sum and the overlap
 // test, EwaldSumRealSpace.f90:338-498) and its old sites, and stores j's
 // back terms of both (uu, -F summed over the sites in index order, as the
-// back arrays of the two partial sums hold them, MC.f90:1251-1275) in
-// c.rsb, so that an accepted move commits them without a second pair pass.
+// back arrays of the two partial sums hold them, MC.f90:1251-1275) as one
+// difference new - old in c.rsb, so that an accepted move commits them
+// without a second pair pass.
 template <int NA>
 __device__ void yk_rs_exact(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
                             const int *list, int count, double (&acc)[YK_NRX], int &ov, int rank, int size) {
@@ -946,9 +947,8 @@ __device__ void yk_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
             acc[YK_NRS + 1] += pr.potlj6;
             acc[YK_NRS + 2] += pr.potlj12;
         }
-        double *b = c.rsb + (size_t)i * 8;
-        b[0] = nb[0]; b[1] = nb[1]; b[2] = nb[2]; b[3] = nb[3];
-        b[4] = ob[0]; b[5] = ob[1]; b[6] = ob[2]; b[7] = ob[3];
+        double *b = c.rsb + (size_t)i * 4;
+        b[0] = nb[0] - ob[0]; b[1] = nb[1] - ob[1]; b[2] = nb[2] - ob[2]; b[3] = nb[3] - ob[3];
     }
 }
 
@@ -956,11 +956,11 @@ __device__ void yk_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
 __device__ void yk_rs_commit_list(const YdChain &c, const int *list, int count, int rank, int size) {
     for (int i = rank; i < count; i += size) {
         int j = list[i];
-        const double *b = c.rsb + (size_t)i * 8;
-        c.uu[j] = c.uu[j] + b[0] - b[4];
-        c.Fx[j] = c.Fx[j] + b[1] - b[5];
-        c.Fy[j] = c.Fy[j] + b[2] - b[6];
-        c.Fz[j] = c.Fz[j] + b[3] - b[7];
+        const double *b = c.rsb + (size_t)i * 4;
+        c.uu[j] = c.uu[j] + b[0];
+        c.Fx[j] = c.Fx[j] + b[1];
+        c.Fy[j] = c.Fy[j] + b[2];
+        c.Fz[j] = c.Fz[j] + b[3];
     }
 }
 
@@ -1476,9 +1476,8 @@ __device__ void yl_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
             acc[YK_NRS + 1] += pr.potlj6;
             acc[YK_NRS + 2] += pr.potlj12;
         }
-        double *b = c.rsb + (size_t)i * 8;
-        b[0] = nb[0]; b[1] = nb[1]; b[2] = nb[2]; b[3] = nb[3];
-        b[4] = ob[0]; b[5] = ob[1]; b[6] = ob[2]; b[7] = ob[3];
+        double *b = c.rsb + (size_t)i * 4;
+        b[0] = nb[0] - ob[0]; b[1] = nb[1] - ob[1]; b[2] = nb[2] - ob[2]; b[3] = nb[3] - ob[3];
     }
 }
 
@@ -1742,10 +1741,12 @@ __device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m
 // spread over the workgroup.  mode 0: run_mc cycles (runmc.f90:204-268)
 // with the Philox stream, nsteps = cycles; mode 1: nsteps moves with
 // caller-given molecule and constant uniform per move.
+// Both shapes are built for two waves per SIMD (the 256-thread form so that
+// two chains share a CU when the launch leaves the positions in HBM).
 template <bool F32, int BS>
-__global__ void __launch_bounds__(BS) yk_chain(const YdChain *chains, const double *const *sig6s,
-                                               const double *const *feps_s, int nsteps, int mode, int max_moves,
-                                               int lds_atoms) {
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
+yk_chain(const YdChain *chains, const double *const *sig6s, const double *const *feps_s, int nsteps, int mode,
+         int max_moves, int lds_atoms) {
     __shared__ YkShared sh;
     extern __shared__ float4 lds_pos[];  // FP32 positions when the launch provides LDS for them
     const YdChain &c = chains[blockIdx.x];
@@ -1753,9 +1754,11 @@ __global__ void __launch_bounds__(BS) yk_chain(const YdChain *chains, const doub
     const int tid = threadIdx.x;
     const YdParams &p = c.p;
     const bool use_lds = lds_atoms >= p.natom;
-    if (use_lds)
-        for (int i = tid; i < p.natom; i += (int)blockDim.x)
-            lds_pos[i] = make_float4((float)c.x[i], (float)c.y[i], (float)c.z[i], __int_as_float(c.molnum[i]));
+    // FP32 copy of the positions for the real-space scan: in LDS when the
+    // launch gives room for it, else in the context's pos4 buffer
+    float4 *const spos = use_lds ? lds_pos : c.pos4;
+    for (int i = tid; i < p.natom; i += (int)blockDim.x)
+        spos[i] = make_float4((float)c.x[i], (float)c.y[i], (float)c.z[i], __int_as_float(c.molnum[i]));
     YkScalar &t0 = sh.t0;
     // mu = sum_j q_j r_j from scratch once per launch (EwaldSumExternal.f90:
     // 165-183); afterwards it is carried incrementally, mu + dmu on each
@@ -1838,7 +1841,7 @@ __global__ void __launch_bounds__(BS) yk_chain(const YdChain *chains, const doub
             if (nwords <= YK_BMWORDS) {
                 for (int i = tid; i < nwords; i += (int)blockDim.x) sh.bm[i] = 0u;
                 __syncthreads();
-                yk_rs_scan_n(c, sh.m, use_lds ? lds_pos : nullptr, sh.bm);
+                yk_rs_scan_n(c, sh.m, spos, sh.bm);
                 __syncthreads();
                 yk_compact(sh.bm, nwords, sh.nblist, &sh.nbcount, sh.wsum);
             } else if (tid == 0) {
@@ -1915,7 +1918,7 @@ __global__ void __launch_bounds__(BS) yk_chain(const YdChain *chains, const doub
         __syncthreads();
         YK_TICK(7);
         if (tid == 0) {
-            yk_sec_finalize(c, sh.m, t0, use_lds ? lds_pos : nullptr);
+            yk_sec_finalize(c, sh.m, t0, spos);
             for (int d = 0; d < 3; ++d) t0.mu[d] = t0.mu_new[d];
         }
         __syncthreads();
