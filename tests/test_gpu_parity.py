@@ -112,7 +112,7 @@ def test_kspace_delta_and_forces(lib, name):
 
 
 @pytest.mark.parametrize("block", [256, 512, 64, 128])
-@pytest.mark.parametrize("name,ncyc", [("small", 20), ("small_nobias", 20), ("c1", 1)])
+@pytest.mark.parametrize("name,ncyc", [("small", 20), ("small_nobias", 20), ("c1", 1), ("cells", 1)])
 def test_mc_trace_matches_oracle(lib, name, ncyc, block):
     """Every launch shape of the chain kernel (workgroup of 256 or 512 per
     chain, or one wave per chain) against the oracle on one RNG stream."""
@@ -446,7 +446,7 @@ def test_fp32c_rdf_within_statistical_error(lib):
         assert (z ** 2).mean() < 2.5, (i, j, (z ** 2).mean())
 
 
-@pytest.mark.parametrize("name,ncyc,psm", [("small", 24, 1.0), ("c1", 4, 1.0)])
+@pytest.mark.parametrize("name,ncyc,psm", [("small", 24, 1.0), ("c1", 4, 1.0), ("cells", 4, 1.0)])
 def test_npt_cycles_match_oracle(lib, name, ncyc, psm):
     """run_mc cycles with volume steps (runmc.f90:204-268, mcvol
     MC.f90:701-825) on one RNG stream: identical move and volume decisions,
@@ -656,17 +656,20 @@ def test_runmc_driver_npt_files(lib, tmp_path):
     assert abs(e[-1, 1] - e[-1, 2] - e[-1, 3]) < 1e-9 * abs(e[-1, 1])
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_batch_two_chains_per_cu_equals_single(lib, precision):
+@pytest.mark.parametrize("precision,cells", [(0, False), (1, False), (1, True)])
+def test_batch_two_chains_per_cu_equals_single(lib, precision, cells):
     """More 256-thread chains than CUs: the launch keeps the scan's position
     copy in HBM so that two chains share a CU; every replica still equals its
-    own run alone."""
+    own run alone (also with the cell-list scan: 3 A cut-off, 4 cells per
+    axis)."""
     import torch
     from yamc_amd import replica_system
     from yamc_amd.ewald import EwaldMC, batch_run
     ncu = torch.cuda.get_device_properties(0).multi_processor_count
     n = ncu + 8
-    systems = [replica_system(27 + (i % 5) * 4, 0.5 * (i % 3), 290.0 + i % 40, seed=100 + i) for i in range(n)]
+    kw = dict(rc_angstrom=3.0, alpha_per_angstrom=1.0667, sk=3.0) if cells else {}
+    systems = [replica_system(64 if cells else 27 + (i % 5) * 4, 0.5 * (i % 3), 290.0 + i % 40, seed=100 + i, **kw)
+               for i in range(n)]
     a = [EwaldMC(s, precision=precision) for s in systems]
     for i, x in enumerate(a):
         x.set_block_size(256)

@@ -241,6 +241,24 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
         TRY(dalloc(c, &h.rsb, (size_t)YD_NBCAP * 4));
         TRY(dalloc(c, &h.nbl, (size_t)YD_NBCAP)); TRY(dalloc(c, &h.pos4, (size_t)N));
         TRY(dalloc(c, &h.freq_end, M)); TRY(dalloc(c, &h.freq_beg, M)); TRY(dalloc(c, &h.st, 1));
+        // cell list of the real-space scan: cells of width >= rc (box units;
+        // rmax2 is constant in box units, so volume steps keep nc); only
+        // worth it from 4 cells per axis on
+        {
+            const double rc = sqrt(d->rmax2);
+            int nc = rc > 0 ? (int)floor(1.0 / (rc * (1.0 + 1e-9))) : 0;
+            if (nc > 40) nc = 40;
+            if (nc >= 4) {
+                const int ncell = nc * nc * nc;
+                const int cap = ((2 * N) / ncell + 64 + 63) / 64 * 64;
+                h.nc = nc;
+                h.ccap = cap;
+                TRY(dalloc(c, &h.cell_cnt, (size_t)ncell));
+                TRY(dalloc(c, &h.cell_atoms, (size_t)ncell * cap));
+                TRY(dalloc(c, &h.atom_cell, (size_t)N));
+                TRY(dalloc(c, &h.atom_slot, (size_t)N));
+            }
+        }
         TRY(dalloc(c, &c->d_lj6, TT)); TRY(dalloc(c, &c->d_lj12, TT));
         TRY(dalloc(c, &c->d_sig6, TT)); TRY(dalloc(c, &c->d_4eps, TT));
         TRY(dalloc(c, &c->d_counts, T > 0 ? T : 1));

@@ -77,7 +77,19 @@ struct YdChain {
     const int *cimol;     // constant-uniform mode: molecule per move
     const double *cu;     // constant-uniform mode: uniform per move
     int mv_begin;         // step number of the launch's first step in its run (cycle starts: multiples of nmol)
+    // Cell list of the workgroup chain kernel's real-space scan (nc = 0: none).
+    // nc^3 cubic cells of width 1/nc >= the cut-off (box units); cell
+    // (ix, iy, iz) = floor((r + 0.5) nc) mod nc per axis, id (ix nc + iy) nc + iz;
+    // bucket c holds cell_cnt[c] <= cap atoms in cell_atoms[c cap ..].
+    int nc, ccap;
+    int *cell_cnt, *cell_atoms, *atom_cell, *atom_slot;
 };
+
+// Cell coordinate of one position component (box units).
+__device__ __forceinline__ int yd_cell1(double x, int nc) {
+    int i = (int)floor((x + 0.5) * (double)nc) % nc;
+    return i < 0 ? i + nc : i;
+}
 
 // ---------------------------------------------------------------- math ----
 // Functions.f90:116-142 erfc_Luc

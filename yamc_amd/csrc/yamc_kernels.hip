@@ -28,6 +28,7 @@
 // Candidate bitmap over atom indices (order-independent marking, then a
 // deterministic ascending compaction into the list).
 #define YK_BMWORDS 4096  // 131072 atoms
+#define YK_MAXCL 125     // cells visited by one cell-list scan (5 per axis)
 
 // ------------------------------------------------------------------------
 // Functions.f90:26-111 erfc_Luc_bet6_bet12
@@ -368,6 +369,10 @@ struct YkShared {
     unsigned int bm[YK_BMWORDS];
     int wsum[YK_NW];
     unsigned long long ph[9], t_last;  // phase timers (thread 0)
+    // cell-list scan: the cells around the molecule's old and new sites and
+    // the running offsets of their buckets; cell_ok = 0 after a bucket
+    // overflowed (the scan then visits every atom until the next launch)
+    int cl_cell[YK_MAXCL], cl_start[YK_MAXCL + 1], ncl, cell_ok;
 };
 
 // Loads molecule imol (old positions from the state) into LDS; thread 0.
@@ -845,6 +850,105 @@ __device__ void yk_rs_commit_all(const YdChain &c, const YkMol &m, const double 
 //     site change, MC.f90:1251-1275).
 // Returns false when the list overflowed (the caller then uses the full
 // FP64 pass yk_rs_move_n and the full commit).
+// FP32 distance test of one atom against the molecule's old and new sites
+// (margin 1e-5 box^2 on r^2, far above FP32 rounding), single-wrap minimum
+// image.
+template <int NA>
+__device__ __forceinline__ bool yk_near(const float (&xof)[NA], const float (&yof)[NA], const float (&zof)[NA],
+                                        const float (&xnf)[NA], const float (&ynf)[NA], const float (&znf)[NA],
+                                        float4 pj, float r2f) {
+    bool near = false;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        float dx = xnf[a] - pj.x, dy = ynf[a] - pj.y, dz = znf[a] - pj.z;
+        dx = dx > 0.5f ? dx - 1.0f : dx; dx = dx < -0.5f ? dx + 1.0f : dx;
+        dy = dy > 0.5f ? dy - 1.0f : dy; dy = dy < -0.5f ? dy + 1.0f : dy;
+        dz = dz > 0.5f ? dz - 1.0f : dz; dz = dz < -0.5f ? dz + 1.0f : dz;
+        near |= dx * dx + dy * dy + dz * dz < r2f;
+        float ex = xof[a] - pj.x, ey = yof[a] - pj.y, ez = zof[a] - pj.z;
+        ex = ex > 0.5f ? ex - 1.0f : ex; ex = ex < -0.5f ? ex + 1.0f : ex;
+        ey = ey > 0.5f ? ey - 1.0f : ey; ey = ey < -0.5f ? ey + 1.0f : ey;
+        ez = ez > 0.5f ? ez - 1.0f : ez; ez = ez < -0.5f ? ez + 1.0f : ez;
+        near |= ex * ex + ey * ey + ez * ez < r2f;
+    }
+    return near;
+}
+
+// Cell-list form of the scan: only the atoms in the cells around the old
+// and new sites are tested.  Cells are at least a cut-off wide, so every
+// atom within the cut-off of a site is in the site's cell or a neighbour of
+// it (per axis, modulo nc); the visited range per axis spans the sites'
+// cells +-1 (the whole axis when that reaches nc cells, or when the sites
+// are not within one cell of each other).  cl_* are the workgroup's LDS
+// arrays; all threads call this.
+template <int NA>
+__device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const float4 *pos, unsigned int *bm, int *cl_cell,
+                                 int *cl_start, int *ncl_out) {
+    const int nc = c.nc, me = m.imol + 1, tid = threadIdx.x;
+    int start[3], len[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double *po = d == 0 ? m.xo : d == 1 ? m.yo : m.zo;
+        const double *pn = d == 0 ? m.xn : d == 1 ? m.yn : m.zn;
+        const int base = yd_cell1(po[0], nc);
+        int lo = 0, hi = 0;
+        bool full = false;
+        for (int k = 0; k < 2 * NA; ++k) {
+            int ci = yd_cell1(k < NA ? po[k] : pn[k - NA], nc);
+            int off = ci - base;
+            if (off < 0) off += nc;
+            int dd = off == 0 ? 0 : off == 1 ? 1 : off == nc - 1 ? -1 : 2;
+            if (dd == 2) full = true;
+            lo = dd < lo ? dd : lo;
+            hi = dd > hi && dd < 2 ? dd : hi;
+        }
+        lo -= 1;
+        hi += 1;
+        if (full || hi - lo + 1 >= nc) { start[d] = 0; len[d] = nc; }
+        else { start[d] = base + lo; len[d] = hi - lo + 1; }
+    }
+    const int ncl = len[0] * len[1] * len[2];
+    if (ncl > YK_MAXCL) return false;  // uniform: the caller scans every atom instead
+    if (tid < ncl) {
+        int t = tid, iz = t % len[2], iy = (t / len[2]) % len[1], ix = t / (len[2] * len[1]);
+        int cx = (start[0] + ix) % nc, cy = (start[1] + iy) % nc, cz = (start[2] + iz) % nc;
+        if (cx < 0) cx += nc;
+        if (cy < 0) cy += nc;
+        if (cz < 0) cz += nc;
+        int id = (cx * nc + cy) * nc + cz;
+        cl_cell[tid] = id;
+        cl_start[tid + 1] = c.cell_cnt[id];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        cl_start[0] = 0;
+        for (int k = 0; k < ncl; ++k) cl_start[k + 1] += cl_start[k];
+        *ncl_out = ncl;
+    }
+    __syncthreads();
+    const int total = cl_start[ncl];
+    const float r2f = (float)c.p.rmax2 + 1e-5f;
+    float xof[NA], yof[NA], zof[NA], xnf[NA], ynf[NA], znf[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        xof[a] = (float)m.xo[a]; yof[a] = (float)m.yo[a]; zof[a] = (float)m.zo[a];
+        xnf[a] = (float)m.xn[a]; ynf[a] = (float)m.yn[a]; znf[a] = (float)m.zn[a];
+    }
+    for (int t = tid; t < total; t += (int)blockDim.x) {
+        int lo = 0, hi = ncl - 1;  // the bucket holding flat index t
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (cl_start[mid] <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        int j = c.cell_atoms[(size_t)cl_cell[lo] * c.ccap + (t - cl_start[lo])];
+        float4 pj = pos[j];
+        if (__float_as_int(pj.w) != me && yk_near<NA>(xof, yof, zof, xnf, ynf, znf, pj, r2f))
+            atomicOr(&bm[j >> 5], 1u << (j & 31));
+    }
+    return true;
+}
+
 template <int NA>
 __device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const float4 *pos, unsigned int *bm) {
     const int N = c.p.natom, me = m.imol + 1;
@@ -858,21 +962,8 @@ __device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const float4 *pos, 
         }
         for (int j = threadIdx.x; j < N; j += (int)blockDim.x) {
             float4 pj = pos[j];
-            bool near = false;
-#pragma unroll
-            for (int a = 0; a < NA; ++a) {
-                float dx = xnf[a] - pj.x, dy = ynf[a] - pj.y, dz = znf[a] - pj.z;
-                dx = dx > 0.5f ? dx - 1.0f : dx; dx = dx < -0.5f ? dx + 1.0f : dx;
-                dy = dy > 0.5f ? dy - 1.0f : dy; dy = dy < -0.5f ? dy + 1.0f : dy;
-                dz = dz > 0.5f ? dz - 1.0f : dz; dz = dz < -0.5f ? dz + 1.0f : dz;
-                near |= dx * dx + dy * dy + dz * dz < r2f;
-                float ex = xof[a] - pj.x, ey = yof[a] - pj.y, ez = zof[a] - pj.z;
-                ex = ex > 0.5f ? ex - 1.0f : ex; ex = ex < -0.5f ? ex + 1.0f : ex;
-                ey = ey > 0.5f ? ey - 1.0f : ey; ey = ey < -0.5f ? ey + 1.0f : ey;
-                ez = ez > 0.5f ? ez - 1.0f : ez; ez = ez < -0.5f ? ez + 1.0f : ez;
-                near |= ex * ex + ey * ey + ez * ez < r2f;
-            }
-            if (near && __float_as_int(pj.w) != me) atomicOr(&bm[j >> 5], 1u << (j & 31));
+            if (yk_near<NA>(xof, yof, zof, xnf, ynf, znf, pj, r2f) && __float_as_int(pj.w) != me)
+                atomicOr(&bm[j >> 5], 1u << (j & 31));
         }
     } else {
         for (int j = threadIdx.x; j < N; j += (int)blockDim.x) {
@@ -971,6 +1062,57 @@ __device__ __forceinline__ void yk_rs_scan_n(const YdChain &c, const YkMol &m, c
         case 2: yk_rs_scan<2>(c, m, pos, bm); break;
         case 3: yk_rs_scan<3>(c, m, pos, bm); break;
         default: yk_rs_scan<4>(c, m, pos, bm); break;
+    }
+}
+__device__ __forceinline__ bool yk_rs_scan_cells_n(const YdChain &c, const YkMol &m, const float4 *pos,
+                                                   unsigned int *bm, int *cl_cell, int *cl_start, int *ncl) {
+    switch (__builtin_amdgcn_readfirstlane(m.na)) {
+        case 1: return yk_rs_scan_cells<1>(c, m, pos, bm, cl_cell, cl_start, ncl);
+        case 2: return yk_rs_scan_cells<2>(c, m, pos, bm, cl_cell, cl_start, ncl);
+        case 3: return yk_rs_scan_cells<3>(c, m, pos, bm, cl_cell, cl_start, ncl);
+        default: return yk_rs_scan_cells<4>(c, m, pos, bm, cl_cell, cl_start, ncl);
+    }
+}
+
+// Rebuilds the cell buckets from the current positions (all threads of the
+// workgroup); *ok = 0 if a bucket overflowed.
+__device__ void yk_cells_build(const YdChain &c, int *ok) {
+    const int nc = c.nc, ncell = nc * nc * nc, N = c.p.natom;
+    for (int i = threadIdx.x; i < ncell; i += (int)blockDim.x) c.cell_cnt[i] = 0;
+    if (threadIdx.x == 0) *ok = 1;
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += (int)blockDim.x) {
+        int id = (yd_cell1(c.x[i], nc) * nc + yd_cell1(c.y[i], nc)) * nc + yd_cell1(c.z[i], nc);
+        int slot = atomicAdd(&c.cell_cnt[id], 1);
+        c.atom_cell[i] = id;
+        c.atom_slot[i] = slot;
+        if (slot < c.ccap) c.cell_atoms[(size_t)id * c.ccap + slot] = i;
+        else *ok = 0;
+    }
+    __syncthreads();
+}
+
+// Keeps the buckets current after an accepted move (thread 0): each moved
+// atom that changed cell leaves its bucket (the bucket's last atom takes its
+// slot) and joins the new one.  *ok = 0 on overflow.
+__device__ void yk_cells_move(const YdChain &c, const YkMol &m, int *ok) {
+    const int nc = c.nc, cap = c.ccap;
+    for (int a = 0; a < m.na; ++a) {
+        const int g = m.first + a;
+        const int id = (yd_cell1(m.xn[a], nc) * nc + yd_cell1(m.yn[a], nc)) * nc + yd_cell1(m.zn[a], nc);
+        const int old = c.atom_cell[g];
+        if (id == old) continue;
+        const int s = c.atom_slot[g], last = c.cell_cnt[old] - 1;
+        const int moved = c.cell_atoms[(size_t)old * cap + last];
+        c.cell_atoms[(size_t)old * cap + s] = moved;
+        c.atom_slot[moved] = s;
+        c.cell_cnt[old] = last;
+        const int ns = c.cell_cnt[id];
+        if (ns >= cap) { *ok = 0; return; }
+        c.cell_atoms[(size_t)id * cap + ns] = g;
+        c.atom_slot[g] = ns;
+        c.atom_cell[g] = id;
+        c.cell_cnt[id] = ns + 1;
     }
 }
 
@@ -1759,6 +1901,7 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
     float4 *const spos = use_lds ? lds_pos : c.pos4;
     for (int i = tid; i < p.natom; i += (int)blockDim.x)
         spos[i] = make_float4((float)c.x[i], (float)c.y[i], (float)c.z[i], __int_as_float(c.molnum[i]));
+    if (c.nc > 0) yk_cells_build(c, &sh.cell_ok);  // workgroup-uniform; ends with a barrier
     YkScalar &t0 = sh.t0;
     // mu = sum_j q_j r_j from scratch once per launch (EwaldSumExternal.f90:
     // 165-183); afterwards it is carried incrementally, mu + dmu on each
@@ -1841,7 +1984,10 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
             if (nwords <= YK_BMWORDS) {
                 for (int i = tid; i < nwords; i += (int)blockDim.x) sh.bm[i] = 0u;
                 __syncthreads();
-                yk_rs_scan_n(c, sh.m, spos, sh.bm);
+                bool scanned = false;
+                if (c.nc > 0 && sh.cell_ok)
+                    scanned = yk_rs_scan_cells_n(c, sh.m, spos, sh.bm, sh.cl_cell, sh.cl_start, &sh.ncl);
+                if (!scanned) yk_rs_scan_n(c, sh.m, spos, sh.bm);
                 __syncthreads();
                 yk_compact(sh.bm, nwords, sh.nblist, &sh.nbcount, sh.wsum);
             } else if (tid == 0) {
@@ -1919,6 +2065,7 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         YK_TICK(7);
         if (tid == 0) {
             yk_sec_finalize(c, sh.m, t0, spos);
+            if (c.nc > 0 && sh.cell_ok) yk_cells_move(c, sh.m, &sh.cell_ok);
             for (int d = 0; d < 3; ++d) t0.mu[d] = t0.mu_new[d];
         }
         __syncthreads();

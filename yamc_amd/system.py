@@ -279,6 +279,8 @@ def config(name: str, seed: int = 20261203, precision_scale: float = 1.0) -> Sys
     c3: 8192 SPC/E + 160 Na + 160 Cl, L=62.6 A, rc=12 A (alphaL 16.6933), kmax=16.
     c5: 32768 SPC/E + 600 Na + 600 Cl, L=99.4 A, rc=12 A (alphaL 26.5067), kmax=24.
     small: 8 SPC/E + 1 Na + 1 Cl in L=12 A (unit tests).
+    cells: 512 SPC/E + 5 Na + 5 Cl, L=25.0 A, rc=6 A (4 cut-off cells per axis,
+        the cell-list scan), kmax=13, force bias 0.5 (unit tests).
     """
     if name == "c1":
         p = Params(box_length=19.72, temp=298.0, sr=3.2, sk=3.1364, alpha=7.0116, external_permutivity=50.0,
@@ -304,6 +306,8 @@ def config(name: str, seed: int = 20261203, precision_scale: float = 1.0) -> Sys
         p = Params(box_length=12.0, temp=298.0, sr=3.2, sk=3.0, alpha=6.4, external_permutivity=50.0,
                    dr_a=0.3, d_angle_degree=15.0, lambda_f=0.0, lambda_c=0.0)
         return lattice_system(p, [SPCE, NA_DANG, CL_DANG], [8, 1, 1], seed)
+    if name == "cells":
+        return replica_system(512, 0.5, 300.0, seed, rc_angstrom=6.0, alpha_per_angstrom=0.5333, sk=3.0)
     raise ValueError(f"unknown workload {name!r}")
 
 
