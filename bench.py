@@ -548,6 +548,7 @@ def large_check(args):
            "workload": "c5: 32768 SPC/E + 600 Na + 600 Cl, L=99.4 A, rc=12 A, kmax=24",
            "cycles": args.large_check, "moves": args.large_check * s.nmol}
     recs = {}
+    final_pos = None
     for form, prec, rs in (("fp64", YAMC_FP64, 0), ("fp32c", YAMC_FP32C, 1000)):
         g = EwaldMC(s, precision=prec)
         g.mc_init()
@@ -563,15 +564,61 @@ def large_check(args):
                          kspace_drift_rel=max(abs(inc[k] - fresh[k]) / max(1.0, abs(fresh[k]))
                                               for k in ("E_kC", "E_kLJ")))
         print(f"large-check: {form} done, {len(r)} moves in {dt:.1f} s", file=sys.stderr, flush=True)
+        final_pos = g.positions()
         g.close()
+    # k-space dE and forces of the two forms on one configuration (the
+    # FP32C run's final state) for 2000 seeded trial displacements: the
+    # per-move error of the FP32C sums themselves, free of the slow
+    # divergence of the two trajectories (force-biased proposals follow the
+    # forms' forces, so the chains drift apart at the 1e-9 level over 10^5
+    # moves while keeping identical decisions)
+    a64, a32 = EwaldMC(s, precision=YAMC_FP64), EwaldMC(s, precision=YAMC_FP32C)
+    for h in (a64, a32):
+        h.set_positions(*final_pos)
+        h.mc_init()
+    rng = np.random.default_rng(20261208)
+    x, y, z = final_pos
+    errs, scales, ferr = [], [], []
+    for _ in range(2000):
+        im = int(rng.integers(0, s.nmol))
+        f, l = int(s.mol_first[im]), int(s.mol_last[im])
+        dv = rng.uniform(-1.0, 1.0, 3) * s.derived["dr"]
+        nx, ny, nz = x[f:l + 1] + dv[0], y[f:l + 1] + dv[1], z[f:l + 1] + dv[2]
+        c64, l64 = a64.kspace_delta(im, nx, ny, nz)
+        c32, l32 = a32.kspace_delta(im, nx, ny, nz)
+        errs.append(abs((c32 + l32) - (c64 + l64)))
+        scales.append(abs(c64) + abs(l64))
+        g64 = np.stack(a64.kspace_forces(f, nx, ny, nz), 1)
+        g32 = np.stack(a32.kspace_forces(f, nx, ny, nz), 1)
+        ferr.append(float(np.max(np.linalg.norm(g32 - g64, axis=1) / np.maximum(np.linalg.norm(g64, axis=1), 1e-300))))
+    a64.close(); a32.close()
+    errs = np.array(errs)
+    out["same_configuration_kspace"] = {
+        "trial_moves": len(errs), "max_dE_kT": float(errs.max()), "max_dE_kJmol": float(errs.max() * kT_kjmol),
+        "max_rel_dE_vs_scale": float((errs / np.maximum(scales, 1e-300)).max()),
+        "max_rel_force_err_norm": float(max(ferr))}
     a, b = recs["fp64"], recs["fp32c"]
     same = (a["accepted"] == b["accepted"]) & (a["imol"] == b["imol"])
     n = int(np.argmin(same)) if not same.all() else len(a)
     ok = a["overlap"][:n] == 0
     d = np.abs(a["du_total"][:n] - b["du_total"][:n])[ok]
+    scale = (np.abs(a["du_real"]) + np.abs(a["du_kc"]) + np.abs(a["du_klj"]) + np.abs(a["du_ext"]))[:n][ok]
+    # along the two chains (same stream, diverging at the 1e-9 level)
     out["moves_before_first_differing_decision"] = n
     out["max_dE_kT"] = float(d.max()) if d.size else 0.0
     out["max_dE_kJmol"] = float(d.max() * kT_kjmol) if d.size else 0.0
+    if d.size:
+        k = int(np.argmax(d))
+        rel = d / np.maximum(scale, 1e-300)
+        out["at_max_dE"] = {"dE_fp64_kT": float(a["du_total"][:n][ok][k]), "scale_kT": float(scale[k]),
+                            "dE_real_kT": float(a["du_real"][:n][ok][k]), "dE_kC_kT": float(a["du_kc"][:n][ok][k]),
+                            "dE_kLJ_kT": float(a["du_klj"][:n][ok][k]),
+                            "kC_err": float(abs(a["du_kc"][:n][ok][k] - b["du_kc"][:n][ok][k])),
+                            "kLJ_err": float(abs(a["du_klj"][:n][ok][k] - b["du_klj"][:n][ok][k])),
+                            "real_err": float(abs(a["du_real"][:n][ok][k] - b["du_real"][:n][ok][k]))}
+        out["max_rel_dE_vs_scale"] = float(rel.max())
+        out["dE_err_quantiles_kT"] = {q: float(np.quantile(d, float(q))) for q in ("0.5", "0.99", "0.9999")}
+        out["moves_over_1e-6_kJmol"] = int((d * kT_kjmol > 1e-6).sum())
     print(json.dumps(out))
 
 
