@@ -353,8 +353,10 @@ struct YkMcm {
 struct YkScalar {
     YkMcm mcm;
     YdRng rng;
+    YkMcm mcm2;  // the rotation's copy when displacement and rotation run on two waves
+    YdRng rng2;
     double U_old, frn[YD_MAXA][3], uun[YD_MAXA], cmp_new[3], cmp_old[3], duc, dulj, du_real, du_ext;
-    double mu[3], mu_new[3], muo[3];
+    double mu[3], mu_new[3], muo[3], com[3];
     long long acc_count, nmv, nov;
 };
 
@@ -1276,6 +1278,50 @@ __device__ void yk_displacement(const YdParams &p, YkMcm &m, YdRng &rng) {
     m.umaxold = umax;
 }
 
+// The draws of yk_displacement without its arithmetic: the same branches and
+// rejection tests (none of which depends on the Newton solve), so that the
+// rotation's draws can start where the displacement's end while another
+// wave computes the displacement.
+__device__ void yk_displacement_draws(const YdParams &p, YkMcm &m, YdRng &rng) {
+    const double fxo = m.fo[0], fyo = m.fo[1], fzo = m.fo[2];
+    const double umax = p.lambda_f * m.ftotold * p.dr;
+    if (umax > 1e-2) {
+        double xksi = 0.0;
+        for (int g = 0; xksi < 1e-10; ++g) {
+            if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+            xksi = rng.next();
+        }
+        xksi = 1.0;
+        for (int g = 0; xksi > (1.0 - 1e-10) || xksi < 1e-10; ++g) {
+            if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+            xksi = rng.next();
+        }
+        double w = -1.0, wv[3] = {0, 0, 0};
+        for (int g = 0; w <= 1e-10; ++g) {
+            if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+            double v2 = 2.0, vx = 0, vy = 0, vz = 0;
+            for (int g = 0; v2 > 1.0; ++g) {
+                if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+                vx = 2.0 * rng.next() - 1.0;
+                vy = 2.0 * rng.next() - 1.0;
+                vz = 2.0 * rng.next() - 1.0;
+                v2 = vx * vx + vy * vy + vz * vz;
+            }
+            yk_prod(vx, vy, vz, fxo, fyo, fzo, wv);
+            w = sqrt(wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2]);
+        }
+    } else {
+        double v2 = 2.0;
+        for (int g = 0; v2 > 1.0; ++g) {
+            if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
+            double vx = 2.0 * rng.next() - 1.0;
+            double vy = 2.0 * rng.next() - 1.0;
+            double vz = 2.0 * rng.next() - 1.0;
+            v2 = vx * vx + vy * vy + vz * vz;
+        }
+    }
+}
+
 // MonteCarloMove.f90:269-444 chooseRotation (WAVE: called by a whole wave
 // with identical arguments in every lane).
 template <bool WAVE>
@@ -1692,8 +1738,8 @@ __device__ __forceinline__ void yk_sec_slots(YkMol &m) {
 // parts, then the force/torque-biased displacement and rotation.  Returns
 // the stall flag.
 template <bool WAVE>
-__device__ int yk_sec_propose(const YdChain &c, YkMol &m, YkScalar &t0, const double *out,
-                              const double (*fr_old)[3], const double *uu_old) {
+__device__ void yk_sec_propose_pre(const YdChain &c, YkMol &m, YkScalar &t0, const double *out,
+                                   const double (*fr_old)[3], const double *uu_old) {
     const YdParams &p = c.p;
     YkMcm &mcm = t0.mcm;
     int na = m.na;
@@ -1735,12 +1781,17 @@ __device__ int yk_sec_propose(const YdChain &c, YkMol &m, YkScalar &t0, const do
         yk_prod(m.xo[a] - com[0], m.yo[a] - com[1], m.zo[a] - com[2], fx[a], fy[a], fz[a], tq);
         To[0] = To[0] + tq[0]; To[1] = To[1] + tq[1]; To[2] = To[2] + tq[2];
     }
-    for (int d = 0; d < 3; ++d) { mcm.fo[d] = Fo[d]; mcm.co[d] = To[d]; }
+    for (int d = 0; d < 3; ++d) { mcm.fo[d] = Fo[d]; mcm.co[d] = To[d]; t0.com[d] = com[d]; }
     mcm.ftotold = sqrt(Fo[0] * Fo[0] + Fo[1] * Fo[1] + Fo[2] * Fo[2]);
     mcm.couplold = sqrt(To[0] * To[0] + To[1] * To[1] + To[2] * To[2]);
     mcm.stall = 0;
-    yk_displacement(p, mcm, t0.rng);
-    yk_rotation<WAVE>(p, mcm, t0.rng);
+}
+
+// The new positions from the displacement and the rotation (MC.f90:1045-1067).
+__device__ int yk_sec_propose_post(YkMol &m, YkScalar &t0) {
+    YkMcm &mcm = t0.mcm;
+    const int na = m.na;
+    const double *com = t0.com;
     double cn[3];
     for (int d = 0; d < 3; ++d) cn[d] = com[d] + mcm.d[d] - round(com[d] + mcm.d[d]);
     if (na > 1) {
@@ -1766,6 +1817,17 @@ __device__ int yk_sec_propose(const YdChain &c, YkMol &m, YkScalar &t0, const do
         m.xn[0] = cn[0]; m.yn[0] = cn[1]; m.zn[0] = cn[2];
     }
     return mcm.stall;
+}
+
+// The whole proposal on one wave (the wave kernel): forces and torques,
+// displacement, rotation, new positions (MC.f90:984-1067).
+template <bool WAVE>
+__device__ int yk_sec_propose(const YdChain &c, YkMol &m, YkScalar &t0, const double *out,
+                              const double (*fr_old)[3], const double *uu_old) {
+    yk_sec_propose_pre<WAVE>(c, m, t0, out, fr_old, uu_old);
+    yk_displacement(c.p, t0.mcm, t0.rng);
+    yk_rotation<WAVE>(c.p, t0.mcm, t0.rng);
+    return yk_sec_propose_post(m, t0);
 }
 
 // Acceptance (MC.f90:1176-1233): new forces and torque, dU_ext, the
@@ -1958,9 +2020,32 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
             yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
         }
         YK_TICK(2);
-        // ---- proposal on wave 0 (MC.f90:984-1067) -----------------------
+        // ---- proposal (MC.f90:984-1067): forces and torques on wave 0,
+        // then the displacement on wave 0 and the rotation on wave 1 at
+        // once (the rotation's draws start after the displacement's, which
+        // wave 1 replays without the arithmetic), then the new positions
         if (tid < 64) {
-            int stall = yk_sec_propose<true>(c, sh.m, t0, sh.out, sh.fr_old, sh.uu_old);
+            yk_sec_propose_pre<true>(c, sh.m, t0, sh.out, sh.fr_old, sh.uu_old);
+            t0.mcm2 = t0.mcm;
+            t0.rng2 = t0.rng;
+        }
+        __syncthreads();
+        if (tid < 64) {
+            yk_displacement(p, t0.mcm, t0.rng);
+        } else if (tid < 128) {
+            yk_displacement_draws(p, t0.mcm2, t0.rng2);
+            yk_rotation<true>(p, t0.mcm2, t0.rng2);
+        }
+        __syncthreads();
+        if (tid < 64) {
+            YkMcm &a = t0.mcm;
+            const YkMcm &b = t0.mcm2;
+            a.ca = b.ca; a.sa = b.sa; a.ct = b.ct; a.st = b.st; a.cphi = b.cphi; a.sphi = b.sphi;
+            a.v[0] = b.v[0]; a.v[1] = b.v[1]; a.v[2] = b.v[2];
+            a.vmaxold = b.vmaxold; a.angle = b.angle; a.qold = b.qold;
+            a.stall = b.stall;  // the replay stalls exactly where the displacement does
+            t0.rng = t0.rng2;
+            int stall = yk_sec_propose_post(sh.m, t0);
             if (tid == 0) sh.stall = stall;
         }
         __syncthreads();
