@@ -242,7 +242,8 @@ def main():
     # --------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
-        from oracle import pyoracle
+        from oracle import pyoracle, refcase
+        have_ref = refcase.available()
         o = pyoracle.Oracle(base)
         o.mc_init()
         o.set_rng(20261203, 0)
@@ -250,7 +251,7 @@ def main():
         t0 = time.perf_counter()
         o.run_const(im, np.full(64, 0.4))
         per = (time.perf_counter() - t0) / 64
-        nmv = args.cpu_moves or max(64, min(20000, int(15.0 / per)))
+        nmv = args.cpu_moves or max(64, min(20000, int((5.0 if have_ref else 15.0) / per)))
         o2 = pyoracle.Oracle(base)
         o2.mc_init()
         o2.set_rng(20261203, 0)
@@ -264,8 +265,20 @@ def main():
             o2.run_const(im, np.full(nmv, 0.4))
             done = nmv
         dtc = time.perf_counter() - t0
-        cpu = dict(value=done / dtc, unit="trial moves/s", cores=1, kind="port",
-                   sample=f"{done} FP64 trial moves of the same 6224-atom workload, plain-C oracle, 1 thread")
+        port = dict(value=done / dtc, unit="trial moves/s", cores=1, kind="port",
+                    sample=f"{done} FP64 trial moves of the same 6224-atom workload, plain-C oracle, 1 thread")
+        cpu = port
+        if have_ref:
+            try:
+                ref = time_reference(base, pyoracle, refcase)
+                cpu = dict(value=ref["moves_per_s"], unit="trial moves/s", cores=1, kind="reference",
+                           sample=f"{ref['moves']} movemc trial moves of the same 6224-atom workload by the "
+                                  f"reference's own Fortran (compiled from its sources with amdflang -O2, "
+                                  f"RND_CONSTANT uniforms, unbiased proposals, set-up time subtracted), "
+                                  f"1 thread; the plain-C "
+                                  f"port of the same path: {port['value']:.0f} trial moves/s")
+            except Exception as e:  # the port stays the baseline
+                print(f"reference CPU timing failed ({e}); reporting the port", file=sys.stderr)
 
     value = res[main_form]["moves_per_s"]
     out = {
@@ -526,6 +539,37 @@ def npt_check(args):
         "rdf_frames": res["rdf_frames"], "rdf": rdfs,
     }
     print(json.dumps(out))
+
+
+def time_reference(sysm, pyoracle, refcase, seconds=12.0):
+    """The reference's own movemc (oracle/_ref/ref_driver) on sysm's
+    configuration: driver runs with 0 and with M constant-uniform moves,
+    per-move time = difference / M.  The moves are unbiased (lambda_f =
+    lambda_c = 0): with one constant uniform per move the force-biased
+    rejection loops of strongly pushed molecules never end, and a move costs
+    the same either way (movemc computes the k-space forces of every move,
+    MC.f90:963-968; the bias only changes a few scalar expressions)."""
+    import dataclasses
+    import tempfile
+
+    from yamc_amd import system as ys
+
+    sysm = ys.build_system(dataclasses.replace(sysm.params, lambda_f=0.0, lambda_c=0.0), sysm.species, sysm.counts,
+                           sysm.mol_type, sysm.mol_pos, sysm.mol_angles)
+
+    def timed(nmoves, seed):
+        imol, u = refcase.safe_moves(sysm, nmoves, seed, pyoracle, (0.3, 0.45), True) if nmoves else ([], [])
+        with tempfile.TemporaryDirectory() as td:
+            refcase.write_case(td, sysm, imol, u)
+            _, dt = refcase.run(td, timeout=300)
+        return dt
+
+    t_setup = timed(0, 0)
+    t_probe = timed(100, 1)
+    per = max((t_probe - t_setup) / 100, 1e-6)
+    m = int(min(20000, max(200, seconds / per)))
+    t_m = timed(m, 2)
+    return {"moves": m, "moves_per_s": m / max(t_m - t_setup, 1e-9), "setup_s": t_setup}
 
 
 def torch_sync():

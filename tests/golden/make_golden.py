@@ -17,7 +17,6 @@ loop of the proposal spins forever (checked with the oracle first).
 from __future__ import annotations
 
 import os
-import subprocess
 import sys
 import tempfile
 
@@ -26,17 +25,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 
-from oracle import pyoracle  # noqa: E402
+from oracle import pyoracle, refcase  # noqa: E402
 from yamc_amd import system as ys  # noqa: E402
 
-DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
 OUT = os.path.dirname(os.path.abspath(__file__))
-
-
-def write_mol(path, mol):
-    with open(path, "w") as f:
-        for r in mol.atoms:
-            f.write("%s %.15f %.15f %.15f %.6f %.6f %.6f %.4f %.4f\n" % r)
 
 
 def parse(text):
@@ -107,20 +99,8 @@ def parse(text):
 
 
 def pick_moves(sysm, nmoves, seed):
-    """Molecules and constant uniforms such that the oracle finishes every
-    proposal (a stalled proposal would make the reference loop forever)."""
-    rng = np.random.default_rng(seed)
-    imol = rng.integers(0, sysm.nmol, nmoves).astype(np.int32)
-    u = rng.uniform(0.25, 0.75, nmoves)
-    for _ in range(200):
-        o = pyoracle.Oracle(sysm)
-        o.mc_init()
-        _, recs = o.run_const(imol, u)
-        bad = np.nonzero(recs["overlap"] == 2)[0]
-        if not len(bad):
-            return imol, u
-        u[bad[0]] = rng.uniform(0.25, 0.75)
-    raise RuntimeError("could not find non-stalling uniforms")
+    """Molecules and constant uniforms such that every proposal finishes."""
+    return refcase.safe_moves(sysm, nmoves, seed, pyoracle)
 
 
 def make_case(name, sysm, nmoves, seed, vols=(), xchgs=()):
@@ -130,40 +110,13 @@ def make_case(name, sysm, nmoves, seed, vols=(), xchgs=()):
     imol, u = pick_moves(sysm, nmoves, seed)
     p = sysm.params
     with tempfile.TemporaryDirectory() as td:
-        names = []
-        for k, mol in enumerate(sysm.species):
-            fn = f"species{k}.mol"
-            write_mol(os.path.join(td, fn), mol)
-            names.append(fn)
-        with open(os.path.join(td, "case.composition"), "w") as f:
-            f.write(f"{len(sysm.species)}\n")
-            for fn, cnt in zip(names, sysm.counts):
-                f.write(f"{fn} {cnt}\n")
-        with open(os.path.join(td, "case.in"), "w") as f:
-            f.write("case.composition\n")
-            f.write(" ".join("%.17g" % v for v in (p.box_length, p.temp, p.sr, p.sk, p.alpha, p.external_permutivity,
-                                                  p.dr_a, p.d_angle_degree, p.lambda_f, p.lambda_c)) + "\n")
-            f.write(f"{sysm.nmol}\n")
-            for i in range(sysm.nmol):
-                f.write("%d %.17g %.17g %.17g %.17g %.17g %.17g\n" % (
-                    sysm.mol_type[i] + 1, *sysm.mol_pos[i], *sysm.mol_angles[i]))
-            f.write(f"{nmoves}\n")
-            for a, b in zip(imol, u):
-                f.write("%d %.17g\n" % (a + 1, b))
-            f.write(f"{len(vols)}\n")
-            for vu, vp, vm in vols:
-                f.write("%.17g %.17g %.17g\n" % (vu, vp, vm))
-            f.write(f"{len(xchgs)}\n")
-            for a, b, xu in xchgs:
-                f.write("%d %d %.17g\n" % (a + 1, b + 1, xu))
-        r = subprocess.run([DRIVER, "case.in"], cwd=td, capture_output=True, text=True, timeout=120)
-        if r.returncode != 0:
-            raise RuntimeError(r.stderr)
+        refcase.write_case(td, sysm, imol, u, vols, xchgs)
+        stdout, _ = refcase.run(td, timeout=120)
         blobs = {}
         for fn in ("moltab1.bin", "moltab2.bin", "moltab3.bin", "floats.bin"):
             with open(os.path.join(td, fn), "rb") as fb:
                 blobs[fn] = np.frombuffer(fb.read(), dtype=np.uint8).copy()
-    g = parse(r.stdout)
+    g = parse(stdout)
     natom, nk, nt = g["SIZES"]
     moves = g["MOVES"]
     assert len(moves) == nmoves
@@ -229,7 +182,7 @@ def make_case(name, sysm, nmoves, seed, vols=(), xchgs=()):
 
 
 def main():
-    if not os.path.exists(DRIVER):
+    if not refcase.available():
         raise SystemExit("oracle/_ref/ref_driver is not built (oracle/build_ref.sh)")
     # volume steps: shrink and grow at 1 bar and at 5 kbar, 2 % maximum scaling
     vols = [(0.30, 1.0e5, 1.02), (0.85, 1.0e5, 1.02), (0.10, 5.0e8, 1.02), (0.65, 1.0e5, 1.05)]
