@@ -196,6 +196,22 @@ module yamc_ewald
          integer(c_int32_t), intent(out) :: accepted
          type(c_ptr), value :: rec
       end function
+      integer(c_int) function yamc_set_box(ctx, box_length, scale_positions, overlap) bind(C, name="yamc_set_box")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: box_length
+         integer(c_int32_t), value :: scale_positions
+         integer(c_int32_t), intent(out) :: overlap
+      end function
+      integer(c_int) function yamc_mc_commit_move(ctx, imol, x, y, z, du_c, du_lj, overlap) &
+            bind(C, name="yamc_mc_commit_move")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: imol
+         real(c_double), intent(in) :: x(*), y(*), z(*)
+         real(c_double), value :: du_c, du_lj
+         integer(c_int32_t), intent(out) :: overlap
+      end function
       integer(c_int) function yamc_box_length(ctx, box_length) bind(C, name="yamc_box_length")
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: ctx

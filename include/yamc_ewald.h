@@ -230,6 +230,24 @@ int yamc_mc_run_npt(yamc_ctx *ctx, int32_t ncycles, double pressure_step_multipl
                     int64_t *n_vol_accepted);
 /* Current box length in Angstrom (atomic_data%BoxLength). */
 int yamc_box_length(yamc_ctx *ctx, double *box_length);
+/* The box change of mcvol without its acceptance test (MC.f90:760-777):
+ * scale_box when scale_positions != 0 (molecules keep their centres of mass),
+ * Parameters_recalc(box_length), LJTypes_scale_sigma, initBetaLJ, then
+ * mc_calc_total_sums with beta scaled by xclb_new / xclb_old.  *overlap
+ * (may be NULL) is the recompute's overlap flag. */
+int yamc_set_box(yamc_ctx *ctx, double box_length, int32_t scale_positions, int32_t *overlap);
+
+/* ---- host-driven moves ------------------------------------------------- */
+/* For hosts that keep their own move loop over the per-function surface
+ * (yamc_kspace_delta, yamc_rspace_partial, their own acceptance test): the
+ * commit of an accepted movemc step (MC.f90:1242-1320) for molecule imol
+ * moved from its current place to (x, y, z): S(k) += the molecule's delta
+ * (RhoSquared_add), per-atom real-space arrays from the old and new partial
+ * sums' back arrays, the real-space components, E_kC += du_c and
+ * E_kLJ += du_lj (as yamc_kspace_delta returned them), positions and mu.
+ * If the new place overlaps, nothing is committed and *overlap = 1. */
+int yamc_mc_commit_move(yamc_ctx *ctx, int32_t imol, const double *x, const double *y, const double *z,
+                        double du_c, double du_lj, int32_t *overlap);
 
 /* ---- batched replicas -------------------------------------------------- */
 /* Advances n independent contexts (uneven sizes, compositions and

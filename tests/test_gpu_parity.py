@@ -688,3 +688,66 @@ def test_batch_two_chains_per_cu_equals_single(lib, precision, cells):
         y.close()
     for x in a:
         x.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_host_driven_moves_commit_consistently(lib, precision):
+    """A host-side move loop over the per-function surface: yamc_kspace_delta
+    for dU_k, yamc_mc_commit_move for accepted displacements.  After 30 moves
+    the carried S(k), energy components and per-atom real-space arrays equal
+    a rebuild at the final positions, and dU_k equals the change of E_k."""
+    from yamc_amd import config
+    from yamc_amd.ewald import EwaldMC
+    s = config("small", seed=12)
+    g = EwaldMC(s, precision=precision)
+    g.mc_init()
+    rng = np.random.default_rng(4)
+    ncommit = 0
+    for _ in range(30):
+        im = int(rng.integers(0, s.nmol))
+        f, l = int(s.mol_first[im]), int(s.mol_last[im])
+        x, y, z = (a[f:l + 1] for a in g.positions())
+        d = rng.uniform(-0.01, 0.01, 3)
+        nx, ny, nz = x + d[0], y + d[1], z + d[2]
+        dc, dl = g.kspace_delta(im, nx, ny, nz)
+        e0 = g.energies()
+        if g.commit_move(im, nx, ny, nz, dc, dl):
+            ncommit += 1
+            e1 = g.energies()
+            assert abs((e1["E_kC"] - e0["E_kC"]) - dc) <= 1e-12 * max(1.0, abs(e0["E_kC"]))
+    assert ncommit > 20
+    h = EwaldMC(s, precision=precision)
+    h.set_positions(*g.positions())
+    e_new, ov = h.mc_init()
+    assert not ov
+    tol = 1e-10 if precision == 0 else 1e-6
+    eg = g.energies()
+    for k in e_new:
+        assert abs(eg[k] - e_new[k]) <= tol * max(1.0, abs(e_new[k])), (k, eg[k], e_new[k])
+    for a, b in zip(g.real_arrays(), h.real_arrays()):
+        assert rel(a, b, np.abs(b).max()) < 1e-10
+    assert rel(g.sums(), h.sums(), np.abs(h.sums()).max()) < (1e-10 if precision == 0 else 1e-7)
+
+
+def test_set_box_matches_accepted_volume_step(lib):
+    """yamc_set_box with scaled positions reproduces the state of an accepted
+    mcvol step of the oracle to the same new box length (and a second call
+    back to the old box restores the energies)."""
+    import golden_cases as gc
+    s, g, o = make("small", seed=21)
+    g.mc_init()
+    o.mc_init()
+    e0 = g.energies()
+    acc, L, ov = o.volume_move(gc.pressure_kT_A3(1e5, s.params.temp), float(np.log(1.02)), u=0.52)
+    assert acc and not ov
+    assert not g.set_box(L, True)
+    assert abs(g.box_length() - L) < 1e-13 * L
+    eg, eo = g.energies(), o.energies()
+    for k in eo:
+        assert rel(eg[k], eo[k], max(1.0, abs(eo[k]))) < RTOL, k
+    for a, b in zip(g.positions(), o.positions()):
+        assert rel(a, b, 1.0) < 1e-13
+    assert not g.set_box(s.params.box_length, True)
+    e2 = g.energies()
+    for k in e0:
+        assert rel(e2[k], e0[k], max(1.0, abs(e0[k]))) < 1e-9, k

@@ -28,7 +28,7 @@ EXPORTS = [
     "yamc_kspace_forces", "yamc_rspace_full", "yamc_rspace_partial", "yamc_tails", "yamc_external_mu",
     "yamc_mc_init", "yamc_mc_energies", "yamc_mc_real_arrays", "yamc_mc_set_rng", "yamc_mc_get_rng",
     "yamc_mc_run", "yamc_mc_run_const", "yamc_resync", "yamc_mc_volume_move", "yamc_mc_run_npt", "yamc_box_length",
-    "yamc_mc_set_frequencies", "yamc_mc_xchange", "yamc_batch_run", "yamc_batch_create", "yamc_batch_launch",
+    "yamc_mc_set_frequencies", "yamc_mc_xchange", "yamc_set_box", "yamc_mc_commit_move", "yamc_batch_run", "yamc_batch_create", "yamc_batch_launch",
     "yamc_batch_destroy", "yamc_forces", "yamc_rdf",
     "yamc_set_block_size", "yamc_last_kernel_ms", "yamc_phase_times", "yamc_last_error",
 ]

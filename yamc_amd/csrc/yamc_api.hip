@@ -1132,6 +1132,48 @@ static int vol_buffers(yamc_ctx *c, std::vector<std::pair<void *, size_t>> &bufs
     return YAMC_OK;
 }
 
+// The box change of a volume step: scale_box (optional) + Parameters_recalc
+// (L_new) + LJTypes_scale_sigma + initBetaLJ (MC.f90:760-770: sigma scales
+// by sc = L_old / L_new, so sigma^6 and LJ6 by sc^6, LJ12 by sc^12), then
+// mc_calc_total_sums with beta scaled by xclb_new / xclb_old.
+static int change_box(yamc_ctx *c, double L_new, bool scale_positions, int32_t *overlap) {
+    YdParams &p = c->ch.p;
+    const double L_old = p.L, sc = L_old / L_new;
+    if (scale_positions) {
+        yk_scale_box<<<(c->nmol + 255) / 256, 256, 0, c->stream>>>(c->ch, sc);
+        HIPCHK(hipGetLastError());
+    }
+    const double xclb_new = (p.xclb * L_old) / L_new;
+    p.L = L_new;
+    p.kext = p.kext * (xclb_new / p.xclb);
+    p.m2kext = -2 * p.kext;
+    p.dr = (p.dr * L_old) / L_new;
+    const double scale_beta = xclb_new / p.xclb;
+    p.xclb = xclb_new;
+    const double sc2 = sc * sc, sc6 = sc2 * sc2 * sc2;
+    for (size_t t = 0; t < c->lj6.size(); ++t) {
+        c->sigma6[t] = c->sigma6[t] * sc6;
+        c->lj6[t] = c->lj6[t] * sc6;
+        c->lj12[t] = c->lj12[t] * sc6 * sc6;
+    }
+    TRY(upload_box_tables(c, L_new));
+    return calc_total_sums(c, scale_beta, overlap);
+}
+
+extern "C" int yamc_set_box(yamc_ctx *c, double box_length, int32_t scale_positions, int32_t *overlap) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    if (!c->state_ready) return fail(YAMC_ERROR_INITIALIZATION, "yamc_mc_init has not run on this state");
+    if (!(box_length > 0.0) || !std::isfinite(box_length))
+        return fail(YAMC_ERROR_PARAMETER, "box length %g must be positive", box_length);
+    TRY(sync(c));
+    int32_t ov = 0;
+    TRY(change_box(c, box_length, scale_positions != 0, &ov));
+    HIPCHK(hipMemcpyAsync(c->d_chain, &c->ch, sizeof(YdChain), hipMemcpyHostToDevice, c->stream));
+    TRY(sync(c));
+    if (overlap) *overlap = ov;
+    return YAMC_OK;
+}
+
 extern "C" int yamc_mc_volume_move(yamc_ctx *c, double pressure_kT_A3, double log_max_volume_scaling,
                                    const double *u, int32_t *accepted, double *box_length) {
     if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
@@ -1154,28 +1196,8 @@ extern "C" int yamc_mc_volume_move(yamc_ctx *c, double pressure_kT_A3, double lo
         HIPCHK(hipMemcpyAsync(c->vol_save[i], bufs[i].first, bufs[i].second, hipMemcpyDeviceToDevice, c->stream));
     const YdParams p_old = p;
     const std::vector<double> lj6_old = c->lj6, lj12_old = c->lj12, sig6_old = c->sigma6;
-    // scale_box + Parameters_recalc(L_new) + LJTypes_scale_sigma + initBetaLJ
-    // (MC.f90:760-770): sigma scales by sc, so sigma^6 and LJ6 by sc^6, LJ12 by sc^12
-    const double sc = L_old / L_new;
-    yk_scale_box<<<(c->nmol + 255) / 256, 256, 0, c->stream>>>(c->ch, sc);
-    HIPCHK(hipGetLastError());
-    const double xclb_new = (p.xclb * L_old) / L_new;
-    p.L = L_new;
-    p.kext = p.kext * (xclb_new / p.xclb);
-    p.m2kext = -2 * p.kext;
-    p.dr = (p.dr * L_old) / L_new;
-    const double scale_beta = xclb_new / p.xclb;
-    p.xclb = xclb_new;
-    const double sc2 = sc * sc, sc6 = sc2 * sc2 * sc2;
-    for (size_t t = 0; t < c->lj6.size(); ++t) {
-        c->sigma6[t] = c->sigma6[t] * sc6;
-        c->lj6[t] = c->lj6[t] * sc6;
-        c->lj12[t] = c->lj12[t] * sc6 * sc6;
-    }
-    TRY(upload_box_tables(c, L_new));
-    // mc_calc_total_sums(scale_beta = xclb_new / xclb_old)
     int32_t ov = 0;
-    TRY(calc_total_sums(c, scale_beta, &ov));
+    TRY(change_box(c, L_new, true, &ov));
     double proba = -1.0;
     if (!ov) {
         TRY(yamc_mc_energies(c, e));
@@ -1549,5 +1571,69 @@ extern "C" int yamc_mc_xchange(yamc_ctx *c, int32_t imol1, int32_t imol2, const 
     int acc = 0;
     TRY(xchange_step(c, imol1, imol2, u, rec, &acc));
     if (accepted) *accepted = acc;
+    return YAMC_OK;
+}
+
+// ---- host-driven moves ------------------------------------------------------
+// movemc's commit (MC.f90:1242-1320) for molecule imol moved from its current
+// place to (x, y, z): S(k) += the molecule's delta (FP32C: hi/lo re-split),
+// per-atom real-space arrays from the back arrays of the old and new partial
+// sums, the real-space components, E_kC += du_c, E_kLJ += du_lj (the values
+// yamc_kspace_delta returned for the move), positions, mu.
+extern "C" int yamc_mc_commit_move(yamc_ctx *c, int32_t imol, const double *x, const double *y, const double *z,
+                                   double du_c, double du_lj, int32_t *overlap) {
+    if (!c || !x || !y || !z) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    if (!c->state_ready) return fail(YAMC_ERROR_INITIALIZATION, "yamc_mc_init has not run on this state");
+    if (imol < 0 || imol >= c->nmol) return fail(YAMC_ERROR_PARAMETER, "molecule index %d out of range", imol);
+    TRY(sync(c));
+    TRY(xchg_buffers(c));
+    const int N = c->natom, f = c->h_first[imol], n = c->h_last[imol] - f + 1;
+    double xo[YD_MAXA], yo[YD_MAXA], zo[YD_MAXA], xn[YD_MAXA], yn[YD_MAXA], zn[YD_MAXA];
+    HIPCHK(hipMemcpyAsync(xo, c->ch.x + f, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(yo, c->ch.y + f, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(zo, c->ch.z + f, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    TRY(sync(c));
+    for (int a = 0; a < n; ++a) { xn[a] = x[a]; yn[a] = y[a]; zn[a] = z[a]; }
+    double *bn = c->d_xb, *bo = bn + 4 * (size_t)N;
+    double on[YK_NRS + 1], oo[YK_NRS + 1];
+    TRY(partial_dev(c, imol, xn, yn, zn, bn, on));
+    if (on[YK_NRS] != 0.0) {  // the new place overlaps: nothing is committed
+        if (overlap) *overlap = 1;
+        return YAMC_OK;
+    }
+    TRY(partial_dev(c, imol, xo, yo, zo, bo, oo));
+    YxAtoms at = {};
+    at.n = n;
+    for (int a = 0; a < n; ++a) {
+        at.typ[a] = c->h_type[f + a];
+        at.q[a] = c->h_q[f + a];
+        if (at.typ[a] > 0) at.pmask |= 1u << (at.typ[a] - 1);
+        at.pos[0][0][a] = xo[a]; at.pos[0][1][a] = yo[a]; at.pos[0][2][a] = zo[a];
+        at.pos[1][0][a] = xn[a]; at.pos[1][1][a] = yn[a]; at.pos[1][2][a] = zn[a];
+    }
+    yk_xchg_delta<<<(c->nk + 255) / 256, 256, 0, c->stream>>>(c->ch, at, c->d_xd);
+    const unsigned nsb = (unsigned)((((size_t)(1 + c->ntype) * 8 * c->nk) + 255) / 256);
+    if (c->precision == YAMC_FP32C) yk_xchg_commit_sums<true><<<nsb, 256, 0, c->stream>>>(c->ch, c->d_xd, at.pmask);
+    else yk_xchg_commit_sums<false><<<nsb, 256, 0, c->stream>>>(c->ch, c->d_xd, at.pmask);
+    HIPCHK(hipGetLastError());
+    double mv[8 * YD_MAXA] = {};
+    for (int a = 0; a < n; ++a)
+        for (int k = 0; k < 4; ++k) mv[4 * a + k] = k == 0 ? on[a] : on[YD_MAXA + 3 * a + k - 1];
+    HIPCHK(hipMemcpyAsync(c->d_xmv, mv, sizeof mv, hipMemcpyHostToDevice, c->stream));
+    yk_xchg_commit_rs<<<(N + 255) / 256, 256, 0, c->stream>>>(c->ch, bn, bo, nullptr, nullptr, c->d_xmv, f, n, 0, 0);
+    HIPCHK(hipGetLastError());
+    TRY(write_pos(c, f, n, xn, yn, zn));
+    double mu[3];
+    TRY(yamc_external_mu(c, mu));
+    YdState st;
+    TRY(get_state(c, &st));
+    const int E = 4 * YD_MAXA;
+    st.e_kc = st.e_kc + du_c;
+    st.e_klj = st.e_klj + du_lj;
+    st.uu_ew = st.uu_ew + on[E] - oo[E];
+    st.uu_lj6 = st.uu_lj6 + on[E + 1] - oo[E + 1];
+    st.uu_lj12 = st.uu_lj12 + on[E + 2] - oo[E + 2];
+    TRY(put_state(c, &st));
+    if (overlap) *overlap = 0;
     return YAMC_OK;
 }

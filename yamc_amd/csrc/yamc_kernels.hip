@@ -2678,6 +2678,7 @@ __global__ void yk_rspace_pair(YdChain c, int imol1, int imol2, const double *si
 // Per-atom real-space arrays after an accepted exchange (MC.f90:1786-1810):
 // every atom takes new1 - old1 + new2 - old2 of the back arrays, then the
 // two molecules' atoms take their new partial sums (mv[m][a] = uu, fx, fy, fz).
+// With n2 = 0 (one molecule, bn2/bo2 unused) it is movemc's commit.
 __global__ void yk_xchg_commit_rs(YdChain c, const double *bn1, const double *bo1, const double *bn2,
                                   const double *bo2, const double *mv, int f1, int n1, int f2, int n2) {
     const int N = c.p.natom;
@@ -2688,11 +2689,16 @@ __global__ void yk_xchg_commit_rs(YdChain c, const double *bn1, const double *bo
         } else if (j >= f2 && j < f2 + n2) {
             const double *v = mv + 4 * (YD_MAXA + j - f2);
             c.uu[j] = v[0]; c.Fx[j] = v[1]; c.Fy[j] = v[2]; c.Fz[j] = v[3];
-        } else {
+        } else if (n2 > 0) {
             c.uu[j] = c.uu[j] + bn1[j] - bo1[j] + bn2[j] - bo2[j];
             c.Fx[j] = c.Fx[j] + bn1[N + j] - bo1[N + j] + bn2[N + j] - bo2[N + j];
             c.Fy[j] = c.Fy[j] + bn1[2 * N + j] - bo1[2 * N + j] + bn2[2 * N + j] - bo2[2 * N + j];
             c.Fz[j] = c.Fz[j] + bn1[3 * N + j] - bo1[3 * N + j] + bn2[3 * N + j] - bo2[3 * N + j];
+        } else {  // one molecule (movemc's commit, MC.f90:1257-1275)
+            c.uu[j] = c.uu[j] + bn1[j] - bo1[j];
+            c.Fx[j] = c.Fx[j] + bn1[N + j] - bo1[N + j];
+            c.Fy[j] = c.Fy[j] + bn1[2 * N + j] - bo1[2 * N + j];
+            c.Fz[j] = c.Fz[j] + bn1[3 * N + j] - bo1[3 * N + j];
         }
     }
 }

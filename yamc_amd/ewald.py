@@ -252,6 +252,22 @@ class EwaldMC:
                                        C.byref(a), C.byref(v), C.byref(va)))
         return a.value, v.value, va.value
 
+    def set_box(self, box_length: float, scale_positions: bool = True) -> bool:
+        """The box change of mcvol without its acceptance test; returns the
+        recompute's overlap flag."""
+        ov = C.c_int32()
+        check(self.lib.yamc_set_box(self.ctx, C.c_double(box_length), int(bool(scale_positions)), C.byref(ov)))
+        return bool(ov.value)
+
+    def commit_move(self, imol: int, x, y, z, du_c: float, du_lj: float) -> bool:
+        """movemc's commit (MC.f90:1242-1320) of molecule imol moved to
+        (x, y, z); returns False (nothing committed) if the place overlaps."""
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+        ov = C.c_int32()
+        check(self.lib.yamc_mc_commit_move(self.ctx, int(imol), dptr(x), dptr(y), dptr(z), C.c_double(du_c),
+                                           C.c_double(du_lj), C.byref(ov)))
+        return not ov.value
+
     def box_length(self) -> float:
         L = C.c_double()
         check(self.lib.yamc_box_length(self.ctx, C.byref(L)))
