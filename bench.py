@@ -14,9 +14,11 @@ Forms timed:
   batch     FP32-compensated, R independent replicas of the same system (own
             Philox streams) advanced by one captured launch per sweep, launch
             shape chosen by yamc_amd.tune -> `value`
+  batch_fp64  the same batch in the plain FP64 form
 The per-move dE and force errors of fp32c against fp64 are measured on one
 stream (moves up to the first differing decision).  The CPU baseline is the
-FP64 oracle (a plain-C restatement of the reference) timed on this host.
+reference's own Fortran compiled from its sources (oracle/_ref, when built;
+else the plain-C oracle), timed on this host.
 
 Prints one JSON line.
 """
@@ -61,7 +63,7 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="batch launch shape (64, 128, 256, 512); 0 = autotune")
     ap.add_argument("--cpu-moves", type=int, default=0, help="moves for the CPU baseline (0 = auto ~15 s)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp64,fp32c,batch")
+    ap.add_argument("--forms", default="fp64,fp32c,batch,batch_fp64")
     ap.add_argument("--forces-check", action="store_true",
                     help="instead of the headline line: forward energy + adjoint forces of the 24,896-atom "
                          "workload (FP32C), timed, with a central-difference check on 64 seeded atoms and the "
@@ -216,6 +218,35 @@ def main():
         for r in reps:
             r.close()
 
+    # ------------------------------------- the same batch in the plain FP64 form
+    if "batch_fp64" in forms:
+        R = args.replicas
+        reps = []
+        for i in range(R):
+            r = EwaldMC(base, precision=YAMC_FP64)
+            r.mc_init()
+            r.set_rng(20261203, i)
+            reps.append(r)
+        shape64, tuned64 = tune_batch(reps)
+        bat = Batch(reps, 1)
+        for _ in range(args.warmup):
+            bat.launch()
+        sync()
+        t0 = time.perf_counter()
+        nacc = 0
+        for _ in range(args.steps):
+            acc, _ = bat.launch()
+            nacc += int(acc.sum())
+        sync()
+        dt = time.perf_counter() - t0
+        moves = args.steps * nmol * R
+        res["batch_fp64"] = dict(replicas=R, moves_per_s=moves / dt, ms_per_step=1e3 * dt / args.steps,
+                                 acceptance=nacc / moves, launch_shape=shape64,
+                                 tuner_moves_per_s={str(k): v for k, v in tuned64.items()})
+        bat.close()
+        for r in reps:
+            r.close()
+
     # ------------------------------------------------------------ roofline
     if grid is None:
         g0 = EwaldMC(base)
@@ -319,6 +350,8 @@ def main():
     }
     if "fp64" in res and "batch" in res:
         out["batch_vs_fp64_single"] = res["batch"]["moves_per_s"] / res["fp64"]["moves_per_s"]
+    if "batch_fp64" in res and "batch" in res:
+        out["batch_fp32c_vs_batch_fp64"] = res["batch"]["moves_per_s"] / res["batch_fp64"]["moves_per_s"]
     print(json.dumps(out))
 
 
