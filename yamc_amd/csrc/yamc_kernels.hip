@@ -29,6 +29,9 @@
 // deterministic ascending compaction into the list).
 #define YK_BMWORDS 4096  // 131072 atoms
 #define YK_MAXCL 125     // cells visited by one cell-list scan (5 per axis)
+// Positions each thread loads ahead in the real-space scan (independent
+// loads in flight per thread when the FP32 positions are read from HBM).
+#define YK_SCANU 8
 
 // ------------------------------------------------------------------------
 // Functions.f90:26-111 erfc_Luc_bet6_bet12
@@ -469,6 +472,38 @@ __device__ __forceinline__ void yk_ld8(const YdChain &c, int set, int kk, double
     }
 }
 
+// The same 8 components kept as loaded (FP32C: hi and, for the Coulomb set,
+// lo words; FP64: the doubles), so that a sweep can issue the loads of a k
+// well before it needs the values: converting at load time would wait for
+// them there.
+template <bool F32> struct YkRaw8 { double v[8]; };
+template <> struct YkRaw8<true> { float h[8], l[8]; };
+
+template <bool F32, bool LO>
+__device__ __forceinline__ void yk_raw8_load(const YdChain &c, int set, int kk, YkRaw8<F32> &r) {
+    const size_t base = (size_t)set * 8 * c.p.nk + kk;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const size_t idx = base + (size_t)m * c.p.nk;
+        if constexpr (F32) {
+            r.h[m] = c.Shi[idx];
+            if (LO) r.l[m] = c.Slo[idx];
+        } else {
+            r.v[m] = c.S[idx];
+        }
+    }
+}
+
+// Values of yk_ld8 from a raw load (LO: the Coulomb set's hi + lo).
+template <bool F32, bool LO>
+__device__ __forceinline__ void yk_raw8_get(const YkRaw8<F32> &r, double *C) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        if constexpr (F32) C[m] = LO ? (double)r.h[m] + (double)r.l[m] : (double)r.h[m];
+        else C[m] = r.v[m];
+    }
+}
+
 // Accumulator layout of the sweeps: [0] dU_C, [1] dU_LJ, [2 + 3a + d] the
 // k-space force on atom a (Coulomb and LJ parts summed per k; the total over
 // k differs from the reference's separate sums by rounding only).
@@ -663,22 +698,40 @@ __device__ void yk_sweep(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC
 // and the molecule's LJ sets.  FP32C keeps (hi, lo) as a two-sum pair.  All
 // touched values of a k are loaded before the first store.
 template <bool F32>
+__device__ __forceinline__ void yk_commit_load(const YdChain &c, const YkMol &m, int kk, int &kxyz,
+                                               double (&v)[1 + YK_MAXP][8]) {
+    const int nk = c.p.nk;
+    kxyz = c.kxyz[kk];
+#pragma unroll
+    for (int set = 0; set <= YK_MAXP; ++set) {
+        if (set > m.np) break;
+        int sidx = set == 0 ? 0 : m.ptype[set - 1];
+        size_t base = (size_t)sidx * 8 * nk + kk;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            size_t idx = base + (size_t)i * nk;
+            v[set][i] = F32 ? (double)c.Shi[idx] + (double)c.Slo[idx] : c.S[idx];
+        }
+    }
+}
+
+// The loads of a thread's next k are issued before the current k's stores
+// (a different kk, so they never alias), which keeps one k of loads in
+// flight while the current one is updated.
+template <bool F32>
 __device__ void yk_commit(const YdChain &c, const YkMol &m, int rank, int size) {
     const int nk = c.p.nk;
+    int kxyz_n = 0;
+    double vn[1 + YK_MAXP][8];
+    if (rank < nk) yk_commit_load<F32>(c, m, rank, kxyz_n, vn);
     for (int kk = rank; kk < nk; kk += size) {
-        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        const int kxyz = kxyz_n, kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
         double v[1 + YK_MAXP][8];
 #pragma unroll
-        for (int set = 0; set <= YK_MAXP; ++set) {
-            if (set > m.np) break;
-            int sidx = set == 0 ? 0 : m.ptype[set - 1];
-            size_t base = (size_t)sidx * 8 * nk + kk;
+        for (int set = 0; set <= YK_MAXP; ++set)
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                size_t idx = base + (size_t)i * nk;
-                v[set][i] = F32 ? (double)c.Shi[idx] + (double)c.Slo[idx] : c.S[idx];
-            }
-        }
+            for (int i = 0; i < 8; ++i) v[set][i] = vn[set][i];
+        if (kk + size < nk) yk_commit_load<F32>(c, m, kk + size, kxyz_n, vn);
 #pragma unroll
         for (int set = 0; set <= YK_MAXP; ++set) {
             if (set > m.np) break;
@@ -936,17 +989,33 @@ __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const float4 
         xof[a] = (float)m.xo[a]; yof[a] = (float)m.yo[a]; zof[a] = (float)m.zo[a];
         xnf[a] = (float)m.xn[a]; ynf[a] = (float)m.yn[a]; znf[a] = (float)m.zn[a];
     }
-    for (int t = tid; t < total; t += (int)blockDim.x) {
-        int lo = 0, hi = ncl - 1;  // the bucket holding flat index t
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (cl_start[mid] <= t) lo = mid;
-            else hi = mid - 1;
+    const int bs = (int)blockDim.x;
+    for (int t0 = tid; t0 < total; t0 += YK_SCANU * bs) {
+        int jj[YK_SCANU];  // atom indices first, then their positions, YK_SCANU loads at a time
+#pragma unroll
+        for (int u = 0; u < YK_SCANU; ++u) {
+            const int t = t0 + u * bs;
+            jj[u] = -1;
+            if (t < total) {
+                int lo = 0, hi = ncl - 1;  // the bucket holding flat index t
+                while (lo < hi) {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (cl_start[mid] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                jj[u] = c.cell_atoms[(size_t)cl_cell[lo] * c.ccap + (t - cl_start[lo])];
+            }
         }
-        int j = c.cell_atoms[(size_t)cl_cell[lo] * c.ccap + (t - cl_start[lo])];
-        float4 pj = pos[j];
-        if (__float_as_int(pj.w) != me && yk_near<NA>(xof, yof, zof, xnf, ynf, znf, pj, r2f))
-            atomicOr(&bm[j >> 5], 1u << (j & 31));
+        float4 pj[YK_SCANU];
+#pragma unroll
+        for (int u = 0; u < YK_SCANU; ++u)
+            pj[u] = jj[u] >= 0 ? pos[jj[u]] : make_float4(0.f, 0.f, 0.f, __int_as_float(me));
+#pragma unroll
+        for (int u = 0; u < YK_SCANU; ++u) {
+            const int j = jj[u];
+            if (__float_as_int(pj[u].w) != me && yk_near<NA>(xof, yof, zof, xnf, ynf, znf, pj[u], r2f))
+                atomicOr(&bm[j >> 5], 1u << (j & 31));
+        }
     }
     return true;
 }
@@ -962,10 +1031,22 @@ __device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const float4 *pos, 
             xof[a] = (float)m.xo[a]; yof[a] = (float)m.yo[a]; zof[a] = (float)m.zo[a];
             xnf[a] = (float)m.xn[a]; ynf[a] = (float)m.yn[a]; znf[a] = (float)m.zn[a];
         }
-        for (int j = threadIdx.x; j < N; j += (int)blockDim.x) {
-            float4 pj = pos[j];
-            if (yk_near<NA>(xof, yof, zof, xnf, ynf, znf, pj, r2f) && __float_as_int(pj.w) != me)
-                atomicOr(&bm[j >> 5], 1u << (j & 31));
+        // YK_SCANU positions per thread are loaded before any is tested, so
+        // that many loads are in flight at once when pos is in HBM
+        const int bs = (int)blockDim.x;
+        for (int j0 = threadIdx.x; j0 < N; j0 += YK_SCANU * bs) {
+            float4 pj[YK_SCANU];
+#pragma unroll
+            for (int u = 0; u < YK_SCANU; ++u) {
+                const int j = j0 + u * bs;
+                pj[u] = j < N ? pos[j] : make_float4(0.f, 0.f, 0.f, __int_as_float(me));
+            }
+#pragma unroll
+            for (int u = 0; u < YK_SCANU; ++u) {
+                const int j = j0 + u * bs;
+                if (__float_as_int(pj[u].w) != me && yk_near<NA>(xof, yof, zof, xnf, ynf, znf, pj[u], r2f))
+                    atomicOr(&bm[j >> 5], 1u << (j & 31));
+            }
         }
     } else {
         for (int j = threadIdx.x; j < N; j += (int)blockDim.x) {
@@ -1046,14 +1127,32 @@ __device__ void yk_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
 }
 
 // Commit of the per-atom arrays from the stored back terms of the list.
+// Four candidates per thread are loaded before the first store (the list
+// holds distinct atoms, so no store of one candidate aliases another's load).
 __device__ void yk_rs_commit_list(const YdChain &c, const int *list, int count, int rank, int size) {
-    for (int i = rank; i < count; i += size) {
-        int j = list[i];
-        const double *b = c.rsb + (size_t)i * 4;
-        c.uu[j] = c.uu[j] + b[0];
-        c.Fx[j] = c.Fx[j] + b[1];
-        c.Fy[j] = c.Fy[j] + b[2];
-        c.Fz[j] = c.Fz[j] + b[3];
+    constexpr int U = 4;
+    for (int i0 = rank; i0 < count; i0 += U * size) {
+        int jj[U];
+        double b[U][4], v[U][4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * size;
+            jj[u] = i < count ? list[i] : -1;
+            if (jj[u] < 0) continue;
+            const double *bi = c.rsb + (size_t)i * 4;
+            const int j = jj[u];
+            b[u][0] = bi[0]; b[u][1] = bi[1]; b[u][2] = bi[2]; b[u][3] = bi[3];
+            v[u][0] = c.uu[j]; v[u][1] = c.Fx[j]; v[u][2] = c.Fy[j]; v[u][3] = c.Fz[j];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = jj[u];
+            if (j < 0) continue;
+            c.uu[j] = v[u][0] + b[u][0];
+            c.Fx[j] = v[u][1] + b[u][1];
+            c.Fy[j] = v[u][2] + b[u][2];
+            c.Fz[j] = v[u][3] + b[u][3];
+        }
     }
 }
 
@@ -1439,8 +1538,17 @@ __device__ int yk_choose(const YdChain &c, double rnd) {
     const int n = c.p.nmol;
     int g = (int)(rnd * n);
     if (g > n - 1) g = n - 1;
-    for (int i = (g > 0 ? g - 1 : 0); i <= (g + 1 < n ? g + 1 : n - 1); ++i)
-        if (rnd > c.freq_beg[i] && rnd < c.freq_end[i]) return i;
+    const int i0 = g > 0 ? g - 1 : 0, i1 = g + 1 < n ? g + 1 : n - 1;
+    double b[3], e[3];  // the three intervals are loaded before the first test
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        const int i = i0 + u < i1 ? i0 + u : i1;
+        b[u] = c.freq_beg[i];
+        e[u] = c.freq_end[i];
+    }
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+        if (i0 + u <= i1 && rnd > b[u] && rnd < e[u]) return i0 + u;
     // the intervals are disjoint: only the last one with beg < rnd can hold rnd
     int lo = 0, hi = n - 1;
     while (lo < hi) {
@@ -1461,15 +1569,31 @@ __device__ int yk_choose(const YdChain &c, double rnd) {
 
 // Coulomb set of a sweep (MC.f90:963-968 / 1136-1173, Coulomb parts):
 // DELTA adds dU_C (acc[0]) and evaluates the forces against S + dS.
+// The loads of a thread's next k (index, beta, S) are issued before the
+// current k's arithmetic.
 template <bool F32, bool DELTA, int NA>
 __device__ void yl_sweep_coul(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size) {
     const int nk = c.p.nk;
     const double *tabs = DELTA ? m.tab_n : m.tab_o;
+    int kxyz_n = 0;
+    double bt_n = 0.0;
+    YkRaw8<F32> r_n;
+    if (rank < nk) {
+        kxyz_n = c.kxyz[rank];
+        bt_n = c.beta[rank];
+        yk_raw8_load<F32, true>(c, 0, rank, r_n);
+    }
     for (int kk = rank; kk < nk; kk += size) {
-        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
-        double bt = c.beta[kk];
+        const int kxyz = kxyz_n, kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        const double bt = bt_n;
+        const YkRaw8<F32> r = r_n;
+        if (kk + size < nk) {
+            kxyz_n = c.kxyz[kk + size];
+            bt_n = c.beta[kk + size];
+            yk_raw8_load<F32, true>(c, 0, kk + size, r_n);
+        }
         double C[8];
-        yk_ld8<F32>(c, 0, kk, C);
+        yk_raw8_get<F32, true>(r, C);
         double kn[NA][8];
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
@@ -1513,8 +1637,13 @@ __device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_N
     const int nk = c.p.nk, T = c.p.ntype;
     if (T == 0 || m.np == 0) return;
     const double *tabs = DELTA ? m.tab_n : m.tab_o;
+    // the next k's index is loaded ahead; the S rows are loaded per set (a
+    // prefetch of the next set's row costs registers this loop does not have)
+    int kxyz_n = 0;
+    if (rank < nk) kxyz_n = c.kxyz[rank];
     for (int kk = rank; kk < nk; kk += size) {
-        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        const int kxyz = kxyz_n, kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        if (kk + size < nk) kxyz_n = c.kxyz[kk + size];
         double kl[NA][8];
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
@@ -1623,10 +1752,18 @@ template <int NA>
 __device__ void yl_rs_exact(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
                             const int *list, int count, double (&acc)[YK_NRX], int &ov, int rank, int size) {
     const int T = c.p.ntype;
+    // the next candidate's atom data are loaded before the current one's pairs
+    int jt_n = 0;
+    double q_n = 0.0, dh_n = 0.0, x_n = 0.0, y_n = 0.0, z_n = 0.0;
+    auto load_cand = [&](int i) {
+        const int j = list[i];
+        jt_n = c.type[j]; q_n = c.q[j]; dh_n = c.dh[j]; x_n = c.x[j]; y_n = c.y[j]; z_n = c.z[j];
+    };
+    if (rank < count) load_cand(rank);
     for (int i = rank; i < count; i += size) {
-        int j = list[i];
-        int jt = c.type[j];
-        double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+        const int jt = jt_n;
+        const double qj = q_n, dhj = dh_n, xj = x_n, yj = y_n, zj = z_n;
+        if (i + size < count) load_cand(i + size);
         double nb[4] = {0, 0, 0, 0}, ob[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
@@ -1921,6 +2058,11 @@ __device__ __forceinline__ void yk_sec_reject_rec(const YdChain &c, const YkMol 
 // energy components (MC.f90:1242-1318); single lane.
 __device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m, YkScalar &t0, float4 *pos) {
     int f = m.first;
+    YdState *st = c.st;
+    // the state's sums are read before the per-atom stores, which the
+    // compiler cannot tell apart from them
+    const double e_kc = st->e_kc, e_klj = st->e_klj, uu_ew = st->uu_ew, uu_lj6 = st->uu_lj6,
+                 uu_lj12 = st->uu_lj12;
     for (int a = 0; a < m.na; ++a) {
         c.uu[f + a] = t0.uun[a];
         c.Fx[f + a] = t0.frn[a][0];
@@ -1931,12 +2073,11 @@ __device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m
         c.z[f + a] = m.zn[a];
         if (pos) pos[f + a] = make_float4((float)m.xn[a], (float)m.yn[a], (float)m.zn[a], __int_as_float(m.imol + 1));
     }
-    YdState *st = c.st;
-    st->e_kc = st->e_kc + t0.duc;
-    st->e_klj = st->e_klj + t0.dulj;
-    st->uu_ew = st->uu_ew + t0.cmp_new[0] - t0.cmp_old[0];
-    st->uu_lj6 = st->uu_lj6 + t0.cmp_new[1] - t0.cmp_old[1];
-    st->uu_lj12 = st->uu_lj12 + t0.cmp_new[2] - t0.cmp_old[2];
+    st->e_kc = e_kc + t0.duc;
+    st->e_klj = e_klj + t0.dulj;
+    st->uu_ew = uu_ew + t0.cmp_new[0] - t0.cmp_old[0];
+    st->uu_lj6 = uu_lj6 + t0.cmp_new[1] - t0.cmp_old[1];
+    st->uu_lj12 = uu_lj12 + t0.cmp_new[2] - t0.cmp_old[2];
     for (int d = 0; d < 3; ++d) st->mu[d] = t0.mu_new[d];
 }
 
