@@ -221,6 +221,7 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
     p.dr = d->dr; p.d_angle = d->d_angle; p.lambda_f = d->lambda_f; p.lambda_c = d->lambda_c;
     p.alpha_over_sqrt_pi = d->alpha / sqrt(YD_PI);  // parameters.f90:400-401
     p.two_alpha_over_sqrt_pi = 2.0 * p.alpha_over_sqrt_pi;
+    p.inv_alpha = 1.0 / d->alpha;
     YdChain &h = c->ch;
     int TT = T * T > 0 ? T * T : 1;
     int rc = YAMC_OK;

@@ -33,7 +33,7 @@
 struct YdParams {
     int natom, nmol, ntype, kmax, nk, precision;
     double L, alpha, rmax2, xclb, kext, m2kext, dr, d_angle, lambda_f, lambda_c;
-    double alpha_over_sqrt_pi, two_alpha_over_sqrt_pi;
+    double alpha_over_sqrt_pi, two_alpha_over_sqrt_pi, inv_alpha;
     double pxch;  // xchange_probability (runmc.f90:61); 0 = no exchange steps
 };
 
@@ -99,6 +99,38 @@ __device__ __forceinline__ double yd_erfc_luc(double x, double x2, double ee2) {
     if (x < 0.455) return 1.0 - 2.0 / pi05 * x * (1.0 - x2 * (1.0 / 3.0 - x2 * (0.1 - x2 * (1.0 / 42.0 - x2 / 216.0))));
     if (x > 3.0) return (1.0 - 0.5 * (1.0 - 1.5 / x2) / x2) / (pi05 * x) * ee2;
     double T = 1.0 / (1.0 + P * x);
+    return T * (A1 + T * (A2 + T * (A3 + T * (A4 + T * A5)))) * ee2;
+}
+
+// 1/x and 1/sqrt(x) for positive normal x: the hardware estimate refined by
+// two Newton-Raphson steps, each of which squares the relative error (the
+// results are within a few ulp of the correctly rounded quotient or root).
+__device__ __forceinline__ double yd_rcp(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+__device__ __forceinline__ double yd_rsqrt(double x) {
+    const double h = 0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-h * y, y, 1.5);
+    return y * fma(-h * y, y, 1.5);
+}
+
+// erfc_Luc for the real-space pair sums, given ix = 1/x: the asymptotic
+// branch's quotients become products with ix and the middle branch's
+// reciprocal is yd_rcp (the values differ from erfc_Luc by rounding only).
+__device__ __forceinline__ double yd_erfc_luc_ix(double x, double x2, double ix, double ee2) {
+    const double P = 0.3275911, A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741, A4 = -1.453152027,
+                 A5 = 1.061405429, pi05 = 1.7724538509055;
+    if (x < 0.455) return 1.0 - 2.0 / pi05 * x * (1.0 - x2 * (1.0 / 3.0 - x2 * (0.1 - x2 * (1.0 / 42.0 - x2 / 216.0))));
+    if (x > 3.0) {
+        const double ix2 = ix * ix;
+        return (1.0 - 0.5 * (1.0 - 1.5 * ix2) * ix2) * ix * (1.0 / pi05) * ee2;
+    }
+    double T = yd_rcp(1.0 + P * x);
     return T * (A1 + T * (A2 + T * (A3 + T * (A4 + T * A5)))) * ee2;
 }
 
@@ -349,10 +381,11 @@ struct YdPair {
 
 __device__ __forceinline__ void yd_pair(const YdParams &p, double r2, bool same_mol, double qi, double qj, bool lj,
                                         double sigma6, double four_eps, YdPair &o) {
-    // 1/r and 1/r^2 once; the divisions of the reference expressions become
-    // multiplications (each within an ulp of the reference's quotient).
-    double r = sqrt(r2), ir = 1.0 / r, ir2 = ir * ir, a = p.alpha * r, a2 = a * a, e = exp(-a2);
-    double erfc_a = same_mol ? -yd_erf_luc(a, a2, e) : yd_erfc_luc(a, a2, e);
+    // 1/r and 1/r^2 once, from a refined reciprocal square root; the
+    // divisions of the reference expressions become multiplications (the
+    // values differ from the reference's by rounding only).
+    double ir = yd_rsqrt(r2), r = r2 * ir, ir2 = ir * ir, a = p.alpha * r, a2 = a * a, e = exp(-a2);
+    double erfc_a = same_mol ? -yd_erf_luc(a, a2, e) : yd_erfc_luc_ix(a, a2, ir * p.inv_alpha, e);
     double qq = p.xclb * qi * qj;
     o.potew = qq * erfc_a * ir;
     o.pot = o.potew;
