@@ -348,6 +348,10 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    # atom x k-vector evaluations: each trial move evaluates the moved molecule's
+    # atoms at every nonzero sign slot, at the old and at the new positions
+    # (2 n_m n_s); n_m averaged over the uniformly picked molecules
+    out["atom_k_evals_per_s"] = 2.0 * (base.natom / nmol) * n_s * value
     if "fp64" in res and "batch" in res:
         out["batch_vs_fp64_single"] = res["batch"]["moves_per_s"] / res["fp64"]["moves_per_s"]
     if "batch_fp64" in res and "batch" in res:

@@ -906,8 +906,10 @@ __device__ void yk_rs_commit_all(const YdChain &c, const YkMol &m, const double 
 // Returns false when the list overflowed (the caller then uses the full
 // FP64 pass yk_rs_move_n and the full commit).
 // FP32 distance test of one atom against the molecule's old and new sites
-// (margin 1e-5 box^2 on r^2, far above FP32 rounding), single-wrap minimum
-// image.
+// (margin 1e-5 box^2 on r^2, far above FP32 rounding).  The image is
+// d - rint(d), the nearest one: it equals the pair sums' single-wrap image
+// for |d| < 1.5 and is never farther, so every atom the exact pass can
+// count passes the test.
 template <int NA>
 __device__ __forceinline__ bool yk_near(const float (&xof)[NA], const float (&yof)[NA], const float (&zof)[NA],
                                         const float (&xnf)[NA], const float (&ynf)[NA], const float (&znf)[NA],
@@ -916,14 +918,10 @@ __device__ __forceinline__ bool yk_near(const float (&xof)[NA], const float (&yo
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
         float dx = xnf[a] - pj.x, dy = ynf[a] - pj.y, dz = znf[a] - pj.z;
-        dx = dx > 0.5f ? dx - 1.0f : dx; dx = dx < -0.5f ? dx + 1.0f : dx;
-        dy = dy > 0.5f ? dy - 1.0f : dy; dy = dy < -0.5f ? dy + 1.0f : dy;
-        dz = dz > 0.5f ? dz - 1.0f : dz; dz = dz < -0.5f ? dz + 1.0f : dz;
+        dx -= rintf(dx); dy -= rintf(dy); dz -= rintf(dz);
         near |= dx * dx + dy * dy + dz * dz < r2f;
         float ex = xof[a] - pj.x, ey = yof[a] - pj.y, ez = zof[a] - pj.z;
-        ex = ex > 0.5f ? ex - 1.0f : ex; ex = ex < -0.5f ? ex + 1.0f : ex;
-        ey = ey > 0.5f ? ey - 1.0f : ey; ey = ey < -0.5f ? ey + 1.0f : ey;
-        ez = ez > 0.5f ? ez - 1.0f : ez; ez = ez < -0.5f ? ez + 1.0f : ez;
+        ex -= rintf(ex); ey -= rintf(ey); ez -= rintf(ez);
         near |= ex * ex + ey * ey + ez * ez < r2f;
     }
     return near;
@@ -1817,6 +1815,33 @@ __device__ __forceinline__ void yl_rs_exact_n(const YdChain &c, const YkMol &m, 
     }
 }
 
+// yd_block_sum over the accumulators a move uses: entries with !used(i)
+// (uniform) are zero in every thread and are written as zero without the
+// shuffles; the used ones are summed in yd_block_sum's order.
+template <int NV, typename Used>
+__device__ __forceinline__ void yk_block_sum_used(double (&v)[NV], double *red, double *out, Used used) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        if (!used(i)) continue;
+        double a = v[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        v[i] = a;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) red[w * NV + i] = used(i) ? v[i] : 0.0;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double a = 0.0;
+        for (int j = 0; j < nw; ++j) a += red[j * NV + threadIdx.x];
+        out[threadIdx.x] = a;
+    }
+    __syncthreads();
+}
+
 // Phase timers of the chain kernel: thread 0 adds wall-clock ticks between
 // workgroup barriers into st->phase[] (pick, tables, old sweep, proposal,
 // real-space pass, new sweep, acceptance, commit, finalize).
@@ -2158,7 +2183,8 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
             yl_sweep_n<F32, false>(c, sh.m, acc, tid, blockDim.x);
-            yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
+            const int na = __builtin_amdgcn_readfirstlane(sh.m.na);
+            yk_block_sum_used(acc, sh.red, sh.out, [na](int i) { return i < 2 + 3 * na; });
         }
         YK_TICK(2);
         // ---- proposal (MC.f90:984-1067): forces and torques on wave 0,
@@ -2232,7 +2258,10 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
                 for (int i = 0; i < YK_NRS; ++i) acc[i] = a2[i];
             }
             if (ov) sh.ov = 1;
-            yd_block_sum<YK_NRX>(acc, sh.red, sh.out);
+            const int na = __builtin_amdgcn_readfirstlane(sh.m.na);
+            yk_block_sum_used(acc, sh.red, sh.out, [na](int i) {
+                return i < na || (i >= YD_MAXA && i < YD_MAXA + 3 * na) || i >= 4 * YD_MAXA;
+            });
             if (tid == 0) {
                 for (int a = 0; a < sh.m.na; ++a) {
                     t0.uun[a] = sh.out[a];
@@ -2261,7 +2290,8 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
             yl_sweep_n<F32, true>(c, sh.m, acc, tid, blockDim.x);
-            yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
+            const int na = __builtin_amdgcn_readfirstlane(sh.m.na);
+            yk_block_sum_used(acc, sh.red, sh.out, [na](int i) { return i < 2 + 3 * na; });
         }
         YK_TICK(5);
         // ---- acceptance on wave 0 (MC.f90:1176-1233) --------------------
@@ -2375,20 +2405,7 @@ __device__ int yw_scan(const YdChain &c, const YkMol &m, int lane) {
         bool near = false;
         if (j < N) {
             float4 pj = c.pos4[j];
-#pragma unroll
-            for (int a = 0; a < NA; ++a) {
-                float dx = xnf[a] - pj.x, dy = ynf[a] - pj.y, dz = znf[a] - pj.z;
-                dx = dx > 0.5f ? dx - 1.0f : dx; dx = dx < -0.5f ? dx + 1.0f : dx;
-                dy = dy > 0.5f ? dy - 1.0f : dy; dy = dy < -0.5f ? dy + 1.0f : dy;
-                dz = dz > 0.5f ? dz - 1.0f : dz; dz = dz < -0.5f ? dz + 1.0f : dz;
-                near |= dx * dx + dy * dy + dz * dz < r2f;
-                float ex = xof[a] - pj.x, ey = yof[a] - pj.y, ez = zof[a] - pj.z;
-                ex = ex > 0.5f ? ex - 1.0f : ex; ex = ex < -0.5f ? ex + 1.0f : ex;
-                ey = ey > 0.5f ? ey - 1.0f : ey; ey = ey < -0.5f ? ey + 1.0f : ey;
-                ez = ez > 0.5f ? ez - 1.0f : ez; ez = ez < -0.5f ? ez + 1.0f : ez;
-                near |= ex * ex + ey * ey + ez * ez < r2f;
-            }
-            near = near && __float_as_int(pj.w) != me;
+            near = yk_near<NA>(xof, yof, zof, xnf, ynf, znf, pj, r2f) && __float_as_int(pj.w) != me;
         }
         unsigned long long mask = __ballot(near);
         if (near) {
