@@ -829,6 +829,7 @@ struct yamc_batch {
     std::vector<yamc_ctx *> ctxs;
     YdChain *d_ch = nullptr;
     const double **d_s6 = nullptr, **d_fe = nullptr;
+    long long *d_cnt = nullptr;  // accepted-move counters before and after a launch (2 x n)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -859,6 +860,7 @@ static int batch_stage(yamc_ctx *const *ctxs, int32_t n, yamc_batch *b) {
     HIPCHK(hipMalloc((void **)&b->d_ch, sizeof(YdChain) * n));
     HIPCHK(hipMalloc((void **)&b->d_s6, sizeof(void *) * n));
     HIPCHK(hipMalloc((void **)&b->d_fe, sizeof(void *) * n));
+    HIPCHK(hipMalloc((void **)&b->d_cnt, sizeof(long long) * 2 * n));
     HIPCHK(hipMemcpy(b->d_ch, chains.data(), sizeof(YdChain) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_s6, s6.data(), sizeof(void *) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_fe, fe.data(), sizeof(void *) * n, hipMemcpyHostToDevice));
@@ -871,40 +873,51 @@ static void batch_free(yamc_batch *b) {
     if (b->d_ch) (void)hipFree(b->d_ch);
     if (b->d_s6) (void)hipFree(b->d_s6);
     if (b->d_fe) (void)hipFree(b->d_fe);
+    if (b->d_cnt) (void)hipFree(b->d_cnt);
     delete b;
 }
 
-static int batch_counts(const yamc_batch *b, std::vector<long long> &acc) {
-    acc.resize(b->n);
-    for (int i = 0; i < b->n; ++i) {
-        YdState st;
-        HIPCHK(hipMemcpy(&st, b->ctxs[i]->ch.st, sizeof st, hipMemcpyDeviceToHost));
-        acc[i] = st.n_accepted;
-    }
+// Copies every member's accepted-move counter into out[0..n) on the stream
+// (one small kernel instead of a device-to-host copy per member).
+__global__ void yk_batch_counts(const YdChain *chains, int n, long long *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = chains[i].st->n_accepted;
+}
+
+static int batch_counts_gather(const yamc_batch *b, int slot, hipStream_t s) {
+    yk_batch_counts<<<(b->n + 255) / 256, 256, 0, s>>>(b->d_ch, b->n, b->d_cnt + (size_t)slot * b->n);
+    HIPCHK(hipGetLastError());
+    return YAMC_OK;
+}
+
+// After gathers into slots 0 (before) and 1 (after) on stream s: accepted
+// moves per member over the launch.
+static int batch_counts_diff(const yamc_batch *b, hipStream_t s, int64_t *n_accepted) {
+    std::vector<long long> h(2 * (size_t)b->n);
+    HIPCHK(hipMemcpyAsync(h.data(), b->d_cnt, sizeof(long long) * 2 * b->n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int i = 0; i < b->n; ++i) n_accepted[i] = h[(size_t)b->n + i] - h[i];
     return YAMC_OK;
 }
 
 extern "C" int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int64_t *n_accepted) {
     yamc_batch *b = new yamc_batch;
     int rc = batch_stage(ctxs, n, b);
-    std::vector<long long> acc0, acc1;
-    if (rc == YAMC_OK && n_accepted) rc = batch_counts(b, acc0);
     if (rc == YAMC_OK) {
         yamc_ctx *c0 = b->c0;
         hipStream_t s = c0->stream;
         auto go = [&]() -> int {
+            if (n_accepted) TRY(batch_counts_gather(b, 0, s));
             HIPCHK(hipEventRecord(c0->ev0, s));
             TRY(launch_chain(c0, b->d_ch, b->d_s6, b->d_fe, n, ncycles, 0, b->max_natom));
             HIPCHK(hipEventRecord(c0->ev1, s));
+            if (n_accepted) TRY(batch_counts_gather(b, 1, s));
             TRY(sync(c0));
             HIPCHK(hipEventElapsedTime(&c0->last_ms, c0->ev0, c0->ev1));
+            if (n_accepted) TRY(batch_counts_diff(b, s, n_accepted));
             return YAMC_OK;
         };
         rc = go();
-    }
-    if (rc == YAMC_OK && n_accepted) {
-        rc = batch_counts(b, acc1);
-        for (int i = 0; rc == YAMC_OK && i < n; ++i) n_accepted[i] = acc1[i] - acc0[i];
     }
     batch_free(b);
     return rc;
@@ -946,19 +959,16 @@ extern "C" int yamc_batch_create(yamc_ctx *const *ctxs, int32_t n, int32_t ncycl
 
 extern "C" int yamc_batch_launch(yamc_batch *b, int64_t *n_accepted) {
     if (!b) return fail(YAMC_ERROR_PARAMETER, "null batch");
-    std::vector<long long> acc0, acc1;
-    if (n_accepted) TRY(batch_counts(b, acc0));
     yamc_ctx *c0 = b->c0;
     hipStream_t s = c0->stream;
+    if (n_accepted) TRY(batch_counts_gather(b, 0, s));
     HIPCHK(hipEventRecord(c0->ev0, s));
     HIPCHK(hipGraphLaunch(b->exec, s));
     HIPCHK(hipEventRecord(c0->ev1, s));
+    if (n_accepted) TRY(batch_counts_gather(b, 1, s));
     TRY(sync(c0));
     HIPCHK(hipEventElapsedTime(&c0->last_ms, c0->ev0, c0->ev1));
-    if (n_accepted) {
-        TRY(batch_counts(b, acc1));
-        for (int i = 0; i < b->n; ++i) n_accepted[i] = acc1[i] - acc0[i];
-    }
+    if (n_accepted) TRY(batch_counts_diff(b, s, n_accepted));
     return YAMC_OK;
 }
 
