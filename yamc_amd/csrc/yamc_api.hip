@@ -241,6 +241,7 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
         TRY(dalloc(c, &h.uu, N)); TRY(dalloc(c, &h.Fx, N)); TRY(dalloc(c, &h.Fy, N)); TRY(dalloc(c, &h.Fz, N));
         TRY(dalloc(c, &h.rsb, (size_t)YD_NBCAP * 4));
         TRY(dalloc(c, &h.nbl, (size_t)YD_NBCAP)); TRY(dalloc(c, &h.pos4, (size_t)N));
+        TRY(dalloc(c, &h.arec, (size_t)N * 8)); TRY(dalloc(c, &h.uf, (size_t)N * 4));
         TRY(dalloc(c, &h.freq_end, M)); TRY(dalloc(c, &h.freq_beg, M)); TRY(dalloc(c, &h.st, 1));
         // cell list of the real-space scan: cells of width >= rc (box units;
         // rmax2 is constant in box units, so volume steps keep nc); only

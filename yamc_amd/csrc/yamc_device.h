@@ -71,6 +71,12 @@ struct YdChain {
     double *rsb;          // per candidate of the current move: new - old back terms (YD_NBCAP x 4)
     int *nbl;             // candidate list of the current move (YD_NBCAP), wave kernel
     float4 *pos4;         // FP32 positions + molecule number bits, wave kernel's scan
+    // Launch-local atom records of the workgroup chain kernel, built at the
+    // start of each launch: arec (8 doubles per atom: x, y, z, q, dh, type,
+    // unused, unused) feeds the exact real-space pass with one 64-byte record
+    // per candidate; uf (uu, Fx, Fy, Fz per atom) carries the per-atom
+    // real-space arrays through the launch and is written back at its end.
+    double *arec, *uf;
     double *freq_beg, *freq_end;  // move intervals (BiasedRandom.f90:41-75)
     YdState *st;
     yamc_move_rec *rec;   // optional per-move records

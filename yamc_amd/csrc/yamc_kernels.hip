@@ -847,6 +847,7 @@ __device__ void yk_rs_move(const YdChain &c, const YkMol &m, const double *sig6,
 // minus old contribution, summed over the molecule's sites in index order as
 // the back arrays of the two partial sums hold them; acc[0..2] gather the
 // old partial sum's uu_ew, uu_lj6, uu_lj12.
+template <bool AOS>
 __device__ __forceinline__ void yk_rs_apply(const YdChain &c, const YkMol &m, const double *sig6,
                                             const double *feps, int j, double (&acc)[3]) {
     const int T = c.p.ntype;
@@ -876,6 +877,16 @@ __device__ __forceinline__ void yk_rs_apply(const YdChain &c, const YkMol &m, co
             }
         }
     }
+    if (AOS) {  // the chain kernel's launch copy of the per-atom arrays
+        double4 *u = reinterpret_cast<double4 *>(c.uf) + j;
+        double4 v = *u;
+        v.x = v.x + nb[0] - ob[0];
+        v.y = v.y + nb[1] - ob[1];
+        v.z = v.z + nb[2] - ob[2];
+        v.w = v.w + nb[3] - ob[3];
+        *u = v;
+        return;
+    }
     c.uu[j] = c.uu[j] + nb[0] - ob[0];
     c.Fx[j] = c.Fx[j] + nb[1] - ob[1];
     c.Fy[j] = c.Fy[j] + nb[2] - ob[2];
@@ -884,11 +895,12 @@ __device__ __forceinline__ void yk_rs_apply(const YdChain &c, const YkMol &m, co
 
 // Commit of the real-space arrays over every other atom (the candidate list
 // overflowed); acc[0..2] gather the old partial sum's uu_ew, uu_lj6, uu_lj12.
+template <bool AOS = false>
 __device__ void yk_rs_commit_all(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
                                  double (&acc)[3], int rank, int size) {
     const int me = m.imol + 1;
     for (int j = rank; j < c.p.natom; j += size)
-        if (c.molnum[j] != me) yk_rs_apply(c, m, sig6, feps, j, acc);
+        if (c.molnum[j] != me) yk_rs_apply<AOS>(c, m, sig6, feps, j, acc);
 }
 
 // Real-space pass of a trial move in two stages.
@@ -1127,6 +1139,8 @@ __device__ void yk_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
 // Commit of the per-atom arrays from the stored back terms of the list.
 // Four candidates per thread are loaded before the first store (the list
 // holds distinct atoms, so no store of one candidate aliases another's load).
+// AOS: on the chain kernel's launch copy (c.uf) of the per-atom arrays.
+template <bool AOS = false>
 __device__ void yk_rs_commit_list(const YdChain &c, const int *list, int count, int rank, int size) {
     constexpr int U = 4;
     for (int i0 = rank; i0 < count; i0 += U * size) {
@@ -1140,12 +1154,22 @@ __device__ void yk_rs_commit_list(const YdChain &c, const int *list, int count, 
             const double *bi = c.rsb + (size_t)i * 4;
             const int j = jj[u];
             b[u][0] = bi[0]; b[u][1] = bi[1]; b[u][2] = bi[2]; b[u][3] = bi[3];
-            v[u][0] = c.uu[j]; v[u][1] = c.Fx[j]; v[u][2] = c.Fy[j]; v[u][3] = c.Fz[j];
+            if (AOS) {
+                const double4 w = reinterpret_cast<const double4 *>(c.uf)[j];
+                v[u][0] = w.x; v[u][1] = w.y; v[u][2] = w.z; v[u][3] = w.w;
+            } else {
+                v[u][0] = c.uu[j]; v[u][1] = c.Fx[j]; v[u][2] = c.Fy[j]; v[u][3] = c.Fz[j];
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int j = jj[u];
             if (j < 0) continue;
+            if (AOS) {
+                reinterpret_cast<double4 *>(c.uf)[j] =
+                    make_double4(v[u][0] + b[u][0], v[u][1] + b[u][1], v[u][2] + b[u][2], v[u][3] + b[u][3]);
+                continue;
+            }
             c.uu[j] = v[u][0] + b[u][0];
             c.Fx[j] = v[u][1] + b[u][1];
             c.Fy[j] = v[u][2] + b[u][2];
@@ -1746,7 +1770,7 @@ __device__ __forceinline__ void yl_sweep_n(const YdChain &c, const YkMol &m, dou
 // the new sites, whose per-site sums need compile-time accumulator indices,
 // are unrolled; the old sites, which only feed site-independent sums, are
 // visited one at a time.
-template <int NA>
+template <int NA, bool AOS>
 __device__ void yl_rs_exact(const YdChain &c, const YkMol &m, const double *sig6, const double *feps,
                             const int *list, int count, double (&acc)[YK_NRX], int &ov, int rank, int size) {
     const int T = c.p.ntype;
@@ -1755,7 +1779,13 @@ __device__ void yl_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
     double q_n = 0.0, dh_n = 0.0, x_n = 0.0, y_n = 0.0, z_n = 0.0;
     auto load_cand = [&](int i) {
         const int j = list[i];
-        jt_n = c.type[j]; q_n = c.q[j]; dh_n = c.dh[j]; x_n = c.x[j]; y_n = c.y[j]; z_n = c.z[j];
+        if (AOS) {  // one 64-byte record of the launch copy
+            const double4 *r4 = reinterpret_cast<const double4 *>(c.arec) + 2 * (size_t)j;
+            const double4 a = r4[0], b = r4[1];
+            x_n = a.x; y_n = a.y; z_n = a.z; q_n = a.w; dh_n = b.x; jt_n = (int)b.y;
+        } else {
+            jt_n = c.type[j]; q_n = c.q[j]; dh_n = c.dh[j]; x_n = c.x[j]; y_n = c.y[j]; z_n = c.z[j];
+        }
     };
     if (rank < count) load_cand(rank);
     for (int i = rank; i < count; i += size) {
@@ -1804,14 +1834,15 @@ __device__ void yl_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
     }
 }
 
+template <bool AOS = false>
 __device__ __forceinline__ void yl_rs_exact_n(const YdChain &c, const YkMol &m, const double *sig6,
                                               const double *feps, const int *list, int count,
                                               double (&acc)[YK_NRX], int &ov, int rank, int size) {
     switch (__builtin_amdgcn_readfirstlane(m.na)) {
-        case 1: yl_rs_exact<1>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
-        case 2: yl_rs_exact<2>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
-        case 3: yl_rs_exact<3>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
-        default: yl_rs_exact<4>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+        case 1: yl_rs_exact<1, AOS>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+        case 2: yl_rs_exact<2, AOS>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+        case 3: yl_rs_exact<3, AOS>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
+        default: yl_rs_exact<4, AOS>(c, m, sig6, feps, list, count, acc, ov, rank, size); break;
     }
 }
 
@@ -2081,7 +2112,8 @@ __device__ __forceinline__ void yk_sec_reject_rec(const YdChain &c, const YkMol 
 
 // Commit of the moved molecule's own per-atom arrays, positions and the
 // energy components (MC.f90:1242-1318); single lane.
-__device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m, YkScalar &t0, float4 *pos) {
+__device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m, YkScalar &t0, float4 *pos,
+                                                bool aos = false) {
     int f = m.first;
     YdState *st = c.st;
     // the state's sums are read before the per-atom stores, which the
@@ -2097,6 +2129,11 @@ __device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m
         c.y[f + a] = m.yn[a];
         c.z[f + a] = m.zn[a];
         if (pos) pos[f + a] = make_float4((float)m.xn[a], (float)m.yn[a], (float)m.zn[a], __int_as_float(m.imol + 1));
+        if (aos) {  // the chain kernel's launch copies
+            reinterpret_cast<double4 *>(c.uf)[f + a] = make_double4(t0.uun[a], t0.frn[a][0], t0.frn[a][1], t0.frn[a][2]);
+            double4 *r4 = reinterpret_cast<double4 *>(c.arec) + 2 * (size_t)(f + a);
+            r4[0] = make_double4(m.xn[a], m.yn[a], m.zn[a], m.q[a]);
+        }
     }
     st->e_kc = e_kc + t0.duc;
     st->e_klj = e_klj + t0.dulj;
@@ -2127,8 +2164,15 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
     // FP32 copy of the positions for the real-space scan: in LDS when the
     // launch gives room for it, else in the context's pos4 buffer
     float4 *const spos = use_lds ? lds_pos : c.pos4;
-    for (int i = tid; i < p.natom; i += (int)blockDim.x)
+    for (int i = tid; i < p.natom; i += (int)blockDim.x) {
         spos[i] = make_float4((float)c.x[i], (float)c.y[i], (float)c.z[i], __int_as_float(c.molnum[i]));
+        // the launch copies: one record per atom for the exact pass, the
+        // per-atom real-space arrays as (uu, F) quadruples
+        double4 *r4 = reinterpret_cast<double4 *>(c.arec) + 2 * (size_t)i;
+        r4[0] = make_double4(c.x[i], c.y[i], c.z[i], c.q[i]);
+        r4[1] = make_double4(c.dh[i], (double)c.type[i], 0.0, 0.0);
+        reinterpret_cast<double4 *>(c.uf)[i] = make_double4(c.uu[i], c.Fx[i], c.Fy[i], c.Fz[i]);
+    }
     if (c.nc > 0) yk_cells_build(c, &sh.cell_ok);  // workgroup-uniform; ends with a barrier
     YkScalar &t0 = sh.t0;
     // mu = sum_j q_j r_j from scratch once per launch (EwaldSumExternal.f90:
@@ -2167,8 +2211,9 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
             int a = tid, g = m.first + a;
             m.q[a] = c.q[g]; m.typ[a] = c.type[g]; m.mass[a] = c.mass[g]; m.dh[a] = c.dh[g];
             m.xo[a] = c.x[g]; m.yo[a] = c.y[g]; m.zo[a] = c.z[g];
-            sh.uu_old[a] = c.uu[g];
-            sh.fr_old[a][0] = c.Fx[g]; sh.fr_old[a][1] = c.Fy[g]; sh.fr_old[a][2] = c.Fz[g];
+            const double4 u = reinterpret_cast<const double4 *>(c.uf)[g];
+            sh.uu_old[a] = u.x;
+            sh.fr_old[a][0] = u.y; sh.fr_old[a][1] = u.z; sh.fr_old[a][2] = u.w;
         }
         __syncthreads();
         if (tid == 0) yk_sec_slots(sh.m);
@@ -2247,7 +2292,7 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
             }
             __syncthreads();
             if (sh.nbcount <= YK_NBCAP) {
-                yl_rs_exact_n(c, sh.m, sig6, feps, sh.nblist, sh.nbcount, acc, ov, tid, blockDim.x);
+                yl_rs_exact_n<true>(c, sh.m, sig6, feps, sh.nblist, sh.nbcount, acc, ov, tid, blockDim.x);
             } else {  // list overflow: full FP64 pass (the commit then visits every atom)
                 int junk = 0;
                 double a2[YK_NRS];
@@ -2309,10 +2354,10 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         // ---- commit (MC.f90:1242-1318) ----------------------------------
         yk_commit<F32>(c, sh.m, tid, blockDim.x);
         if (sh.nbcount <= YK_NBCAP) {
-            yk_rs_commit_list(c, sh.nblist, sh.nbcount, tid, blockDim.x);
+            yk_rs_commit_list<true>(c, sh.nblist, sh.nbcount, tid, blockDim.x);
         } else {
             double acc[3] = {0, 0, 0};
-            yk_rs_commit_all(c, sh.m, sig6, feps, acc, tid, blockDim.x);
+            yk_rs_commit_all<true>(c, sh.m, sig6, feps, acc, tid, blockDim.x);
             yd_block_sum<3>(acc, sh.red, sh.out);
             if (tid == 0)
                 for (int i = 0; i < 3; ++i) t0.cmp_old[i] = sh.out[i];
@@ -2320,12 +2365,18 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         __syncthreads();
         YK_TICK(7);
         if (tid == 0) {
-            yk_sec_finalize(c, sh.m, t0, spos);
+            yk_sec_finalize(c, sh.m, t0, spos, true);
             if (c.nc > 0 && sh.cell_ok) yk_cells_move(c, sh.m, &sh.cell_ok);
             for (int d = 0; d < 3; ++d) t0.mu[d] = t0.mu_new[d];
         }
         __syncthreads();
         YK_TICK(8);
+    }
+    // the launch copy of the per-atom real-space arrays back into the state
+    __syncthreads();
+    for (int i = tid; i < p.natom; i += (int)blockDim.x) {
+        const double4 u = reinterpret_cast<const double4 *>(c.uf)[i];
+        c.uu[i] = u.x; c.Fx[i] = u.y; c.Fy[i] = u.z; c.Fz[i] = u.w;
     }
     if (tid == 0) {
         if (mode != 1) c.st->ctr = t0.rng.ctr;
