@@ -685,22 +685,23 @@ template <bool F32, int BS>
 static int launch_chain_t(yamc_ctx *c, const YdChain *d_chain, const double *const *sig6s,
                           const double *const *feps, int nchains, int nsteps, int mode, int max_natom,
                           int max_moves) {
-    // The FP32 position copy of the real-space scan goes into dynamic LDS when
-    // it fits next to the kernel's static LDS.
+    // The packed position copy of the real-space scan (4 bytes per atom) goes
+    // into dynamic LDS when it fits next to the kernel's static LDS: within
+    // half of a CU's LDS when two chains share a CU (more chains than CUs at
+    // 256 threads), else within all of it; otherwise it stays in HBM.
     int lds_max = 0;
     HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
     const void *fn = (const void *)yk_chain<F32, BS>;
     hipFuncAttributes fa;
     HIPCHK(hipFuncGetAttributes(&fa, fn));
-    size_t want = (size_t)max_natom * 16;
+    size_t want = (size_t)max_natom * 4;
     int lds_atoms = 0;
     size_t dyn = 0;
-    // More chains than CUs at 256 threads: leave the position copy in HBM so
-    // that two chains fit on a CU.
     int ncu = 0;
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
     const bool two_per_cu = BS == 256 && nchains > ncu;
-    if (!two_per_cu && want + fa.sharedSizeBytes + 1024 <= (size_t)lds_max) {
+    const size_t room = two_per_cu ? (size_t)lds_max / 2 : (size_t)lds_max;
+    if (want + fa.sharedSizeBytes + 1024 <= room) {
         lds_atoms = max_natom;
         dyn = want;
         HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));

@@ -917,6 +917,32 @@ __device__ void yk_rs_commit_all(const YdChain &c, const YkMol &m, const double 
 //     site change, MC.f90:1251-1275).
 // Returns false when the list overflowed (the caller then uses the full
 // FP64 pass yk_rs_move_n and the full commit).
+// Packed position of the workgroup kernel's scan: the fractional part of
+// each coordinate (box units) in 10 bits, x | y << 10 | z << 20, one 32-bit
+// word per atom (so the copy fits in LDS next to two chains' workgroup data).
+// A packed coordinate stands for the centre of its 1/1024 bin, within
+// 0.5/1024 of the position, so the packed position is within sqrt(3)/2048
+// box units of the true one (modulo the box).
+__device__ __forceinline__ unsigned int yk_pack_pos(double x, double y, double z) {
+    auto bin = [](double v) {
+        const int i = (int)((v - floor(v)) * 1024.0);
+        return (unsigned int)(i < 0 ? 0 : i > 1023 ? 1023 : i);
+    };
+    return bin(x) | (bin(y) << 10) | (bin(z) << 20);
+}
+__device__ __forceinline__ float4 yk_unpack_pos(unsigned int v) {
+    const float s = 1.0f / 1024.0f;
+    return make_float4(((float)(v & 1023u) + 0.5f) * s, ((float)((v >> 10) & 1023u) + 0.5f) * s,
+                       ((float)(v >> 20) + 0.5f) * s, 0.0f);
+}
+// r^2 bound of the scan on packed positions: the cut-off grown by the
+// packing error (8.5e-4 > sqrt(3)/2048 box units), squared, plus 1e-5 box^2
+// for FP32 rounding.
+__device__ __forceinline__ float yk_scan_r2(const YdChain &c) {
+    const float r = sqrtf((float)c.p.rmax2) + 8.5e-4f;
+    return r * r + 1e-5f;
+}
+
 // FP32 distance test of one atom against the molecule's old and new sites
 // (margin 1e-5 box^2 on r^2, far above FP32 rounding).  The image is
 // d - rint(d), the nearest one: it equals the pair sums' single-wrap image
@@ -947,9 +973,10 @@ __device__ __forceinline__ bool yk_near(const float (&xof)[NA], const float (&yo
 // are not within one cell of each other).  cl_* are the workgroup's LDS
 // arrays; all threads call this.
 template <int NA>
-__device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const float4 *pos, unsigned int *bm, int *cl_cell,
+__device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int *cl_cell,
                                  int *cl_start, int *ncl_out) {
-    const int nc = c.nc, me = m.imol + 1, tid = threadIdx.x;
+    const int nc = c.nc, tid = threadIdx.x;
+    const unsigned int f0 = (unsigned int)m.first, na = (unsigned int)m.na;
     int start[3], len[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -992,7 +1019,7 @@ __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const float4 
     }
     __syncthreads();
     const int total = cl_start[ncl];
-    const float r2f = (float)c.p.rmax2 + 1e-5f;
+    const float r2f = yk_scan_r2(c);
     float xof[NA], yof[NA], zof[NA], xnf[NA], ynf[NA], znf[NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
@@ -1016,25 +1043,28 @@ __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const float4 
                 jj[u] = c.cell_atoms[(size_t)cl_cell[lo] * c.ccap + (t - cl_start[lo])];
             }
         }
-        float4 pj[YK_SCANU];
+        unsigned int pj[YK_SCANU];
 #pragma unroll
-        for (int u = 0; u < YK_SCANU; ++u)
-            pj[u] = jj[u] >= 0 ? pos[jj[u]] : make_float4(0.f, 0.f, 0.f, __int_as_float(me));
+        for (int u = 0; u < YK_SCANU; ++u) pj[u] = jj[u] >= 0 ? pos[jj[u]] : 0u;
 #pragma unroll
         for (int u = 0; u < YK_SCANU; ++u) {
             const int j = jj[u];
-            if (__float_as_int(pj[u].w) != me && yk_near<NA>(xof, yof, zof, xnf, ynf, znf, pj[u], r2f))
+            if (j >= 0 && (unsigned int)j - f0 >= na &&
+                yk_near<NA>(xof, yof, zof, xnf, ynf, znf, yk_unpack_pos(pj[u]), r2f))
                 atomicOr(&bm[j >> 5], 1u << (j & 31));
         }
     }
     return true;
 }
 
+// pos: packed positions (yk_pack_pos); the molecule's own atoms, indices
+// first .. first + na - 1, are skipped.
 template <int NA>
-__device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const float4 *pos, unsigned int *bm) {
+__device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm) {
     const int N = c.p.natom, me = m.imol + 1;
     if (pos) {
-        const float r2f = (float)c.p.rmax2 + 1e-5f;
+        const float r2f = yk_scan_r2(c);
+        const unsigned int f0 = (unsigned int)m.first, na = (unsigned int)m.na;
         float xof[NA], yof[NA], zof[NA], xnf[NA], ynf[NA], znf[NA];
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
@@ -1045,16 +1075,17 @@ __device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const float4 *pos, 
         // that many loads are in flight at once when pos is in HBM
         const int bs = (int)blockDim.x;
         for (int j0 = threadIdx.x; j0 < N; j0 += YK_SCANU * bs) {
-            float4 pj[YK_SCANU];
+            unsigned int pj[YK_SCANU];
 #pragma unroll
             for (int u = 0; u < YK_SCANU; ++u) {
                 const int j = j0 + u * bs;
-                pj[u] = j < N ? pos[j] : make_float4(0.f, 0.f, 0.f, __int_as_float(me));
+                pj[u] = j < N ? pos[j] : 0u;
             }
 #pragma unroll
             for (int u = 0; u < YK_SCANU; ++u) {
                 const int j = j0 + u * bs;
-                if (__float_as_int(pj[u].w) != me && yk_near<NA>(xof, yof, zof, xnf, ynf, znf, pj[u], r2f))
+                if (j < N && (unsigned int)j - f0 >= na &&
+                    yk_near<NA>(xof, yof, zof, xnf, ynf, znf, yk_unpack_pos(pj[u]), r2f))
                     atomicOr(&bm[j >> 5], 1u << (j & 31));
             }
         }
@@ -1178,7 +1209,7 @@ __device__ void yk_rs_commit_list(const YdChain &c, const int *list, int count, 
     }
 }
 
-__device__ __forceinline__ void yk_rs_scan_n(const YdChain &c, const YkMol &m, const float4 *pos,
+__device__ __forceinline__ void yk_rs_scan_n(const YdChain &c, const YkMol &m, const unsigned int *pos,
                                              unsigned int *bm) {
     switch (m.na) {
         case 1: yk_rs_scan<1>(c, m, pos, bm); break;
@@ -1187,7 +1218,7 @@ __device__ __forceinline__ void yk_rs_scan_n(const YdChain &c, const YkMol &m, c
         default: yk_rs_scan<4>(c, m, pos, bm); break;
     }
 }
-__device__ __forceinline__ bool yk_rs_scan_cells_n(const YdChain &c, const YkMol &m, const float4 *pos,
+__device__ __forceinline__ bool yk_rs_scan_cells_n(const YdChain &c, const YkMol &m, const unsigned int *pos,
                                                    unsigned int *bm, int *cl_cell, int *cl_start, int *ncl) {
     switch (__builtin_amdgcn_readfirstlane(m.na)) {
         case 1: return yk_rs_scan_cells<1>(c, m, pos, bm, cl_cell, cl_start, ncl);
@@ -2113,7 +2144,7 @@ __device__ __forceinline__ void yk_sec_reject_rec(const YdChain &c, const YkMol 
 // Commit of the moved molecule's own per-atom arrays, positions and the
 // energy components (MC.f90:1242-1318); single lane.
 __device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m, YkScalar &t0, float4 *pos,
-                                                bool aos = false) {
+                                                bool aos = false, unsigned int *posq = nullptr) {
     int f = m.first;
     YdState *st = c.st;
     // the state's sums are read before the per-atom stores, which the
@@ -2129,6 +2160,7 @@ __device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m
         c.y[f + a] = m.yn[a];
         c.z[f + a] = m.zn[a];
         if (pos) pos[f + a] = make_float4((float)m.xn[a], (float)m.yn[a], (float)m.zn[a], __int_as_float(m.imol + 1));
+        if (posq) posq[f + a] = yk_pack_pos(m.xn[a], m.yn[a], m.zn[a]);
         if (aos) {  // the chain kernel's launch copies
             reinterpret_cast<double4 *>(c.uf)[f + a] = make_double4(t0.uun[a], t0.frn[a][0], t0.frn[a][1], t0.frn[a][2]);
             double4 *r4 = reinterpret_cast<double4 *>(c.arec) + 2 * (size_t)(f + a);
@@ -2155,17 +2187,17 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
 yk_chain(const YdChain *chains, const double *const *sig6s, const double *const *feps_s, int nsteps, int mode,
          int max_moves, int lds_atoms) {
     __shared__ YkShared sh;
-    extern __shared__ float4 lds_pos[];  // FP32 positions when the launch provides LDS for them
+    extern __shared__ unsigned int lds_pos[];  // packed positions when the launch provides LDS for them
     const YdChain &c = chains[blockIdx.x];
     const double *sig6 = sig6s[blockIdx.x], *feps = feps_s[blockIdx.x];
     const int tid = threadIdx.x;
     const YdParams &p = c.p;
     const bool use_lds = lds_atoms >= p.natom;
-    // FP32 copy of the positions for the real-space scan: in LDS when the
-    // launch gives room for it, else in the context's pos4 buffer
-    float4 *const spos = use_lds ? lds_pos : c.pos4;
+    // packed copy of the positions for the real-space scan (yk_pack_pos): in
+    // LDS when the launch gives room for it, else in the context's pos4 buffer
+    unsigned int *const spos = use_lds ? lds_pos : reinterpret_cast<unsigned int *>(c.pos4);
     for (int i = tid; i < p.natom; i += (int)blockDim.x) {
-        spos[i] = make_float4((float)c.x[i], (float)c.y[i], (float)c.z[i], __int_as_float(c.molnum[i]));
+        spos[i] = yk_pack_pos(c.x[i], c.y[i], c.z[i]);
         // the launch copies: one record per atom for the exact pass, the
         // per-atom real-space arrays as (uu, F) quadruples
         double4 *r4 = reinterpret_cast<double4 *>(c.arec) + 2 * (size_t)i;
@@ -2365,7 +2397,7 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         __syncthreads();
         YK_TICK(7);
         if (tid == 0) {
-            yk_sec_finalize(c, sh.m, t0, spos, true);
+            yk_sec_finalize(c, sh.m, t0, nullptr, true, spos);
             if (c.nc > 0 && sh.cell_ok) yk_cells_move(c, sh.m, &sh.cell_ok);
             for (int d = 0; d < 3; ++d) t0.mu[d] = t0.mu_new[d];
         }
