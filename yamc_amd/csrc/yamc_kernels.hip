@@ -30,7 +30,7 @@
 #define YK_BMWORDS 4096  // 131072 atoms
 #define YK_MAXCL 125     // cells visited by one cell-list scan (5 per axis)
 // Positions each thread loads ahead in the real-space scan (independent
-// loads in flight per thread when the FP32 positions are read from HBM).
+// loads in flight per thread when the packed positions are read from HBM).
 #define YK_SCANU 8
 
 // ------------------------------------------------------------------------
@@ -905,18 +905,18 @@ __device__ void yk_rs_commit_all(const YdChain &c, const YkMol &m, const double 
 
 // Real-space pass of a trial move in two stages.
 //  1. Scan: every other atom j is tested against the molecule's old and new
-//     sites; survivors are appended to the LDS candidate list.  With an
-//     LDS-resident FP32 copy of the positions (x, y, z, molecule number as
-//     bits in w) the test is in FP32 with a margin of 1e-5 box units^2 on
-//     r^2, far above the FP32 rounding of r^2 (about 1e-6 for |d| <= 1.5), so
-//     no atom the FP64 test accepts is lost; without it the test is FP64.
+//     sites; survivors are marked in an LDS bitmap and compacted into the
+//     candidate list.  The workgroup kernel tests packed positions (below)
+//     in FP32 against a bound grown by the packing error, the wave kernel
+//     float4 positions with a margin of 1e-5 box units^2 on r^2; either way
+//     no atom the FP64 test accepts is lost.
 //  2. Exact pass over the compacted list: the FP64 pair terms of the new
 //     sites (EwaldSumRealSpace.f90:338-498, partial mode) and the overlap
 //     test, with every lane busy.  The same list then drives the commit of
 //     the per-atom arrays (only atoms within the cut-off of an old or new
-//     site change, MC.f90:1251-1275).
-// Returns false when the list overflowed (the caller then uses the full
-// FP64 pass yk_rs_move_n and the full commit).
+//     site change, MC.f90:1251-1275).  When the list overflows the caller
+//     uses the full FP64 pass yk_rs_move_n and the full commit.
+
 // Packed position of the workgroup kernel's scan: the fractional part of
 // each coordinate (box units) in 10 bits, x | y << 10 | z << 20, one 32-bit
 // word per atom (so the copy fits in LDS next to two chains' workgroup data).
