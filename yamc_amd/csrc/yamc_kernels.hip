@@ -2181,7 +2181,7 @@ __device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m
 // with the Philox stream, nsteps = cycles; mode 1: nsteps moves with
 // caller-given molecule and constant uniform per move.
 // Both shapes are built for two waves per SIMD (the 256-thread form so that
-// two chains share a CU when the launch leaves the positions in HBM).
+// two chains share a CU when there are more chains than CUs).
 template <bool F32, int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
 yk_chain(const YdChain *chains, const double *const *sig6s, const double *const *feps_s, int nsteps, int mode,
