@@ -18,7 +18,8 @@ Forms timed:
 The per-move dE and force errors of fp32c against fp64 are measured on one
 stream (moves up to the first differing decision).  The CPU baseline is the
 reference's own Fortran compiled from its sources (oracle/_ref, when built;
-else the plain-C oracle), timed on this host.
+else the plain-C oracle), timed on this host on one core, and, as
+cpu_baseline_all_cores, as one copy per core (up to 16) at once.
 
 Prints one JSON line.
 """
@@ -272,6 +273,7 @@ def main():
 
     # --------------------------------------------------------- CPU baseline
     cpu = None
+    cpu_all = None
     if not args.no_cpu:
         from oracle import pyoracle, refcase
         have_ref = refcase.available()
@@ -302,6 +304,12 @@ def main():
         if have_ref:
             try:
                 ref = time_reference(base, pyoracle, refcase)
+                if "all_cores" in ref:
+                    ac = ref["all_cores"]
+                    cpu_all = dict(value=ac["moves_per_s"], unit="trial moves/s", cores=ac["cores"], kind="reference",
+                                   sample=f"{ac['cores']} copies of the reference's Fortran driver at once, one per "
+                                          f"core, each {ref['moves']} movemc trial moves of the same 6224-atom "
+                                          f"workload (independent replicas), aggregate rate")
                 cpu = dict(value=ref["moves_per_s"], unit="trial moves/s", cores=1, kind="reference",
                            sample=f"{ref['moves']} movemc trial moves of the same 6224-atom workload by the "
                                   f"reference's own Fortran (compiled from its sources with amdflang -O2, "
@@ -348,6 +356,8 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if cpu_all is not None:
+        out["cpu_baseline_all_cores"] = cpu_all
     # atom x k-vector evaluations: each trial move evaluates the moved molecule's
     # atoms at every nonzero sign slot, at the old and at the new positions
     # (2 n_m n_s); n_m averaged over the uniformly picked molecules
@@ -594,19 +604,44 @@ def time_reference(sysm, pyoracle, refcase, seconds=12.0):
     sysm = ys.build_system(dataclasses.replace(sysm.params, lambda_f=0.0, lambda_c=0.0), sysm.species, sysm.counts,
                            sysm.mol_type, sysm.mol_pos, sysm.mol_angles)
 
-    def timed(nmoves, seed):
-        imol, u = refcase.safe_moves(sysm, nmoves, seed, pyoracle, (0.3, 0.45), True) if nmoves else ([], [])
+    def moves(nmoves, seed):
+        return refcase.safe_moves(sysm, nmoves, seed, pyoracle, (0.3, 0.45), True) if nmoves else ([], [])
+
+    def timed(imol, u):
         with tempfile.TemporaryDirectory() as td:
             refcase.write_case(td, sysm, imol, u)
             _, dt = refcase.run(td, timeout=300)
         return dt
 
-    t_setup = timed(0, 0)
-    t_probe = timed(100, 1)
+    t_setup = timed(*moves(0, 0))
+    t_probe = timed(*moves(100, 1))
     per = max((t_probe - t_setup) / 100, 1e-6)
     m = int(min(20000, max(200, seconds / per)))
-    t_m = timed(m, 2)
-    return {"moves": m, "moves_per_s": m / max(t_m - t_setup, 1e-9), "setup_s": t_setup}
+    imol, u = moves(m, 2)
+    t_m = timed(imol, u)
+    out = {"moves": m, "moves_per_s": m / max(t_m - t_setup, 1e-9), "setup_s": t_setup}
+
+    # All cores: P copies of the same driver run at once, one per core (at
+    # most 16, a GPU host's share), each an independent replica of the same
+    # sample; aggregate rate = P m / (wall time - set-up time).
+    import subprocess
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    P = max(1, min(16, ncpu))
+    with tempfile.TemporaryDirectory() as td:
+        dirs = []
+        for i in range(P):
+            d = os.path.join(td, f"r{i}")
+            os.makedirs(d)
+            refcase.write_case(d, sysm, imol, u)
+            dirs.append(d)
+        t0 = time.perf_counter()
+        procs = [subprocess.Popen([refcase.DRIVER, "case.in"], cwd=d, stdout=subprocess.DEVNULL,
+                                  stderr=subprocess.PIPE) for d in dirs]
+        rcs = [pr.wait(timeout=600) for pr in procs]
+        wall = time.perf_counter() - t0
+    if all(rc == 0 for rc in rcs):
+        out["all_cores"] = {"cores": P, "moves": P * m, "moves_per_s": P * m / max(wall - t_setup, 1e-9)}
+    return out
 
 
 def torch_sync():
