@@ -656,32 +656,40 @@ def test_runmc_driver_npt_files(lib, tmp_path):
     assert abs(e[-1, 1] - e[-1, 2] - e[-1, 3]) < 1e-9 * abs(e[-1, 1])
 
 
-@pytest.mark.parametrize("precision,cells", [(0, False), (1, False), (1, True)])
-def test_batch_two_chains_per_cu_equals_single(lib, precision, cells):
-    """More 256-thread chains than CUs: the launch keeps the scan's position
-    copy in HBM so that two chains share a CU; every replica still equals its
-    own run alone (also with the cell-list scan: 3 A cut-off, 4 cells per
-    axis)."""
+@pytest.mark.parametrize("precision,cells,big", [(0, False, False), (1, False, False), (1, True, False),
+                                                 (1, False, True)])
+def test_batch_two_chains_per_cu_equals_single(lib, precision, cells, big):
+    """More 256-thread chains than CUs, so two chains share a CU: every
+    replica still equals its own run alone, where it has a CU to itself (also
+    with the cell-list scan: 3 A cut-off, 4 cells per axis).  The scan's
+    packed position copy (4 bytes per atom) sits in LDS for the small
+    replicas; the big ones (3000 waters, 9000 atoms: 36 KB next to the
+    kernel's static LDS is more than half of a CU's 160 KB) read it from HBM
+    in the batch and from LDS alone."""
     import torch
     from yamc_amd import replica_system
     from yamc_amd.ewald import EwaldMC, batch_run
     ncu = torch.cuda.get_device_properties(0).multi_processor_count
     n = ncu + 8
     kw = dict(rc_angstrom=3.0, alpha_per_angstrom=1.0667, sk=3.0) if cells else {}
-    systems = [replica_system(64 if cells else 27 + (i % 5) * 4, 0.5 * (i % 3), 290.0 + i % 40, seed=100 + i, **kw)
-               for i in range(n)]
+    if big:
+        systems = [replica_system(3000, 0.0, 298.0, seed=100 + i) for i in range(n)]
+    else:
+        systems = [replica_system(64 if cells else 27 + (i % 5) * 4, 0.5 * (i % 3), 290.0 + i % 40, seed=100 + i,
+                                  **kw) for i in range(n)]
+    ncyc = 1 if big else 2
     a = [EwaldMC(s, precision=precision) for s in systems]
     for i, x in enumerate(a):
         x.set_block_size(256)
         x.mc_init()
         x.set_rng(23, i)
-    nacc, _ = batch_run(a, 2)
-    for i in (0, 1, n // 2, n - 1):
+    nacc, _ = batch_run(a, ncyc)
+    for i in ((0, n - 1) if big else (0, 1, n // 2, n - 1)):
         y = EwaldMC(systems[i], precision=precision)
         y.set_block_size(256)
         y.mc_init()
         y.set_rng(23, i)
-        m, _ = y.run(2)
+        m, _ = y.run(ncyc)
         assert m == nacc[i]
         for p, q in zip(a[i].positions(), y.positions()):
             assert np.array_equal(p, q)
