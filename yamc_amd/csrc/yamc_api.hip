@@ -19,7 +19,8 @@ __global__ void yk_beta_lj(YdChain c, const double *lj6, const double *lj12, con
                            const double *beta12);
 __global__ void yk_scale_beta(YdChain c, double s);
 __global__ void yk_atom_tables(YdChain c, const double *x, const double *y, const double *z, double *tab);
-__global__ void yk_full_sums(YdChain c, const double *tab);
+__global__ void yk_full_sums(YdChain c, const double *tab, int chunk, double *part);
+__global__ void yk_full_sums_reduce(YdChain c, const double *part, int ngroups);
 __global__ void yk_kenergy(YdChain c);
 __global__ void yk_mu(YdChain c, double *mu_out);
 __global__ void yk_tails(YdChain c, const double *lj6, const double *lj12, const int *counts);
@@ -383,7 +384,19 @@ static int total_energy(yamc_ctx *c) {
     hipStream_t s = c->stream;
     int N = c->natom, nk = c->nk, T = c->ntype;
     yk_atom_tables<<<(N + 255) / 256, 256, 0, s>>>(c->ch, c->ch.x, c->ch.y, c->ch.z, c->d_tab);
-    yk_full_sums<<<((1 + T) * nk + 127) / 128, 128, 0, s>>>(c->ch, c->d_tab);
+    // atom groups of at least 64 atoms, enough (set, k, group) threads to
+    // fill the GPU (about 2^19), at most 64 groups
+    const int nsk = (1 + T) * nk;
+    int ng = (1 << 19) / nsk;
+    ng = ng < 1 ? 1 : ng > 64 ? 64 : ng;
+    if (ng > N / 64) ng = N / 64 > 1 ? N / 64 : 1;
+    const int chunk = (N + ng - 1) / ng;
+    ng = (N + chunk - 1) / chunk;
+    double *part = nullptr;
+    HIPCHK(hipMallocAsync((void **)&part, sizeof(double) * 8 * (size_t)nsk * ng, s));
+    yk_full_sums<<<dim3((nsk + 127) / 128, ng), 128, 0, s>>>(c->ch, c->d_tab, chunk, part);
+    yk_full_sums_reduce<<<(8 * nsk + 255) / 256, 256, 0, s>>>(c->ch, part, ng);
+    HIPCHK(hipFreeAsync(part, s));
     yk_kenergy<<<1, 1024, 0, s>>>(c->ch);
     HIPCHK(hipGetLastError());
     return YAMC_OK;

@@ -147,17 +147,20 @@ __global__ void yk_atom_tables(YdChain c, const double *x, const double *y, cons
     yd_sincos_axis(z[i], c.p.kmax, t + 4 * YD_KP, t + 5 * YD_KP);
 }
 
-// RhoSquared_addAtoms (RhoSquared.f90:188-243) from scratch: one thread per
-// (set, k), atoms added in index order, as the reference accumulates them.
-// FP32C stores the FP64 result as a (hi, lo) pair.
-__global__ void yk_full_sums(YdChain c, const double *tab) {
+// RhoSquared_addAtoms (RhoSquared.f90:188-243) from scratch, in two steps:
+// one thread per (set, k) and atom group (blockIdx.y: atoms [y chunk,
+// (y + 1) chunk), added in index order) writes a partial sum to
+// part[group][set][8][nk]; yk_full_sums_reduce adds the groups in order
+// (the reference adds all atoms in index order: rounding-level difference).
+__global__ void yk_full_sums(YdChain c, const double *tab, int chunk, double *part) {
     int nk = c.p.nk, T = c.p.ntype;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (1 + T) * nk) return;
     int set = i / nk, kk = i % nk;
     int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+    const int a0 = blockIdx.y * chunk, a1 = a0 + chunk < c.p.natom ? a0 + chunk : c.p.natom;
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int a = 0; a < c.p.natom; ++a) {
+    for (int a = a0; a < a1; ++a) {
         double q = c.q[a];
         int typ = c.type[a];
         if (set == 0 ? q == 0.0 : typ != set) continue;
@@ -167,16 +170,26 @@ __global__ void yk_full_sums(YdChain c, const double *tab) {
 #pragma unroll
         for (int m = 0; m < 8; ++m) s[m] = s[m] + kr[m];
     }
-    size_t base = (size_t)set * 8 * nk + kk;
+    size_t base = ((size_t)blockIdx.y * (1 + T) + set) * 8 * nk + kk;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        if (c.p.precision == YAMC_FP32C) {
-            float hi = (float)s[m];
-            c.Shi[base + (size_t)m * nk] = hi;
-            c.Slo[base + (size_t)m * nk] = (float)(s[m] - (double)hi);
-        } else {
-            c.S[base + (size_t)m * nk] = s[m];
-        }
+    for (int m = 0; m < 8; ++m) part[base + (size_t)m * nk] = s[m];
+}
+
+// The atom groups' partial sums of yk_full_sums added in group order, one
+// thread per S(k) value (flat index = the S layout's), into S (FP32C: the
+// (hi, lo) split).
+__global__ void yk_full_sums_reduce(YdChain c, const double *part, int ngroups) {
+    const int n = (1 + c.p.ntype) * 8 * c.p.nk;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int g = 0; g < ngroups; ++g) s = s + part[(size_t)g * n + i];
+    if (c.p.precision == YAMC_FP32C) {
+        float hi = (float)s;
+        c.Shi[i] = hi;
+        c.Slo[i] = (float)(s - (double)hi);
+    } else {
+        c.S[i] = s;
     }
 }
 
