@@ -175,7 +175,8 @@ int yamc_mc_get_rng(yamc_ctx *ctx, uint64_t *counter);
  * see yamc_mc_set_frequencies).  recs (may be NULL) receives ncycles*nmol
  * records.
  * resync_every > 0 rebuilds S(k) from scratch in FP64 after that many moves
- * (FP32C form); 0 = never. */
+ * (FP32C form; with exchange steps set, at the end of the first whole cycle
+ * after that many moves); 0 = never. */
 int yamc_mc_run(yamc_ctx *ctx, int32_t ncycles, int32_t resync_every, yamc_move_rec *recs,
                 int64_t *n_accepted);
 /* movemc with a caller-given molecule and one constant uniform per move

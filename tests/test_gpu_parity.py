@@ -229,6 +229,38 @@ def test_graph_batch_replays_equal_eager_batch(lib, block, precision):
     g.close()
 
 
+def test_fp32c_resync_every_n_moves_keeps_the_stream(lib):
+    """FP32C with a from-scratch S(k) every 37 moves (not a divisor of the
+    cycle, so segments resume inside a cycle): the same molecules, draws and
+    decisions as the FP64 run of the same stream, the same RNG counter at the
+    end, and S(k) right after the run equal to a fresh rebuild (the last
+    resync is at most 36 moves old)."""
+    from yamc_amd import replica_system
+    from yamc_amd.ewald import EwaldMC
+    s = replica_system(40, 0.5, 300.0, seed=77)
+    a, b = EwaldMC(s, precision=0), EwaldMC(s, precision=1)
+    for x in (a, b):
+        x.mc_init()
+        x.set_rng(5, 3)
+    na, ra = a.run(3, records=True)
+    nb, rb = b.run(3, resync_every=37, records=True)
+    assert s.nmol % 37 != 0
+    assert na == nb
+    assert a.rng_counter() == b.rng_counter()
+    assert np.array_equal(ra["imol"], rb["imol"])
+    assert np.array_equal(ra["accepted"], rb["accepted"])
+    ok = ra["overlap"] == 0
+    scale = (np.abs(ra["du_real"]) + np.abs(ra["du_kc"]) + np.abs(ra["du_klj"]) + np.abs(ra["du_ext"]))[ok]
+    assert rel(rb["du_total"][ok], ra["du_total"][ok], scale) < 1e-6
+    for p, q in zip(a.positions(), b.positions()):
+        assert rel(p, q, 1.0) < 1e-9
+    s0 = b.sums()
+    b.resync()
+    assert rel(s0, b.sums(), np.abs(b.sums()).max()) < 1e-9
+    a.close()
+    b.close()
+
+
 def test_uneven_replicas_fp32c_state_stays_consistent(lib):
     """Batched uneven replicas (sizes, NaCl concentration, temperature), FP32C:
     after two sweeps every replica's incrementally updated energies agree with

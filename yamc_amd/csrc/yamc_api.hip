@@ -19,7 +19,7 @@ __global__ void yk_beta_lj(YdChain c, const double *lj6, const double *lj12, con
                            const double *beta12);
 __global__ void yk_scale_beta(YdChain c, double s);
 __global__ void yk_atom_tables(YdChain c, const double *x, const double *y, const double *z, double *tab);
-__global__ void yk_full_sums(YdChain c, const double *tab, int chunk, double *part);
+__global__ void yk_full_sums(YdChain c, const double *tab, int ngroups, double *part);
 __global__ void yk_full_sums_reduce(YdChain c, const double *part, int ngroups);
 __global__ void yk_kenergy(YdChain c);
 __global__ void yk_mu(YdChain c, double *mu_out);
@@ -243,6 +243,22 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
         TRY(dalloc(c, &h.rsb, (size_t)YD_NBCAP * 4));
         TRY(dalloc(c, &h.nbl, (size_t)YD_NBCAP)); TRY(dalloc(c, &h.pos4, (size_t)N));
         TRY(dalloc(c, &h.arec, (size_t)N * 8)); TRY(dalloc(c, &h.uf, (size_t)N * 4));
+        std::vector<int> soff(T + 2, 0), satoms;  // per-set atom lists of the from-scratch S(k)
+        for (int set = 0; set <= T; ++set) {
+            soff[set] = (int)satoms.size();
+            for (int i = 0; i < N; ++i)
+                if (set == 0 ? d->charge[i] != 0.0 : d->lj_type[i] == set) satoms.push_back(i);
+        }
+        soff[T + 1] = (int)satoms.size();
+        TRY(dalloc(c, &h.set_atoms, satoms.empty() ? (size_t)1 : satoms.size()));
+        TRY(dalloc(c, &h.set_off, (size_t)(T + 2)));
+        // on the context stream, after dalloc's clearing, and complete before
+        // the host vectors go
+        if (!satoms.empty())
+            HIPCHK(hipMemcpyAsync(h.set_atoms, satoms.data(), sizeof(int) * satoms.size(), hipMemcpyHostToDevice,
+                                  c->stream));
+        HIPCHK(hipMemcpyAsync(h.set_off, soff.data(), sizeof(int) * (T + 2), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
         TRY(dalloc(c, &h.freq_end, M)); TRY(dalloc(c, &h.freq_beg, M)); TRY(dalloc(c, &h.st, 1));
         // cell list of the real-space scan: cells of width >= rc (box units;
         // rmax2 is constant in box units, so volume steps keep nc); only
@@ -384,17 +400,16 @@ static int total_energy(yamc_ctx *c) {
     hipStream_t s = c->stream;
     int N = c->natom, nk = c->nk, T = c->ntype;
     yk_atom_tables<<<(N + 255) / 256, 256, 0, s>>>(c->ch, c->ch.x, c->ch.y, c->ch.z, c->d_tab);
-    // atom groups of at least 64 atoms, enough (set, k, group) threads to
-    // fill the GPU (about 2^19), at most 64 groups
+    // each set's atom list in groups (of at least 64 atoms for the largest
+    // list), enough (set, k, group) threads to fill the GPU (about 2^19), at
+    // most 64 groups
     const int nsk = (1 + T) * nk;
     int ng = (1 << 19) / nsk;
     ng = ng < 1 ? 1 : ng > 64 ? 64 : ng;
     if (ng > N / 64) ng = N / 64 > 1 ? N / 64 : 1;
-    const int chunk = (N + ng - 1) / ng;
-    ng = (N + chunk - 1) / chunk;
     double *part = nullptr;
     HIPCHK(hipMallocAsync((void **)&part, sizeof(double) * 8 * (size_t)nsk * ng, s));
-    yk_full_sums<<<dim3((nsk + 127) / 128, ng), 128, 0, s>>>(c->ch, c->d_tab, chunk, part);
+    yk_full_sums<<<dim3((nsk + 127) / 128, ng), 128, 0, s>>>(c->ch, c->d_tab, ng, part);
     yk_full_sums_reduce<<<(8 * nsk + 255) / 256, 256, 0, s>>>(c->ch, part, ng);
     HIPCHK(hipFreeAsync(part, s));
     yk_kenergy<<<1, 1024, 0, s>>>(c->ch);
@@ -432,6 +447,7 @@ extern "C" int yamc_kspace_init(yamc_ctx *c, double scale_beta, double *e_c, dou
 extern "C" int yamc_kspace_get_beta(yamc_ctx *c, double *beta, double *beta6, double *beta12, double *beta_lj) {
     if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
     if (!c->beta_ready) return fail(YAMC_ERROR_INITIALIZATION, "beta not initialised");
+    TRY(sync(c));  // the copies below run on the null stream; the context's stream is non-blocking
     size_t nk = c->nk;
     if (beta) HIPCHK(hipMemcpy(beta, c->ch.beta, sizeof(double) * nk, hipMemcpyDeviceToHost));
     if (beta6) HIPCHK(hipMemcpy(beta6, c->d_beta6, sizeof(double) * nk, hipMemcpyDeviceToHost));
@@ -443,6 +459,7 @@ extern "C" int yamc_kspace_get_beta(yamc_ctx *c, double *beta, double *beta6, do
 
 extern "C" int yamc_kspace_get_sums(yamc_ctx *c, double *sums) {
     if (!c || !sums) return fail(YAMC_ERROR_PARAMETER, "null argument");
+    TRY(sync(c));  // the copies below run on the null stream; the context's stream is non-blocking
     size_t n = (size_t)(1 + c->ntype) * 8 * c->nk;
     if (c->precision == YAMC_FP32C) {
         std::vector<float> hi(n), lo(n);
@@ -464,6 +481,7 @@ static int stage_mol(yamc_ctx *c, int n, const double *x, const double *y, const
 
 static int mol_size(yamc_ctx *c, int imol, int *na) {
     if (imol < 0 || imol >= c->nmol) return fail(YAMC_ERROR_PARAMETER, "molecule index %d out of range", imol);
+    TRY(sync(c));  // the copies below run on the null stream; the context's stream is non-blocking
     int f, l;
     HIPCHK(hipMemcpy(&f, c->ch.first + imol, sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(&l, c->ch.last + imol, sizeof(int), hipMemcpyDeviceToHost));
@@ -766,7 +784,8 @@ extern "C" int yamc_mc_run(yamc_ctx *c, int32_t ncycles, int32_t resync_every, y
     YdState st0;
     HIPCHK(hipMemcpyAsync(&st0, c->ch.st, sizeof st0, hipMemcpyDeviceToHost, c->stream));
     TRY(sync(c));
-    // Segments of whole cycles between FP64 resyncs of S(k).
+    // Without exchange steps the resyncs come every resync_every moves (below);
+    // with them, at the end of the first whole cycle after that many moves.
     int seg = ncycles;
     if (resync_every > 0) {
         seg = (resync_every + c->nmol - 1) / c->nmol;
@@ -776,17 +795,35 @@ extern "C" int yamc_mc_run(yamc_ctx *c, int32_t ncycles, int32_t resync_every, y
     size_t nrec = (size_t)ncycles * c->nmol;
     if (recs && nrec) HIPCHK(hipMallocAsync((void **)&d_rec, sizeof(yamc_move_rec) * nrec, c->stream));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    for (int done = 0; done < ncycles; done += seg) {
-        int n = ncycles - done < seg ? ncycles - done : seg;
-        if (c->pxch > 0) {
-            TRY(run_with_exchanges(c, done, n, d_rec));
-        } else {
+    if (resync_every > 0 && c->pxch <= 0) {
+        // segments of resync_every moves, each resumed where the last one
+        // stopped inside its cycle (mv_begin), with a resync between them
+        const long long total = (long long)ncycles * c->nmol;
+        for (long long done = 0; done < total;) {
+            const long long n = total - done < resync_every ? total - done : resync_every;
             YdChain ch = c->ch;
-            ch.rec = d_rec ? d_rec + (size_t)done * c->nmol : nullptr;
+            ch.rec = d_rec ? d_rec + done : nullptr;
+            ch.mv_begin = (int)(done % c->nmol);
             HIPCHK(hipMemcpyAsync(c->d_chain, &ch, sizeof ch, hipMemcpyHostToDevice, c->stream));
-            TRY(launch_chain(c, c->d_chain, c->d_sig6s, c->d_feps, 1, n, 0, c->natom));
+            const int nsteps = (int)((n + c->nmol - 1) / c->nmol) + 1;
+            TRY(launch_chain(c, c->d_chain, c->d_sig6s, c->d_feps, 1, nsteps, 0, c->natom, (int)n));
+            done += n;
+            if (done < total) TRY(total_energy(c));
         }
-        if (resync_every > 0 && done + n < ncycles) TRY(total_energy(c));
+        HIPCHK(hipMemcpyAsync(c->d_chain, &c->ch, sizeof(YdChain), hipMemcpyHostToDevice, c->stream));
+    } else {
+        for (int done = 0; done < ncycles; done += seg) {
+            int n = ncycles - done < seg ? ncycles - done : seg;
+            if (c->pxch > 0) {
+                TRY(run_with_exchanges(c, done, n, d_rec));
+            } else {
+                YdChain ch = c->ch;
+                ch.rec = d_rec ? d_rec + (size_t)done * c->nmol : nullptr;
+                HIPCHK(hipMemcpyAsync(c->d_chain, &ch, sizeof ch, hipMemcpyHostToDevice, c->stream));
+                TRY(launch_chain(c, c->d_chain, c->d_sig6s, c->d_feps, 1, n, 0, c->natom));
+            }
+            if (resync_every > 0 && done + n < ncycles) TRY(total_energy(c));
+        }
     }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     if (d_rec) {

@@ -77,6 +77,11 @@ struct YdChain {
     // per candidate; uf (uu, Fx, Fy, Fz per atom) carries the per-atom
     // real-space arrays through the launch and is written back at its end.
     double *arec, *uf;
+    // atoms of each S(k) set in index order, for the from-scratch sums:
+    // set_atoms[set_off[s] .. set_off[s + 1]) = the charged atoms (s = 0) or
+    // the atoms of LJ type s; fixed at creation (no step changes a type or
+    // a charge at an index)
+    int *set_atoms, *set_off;
     double *freq_beg, *freq_end;  // move intervals (BiasedRandom.f90:41-75)
     YdState *st;
     yamc_move_rec *rec;   // optional per-move records

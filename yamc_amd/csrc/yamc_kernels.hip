@@ -148,24 +148,26 @@ __global__ void yk_atom_tables(YdChain c, const double *x, const double *y, cons
 }
 
 // RhoSquared_addAtoms (RhoSquared.f90:188-243) from scratch, in two steps:
-// one thread per (set, k) and atom group (blockIdx.y: atoms [y chunk,
-// (y + 1) chunk), added in index order) writes a partial sum to
-// part[group][set][8][nk]; yk_full_sums_reduce adds the groups in order
-// (the reference adds all atoms in index order: rounding-level difference).
-__global__ void yk_full_sums(YdChain c, const double *tab, int chunk, double *part) {
+// one thread per (set, k) and atom group (blockIdx.y: the y-th of ngroups
+// equal parts of the set's atom list, atoms in index order) writes a partial
+// sum to part[group][set][8][nk]; yk_full_sums_reduce adds the groups in
+// order (the reference adds all atoms in index order: rounding-level
+// difference).  The set's list holds exactly the atoms the reference adds to
+// it (charged atoms, or atoms of the set's LJ type).
+__global__ void yk_full_sums(YdChain c, const double *tab, int ngroups, double *part) {
     int nk = c.p.nk, T = c.p.ntype;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (1 + T) * nk) return;
     int set = i / nk, kk = i % nk;
     int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
-    const int a0 = blockIdx.y * chunk, a1 = a0 + chunk < c.p.natom ? a0 + chunk : c.p.natom;
+    const int l0 = c.set_off[set], l1 = c.set_off[set + 1];
+    const int chunk = (l1 - l0 + ngroups - 1) / ngroups;
+    const int t0 = l0 + (int)blockIdx.y * chunk, t1 = t0 + chunk < l1 ? t0 + chunk : l1;
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int a = a0; a < a1; ++a) {
-        double q = c.q[a];
-        int typ = c.type[a];
-        if (set == 0 ? q == 0.0 : typ != set) continue;
+    for (int t = t0; t < t1; ++t) {
+        const int a = c.set_atoms[t];
         double krc[8], krl[8];
-        yd_kr_atom(tab + (size_t)a * 6 * YD_KP, q, typ, kx, ky, kz, krc, krl);
+        yd_kr_atom(tab + (size_t)a * 6 * YD_KP, c.q[a], c.type[a], kx, ky, kz, krc, krl);
         const double *kr = set == 0 ? krc : krl;
 #pragma unroll
         for (int m = 0; m < 8; ++m) s[m] = s[m] + kr[m];
