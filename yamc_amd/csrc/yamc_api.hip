@@ -53,8 +53,8 @@ __global__ void yk_rspace_pair(YdChain c, int imol1, int imol2, const double *si
 __global__ void yk_xchg_commit_rs(YdChain c, const double *bn1, const double *bo1, const double *bn2,
                                   const double *bo2, const double *mv, int f1, int n1, int f2, int n2);
 #define YK_NRS (4 * YD_MAXA + 3)  // yk_rspace_partial output: uu, F, 3 components; overlap flag after
-void yk_launch_forces(const YdChain &c, const double *sig6, const double *feps, const double *tab, double *g,
-                      double *r, hipStream_t s);
+int yk_launch_forces(const YdChain &c, const double *sig6, const double *feps, const double *tab, double *g,
+                     double *r, hipStream_t s);  // 0, or -1 when its scratch allocation failed
 void yk_launch_rdf(const YdChain &c, double *com, const int *species, int nspec, double dr, double rmax2, int nbins,
                    unsigned long long *hist, hipStream_t s);
 
@@ -1041,7 +1041,8 @@ extern "C" int yamc_forces(yamc_ctx *c, double *gx, double *gy, double *gz, doub
     HIPCHK(hipGetLastError());
     double mu[3];
     TRY(yamc_external_mu(c, mu));
-    yk_launch_forces(c->ch, c->d_sig6, c->d_4eps, c->d_tab, dg, dr, s);
+    if (yk_launch_forces(c->ch, c->d_sig6, c->d_4eps, c->d_tab, dg, dr, s) != 0)
+        return fail(YAMC_ERROR_DEVICE, "forces: scratch allocation failed");
     HIPCHK(hipGetLastError());
     std::vector<double> hg(3 * (size_t)N), hr(3 * (size_t)N);
     HIPCHK(hipMemcpyAsync(hg.data(), dg, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, s));
