@@ -25,7 +25,9 @@ __global__ void yk_kenergy(YdChain c);
 __global__ void yk_mu(YdChain c, double *mu_out);
 __global__ void yk_tails(YdChain c, const double *lj6, const double *lj12, const int *counts);
 __global__ void yk_scale_box(YdChain c, double sc);
-__global__ void yk_rspace_full(YdChain c, const double *lj_sig6, const double *lj_4eps, double *part, int *ov);
+__global__ void yk_rspace_full(YdChain c, const double *lj_sig6, const double *lj_4eps, int ngroups, double *part,
+                               double *pat, int *ov);
+__global__ void yk_rspace_full_atoms(YdChain c, const double *pat, int ngroups);
 __global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, double *comps_out);
 template <bool F32, int BS>
 __global__ void yk_chain(const YdChain *chains, const double *const *sig6s, const double *const *feps_s, int nsteps,
@@ -91,7 +93,7 @@ struct yamc_ctx {
     std::vector<void *> owned;
     double *d_lj6 = nullptr, *d_lj12 = nullptr, *d_sig6 = nullptr, *d_4eps = nullptr;
     int *d_counts = nullptr;
-    double *d_xk2 = nullptr, *d_beta6 = nullptr, *d_beta12 = nullptr, *d_tab = nullptr, *d_part = nullptr;
+    double *d_xk2 = nullptr, *d_beta6 = nullptr, *d_beta12 = nullptr, *d_tab = nullptr;
     int *d_ov = nullptr;
     double *d_io = nullptr;  // small staging buffer for single-call results (64 doubles)
     double *d_mol = nullptr; // staged molecule coordinates (3 * YD_MAXA)
@@ -283,7 +285,6 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
         TRY(dalloc(c, &c->d_counts, T > 0 ? T : 1));
         TRY(dalloc(c, &c->d_xk2, nk)); TRY(dalloc(c, &c->d_beta6, nk)); TRY(dalloc(c, &c->d_beta12, nk));
         TRY(dalloc(c, &c->d_tab, (size_t)N * 6 * YD_KP));
-        TRY(dalloc(c, &c->d_part, (size_t)((N + 255) / 256 + 1) * 6));
         TRY(dalloc(c, &c->d_ov, 1)); TRY(dalloc(c, &c->d_io, 64)); TRY(dalloc(c, &c->d_mol, 3 * YD_MAXA));
         TRY(dalloc(c, &c->d_chain, 1)); TRY(dalloc(c, &c->d_sig6s, 1)); TRY(dalloc(c, &c->d_feps, 1));
         // host staging
@@ -541,9 +542,19 @@ extern "C" int yamc_kspace_forces(yamc_ctx *c, int32_t first, int32_t n, const d
 static int rspace_full(yamc_ctx *c, double *comps, int32_t *overlap) {
     hipStream_t s = c->stream;
     int N = c->natom, nb = (N + 255) / 256;
+    // partner groups of at least 64 atoms, about 2^19 (atom, group) threads, at most 64
+    int ng = (1 << 19) / N;
+    ng = ng < 1 ? 1 : ng > 64 ? 64 : ng;
+    if (ng > N / 64) ng = N / 64 > 1 ? N / 64 : 1;
+    double *part = nullptr, *pat = nullptr;
+    HIPCHK(hipMallocAsync((void **)&part, sizeof(double) * 6 * (size_t)nb * ng, s));
+    HIPCHK(hipMallocAsync((void **)&pat, sizeof(double) * 4 * (size_t)N * ng, s));
     HIPCHK(hipMemsetAsync(c->d_ov, 0, sizeof(int), s));
-    yk_rspace_full<<<nb, 256, 0, s>>>(c->ch, c->d_sig6, c->d_4eps, c->d_part, c->d_ov);
-    yk_rspace_full_final<<<1, 256, 0, s>>>(c->ch, c->d_part, nb, c->d_io);
+    yk_rspace_full<<<dim3(nb, ng), 256, 0, s>>>(c->ch, c->d_sig6, c->d_4eps, ng, part, pat, c->d_ov);
+    yk_rspace_full_atoms<<<nb, 256, 0, s>>>(c->ch, pat, ng);
+    yk_rspace_full_final<<<1, 256, 0, s>>>(c->ch, part, nb * ng, c->d_io);
+    HIPCHK(hipFreeAsync(part, s));
+    HIPCHK(hipFreeAsync(pat, s));
     HIPCHK(hipGetLastError());
     double o[6];
     int ov = 0;

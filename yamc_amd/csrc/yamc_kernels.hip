@@ -281,20 +281,24 @@ __global__ void yk_tails(YdChain c, const double *lj6, const double *lj12, const
 }
 
 // Full real-space sum (EwaldSumRealSpace.f90:338-498, full mode): one thread
-// per atom i visits every j != i, so uu(i) and F(i) are complete without
-// atomics; the six energy components count pairs j > i only and are reduced
-// per workgroup into part[block][6].  ov is set on an LJ hard-core overlap.
+// per atom i and partner group (blockIdx.y of ngroups, a contiguous range of
+// j) visits the group's j != i and writes its partial uu(i), F(i) to
+// pat[group][4][N] (yk_rspace_full_atoms adds the groups in order, no
+// atomics); the six energy components count pairs j > i only and are reduced
+// per workgroup into part[group][block][6].  ov is set on an LJ hard-core
+// overlap.
 __global__ void __launch_bounds__(256) yk_rspace_full(YdChain c, const double *lj_sig6, const double *lj_4eps,
-                                                      double *part, int *ov) {
+                                                      int ngroups, double *part, double *pat, int *ov) {
     __shared__ double red[4 * 6], out[6];
     int N = c.p.natom, T = c.p.ntype;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int jc = (N + ngroups - 1) / ngroups, j0 = (int)blockIdx.y * jc, j1 = j0 + jc < N ? j0 + jc : N;
     double v[6] = {0, 0, 0, 0, 0, 0};
     if (i < N) {
         int im = c.molnum[i], it = c.type[i];
         double qi = c.q[i], dhi = c.dh[i], xi = c.x[i], yi = c.y[i], zi = c.z[i];
         double uu = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
-        for (int j = 0; j < N; ++j) {
+        for (int j = j0; j < j1; ++j) {
             if (j == i) continue;
             int jm = c.molnum[j], jt = c.type[j];
             double xij = yd_wrap(xi - c.x[j]), yij = yd_wrap(yi - c.y[j]), zij = yd_wrap(zi - c.z[j]);
@@ -323,13 +327,29 @@ __global__ void __launch_bounds__(256) yk_rspace_full(YdChain c, const double *l
                 fz += zij * pr.FR;
             }
         }
-        c.uu[i] = uu;
-        c.Fx[i] = fx;
-        c.Fy[i] = fy;
-        c.Fz[i] = fz;
+        double *pa = pat + (size_t)blockIdx.y * 4 * N + i;
+        pa[0] = uu;
+        pa[(size_t)N] = fx;
+        pa[2 * (size_t)N] = fy;
+        pa[3 * (size_t)N] = fz;
     }
     yd_block_sum<6>(v, red, out);
-    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = out[threadIdx.x];
+    if (threadIdx.x < 6) part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 6 + threadIdx.x] = out[threadIdx.x];
+}
+
+// uu(i), F(i) of the full real-space sum from the groups' partial sums, added
+// in group order.
+__global__ void yk_rspace_full_atoms(YdChain c, const double *pat, int ngroups) {
+    const int N = c.p.natom;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double v[4] = {0, 0, 0, 0};
+    for (int g = 0; g < ngroups; ++g)
+        for (int q = 0; q < 4; ++q) v[q] = v[q] + pat[((size_t)g * 4 + q) * N + i];
+    c.uu[i] = v[0];
+    c.Fx[i] = v[1];
+    c.Fy[i] = v[2];
+    c.Fz[i] = v[3];
 }
 
 __global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, double *comps_out) {
