@@ -2267,9 +2267,11 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
     if (max_moves > 0 && total > max_moves) total = max_moves;
     for (int mv = 0; mv < total; ++mv) {
         if (tid == 0) {
+            __builtin_amdgcn_s_setprio(3);
             yk_sec_pick(c, sh.m, t0.rng, mode, mv);
             sh.ov = 0;
             sh.nbcount = 0;
+            __builtin_amdgcn_s_setprio(0);
         }
         __syncthreads();
         if (sh.m.imol < 0) break;  // exchange step: the launch ends here and the host runs it
@@ -2303,6 +2305,11 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         // then the displacement on wave 0 and the rotation on wave 1 at
         // once (the rotation's draws start after the displacement's, which
         // wave 1 replays without the arithmetic), then the new positions
+        // The one- and two-wave sections of a move (proposal, acceptance,
+        // pick, finalize) run at raised wave priority: on a CU shared with
+        // another chain they are this chain's critical path, while the other
+        // chain's wide phases can wait a few cycles.
+        if (tid < 128) __builtin_amdgcn_s_setprio(3);
         if (tid < 64) {
             yk_sec_propose_pre<true>(c, sh.m, t0, sh.out, sh.fr_old, sh.uu_old);
             t0.mcm2 = t0.mcm;
@@ -2327,6 +2334,7 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
             int stall = yk_sec_propose_post(sh.m, t0);
             if (tid == 0) sh.stall = stall;
         }
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         YK_TICK(3);
         if (sh.stall) {  // proposal loop over its bound: rejected, no acceptance draw
@@ -2408,12 +2416,14 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         YK_TICK(5);
         // ---- acceptance on wave 0 (MC.f90:1176-1233) --------------------
         if (tid < 64) {
+            __builtin_amdgcn_s_setprio(3);
             int accd = yk_sec_accept<true>(c, sh.m, t0, sh.out, mv);
             if (tid == 0) {
                 sh.accepted = accd;
                 ++t0.nmv;
                 t0.acc_count += accd;
             }
+            __builtin_amdgcn_s_setprio(0);
         }
         __syncthreads();
         YK_TICK(6);
@@ -2432,9 +2442,11 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         __syncthreads();
         YK_TICK(7);
         if (tid == 0) {
+            __builtin_amdgcn_s_setprio(3);
             yk_sec_finalize(c, sh.m, t0, nullptr, true, spos);
             if (c.nc > 0 && sh.cell_ok) yk_cells_move(c, sh.m, &sh.cell_ok);
             for (int d = 0; d < 3; ++d) t0.mu[d] = t0.mu_new[d];
+            __builtin_amdgcn_s_setprio(0);
         }
         __syncthreads();
         YK_TICK(8);
