@@ -35,6 +35,62 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+RESYNC = 1000  # moves between from-scratch FP64 rebuilds of S(k) in the FP32C forms
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def batch_error(reps, base, kT_kjmol):
+    """Error of the FP32C batch against FP64 on the batch's own states:
+    for each replica after the timed sweeps, (i) its incrementally carried
+    energy components against a from-scratch FP64 recompute of the same
+    configuration, and (ii) one more sweep from that configuration run in
+    FP32C and in FP64 on the same RNG stream: per-move dE and force errors
+    over the moves before the first differing decision."""
+    import numpy as np
+
+    from yamc_amd import YAMC_FP32C, YAMC_FP64
+    from yamc_amd.ewald import EwaldMC
+    state, de, fr, nmv, agree = 0.0, 0.0, 0.0, 0, []
+    for i, r in enumerate(reps):
+        inc = r.energies()
+        x, y, z = r.positions()
+        L = r.box_length()
+        twins = []
+        for prec in (YAMC_FP64, YAMC_FP32C):
+            t = EwaldMC(base, precision=prec)
+            t.set_positions(x, y, z)
+            e, _ = t.mc_init()
+            t.set_rng(99, i)
+            twins.append((t, e))
+        e64 = twins[0][1]
+        state = max(state, max(abs(inc[k] - e64[k]) for k in e64))
+        _, ra = twins[0][0].run(1, records=True)
+        _, rb = twins[1][0].run(1, records=True)
+        same = ra["accepted"] == rb["accepted"]
+        n = int(np.argmin(same)) if not same.all() else len(ra)
+        ok = ra["overlap"][:n] == 0
+        if ok.any():
+            de = max(de, float(np.abs(ra["du_total"][:n] - rb["du_total"][:n])[ok].max()))
+            fa = np.concatenate([ra["f_old"][:n][ok], ra["f_new"][:n][ok]])
+            fb = np.concatenate([rb["f_old"][:n][ok], rb["f_new"][:n][ok]])
+            fr = max(fr, float((np.linalg.norm(fa - fb, axis=1) / np.maximum(np.linalg.norm(fa, axis=1), 1e-300)).max()))
+        nmv += n
+        agree.append(float(same.mean()))
+        for t, _ in twins:
+            t.close()
+        assert abs(L - base.params.box_length) < 1e-12
+    return dict(replicas_checked=len(reps), max_state_energy_err_kT=state,
+                max_dE_kJmol=de * kT_kjmol, max_dE_kT=de, max_rel_force_err_norm=fr, moves_compared=nmv,
+                decision_agreement=float(np.mean(agree)))
 
 
 def algorithmic_bytes_per_move(sysm, nk_info, w, p_acc):
@@ -146,7 +202,7 @@ def main():
         kms = 0.0
         nacc = 0
         for _ in range(args.steps):
-            n, _ = g.run(1, resync_every=1000 if prec == YAMC_FP32C else 0)
+            n, _ = g.run(1, resync_every=RESYNC if prec == YAMC_FP32C else 0)
             kms += g.last_kernel_ms()
             nacc += n
         sync()
@@ -183,6 +239,7 @@ def main():
     # ------------------------------------------------------ batched replicas
     grid = None
     tuned = None
+    batch_err = None
     if "batch" in forms:
         R = args.replicas
         reps = []
@@ -198,7 +255,9 @@ def main():
             shape, tuned = args.block, None
         else:
             shape, tuned = tune_batch(reps)  # three ordinary sweeps, one per launch shape
-        bat = Batch(reps, 1)
+        # one captured sweep per step; every replica's S(k) is rebuilt from
+        # scratch in FP64 every RESYNC moves inside the sweep (batched)
+        bat = Batch(reps, 1, resync_every=RESYNC)
         for _ in range(args.warmup):
             bat.launch()
         sync()
@@ -213,9 +272,10 @@ def main():
         dt = time.perf_counter() - t0
         moves = args.steps * nmol * R
         res["batch"] = dict(replicas=R, moves_per_s=moves / dt, ms_per_step=1e3 * dt / args.steps, kernel_ms=kms,
-                            acceptance=nacc / moves, launch_shape=shape,
+                            acceptance=nacc / moves, launch_shape=shape, resync_every=RESYNC,
                             tuner_moves_per_s={str(k): v for k, v in tuned.items()} if tuned else None)
         bat.close()
+        batch_err = batch_error(reps[:4], base, kT_kjmol)
         for r in reps:
             r.close()
 
@@ -298,7 +358,7 @@ def main():
             o2.run_const(im, np.full(nmv, 0.4))
             done = nmv
         dtc = time.perf_counter() - t0
-        port = dict(value=done / dtc, unit="trial moves/s", cores=1, kind="port",
+        port = dict(value=done / dtc, unit="trial moves/s", cores=1, kind="port", cpu_model=cpu_model(),
                     sample=f"{done} FP64 trial moves of the same 6224-atom workload, plain-C oracle, 1 thread")
         cpu = port
         if have_ref:
@@ -307,10 +367,12 @@ def main():
                 if "all_cores" in ref:
                     ac = ref["all_cores"]
                     cpu_all = dict(value=ac["moves_per_s"], unit="trial moves/s", cores=ac["cores"], kind="reference",
+                                   cpu_model=cpu_model(),
                                    sample=f"{ac['cores']} copies of the reference's Fortran driver at once, one per "
                                           f"core, each {ref['moves']} movemc trial moves of the same 6224-atom "
                                           f"workload (independent replicas), aggregate rate")
                 cpu = dict(value=ref["moves_per_s"], unit="trial moves/s", cores=1, kind="reference",
+                           cpu_model=cpu_model(),
                            sample=f"{ref['moves']} movemc trial moves of the same 6224-atom workload by the "
                                   f"reference's own Fortran (compiled from its sources with amdflang -O2, "
                                   f"RND_CONSTANT uniforms, unbiased proposals, set-up time subtracted), "
@@ -344,6 +406,7 @@ def main():
         },
         "forms": res,
         "dE_error_fp32c_vs_fp64": err,
+        "batch_error_fp32c_vs_fp64": batch_err,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -482,7 +545,7 @@ def replica_check(args):
         kmaxes.append(int(s.derived["kmax"]))
         temps.append(T)
     shape, tuned = tune_batch(reps)
-    bat = Batch(reps, 1)
+    bat = Batch(reps, 1, resync_every=RESYNC)
     bat.launch()
     torch_sync()
     moves_per_sweep = sum(r.sys.nmol for r in reps)

@@ -140,6 +140,12 @@ module yamc_ewald
          type(c_ptr), value :: ctx
          integer(c_int64_t), value :: seed, stream, counter
       end function
+      integer(c_int) function yamc_mc_set_rng_list(ctx, u, n) bind(C, name="yamc_mc_set_rng_list")
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: u(*)
+         integer(c_int64_t), value :: n
+      end function
       integer(c_int) function yamc_mc_get_rng(ctx, counter) bind(C, name="yamc_mc_get_rng")
          import :: c_int, c_ptr, c_int64_t
          type(c_ptr), value :: ctx
@@ -217,16 +223,18 @@ module yamc_ewald
          type(c_ptr), value :: ctx
          real(c_double), intent(out) :: box_length
       end function
-      integer(c_int) function yamc_batch_run(ctxs, n, ncycles, n_accepted) bind(C, name="yamc_batch_run")
+      integer(c_int) function yamc_batch_run(ctxs, n, ncycles, resync_every, n_accepted) &
+            bind(C, name="yamc_batch_run")
          import :: c_int, c_ptr, c_int32_t, c_int64_t
          type(c_ptr), intent(in) :: ctxs(*)
-         integer(c_int32_t), value :: n, ncycles
+         integer(c_int32_t), value :: n, ncycles, resync_every
          integer(c_int64_t), intent(out) :: n_accepted(*)
       end function
-      integer(c_int) function yamc_batch_create(ctxs, n, ncycles, batch) bind(C, name="yamc_batch_create")
+      integer(c_int) function yamc_batch_create(ctxs, n, ncycles, resync_every, batch) &
+            bind(C, name="yamc_batch_create")
          import :: c_int, c_ptr, c_int32_t
          type(c_ptr), intent(in) :: ctxs(*)
-         integer(c_int32_t), value :: n, ncycles
+         integer(c_int32_t), value :: n, ncycles, resync_every
          type(c_ptr), intent(out) :: batch
       end function
       integer(c_int) function yamc_batch_launch(batch, n_accepted) bind(C, name="yamc_batch_launch")

@@ -169,6 +169,13 @@ int yamc_mc_real_arrays(yamc_ctx *ctx, double *uu, double *fx, double *fy, doubl
  * word position = counter; 24-bit uniforms in (0,1) as random.f90 gives. */
 int yamc_mc_set_rng(yamc_ctx *ctx, uint64_t seed, uint64_t stream, uint64_t counter);
 int yamc_mc_get_rng(yamc_ctx *ctx, uint64_t *counter);
+/* Replay mode: every uniform the context draws from now on (rand(),
+ * random.f90:66-93, in the consumption order of run_mc, movemc, mcvol and
+ * mc_xchange) is the next of the n recorded values u[] (each in (0,1)), the
+ * counter being the position; a run that draws past the end fails with
+ * ERROR_LIMITS.  For replaying a stream recorded from the reference's own
+ * random_number; yamc_mc_set_rng returns to Philox.  n = 0 clears the list. */
+int yamc_mc_set_rng_list(yamc_ctx *ctx, const double *u, int64_t n);
 /* run_mc cycles (runmc.f90:204-268, no volume steps): per cycle one
  * pressure-step draw, then nmol steps of (exchange draw, then molecule pick
  * + movemc on the device, or an exchange step when exchange weights are set,
@@ -214,7 +221,9 @@ int yamc_mc_xchange(yamc_ctx *ctx, int32_t imol1, int32_t imol2, const double *u
  * (LJTypes_scale_sigma) and beta_LJ follow the new box, the whole state is
  * recomputed (mc_calc_total_sums with scale_beta = xclb_new / xclb_old) and
  * the step is accepted with min(1, exp(-dU - P dV + (nmol+1) ln(V_new/V_old)))
- * (no acceptance draw after an overlap or when that exceeds 1); a rejected
+ * (the acceptance draw is taken after every step, overlap included: the
+ * reference's `(proba > 1) .or. (rand() < proba)`, MC.f90:804, as compiled
+ * evaluates both operands); a rejected
  * step restores the old state.  pressure_kT_A3 = P / (k_B T) in kT per
  * cubic Angstrom (pressure_angstr_kT, MC.f90:721).  u (may be NULL) is a
  * constant uniform for both draws (RND_CONSTANT); NULL draws them from the
@@ -254,16 +263,23 @@ int yamc_mc_commit_move(yamc_ctx *ctx, int32_t imol, const double *x, const doub
 /* Advances n independent contexts (uneven sizes, compositions and
  * temperatures allowed) by ncycles run_mc cycles each (runmc.f90:204-268)
  * in one persistent launch; the launch shape is the first context's
- * (yamc_set_block_size).  All contexts must live on the same device.
+ * (yamc_set_block_size).  All contexts must live on the same device and
+ * have run yamc_mc_init.
+ * resync_every > 0: each replica's ncycles * nmol moves run in segments of
+ * resync_every moves with a from-scratch FP64 rebuild of its S(k) between
+ * them (all replicas' rebuilds in one batched pass), exactly the segments
+ * and rebuilds of yamc_mc_run(ctx, ncycles, resync_every) alone; 0 = none.
  * n_accepted[n] (may be NULL) receives each replica's accepted moves. */
-int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int64_t *n_accepted);
+int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int32_t resync_every, int64_t *n_accepted);
 /* The same sweep captured once in a HIP graph: yamc_batch_create stages the
- * replicas and captures one launch of ncycles cycles, yamc_batch_launch
- * replays it (each replay continues every replica's chain and RNG stream),
+ * replicas and captures one sweep of ncycles cycles (its segments and
+ * rebuilds included), yamc_batch_launch replays it (each replay continues
+ * every replica's chain and RNG stream, and first re-reads every member's
+ * current box parameters; a member whose state is not ready is refused),
  * yamc_batch_destroy frees the graph.  The contexts must outlive the batch;
  * no reference counterpart (the reference runs one system serially). */
 typedef struct yamc_batch yamc_batch;
-int yamc_batch_create(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, yamc_batch **out);
+int yamc_batch_create(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int32_t resync_every, yamc_batch **out);
 int yamc_batch_launch(yamc_batch *batch, int64_t *n_accepted);
 int yamc_batch_destroy(yamc_batch *batch);
 

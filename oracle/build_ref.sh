@@ -16,7 +16,7 @@ cd "$OUT"
 MODS="SystemSettings constants error string io matrix3x3 geometry random Functions parameters \
 Molecule MoleculeHandler composition AtomicData LJTypes FourierGrid SumSinCosKR RhoSquared \
 EwaldSumKSpace ForceKSpace RealSumLocal EwaldSumRealSpace EwaldSumTails EwaldSumExternal \
-MonteCarloMove MoleculeTable BiasedRandom scale_box MC FloatingPoint"
+MonteCarloMove MoleculeTable BiasedRandom scale_box MC FloatingPoint runmc"
 OBJS=""
 for m in $MODS; do
   if [ ! -f "$m.o" ] || [ "$REF/$m.f90" -nt "$m.o" ]; then

@@ -65,6 +65,7 @@ def load():
             getattr(_lib, f).argtypes = [C.c_void_p]
         _lib.yo_rng_philox.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64]
         _lib.yo_rng_const.argtypes = [C.c_void_p, C.c_double]
+        _lib.yo_rng_list.argtypes = [C.c_void_p, _DP, C.c_uint64]
         _lib.yo_kspace_init.argtypes = [C.c_void_p, C.c_double]
         _lib.yo_volume_move.argtypes = [C.c_void_p, C.c_double, C.c_double, _DP, C.POINTER(C.c_int)]
         _lib.yo_run_npt.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
@@ -213,6 +214,11 @@ class Oracle:
 
     def set_rng(self, seed, stream=0, counter=0):
         self.lib.yo_rng_philox(C.c_void_p(self.h), seed, stream, counter)
+
+    def set_rng_list(self, u):
+        """Replay mode: rand() returns the recorded uniforms u in order."""
+        uu, pu = _d(u)
+        self.lib.yo_rng_list(C.c_void_p(self.h), pu, len(uu))
 
     def rng_counter(self):
         return self.lib.yo_rng_counter(C.c_void_p(self.h))

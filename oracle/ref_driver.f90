@@ -25,6 +25,18 @@
 !   nvol lines: u pressure[Pa] max_volume_scaling   (mcvol steps)
 !   nxchg (optional)
 !   nxchg lines: imol1 imol2 u   (mc_xchange steps, 1-based molecules)
+!   ncycles seed psm pressure[Pa] max_volume_scaling nstream nfreq (optional)
+!   nfreq lines: first last move_weight xchange_weight (1-based molecules)
+!
+! The last section runs the reference's own run_mc (runmc.f90:178-334) with
+! its standard random mode (random.f90:25-93, the compiler's random_number):
+! after randomize(seed) it writes the first nstream uniforms rand() returns
+! (STREAM), re-seeds, and runs ncycles cycles, one run_mc call per cycle,
+! with the volume steps (pressure_step_multiplier psm) and, with nfreq > 0,
+! a frequency file (read_freq_file) that sets move and exchange weights.
+! After each cycle it writes the step counters, the box length and the
+! energies (CYCLE), then the next four uniforms (TAIL: they locate how many
+! uniforms the run consumed) and the final state.
 program ref_driver
    use SystemSettings, only : SYSTEM_STRING_LENGTH
    use parameters
@@ -39,6 +51,8 @@ program ref_driver
    use SumSinCosKR
    use FloatingPoint, only : write_float
    use io, only : io_open, io_close
+   use runmc, only : run_mc, ifirst, n_mv_steps, n_mv_accepted, n_vol_steps, n_vol_accepted, n_xchg_steps, &
+                     n_xchg_accepted
    implicit none
 
    character(SYSTEM_STRING_LENGTH) :: case_file, comp_file
@@ -50,6 +64,8 @@ program ref_driver
    Type(TSinCosKR) :: krc, krl
    integer :: nmol, i, j, natom, nk, t1, t2, nt, nmoves, imol, species, f, l, uin, nvol, ios, imol2, nxchg
    integer :: nb, hfile
+   integer :: nrun, rseed, nstream, nfreq, ff, fl
+   real(8) :: psm, wm, wx
    character(SYSTEM_STRING_LENGTH) :: fname
    real(8), dimension(10) :: specials = (/ 0.d0, 1.d0, -1.d0, 0.75d0, -3.5d0, 2.d0, 1.d-15, -1.d15, 123456.789d0, &
                                            -2.5d-7 /)
@@ -227,6 +243,58 @@ program ref_driver
       call dump_real('FZX', RealSpace_TotalSum % Fz, natom)
       call dump_sums(KSpace_TotalSum % rho_squared_total % sumsincos_coulomb, nk)
    end if
+   ! run_mc cycles with the reference's own random stream
+   read(uin, *, iostat=ios) nrun, rseed, psm, pvol, mvs, nstream, nfreq
+   if (ios /= 0) nrun = 0
+   if (nrun > 0) then
+      if (nfreq > 0) then
+         open(newunit=hfile, file='freq.txt', status='replace')
+         write(hfile, '(A)') '# intervals: first-last move_weight xchange_weight'
+         write(hfile, '(I6)') nfreq
+         do j = 1, nfreq
+            read(uin, *) ff, fl, wm, wx
+            ! fixed-point weights: read_freq_file turns every '-' of the line into a blank
+            write(hfile, '(I0,A,I0,2F18.10)') ff, '-', fl, wm, wx
+         end do
+         close(hfile)
+         freq_file = 'freq.txt'
+      else
+         freq_file = 'none'
+      end if
+      pressure = pvol
+      max_volume_scaling = mvs
+      pressure_step_multiplier = psm
+      n_store_traj_interval = 1.d9
+      n_store_energy_interval = 1.d9
+      call Parameters_recalc(BoxLength)
+      random_method = RND_STANDARD
+      call randomize(rseed)
+      write(*, '(/,A,I10)') 'STREAM', nstream
+      do j = 1, nstream
+         write(*, '(ES26.17)') rand()
+      end do
+      call randomize(rseed)
+      do j = 1, nrun
+         ifirst = j
+         call run_mc(mol_table, j)
+         write(*, '(/,A,I8,6F16.1,ES26.17)') 'CYCLE', j, n_mv_steps, n_mv_accepted, n_vol_steps, &
+                                             n_vol_accepted, n_xchg_steps, n_xchg_accepted, BoxLength
+         call dump_energies('AFTER')
+      end do
+      write(*, '(/,A)') 'TAIL'
+      do j = 1, 4
+         write(*, '(ES26.17)') rand()
+      end do
+      call dump_real('RUNXX', atomic_data % xx, natom)
+      call dump_real('RUNYY', atomic_data % yy, natom)
+      call dump_real('RUNZZ', atomic_data % zz, natom)
+      call dump_real('RUNUU', RealSpace_TotalSum % uu, natom)
+      call dump_real('RUNFX', RealSpace_TotalSum % Fx, natom)
+      call dump_real('RUNFY', RealSpace_TotalSum % Fy, natom)
+      call dump_real('RUNFZ', RealSpace_TotalSum % Fz, natom)
+      call dump_sums(KSpace_TotalSum % rho_squared_total % sumsincos_coulomb, nk)
+   end if
+
    ! formats: molecule table of the final configuration (positions and
    ! orientations from the atoms), its binary moltab at 1, 2 and 3 bytes per
    ! sample, and 24+8-bit floats of the real-space forces / BoxLength (the

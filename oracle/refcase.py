@@ -30,9 +30,12 @@ def write_mol(path, mol):
             f.write("%s %.15f %.15f %.15f %.6f %.6f %.6f %.4f %.4f\n" % r)
 
 
-def write_case(td, sysm, imol, u, vols=(), xchgs=()):
+def write_case(td, sysm, imol, u, vols=(), xchgs=(), runmc=None):
     """Writes the inputs of one driver run into directory td; imol is
-    0-based.  Returns the case file name."""
+    0-based.  runmc (optional): dict(ncycles, seed, psm, pressure [Pa],
+    mvs, nstream, freq=[(first, last, move_weight, xchange_weight), ...]
+    with 1-based molecule intervals, [] = the 'none' file) for the run_mc
+    section.  Returns the case file name."""
     p = sysm.params
     names = []
     for k, mol in enumerate(sysm.species):
@@ -60,6 +63,12 @@ def write_case(td, sysm, imol, u, vols=(), xchgs=()):
         f.write(f"{len(xchgs)}\n")
         for a, b, xu in xchgs:
             f.write("%d %d %.17g\n" % (a + 1, b + 1, xu))
+        if runmc:
+            fr = runmc.get("freq", [])
+            f.write("%d %d %.17g %.17g %.17g %d %d\n" % (runmc["ncycles"], runmc["seed"], runmc["psm"],
+                                                       runmc["pressure"], runmc["mvs"], runmc["nstream"], len(fr)))
+            for a, b, wm, wx in fr:
+                f.write("%d %d %.17g %.17g\n" % (a, b, wm, wx))
     return "case.in"
 
 

@@ -47,6 +47,8 @@ struct yo_sys {
     uint32_t key[2], stream[2];
     uint64_t ctr;
     double const_u;
+    double *ulist;   /* replay mode: the uniforms of a recorded stream */
+    uint64_t nlist;
     /* scratch */
     double *mol_old, *mol_new, *tot_new, *krc, *krl;
     double *back_new, *back_old; /* 4*natom each */
@@ -812,8 +814,24 @@ void yo_rng_const(yo_sys *s, double u) {
     s->const_u = u;
 }
 
+/* Replay mode: rand() returns the recorded uniforms u[0..n) in order (the
+ * counter is the position); past the end it returns 0.5 and the counter
+ * keeps counting, so the caller sees counter > n. */
+void yo_rng_list(yo_sys *s, const double *u, uint64_t n) {
+    free(s->ulist);
+    s->ulist = malloc(sizeof(double) * (n ? n : 1));
+    memcpy(s->ulist, u, sizeof(double) * n);
+    s->nlist = n;
+    s->ctr = 0;
+    s->rng_mode = YO_RNG_LIST;
+}
+
 double yo_rand(yo_sys *s) {
     if (s->rng_mode == YO_RNG_CONST) return s->const_u;
+    if (s->rng_mode == YO_RNG_LIST) {
+        uint64_t n = s->ctr++;
+        return n < s->nlist ? s->ulist[n] : 0.5;
+    }
     for (;;) {
         uint64_t n = s->ctr++;
         uint32_t c[4] = {(uint32_t)(n >> 2), (uint32_t)(n >> 34), s->stream[0], s->stream[1]}, o[4];
@@ -1460,8 +1478,12 @@ int yo_run_const(yo_sys *s, int nmoves, const int32_t *imol, const double *u, yo
  * rescaled from the current ones: xclb, kext and dr scale as 1/L, the LJ
  * tables as sigma^6 ~ L^-6 and sigma^12 ~ L^-12 (four_eps unchanged); alpha,
  * rmax and kmax are constant in box units.  pressure is in kT/A^3.
- * rand() order: lambda, then the acceptance draw only when proba <= 1
- * (the second operand of the reference's .or., MC.f90:804). */
+ * rand() order: lambda, then the acceptance draw, always: the reference's
+ * test `(proba > 1) .or. (rand() < proba)` (MC.f90:804) leaves it to the
+ * compiler whether rand() runs when proba > 1 (Fortran does not fix the
+ * evaluation of .or. operands); the reference as compiled here evaluates
+ * both operands, and its own run_mc stream (tests/golden/runmc_*.npz)
+ * consumes the draw after every volume step. */
 typedef struct {
     double L, xclb, kext, minus_two_kext, dr;
     double *xx, *yy, *zz, *S, *uu, *Fx, *Fy, *Fz, *beta, *lj6, *lj12, *sigma6;
@@ -1567,7 +1589,8 @@ int yo_volume_move(yo_sys *s, double pressure, double log_max_volume_scaling, do
         double arg = -dU - pressure * dV + (double)(s->nmol + 1) * log(V_new / V_old);
         proba = exp(arg);
     }
-    int accepted = proba > 1.0 || yo_rand(s) < proba;
+    const double xksi = yo_rand(s);
+    int accepted = proba > 1.0 || xksi < proba;
     restore_free(s, &v, !accepted);
     if (new_box_length) *new_box_length = s->L;
     if (overlap_out) *overlap_out = ov;
@@ -1682,7 +1705,7 @@ void yo_destroy(yo_sys *s) {
                  s->four_eps, s->sigma6, s->tail_counts, s->xx, s->yy, s->zz, s->uu, s->Fx, s->Fy, s->Fz,
                  s->g.kx, s->g.ky, s->g.kz, s->g.ip, s->g.iq, s->g.xk, s->g.xk2, s->beta, s->beta6,
                  s->beta12, s->beta_lj, s->S, s->mol_old, s->mol_new, s->tot_new, s->krc, s->krl,
-                 s->back_new, s->back_old, s->freq_beg, s->freq_end, s->xfreq_beg, s->xfreq_end};
+                 s->back_new, s->back_old, s->freq_beg, s->freq_end, s->xfreq_beg, s->xfreq_end, s->ulist};
     for (size_t i = 0; i < sizeof p / sizeof p[0]; ++i) free(p[i]);
     free(s);
 }

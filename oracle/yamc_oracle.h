@@ -41,9 +41,10 @@ typedef struct yo_move_rec {
     double angle;
 } yo_move_rec;
 
-/* RNG modes: Philox-4x32-10 stream, or one constant uniform per move
- * (random.f90:80-82 RND_CONSTANT, set per move by the caller). */
-enum { YO_RNG_PHILOX = 0, YO_RNG_CONST = 1 };
+/* RNG modes: Philox-4x32-10 stream, one constant uniform per move
+ * (random.f90:80-82 RND_CONSTANT, set per move by the caller), or the replay
+ * of a recorded list of uniforms (the reference's own rand() stream). */
+enum { YO_RNG_PHILOX = 0, YO_RNG_CONST = 1, YO_RNG_LIST = 2 };
 
 yo_sys *yo_create(const yo_desc *d);
 void yo_destroy(yo_sys *s);
@@ -102,6 +103,7 @@ void yo_get_real_arrays(const yo_sys *s, double *uu, double *fx, double *fy, dou
 void yo_rng_philox(yo_sys *s, uint64_t seed, uint64_t stream, uint64_t counter);
 uint64_t yo_rng_counter(const yo_sys *s);
 void yo_rng_const(yo_sys *s, double u);
+void yo_rng_list(yo_sys *s, const double *u, uint64_t n);
 double yo_rand(yo_sys *s);
 void yo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

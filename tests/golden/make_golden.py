@@ -181,9 +181,125 @@ def make_case(name, sysm, nmoves, seed, vols=(), xchgs=()):
     print(f"{path}: natom={natom} nk={nk} ntype={nt} moves={nmoves} accepted={arrays['move_accepted'].sum()}")
 
 
+def parse_runmc(text):
+    """The run_mc section of the driver's output: the uniform stream, per
+    cycle the step counters, box length and energies, the four uniforms
+    after the run, the final positions, per-atom arrays and Coulomb S(k)."""
+    lines = text.split("\n")
+    out = {"CYCLES": [], "SUMS": []}
+    i = 0
+    cur = None
+    while i < len(lines):
+        tok = lines[i].split()
+        if not tok:
+            i += 1
+            continue
+        if tok[0] == "STREAM":
+            n = int(tok[1])
+            out["STREAM"] = [float(v) for v in lines[i + 1:i + 1 + n]]
+            i += n
+        elif tok[0] == "CYCLE":
+            cur = {"cycle": int(tok[1]), "counts": [float(v) for v in tok[2:8]], "box": float(tok[8])}
+            out["CYCLES"].append(cur)
+        elif tok[0] == "E" and cur is not None and "E" not in cur:
+            cur["E"] = [float(v) for v in " ".join(lines[i + 1:i + 4]).split()]
+            i += 3
+        elif tok[0] == "TAIL":
+            out["TAIL"] = [float(v) for v in lines[i + 1:i + 5]]
+            i += 4
+            cur = None
+        elif tok[0] == "R" and "TAIL" in out:
+            tag, n = tok[1], int(tok[2])
+            vals = [float(v) for v in lines[i + 1:i + 1 + n]]
+            i += n
+            if tag == "SUMS":
+                out["SUMS"].append(vals)
+            else:
+                out[tag] = vals
+        elif tok[0] == "MOLTAB":
+            break
+        i += 1
+    return out
+
+
+def make_runmc_case(name, sysm, ncycles, seed, psm, pressure, mvs, weights=None, nstream=60000):
+    """The reference's own run_mc (runmc.f90:178-334) for ncycles cycles with
+    its standard random stream: records the stream it draws, the state after
+    every cycle and the number of uniforms the run consumed.  weights:
+    per-molecule (move, exchange) weights written as a frequency file (None:
+    the 'none' file)."""
+    freq = []
+    if weights is not None:
+        mv, xc = weights
+        a = 0
+        for b in range(1, sysm.nmol + 1):  # maximal runs of equal weights
+            if b == sysm.nmol or mv[b] != mv[a] or xc[b] != xc[a]:
+                freq.append((a + 1, b, float(mv[a]), float(xc[a])))
+                a = b
+    run = dict(ncycles=ncycles, seed=seed, psm=psm, pressure=pressure, mvs=mvs, nstream=nstream, freq=freq)
+    with tempfile.TemporaryDirectory() as td:
+        refcase.write_case(td, sysm, [], [], (), (), runmc=run)
+        stdout, _ = refcase.run(td, timeout=600)
+    head, _, _ = stdout.partition("\nSTREAM")
+    g = parse(head)
+    r = parse_runmc(stdout)
+    natom, nk, nt = g["SIZES"]
+    stream = np.array(r["STREAM"])
+    tail = np.array(r["TAIL"])
+    # uniforms consumed: the first position where the four uniforms drawn
+    # after the run continue the recorded stream
+    used = [k for k in range(len(stream) - 3) if np.array_equal(stream[k:k + 4], tail)]
+    assert len(used) >= 1, "stream too short or not found"
+    used = used[0]
+    cyc = r["CYCLES"]
+    assert len(cyc) == ncycles
+    arrays = dict(
+        params=np.array([sysm.params.box_length, sysm.params.temp, sysm.params.sr, sysm.params.sk,
+                         sysm.params.alpha, sysm.params.external_permutivity, sysm.params.dr_a,
+                         sysm.params.d_angle_degree, sysm.params.lambda_f, sysm.params.lambda_c]),
+        species=np.array([m.name for m in sysm.species]), counts=np.array(sysm.counts, np.int32),
+        mol_type=sysm.mol_type, mol_pos=sysm.mol_pos, mol_angles=sysm.mol_angles,
+        run_psm=np.float64(psm), run_pressure=np.float64(pressure), run_mvs=np.float64(mvs),
+        move_weights=np.asarray(weights[0] if weights is not None else np.ones(sysm.nmol), np.float64),
+        xchange_weights=np.asarray(weights[1] if weights is not None else np.zeros(sysm.nmol), np.float64),
+        stream=stream[:used + 64], stream_used=np.int64(used),
+        e_init=np.array(g["E_INIT"]),
+        cycle_counts=np.array([c["counts"] for c in cyc]), cycle_box=np.array([c["box"] for c in cyc]),
+        cycle_e=np.array([c["E"] for c in cyc]),
+        xx_end=np.array(r["RUNXX"]), yy_end=np.array(r["RUNYY"]), zz_end=np.array(r["RUNZZ"]),
+        uu_end=np.array(r["RUNUU"]), fx_end=np.array(r["RUNFX"]), fy_end=np.array(r["RUNFY"]),
+        fz_end=np.array(r["RUNFZ"]), sums_coulomb_end=np.array(r["SUMS"][-8:]).reshape(8, nk),
+    )
+    path = os.path.join(OUT, f"{name}.npz")
+    np.savez_compressed(path, **arrays)
+    cc = arrays["cycle_counts"][-1]
+    print(f"{path}: natom={natom} nk={nk} cycles={ncycles} uniforms={used} moves+exchanges={cc[0]:.0f} "
+          f"accepted={cc[1]:.0f} volume steps={cc[2]:.0f} ({cc[3]:.0f} accepted) exchanges={cc[4]:.0f} "
+          f"({cc[5]:.0f} accepted)")
+
+
+def runmc_cases():
+    """Ions with force bias, exchange weights and volume steps; and plain
+    NVT water without force bias (the 'none' frequency file)."""
+    small = ys.config("small", seed=31)
+    t = np.asarray(small.mol_type)
+    mv = np.where(t == 0, 1.0, 2.0)
+    xc = np.where(t == 0, 0.0, 0.5)
+    xc[np.nonzero(t == 0)[0][:3]] = 0.25  # three waters take part in exchanges too
+    make_runmc_case("runmc_small_bias", small, 60, 1234567, 2.0, 1.0e5, 1.05, (mv, xc))
+    w = ys.lattice_system(
+        ys.Params(box_length=12.42, temp=298.0, sr=3.2, sk=3.0, alpha=6.4, external_permutivity=70.0, dr_a=0.3,
+                  d_angle_degree=15.0, lambda_f=0.5, lambda_c=0.5), [ys.SPCE, ys.NA_DANG, ys.CL_DANG],
+        [60, 2, 2], 34)
+    make_runmc_case("runmc_ions64_nvt", w, 12, 7654321, 0.0, 1.0e5, 1.05, None)
+
+
 def main():
     if not refcase.available():
         raise SystemExit("oracle/_ref/ref_driver is not built (oracle/build_ref.sh)")
+    if len(sys.argv) > 1 and sys.argv[1] == "runmc":
+        runmc_cases()
+        return
     # volume steps: shrink and grow at 1 bar and at 5 kbar, 2 % maximum scaling
     vols = [(0.30, 1.0e5, 1.02), (0.85, 1.0e5, 1.02), (0.10, 5.0e8, 1.02), (0.65, 1.0e5, 1.05)]
     small = ys.config("small", seed=31)
@@ -194,6 +310,7 @@ def main():
         ys.Params(box_length=12.42, temp=298.0, sr=3.2, sk=3.0, alpha=6.4, external_permutivity=70.0, dr_a=0.3,
                   d_angle_degree=15.0, lambda_f=0.0, lambda_c=0.0), [ys.SPCE], [64], 33)
     make_case("ref_water64", w64, 30, 3, vols, xchange_pairs(w64, 13))
+    runmc_cases()
 
 
 def xchange_pairs(sysm, seed, n=8):

@@ -20,6 +20,11 @@ def cases():
     return sorted(glob.glob(os.path.join(HERE, "golden", "ref_*.npz")))
 
 
+def runmc_cases():
+    """Fixtures of the reference's own run_mc with its own random stream."""
+    return sorted(glob.glob(os.path.join(HERE, "golden", "runmc_*.npz")))
+
+
 def load(path):
     g = dict(np.load(path, allow_pickle=False))
     p = g["params"]
@@ -156,3 +161,46 @@ def check_xchange_steps(impl, sysm, g, xstep):
     assert rel(np.stack([fx, fy, fz]), fe, fe) < RTOL
     assert rel(impl.sums()[0], g["sums_coulomb_xchg"], g["sums_coulomb_xchg"]) < RTOL
     return len(g["xchg_u"])
+
+
+def check_runmc(impl, sysm, g, cycle):
+    """Replays the reference's recorded uniform stream (g["stream"]) through
+    run_mc cycles: cycle() runs one run_mc cycle (runmc.f90:204-268: the
+    pressure-step draw, then a volume step or nmol steps of exchange draw,
+    pick and movemc or mc_xchange) and returns (accepted steps, volume steps,
+    accepted volume steps).  After every cycle the counters, the box length
+    and every energy component equal the reference's; at the end the
+    positions, per-atom real-space arrays, Coulomb S(k) and the number of
+    uniforms consumed equal the reference's."""
+    acc = nv = nva = 0
+    nmol = sysm.nmol
+    for k in range(len(g["cycle_e"])):
+        a, v, va = cycle()
+        acc, nv, nva = acc + a, nv + v, nva + va
+        cnt = g["cycle_counts"][k]
+        steps = nmol * (k + 1 - nv)
+        assert (steps, acc, nv, nva) == (int(cnt[0]), int(cnt[1]), int(cnt[2]), int(cnt[3])), (k, steps, acc, nv, nva,
+                                                                                              cnt)
+        L = impl.box_length()
+        assert abs(L - g["cycle_box"][k]) <= 1e-12 * g["cycle_box"][k], (k, L, g["cycle_box"][k])
+        ev = energies_vec(impl.energies(), impl.mu(), impl.tails())
+        ref = g["cycle_e"][k]
+        assert rel(ev[:13], ref[:13], ref[:13]) < RTOL, (k, ev[:13] - ref[:13])
+    x, y, z = impl.positions()
+    dpos = rel(np.stack([x, y, z]), np.stack([g["xx_end"], g["yy_end"], g["zz_end"]]), 1.0)
+    assert dpos < RTOL, dpos  # box units
+    uu, fx, fy, fz = impl.real_arrays()
+    assert rel(uu, g["uu_end"], g["uu_end"]) < RTOL
+    fe = np.stack([g["fx_end"], g["fy_end"], g["fz_end"]])
+    assert rel(np.stack([fx, fy, fz]), fe, fe) < RTOL
+    assert rel(impl.sums()[0], g["sums_coulomb_end"], g["sums_coulomb_end"]) < RTOL
+    assert impl.rng_counter() == int(g["stream_used"]), (impl.rng_counter(), int(g["stream_used"]))
+    return len(g["cycle_e"]), int(g["cycle_counts"][-1][4])
+
+
+def runmc_setup(impl, g):
+    """Frequencies and the recorded stream of a run_mc fixture."""
+    impl.set_frequencies(g["move_weights"], g["xchange_weights"])
+    impl.set_rng_list(g["stream"])
+    T = float(g["params"][1])
+    return float(g["run_psm"]), pressure_kT_A3(float(g["run_pressure"]), T), float(np.log(g["run_mvs"]))

@@ -14,8 +14,31 @@ import golden_cases as gc  # noqa: E402
 CASES = gc.cases()
 
 
+RUNMC = gc.runmc_cases()
+
+
 def test_fixtures_present():
     assert len(CASES) >= 3
+    assert len(RUNMC) >= 2
+
+
+@pytest.mark.parametrize("path", RUNMC, ids=lambda p: p.split("/")[-1])
+def test_oracle_replays_reference_run_mc(oracle_mod, path):
+    """run_mc's random-number consumption (runmc.f90:204-268: a pressure
+    draw per cycle, an exchange draw per step, BiasedRandom_choose picks,
+    the rejection loops of the proposals, the acceptance draws of movemc,
+    mcvol and mc_xchange) pinned against the reference's own run: its
+    recorded uniform stream, replayed by the oracle, gives the same step
+    counts, energies after every cycle and final state, and uses up exactly
+    as many uniforms."""
+    sysm, g = gc.load(path)
+    o = oracle_mod.Oracle(sysm)
+    e, _ = o.mc_init()
+    ev = gc.energies_vec(e, o.mu(), o.tails())
+    assert gc.rel(ev, g["e_init"], g["e_init"]) < gc.RTOL
+    psm, p, lmvs = gc.runmc_setup(o, g)
+    ncyc, nx = gc.check_runmc(o, sysm, g, lambda: o.run_npt(1, psm, p, lmvs))
+    assert ncyc == len(g["cycle_e"])
 
 
 @pytest.mark.parametrize("path", CASES, ids=lambda p: p.split("/")[-1])

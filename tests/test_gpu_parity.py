@@ -421,6 +421,42 @@ def test_gpu_matches_reference_golden(lib, case):
     assert gc.check_xchange_steps(gpu, sysm, g, xstep) == 8
 
 
+@pytest.mark.parametrize("case", __import__("golden_cases").runmc_cases(), ids=lambda p: p.split("/")[-1])
+@pytest.mark.parametrize("block", [256, 64])
+def test_gpu_replays_reference_run_mc(lib, case, block):
+    """run_mc cycles on the device (chain kernel: pressure draw, exchange
+    draw, pick, proposal rejection loops, acceptance; host: volume and
+    exchange steps) consuming the reference's own recorded uniform stream:
+    identical step counts, every energy component after every cycle within
+    1e-10, the final state, and the same number of uniforms drawn as the
+    reference's run_mc.  Shape 64 runs the move cycles in the wave kernel
+    (FP64, no exchange weights) or the workgroup kernel."""
+    import golden_cases as gc
+    from yamc_amd.ewald import EwaldMC
+    sysm, g = gc.load(case)
+    gpu = EwaldMC(sysm)
+    gpu.set_block_size(block)
+    e, _ = gpu.mc_init()
+    ev = gc.energies_vec(e, gpu.mu(), gpu.tails())
+    assert gc.rel(ev, g["e_init"], g["e_init"]) < gc.RTOL
+    psm, p, lmvs = gc.runmc_setup(gpu, g)
+    ncyc, _ = gc.check_runmc(gpu, sysm, g, lambda: gpu.run_npt(1, psm, p, lmvs))
+    assert ncyc == len(g["cycle_e"])
+
+
+def test_replay_list_overrun_is_an_error(lib):
+    """A run that needs more uniforms than the replayed list holds fails
+    with ERROR_LIMITS instead of inventing draws."""
+    from yamc_amd import YamcError, config
+    from yamc_amd.ewald import EwaldMC
+    g = EwaldMC(config("small", seed=5))
+    g.mc_init()
+    g.set_rng_list(np.full(20, 0.37))
+    with pytest.raises(YamcError) as ei:
+        g.run(1)
+    assert ei.value.code == 2
+
+
 def test_rdf_counts_match_oracle(lib):
     """yamc_rdf (COM pair counts, mcrdf2.f90 count_distances) equals the CPU
     restatement exactly on the 2128-molecule ionic system, before and after

@@ -26,7 +26,8 @@ EXPORTS = [
     "yamc_ctx_create", "yamc_ctx_destroy", "yamc_ctx_info", "yamc_set_positions", "yamc_get_positions",
     "yamc_grid", "yamc_kspace_init", "yamc_kspace_get_beta", "yamc_kspace_get_sums", "yamc_kspace_delta",
     "yamc_kspace_forces", "yamc_rspace_full", "yamc_rspace_partial", "yamc_tails", "yamc_external_mu",
-    "yamc_mc_init", "yamc_mc_energies", "yamc_mc_real_arrays", "yamc_mc_set_rng", "yamc_mc_get_rng",
+    "yamc_mc_init", "yamc_mc_energies", "yamc_mc_real_arrays", "yamc_mc_set_rng", "yamc_mc_set_rng_list",
+    "yamc_mc_get_rng",
     "yamc_mc_run", "yamc_mc_run_const", "yamc_resync", "yamc_mc_volume_move", "yamc_mc_run_npt", "yamc_box_length",
     "yamc_mc_set_frequencies", "yamc_mc_xchange", "yamc_set_box", "yamc_mc_commit_move", "yamc_batch_run", "yamc_batch_create", "yamc_batch_launch",
     "yamc_batch_destroy", "yamc_forces", "yamc_rdf",

@@ -22,6 +22,7 @@ __global__ void yk_atom_tables(YdChain c, const double *x, const double *y, cons
 __global__ void yk_full_sums(YdChain c, const double *tab, int ngroups, double *part);
 __global__ void yk_full_sums_reduce(YdChain c, const double *part, int ngroups);
 __global__ void yk_kenergy(YdChain c);
+
 __global__ void yk_mu(YdChain c, double *mu_out);
 __global__ void yk_tails(YdChain c, const double *lj6, const double *lj12, const int *counts);
 __global__ void yk_scale_box(YdChain c, double sc);
@@ -31,7 +32,7 @@ __global__ void yk_rspace_full_atoms(YdChain c, const double *pat, int ngroups);
 __global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, double *comps_out);
 template <bool F32, int BS>
 __global__ void yk_chain(const YdChain *chains, const double *const *sig6s, const double *const *feps_s, int nsteps,
-                         int mode, int max_moves, int lds_atoms);
+                         int mode, int max_moves, int lds_atoms, int resync_every);
 template <bool F32, int OCC>
 __global__ void yk_wchain(const YdChain *chains, int nchains, const double *const *sig6s, const double *const *feps_s,
                           int ncycles);  // FP64 instantiations only
@@ -113,6 +114,9 @@ struct yamc_ctx {
     std::vector<double> xfreq_beg, xfreq_end;
     double pxch = 0.0;
     double *d_xd = nullptr, *d_xb = nullptr, *d_xmv = nullptr;
+    // replay mode: the recorded uniforms, device copy and host copy
+    double *d_ulist = nullptr;
+    std::vector<double> h_ulist;
 };
 
 template <typename T>
@@ -175,6 +179,9 @@ static int sync(yamc_ctx *c) {
     HIPCHK(hipStreamSynchronize(c->stream));
     return YAMC_OK;
 }
+
+static int get_state(yamc_ctx *c, YdState *st);
+static int put_state(yamc_ctx *c, const YdState *st);
 
 extern "C" const char *yamc_last_error(void) { return g_err; }
 
@@ -285,6 +292,16 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
         TRY(dalloc(c, &c->d_counts, T > 0 ? T : 1));
         TRY(dalloc(c, &c->d_xk2, nk)); TRY(dalloc(c, &c->d_beta6, nk)); TRY(dalloc(c, &c->d_beta12, nk));
         TRY(dalloc(c, &c->d_tab, (size_t)N * 6 * YD_KP));
+        {
+            // atom groups of the from-scratch S(k): each set's atom list in
+            // groups (of at least 64 atoms for the largest list), enough
+            // (set, k, group) threads to fill the GPU (about 2^19), at most 64
+            const int nsk = (1 + T) * nk;
+            int ng = (1 << 19) / nsk;
+            ng = ng < 1 ? 1 : ng > 64 ? 64 : ng;
+            if (ng > N / 64) ng = N / 64 > 1 ? N / 64 : 1;
+            h.fs_ng = ng;
+        }
         TRY(dalloc(c, &c->d_ov, 1)); TRY(dalloc(c, &c->d_io, 64)); TRY(dalloc(c, &c->d_mol, 3 * YD_MAXA));
         TRY(dalloc(c, &c->d_chain, 1)); TRY(dalloc(c, &c->d_sig6s, 1)); TRY(dalloc(c, &c->d_feps, 1));
         // host staging
@@ -351,6 +368,7 @@ extern "C" int yamc_ctx_destroy(yamc_ctx *c) {
     if (c->d_xd) (void)hipFree(c->d_xd);
     if (c->d_xb) (void)hipFree(c->d_xb);
     if (c->d_xmv) (void)hipFree(c->d_xmv);
+    if (c->d_ulist) (void)hipFree(c->d_ulist);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -401,13 +419,8 @@ static int total_energy(yamc_ctx *c) {
     hipStream_t s = c->stream;
     int N = c->natom, nk = c->nk, T = c->ntype;
     yk_atom_tables<<<(N + 255) / 256, 256, 0, s>>>(c->ch, c->ch.x, c->ch.y, c->ch.z, c->d_tab);
-    // each set's atom list in groups (of at least 64 atoms for the largest
-    // list), enough (set, k, group) threads to fill the GPU (about 2^19), at
-    // most 64 groups
     const int nsk = (1 + T) * nk;
-    int ng = (1 << 19) / nsk;
-    ng = ng < 1 ? 1 : ng > 64 ? 64 : ng;
-    if (ng > N / 64) ng = N / 64 > 1 ? N / 64 : 1;
+    const int ng = c->ch.fs_ng;
     double *part = nullptr;
     HIPCHK(hipMallocAsync((void **)&part, sizeof(double) * 8 * (size_t)nsk * ng, s));
     yk_full_sums<<<dim3((nsk + 127) / 128, ng), 128, 0, s>>>(c->ch, c->d_tab, ng, part);
@@ -703,8 +716,42 @@ extern "C" int yamc_mc_set_rng(yamc_ctx *c, uint64_t seed, uint64_t stream, uint
     st.key[0] = (uint32_t)seed; st.key[1] = (uint32_t)(seed >> 32);
     st.stream[0] = (uint32_t)stream; st.stream[1] = (uint32_t)(stream >> 32);
     st.ctr = counter;
+    st.ulist = nullptr;  // back to the Philox stream
+    st.nlist = 0;
     HIPCHK(hipMemcpyAsync(c->ch.st, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
     return sync(c);
+}
+
+extern "C" int yamc_mc_set_rng_list(yamc_ctx *c, const double *u, int64_t n) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    if (n < 0 || (n > 0 && !u)) return fail(YAMC_ERROR_PARAMETER, "bad uniform list");
+    for (int64_t k = 0; k < n; ++k)
+        if (!(u[k] > 0.0 && u[k] < 1.0)) return fail(YAMC_ERROR_PARAMETER, "uniform %lld outside (0, 1)", (long long)k);
+    YdState st;
+    TRY(get_state(c, &st));
+    if (c->d_ulist) {
+        HIPCHK(hipFree(c->d_ulist));
+        c->d_ulist = nullptr;
+    }
+    c->h_ulist.assign(u, u + n);
+    if (n > 0) {
+        HIPCHK(hipMalloc((void **)&c->d_ulist, sizeof(double) * n));
+        HIPCHK(hipMemcpy(c->d_ulist, u, sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    st.ulist = c->d_ulist;
+    st.nlist = (unsigned long long)n;
+    st.ctr = 0;
+    return put_state(c, &st);
+}
+
+// After a run in replay mode: an error if the run drew past the list.
+static int check_list(yamc_ctx *c) {
+    YdState st;
+    TRY(get_state(c, &st));
+    if (st.ulist && st.ctr > st.nlist)
+        return fail(YAMC_ERROR_LIMITS, "replayed uniform list exhausted (%llu drawn, %llu recorded)", st.ctr,
+                    st.nlist);
+    return YAMC_OK;
 }
 
 extern "C" int yamc_mc_get_rng(yamc_ctx *c, uint64_t *counter) {
@@ -726,7 +773,7 @@ extern "C" int yamc_resync(yamc_ctx *c) {
 template <bool F32, int BS>
 static int launch_chain_t(yamc_ctx *c, const YdChain *d_chain, const double *const *sig6s,
                           const double *const *feps, int nchains, int nsteps, int mode, int max_natom,
-                          int max_moves) {
+                          int max_moves, int resync_every) {
     // The packed position copy of the real-space scan (4 bytes per atom) goes
     // into dynamic LDS when it fits next to the kernel's static LDS: within
     // half of a CU's LDS when two chains share a CU (more chains than CUs at
@@ -748,7 +795,8 @@ static int launch_chain_t(yamc_ctx *c, const YdChain *d_chain, const double *con
         dyn = want;
         HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
     }
-    yk_chain<F32, BS><<<nchains, BS, dyn, c->stream>>>(d_chain, sig6s, feps, nsteps, mode, max_moves, lds_atoms);
+    yk_chain<F32, BS><<<nchains, BS, dyn, c->stream>>>(d_chain, sig6s, feps, nsteps, mode, max_moves, lds_atoms,
+                                                       resync_every);
     HIPCHK(hipGetLastError());
     return YAMC_OK;
 }
@@ -760,9 +808,10 @@ static int launch_chain_t(yamc_ctx *c, const YdChain *d_chain, const double *con
 // on its first GPU launch and its cause is not yet found).
 static int launch_chain(yamc_ctx *c, const YdChain *d_chain, const double *const *sig6s,
                         const double *const *feps, int nchains, int nsteps, int mode, int max_natom,
-                        int max_moves = 0) {
+                        int max_moves = 0, int resync_every = 0) {
     const bool f32 = c->precision == YAMC_FP32C;
-    if ((c->chain_bs == 64 || c->chain_bs == 128) && mode == 0 && !f32 && max_moves == 0) {  // one wave per replica
+    if ((c->chain_bs == 64 || c->chain_bs == 128) && mode == 0 && !f32 && max_moves == 0 &&
+        resync_every == 0) {  // one wave per replica
         int nb = (nchains + YW_WPB - 1) / YW_WPB;
         if (c->chain_bs == 64) yk_wchain<false, 1><<<nb, YW_WPB * 64, 0, c->stream>>>(d_chain, nchains, sig6s, feps, nsteps);
         else yk_wchain<false, 2><<<nb, YW_WPB * 64, 0, c->stream>>>(d_chain, nchains, sig6s, feps, nsteps);
@@ -770,10 +819,14 @@ static int launch_chain(yamc_ctx *c, const YdChain *d_chain, const double *const
         return YAMC_OK;
     }
     if (c->chain_bs == 512)
-        return f32 ? launch_chain_t<true, 512>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom, max_moves)
-                   : launch_chain_t<false, 512>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom, max_moves);
-    return f32 ? launch_chain_t<true, 256>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom, max_moves)
-               : launch_chain_t<false, 256>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom, max_moves);
+        return f32 ? launch_chain_t<true, 512>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom, max_moves,
+                                               resync_every)
+                   : launch_chain_t<false, 512>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom, max_moves,
+                                                resync_every);
+    return f32 ? launch_chain_t<true, 256>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom, max_moves,
+                                           resync_every)
+               : launch_chain_t<false, 256>(c, d_chain, sig6s, feps, nchains, nsteps, mode, max_natom, max_moves,
+                                            resync_every);
 }
 
 extern "C" int yamc_set_block_size(yamc_ctx *c, int32_t block) {
@@ -847,7 +900,8 @@ extern "C" int yamc_mc_run(yamc_ctx *c, int32_t ncycles, int32_t resync_every, y
     HIPCHK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
     if (n_accepted) *n_accepted = st1.n_accepted - st0.n_accepted;
     HIPCHK(hipMemcpyAsync(c->d_chain, &c->ch, sizeof(YdChain), hipMemcpyHostToDevice, c->stream));
-    return sync(c);
+    TRY(sync(c));
+    return check_list(c);
 }
 
 extern "C" int yamc_mc_run_const(yamc_ctx *c, int32_t nmoves, const int32_t *imol, const double *u,
@@ -885,11 +939,18 @@ extern "C" int yamc_mc_run_const(yamc_ctx *c, int32_t nmoves, const int32_t *imo
     return YAMC_OK;
 }
 
-// Staged batch: the chain descriptors of n contexts on the device.
+// Staged batch: the chain descriptors of n contexts on the device.  A sweep
+// of ncycles run_mc cycles per replica runs as one launch; with
+// resync_every = R > 0 each chain rebuilds its S(k) from scratch in FP64
+// after every R moves inside that launch (yk_chain_rebuild), with the same
+// arithmetic as the launch-level rebuilds between the segments of
+// yamc_mc_run(ctx, ncycles, R), so each replica's chain equals its single
+// run.
 struct yamc_batch {
     yamc_ctx *c0 = nullptr;
-    int n = 0, ncycles = 0, max_natom = 0;
+    int n = 0, ncycles = 0, max_natom = 0, resync_every = 0;
     std::vector<yamc_ctx *> ctxs;
+    std::vector<YdChain> h_ch;  // n descriptors, rebuilt from the contexts before every launch
     YdChain *d_ch = nullptr;
     const double **d_s6 = nullptr, **d_fe = nullptr;
     long long *d_cnt = nullptr;  // accepted-move counters before and after a launch (2 x n)
@@ -897,34 +958,50 @@ struct yamc_batch {
     hipGraphExec_t exec = nullptr;
 };
 
-static int batch_stage(yamc_ctx *const *ctxs, int32_t n, yamc_batch *b) {
-    if (!ctxs || n < 1) return fail(YAMC_ERROR_PARAMETER, "empty batch");
-    yamc_ctx *c0 = ctxs[0];
-    if (!c0) return fail(YAMC_ERROR_PARAMETER, "null context in batch");
-    std::vector<YdChain> chains(n);
-    std::vector<const double *> s6(n), fe(n);
-    for (int i = 0; i < n; ++i) {
-        yamc_ctx *c = ctxs[i];
+// Checks the members and writes their current descriptors (box parameters,
+// buffers) into b->h_ch.
+static int batch_descriptors(yamc_batch *b) {
+    yamc_ctx *c0 = b->c0;
+    b->h_ch.resize(b->n);
+    for (int i = 0; i < b->n; ++i) {
+        yamc_ctx *c = b->ctxs[i];
         if (!c) return fail(YAMC_ERROR_PARAMETER, "null context in batch");
         if (c->device != c0->device || c->precision != c0->precision)
             return fail(YAMC_ERROR_PARAMETER, "batch contexts differ in device or precision");
-        if (!c->state_ready) return fail(YAMC_ERROR_INITIALIZATION, "yamc_mc_init has not run on a batch member");
+        if (!c->state_ready)
+            return fail(YAMC_ERROR_INITIALIZATION, "batch member %d: yamc_mc_init has not run on its state", i);
         if (c->pxch > 0)
             return fail(YAMC_ERROR_PARAMETER, "batch member %d has exchange steps: run it with yamc_mc_run", i);
-        HIPCHK(hipStreamSynchronize(c->stream));
-        chains[i] = c->ch;
-        s6[i] = c->d_sig6;
-        fe[i] = c->d_4eps;
-        b->max_natom = c->natom > b->max_natom ? c->natom : b->max_natom;
+        if (!b->graph) b->max_natom = c->natom > b->max_natom ? c->natom : b->max_natom;
+        b->h_ch[i] = c->ch;
+        b->h_ch[i].rec = nullptr;
     }
+    return YAMC_OK;
+}
+
+static int batch_stage(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int32_t resync_every, yamc_batch *b) {
+    if (!ctxs || n < 1) return fail(YAMC_ERROR_PARAMETER, "empty batch");
+    if (ncycles < 0) return fail(YAMC_ERROR_PARAMETER, "negative cycle count");
+    if (resync_every < 0) return fail(YAMC_ERROR_PARAMETER, "negative resync interval");
+    yamc_ctx *c0 = ctxs[0];
+    if (!c0) return fail(YAMC_ERROR_PARAMETER, "null context in batch");
     b->c0 = c0;
     b->n = n;
+    b->ncycles = ncycles;
+    b->resync_every = resync_every;
     b->ctxs.assign(ctxs, ctxs + n);
+    TRY(batch_descriptors(b));
+    std::vector<const double *> s6(n), fe(n);
+    for (int i = 0; i < n; ++i) {
+        HIPCHK(hipStreamSynchronize(ctxs[i]->stream));
+        s6[i] = ctxs[i]->d_sig6;
+        fe[i] = ctxs[i]->d_4eps;
+    }
     HIPCHK(hipMalloc((void **)&b->d_ch, sizeof(YdChain) * n));
     HIPCHK(hipMalloc((void **)&b->d_s6, sizeof(void *) * n));
     HIPCHK(hipMalloc((void **)&b->d_fe, sizeof(void *) * n));
     HIPCHK(hipMalloc((void **)&b->d_cnt, sizeof(long long) * 2 * n));
-    HIPCHK(hipMemcpy(b->d_ch, chains.data(), sizeof(YdChain) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_ch, b->h_ch.data(), sizeof(YdChain) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_s6, s6.data(), sizeof(void *) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_fe, fe.data(), sizeof(void *) * n, hipMemcpyHostToDevice));
     return YAMC_OK;
@@ -938,6 +1015,12 @@ static void batch_free(yamc_batch *b) {
     if (b->d_fe) (void)hipFree(b->d_fe);
     if (b->d_cnt) (void)hipFree(b->d_cnt);
     delete b;
+}
+
+// Enqueues one sweep of the batch on stream s: one kernel launch (so it can
+// be captured).
+static int batch_sweep(yamc_batch *b, hipStream_t) {
+    return launch_chain(b->c0, b->d_ch, b->d_s6, b->d_fe, b->n, b->ncycles, 0, b->max_natom, 0, b->resync_every);
 }
 
 // Copies every member's accepted-move counter into out[0..n) on the stream
@@ -963,42 +1046,54 @@ static int batch_counts_diff(const yamc_batch *b, hipStream_t s, int64_t *n_acce
     return YAMC_OK;
 }
 
-extern "C" int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int64_t *n_accepted) {
+// Before every launch: the members' current descriptors (a volume step or a
+// box change since the last launch changes a member's box parameters) into
+// the device array the launch reads; members must still have a ready state.
+static int batch_refresh(yamc_batch *b, hipStream_t s) {
+    TRY(batch_descriptors(b));
+    for (int i = 0; i < b->n; ++i) HIPCHK(hipStreamSynchronize(b->ctxs[i]->stream));
+    HIPCHK(hipMemcpyAsync(b->d_ch, b->h_ch.data(), sizeof(YdChain) * b->n, hipMemcpyHostToDevice, s));
+    return YAMC_OK;
+}
+
+static int batch_go(yamc_batch *b, int64_t *n_accepted, bool graph) {
+    yamc_ctx *c0 = b->c0;
+    hipStream_t s = c0->stream;
+    TRY(batch_refresh(b, s));
+    if (n_accepted) TRY(batch_counts_gather(b, 0, s));
+    HIPCHK(hipEventRecord(c0->ev0, s));
+    if (graph) HIPCHK(hipGraphLaunch(b->exec, s));
+    else TRY(batch_sweep(b, s));
+    HIPCHK(hipEventRecord(c0->ev1, s));
+    if (n_accepted) TRY(batch_counts_gather(b, 1, s));
+    TRY(sync(c0));
+    HIPCHK(hipEventElapsedTime(&c0->last_ms, c0->ev0, c0->ev1));
+    if (n_accepted) TRY(batch_counts_diff(b, s, n_accepted));
+    return YAMC_OK;
+}
+
+extern "C" int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int32_t resync_every,
+                              int64_t *n_accepted) {
     yamc_batch *b = new yamc_batch;
-    int rc = batch_stage(ctxs, n, b);
-    if (rc == YAMC_OK) {
-        yamc_ctx *c0 = b->c0;
-        hipStream_t s = c0->stream;
-        auto go = [&]() -> int {
-            if (n_accepted) TRY(batch_counts_gather(b, 0, s));
-            HIPCHK(hipEventRecord(c0->ev0, s));
-            TRY(launch_chain(c0, b->d_ch, b->d_s6, b->d_fe, n, ncycles, 0, b->max_natom));
-            HIPCHK(hipEventRecord(c0->ev1, s));
-            if (n_accepted) TRY(batch_counts_gather(b, 1, s));
-            TRY(sync(c0));
-            HIPCHK(hipEventElapsedTime(&c0->last_ms, c0->ev0, c0->ev1));
-            if (n_accepted) TRY(batch_counts_diff(b, s, n_accepted));
-            return YAMC_OK;
-        };
-        rc = go();
-    }
+    int rc = batch_stage(ctxs, n, ncycles, resync_every, b);
+    if (rc == YAMC_OK) rc = batch_go(b, n_accepted, false);
     batch_free(b);
     return rc;
 }
 
-extern "C" int yamc_batch_create(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, yamc_batch **out) {
+extern "C" int yamc_batch_create(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int32_t resync_every,
+                                 yamc_batch **out) {
     if (!out) return fail(YAMC_ERROR_PARAMETER, "null output");
     *out = nullptr;
     if (ncycles < 1) return fail(YAMC_ERROR_PARAMETER, "ncycles must be positive");
     yamc_batch *b = new yamc_batch;
-    int rc = batch_stage(ctxs, n, b);
+    int rc = batch_stage(ctxs, n, ncycles, resync_every, b);
     if (rc == YAMC_OK) {
-        b->ncycles = ncycles;
         yamc_ctx *c0 = b->c0;
         auto go = [&]() -> int {
-            // The capture records one kernel node (no memset or copy nodes).
+            // The capture records kernel nodes only (no memset or copy nodes).
             HIPCHK(hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
-            int lrc = launch_chain(c0, b->d_ch, b->d_s6, b->d_fe, n, ncycles, 0, b->max_natom);
+            int lrc = batch_sweep(b, c0->stream);
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(c0->stream, &g);
             if (lrc != YAMC_OK) {
@@ -1022,17 +1117,7 @@ extern "C" int yamc_batch_create(yamc_ctx *const *ctxs, int32_t n, int32_t ncycl
 
 extern "C" int yamc_batch_launch(yamc_batch *b, int64_t *n_accepted) {
     if (!b) return fail(YAMC_ERROR_PARAMETER, "null batch");
-    yamc_ctx *c0 = b->c0;
-    hipStream_t s = c0->stream;
-    if (n_accepted) TRY(batch_counts_gather(b, 0, s));
-    HIPCHK(hipEventRecord(c0->ev0, s));
-    HIPCHK(hipGraphLaunch(b->exec, s));
-    HIPCHK(hipEventRecord(c0->ev1, s));
-    if (n_accepted) TRY(batch_counts_gather(b, 1, s));
-    TRY(sync(c0));
-    HIPCHK(hipEventElapsedTime(&c0->last_ms, c0->ev0, c0->ev1));
-    if (n_accepted) TRY(batch_counts_diff(b, s, n_accepted));
-    return YAMC_OK;
+    return batch_go(b, n_accepted, true);
 }
 
 extern "C" int yamc_batch_destroy(yamc_batch *b) {
@@ -1142,7 +1227,11 @@ static void host_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint
     o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3;
 }
 
-static double host_uniform(const YdState &st, uint64_t *ctr) {
+static double host_uniform(const yamc_ctx *c, const YdState &st, uint64_t *ctr) {
+    if (st.ulist) {  // replay mode: the host copy of the recorded stream
+        const uint64_t n = (*ctr)++;
+        return n < c->h_ulist.size() ? c->h_ulist[n] : 0.5;
+    }
     for (;;) {
         uint64_t n = (*ctr)++;
         uint32_t o[4];
@@ -1161,7 +1250,7 @@ static int draw(yamc_ctx *c, const double *u, double *out) {
     YdState st;
     TRY(get_state(c, &st));
     uint64_t ctr = st.ctr;
-    *out = host_uniform(st, &ctr);
+    *out = host_uniform(c, st, &ctr);
     st.ctr = ctr;
     return put_state(c, &st);
 }
@@ -1280,12 +1369,12 @@ extern "C" int yamc_mc_volume_move(yamc_ctx *c, double pressure_kT_A3, double lo
         const double arg = -dU - pressure_kT_A3 * dV + (double)(c->nmol + 1) * log(V_new / V_old);
         proba = exp(arg);
     }
-    int acc = proba > 1.0;
-    if (!acc) {  // the acceptance draw is taken only when proba <= 1 (MC.f90:804)
-        double r2;
-        TRY(draw(c, u, &r2));
-        acc = r2 < proba;
-    }
+    // MC.f90:804 `(proba > 1) .or. (rand() < proba)`: the reference as
+    // compiled evaluates both operands, so the acceptance draw is taken after
+    // every volume step (pinned by the reference's own run_mc stream)
+    double r2;
+    TRY(draw(c, u, &r2));
+    const int acc = proba > 1.0 || r2 < proba;
     if (!acc) {  // mc_restore_state and the old box's parameters (MC.f90:806-812)
         YdState st, sv;
         TRY(get_state(c, &st));  // the generator keeps its position
@@ -1320,7 +1409,7 @@ extern "C" int yamc_mc_run_npt(yamc_ctx *c, int32_t ncycles, double pressure_ste
         YdState st;
         TRY(get_state(c, &st));
         uint64_t ctr = st.ctr;
-        const double rnd = host_uniform(st, &ctr) * (pressure_step_multiplier + 1);
+        const double rnd = host_uniform(c, st, &ctr) * (pressure_step_multiplier + 1);
         if (pressure_step_multiplier > 0 && rnd > pressure_step_multiplier) {
             st.ctr = ctr;
             TRY(put_state(c, &st));
@@ -1337,7 +1426,7 @@ extern "C" int yamc_mc_run_npt(yamc_ctx *c, int32_t ncycles, double pressure_ste
     if (n_accepted) *n_accepted = nacc;
     if (n_vol_steps) *n_vol_steps = nv;
     if (n_vol_accepted) *n_vol_accepted = nva;
-    return YAMC_OK;
+    return check_list(c);
 }
 
 extern "C" int yamc_box_length(yamc_ctx *c, double *box_length) {

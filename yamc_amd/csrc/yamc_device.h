@@ -44,6 +44,10 @@ struct YdState {
     double tail_c, tail6, tail12;
     uint32_t key[2], stream[2];
     unsigned long long ctr;
+    // replay mode (ulist != null): the uniforms are ulist[ctr++] of a
+    // recorded stream of nlist values instead of Philox
+    const double *ulist;
+    unsigned long long nlist;
     long long n_moves, n_accepted, n_overlap, since_resync;
     unsigned long long phase[10];  // wall-clock ticks per chain-kernel phase (thread 0)
 };
@@ -88,6 +92,10 @@ struct YdChain {
     const int *cimol;     // constant-uniform mode: molecule per move
     const double *cu;     // constant-uniform mode: uniform per move
     int mv_begin;         // step number of the launch's first step in its run (cycle starts: multiples of nmol)
+    // From-scratch S(k): the number of equal atom groups per set whose
+    // partial sums are added in order (fixed per context, so every rebuild
+    // path rounds alike).
+    int fs_ng;
     // Cell list of the workgroup chain kernel's real-space scan (nc = 0: none).
     // nc^3 cubic cells of width 1/nc >= the cut-off (box units); cell
     // (ix, iy, iz) = floor((r + 0.5) nc) mod nc per axis, id (ix nc + iy) nc + iz;
@@ -253,8 +261,14 @@ struct YdRng {
     unsigned long long ctr;
     int constant;
     double cu;
+    const double *list;  // replay mode: recorded uniforms list[0 .. nlist)
+    unsigned long long nlist;
     __device__ double next() {
         if (constant) return cu;
+        if (list) {  // past the end: 0.5, and the counter shows the overrun to the host
+            const unsigned long long n = ctr++;
+            return n < nlist ? list[n] : 0.5;
+        }
         for (;;) {
             unsigned long long n = ctr++;
             uint32_t o[4];
@@ -307,9 +321,10 @@ __device__ __forceinline__ void yd_sincos_axis(double x, int kmax, double *c, do
 // six axis tables of one atom (cx, sx, cy, sy, cz, sz) with stride YD_KP.
 // The xy factor is rebuilt from the tables at each k, which is the value the
 // reference recurrence carries along a (kx, ky) row.
-__device__ __forceinline__ void yd_kr(const double *tab, double q, int kx, int ky, int kz, double *o) {
-    double cxk = tab[kx], sxk = tab[YD_KP + kx], cyk = tab[2 * YD_KP + ky], syk = tab[3 * YD_KP + ky];
-    double czk = tab[4 * YD_KP + kz], szk = tab[5 * YD_KP + kz];
+// (ks: stride of the six axis tables in tab.)
+__device__ __forceinline__ void yd_kr_s(const double *tab, int ks, double q, int kx, int ky, int kz, double *o) {
+    double cxk = tab[kx], sxk = tab[ks + kx], cyk = tab[2 * ks + ky], syk = tab[3 * ks + ky];
+    double czk = tab[4 * ks + kz], szk = tab[5 * ks + kz];
     double qcc = q * cxk * cyk, qss = q * sxk * syk, qsc = q * sxk * cyk, qcs = q * cxk * syk;
     double qcpy = qcc - qss, qspy = qsc + qcs, qcmy = qcc + qss, qsmy = qsc - qcs;
     double a = qcpy * czk, b = qspy * szk, c = qspy * czk, d = qcpy * szk;
@@ -323,13 +338,16 @@ __device__ __forceinline__ void yd_kr(const double *tab, double q, int kx, int k
     o[6] = (ky > 0 && kz > 0) ? e + f : 0.0;
     o[7] = (ky > 0 && kz > 0) ? g - h : 0.0;
 }
+__device__ __forceinline__ void yd_kr(const double *tab, double q, int kx, int ky, int kz, double *o) {
+    yd_kr_s(tab, YD_KP, q, kx, ky, kz, o);
+}
 
 // Coulomb and LJ parts of one atom's sin/cos at k (RhoSquared.f90:160-183):
 // the LJ part of a charged LJ atom is the Coulomb part times 1/q.
-__device__ __forceinline__ void yd_kr_atom(const double *tab, double q, int typ, int kx, int ky, int kz,
-                                           double *krc, double *krl) {
+__device__ __forceinline__ void yd_kr_atom_s(const double *tab, int ks, double q, int typ, int kx, int ky, int kz,
+                                             double *krc, double *krl) {
     if (q != 0.0) {
-        yd_kr(tab, q, kx, ky, kz, krc);
+        yd_kr_s(tab, ks, q, kx, ky, kz, krc);
         if (typ > 0) {
             double inv = 1.0 / q;
 #pragma unroll
@@ -338,8 +356,12 @@ __device__ __forceinline__ void yd_kr_atom(const double *tab, double q, int typ,
     } else {
 #pragma unroll
         for (int i = 0; i < 8; ++i) krc[i] = 0.0;
-        if (typ > 0) yd_kr(tab, 1.0, kx, ky, kz, krl);
+        if (typ > 0) yd_kr_s(tab, ks, 1.0, kx, ky, kz, krl);
     }
+}
+__device__ __forceinline__ void yd_kr_atom(const double *tab, double q, int typ, int kx, int ky, int kz,
+                                           double *krc, double *krl) {
+    yd_kr_atom_s(tab, YD_KP, q, typ, kx, ky, kz, krc, krl);
 }
 
 // One k of ForceKSpace_coulomb/_LJ (ForceKSpace.f90:116-158): C, S of the

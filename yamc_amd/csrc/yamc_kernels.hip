@@ -9,6 +9,8 @@
 // Scalar work (RNG, proposal, acceptance) runs on thread 0 in FP64; sweeps are
 // spread over the workgroup with k (or atom j) fastest for coalesced loads.
 // Nothing here is a dense contraction, so no MFMA is used.
+#include <stddef.h>
+
 #include "yamc_device.h"
 
 // Largest workgroup of the chain kernel (8 waves) and the size of the
@@ -137,14 +139,20 @@ __global__ void yk_scale_box(YdChain c, double sc) {
     }
 }
 
-// SumSinCosKR.f90:181-223 for every atom: tab[i][6][YD_KP].
-__global__ void yk_atom_tables(YdChain c, const double *x, const double *y, const double *z, double *tab) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.p.natom) return;
+// SumSinCosKR.f90:181-223 for one atom: tab[i][6][YD_KP].
+__device__ __forceinline__ void yk_atom_table1(const YdChain &c, const double *x, const double *y, const double *z,
+                                               double *tab, int i) {
     double *t = tab + (size_t)i * 6 * YD_KP;
     yd_sincos_axis(x[i], c.p.kmax, t, t + YD_KP);
     yd_sincos_axis(y[i], c.p.kmax, t + 2 * YD_KP, t + 3 * YD_KP);
     yd_sincos_axis(z[i], c.p.kmax, t + 4 * YD_KP, t + 5 * YD_KP);
+}
+
+// SumSinCosKR.f90:181-223 for every atom.
+__global__ void yk_atom_tables(YdChain c, const double *x, const double *y, const double *z, double *tab) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.p.natom) return;
+    yk_atom_table1(c, x, y, z, tab, i);
 }
 
 // RhoSquared_addAtoms (RhoSquared.f90:188-243) from scratch, in two steps:
@@ -200,36 +208,42 @@ __device__ __forceinline__ double yk_ldS(const YdChain &c, size_t idx) {
     return c.S[idx];
 }
 
-// calc_coulomb_energy + calc_LJ_energy (EwaldSumKSpace.f90:389-526), one
-// workgroup; result into st->e_kc, st->e_klj.
-__global__ void yk_kenergy(YdChain c) {
-    __shared__ double red[16 * 2], out[2];
-    int nk = c.p.nk, T = c.p.ntype;
-    double v[2] = {0.0, 0.0};
-    for (int kk = threadIdx.x; kk < nk; kk += blockDim.x) {
-        int kxyz = c.kxyz[kk], ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
-        double C[8];
-        for (int m = 0; m < 8; ++m) C[m] = yk_ldS(c, (size_t)m * nk + kk);
-        double r2 = C[0] * C[0] + C[1] * C[1];
-        if (ky > 0) r2 = r2 + C[4] * C[4] + C[5] * C[5];
-        if (kz > 0) r2 = r2 + C[2] * C[2] + C[3] * C[3];
-        if (ky > 0 && kz > 0) r2 = r2 + C[6] * C[6] + C[7] * C[7];
-        v[0] += c.beta[kk] * r2;
-        for (int t1 = 0; t1 < T; ++t1) {
-            double A[8];
-            for (int m = 0; m < 8; ++m) A[m] = yk_ldS(c, ((size_t)(1 + t1) * 8 + m) * nk + kk);
-            for (int t2 = t1; t2 < T; ++t2) {
-                double B[8];
-                for (int m = 0; m < 8; ++m) B[m] = yk_ldS(c, ((size_t)(1 + t2) * 8 + m) * nk + kk);
-                double loc = A[0] * B[0] + A[1] * B[1];
-                if (ky > 0) loc = loc + A[4] * B[4] + A[5] * B[5];
-                if (kz > 0) loc = loc + A[2] * B[2] + A[3] * B[3];
-                if (ky > 0 && kz > 0) loc = loc + A[6] * B[6] + A[7] * B[7];
-                double term = c.beta_lj[((size_t)t1 * T + t2) * nk + kk] * loc;
-                v[1] += (t1 != t2) ? term + term : term;
-            }
+// calc_coulomb_energy + calc_LJ_energy (EwaldSumKSpace.f90:389-526): the
+// terms of one k added into v[0] (Coulomb) and v[1] (LJ).
+__device__ __forceinline__ void yk_kenergy_k(const YdChain &c, int kk, double *v) {
+    const int nk = c.p.nk, T = c.p.ntype;
+    int kxyz = c.kxyz[kk], ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+    double C[8];
+    for (int m = 0; m < 8; ++m) C[m] = yk_ldS(c, (size_t)m * nk + kk);
+    double r2 = C[0] * C[0] + C[1] * C[1];
+    if (ky > 0) r2 = r2 + C[4] * C[4] + C[5] * C[5];
+    if (kz > 0) r2 = r2 + C[2] * C[2] + C[3] * C[3];
+    if (ky > 0 && kz > 0) r2 = r2 + C[6] * C[6] + C[7] * C[7];
+    v[0] += c.beta[kk] * r2;
+    for (int t1 = 0; t1 < T; ++t1) {
+        double A[8];
+        for (int m = 0; m < 8; ++m) A[m] = yk_ldS(c, ((size_t)(1 + t1) * 8 + m) * nk + kk);
+        for (int t2 = t1; t2 < T; ++t2) {
+            double B[8];
+            for (int m = 0; m < 8; ++m) B[m] = yk_ldS(c, ((size_t)(1 + t2) * 8 + m) * nk + kk);
+            double loc = A[0] * B[0] + A[1] * B[1];
+            if (ky > 0) loc = loc + A[4] * B[4] + A[5] * B[5];
+            if (kz > 0) loc = loc + A[2] * B[2] + A[3] * B[3];
+            if (ky > 0 && kz > 0) loc = loc + A[6] * B[6] + A[7] * B[7];
+            double term = c.beta_lj[((size_t)t1 * T + t2) * nk + kk] * loc;
+            v[1] += (t1 != t2) ? term + term : term;
         }
     }
+}
+
+// The k-space energies over all k, one workgroup of YK_KE_THREADS threads
+// (thread t adds k = t, t + YK_KE_THREADS, ...; then the block sum); result
+// into st->e_kc, st->e_klj.
+#define YK_KE_THREADS 1024
+__global__ void __launch_bounds__(YK_KE_THREADS) yk_kenergy(YdChain c) {
+    __shared__ double red[(YK_KE_THREADS / 64) * 2], out[2];
+    double v[2] = {0.0, 0.0};
+    for (int kk = threadIdx.x; kk < c.p.nk; kk += YK_KE_THREADS) yk_kenergy_k(c, kk, v);
     yd_block_sum<2>(v, red, out);
     if (threadIdx.x == 0) {
         c.st->e_kc = out[0];
@@ -2211,6 +2225,150 @@ __device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m
 }
 
 // ========================================================================
+// mu = sum_j q_j r_j from scratch (EwaldSumExternal.f90:165-183) over the
+// workgroup into t0.mu (thread 0).
+__device__ __forceinline__ void yk_chain_mu(const YdChain &c, YkShared &sh, int tid) {
+    double v[3] = {0, 0, 0};
+    for (int i = tid; i < c.p.natom; i += (int)blockDim.x) {
+        double q = c.q[i];
+        v[0] += q * c.x[i]; v[1] += q * c.y[i]; v[2] += q * c.z[i];
+    }
+    yd_block_sum<3>(v, sh.red, sh.out);
+    if (tid == 0) {
+        sh.t0.mu[0] = sh.out[0]; sh.t0.mu[1] = sh.out[1]; sh.t0.mu[2] = sh.out[2];
+    }
+}
+
+static_assert(offsetof(YkShared, bm) % sizeof(double) == 0, "the rebuild's LDS tables live in bm");
+static_assert(sizeof(YkShared::red) >= sizeof(double) * 2 * (YK_KE_THREADS / 64), "red holds the energy sums");
+static_assert(sizeof(YkShared::red) >= 32 * sizeof(double) && sizeof(YkShared::out) >= 32 * sizeof(double),
+              "red and out hold a rebuild chunk's charges and inverse charges");
+
+// The periodic from-scratch rebuild of a chain inside its own launch
+// (calc_total_energy, EwaldSumKSpace.f90:225-260, plus mu): what a
+// launch-level rebuild (yk_atom_tables, yk_full_sums, yk_full_sums_reduce,
+// yk_kenergy) followed by a new launch computes, with the same additions in
+// the same order, so a chain rebuilt here equals the chain run in segments
+// with rebuilds between them bit for bit.
+// S(k): for each set and tile of blockDim consecutive k (one thread per k),
+// the set's atoms in list order, group by group (c.fs_ng equal groups, each
+// group's sum added to the total in group order), the axis tables of each
+// chunk of atoms built in LDS (the scan bitmap's space, free between moves).
+// E_kC, E_kLJ: yk_kenergy's per-thread k strides and block sum of
+// YK_KE_THREADS threads, each thread here standing in for
+// YK_KE_THREADS / blockDim of them.  Then mu and the cell buckets, as at
+// the start of a launch.
+__device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid) {
+    const int bs = (int)blockDim.x;
+    const int nk = c.p.nk, T = c.p.ntype, kmax = c.p.kmax, kp = kmax + 1;
+    double *tb = reinterpret_cast<double *>(sh.bm);
+    double *qs = sh.red;   // chunk charges
+    double *iq = sh.out;   // chunk 1/q (charged atoms of an LJ set)
+    int C = (int)(sizeof(sh.bm) / (sizeof(double) * 6 * (size_t)kp));
+    if (C > 32) C = 32;
+    for (int set = 0; set <= T; ++set) {
+        const int l0 = c.set_off[set], l1 = c.set_off[set + 1], ng = c.fs_ng;
+        const int gsize = (l1 - l0 + ng - 1) / ng;
+        for (int k0 = 0; k0 < nk; k0 += bs) {
+            const int kk = k0 + tid;
+            const bool act = kk < nk;
+            int kx = 0, ky = 0, kz = 0;
+            if (act) {
+                const int kxyz = c.kxyz[kk];
+                kx = kxyz & 255; ky = (kxyz >> 8) & 255; kz = (kxyz >> 16) & 255;
+            }
+            int gend = l0 + gsize;  // end of the current group
+            double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0}, sg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int base = l0; base < l1; base += C) {
+                const int nch = l1 - base < C ? l1 - base : C;
+                __syncthreads();  // the previous chunk's tables are no longer read
+                if (tid < 3 * nch) {
+                    const int a = tid / 3, d = tid % 3;
+                    const int at = c.set_atoms[base + a];
+                    const double r = d == 0 ? c.x[at] : d == 1 ? c.y[at] : c.z[at];
+                    double *t = tb + (size_t)a * 6 * kp + 2 * d * kp;
+                    yd_sincos_axis(r, kmax, t, t + kp);
+                    if (d == 0) {
+                        const double q = c.q[at];
+                        qs[a] = q;
+                        iq[a] = q != 0.0 ? 1.0 / q : 0.0;
+                    }
+                }
+                __syncthreads();
+                if (act) {
+                    for (int a = 0; a < nch; ++a) {
+                        if (base + a == gend) {  // group boundary: the group's sum into the total
+#pragma unroll
+                            for (int m = 0; m < 8; ++m) {
+                                tot[m] = tot[m] + sg[m];
+                                sg[m] = 0.0;
+                            }
+                            gend += gsize;
+                        }
+                        // yd_kr_atom's values for this set: the Coulomb set
+                        // holds charged atoms only (q sin/cos); an LJ set takes
+                        // (q sin/cos) (1/q) of a charged atom, else sin/cos
+                        const double q = qs[a];
+                        double kr[8];
+                        yd_kr_s(tb + (size_t)a * 6 * kp, kp, set == 0 || q != 0.0 ? q : 1.0, kx, ky, kz, kr);
+                        if (set > 0 && q != 0.0) {
+                            const double inv = iq[a];
+#pragma unroll
+                            for (int m = 0; m < 8; ++m) kr[m] = kr[m] * inv;
+                        }
+#pragma unroll
+                        for (int m = 0; m < 8; ++m) sg[m] = sg[m] + kr[m];
+                    }
+                }
+            }
+            if (act) {
+                if (l1 > l0) {
+#pragma unroll
+                    for (int m = 0; m < 8; ++m) tot[m] = tot[m] + sg[m];  // the last non-empty group
+                }
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const size_t idx = ((size_t)set * 8 + m) * nk + kk;
+                    if (c.p.precision == YAMC_FP32C) {
+                        const float hi = (float)tot[m];
+                        c.Shi[idx] = hi;
+                        c.Slo[idx] = (float)(tot[m] - (double)hi);
+                    } else {
+                        c.S[idx] = tot[m];
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();  // S(k) complete (workgroup-scope visibility of the stores)
+    {
+        const int J = YK_KE_THREADS / bs, nw = bs >> 6, lane = tid & 63, w = tid >> 6;
+        for (int j = 0; j < J; ++j) {
+            double v[2] = {0.0, 0.0};
+            for (int kk = tid + bs * j; kk < nk; kk += YK_KE_THREADS) yk_kenergy_k(c, kk, v);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                double a = v[i];
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+                if (lane == 0) sh.red[(w + nw * j) * 2 + i] = a;
+            }
+        }
+        __syncthreads();
+        if (tid < 2) {
+            double a = 0.0;
+            for (int vw = 0; vw < YK_KE_THREADS / 64; ++vw) a += sh.red[vw * 2 + tid];
+            if (tid == 0) c.st->e_kc = a;
+            else c.st->e_klj = a;
+        }
+        __syncthreads();
+    }
+    yk_chain_mu(c, sh, tid);
+    if (c.nc > 0) yk_cells_build(c, &sh.cell_ok);
+    __syncthreads();
+}
+
+// ========================================================================
 // The workgroup chain kernel: one workgroup per chain, every per-move sum
 // spread over the workgroup.  mode 0: run_mc cycles (runmc.f90:204-268)
 // with the Philox stream, nsteps = cycles; mode 1: nsteps moves with
@@ -2220,13 +2378,15 @@ __device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m
 template <bool F32, int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
 yk_chain(const YdChain *chains, const double *const *sig6s, const double *const *feps_s, int nsteps, int mode,
-         int max_moves, int lds_atoms) {
+         int max_moves, int lds_atoms, int resync_every) {
     __shared__ YkShared sh;
     extern __shared__ unsigned int lds_pos[];  // packed positions when the launch provides LDS for them
     const YdChain &c = chains[blockIdx.x];
     const double *sig6 = sig6s[blockIdx.x], *feps = feps_s[blockIdx.x];
     const int tid = threadIdx.x;
     const YdParams &p = c.p;
+    int total = mode == 1 ? nsteps : nsteps * p.nmol;
+    if (max_moves > 0 && total > max_moves) total = max_moves;
     const bool use_lds = lds_atoms >= p.natom;
     // packed copy of the positions for the real-space scan (yk_pack_pos): in
     // LDS when the launch gives room for it, else in the context's pos4 buffer
@@ -2245,16 +2405,8 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
     // mu = sum_j q_j r_j from scratch once per launch (EwaldSumExternal.f90:
     // 165-183); afterwards it is carried incrementally, mu + dmu on each
     // accepted move, which is the value movemc recomputes (MC.f90:916).
-    {
-        double v[3] = {0, 0, 0};
-        for (int i = tid; i < p.natom; i += (int)blockDim.x) {
-            double q = c.q[i];
-            v[0] += q * c.x[i]; v[1] += q * c.y[i]; v[2] += q * c.z[i];
-        }
-        yd_block_sum<3>(v, sh.red, sh.out);
-    }
+    yk_chain_mu(c, sh, tid);
     if (tid == 0) {
-        t0.mu[0] = sh.out[0]; t0.mu[1] = sh.out[1]; t0.mu[2] = sh.out[2];
         for (int i = 0; i < 9; ++i) sh.ph[i] = 0;
         sh.t_last = wall_clock64();
         t0.acc_count = 0; t0.nmv = 0; t0.nov = 0;
@@ -2262,10 +2414,12 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         rng.key0 = c.st->key[0]; rng.key1 = c.st->key[1];
         rng.s0 = c.st->stream[0]; rng.s1 = c.st->stream[1];
         rng.ctr = c.st->ctr; rng.constant = mode == 1; rng.cu = 0.0;
+        rng.list = c.st->ulist; rng.nlist = c.st->nlist;
     }
-    int total = mode == 1 ? nsteps : nsteps * p.nmol;
-    if (max_moves > 0 && total > max_moves) total = max_moves;
     for (int mv = 0; mv < total; ++mv) {
+        // the periodic FP64 rebuild between move resync_every k - 1 and
+        // resync_every k (workgroup-uniform)
+        if (resync_every > 0 && mv > 0 && mv % resync_every == 0) yk_chain_rebuild(c, sh, tid);
         if (tid == 0) {
             __builtin_amdgcn_s_setprio(3);
             yk_sec_pick(c, sh.m, t0.rng, mode, mv);
@@ -2469,7 +2623,7 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
 
 #define YK_CHAIN_INST(F, B)                                                                                    \
     template __global__ void yk_chain<F, B>(const YdChain *, const double *const *, const double *const *, int, int, \
-                                            int, int);
+                                            int, int, int);
 YK_CHAIN_INST(false, 256)
 YK_CHAIN_INST(false, 512)
 YK_CHAIN_INST(true, 256)
@@ -2591,6 +2745,7 @@ yk_wchain(const YdChain *chains, int nchains,
         rng.key0 = c.st->key[0]; rng.key1 = c.st->key[1];
         rng.s0 = c.st->stream[0]; rng.s1 = c.st->stream[1];
         rng.ctr = c.st->ctr; rng.constant = 0; rng.cu = 0.0;
+        rng.list = c.st->ulist; rng.nlist = c.st->nlist;
     }
     yw_sync();
     const int total = ncycles * p.nmol;

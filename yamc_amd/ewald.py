@@ -187,6 +187,12 @@ class EwaldMC:
     def set_rng(self, seed: int, stream: int = 0, counter: int = 0):
         check(self.lib.yamc_mc_set_rng(self.ctx, C.c_uint64(seed), C.c_uint64(stream), C.c_uint64(counter)))
 
+    def set_rng_list(self, u: Sequence[float]) -> None:
+        """Replay mode: the recorded uniforms u are drawn in order instead
+        of the Philox stream (yamc_mc_set_rng_list)."""
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        check(self.lib.yamc_mc_set_rng_list(self.ctx, dptr(uu), C.c_int64(len(uu))))
+
     def rng_counter(self) -> int:
         v = C.c_uint64()
         check(self.lib.yamc_mc_get_rng(self.ctx, C.byref(v)))
@@ -313,12 +319,14 @@ class EwaldMC:
         return v.value
 
 
-def batch_run(reps: Sequence[EwaldMC], ncycles: int):
-    """yamc_batch_run: all replicas advance in one persistent launch."""
+def batch_run(reps: Sequence[EwaldMC], ncycles: int, resync_every: int = 0):
+    """yamc_batch_run: all replicas advance in one persistent launch (or in
+    segments of resync_every moves with batched S(k) rebuilds between)."""
     lib = _lib.load()
     arr = (C.c_void_p * len(reps))(*[r.ctx.value for r in reps])
     nacc = np.zeros(len(reps), dtype=np.int64)
-    check(lib.yamc_batch_run(arr, len(reps), int(ncycles), nacc.ctypes.data_as(C.POINTER(C.c_int64))))
+    check(lib.yamc_batch_run(arr, len(reps), int(ncycles), int(resync_every),
+                             nacc.ctypes.data_as(C.POINTER(C.c_int64))))
     ms = reps[0].last_kernel_ms()
     return nacc, ms
 
@@ -327,12 +335,12 @@ class Batch:
     """yamc_batch_create/launch/destroy: one sweep of ncycles cycles of all
     replicas captured in a HIP graph and replayed per launch()."""
 
-    def __init__(self, reps: Sequence[EwaldMC], ncycles: int = 1):
+    def __init__(self, reps: Sequence[EwaldMC], ncycles: int = 1, resync_every: int = 0):
         self.lib = _lib.load()
         self.reps = list(reps)
         arr = (C.c_void_p * len(self.reps))(*[r.ctx.value for r in self.reps])
         h = C.c_void_p()
-        check(self.lib.yamc_batch_create(arr, len(self.reps), int(ncycles), C.byref(h)))
+        check(self.lib.yamc_batch_create(arr, len(self.reps), int(ncycles), int(resync_every), C.byref(h)))
         self.h = h
 
     def launch(self):
