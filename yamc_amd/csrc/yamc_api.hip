@@ -135,6 +135,24 @@ static int dalloc(yamc_ctx *c, T **p, size_t n) {
         if (r_ != YAMC_OK) return r_; \
     } while (0)
 
+// Stream-ordered device scratch, freed (stream-ordered) on every exit path.
+struct StreamScratch {
+    hipStream_t s;
+    std::vector<void *> held;
+    explicit StreamScratch(hipStream_t st) : s(st) {}
+    ~StreamScratch() {
+        for (void *q : held) (void)hipFreeAsync(q, s);
+    }
+    template <typename T>
+    int get(T **out, size_t n) {
+        void *q = nullptr;
+        HIPCHK(hipMallocAsync(&q, sizeof(T) * (n ? n : 1), s));
+        held.push_back(q);
+        *out = (T *)q;
+        return YAMC_OK;
+    }
+};
+
 // FourierGrid.f90:118-189: half-space k list, row flags ip, zero pattern iq.
 static void build_grid(int kmax, std::vector<int> &kx, std::vector<int> &ky, std::vector<int> &kz,
                        std::vector<int> &ip, std::vector<int> &iq, std::vector<double> &xk2) {
@@ -421,11 +439,11 @@ static int total_energy(yamc_ctx *c) {
     yk_atom_tables<<<(N + 255) / 256, 256, 0, s>>>(c->ch, c->ch.x, c->ch.y, c->ch.z, c->d_tab);
     const int nsk = (1 + T) * nk;
     const int ng = c->ch.fs_ng;
+    StreamScratch sc(s);
     double *part = nullptr;
-    HIPCHK(hipMallocAsync((void **)&part, sizeof(double) * 8 * (size_t)nsk * ng, s));
+    TRY(sc.get(&part, 8 * (size_t)nsk * ng));
     yk_full_sums<<<dim3((nsk + 127) / 128, ng), 128, 0, s>>>(c->ch, c->d_tab, ng, part);
     yk_full_sums_reduce<<<(8 * nsk + 255) / 256, 256, 0, s>>>(c->ch, part, ng);
-    HIPCHK(hipFreeAsync(part, s));
     yk_kenergy<<<1, 1024, 0, s>>>(c->ch);
     HIPCHK(hipGetLastError());
     return YAMC_OK;
@@ -552,22 +570,36 @@ extern "C" int yamc_kspace_forces(yamc_ctx *c, int32_t first, int32_t n, const d
     return YAMC_OK;
 }
 
-static int rspace_full(yamc_ctx *c, double *comps, int32_t *overlap) {
+// Launch shape of the full real-space sum for N atoms: nb blocks of 256
+// atoms, ng partner groups (of at least 64 atoms, about 2^19 (atom, group)
+// threads, at most 64); its scratch is part[ng][nb][6] and pat[ng][4][N].
+static void rspace_full_shape(int N, int *nb, int *ng) {
+    *nb = (N + 255) / 256;
+    int g = (1 << 19) / N;
+    g = g < 1 ? 1 : g > 64 ? 64 : g;
+    if (g > N / 64) g = N / 64 > 1 ? N / 64 : 1;
+    *ng = g;
+}
+
+static size_t rspace_full_scratch_doubles(int N) {
+    int nb, ng;
+    rspace_full_shape(N, &nb, &ng);
+    return 6 * (size_t)nb * ng + 4 * (size_t)N * ng;
+}
+
+// EwaldSumRealSpace_calc, full mode, with the caller's scratch (at least
+// rspace_full_scratch_doubles(N) doubles): every value of it is written
+// before it is read, so it needs no clearing.
+static int rspace_full_run(yamc_ctx *c, double *scratch, double *comps, int32_t *overlap) {
     hipStream_t s = c->stream;
-    int N = c->natom, nb = (N + 255) / 256;
-    // partner groups of at least 64 atoms, about 2^19 (atom, group) threads, at most 64
-    int ng = (1 << 19) / N;
-    ng = ng < 1 ? 1 : ng > 64 ? 64 : ng;
-    if (ng > N / 64) ng = N / 64 > 1 ? N / 64 : 1;
-    double *part = nullptr, *pat = nullptr;
-    HIPCHK(hipMallocAsync((void **)&part, sizeof(double) * 6 * (size_t)nb * ng, s));
-    HIPCHK(hipMallocAsync((void **)&pat, sizeof(double) * 4 * (size_t)N * ng, s));
+    const int N = c->natom;
+    int nb, ng;
+    rspace_full_shape(N, &nb, &ng);
+    double *part = scratch, *pat = scratch + 6 * (size_t)nb * ng;
     HIPCHK(hipMemsetAsync(c->d_ov, 0, sizeof(int), s));
     yk_rspace_full<<<dim3(nb, ng), 256, 0, s>>>(c->ch, c->d_sig6, c->d_4eps, ng, part, pat, c->d_ov);
     yk_rspace_full_atoms<<<nb, 256, 0, s>>>(c->ch, pat, ng);
     yk_rspace_full_final<<<1, 256, 0, s>>>(c->ch, part, nb * ng, c->d_io);
-    HIPCHK(hipFreeAsync(part, s));
-    HIPCHK(hipFreeAsync(pat, s));
     HIPCHK(hipGetLastError());
     double o[6];
     int ov = 0;
@@ -578,6 +610,13 @@ static int rspace_full(yamc_ctx *c, double *comps, int32_t *overlap) {
         for (int m = 0; m < 6; ++m) comps[m] = o[m];
     if (overlap) *overlap = ov;
     return YAMC_OK;
+}
+
+static int rspace_full(yamc_ctx *c, double *comps, int32_t *overlap) {
+    StreamScratch sc(c->stream);
+    double *scratch = nullptr;
+    TRY(sc.get(&scratch, rspace_full_scratch_doubles(c->natom)));
+    return rspace_full_run(c, scratch, comps, overlap);
 }
 
 extern "C" int yamc_rspace_full(yamc_ctx *c, double *comps, double *uu, double *fx, double *fy, double *fz,
@@ -598,7 +637,8 @@ extern "C" int yamc_rspace_partial(yamc_ctx *c, int32_t imol, const double *x, c
     int N = c->natom;
     double *back = nullptr;
     bool want_back = uu_back || fx_back || fy_back || fz_back;
-    if (want_back) HIPCHK(hipMallocAsync((void **)&back, sizeof(double) * 4 * N, c->stream));
+    StreamScratch sc(c->stream);
+    if (want_back) TRY(sc.get(&back, 4 * (size_t)N));
     const double *dm = c->d_mol;
     yk_rspace_partial<<<1, YK_BS, 0, c->stream>>>(c->ch, imol, dm, dm + YD_MAXA, dm + 2 * YD_MAXA, c->d_sig6,
                                                   c->d_4eps, c->d_io, back, back ? back + N : nullptr,
@@ -610,7 +650,6 @@ extern "C" int yamc_rspace_partial(yamc_ctx *c, int32_t imol, const double *x, c
     if (want_back) {
         hb.resize(4 * (size_t)N);
         HIPCHK(hipMemcpyAsync(hb.data(), back, sizeof(double) * 4 * N, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipFreeAsync(back, c->stream));
     }
     TRY(sync(c));
     for (int a = 0; a < na; ++a) {
@@ -857,7 +896,8 @@ extern "C" int yamc_mc_run(yamc_ctx *c, int32_t ncycles, int32_t resync_every, y
     }
     yamc_move_rec *d_rec = nullptr;
     size_t nrec = (size_t)ncycles * c->nmol;
-    if (recs && nrec) HIPCHK(hipMallocAsync((void **)&d_rec, sizeof(yamc_move_rec) * nrec, c->stream));
+    StreamScratch sc(c->stream);
+    if (recs && nrec) TRY(sc.get(&d_rec, nrec));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     if (resync_every > 0 && c->pxch <= 0) {
         // segments of resync_every moves, each resumed where the last one
@@ -892,7 +932,6 @@ extern "C" int yamc_mc_run(yamc_ctx *c, int32_t ncycles, int32_t resync_every, y
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     if (d_rec) {
         HIPCHK(hipMemcpyAsync(recs, d_rec, sizeof(yamc_move_rec) * nrec, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipFreeAsync(d_rec, c->stream));
     }
     YdState st1;
     HIPCHK(hipMemcpyAsync(&st1, c->ch.st, sizeof st1, hipMemcpyDeviceToHost, c->stream));
@@ -915,11 +954,12 @@ extern "C" int yamc_mc_run_const(yamc_ctx *c, int32_t nmoves, const int32_t *imo
     int *d_imol = nullptr;
     double *d_u = nullptr;
     yamc_move_rec *d_rec = nullptr;
-    HIPCHK(hipMallocAsync((void **)&d_imol, sizeof(int) * (nmoves + 1), c->stream));
-    HIPCHK(hipMallocAsync((void **)&d_u, sizeof(double) * (nmoves + 1), c->stream));
+    StreamScratch sc(c->stream);
+    TRY(sc.get(&d_imol, (size_t)nmoves + 1));
+    TRY(sc.get(&d_u, (size_t)nmoves + 1));
     HIPCHK(hipMemcpyAsync(d_imol, imol, sizeof(int) * nmoves, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(d_u, u, sizeof(double) * nmoves, hipMemcpyHostToDevice, c->stream));
-    if (recs) HIPCHK(hipMallocAsync((void **)&d_rec, sizeof(yamc_move_rec) * (nmoves + 1), c->stream));
+    if (recs) TRY(sc.get(&d_rec, (size_t)nmoves + 1));
     YdChain ch = c->ch;
     ch.rec = d_rec; ch.cimol = d_imol; ch.cu = d_u;
     HIPCHK(hipMemcpyAsync(c->d_chain, &ch, sizeof ch, hipMemcpyHostToDevice, c->stream));
@@ -930,9 +970,6 @@ extern "C" int yamc_mc_run_const(yamc_ctx *c, int32_t nmoves, const int32_t *imo
     YdState st1;
     HIPCHK(hipMemcpyAsync(&st1, c->ch.st, sizeof st1, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_chain, &c->ch, sizeof(YdChain), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipFreeAsync(d_imol, c->stream));
-    HIPCHK(hipFreeAsync(d_u, c->stream));
-    if (d_rec) HIPCHK(hipFreeAsync(d_rec, c->stream));
     TRY(sync(c));
     HIPCHK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
     if (n_accepted) *n_accepted = st1.n_accepted - st0.n_accepted;
@@ -1129,10 +1166,11 @@ extern "C" int yamc_forces(yamc_ctx *c, double *gx, double *gy, double *gz, doub
     if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
     if (!c->state_ready) return fail(YAMC_ERROR_INITIALIZATION, "yamc_mc_init has not run on this state");
     int N = c->natom;
-    double *dg = nullptr, *dr = nullptr;
-    HIPCHK(hipMallocAsync((void **)&dg, sizeof(double) * 3 * N, c->stream));
-    HIPCHK(hipMallocAsync((void **)&dr, sizeof(double) * 3 * N, c->stream));
     hipStream_t s = c->stream;
+    StreamScratch sc(s);
+    double *dg = nullptr, *dr = nullptr;
+    TRY(sc.get(&dg, 3 * (size_t)N));
+    TRY(sc.get(&dr, 3 * (size_t)N));
     yk_atom_tables<<<(N + 255) / 256, 256, 0, s>>>(c->ch, c->ch.x, c->ch.y, c->ch.z, c->d_tab);
     HIPCHK(hipGetLastError());
     double mu[3];
@@ -1143,8 +1181,6 @@ extern "C" int yamc_forces(yamc_ctx *c, double *gx, double *gy, double *gz, doub
     std::vector<double> hg(3 * (size_t)N), hr(3 * (size_t)N);
     HIPCHK(hipMemcpyAsync(hg.data(), dg, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(hr.data(), dr, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipFreeAsync(dg, s));
-    HIPCHK(hipFreeAsync(dr, s));
     TRY(sync(c));
     for (int i = 0; i < N; ++i) {
         if (gx) { gx[i] = hg[i]; gy[i] = hg[N + i]; gz[i] = hg[2 * N + i]; }
@@ -1186,18 +1222,16 @@ extern "C" int yamc_rdf(yamc_ctx *c, const int32_t *mol_species, int32_t nspecie
     double *com = nullptr;
     int *sp = nullptr;
     unsigned long long *hist = nullptr;
-    HIPCHK(hipMallocAsync((void **)&com, sizeof(double) * 3 * c->nmol, s));
-    HIPCHK(hipMallocAsync((void **)&sp, sizeof(int) * c->nmol, s));
-    HIPCHK(hipMallocAsync((void **)&hist, sizeof(unsigned long long) * nh, s));
+    StreamScratch sc(s);
+    TRY(sc.get(&com, 3 * (size_t)c->nmol));
+    TRY(sc.get(&sp, (size_t)c->nmol));
+    TRY(sc.get(&hist, nh));
     HIPCHK(hipMemcpyAsync(sp, mol_species, sizeof(int) * c->nmol, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemsetAsync(hist, 0, sizeof(unsigned long long) * nh, s));
     yk_launch_rdf(c->ch, com, sp, nspecies, dr_angstr, rmax_angstr * rmax_angstr, nbins, hist, s);
     HIPCHK(hipGetLastError());
     std::vector<unsigned long long> h(nh);
     HIPCHK(hipMemcpyAsync(h.data(), hist, sizeof(unsigned long long) * nh, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipFreeAsync(com, s));
-    HIPCHK(hipFreeAsync(sp, s));
-    HIPCHK(hipFreeAsync(hist, s));
     TRY(sync(c));
     for (size_t k = 0; k < nh; ++k) counts[k] += h[k];
     return YAMC_OK;
