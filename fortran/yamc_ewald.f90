@@ -100,6 +100,14 @@ module yamc_ewald
          type(c_ptr), value :: uu, fx, fy, fz
          integer(c_int32_t), intent(out) :: overlap
       end function
+      integer(c_int) function yamc_rspace_full_noalloc(ctx, comps, uu, fx, fy, fz, overlap) &
+            bind(C, name="yamc_rspace_full_noalloc")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: comps(6)
+         type(c_ptr), value :: uu, fx, fy, fz
+         integer(c_int32_t), intent(out) :: overlap
+      end function
       integer(c_int) function yamc_rspace_partial(ctx, imol, x, y, z, comps, uu, fx, fy, fz, uu_back, fx_back, &
                                                   fy_back, fz_back, overlap) bind(C, name="yamc_rspace_partial")
          import :: c_int, c_ptr, c_int32_t, c_double

@@ -142,6 +142,16 @@ int yamc_kspace_forces(yamc_ctx *ctx, int32_t first, int32_t n, const double *x,
  * uu, fx, fy, fz over natom (may be NULL). */
 int yamc_rspace_full(yamc_ctx *ctx, double *comps, double *uu, double *fx, double *fy, double *fz,
                      int32_t *overlap);
+/* yamc_rspace_full with no allocation at all: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it).  Its
+ * scratch is a static device array of 24 MiB that each GPU holds until the
+ * process ends; systems whose full sum needs more (above about 780,000
+ * atoms) get ERROR_LIMITS.  Calls on one GPU share that array: calls that
+ * overlap in time on one GPU (two host threads, two contexts) race.  Same
+ * results as yamc_rspace_full. */
+int yamc_rspace_full_noalloc(yamc_ctx *ctx, double *comps, double *uu, double *fx, double *fy, double *fz,
+                             int32_t *overlap);
 /* EwaldSumRealSpace_set_molecule + _calc, partial sum of molecule imol at
  * (x,y,z): uu/F over its n_m atoms, *_back over all natom atoms (may be
  * NULL), comps[6] as above. */

@@ -25,7 +25,7 @@ ERRORS = {1: "ERROR_IO", 2: "ERROR_LIMITS", 3: "ERROR_PARAMETER", 4: "ERROR_INIT
 EXPORTS = [
     "yamc_ctx_create", "yamc_ctx_destroy", "yamc_ctx_info", "yamc_set_positions", "yamc_get_positions",
     "yamc_grid", "yamc_kspace_init", "yamc_kspace_get_beta", "yamc_kspace_get_sums", "yamc_kspace_delta",
-    "yamc_kspace_forces", "yamc_rspace_full", "yamc_rspace_partial", "yamc_tails", "yamc_external_mu",
+    "yamc_kspace_forces", "yamc_rspace_full", "yamc_rspace_full_noalloc", "yamc_rspace_partial", "yamc_tails", "yamc_external_mu",
     "yamc_mc_init", "yamc_mc_energies", "yamc_mc_real_arrays", "yamc_mc_set_rng", "yamc_mc_set_rng_list",
     "yamc_mc_get_rng",
     "yamc_mc_run", "yamc_mc_run_const", "yamc_resync", "yamc_mc_volume_move", "yamc_mc_run_npt", "yamc_box_length",

@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "yamc_device.h"
@@ -624,6 +625,51 @@ extern "C" int yamc_rspace_full(yamc_ctx *c, double *comps, double *uu, double *
     if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
     TRY(rspace_full(c, comps, overlap));
     c->state_ready = false;  // uu/F/components were overwritten; k-space and tails still need mc_init
+    return yamc_mc_real_arrays(c, uu, fx, fy, fz);
+}
+
+// ---- the no-allocation variant of yamc_rspace_full ----------------------
+// Its scratch (part + pat of the full real-space sum) is a file-scope device
+// array: each GPU the library runs on holds one copy of it until the process
+// ends.  The array's name is only a host-side stand-in; its device address
+// on each GPU comes from hipGetSymbolAddress, taken once per device with
+// that device current, and is the only handle host code passes on.
+#define YAMC_NOALLOC_SCRATCH_DOUBLES (3u << 20)  // 24 MiB
+#define YAMC_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[YAMC_NOALLOC_SCRATCH_DOUBLES];
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[YAMC_NOALLOC_MAX_DEVICES] = {};
+
+static int scratch_ptr(int device, double **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= YAMC_NOALLOC_MAX_DEVICES) return fail(YAMC_ERROR_DEVICE, "device %d out of range", device);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != device) HIPCHK(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(YAMC_ERROR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        g_scratch_ptr[device] = (double *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return YAMC_OK;
+}
+
+extern "C" int yamc_rspace_full_noalloc(yamc_ctx *c, double *comps, double *uu, double *fx, double *fy, double *fz,
+                                        int32_t *overlap) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    const size_t need = rspace_full_scratch_doubles(c->natom);
+    if (need > YAMC_NOALLOC_SCRATCH_DOUBLES)
+        return fail(YAMC_ERROR_LIMITS,
+                    "yamc_rspace_full_noalloc: %d atoms need %zu doubles of scratch, the static buffer holds %u "
+                    "(use yamc_rspace_full)", c->natom, need, (unsigned)YAMC_NOALLOC_SCRATCH_DOUBLES);
+    double *sp = nullptr;
+    TRY(scratch_ptr(c->device, &sp));
+    TRY(rspace_full_run(c, sp, comps, overlap));
+    c->state_ready = false;  // as yamc_rspace_full
     return yamc_mc_real_arrays(c, uu, fx, fy, fz);
 }
 
