@@ -134,8 +134,11 @@ def test_mc_trace_matches_oracle(lib, name, ncyc, block):
     for k in ("du_real", "du_kc", "du_klj", "du_ext"):
         assert rel(rg[k][ok], ro[k][ok], scale) < RTOL, k
     for k in ("f_old", "t_old", "f_new", "t_new"):
-        fs = np.abs(ro[k]).max()
-        assert rel(rg[k][ok], ro[k][ok], fs) < 1e-8, k
+        # per component, relative to the component or, for components much
+        # smaller than their vector, to 1e-4 of the vector's length
+        a, b = rg[k][ok], ro[k][ok]
+        den = np.maximum(np.abs(b), 1e-4 * np.linalg.norm(b, axis=1, keepdims=True))
+        assert float(np.max(np.abs(a - b) / np.maximum(den, 1e-300))) < 1e-8, k
     assert g.rng_counter() == o.rng_counter()
     # state after the run
     eg, eo = g.energies(), o.energies()
@@ -333,9 +336,10 @@ def test_forces_reference_convention_and_gradient(lib):
 
 
 def test_fp32c_error_is_bounded(lib):
-    """FP32-compensated form against FP64 on the same stream: errors are
-    reported, and bounded loosely here (the measured figures are in the
-    benchmark line)."""
+    """FP32-compensated form against FP64 on the same stream (256 waters, no
+    force bias; the force-biased 2128-molecule case is in
+    test_gpu_configs.py): identical decisions over the whole sweep, per-move
+    dE within 1e-6 kJ/mol, forces within 1e-5."""
     from yamc_amd import config
     from yamc_amd.ewald import EwaldMC
     s = config("c1", seed=3)
@@ -347,8 +351,9 @@ def test_fp32c_error_is_bounded(lib):
     a.set_rng(3); b.set_rng(3)
     _, ra = a.run(1, records=True)
     _, rb = b.run(1, records=True)
-    n = np.argmax(ra["accepted"] != rb["accepted"]) if np.any(ra["accepted"] != rb["accepted"]) else len(ra)
-    assert n > 10
+    assert np.array_equal(ra["imol"], rb["imol"])
+    assert np.array_equal(ra["accepted"], rb["accepted"]), "decisions differ"
+    n = len(ra)
     ok = ra["overlap"][:n] == 0
     d = np.abs(ra["du_total"][:n] - rb["du_total"][:n])[ok]
     # per-move dE within 1e-6 kJ/mol = 4.04e-7 kT at 298 K
