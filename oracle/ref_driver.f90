@@ -65,6 +65,7 @@ program ref_driver
    integer :: nmol, i, j, natom, nk, t1, t2, nt, nmoves, imol, species, f, l, uin, nvol, ios, imol2, nxchg
    integer :: nb, hfile
    integer :: nrun, rseed, nstream, nfreq, ff, fl
+   real(8), allocatable :: fkx(:), fky(:), fkz(:), fex(:), fey(:), fez(:)
    real(8) :: psm, wm, wx
    character(SYSTEM_STRING_LENGTH) :: fname
    real(8), dimension(10) :: specials = (/ 0.d0, 1.d0, -1.d0, 0.75d0, -3.5d0, 2.d0, 1.d-15, -1.d15, 123456.789d0, &
@@ -159,6 +160,7 @@ program ref_driver
    call RhoSquared_nulify(scratch)
    call SinCosKR_alloc(krc, KSpace_TotalSum % grid)
    call SinCosKR_alloc(krl, KSpace_TotalSum % grid)
+   allocate(fkx(natom), fky(natom), fkz(natom), fex(natom), fey(natom), fez(natom))
    write(*, '(A,I10)') 'FK', natom
    do i = 1, natom
       call RhoSquared_addAtom(scratch, atomic_data % xx(i), atomic_data % yy(i), atomic_data % zz(i), &
@@ -171,7 +173,26 @@ program ref_driver
                              lj_types % type_by_index(i), krl, fxl, fyl, fzl)
       end if
       write(*, '(6ES26.17)') fxc, fyc, fzc, fxl, fyl, fzl
+      fkx(i) = fxc + fxl
+      fky(i) = fyc + fyl
+      fkz(i) = fzc + fzl
    end do
+   ! the forces of this configuration in mc_calc_forces' output layout
+   ! (store_forces, mc_calc_forces.f90:348-368: per atom F/L x, y, z as
+   ! 24+8-bit floats), with the movemc real-space convention (pairs j > i,
+   ! RealSpace_TotalSum%F) instead of calc_forces' first = 1 sum, plus the
+   ! k-space and external forces as calc_forces adds them
+   call EwaldSumExternal_calc_mu(Ext_TotalSum)
+   call external_sum_calc_forces(Ext_TotalSum % mu_x, Ext_TotalSum % mu_y, Ext_TotalSum % mu_z, &
+                                 atomic_data % charge, natom, fex, fey, fez)
+   fname = 'forces.ftraj'
+   hfile = io_open(fname, 'b')
+   do i = 1, natom
+      call write_float(hfile, (RealSpace_TotalSum % Fx(i) + fkx(i) + fex(i)) / BoxLength)
+      call write_float(hfile, (RealSpace_TotalSum % Fy(i) + fky(i) + fey(i)) / BoxLength)
+      call write_float(hfile, (RealSpace_TotalSum % Fz(i) + fkz(i) + fez(i)) / BoxLength)
+   end do
+   call io_close(hfile)
 
    ! move trace with one constant uniform per move
    read(uin, *) nmoves

@@ -113,7 +113,7 @@ def make_case(name, sysm, nmoves, seed, vols=(), xchgs=()):
         refcase.write_case(td, sysm, imol, u, vols, xchgs)
         stdout, _ = refcase.run(td, timeout=120)
         blobs = {}
-        for fn in ("moltab1.bin", "moltab2.bin", "moltab3.bin", "floats.bin"):
+        for fn in ("moltab1.bin", "moltab2.bin", "moltab3.bin", "floats.bin", "forces.ftraj"):
             with open(os.path.join(td, fn), "rb") as fb:
                 blobs[fn] = np.frombuffer(fb.read(), dtype=np.uint8).copy()
     g = parse(stdout)
@@ -174,7 +174,7 @@ def make_case(name, sysm, nmoves, seed, vols=(), xchgs=()):
     arrays.update(
         moltab_final=np.array(g["MOLTAB"]), moltab_box=np.float64(g["MOLTAB_L"]),
         moltab1=blobs["moltab1.bin"], moltab2=blobs["moltab2.bin"], moltab3=blobs["moltab3.bin"],
-        floats=blobs["floats.bin"],
+        floats=blobs["floats.bin"], ftraj_init=blobs["forces.ftraj"],
         float_specials=np.array([0.0, 1.0, -1.0, 0.75, -3.5, 2.0, 1e-15, -1e15, 123456.789, -2.5e-7]))
     path = os.path.join(OUT, f"{name}.npz")
     np.savez_compressed(path, **arrays)

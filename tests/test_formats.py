@@ -118,3 +118,30 @@ def test_run_files(tmp_path):
     raw = open(rf.path("traj_file"), "rb").read()
     p2, a2 = fm.moltab_decode(raw, 1, 2, 2)
     assert np.max(np.abs(p2 - pos)) < 1e-5 and np.max(np.abs(a2 - ang)) < 1e-4
+
+
+def ftraj_compare(mine: bytes, ref: bytes, n: int):
+    """Two force files of n values: the same layout and size, every value
+    within the 24-bit mantissa's quantum (values that differ by rounding
+    in the last place of FP64 can land on either side of a quantum step),
+    and nearly all 4-byte records identical."""
+    assert len(mine) == len(ref) == 4 * n
+    a, b = fm.float_decode(mine, n), fm.float_decode(ref, n)
+    assert np.all(np.abs(a - b) <= np.abs(b) * 2.0 ** -22 + 1e-300)
+    same = np.all(np.frombuffer(mine, np.uint8).reshape(n, 4) == np.frombuffer(ref, np.uint8).reshape(n, 4), axis=1)
+    assert same.mean() > 0.99, same.mean()
+
+
+@pytest.mark.parametrize("path", CASES, ids=lambda p: p.split("/")[-1])
+def test_force_file_frame_matches_reference(oracle_mod, path):
+    """A force-file frame (store_forces layout, mc_calc_forces.f90:348-368:
+    per atom F/L x, y, z as 24+8-bit floats) of the oracle's forces in the
+    reference convention (real space j > i + k space + external) against the
+    frame the compiled reference wrote with write_float from its own forces
+    of the same configuration."""
+    from yamc_amd.calc_forces import ftraj_frame
+    sysm, g = gc.load(path)
+    o = oracle_mod.Oracle(sysm)
+    o.mc_init()
+    F = o.forces_refconv()
+    ftraj_compare(ftraj_frame(F, sysm.params.box_length), g["ftraj_init"].tobytes(), 3 * sysm.natom)
