@@ -31,6 +31,15 @@ module yamc_ewald
       real(c_double) :: angle
    end type yamc_move_rec
 
+   type, bind(C) :: yamc_rs_desc
+      integer(c_int32_t) :: natom, ntype, full, nmol_atom, first_atom, j0, j1, stop_on_overlap, pad
+      real(c_double) :: box_length, alpha, rmax2, xclb
+      type(c_ptr) :: xx, yy, zz, charge, hard_core_angstr
+      type(c_ptr) :: molnum, lj_type
+      type(c_ptr) :: sigma2, four_eps, sigma6
+      type(c_ptr) :: xmol, ymol, zmol
+   end type yamc_rs_desc
+
    interface
       integer(c_int) function yamc_ctx_create(desc, ctx) bind(C, name="yamc_ctx_create")
          import :: c_int, c_ptr, yamc_system_desc
@@ -266,6 +275,143 @@ module yamc_ewald
          integer(c_int32_t), value :: nspecies, nbins
          real(c_double), value :: dr_angstr, rmax_angstr
          integer(c_int64_t), intent(inout) :: counts(*)
+      end function
+      ! ---- per-procedure surface (fortran/shim/ forwards the reference's
+      ! module procedures to these); 8-pointer arguments are the 8 sin/cos
+      ! arrays of a set, beta_LJ the ntype*ntype row pointers
+      integer(c_int) function yamc_ws_create(kmax, ws) bind(C, name="yamc_ws_create")
+         import :: c_int, c_ptr, c_int32_t
+         integer(c_int32_t), value :: kmax
+         type(c_ptr), intent(out) :: ws
+      end function
+      integer(c_int) function yamc_ws_destroy(ws) bind(C, name="yamc_ws_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ws
+      end function
+      integer(c_int) function yamc_ws_grid(ws, nk, kx, ky, kz, ip, iq, xk, xk2) bind(C, name="yamc_ws_grid")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ws
+         integer(c_int32_t), intent(out) :: nk
+         type(c_ptr), value :: kx, ky, kz, ip, iq, xk, xk2
+      end function
+      integer(c_int) function yamc_op_beta(ws, alpha, xclb, beta, beta6, beta12) bind(C, name="yamc_op_beta")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ws
+         real(c_double), value :: alpha, xclb
+         real(c_double), intent(out) :: beta(*), beta6(*), beta12(*)
+      end function
+      integer(c_int) function yamc_op_beta_lj(ws, lj6, lj12, beta6, beta12, out) bind(C, name="yamc_op_beta_lj")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ws
+         real(c_double), value :: lj6, lj12
+         real(c_double), intent(in) :: beta6(*), beta12(*)
+         type(c_ptr), value :: out
+      end function
+      integer(c_int) function yamc_op_sincos_xyz(ws, x, y, z, tab6) bind(C, name="yamc_op_sincos_xyz")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ws
+         real(c_double), value :: x, y, z
+         type(c_ptr), intent(in) :: tab6(6)
+      end function
+      integer(c_int) function yamc_op_sincos_kr(ws, tab6, q, kr8) bind(C, name="yamc_op_sincos_kr")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ws
+         type(c_ptr), intent(in) :: tab6(6)
+         real(c_double), value :: q
+         type(c_ptr), intent(in) :: kr8(8)
+      end function
+      integer(c_int) function yamc_op_sincos_kr_atom(ws, x, y, z, q, kr8) bind(C, name="yamc_op_sincos_kr_atom")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ws
+         real(c_double), value :: x, y, z, q
+         type(c_ptr), intent(in) :: kr8(8)
+      end function
+      integer(c_int) function yamc_op_sums(ws, n, x, y, z, q, lj_type, ntype, sums, present) &
+            bind(C, name="yamc_op_sums")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ws
+         integer(c_int32_t), value :: n, ntype
+         real(c_double), intent(in) :: x(*), y(*), z(*), q(*)
+         integer(c_int32_t), intent(in) :: lj_type(*)
+         type(c_ptr), intent(in) :: sums(*)
+         integer(c_int32_t), intent(out) :: present(*)
+      end function
+      integer(c_int) function yamc_op_rho_squared(ws, s8, rho2) bind(C, name="yamc_op_rho_squared")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ws
+         type(c_ptr), intent(in) :: s8(8)
+         real(c_double), intent(out) :: rho2(*)
+      end function
+      integer(c_int) function yamc_op_energy_coulomb(ws, beta, rho2, e) bind(C, name="yamc_op_energy_coulomb")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ws
+         real(c_double), intent(in) :: beta(*), rho2(*)
+         real(c_double), intent(out) :: e
+      end function
+      integer(c_int) function yamc_op_energy_lj(ws, ntype, beta_lj, sums_lj, e) bind(C, name="yamc_op_energy_lj")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ws
+         integer(c_int32_t), value :: ntype
+         type(c_ptr), intent(in) :: beta_lj(*), sums_lj(*)
+         real(c_double), intent(out) :: e
+      end function
+      integer(c_int) function yamc_op_du_coulomb(ws, beta, s8, d8, du) bind(C, name="yamc_op_du_coulomb")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ws
+         real(c_double), intent(in) :: beta(*)
+         type(c_ptr), intent(in) :: s8(8), d8(8)
+         real(c_double), intent(out) :: du
+      end function
+      integer(c_int) function yamc_op_du_lj(ws, ntype, beta_lj, sums_lj, delta_lj, present, du) &
+            bind(C, name="yamc_op_du_lj")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ws
+         integer(c_int32_t), value :: ntype
+         type(c_ptr), intent(in) :: beta_lj(*), sums_lj(*), delta_lj(*)
+         integer(c_int32_t), intent(in) :: present(*)
+         real(c_double), intent(out) :: du
+      end function
+      integer(c_int) function yamc_op_force_coulomb(ws, beta, s8, kr8, f3) bind(C, name="yamc_op_force_coulomb")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ws
+         real(c_double), intent(in) :: beta(*)
+         type(c_ptr), intent(in) :: s8(8), kr8(8)
+         real(c_double), intent(out) :: f3(3)
+      end function
+      integer(c_int) function yamc_op_force_lj(ws, ntype, ityp, beta_lj, sums_lj, kr8, f3) &
+            bind(C, name="yamc_op_force_lj")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ws
+         integer(c_int32_t), value :: ntype, ityp
+         type(c_ptr), intent(in) :: beta_lj(*), sums_lj(*), kr8(8)
+         real(c_double), intent(out) :: f3(3)
+      end function
+      integer(c_int) function yamc_op_rspace(ws, d, uu, fx, fy, fz, uu_back, fx_back, fy_back, fz_back, comps, &
+                                             overlap) bind(C, name="yamc_op_rspace")
+         import :: c_int, c_ptr, c_int32_t, c_double, yamc_rs_desc
+         type(c_ptr), value :: ws
+         type(yamc_rs_desc), intent(in) :: d
+         type(c_ptr), value :: uu, fx, fy, fz, uu_back, fx_back, fy_back, fz_back
+         real(c_double), intent(out) :: comps(6)
+         integer(c_int32_t), intent(out) :: overlap
+      end function
+      integer(c_int) function yamc_op_pair(ws, alpha, xclb, r2, same_molecule, qi, qj, lj, sigma6, four_eps, out) &
+            bind(C, name="yamc_op_pair")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ws
+         real(c_double), value :: alpha, xclb, r2, qi, qj, sigma6, four_eps
+         integer(c_int32_t), value :: same_molecule, lj
+         real(c_double), intent(out) :: out(14)
+      end function
+      integer(c_int) function yamc_op_tails(ws, natom, charge, ntype, lj6, lj12, counts, alpha, xclb, self_coulomb, &
+                                            lj6_tail, lj12_tail) bind(C, name="yamc_op_tails")
+         import :: c_int, c_ptr, c_int32_t, c_double
+         type(c_ptr), value :: ws
+         integer(c_int32_t), value :: natom, ntype
+         real(c_double), intent(in) :: charge(*), lj6(*), lj12(*)
+         integer(c_int32_t), intent(in) :: counts(*)
+         real(c_double), value :: alpha, xclb
+         real(c_double), intent(out) :: self_coulomb, lj6_tail, lj12_tail
       end function
       integer(c_int) function yamc_set_block_size(ctx, block) bind(C, name="yamc_set_block_size")
          import :: c_int, c_ptr, c_int32_t

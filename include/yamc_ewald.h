@@ -312,6 +312,102 @@ int yamc_forces(yamc_ctx *ctx, double *gx, double *gy, double *gz, double *rx, d
 int yamc_rdf(yamc_ctx *ctx, const int32_t *mol_species, int32_t nspecies, double dr_angstr, double rmax_angstr,
              int32_t nbins, uint64_t *counts);
 
+/* ---- per-procedure surface (the Fortran drop-in modules) ---------------- */
+/* One call per procedure of the reference's EwaldSumKSpace, SumSinCosKR,
+ * RhoSquared, FourierGrid, ForceKSpace, EwaldSumRealSpace, RealSumLocal and
+ * EwaldSumTails modules, on the caller's host arrays: each call copies its
+ * inputs to the device, computes there and copies its results back before it
+ * returns.  fortran/shim/ holds modules with the reference's names, types and
+ * procedure signatures that forward to these calls, so code written against
+ * the reference's modules (MC.f90) builds and runs unchanged on the device.
+ * Arrays are k-ordered as the workspace's grid (FourierGrid_init order); a
+ * set of sums is 8 arrays cos_ppp, sin_ppp, cos_ppm, sin_ppm, cos_pmp,
+ * sin_pmp, cos_pmm, sin_pmm (TSumSinCosKR / TSinCosKR fields, passed as 8
+ * pointers); beta_LJ is ntype*ntype row pointers, row (t1-1)*ntype+(t2-1).
+ * A workspace is used by one host thread at a time. */
+typedef struct yamc_ws yamc_ws;
+/* FourierGrid_calcNalloc + FourierGrid_init (FourierGrid.f90:37-189) for kmax
+ * and the workspace's device buffers. */
+int yamc_ws_create(int32_t kmax, yamc_ws **out);
+int yamc_ws_destroy(yamc_ws *ws);
+/* TFourierGrid fields nk, kx, ky, kz, ip, iq, xk, xk2 (any may be NULL). */
+int yamc_ws_grid(const yamc_ws *ws, int32_t *nk, int32_t *kx, int32_t *ky, int32_t *kz, int32_t *ip, int32_t *iq,
+                 double *xk, double *xk2);
+/* EwaldSumKSpace_initBeta (EwaldSumKSpace.f90:263-350): beta, beta6, beta12
+ * [nk] for alpha (1/box length) and xclb. */
+int yamc_op_beta(yamc_ws *ws, double alpha, double xclb, double *beta, double *beta6, double *beta12);
+/* One row of EwaldSumKSpace_initBetaLJ (:352-386): out = lj12 beta12 - lj6 beta6. */
+int yamc_op_beta_lj(yamc_ws *ws, double lj6, double lj12, const double *beta6, const double *beta12, double *out);
+/* SinCosXYZ_fill (SumSinCosKR.f90:181-223): tab6 = cos_xkx, cos_yky, cos_zkz,
+ * sin_xkx, sin_yky, sin_zkz, each [kmax+1]. */
+int yamc_op_sincos_xyz(yamc_ws *ws, double x, double y, double z, double *const *tab6);
+/* SinCosKR_fill (SumSinCosKR.f90:297-455) from the tables tab6 with charge q. */
+int yamc_op_sincos_kr(yamc_ws *ws, const double *const *tab6, double q, double *const *kr8);
+/* SinCosXYZ_fill + SinCosKR_fill of one atom at (x, y, z) with charge q. */
+int yamc_op_sincos_kr_atom(yamc_ws *ws, double x, double y, double z, double q, double *const *kr8);
+/* The sums RhoSquared_addAtoms (RhoSquared.f90:188-243) adds for n atoms:
+ * sums[(1+ntype)*8] row pointers (set 0 = Coulomb, set t = LJ type t) are
+ * overwritten with the atoms' sums; present[ntype] (may be NULL) = whether an
+ * atom of type t is among them. */
+int yamc_op_sums(yamc_ws *ws, int32_t n, const double *x, const double *y, const double *z, const double *q,
+                 const int32_t *lj_type, int32_t ntype, double *const *sums, int32_t *present);
+/* RhoSquared_calc_rho_squared (RhoSquared.f90:245-273). */
+int yamc_op_rho_squared(yamc_ws *ws, const double *const *s8, double *rho2);
+/* EwaldSumKSpace_calc_coulomb_energy (EwaldSumKSpace.f90:389-416) from
+ * rho_squared [nk], and _calc_LJ_energy (:418-526). */
+int yamc_op_energy_coulomb(yamc_ws *ws, const double *beta, const double *rho2, double *e);
+int yamc_op_energy_lj(yamc_ws *ws, int32_t ntype, const double *const *beta_lj, const double *const *sums_lj,
+                      double *e);
+/* EwaldSumKSpace_calc_dU_coulomb / _calc_dU_LJ (EwaldSumKSpace.f90:528-753):
+ * s8 / sums_lj the total sums, d8 / delta_lj the delta sums, present[ntype]
+ * the delta's type_is_present. */
+int yamc_op_du_coulomb(yamc_ws *ws, const double *beta, const double *const *s8, const double *const *d8, double *du);
+int yamc_op_du_lj(yamc_ws *ws, int32_t ntype, const double *const *beta_lj, const double *const *sums_lj,
+                  const double *const *delta_lj, const int32_t *present, double *du);
+/* ForceKSpace_coulomb / ForceKSpace_LJ (ForceKSpace.f90:59-302) of the atom
+ * whose sin/cos are kr8; ityp 1-based.  f3 = fx, fy, fz. */
+int yamc_op_force_coulomb(yamc_ws *ws, const double *beta, const double *const *s8, const double *const *kr8,
+                          double *f3);
+int yamc_op_force_lj(yamc_ws *ws, int32_t ntype, int32_t ityp, const double *const *beta_lj,
+                     const double *const *sums_lj, const double *const *kr8, double *f3);
+/* Inputs of EwaldSumRealSpace_calc (EwaldSumRealSpace.f90:233-500). */
+typedef struct yamc_rs_desc {
+    int32_t natom;            /* TAtomicData%natom */
+    int32_t ntype;            /* TLJTypes%NType */
+    int32_t full;             /* 1: full sum (TEwaldSumRealSpace%full_sum), 0: partial */
+    int32_t nmol_atom;        /* partial: the molecule's atoms */
+    int32_t first_atom;       /* partial: its first atom, 0-based */
+    int32_t j0, j1;           /* the optional first, last (0-based); -1: the defaults */
+    int32_t stop_on_overlap;  /* 1 unless dont_stop_on_overlap is .TRUE. */
+    int32_t pad;
+    double box_length, alpha, rmax2, xclb;  /* parameters: BoxLength, alpha, rmax2, xclb */
+    const double *xx, *yy, *zz, *charge, *hard_core_angstr;  /* [natom] */
+    const int32_t *molnum, *lj_type;                         /* [natom] */
+    const double *sigma2, *four_eps, *sigma6;                /* [ntype*ntype] */
+    const double *xmol, *ymol, *zmol;                        /* partial: the molecule's positions */
+} yamc_rs_desc;
+/* EwaldSumRealSpace_calc: uu/F over the rows (full: natom, partial:
+ * nmol_atom), the back arrays over natom (full sum: pass the same arrays as
+ * uu/F, they are added to), comps[6] = uu_ew, uu_lj6, uu_lj12 and their
+ * intramolecular parts; with stop_on_overlap the sums stop at the first
+ * overlapping pair in the reference's loop order, as it returns there.
+ * *overlap = the number of overlapping pairs the reference's loop visits (0
+ * or 1 with stop_on_overlap), i.e. the 'O' marks it prints.  Any of the
+ * array outputs may be NULL. */
+int yamc_op_rspace(yamc_ws *ws, const yamc_rs_desc *d, double *uu, double *fx, double *fy, double *fz,
+                   double *uu_back, double *fx_back, double *fy_back, double *fz_back, double *comps,
+                   int32_t *overlap);
+/* RealSumLocal (RealSumLocal.f90:63-168) for one pair at r2 > 0: out[14] =
+ * potew, potlj6, potlj12, pot, FR, then the TRealSumLocal fields r, a, a2,
+ * exp_minus_a2, erfc_a, C6, C12, a6, sigma_over_r_6.  lj = 0 leaves the LJ
+ * terms out (potlj6 = potlj12 = 0). */
+int yamc_op_pair(yamc_ws *ws, double alpha, double xclb, double r2, int32_t same_molecule, double qi, double qj,
+                 int32_t lj, double sigma6, double four_eps, double *out);
+/* ewald_sum_coulomb_tail + ewald_sum_lj_tails (EwaldSumTails.f90:18-127). */
+int yamc_op_tails(yamc_ws *ws, int32_t natom, const double *charge, int32_t ntype, const double *lj6,
+                  const double *lj12, const int32_t *counts, double alpha, double xclb, double *self_coulomb,
+                  double *lj6_tail, double *lj12_tail);
+
 /* ---- launch shape ------------------------------------------------------ */
 /* Launch shape of the chain kernel for this context (and for a batch whose
  * first context it is): 256 or 512 = one workgroup of that size per chain;

@@ -18,6 +18,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+# the same driver and the reference's MC code over the drop-in modules of
+# fortran/shim/ and the device library (build_shim.sh)
+SHIM_DRIVER = os.path.join(HERE, "_ref", "shim", "shim_driver")
 
 
 def available() -> bool:
@@ -72,10 +75,11 @@ def write_case(td, sysm, imol, u, vols=(), xchgs=(), runmc=None):
     return "case.in"
 
 
-def run(td, timeout=600):
-    """Runs the driver in td; returns (stdout, seconds)."""
+def run(td, timeout=600, driver=DRIVER):
+    """Runs the driver (default: the reference build) in td; returns
+    (stdout, seconds)."""
     t0 = time.perf_counter()
-    r = subprocess.run([DRIVER, "case.in"], cwd=td, capture_output=True, text=True, timeout=timeout)
+    r = subprocess.run([driver, "case.in"], cwd=td, capture_output=True, text=True, timeout=timeout)
     dt = time.perf_counter() - t0
     if r.returncode != 0:
         raise RuntimeError(r.stderr)

@@ -222,20 +222,24 @@ def parse_runmc(text):
     return out
 
 
+def freq_intervals(mv, xc):
+    """(first, last, move weight, exchange weight) of the maximal runs of
+    equal weights, 1-based molecules: the lines of a frequency file."""
+    freq, a, n = [], 0, len(mv)
+    for b in range(1, n + 1):
+        if b == n or mv[b] != mv[a] or xc[b] != xc[a]:
+            freq.append((a + 1, b, float(mv[a]), float(xc[a])))
+            a = b
+    return freq
+
+
 def make_runmc_case(name, sysm, ncycles, seed, psm, pressure, mvs, weights=None, nstream=60000):
     """The reference's own run_mc (runmc.f90:178-334) for ncycles cycles with
     its standard random stream: records the stream it draws, the state after
     every cycle and the number of uniforms the run consumed.  weights:
     per-molecule (move, exchange) weights written as a frequency file (None:
     the 'none' file)."""
-    freq = []
-    if weights is not None:
-        mv, xc = weights
-        a = 0
-        for b in range(1, sysm.nmol + 1):  # maximal runs of equal weights
-            if b == sysm.nmol or mv[b] != mv[a] or xc[b] != xc[a]:
-                freq.append((a + 1, b, float(mv[a]), float(xc[a])))
-                a = b
+    freq = freq_intervals(*weights) if weights is not None else []
     run = dict(ncycles=ncycles, seed=seed, psm=psm, pressure=pressure, mvs=mvs, nstream=nstream, freq=freq)
     with tempfile.TemporaryDirectory() as td:
         refcase.write_case(td, sysm, [], [], (), (), runmc=run)
@@ -260,6 +264,7 @@ def make_runmc_case(name, sysm, ncycles, seed, psm, pressure, mvs, weights=None,
         species=np.array([m.name for m in sysm.species]), counts=np.array(sysm.counts, np.int32),
         mol_type=sysm.mol_type, mol_pos=sysm.mol_pos, mol_angles=sysm.mol_angles,
         run_psm=np.float64(psm), run_pressure=np.float64(pressure), run_mvs=np.float64(mvs),
+        run_seed=np.int64(seed), run_weighted=np.int32(weights is not None),
         move_weights=np.asarray(weights[0] if weights is not None else np.ones(sysm.nmol), np.float64),
         xchange_weights=np.asarray(weights[1] if weights is not None else np.zeros(sysm.nmol), np.float64),
         stream=stream[:used + 64], stream_used=np.int64(used),

@@ -6,7 +6,8 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", f) for f in ("yamc_kernels.hip", "yamc_forces.hip", "yamc_rdf.hip", "yamc_api.hip")]
+SRC = [os.path.join(HERE, "csrc", f)
+       for f in ("yamc_kernels.hip", "yamc_forces.hip", "yamc_rdf.hip", "yamc_ops.hip", "yamc_api.hip")]
 OUT = os.path.join(HERE, "libyamc_ewald.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 VALU ops.

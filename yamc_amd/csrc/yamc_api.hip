@@ -204,6 +204,9 @@ static int put_state(yamc_ctx *c, const YdState *st);
 
 extern "C" const char *yamc_last_error(void) { return g_err; }
 
+// The error of a failing call in another source file of the library.
+int yamc_internal_fail(int code, const char *msg) { return fail(code, "%s", msg); }
+
 extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
     if (!d || !out) return fail(YAMC_ERROR_PARAMETER, "yamc_ctx_create: null argument");
     *out = nullptr;
