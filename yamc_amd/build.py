@@ -25,14 +25,31 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
 
 
 def build(verbose: bool = False) -> str:
+    """Compiles every source to an object (in parallel) and links the
+    library; the objects go to a temporary directory."""
+    import concurrent.futures as cf
+    import tempfile
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *SRC]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stderr)
-        raise RuntimeError("hipcc failed building libyamc_ewald.so")
-    if verbose:
-        sys.stderr.write(r.stderr)
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory(prefix="yamc_build_") as td:
+        objs = [os.path.join(td, os.path.basename(src) + ".o") for src in SRC]
+
+        def cc(src, obj):
+            return subprocess.run([HIPCC, *compile_flags, "-c", "-o", obj, src], capture_output=True, text=True)
+
+        with cf.ThreadPoolExecutor(max_workers=min(len(SRC), os.cpu_count() or 1)) as ex:
+            results = list(ex.map(cc, SRC, objs))
+        for src, r in zip(SRC, results):
+            if r.returncode != 0:
+                sys.stderr.write(r.stderr)
+                raise RuntimeError(f"hipcc failed compiling {os.path.basename(src)}")
+            if verbose:
+                sys.stderr.write(r.stderr)
+        r = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stderr)
+            raise RuntimeError("hipcc failed linking libyamc_ewald.so")
     os.replace(tmp, OUT)
     return OUT
 

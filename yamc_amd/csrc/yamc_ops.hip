@@ -2,9 +2,10 @@
 // (include/yamc_ewald.h, "per-procedure surface"): one call per reference
 // procedure of EwaldSumKSpace, SumSinCosKR, RhoSquared, ForceKSpace,
 // EwaldSumRealSpace, RealSumLocal and EwaldSumTails, on the caller's host
-// arrays.  Each call copies its inputs to the device, runs its kernels and
-// copies its results back before it returns; a workspace (yamc_ws) holds a
-// k grid and the device buffers the calls reuse.  The Fortran modules of
+// arrays.  Each call gathers its inputs into one pinned host block, sends it
+// with one copy, runs its kernels and brings its results back with one copy
+// before it returns; a workspace (yamc_ws) holds a k grid and the buffers
+// the calls reuse.  The Fortran modules of
 // fortran/shim/ forward the reference's procedures to these calls.
 #include <math.h>
 #include <stdarg.h>
@@ -40,8 +41,10 @@ static int ofail(int code, const char *fmt, ...) {
     } while (0)
 
 // Device buffers of a workspace: slot -> (pointer, capacity in bytes).
-enum { WB_A, WB_B, WB_C, WB_D, WB_E, WB_F, WB_G, WB_H, WB_I, WB_J, WB_K, WB_L, WB_OUT, WB_KEY, WB_N };
+enum { WB_L, WB_OUT, WB_N };
 
+// A call's inputs are gathered into one pinned host block and sent with one
+// copy; its results come back with one copy into a pinned landing block.
 struct yamc_ws {
     int device = 0, kmax = 0, nk = 0;
     hipStream_t s = nullptr;
@@ -51,6 +54,8 @@ struct yamc_ws {
     double *d_xk2 = nullptr;
     void *buf[WB_N] = {};
     size_t cap[WB_N] = {};
+    char *h_in = nullptr, *d_in = nullptr, *h_out = nullptr;  // staging (pinned), its device image, landing
+    size_t h_in_cap = 0, d_in_cap = 0, h_out_cap = 0, in_used = 0;
 };
 
 // Room for n bytes in slot b (grown, never shrunk).
@@ -66,33 +71,82 @@ static int wbuf(yamc_ws *w, int b, size_t n, void **out) {
     return YAMC_OK;
 }
 
-template <typename T>
-static int upload(yamc_ws *w, int b, const T *h, size_t n, T **d) {
-    void *p = nullptr;
-    OTRY(wbuf(w, b, sizeof(T) * (n ? n : 1), &p));
-    if (n) OCHK(hipMemcpyAsync(p, h, sizeof(T) * n, hipMemcpyHostToDevice, w->s));
-    *d = (T *)p;
+// Pinned host block of at least n bytes keeping its first keep bytes.  Every
+// call ends synchronised with its stream, so no copy still reads the block.
+static int pinned(char **p, size_t *cap, size_t n, size_t keep) {
+    if (*cap >= n) return YAMC_OK;
+    size_t c = *cap ? *cap : ((size_t)1 << 16);
+    while (c < n) c *= 2;
+    char *q = nullptr;
+    OCHK(hipHostMalloc((void **)&q, c));
+    if (keep) memcpy(q, *p, keep);
+    if (*p) OCHK(hipHostFree(*p));
+    *p = q;
+    *cap = c;
     return YAMC_OK;
 }
 
-// m arrays of nk doubles (host pointers) into one device block [m][nk].
-static int upload_rows(yamc_ws *w, int b, const double *const *rows, int m, double **d) {
-    void *p = nullptr;
-    OTRY(wbuf(w, b, sizeof(double) * (size_t)m * w->nk, &p));
+static void stage_begin(yamc_ws *w) { w->in_used = 0; }
+
+// Appends n bytes at h to the call's input block; *off = their offset.
+static int stage_put(yamc_ws *w, const void *h, size_t n, size_t *off) {
+    const size_t o = (w->in_used + 255) & ~(size_t)255;
+    OTRY(pinned(&w->h_in, &w->h_in_cap, o + (n ? n : 1), w->in_used));
+    if (n && h) memcpy(w->h_in + o, h, n);  // h NULL: room the caller fills
+    w->in_used = o + (n ? n : 1);
+    *off = o;
+    return YAMC_OK;
+}
+
+// m arrays of nk doubles (host pointers) as one block [m][nk].
+static int stage_rows(yamc_ws *w, const double *const *rows, int m, size_t *off) {
+    const size_t row = sizeof(double) * w->nk, o = (w->in_used + 255) & ~(size_t)255;
+    const size_t n = row * (m ? m : 1);
+    OTRY(pinned(&w->h_in, &w->h_in_cap, o + n, w->in_used));
     for (int i = 0; i < m; ++i) {
         if (!rows[i]) return ofail(YAMC_ERROR_PARAMETER, "null array");
-        OCHK(hipMemcpyAsync((double *)p + (size_t)i * w->nk, rows[i], sizeof(double) * w->nk, hipMemcpyHostToDevice,
-                            w->s));
+        memcpy(w->h_in + o + row * i, rows[i], row);
     }
-    *d = (double *)p;
+    w->in_used = o + n;
+    *off = o;
     return YAMC_OK;
 }
 
-static int download_rows(yamc_ws *w, const double *d, double *const *rows, int m) {
-    for (int i = 0; i < m; ++i)
-        if (rows[i])
-            OCHK(hipMemcpyAsync(rows[i], d + (size_t)i * w->nk, sizeof(double) * w->nk, hipMemcpyDeviceToHost, w->s));
+// One copy of the staged inputs to the device.
+static int stage_flush(yamc_ws *w) {
+    if (w->d_in_cap < w->in_used) {
+        if (w->d_in) OCHK(hipFree(w->d_in));
+        w->d_in = nullptr;
+        w->d_in_cap = 0;
+        size_t c = (size_t)1 << 16;
+        while (c < w->in_used) c *= 2;
+        OCHK(hipMalloc((void **)&w->d_in, c));
+        w->d_in_cap = c;
+    }
+    if (w->in_used) OCHK(hipMemcpyAsync(w->d_in, w->h_in, w->in_used, hipMemcpyHostToDevice, w->s));
     return YAMC_OK;
+}
+
+template <typename T>
+static T *stage_dev(yamc_ws *w, size_t off) {
+    return (T *)(w->d_in + off);
+}
+
+static int wsync(yamc_ws *w);
+
+// n bytes of device memory d to the pinned landing block, synchronised.
+static int fetch(yamc_ws *w, const void *d, size_t n, const char **h) {
+    OTRY(pinned(&w->h_out, &w->h_out_cap, n ? n : 1, 0));
+    if (n) OCHK(hipMemcpyAsync(w->h_out, d, n, hipMemcpyDeviceToHost, w->s));
+    OTRY(wsync(w));
+    *h = w->h_out;
+    return YAMC_OK;
+}
+
+// Rows [m][nk] of a fetched block to the caller's arrays (NULL rows skipped).
+static void scatter_rows(const yamc_ws *w, const char *h, double *const *rows, int m) {
+    for (int i = 0; i < m; ++i)
+        if (rows[i]) memcpy(rows[i], h + sizeof(double) * w->nk * i, sizeof(double) * w->nk);
 }
 
 static int wsync(yamc_ws *w) {
@@ -163,6 +217,9 @@ extern "C" int yamc_ws_destroy(yamc_ws *w) {
         if (w->buf[b]) (void)hipFree(w->buf[b]);
     if (w->d_kxyz) (void)hipFree(w->d_kxyz);
     if (w->d_xk2) (void)hipFree(w->d_xk2);
+    if (w->d_in) (void)hipFree(w->d_in);
+    if (w->h_in) (void)hipHostFree(w->h_in);
+    if (w->h_out) (void)hipHostFree(w->h_out);
     if (w->s) (void)hipStreamDestroy(w->s);
     delete w;
     return YAMC_OK;
@@ -194,18 +251,19 @@ extern "C" int yamc_op_beta(yamc_ws *w, double alpha, double xclb, double *beta,
     c.p.alpha = alpha;
     c.p.xclb = xclb;
     c.kxyz = w->d_kxyz;
-    void *pb = nullptr, *p6 = nullptr, *p12 = nullptr;
-    OTRY(wbuf(w, WB_A, sizeof(double) * w->nk, &pb));
-    OTRY(wbuf(w, WB_B, sizeof(double) * w->nk, &p6));
-    OTRY(wbuf(w, WB_C, sizeof(double) * w->nk, &p12));
-    c.beta = (double *)pb;
-    yk_beta<<<(w->nk + 63) / 64, 64, 0, w->s>>>(c, w->d_xk2, (double *)p6, (double *)p12);
-    OCHK(hipGetLastError());
     const size_t n = sizeof(double) * w->nk;
-    OCHK(hipMemcpyAsync(beta, pb, n, hipMemcpyDeviceToHost, w->s));
-    OCHK(hipMemcpyAsync(beta6, p6, n, hipMemcpyDeviceToHost, w->s));
-    OCHK(hipMemcpyAsync(beta12, p12, n, hipMemcpyDeviceToHost, w->s));
-    return wsync(w);
+    void *po = nullptr;
+    OTRY(wbuf(w, WB_OUT, 3 * n, &po));
+    double *pb = (double *)po;
+    c.beta = pb;
+    yk_beta<<<(w->nk + 63) / 64, 64, 0, w->s>>>(c, w->d_xk2, pb + w->nk, pb + 2 * w->nk);
+    OCHK(hipGetLastError());
+    const char *h;
+    OTRY(fetch(w, po, 3 * n, &h));
+    memcpy(beta, h, n);
+    memcpy(beta6, h + n, n);
+    memcpy(beta12, h + 2 * n, n);
+    return YAMC_OK;
 }
 
 __global__ void yo_beta_lj(int nk, double lj6, double lj12, const double *b6, const double *b12, double *out) {
@@ -216,15 +274,21 @@ __global__ void yo_beta_lj(int nk, double lj6, double lj12, const double *b6, co
 extern "C" int yamc_op_beta_lj(yamc_ws *w, double lj6, double lj12, const double *beta6, const double *beta12,
                                double *out) {
     if (!w || !beta6 || !beta12 || !out) return ofail(YAMC_ERROR_PARAMETER, "null argument");
-    double *d6, *d12;
+    const size_t n = sizeof(double) * w->nk;
+    size_t o6, o12;
     void *po = nullptr;
-    OTRY(upload(w, WB_A, beta6, w->nk, &d6));
-    OTRY(upload(w, WB_B, beta12, w->nk, &d12));
-    OTRY(wbuf(w, WB_C, sizeof(double) * w->nk, &po));
-    yo_beta_lj<<<(w->nk + 255) / 256, 256, 0, w->s>>>(w->nk, lj6, lj12, d6, d12, (double *)po);
+    stage_begin(w);
+    OTRY(stage_put(w, beta6, n, &o6));
+    OTRY(stage_put(w, beta12, n, &o12));
+    OTRY(stage_flush(w));
+    OTRY(wbuf(w, WB_OUT, n, &po));
+    yo_beta_lj<<<(w->nk + 255) / 256, 256, 0, w->s>>>(w->nk, lj6, lj12, stage_dev<double>(w, o6),
+                                                      stage_dev<double>(w, o12), (double *)po);
     OCHK(hipGetLastError());
-    OCHK(hipMemcpyAsync(out, po, sizeof(double) * w->nk, hipMemcpyDeviceToHost, w->s));
-    return wsync(w);
+    const char *h;
+    OTRY(fetch(w, po, n, &h));
+    memcpy(out, h, n);
+    return YAMC_OK;
 }
 
 // SinCosXYZ_fill (SumSinCosKR.f90:181-223) of one position: tab = cos x, sin
@@ -247,52 +311,60 @@ __global__ void yo_sincos_kr(int nk, const int *kxyz, const double *tab, int kp,
     for (int m = 0; m < 8; ++m) kr[(size_t)m * nk + k] = o[m];
 }
 
+// tab6 order cos x, cos y, cos z, sin x, sin y, sin z; device order cx, sx,
+// cy, sy, cz, sz
+static const int kTabSlot[6] = {0, 2, 4, 1, 3, 5};
+
 extern "C" int yamc_op_sincos_xyz(yamc_ws *w, double x, double y, double z, double *const *tab6) {
     if (!w || !tab6) return ofail(YAMC_ERROR_PARAMETER, "null argument");
+    for (int i = 0; i < 6; ++i)
+        if (!tab6[i]) return ofail(YAMC_ERROR_PARAMETER, "null table");
     const int kp = w->kmax + 1;
     void *pt = nullptr;
-    OTRY(wbuf(w, WB_A, sizeof(double) * 6 * kp, &pt));
+    OTRY(wbuf(w, WB_OUT, sizeof(double) * 6 * kp, &pt));
     yo_sincos_xyz<<<1, 64, 0, w->s>>>(x, y, z, w->kmax, (double *)pt);
     OCHK(hipGetLastError());
-    // device order cx, sx, cy, sy, cz, sz; tab6 order cos x, cos y, cos z, sin x, sin y, sin z
-    const int src[6] = {0, 2, 4, 1, 3, 5};
-    for (int i = 0; i < 6; ++i) {
-        if (!tab6[i]) return ofail(YAMC_ERROR_PARAMETER, "null table");
-        OCHK(hipMemcpyAsync(tab6[i], (double *)pt + (size_t)src[i] * kp, sizeof(double) * kp, hipMemcpyDeviceToHost,
-                            w->s));
-    }
-    return wsync(w);
+    const char *h;
+    OTRY(fetch(w, pt, sizeof(double) * 6 * kp, &h));
+    for (int i = 0; i < 6; ++i) memcpy(tab6[i], h + sizeof(double) * kTabSlot[i] * kp, sizeof(double) * kp);
+    return YAMC_OK;
 }
 
 extern "C" int yamc_op_sincos_kr(yamc_ws *w, const double *const *tab6, double q, double *const *kr8) {
     if (!w || !tab6 || !kr8) return ofail(YAMC_ERROR_PARAMETER, "null argument");
     const int kp = w->kmax + 1;
-    void *pt = nullptr, *pk = nullptr;
-    OTRY(wbuf(w, WB_A, sizeof(double) * 6 * kp, &pt));
-    const int dst[6] = {0, 2, 4, 1, 3, 5};
+    stage_begin(w);
+    size_t ot;
+    OTRY(stage_put(w, nullptr, sizeof(double) * 6 * kp, &ot));
     for (int i = 0; i < 6; ++i) {
         if (!tab6[i]) return ofail(YAMC_ERROR_PARAMETER, "null table");
-        OCHK(hipMemcpyAsync((double *)pt + (size_t)dst[i] * kp, tab6[i], sizeof(double) * kp, hipMemcpyHostToDevice,
-                            w->s));
+        memcpy(w->h_in + ot + sizeof(double) * kTabSlot[i] * kp, tab6[i], sizeof(double) * kp);
     }
-    OTRY(wbuf(w, WB_B, sizeof(double) * 8 * w->nk, &pk));
-    yo_sincos_kr<<<(w->nk + 255) / 256, 256, 0, w->s>>>(w->nk, w->d_kxyz, (double *)pt, kp, q, (double *)pk);
+    OTRY(stage_flush(w));
+    void *pk = nullptr;
+    OTRY(wbuf(w, WB_OUT, sizeof(double) * 8 * w->nk, &pk));
+    yo_sincos_kr<<<(w->nk + 255) / 256, 256, 0, w->s>>>(w->nk, w->d_kxyz, stage_dev<double>(w, ot), kp, q,
+                                                        (double *)pk);
     OCHK(hipGetLastError());
-    OTRY(download_rows(w, (double *)pk, kr8, 8));
-    return wsync(w);
+    const char *h;
+    OTRY(fetch(w, pk, sizeof(double) * 8 * w->nk, &h));
+    scatter_rows(w, h, kr8, 8);
+    return YAMC_OK;
 }
 
 extern "C" int yamc_op_sincos_kr_atom(yamc_ws *w, double x, double y, double z, double q, double *const *kr8) {
     if (!w || !kr8) return ofail(YAMC_ERROR_PARAMETER, "null argument");
     const int kp = w->kmax + 1;
     void *pt = nullptr, *pk = nullptr;
-    OTRY(wbuf(w, WB_A, sizeof(double) * 6 * kp, &pt));
-    OTRY(wbuf(w, WB_B, sizeof(double) * 8 * w->nk, &pk));
+    OTRY(wbuf(w, WB_L, sizeof(double) * 6 * kp, &pt));
+    OTRY(wbuf(w, WB_OUT, sizeof(double) * 8 * w->nk, &pk));
     yo_sincos_xyz<<<1, 64, 0, w->s>>>(x, y, z, w->kmax, (double *)pt);
     yo_sincos_kr<<<(w->nk + 255) / 256, 256, 0, w->s>>>(w->nk, w->d_kxyz, (double *)pt, kp, q, (double *)pk);
     OCHK(hipGetLastError());
-    OTRY(download_rows(w, (double *)pk, kr8, 8));
-    return wsync(w);
+    const char *h;
+    OTRY(fetch(w, pk, sizeof(double) * 8 * w->nk, &h));
+    scatter_rows(w, h, kr8, 8);
+    return YAMC_OK;
 }
 
 // RhoSquared_addAtoms' sums (RhoSquared.f90:160-243): thread per (set, k),
@@ -344,25 +416,31 @@ extern "C" int yamc_op_sums(yamc_ws *w, int32_t n, const double *x, const double
     soff[nset] = (int)satoms.size();
     if (present)
         for (int t = 1; t <= ntype; ++t) present[t - 1] = soff[t + 1] > soff[t];
-    double *dx, *dy, *dz, *dq;
-    int *dt, *doff, *dat;
-    OTRY(upload(w, WB_A, x, n, &dx));
-    OTRY(upload(w, WB_B, y, n, &dy));
-    OTRY(upload(w, WB_C, z, n, &dz));
-    OTRY(upload(w, WB_D, q, n, &dq));
-    OTRY(upload(w, WB_E, lj_type, n, &dt));
-    OTRY(upload(w, WB_F, soff.data(), soff.size(), &doff));
-    OTRY(upload(w, WB_G, satoms.data(), satoms.size(), &dat));
+    size_t ox, oy, oz, oq, ot, ooff, oat;
+    stage_begin(w);
+    OTRY(stage_put(w, x, sizeof(double) * n, &ox));
+    OTRY(stage_put(w, y, sizeof(double) * n, &oy));
+    OTRY(stage_put(w, z, sizeof(double) * n, &oz));
+    OTRY(stage_put(w, q, sizeof(double) * n, &oq));
+    OTRY(stage_put(w, lj_type, sizeof(int32_t) * n, &ot));
+    OTRY(stage_put(w, soff.data(), sizeof(int) * soff.size(), &ooff));
+    OTRY(stage_put(w, satoms.data(), sizeof(int) * satoms.size(), &oat));
+    OTRY(stage_flush(w));
     void *po = nullptr, *ptab = nullptr;
     const int kp = w->kmax + 1;
     OTRY(wbuf(w, WB_L, sizeof(double) * 6 * (size_t)kp * (n ? n : 1), &ptab));
     OTRY(wbuf(w, WB_OUT, sizeof(double) * 8 * (size_t)nset * w->nk, &po));
-    if (n) yo_tables<<<(n + 255) / 256, 256, 0, w->s>>>(n, dx, dy, dz, kp, (double *)ptab);
-    yo_sums<<<(nset * w->nk + 127) / 128, 128, 0, w->s>>>(w->nk, w->d_kxyz, (double *)ptab, dq, dt, nset, doff, dat,
-                                                          kp, (double *)po);
+    if (n)
+        yo_tables<<<(n + 255) / 256, 256, 0, w->s>>>(n, stage_dev<double>(w, ox), stage_dev<double>(w, oy),
+                                                     stage_dev<double>(w, oz), kp, (double *)ptab);
+    yo_sums<<<(nset * w->nk + 127) / 128, 128, 0, w->s>>>(w->nk, w->d_kxyz, (double *)ptab, stage_dev<double>(w, oq),
+                                                          stage_dev<int>(w, ot), nset, stage_dev<int>(w, ooff),
+                                                          stage_dev<int>(w, oat), kp, (double *)po);
     OCHK(hipGetLastError());
-    OTRY(download_rows(w, (double *)po, sums, 8 * nset));
-    return wsync(w);
+    const char *h;
+    OTRY(fetch(w, po, sizeof(double) * 8 * (size_t)nset * w->nk, &h));
+    scatter_rows(w, h, sums, 8 * nset);
+    return YAMC_OK;
 }
 
 // Per-k terms reduced per block (deterministic): a block of 256 threads over
@@ -377,9 +455,9 @@ __device__ __forceinline__ void yo_block_out(double (&v)[NV], double *part) {
 
 template <int NV>
 static int reduce_blocks(yamc_ws *w, const double *dpart, int nblk, double *out) {
-    std::vector<double> h((size_t)nblk * NV);
-    OCHK(hipMemcpyAsync(h.data(), dpart, sizeof(double) * h.size(), hipMemcpyDeviceToHost, w->s));
-    OTRY(wsync(w));
+    const char *hc;
+    OTRY(fetch(w, dpart, sizeof(double) * (size_t)nblk * NV, &hc));
+    const double *h = (const double *)hc;
     for (int i = 0; i < NV; ++i) {
         double a = 0.0;
         for (int b = 0; b < nblk; ++b) a = a + h[(size_t)b * NV + i];
@@ -404,14 +482,18 @@ __global__ void yo_rho2(int nk, const int *kxyz, const double *S, double *rho2) 
 
 extern "C" int yamc_op_rho_squared(yamc_ws *w, const double *const *s8, double *rho2) {
     if (!w || !s8 || !rho2) return ofail(YAMC_ERROR_PARAMETER, "null argument");
-    double *dS;
+    size_t oS;
+    stage_begin(w);
+    OTRY(stage_rows(w, s8, 8, &oS));
+    OTRY(stage_flush(w));
     void *po = nullptr;
-    OTRY(upload_rows(w, WB_A, s8, 8, &dS));
     OTRY(wbuf(w, WB_OUT, sizeof(double) * w->nk, &po));
-    yo_rho2<<<(w->nk + 255) / 256, 256, 0, w->s>>>(w->nk, w->d_kxyz, dS, (double *)po);
+    yo_rho2<<<(w->nk + 255) / 256, 256, 0, w->s>>>(w->nk, w->d_kxyz, stage_dev<double>(w, oS), (double *)po);
     OCHK(hipGetLastError());
-    OCHK(hipMemcpyAsync(rho2, po, sizeof(double) * w->nk, hipMemcpyDeviceToHost, w->s));
-    return wsync(w);
+    const char *h;
+    OTRY(fetch(w, po, sizeof(double) * w->nk, &h));
+    memcpy(rho2, h, sizeof(double) * w->nk);
+    return YAMC_OK;
 }
 
 // calc_coulomb_energy (EwaldSumKSpace.f90:389-416): sum_k beta rho^2, from
@@ -424,13 +506,15 @@ __global__ void yo_e_coulomb(int nk, const double *beta, const double *rho2, dou
 
 extern "C" int yamc_op_energy_coulomb(yamc_ws *w, const double *beta, const double *rho2, double *e) {
     if (!w || !beta || !rho2 || !e) return ofail(YAMC_ERROR_PARAMETER, "null argument");
-    double *db, *dr;
+    size_t ob, orr;
+    stage_begin(w);
+    OTRY(stage_put(w, beta, sizeof(double) * w->nk, &ob));
+    OTRY(stage_put(w, rho2, sizeof(double) * w->nk, &orr));
+    OTRY(stage_flush(w));
     void *pp = nullptr;
     const int nblk = (w->nk + 255) / 256;
-    OTRY(upload(w, WB_A, beta, w->nk, &db));
-    OTRY(upload(w, WB_B, rho2, w->nk, &dr));
     OTRY(wbuf(w, WB_OUT, sizeof(double) * nblk, &pp));
-    yo_e_coulomb<<<nblk, 256, 0, w->s>>>(w->nk, db, dr, (double *)pp);
+    yo_e_coulomb<<<nblk, 256, 0, w->s>>>(w->nk, stage_dev<double>(w, ob), stage_dev<double>(w, orr), (double *)pp);
     OCHK(hipGetLastError());
     return reduce_blocks<1>(w, (const double *)pp, nblk, e);
 }
@@ -468,13 +552,16 @@ extern "C" int yamc_op_energy_lj(yamc_ws *w, int32_t ntype, const double *const 
         *e = 0.0;
         return YAMC_OK;
     }
-    double *dB, *dS;
+    size_t oB, oS;
+    stage_begin(w);
+    OTRY(stage_rows(w, beta_lj, ntype * ntype, &oB));
+    OTRY(stage_rows(w, sums_lj, 8 * ntype, &oS));
+    OTRY(stage_flush(w));
     void *pp = nullptr;
     const int nblk = (w->nk + 255) / 256;
-    OTRY(upload_rows(w, WB_A, beta_lj, ntype * ntype, &dB));
-    OTRY(upload_rows(w, WB_B, sums_lj, 8 * ntype, &dS));
     OTRY(wbuf(w, WB_OUT, sizeof(double) * nblk, &pp));
-    yo_e_lj<<<nblk, 256, 0, w->s>>>(w->nk, w->d_kxyz, ntype, dB, dS, (double *)pp);
+    yo_e_lj<<<nblk, 256, 0, w->s>>>(w->nk, w->d_kxyz, ntype, stage_dev<double>(w, oB), stage_dev<double>(w, oS),
+                                    (double *)pp);
     OCHK(hipGetLastError());
     return reduce_blocks<1>(w, (const double *)pp, nblk, e);
 }
@@ -499,14 +586,17 @@ __global__ void yo_du_coulomb(int nk, const int *kxyz, const double *beta, const
 extern "C" int yamc_op_du_coulomb(yamc_ws *w, const double *beta, const double *const *s8, const double *const *d8,
                                   double *du) {
     if (!w || !beta || !s8 || !d8 || !du) return ofail(YAMC_ERROR_PARAMETER, "null argument");
-    double *db, *dS, *dD;
+    size_t ob, oS, oD;
+    stage_begin(w);
+    OTRY(stage_put(w, beta, sizeof(double) * w->nk, &ob));
+    OTRY(stage_rows(w, s8, 8, &oS));
+    OTRY(stage_rows(w, d8, 8, &oD));
+    OTRY(stage_flush(w));
     void *pp = nullptr;
     const int nblk = (w->nk + 255) / 256;
-    OTRY(upload(w, WB_A, beta, w->nk, &db));
-    OTRY(upload_rows(w, WB_B, s8, 8, &dS));
-    OTRY(upload_rows(w, WB_C, d8, 8, &dD));
     OTRY(wbuf(w, WB_OUT, sizeof(double) * nblk, &pp));
-    yo_du_coulomb<<<nblk, 256, 0, w->s>>>(w->nk, w->d_kxyz, db, dS, dD, (double *)pp);
+    yo_du_coulomb<<<nblk, 256, 0, w->s>>>(w->nk, w->d_kxyz, stage_dev<double>(w, ob), stage_dev<double>(w, oS),
+                                          stage_dev<double>(w, oD), (double *)pp);
     OCHK(hipGetLastError());
     return reduce_blocks<1>(w, (const double *)pp, nblk, du);
 }
@@ -548,14 +638,17 @@ extern "C" int yamc_op_du_lj(yamc_ws *w, int32_t ntype, const double *const *bet
         *du = 0.0;
         return YAMC_OK;
     }
-    double *dB, *dS, *dD;
+    size_t oB, oS, oD;
+    stage_begin(w);
+    OTRY(stage_rows(w, beta_lj, ntype * ntype, &oB));
+    OTRY(stage_rows(w, sums_lj, 8 * ntype, &oS));
+    OTRY(stage_rows(w, delta_lj, 8 * ntype, &oD));
+    OTRY(stage_flush(w));
     void *pp = nullptr;
     const int nblk = (w->nk + 255) / 256;
-    OTRY(upload_rows(w, WB_A, beta_lj, ntype * ntype, &dB));
-    OTRY(upload_rows(w, WB_B, sums_lj, 8 * ntype, &dS));
-    OTRY(upload_rows(w, WB_C, delta_lj, 8 * ntype, &dD));
     OTRY(wbuf(w, WB_OUT, sizeof(double) * nblk, &pp));
-    yo_du_lj<<<nblk, 256, 0, w->s>>>(w->nk, w->d_kxyz, ntype, pmask, dB, dS, dD, (double *)pp);
+    yo_du_lj<<<nblk, 256, 0, w->s>>>(w->nk, w->d_kxyz, ntype, pmask, stage_dev<double>(w, oB),
+                                     stage_dev<double>(w, oS), stage_dev<double>(w, oD), (double *)pp);
     OCHK(hipGetLastError());
     return reduce_blocks<1>(w, (const double *)pp, nblk, du);
 }
@@ -582,14 +675,17 @@ __global__ void yo_force(int nk, const int *kxyz, int nset, const double *B, con
 
 static int force_sets(yamc_ws *w, int nset, const double *const *brows, const double *const *srows,
                       const double *const *kr8, double *f3) {
-    double *dB, *dS, *dK;
+    size_t oB, oS, oK;
+    stage_begin(w);
+    OTRY(stage_rows(w, brows, nset, &oB));
+    OTRY(stage_rows(w, srows, 8 * nset, &oS));
+    OTRY(stage_rows(w, kr8, 8, &oK));
+    OTRY(stage_flush(w));
     void *pp = nullptr;
     const int nblk = (w->nk + 255) / 256;
-    OTRY(upload_rows(w, WB_A, brows, nset, &dB));
-    OTRY(upload_rows(w, WB_B, srows, 8 * nset, &dS));
-    OTRY(upload_rows(w, WB_C, kr8, 8, &dK));
     OTRY(wbuf(w, WB_OUT, sizeof(double) * 3 * nblk, &pp));
-    yo_force<<<nblk, 256, 0, w->s>>>(w->nk, w->d_kxyz, nset, dB, dS, dK, (double *)pp);
+    yo_force<<<nblk, 256, 0, w->s>>>(w->nk, w->d_kxyz, nset, stage_dev<double>(w, oB), stage_dev<double>(w, oS),
+                                     stage_dev<double>(w, oK), (double *)pp);
     OCHK(hipGetLastError());
     double f[3];
     OTRY(reduce_blocks<3>(w, (const double *)pp, nblk, f));
@@ -673,10 +769,12 @@ __global__ void yo_rs_first_overlap(YoRs r, unsigned long long *key) {
 }
 
 // Pass 2: rows.  Thread per row i and column group g: the group's pairs
-// before the cut-off key; partial uu(i), F(i) to pat[g][4][nrow] and the six
+// before the cut-off key (the first overlap when the sum stops there, kc[0]
+// = ~0 when none); partial uu(i), F(i) to pat[g][4][nrow] and the six
 // energy components of the block to part[g][block][6].
-__global__ void __launch_bounds__(256) yo_rs_rows(YoRs r, int ngroups, unsigned long long key, double *pat,
+__global__ void __launch_bounds__(256) yo_rs_rows(YoRs r, int ngroups, const unsigned long long *kc, double *pat,
                                                   double *part) {
+    const unsigned long long key = r.stop ? kc[0] : ~0ull;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int jc = (r.natom + ngroups - 1) / ngroups, jb = (int)blockIdx.y * jc;
     const int je = jb + jc < r.natom ? jb + jc : r.natom;
@@ -710,7 +808,9 @@ __global__ void __launch_bounds__(256) yo_rs_rows(YoRs r, int ngroups, unsigned 
 }
 
 // Pass 3: columns (the back arrays): thread per atom j and row group g.
-__global__ void __launch_bounds__(256) yo_rs_cols(YoRs r, int ngroups, unsigned long long key, double *pcol) {
+__global__ void __launch_bounds__(256) yo_rs_cols(YoRs r, int ngroups, const unsigned long long *kc,
+                                                  double *pcol) {
+    const unsigned long long key = r.stop ? kc[0] : ~0ull;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= r.natom) return;
     const int ic = (r.nrow + ngroups - 1) / ngroups, ib = (int)blockIdx.y * ic;
@@ -733,6 +833,16 @@ __global__ void __launch_bounds__(256) yo_rs_cols(YoRs r, int ngroups, unsigned 
     pc[3 * (size_t)r.natom] = fz;
 }
 
+// Group partials [ng][n] summed per element in group order (the order a
+// host loop over the groups would add them in): out[t] = 0 + g0 + g1 + ...
+__global__ void yo_group_sum(const double *part, int ng, size_t n, double *out) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double a = 0.0;
+    for (int g = 0; g < ng; ++g) a = a + part[(size_t)g * n + t];
+    out[t] = a;
+}
+
 extern "C" int yamc_op_rspace(yamc_ws *w, const yamc_rs_desc *d, double *uu, double *fx, double *fy, double *fz,
                               double *uu_back, double *fx_back, double *fy_back, double *fz_back, double *comps,
                               int32_t *overlap) {
@@ -742,7 +852,8 @@ extern "C" int yamc_op_rspace(yamc_ws *w, const yamc_rs_desc *d, double *uu, dou
     const int nrow = d->full ? N : d->nmol_atom;
     if (nrow < 1 || (!d->full && (d->first_atom < 0 || d->first_atom + nrow > N)))
         return ofail(YAMC_ERROR_PARAMETER, "bad molecule range");
-    if (d->j1 >= N || d->j0 >= N) return ofail(YAMC_ERROR_PARAMETER, "j range outside the atoms");
+    if (d->j1 >= N || d->j0 >= N || d->j0 < -1 || d->j1 < -1)
+        return ofail(YAMC_ERROR_PARAMETER, "j range outside the atoms");
     if (nrow > 65535) return ofail(YAMC_ERROR_LIMITS, "%d rows: this per-procedure sum takes at most 65535", nrow);
     YoRs r{};
     r.natom = N; r.ntype = d->ntype; r.full = d->full != 0; r.nrow = nrow; r.first_atom = d->first_atom;
@@ -751,79 +862,85 @@ extern "C" int yamc_op_rspace(yamc_ws *w, const yamc_rs_desc *d, double *uu, dou
     r.p.alpha_over_sqrt_pi = d->alpha / sqrt(YD_PI);  // parameters.f90:400-401
     r.p.two_alpha_over_sqrt_pi = 2.0 * r.p.alpha_over_sqrt_pi;
     const int TT = d->ntype * d->ntype > 0 ? d->ntype * d->ntype : 1;
-    double *dx, *dy, *dz, *dq, *dh, *ds2, *dfe, *ds6, *dxm, *dym, *dzm;
-    int *dmol, *dtyp;
-    OTRY(upload(w, WB_A, d->xx, N, &dx));
-    OTRY(upload(w, WB_B, d->yy, N, &dy));
-    OTRY(upload(w, WB_C, d->zz, N, &dz));
-    OTRY(upload(w, WB_D, d->charge, N, &dq));
-    OTRY(upload(w, WB_E, d->hard_core_angstr, N, &dh));
-    OTRY(upload(w, WB_F, d->molnum, N, &dmol));
-    OTRY(upload(w, WB_G, d->lj_type, N, &dtyp));
-    const double zero = 0.0;
-    OTRY(upload(w, WB_H, d->ntype ? d->sigma2 : &zero, d->ntype ? TT : 1, &ds2));
-    OTRY(upload(w, WB_I, d->ntype ? d->four_eps : &zero, d->ntype ? TT : 1, &dfe));
-    OTRY(upload(w, WB_J, d->ntype ? d->sigma6 : &zero, d->ntype ? TT : 1, &ds6));
-    if (d->full) {
-        dxm = dx; dym = dy; dzm = dz;
-    } else {
-        if (!d->xmol || !d->ymol || !d->zmol) return ofail(YAMC_ERROR_PARAMETER, "null molecule positions");
-        std::vector<double> m(3 * (size_t)nrow);
+    if (!d->xx || !d->yy || !d->zz || !d->charge || !d->hard_core_angstr || !d->molnum || !d->lj_type ||
+        (d->ntype && (!d->sigma2 || !d->four_eps || !d->sigma6)))
+        return ofail(YAMC_ERROR_PARAMETER, "null atom array");
+    if (!d->full && (!d->xmol || !d->ymol || !d->zmol)) return ofail(YAMC_ERROR_PARAMETER, "null molecule positions");
+    for (int a = 0; a < N; ++a)  // the pair tables are indexed by type
+        if (d->lj_type[a] < 0 || d->lj_type[a] > d->ntype)
+            return ofail(YAMC_ERROR_PARAMETER, "LJ type %d of atom %d outside 0..%d", d->lj_type[a], a, d->ntype);
+    const double zero3[3] = {0.0, 0.0, 0.0};
+    const unsigned long long kc0[2] = {~0ull, 0ull};  // first overlap key, number of overlapping pairs
+    size_t ox, oy, oz, oq, oh, omol, otyp, os2, ofe, os6, om, okc;
+    stage_begin(w);
+    OTRY(stage_put(w, d->xx, sizeof(double) * N, &ox));
+    OTRY(stage_put(w, d->yy, sizeof(double) * N, &oy));
+    OTRY(stage_put(w, d->zz, sizeof(double) * N, &oz));
+    OTRY(stage_put(w, d->charge, sizeof(double) * N, &oq));
+    OTRY(stage_put(w, d->hard_core_angstr, sizeof(double) * N, &oh));
+    OTRY(stage_put(w, d->molnum, sizeof(int32_t) * N, &omol));
+    OTRY(stage_put(w, d->lj_type, sizeof(int32_t) * N, &otyp));
+    OTRY(stage_put(w, d->ntype ? d->sigma2 : zero3, sizeof(double) * (d->ntype ? TT : 1), &os2));
+    OTRY(stage_put(w, d->ntype ? d->four_eps : zero3, sizeof(double) * (d->ntype ? TT : 1), &ofe));
+    OTRY(stage_put(w, d->ntype ? d->sigma6 : zero3, sizeof(double) * (d->ntype ? TT : 1), &os6));
+    om = ox;
+    if (!d->full) {
+        OTRY(stage_put(w, nullptr, sizeof(double) * 3 * nrow, &om));
+        double *m = (double *)(w->h_in + om);
         for (int a = 0; a < nrow; ++a) { m[a] = d->xmol[a]; m[nrow + a] = d->ymol[a]; m[2 * nrow + a] = d->zmol[a]; }
-        double *dm;
-        OTRY(upload(w, WB_K, m.data(), m.size(), &dm));
-        dxm = dm; dym = dm + nrow; dzm = dm + 2 * nrow;
     }
-    r.x = dx; r.y = dy; r.z = dz; r.q = dq; r.dh = dh; r.s2 = ds2; r.fe = dfe; r.s6 = ds6;
-    r.xm = dxm; r.ym = dym; r.zm = dzm; r.mol = dmol; r.typ = dtyp;
+    OTRY(stage_put(w, kc0, sizeof kc0, &okc));
+    OTRY(stage_flush(w));
+    r.x = stage_dev<double>(w, ox); r.y = stage_dev<double>(w, oy); r.z = stage_dev<double>(w, oz);
+    r.q = stage_dev<double>(w, oq); r.dh = stage_dev<double>(w, oh);
+    r.s2 = stage_dev<double>(w, os2); r.fe = stage_dev<double>(w, ofe); r.s6 = stage_dev<double>(w, os6);
+    r.mol = stage_dev<int>(w, omol); r.typ = stage_dev<int>(w, otyp);
+    if (d->full) {
+        r.xm = r.x; r.ym = r.y; r.zm = r.z;
+    } else {
+        r.xm = stage_dev<double>(w, om); r.ym = r.xm + nrow; r.zm = r.xm + 2 * nrow;
+    }
+    unsigned long long *dkc = stage_dev<unsigned long long>(w, okc);
     // pass 1: the first overlap in the reference's loop order
-    unsigned long long kc[2] = {~0ull, 0ull}, *dkey;  // first overlap key, number of overlapping pairs
-    OTRY(upload(w, WB_KEY, kc, 2, &dkey));
-    {
-        dim3 g((N + 255) / 256, nrow);
-        yo_rs_first_overlap<<<g, 256, 0, w->s>>>(r, dkey);
-        OCHK(hipGetLastError());
-        OCHK(hipMemcpyAsync(kc, dkey, sizeof kc, hipMemcpyDeviceToHost, w->s));
-        OTRY(wsync(w));
-    }
-    const unsigned long long key = kc[0];
-    const bool ov = key != ~0ull;
-    const unsigned long long limit = ov && r.stop ? key : ~0ull;
-    // pass 2: rows, about 2^18 (row, group) threads
+    yo_rs_first_overlap<<<dim3((N + 255) / 256, nrow), 256, 0, w->s>>>(r, dkc);
+    OCHK(hipGetLastError());
+    // pass 2: rows, about 2^18 (row, group) threads, partials pat[ngr][4][nrow];
+    // pass 3: columns, about 2^18 (atom, group) threads, col[ngc][4][N]; the
+    // groups are summed on the device.  Output block: the two key words,
+    // part[ngr][nbr][6], row sums [4][nrow], column sums [4][N].
     int ngr = (1 << 18) / nrow;
     ngr = ngr < 1 ? 1 : ngr > 256 ? 256 : ngr;
     if (ngr > N) ngr = N;
     const int nbr = (nrow + 255) / 256;
-    void *ppat = nullptr, *ppart = nullptr;
-    OTRY(wbuf(w, WB_L, sizeof(double) * 4 * (size_t)nrow * ngr, &ppat));
-    OTRY(wbuf(w, WB_OUT, sizeof(double) * 6 * (size_t)nbr * ngr, &ppart));
-    yo_rs_rows<<<dim3(nbr, ngr), 256, 0, w->s>>>(r, ngr, limit, (double *)ppat, (double *)ppart);
-    OCHK(hipGetLastError());
-    std::vector<double> hpat(4 * (size_t)nrow * ngr), hpart(6 * (size_t)nbr * ngr);
-    OCHK(hipMemcpyAsync(hpat.data(), ppat, sizeof(double) * hpat.size(), hipMemcpyDeviceToHost, w->s));
-    OCHK(hipMemcpyAsync(hpart.data(), ppart, sizeof(double) * hpart.size(), hipMemcpyDeviceToHost, w->s));
-    OTRY(wsync(w));
-    // pass 3: columns, about 2^18 (atom, group) threads
     int ngc = (1 << 18) / N;
     ngc = ngc < 1 ? 1 : ngc > 64 ? 64 : ngc;
     if (ngc > nrow) ngc = nrow;
-    void *pcol = nullptr;  // WB_OUT: the row pass's block sums are already on the host
-    OTRY(wbuf(w, WB_OUT, sizeof(double) * 4 * (size_t)N * ngc, &pcol));
-    yo_rs_cols<<<dim3((N + 255) / 256, ngc), 256, 0, w->s>>>(r, ngc, limit, (double *)pcol);
+    const size_t npat = 4 * (size_t)nrow * ngr, npart = 6 * (size_t)nbr * ngr, ncol = 4 * (size_t)N * ngc;
+    const size_t nrowsum = 4 * (size_t)nrow, ncolsum = 4 * (size_t)N;
+    void *pgrp = nullptr, *pout = nullptr;
+    OTRY(wbuf(w, WB_L, sizeof(double) * (npat + ncol), &pgrp));
+    OTRY(wbuf(w, WB_OUT, sizeof(double) * (2 + npart + nrowsum + ncolsum), &pout));
+    double *dpat = (double *)pgrp, *dcol = dpat + npat;
+    double *dpart = (double *)pout + 2, *drow = dpart + npart, *dcolsum = drow + nrowsum;
+    yo_rs_rows<<<dim3(nbr, ngr), 256, 0, w->s>>>(r, ngr, dkc, dpat, dpart);
     OCHK(hipGetLastError());
-    std::vector<double> hcol(4 * (size_t)N * ngc);
-    OCHK(hipMemcpyAsync(hcol.data(), pcol, sizeof(double) * hcol.size(), hipMemcpyDeviceToHost, w->s));
-    OTRY(wsync(w));
+    yo_rs_cols<<<dim3((N + 255) / 256, ngc), 256, 0, w->s>>>(r, ngc, dkc, dcol);
+    OCHK(hipGetLastError());
+    yo_group_sum<<<(unsigned)((nrowsum + 255) / 256), 256, 0, w->s>>>(dpat, ngr, nrowsum, drow);
+    yo_group_sum<<<(unsigned)((ncolsum + 255) / 256), 256, 0, w->s>>>(dcol, ngc, ncolsum, dcolsum);
+    OCHK(hipGetLastError());
+    OCHK(hipMemcpyAsync(pout, dkc, sizeof kc0, hipMemcpyDeviceToDevice, w->s));
+    const char *hc;
+    OTRY(fetch(w, pout, sizeof(double) * (2 + npart + nrowsum + ncolsum), &hc));
+    unsigned long long kc[2];
+    memcpy(kc, hc, sizeof kc);
+    const double *hpart = (const double *)hc + 2, *row = hpart + npart, *col = row + nrowsum;
+    const bool ov = kc[0] != ~0ull;
     for (int m = 0; m < 6; ++m) {
         double a = 0.0;
         for (size_t b = 0; b < (size_t)nbr * ngr; ++b) a = a + hpart[b * 6 + m];
         comps[m] = a;
     }
-    std::vector<double> row(4 * (size_t)nrow, 0.0), col(4 * (size_t)N, 0.0);
-    for (int g = 0; g < ngr; ++g)
-        for (size_t t = 0; t < row.size(); ++t) row[t] = row[t] + hpat[(size_t)g * 4 * nrow + t];
-    for (int g = 0; g < ngc; ++g)
-        for (size_t t = 0; t < col.size(); ++t) col[t] = col[t] + hcol[(size_t)g * 4 * N + t];
     double *ro[4] = {uu, fx, fy, fz}, *co[4] = {uu_back, fx_back, fy_back, fz_back};
     for (int q = 0; q < 4; ++q) {
         if (ro[q])
@@ -872,8 +989,10 @@ extern "C" int yamc_op_pair(yamc_ws *w, double alpha, double xclb, double r2, in
     OTRY(wbuf(w, WB_OUT, sizeof(double) * 14, &po));
     yo_pair<<<1, 64, 0, w->s>>>(p, r2, same_molecule, qi, qj, lj, sigma6, four_eps, (double *)po);
     OCHK(hipGetLastError());
-    OCHK(hipMemcpyAsync(out, po, sizeof(double) * 14, hipMemcpyDeviceToHost, w->s));
-    return wsync(w);
+    const char *h;
+    OTRY(fetch(w, po, sizeof(double) * 14, &h));
+    memcpy(out, h, sizeof(double) * 14);
+    return YAMC_OK;
 }
 
 // ---- constant terms ------------------------------------------------------------
@@ -913,19 +1032,22 @@ extern "C" int yamc_op_tails(yamc_ws *w, int32_t natom, const double *charge, in
     const int TT = ntype * ntype > 0 ? ntype * ntype : 1;
     const double zero = 0.0;
     const int izero = 0;
-    double *dq, *d6, *d12;
-    int *dc;
-    OTRY(upload(w, WB_A, natom ? charge : &zero, natom ? natom : 1, &dq));
-    OTRY(upload(w, WB_B, ntype ? lj6 : &zero, ntype ? TT : 1, &d6));
-    OTRY(upload(w, WB_C, ntype ? lj12 : &zero, ntype ? TT : 1, &d12));
-    OTRY(upload(w, WB_D, ntype ? counts : &izero, ntype ? ntype : 1, &dc));
+    size_t oq, o6, o12, oc;
+    stage_begin(w);
+    OTRY(stage_put(w, natom ? charge : &zero, sizeof(double) * (natom ? natom : 1), &oq));
+    OTRY(stage_put(w, ntype ? lj6 : &zero, sizeof(double) * (ntype ? TT : 1), &o6));
+    OTRY(stage_put(w, ntype ? lj12 : &zero, sizeof(double) * (ntype ? TT : 1), &o12));
+    OTRY(stage_put(w, ntype ? counts : &izero, sizeof(int32_t) * (ntype ? ntype : 1), &oc));
+    OTRY(stage_flush(w));
     void *po = nullptr;
     OTRY(wbuf(w, WB_OUT, sizeof(double) * 3, &po));
-    yo_tails<<<1, 64, 0, w->s>>>(natom, dq, ntype, d6, d12, dc, alpha, xclb, (double *)po);
+    yo_tails<<<1, 64, 0, w->s>>>(natom, stage_dev<double>(w, oq), ntype, stage_dev<double>(w, o6),
+                                 stage_dev<double>(w, o12), stage_dev<int>(w, oc), alpha, xclb, (double *)po);
     OCHK(hipGetLastError());
+    const char *h;
+    OTRY(fetch(w, po, sizeof(double) * 3, &h));
     double o[3];
-    OCHK(hipMemcpyAsync(o, po, sizeof o, hipMemcpyDeviceToHost, w->s));
-    OTRY(wsync(w));
+    memcpy(o, h, sizeof o);
     if (self_c) *self_c = o[0];
     if (lj6_tail) *lj6_tail = o[1];
     if (lj12_tail) *lj12_tail = o[2];
