@@ -418,6 +418,11 @@ module yamc_ewald
          type(c_ptr), value :: ctx
          integer(c_int32_t), value :: block
       end function
+      integer(c_int) function yamc_set_chain_groups(ctx, groups) bind(C, name="yamc_set_chain_groups")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: groups
+      end function
       integer(c_int) function yamc_last_kernel_ms(ctx, ms) bind(C, name="yamc_last_kernel_ms")
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: ctx

@@ -418,6 +418,16 @@ int yamc_op_tails(yamc_ws *ws, int32_t natom, const double *charge, int32_t ntyp
  * reference counterpart (the reference is serial); the autotuner
  * (yamc_amd.tune) measures them. */
 int yamc_set_block_size(yamc_ctx *ctx, int32_t block);
+/* Workgroups per chain for this context's single-chain runs (yamc_mc_run,
+ * yamc_mc_run_const, exchange and NPT runs): 1 = one workgroup (default);
+ * G > 1 = one chain spread over G co-resident workgroups of 256 threads (a
+ * cooperative launch) that split every k sweep and S(k) commit by k range
+ * and the real-space pass by atom range, with two grid barriers per move
+ * (the loops of EwaldSumKSpace.f90:528-753 and EwaldSumRealSpace.f90:
+ * 367-496 split across CUs).  Same decisions and results as G = 1 up to the
+ * summation order of the partial sums.  Batches always run one workgroup
+ * per chain.  No reference counterpart (the reference is serial). */
+int yamc_set_chain_groups(yamc_ctx *ctx, int32_t groups);
 
 /* ---- diagnostics ------------------------------------------------------- */
 /* Time of the last yamc_mc_run / yamc_batch_run kernel launch in ms (HIP

@@ -34,6 +34,11 @@ __global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, do
 template <bool F32, int BS>
 __global__ void yk_chain(const YdChain *chains, const double *const *sig6s, const double *const *feps_s, int nsteps,
                          int mode, int max_moves, int lds_atoms, int resync_every);
+template <bool F32, int BS>
+__global__ void yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nsteps, int mode,
+                          int max_moves, int lds_atoms, unsigned int *bar, double *part);
+__global__ void yk_gchain_prep(const YdChain *src, YdChain *dst, int G, double *rsb, unsigned int *bar);
+#define YG_PART 64  // yk_gchain: doubles per workgroup per phase of the partials buffer
 template <bool F32, int OCC>
 __global__ void yk_wchain(const YdChain *chains, int nchains, const double *const *sig6s, const double *const *feps_s,
                           int ncycles);  // FP64 instantiations only
@@ -103,6 +108,12 @@ struct yamc_ctx {
     const double **d_sig6s = nullptr, **d_feps = nullptr;
     bool beta_ready = false, state_ready = false;
     int chain_bs = 256;  // launch shape of the chain kernel (yamc_set_block_size)
+    // grid chain (yamc_set_chain_groups): workgroups per chain and their
+    // buffers (per-workgroup descriptors, back terms, partial sums, barrier)
+    int groups = 1, gcap = 0;
+    YdChain *d_gch = nullptr;
+    double *d_grsb = nullptr, *d_gpart = nullptr;
+    unsigned int *d_gbar = nullptr;
     // host copies of the box-dependent inputs, scaled to the current box
     std::vector<double> hard_core_angstr, lj6, lj12, sigma6;
     // device copies of the state kept across a volume step (allocated on first use)
@@ -889,6 +900,46 @@ static int launch_chain_t(yamc_ctx *c, const YdChain *d_chain, const double *con
     return YAMC_OK;
 }
 
+// One chain over c->groups co-resident workgroups (cooperative launch).
+template <bool F32>
+static int launch_gchain(yamc_ctx *c, const YdChain *d_chain, int nsteps, int mode, int max_natom, int max_moves) {
+    const int G = c->groups;
+    const void *fn = (const void *)yk_gchain<F32, 256>;
+    int lds_max = 0, ncu = 0, coop = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device));
+    if (!coop) return fail(YAMC_ERROR_DEVICE, "the device has no cooperative launch: use one workgroup per chain");
+    hipFuncAttributes fa;
+    HIPCHK(hipFuncGetAttributes(&fa, fn));
+    // the packed positions go to LDS when they fit there (each workgroup
+    // keeps a whole copy); else they stay in HBM (c.pos4)
+    const size_t want = (size_t)max_natom * 4;
+    const size_t room = G > ncu ? (size_t)lds_max / 2 : (size_t)lds_max;
+    int lds_atoms = 0;
+    size_t dyn = 0;
+    if (want + fa.sharedSizeBytes + 1024 <= room) {
+        lds_atoms = max_natom;
+        dyn = want;
+        HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    }
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, dyn));
+    if (per_cu < 1 || G > per_cu * ncu)
+        return fail(YAMC_ERROR_LIMITS, "%d workgroups per chain do not fit on the device at once (%d per CU, %d CUs)",
+                    G, per_cu, ncu);
+    yk_gchain_prep<<<(G + 63) / 64, 64, 0, c->stream>>>(d_chain, c->d_gch, G, c->d_grsb, c->d_gbar);
+    HIPCHK(hipGetLastError());
+    const YdChain *chains = c->d_gch;
+    const double *sig6 = c->d_sig6, *feps = c->d_4eps;
+    unsigned int *bar = c->d_gbar;
+    double *part = c->d_gpart;
+    void *args[] = {(void *)&chains, (void *)&sig6, (void *)&feps, (void *)&nsteps, (void *)&mode,
+                    (void *)&max_moves, (void *)&lds_atoms, (void *)&bar, (void *)&part};
+    HIPCHK(hipLaunchCooperativeKernel(fn, dim3(G), dim3(256), args, (unsigned int)dyn, c->stream));
+    return YAMC_OK;
+}
+
 // The wave shapes (64, 128) run the wave kernel for FP64 run_mc cycles;
 // FP32C chains, constant-uniform traces and runs with exchange steps (which
 // launch step ranges, max_moves > 0) run them as workgroups of 256
@@ -898,6 +949,9 @@ static int launch_chain(yamc_ctx *c, const YdChain *d_chain, const double *const
                         const double *const *feps, int nchains, int nsteps, int mode, int max_natom,
                         int max_moves = 0, int resync_every = 0) {
     const bool f32 = c->precision == YAMC_FP32C;
+    if (c->groups > 1 && nchains == 1 && resync_every == 0)
+        return f32 ? launch_gchain<true>(c, d_chain, nsteps, mode, max_natom, max_moves)
+                   : launch_gchain<false>(c, d_chain, nsteps, mode, max_natom, max_moves);
     if ((c->chain_bs == 64 || c->chain_bs == 128) && mode == 0 && !f32 && max_moves == 0 &&
         resync_every == 0) {  // one wave per replica
         int nb = (nchains + YW_WPB - 1) / YW_WPB;
@@ -923,6 +977,31 @@ extern "C" int yamc_set_block_size(yamc_ctx *c, int32_t block) {
         return fail(YAMC_ERROR_PARAMETER, "launch shape %d: must be 64 or 128 (one wave per replica), 256 or 512",
                     block);
     c->chain_bs = block;
+    return YAMC_OK;
+}
+
+extern "C" int yamc_set_chain_groups(yamc_ctx *c, int32_t groups) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    if (groups < 1 || groups > 256)
+        return fail(YAMC_ERROR_PARAMETER, "%d workgroups per chain: must be 1..256", groups);
+    if (groups > c->nk || groups > c->natom)
+        return fail(YAMC_ERROR_PARAMETER, "%d workgroups per chain for %d k vectors and %d atoms", groups, c->nk,
+                    c->natom);
+    if (groups > c->gcap) {
+        TRY(dalloc(c, &c->d_gch, (size_t)groups));
+        TRY(dalloc(c, &c->d_grsb, (size_t)groups * YD_NBCAP * 4));
+        TRY(dalloc(c, &c->d_gpart, (size_t)2 * groups * YG_PART));
+        TRY(dalloc(c, &c->d_gbar, 2));
+        c->gcap = groups;
+    }
+    c->groups = groups;
+    return YAMC_OK;
+}
+
+// A grid chain whose barrier wait ran past its bound stopped early.
+static int check_grid(yamc_ctx *c, const YdState &st) {
+    if (st.grid_fail) return fail(YAMC_ERROR_DEVICE, "grid chain stopped: a barrier wait ran past its bound");
+    (void)c;
     return YAMC_OK;
 }
 
@@ -989,6 +1068,7 @@ extern "C" int yamc_mc_run(yamc_ctx *c, int32_t ncycles, int32_t resync_every, y
     if (n_accepted) *n_accepted = st1.n_accepted - st0.n_accepted;
     HIPCHK(hipMemcpyAsync(c->d_chain, &c->ch, sizeof(YdChain), hipMemcpyHostToDevice, c->stream));
     TRY(sync(c));
+    TRY(check_grid(c, st1));
     return check_list(c);
 }
 
@@ -1022,7 +1102,7 @@ extern "C" int yamc_mc_run_const(yamc_ctx *c, int32_t nmoves, const int32_t *imo
     TRY(sync(c));
     HIPCHK(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
     if (n_accepted) *n_accepted = st1.n_accepted - st0.n_accepted;
-    return YAMC_OK;
+    return check_grid(c, st1);
 }
 
 // Staged batch: the chain descriptors of n contexts on the device.  A sweep

@@ -50,6 +50,8 @@ struct YdState {
     unsigned long long nlist;
     long long n_moves, n_accepted, n_overlap, since_resync;
     unsigned long long phase[10];  // wall-clock ticks per chain-kernel phase (thread 0)
+    int grid_fail;                 // a grid chain's barrier wait ran past its bound (the chain stopped)
+    int grid_pad;
 };
 
 // The two molecules of an exchange step (mc_xchange, MC.f90:1325-1954):

@@ -768,19 +768,19 @@ __device__ __forceinline__ void yk_commit_load(const YdChain &c, const YkMol &m,
 // (a different kk, so they never alias), which keeps one k of loads in
 // flight while the current one is updated.
 template <bool F32>
-__device__ void yk_commit(const YdChain &c, const YkMol &m, int rank, int size) {
-    const int nk = c.p.nk;
+__device__ void yk_commit(const YdChain &c, const YkMol &m, int rank, int size, int kend = -1) {
+    const int nk = c.p.nk, ke = kend >= 0 ? kend : nk;
     int kxyz_n = 0;
     double vn[1 + YK_MAXP][8];
-    if (rank < nk) yk_commit_load<F32>(c, m, rank, kxyz_n, vn);
-    for (int kk = rank; kk < nk; kk += size) {
+    if (rank < ke) yk_commit_load<F32>(c, m, rank, kxyz_n, vn);
+    for (int kk = rank; kk < ke; kk += size) {
         const int kxyz = kxyz_n, kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
         double v[1 + YK_MAXP][8];
 #pragma unroll
         for (int set = 0; set <= YK_MAXP; ++set)
 #pragma unroll
             for (int i = 0; i < 8; ++i) v[set][i] = vn[set][i];
-        if (kk + size < nk) yk_commit_load<F32>(c, m, kk + size, kxyz_n, vn);
+        if (kk + size < ke) yk_commit_load<F32>(c, m, kk + size, kxyz_n, vn);
 #pragma unroll
         for (int set = 0; set <= YK_MAXP; ++set) {
             if (set > m.np) break;
@@ -1023,7 +1023,7 @@ __device__ __forceinline__ bool yk_near(const float (&xof)[NA], const float (&yo
 // arrays; all threads call this.
 template <int NA>
 __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int *cl_cell,
-                                 int *cl_start, int *ncl_out) {
+                                 int *cl_start, int *ncl_out, int a0 = 0, int a1 = 0x7fffffff) {
     const int nc = c.nc, tid = threadIdx.x;
     const unsigned int f0 = (unsigned int)m.first, na = (unsigned int)m.na;
     int start[3], len[3];
@@ -1094,7 +1094,10 @@ __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigne
         }
         unsigned int pj[YK_SCANU];
 #pragma unroll
-        for (int u = 0; u < YK_SCANU; ++u) pj[u] = jj[u] >= 0 ? pos[jj[u]] : 0u;
+        for (int u = 0; u < YK_SCANU; ++u) {  // atoms outside [a0, a1) belong to another workgroup's scan
+            if (jj[u] >= 0 && (unsigned int)(jj[u] - a0) >= (unsigned int)(a1 - a0)) jj[u] = -1;
+            pj[u] = jj[u] >= 0 ? pos[jj[u]] : 0u;
+        }
 #pragma unroll
         for (int u = 0; u < YK_SCANU; ++u) {
             const int j = jj[u];
@@ -1109,8 +1112,9 @@ __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigne
 // pos: packed positions (yk_pack_pos); the molecule's own atoms, indices
 // first .. first + na - 1, are skipped.
 template <int NA>
-__device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm) {
-    const int N = c.p.natom, me = m.imol + 1;
+__device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int a0 = 0,
+                           int a1 = -1) {
+    const int N = a1 >= 0 ? a1 : c.p.natom, me = m.imol + 1;
     if (pos) {
         const float r2f = yk_scan_r2(c);
         const unsigned int f0 = (unsigned int)m.first, na = (unsigned int)m.na;
@@ -1123,7 +1127,7 @@ __device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const unsigned int 
         // YK_SCANU positions per thread are loaded before any is tested, so
         // that many loads are in flight at once when pos is in HBM
         const int bs = (int)blockDim.x;
-        for (int j0 = threadIdx.x; j0 < N; j0 += YK_SCANU * bs) {
+        for (int j0 = a0 + (int)threadIdx.x; j0 < N; j0 += YK_SCANU * bs) {
             unsigned int pj[YK_SCANU];
 #pragma unroll
             for (int u = 0; u < YK_SCANU; ++u) {
@@ -1139,7 +1143,7 @@ __device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const unsigned int 
             }
         }
     } else {
-        for (int j = threadIdx.x; j < N; j += (int)blockDim.x) {
+        for (int j = a0 + (int)threadIdx.x; j < N; j += (int)blockDim.x) {
             double xj = c.x[j], yj = c.y[j], zj = c.z[j];
             int jm = c.molnum[j];
             bool near = false;
@@ -1259,21 +1263,22 @@ __device__ void yk_rs_commit_list(const YdChain &c, const int *list, int count, 
 }
 
 __device__ __forceinline__ void yk_rs_scan_n(const YdChain &c, const YkMol &m, const unsigned int *pos,
-                                             unsigned int *bm) {
+                                             unsigned int *bm, int a0 = 0, int a1 = -1) {
     switch (m.na) {
-        case 1: yk_rs_scan<1>(c, m, pos, bm); break;
-        case 2: yk_rs_scan<2>(c, m, pos, bm); break;
-        case 3: yk_rs_scan<3>(c, m, pos, bm); break;
-        default: yk_rs_scan<4>(c, m, pos, bm); break;
+        case 1: yk_rs_scan<1>(c, m, pos, bm, a0, a1); break;
+        case 2: yk_rs_scan<2>(c, m, pos, bm, a0, a1); break;
+        case 3: yk_rs_scan<3>(c, m, pos, bm, a0, a1); break;
+        default: yk_rs_scan<4>(c, m, pos, bm, a0, a1); break;
     }
 }
 __device__ __forceinline__ bool yk_rs_scan_cells_n(const YdChain &c, const YkMol &m, const unsigned int *pos,
-                                                   unsigned int *bm, int *cl_cell, int *cl_start, int *ncl) {
+                                                   unsigned int *bm, int *cl_cell, int *cl_start, int *ncl,
+                                                   int a0 = 0, int a1 = 0x7fffffff) {
     switch (__builtin_amdgcn_readfirstlane(m.na)) {
-        case 1: return yk_rs_scan_cells<1>(c, m, pos, bm, cl_cell, cl_start, ncl);
-        case 2: return yk_rs_scan_cells<2>(c, m, pos, bm, cl_cell, cl_start, ncl);
-        case 3: return yk_rs_scan_cells<3>(c, m, pos, bm, cl_cell, cl_start, ncl);
-        default: return yk_rs_scan_cells<4>(c, m, pos, bm, cl_cell, cl_start, ncl);
+        case 1: return yk_rs_scan_cells<1>(c, m, pos, bm, cl_cell, cl_start, ncl, a0, a1);
+        case 2: return yk_rs_scan_cells<2>(c, m, pos, bm, cl_cell, cl_start, ncl, a0, a1);
+        case 3: return yk_rs_scan_cells<3>(c, m, pos, bm, cl_cell, cl_start, ncl, a0, a1);
+        default: return yk_rs_scan_cells<4>(c, m, pos, bm, cl_cell, cl_start, ncl, a0, a1);
     }
 }
 
@@ -1322,9 +1327,12 @@ __device__ void yk_cells_move(const YdChain &c, const YkMol &m, int *ok) {
 // Ascending compaction of the candidate bitmap into list[] (deterministic:
 // thread t owns a contiguous run of words, offsets from a block prefix sum).
 // *count gets the number of candidates (may exceed YK_NBCAP: list partial).
-__device__ void yk_compact(const unsigned int *bm, int nwords, int *list, int *count, int *wsum) {
+// wb: the first word of the range compacted (a grid chain's workgroup
+// compacts the words of its own atoms).
+__device__ void yk_compact(const unsigned int *bm, int nwords, int *list, int *count, int *wsum, int wb = 0) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int bs = blockDim.x, per = (nwords + bs - 1) / bs, w0 = tid * per;
+    const int bs = blockDim.x, per = (nwords + bs - 1) / bs, w0 = wb + tid * per;
+    nwords += wb;
     int cnt = 0;
     for (int k = 0; k < per; ++k)
         if (w0 + k < nwords) cnt += __popc(bm[w0 + k]);
@@ -1673,9 +1681,12 @@ __device__ int yk_choose(const YdChain &c, double rnd) {
 // DELTA adds dU_C (acc[0]) and evaluates the forces against S + dS.
 // The loads of a thread's next k (index, beta, S) are issued before the
 // current k's arithmetic.
+// kend (>= 0): the sweep stops there instead of at nk (a workgroup of a
+// grid chain sweeps its own range of k).
 template <bool F32, bool DELTA, int NA>
-__device__ void yl_sweep_coul(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size) {
-    const int nk = c.p.nk;
+__device__ void yl_sweep_coul(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size,
+                              int kend = -1) {
+    const int nk = kend >= 0 ? kend : c.p.nk;
     const double *tabs = DELTA ? m.tab_n : m.tab_o;
     int kxyz_n = 0;
     double bt_n = 0.0;
@@ -1735,17 +1746,18 @@ __device__ void yl_sweep_coul(const YdChain &c, const YkMol &m, double (&acc)[YK
 // against every LJ set (+ its delta).  The LJ q-weighted sin/cos of a
 // charged site is its Coulomb one times 1/q (RhoSquared.f90:173-176).
 template <bool F32, bool DELTA, int NA>
-__device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size) {
-    const int nk = c.p.nk, T = c.p.ntype;
+__device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size,
+                            int kend = -1) {
+    const int nk = c.p.nk, T = c.p.ntype, ke = kend >= 0 ? kend : nk;
     if (T == 0 || m.np == 0) return;
     const double *tabs = DELTA ? m.tab_n : m.tab_o;
     // the next k's index is loaded ahead; the S rows are loaded per set (a
     // prefetch of the next set's row costs registers this loop does not have)
     int kxyz_n = 0;
-    if (rank < nk) kxyz_n = c.kxyz[rank];
-    for (int kk = rank; kk < nk; kk += size) {
+    if (rank < ke) kxyz_n = c.kxyz[rank];
+    for (int kk = rank; kk < ke; kk += size) {
         const int kxyz = kxyz_n, kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
-        if (kk + size < nk) kxyz_n = c.kxyz[kk + size];
+        if (kk + size < ke) kxyz_n = c.kxyz[kk + size];
         double kl[NA][8];
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
@@ -1837,12 +1849,12 @@ __device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_N
 
 template <bool F32, bool DELTA>
 __device__ __forceinline__ void yl_sweep_n(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank,
-                                           int size) {
+                                           int size, int kend = -1) {
     switch (__builtin_amdgcn_readfirstlane(m.na)) {
-        case 1: yl_sweep_coul<F32, DELTA, 1>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 1>(c, m, acc, rank, size); break;
-        case 2: yl_sweep_coul<F32, DELTA, 2>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 2>(c, m, acc, rank, size); break;
-        case 3: yl_sweep_coul<F32, DELTA, 3>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 3>(c, m, acc, rank, size); break;
-        default: yl_sweep_coul<F32, DELTA, 4>(c, m, acc, rank, size); yl_sweep_lj<F32, DELTA, 4>(c, m, acc, rank, size); break;
+        case 1: yl_sweep_coul<F32, DELTA, 1>(c, m, acc, rank, size, kend); yl_sweep_lj<F32, DELTA, 1>(c, m, acc, rank, size, kend); break;
+        case 2: yl_sweep_coul<F32, DELTA, 2>(c, m, acc, rank, size, kend); yl_sweep_lj<F32, DELTA, 2>(c, m, acc, rank, size, kend); break;
+        case 3: yl_sweep_coul<F32, DELTA, 3>(c, m, acc, rank, size, kend); yl_sweep_lj<F32, DELTA, 3>(c, m, acc, rank, size, kend); break;
+        default: yl_sweep_coul<F32, DELTA, 4>(c, m, acc, rank, size, kend); yl_sweep_lj<F32, DELTA, 4>(c, m, acc, rank, size, kend); break;
     }
 }
 
@@ -2628,6 +2640,413 @@ YK_CHAIN_INST(false, 256)
 YK_CHAIN_INST(false, 512)
 YK_CHAIN_INST(true, 256)
 YK_CHAIN_INST(true, 512)
+
+// ========================================================================
+// Grid chain: one Markov chain spread over G co-resident workgroups (a
+// cooperative launch), for systems too large for one CU to step quickly.
+// Workgroup g owns a contiguous range of k vectors (its share of every
+// sweep and of the S(k) commit: the same range every move, so its S(k)
+// values are only ever read and written by it) and a contiguous range of
+// atoms (its share of the real-space scan, exact pass and back-term commit).
+// Every workgroup runs the scalar sections itself (pick, proposal,
+// acceptance) from the same RNG state and the same reduced sums, so all of
+// them take the same decisions; workgroup 0 alone writes the shared state
+// (positions and per-atom arrays of the moved molecule, energies, records,
+// cell buckets).  Two grid barriers per move:
+//   B1 after the old sweep: the per-workgroup sums of the old k-space forces;
+//   B2 after the real-space pass and the new sweep: their sums.
+// The partial sums of the workgroups are added in workgroup order.
+//
+// Ordering of the shared data: a workgroup's writes of move m (its S(k)
+// range, its atoms' back terms, workgroup 0's finalize) are released before
+// it arrives at B1 of move m + 1 and acquired by every workgroup leaving
+// it.  Reads that precede B1 in a move (the picked molecule's positions for
+// the old sweep) can race only with workgroup 0's finalize of the previous
+// move; every workgroup therefore keeps that molecule's new positions in
+// its own LDS for one move and takes them from there.  The per-site
+// real-space arrays of the picked molecule are read after B1.
+#define YG_PART 64                 // doubles per workgroup per phase in the partials buffer
+#define YG_SPIN (1u << 22)         // bound of a barrier wait, in polls (seconds)
+
+struct YgShared {
+    YkShared s;
+    double out2[YK_NRX + 1 + YK_NACC];              // the reduced real-space and new-sweep sums
+    double lx[YD_MAXA], ly[YD_MAXA], lz[YD_MAXA];  // the last finalized molecule's positions
+    int last_imol;                                  // -1: none in the previous move
+    int gen, sync_ok;
+};
+
+// Grid-wide barrier of the G workgroups of a chain: bar[0] counts arrivals,
+// bar[1] is the generation.  Thread 0 releases the workgroup's writes at
+// agent scope (they reach the L2 of every XCD), arrives, waits for the
+// generation to move on and acquires.  The wait is bounded: past YG_SPIN
+// polls it returns false, and the caller stops the chain (every workgroup
+// then stops at the same barrier or the next one).
+__device__ bool yg_sync(unsigned int *bar, unsigned int G, int &gen, int *ok_lds) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        const unsigned int g0 = (unsigned int)gen;
+        __threadfence();
+        const unsigned int a = atomicAdd(&bar[0], 1u);
+        if (a == G - 1) {
+            atomicExch(&bar[0], 0u);
+            __threadfence();
+            atomicAdd(&bar[1], 1u);
+        } else {
+            unsigned int polls = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+                if (++polls > YG_SPIN) { ok = 0; break; }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __threadfence();
+        gen = (int)(g0 + 1);
+        *ok_lds = ok;
+    }
+    __syncthreads();
+    return *ok_lds != 0;
+}
+
+// Sums of the workgroups' partial vectors part[g * YG_PART + off + i],
+// i < n, in workgroup order into out[i] (threads i < n).
+__device__ __forceinline__ void yg_reduce(const double *part, int G, int off, int n, double *out) {
+    if ((int)threadIdx.x < n) {
+        double a = 0.0;
+        for (int g = 0; g < G; ++g)
+            a += __hip_atomic_load(part + (size_t)g * YG_PART + off + threadIdx.x, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        out[threadIdx.x] = a;
+    }
+    __syncthreads();
+}
+
+// Full FP64 pass over atoms [a0, a1) when the workgroup's candidate list
+// overflowed: the new sites' partial sums and overlap test and the old
+// sites' components, like the exact pass, without stored back terms (the
+// commit then recomputes them over the same range).
+template <int NA>
+__device__ void yg_rs_full(const YdChain &c, const YkMol &m, const double *sig6, const double *feps, int a0,
+                           int a1, double (&acc)[YK_NRX], int &ov) {
+    const int T = c.p.ntype, me = m.imol + 1;
+    for (int j = a0 + (int)threadIdx.x; j < a1; j += (int)blockDim.x) {
+        if (c.molnum[j] == me) continue;
+        const int jt = c.type[j];
+        const double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            const int it = m.typ[a];
+            double xij = yd_wrap(m.xn[a] - xj), yij = yd_wrap(m.yn[a] - yj), zij = yd_wrap(m.zn[a] - zj);
+            double r2 = xij * xij + yij * yij + zij * zij;
+            const double d12 = (m.dh[a] + dhj) / 2.0;
+            if (it > 0 && jt > 0 && r2 < d12 * d12) ov = 1;
+            if (r2 < c.p.rmax2) {
+                const bool lj = it != 0 && jt != 0;
+                const int off = lj ? (it - 1) * T + (jt - 1) : 0;
+                YdPair pr;
+                yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+                yk_rs_acc(acc, a, pr, xij, yij, zij);
+            }
+            xij = yd_wrap(m.xo[a] - xj); yij = yd_wrap(m.yo[a] - yj); zij = yd_wrap(m.zo[a] - zj);
+            r2 = xij * xij + yij * yij + zij * zij;
+            if (r2 < c.p.rmax2) {
+                const bool lj = it != 0 && jt != 0;
+                const int off = lj ? (it - 1) * T + (jt - 1) : 0;
+                YdPair pr;
+                yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+                acc[YK_NRS + 0] += pr.potew;
+                acc[YK_NRS + 1] += pr.potlj6;
+                acc[YK_NRS + 2] += pr.potlj12;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void yg_rs_full_n(const YdChain &c, const YkMol &m, const double *sig6,
+                                             const double *feps, int a0, int a1, double (&acc)[YK_NRX], int &ov) {
+    switch (__builtin_amdgcn_readfirstlane(m.na)) {
+        case 1: yg_rs_full<1>(c, m, sig6, feps, a0, a1, acc, ov); break;
+        case 2: yg_rs_full<2>(c, m, sig6, feps, a0, a1, acc, ov); break;
+        case 3: yg_rs_full<3>(c, m, sig6, feps, a0, a1, acc, ov); break;
+        default: yg_rs_full<4>(c, m, sig6, feps, a0, a1, acc, ov); break;
+    }
+}
+
+// Per-workgroup chain descriptors of a grid chain from the context's one:
+// workgroup g gets its own back-term buffer, workgroup 0 alone the move
+// records; the barrier words start at zero.
+__global__ void yk_gchain_prep(const YdChain *src, YdChain *dst, int G, double *rsb, unsigned int *bar) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < G) {
+        YdChain ch = *src;
+        ch.rsb = rsb + (size_t)g * YD_NBCAP * 4;
+        if (g > 0) ch.rec = nullptr;
+        dst[g] = ch;
+    }
+    if (g < 2) bar[g] = 0u;
+}
+
+template <bool F32, int BS>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
+yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nsteps, int mode, int max_moves,
+          int lds_atoms, unsigned int *bar, double *part) {
+    __shared__ YgShared gs;
+    extern __shared__ unsigned int lds_pos[];
+    YkShared &sh = gs.s;
+    const int G = (int)gridDim.x, g = (int)blockIdx.x, tid = threadIdx.x;
+    const YdChain &c = chains[g];
+    const YdParams &p = c.p;
+    const bool lead = g == 0;
+    int total = mode == 1 ? nsteps : nsteps * p.nmol;
+    if (max_moves > 0 && total > max_moves) total = max_moves;
+    // own ranges: k in [k0, k1), atoms in [a0, a1)
+    const int k0 = (int)((long long)p.nk * g / G), k1 = (int)((long long)p.nk * (g + 1) / G);
+    const int a0 = (int)((long long)p.natom * g / G), a1 = (int)((long long)p.natom * (g + 1) / G);
+    double *part1 = part, *part2 = part + (size_t)G * YG_PART;
+    const bool use_lds = lds_atoms >= p.natom;
+    unsigned int *const spos = use_lds ? lds_pos : reinterpret_cast<unsigned int *>(c.pos4);
+    if (use_lds) {
+        for (int i = tid; i < p.natom; i += BS) spos[i] = yk_pack_pos(c.x[i], c.y[i], c.z[i]);
+    } else {
+        for (int i = a0 + tid; i < a1; i += BS) spos[i] = yk_pack_pos(c.x[i], c.y[i], c.z[i]);
+    }
+    for (int i = a0 + tid; i < a1; i += BS) {  // the launch copies of the own atoms
+        double4 *r4 = reinterpret_cast<double4 *>(c.arec) + 2 * (size_t)i;
+        r4[0] = make_double4(c.x[i], c.y[i], c.z[i], c.q[i]);
+        r4[1] = make_double4(c.dh[i], (double)c.type[i], 0.0, 0.0);
+        reinterpret_cast<double4 *>(c.uf)[i] = make_double4(c.uu[i], c.Fx[i], c.Fy[i], c.Fz[i]);
+    }
+    if (tid == 0) { gs.gen = 0; gs.last_imol = -1; }
+    if (c.nc > 0 && lead) yk_cells_build(c, &sh.cell_ok);
+    if (c.nc > 0 && lead && tid == 0) part1[YG_PART - 1] = sh.cell_ok;  // header word: cells usable
+    YkScalar &t0 = sh.t0;
+    yk_chain_mu(c, sh, tid);
+    if (tid == 0) {
+        for (int i = 0; i < 9; ++i) sh.ph[i] = 0;
+        sh.t_last = wall_clock64();
+        t0.acc_count = 0; t0.nmv = 0; t0.nov = 0;
+        YdRng &rng = t0.rng;
+        rng.key0 = c.st->key[0]; rng.key1 = c.st->key[1];
+        rng.s0 = c.st->stream[0]; rng.s1 = c.st->stream[1];
+        rng.ctr = c.st->ctr; rng.constant = mode == 1; rng.cu = 0.0;
+        rng.list = c.st->ulist; rng.nlist = c.st->nlist;
+    }
+    bool ok = yg_sync(bar, G, gs.gen, &gs.sync_ok);
+    if (ok && c.nc > 0 && !lead && tid == 0)
+        sh.cell_ok = (int)__hip_atomic_load(part1 + YG_PART - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    for (int mv = 0; ok && mv < total; ++mv) {
+        if (tid == 0) {
+            yk_sec_pick(c, sh.m, t0.rng, mode, mv);
+            sh.ov = 0;
+            sh.nbcount = 0;
+        }
+        __syncthreads();
+        if (sh.m.imol < 0) break;  // exchange step: the launch ends here and the host runs it
+        const bool cached = gs.last_imol == sh.m.imol;
+        if (tid < sh.m.na) {
+            YkMol &m = sh.m;
+            const int a = tid, gi = m.first + a;
+            m.q[a] = c.q[gi]; m.typ[a] = c.type[gi]; m.mass[a] = c.mass[gi]; m.dh[a] = c.dh[gi];
+            if (cached) { m.xo[a] = gs.lx[a]; m.yo[a] = gs.ly[a]; m.zo[a] = gs.lz[a]; }
+            else { m.xo[a] = c.x[gi]; m.yo[a] = c.y[gi]; m.zo[a] = c.z[gi]; }
+        }
+        __syncthreads();
+        if (tid == 0) { yk_sec_slots(sh.m); gs.last_imol = -1; }
+        YK_TICK(0);
+        yk_mol_tables(c, sh.m, false, tid);
+        __syncthreads();
+        YK_TICK(1);
+        // ---- old k-space forces over the own k range --------------------
+        {
+            double acc[YK_NACC];
+#pragma unroll
+            for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
+            yl_sweep_n<F32, false>(c, sh.m, acc, k0 + tid, BS, k1);
+            yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
+            if (tid < YK_NACC) part1[(size_t)g * YG_PART + tid] = sh.out[tid];
+        }
+        if (!(ok = yg_sync(bar, G, gs.gen, &gs.sync_ok))) break;
+        if (c.nc > 0 && !lead && tid == 0)
+            sh.cell_ok = (int)__hip_atomic_load(part1 + YG_PART - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        yg_reduce(part1, G, 0, YK_NACC, sh.out);
+        if (tid < sh.m.na) {  // the per-site real-space arrays (written before B1 by their owner)
+            const double4 u = reinterpret_cast<const double4 *>(c.uf)[sh.m.first + tid];
+            sh.uu_old[tid] = u.x;
+            sh.fr_old[tid][0] = u.y; sh.fr_old[tid][1] = u.z; sh.fr_old[tid][2] = u.w;
+        }
+        __syncthreads();
+        YK_TICK(2);
+        // ---- proposal (every workgroup, the same result) ----------------
+        if (tid < 64) {
+            yk_sec_propose_pre<true>(c, sh.m, t0, sh.out, sh.fr_old, sh.uu_old);
+            t0.mcm2 = t0.mcm;
+            t0.rng2 = t0.rng;
+        }
+        __syncthreads();
+        if (tid < 64) {
+            yk_displacement(p, t0.mcm, t0.rng);
+        } else if (tid < 128) {
+            yk_displacement_draws(p, t0.mcm2, t0.rng2);
+            yk_rotation<true>(p, t0.mcm2, t0.rng2);
+        }
+        __syncthreads();
+        if (tid < 64) {
+            YkMcm &a = t0.mcm;
+            const YkMcm &b = t0.mcm2;
+            a.ca = b.ca; a.sa = b.sa; a.ct = b.ct; a.st = b.st; a.cphi = b.cphi; a.sphi = b.sphi;
+            a.v[0] = b.v[0]; a.v[1] = b.v[1]; a.v[2] = b.v[2];
+            a.vmaxold = b.vmaxold; a.angle = b.angle; a.qold = b.qold;
+            a.stall = b.stall;
+            t0.rng = t0.rng2;
+            int stall = yk_sec_propose_post(sh.m, t0);
+            if (tid == 0) sh.stall = stall;
+        }
+        __syncthreads();
+        YK_TICK(3);
+        if (sh.stall) {
+            if (tid == 0) {
+                ++t0.nmv;
+                yk_sec_reject_rec(c, sh.m, t0.mcm, mv, 2);
+            }
+            __syncthreads();
+            continue;
+        }
+        // ---- real-space pass over the own atoms, then the new sweep over
+        // the own k range; both partials go out before B2 -------------------
+        {
+            double acc[YK_NRX];
+#pragma unroll
+            for (int i = 0; i < YK_NRX; ++i) acc[i] = 0.0;
+            int ov = 0;
+            yk_mol_tables(c, sh.m, true, tid);
+            const int wb = a0 >> 5, nwords = a1 > a0 ? ((a1 + 31) >> 5) - wb : 0;
+            if (nwords <= YK_BMWORDS && ((a1 + 31) >> 5) <= YK_BMWORDS) {
+                for (int i = tid; i < nwords; i += BS) sh.bm[wb + i] = 0u;
+                __syncthreads();
+                bool scanned = false;
+                if (c.nc > 0 && sh.cell_ok)
+                    scanned = yk_rs_scan_cells_n(c, sh.m, spos, sh.bm, sh.cl_cell, sh.cl_start, &sh.ncl, a0, a1);
+                if (!scanned) yk_rs_scan_n(c, sh.m, spos, sh.bm, a0, a1);
+                __syncthreads();
+                yk_compact(sh.bm, nwords, sh.nblist, &sh.nbcount, sh.wsum, wb);
+            } else if (tid == 0) {
+                sh.nbcount = YK_NBCAP + 1;
+            }
+            __syncthreads();
+            if (sh.nbcount <= YK_NBCAP)
+                yl_rs_exact_n<true>(c, sh.m, sig6, feps, sh.nblist, sh.nbcount, acc, ov, tid, BS);
+            else
+                yg_rs_full_n(c, sh.m, sig6, feps, a0, a1, acc, ov);
+            yd_block_sum<YK_NRX>(acc, sh.red, sh.out);
+            if (tid < YK_NRX) part2[(size_t)g * YG_PART + tid] = sh.out[tid];
+            // overlap flag of the workgroup (any thread)
+            if (ov) sh.ov = 1;
+            __syncthreads();
+            if (tid == 0) part2[(size_t)g * YG_PART + YK_NRX] = (double)sh.ov;
+        }
+        YK_TICK(4);
+        {
+            double acc[YK_NACC];
+#pragma unroll
+            for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
+            yl_sweep_n<F32, true>(c, sh.m, acc, k0 + tid, BS, k1);
+            yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
+            if (tid < YK_NACC) part2[(size_t)g * YG_PART + YK_NRX + 1 + tid] = sh.out[tid];
+        }
+        YK_TICK(5);
+        if (!(ok = yg_sync(bar, G, gs.gen, &gs.sync_ok))) break;
+        yg_reduce(part2, G, 0, YK_NRX + 1 + YK_NACC, gs.out2);
+        if (tid == 0) {
+            const double *o = gs.out2;
+            sh.ov = o[YK_NRX] != 0.0;
+            for (int a = 0; a < sh.m.na; ++a) {
+                t0.uun[a] = o[a];
+                t0.frn[a][0] = o[YD_MAXA + 3 * a + 0];
+                t0.frn[a][1] = o[YD_MAXA + 3 * a + 1];
+                t0.frn[a][2] = o[YD_MAXA + 3 * a + 2];
+            }
+            for (int i = 0; i < 3; ++i) {
+                t0.cmp_new[i] = o[4 * YD_MAXA + i];
+                t0.cmp_old[i] = o[YK_NRS + i];
+            }
+        }
+        __syncthreads();
+        if (sh.ov) {
+            if (tid == 0) {
+                ++t0.nmv; ++t0.nov;
+                yk_sec_reject_rec(c, sh.m, t0.mcm, mv, 1);
+            }
+            __syncthreads();
+            continue;
+        }
+        // ---- acceptance (every workgroup, the same decision) ------------
+        if (tid < 64) {
+            int accd = yk_sec_accept<true>(c, sh.m, t0, gs.out2 + YK_NRX + 1, mv);
+            if (tid == 0) {
+                sh.accepted = accd;
+                ++t0.nmv;
+                t0.acc_count += accd;
+            }
+        }
+        __syncthreads();
+        YK_TICK(6);
+        if (!sh.accepted) continue;
+        // ---- commit of the own k range and the own atoms' back terms ----
+        yk_commit<F32>(c, sh.m, k0 + tid, BS, k1);
+        if (sh.nbcount <= YK_NBCAP) {
+            yk_rs_commit_list<true>(c, sh.nblist, sh.nbcount, tid, BS);
+        } else {
+            double acc[3] = {0, 0, 0};
+            const int me = sh.m.imol + 1;
+            for (int j = a0 + tid; j < a1; j += BS)
+                if (c.molnum[j] != me) yk_rs_apply<true>(c, sh.m, sig6, feps, j, acc);
+        }
+        __syncthreads();
+        YK_TICK(7);
+        if (tid == 0) {
+            if (lead) {
+                yk_sec_finalize(c, sh.m, t0, nullptr, true, use_lds ? nullptr : spos);
+                if (c.nc > 0 && sh.cell_ok) {
+                    yk_cells_move(c, sh.m, &sh.cell_ok);
+                    part1[YG_PART - 1] = sh.cell_ok;
+                }
+            }
+            for (int a = 0; a < sh.m.na; ++a) {
+                if (use_lds) spos[sh.m.first + a] = yk_pack_pos(sh.m.xn[a], sh.m.yn[a], sh.m.zn[a]);
+                gs.lx[a] = sh.m.xn[a]; gs.ly[a] = sh.m.yn[a]; gs.lz[a] = sh.m.zn[a];
+            }
+            gs.last_imol = sh.m.imol;
+            for (int d = 0; d < 3; ++d) t0.mu[d] = t0.mu_new[d];
+        }
+        __syncthreads();
+        YK_TICK(8);
+    }
+    // the launch copy of the per-atom arrays back into the state, own atoms
+    // (after a barrier: workgroup 0's finalize wrote some of them)
+    if (ok) ok = yg_sync(bar, G, gs.gen, &gs.sync_ok);
+    if (ok) {
+        for (int i = a0 + tid; i < a1; i += BS) {
+            const double4 u = reinterpret_cast<const double4 *>(c.uf)[i];
+            c.uu[i] = u.x; c.Fx[i] = u.y; c.Fy[i] = u.z; c.Fz[i] = u.w;
+        }
+    }
+    if (tid == 0 && lead) {
+        if (mode != 1) c.st->ctr = t0.rng.ctr;
+        c.st->n_moves += t0.nmv;
+        c.st->n_accepted += t0.acc_count;
+        c.st->n_overlap += t0.nov;
+        c.st->since_resync += t0.nmv;
+        for (int i = 0; i < 9; ++i) c.st->phase[i] += sh.ph[i];
+        if (!ok) c.st->grid_fail = 1;
+    }
+}
+
+#define YK_GCHAIN_INST(F, B)                                                                                   \
+    template __global__ void yk_gchain<F, B>(const YdChain *, const double *, const double *, int, int, int, int, \
+                                             unsigned int *, double *);
+YK_GCHAIN_INST(false, 256)
+YK_GCHAIN_INST(true, 256)
 
 // ========================================================================
 // The wave chain kernel (batched replicas): one wavefront per replica,

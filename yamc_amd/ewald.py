@@ -313,6 +313,11 @@ class EwaldMC:
         """Workgroup size of the chain kernel (256 or 512)."""
         check(self.lib.yamc_set_block_size(self.ctx, int(block)))
 
+    def set_chain_groups(self, groups: int) -> None:
+        """Workgroups per chain of the single-chain runs (1: one workgroup;
+        G > 1: one chain spread over G co-resident workgroups)."""
+        check(self.lib.yamc_set_chain_groups(self.ctx, int(groups)))
+
     def last_kernel_ms(self) -> float:
         v = C.c_double()
         check(self.lib.yamc_last_kernel_ms(self.ctx, C.byref(v)))
