@@ -31,6 +31,7 @@
 // deterministic ascending compaction into the list).
 #define YK_BMWORDS 4096  // 131072 atoms
 #define YK_MAXCL 125     // cells visited by one cell-list scan (5 per axis)
+static_assert(YK_MAXCL <= 128, "the bucket prefix runs on one wave, two cells per lane");
 // Positions each thread loads ahead in the real-space scan (independent
 // loads in flight per thread when the packed positions are read from HBM).
 #define YK_SCANU 8
@@ -1021,9 +1022,14 @@ __device__ __forceinline__ bool yk_near(const float (&xof)[NA], const float (&yo
 // cells +-1 (the whole axis when that reaches nc cells, or when the sites
 // are not within one cell of each other).  cl_* are the workgroup's LDS
 // arrays; all threads call this.
+// split_G > 1 (a grid chain): workgroup split_g tests only its share
+// [total g / G, total (g + 1) / G) of the visited cells' atoms, at most
+// YK_NBCAP of them; when the cells hold more than G YK_NBCAP atoms it
+// returns false (the same in every workgroup) and the caller scans atom
+// ranges instead.
 template <int NA>
 __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int *cl_cell,
-                                 int *cl_start, int *ncl_out, int a0 = 0, int a1 = 0x7fffffff) {
+                                 int *cl_start, int *ncl_out, int split_g = 0, int split_G = 1) {
     const int nc = c.nc, tid = threadIdx.x;
     const unsigned int f0 = (unsigned int)m.first, na = (unsigned int)m.na;
     int start[3], len[3];
@@ -1061,13 +1067,30 @@ __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigne
         cl_start[tid + 1] = c.cell_cnt[id];
     }
     __syncthreads();
-    if (tid == 0) {
-        cl_start[0] = 0;
-        for (int k = 0; k < ncl; ++k) cl_start[k + 1] += cl_start[k];
-        *ncl_out = ncl;
+    if (tid < 64) {  // inclusive prefix of the bucket sizes on wave 0, two cells per lane (ncl <= 128)
+        const int i0 = 2 * tid, i1 = 2 * tid + 1;
+        const int v0 = i0 < ncl ? cl_start[i0 + 1] : 0, v1 = i1 < ncl ? cl_start[i1 + 1] : 0;
+        int incl = v0 + v1;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int t = __shfl_up(incl, off, 64);
+            if (tid >= off) incl += t;
+        }
+        const int excl = incl - v0 - v1;
+        if (i0 < ncl) cl_start[i0 + 1] = excl + v0;
+        if (i1 < ncl) cl_start[i1 + 1] = excl + v0 + v1;
+        if (tid == 0) {
+            cl_start[0] = 0;
+            *ncl_out = ncl;
+        }
     }
     __syncthreads();
-    const int total = cl_start[ncl];
+    int total = cl_start[ncl], tb = 0;
+    if (split_G > 1) {
+        if (total > split_G * YK_NBCAP) return false;
+        tb = (int)((long long)total * split_g / split_G);
+        total = (int)((long long)total * (split_g + 1) / split_G);
+    }
     const float r2f = yk_scan_r2(c);
     float xof[NA], yof[NA], zof[NA], xnf[NA], ynf[NA], znf[NA];
 #pragma unroll
@@ -1076,7 +1099,7 @@ __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigne
         xnf[a] = (float)m.xn[a]; ynf[a] = (float)m.yn[a]; znf[a] = (float)m.zn[a];
     }
     const int bs = (int)blockDim.x;
-    for (int t0 = tid; t0 < total; t0 += YK_SCANU * bs) {
+    for (int t0 = tb + tid; t0 < total; t0 += YK_SCANU * bs) {
         int jj[YK_SCANU];  // atom indices first, then their positions, YK_SCANU loads at a time
 #pragma unroll
         for (int u = 0; u < YK_SCANU; ++u) {
@@ -1094,10 +1117,7 @@ __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigne
         }
         unsigned int pj[YK_SCANU];
 #pragma unroll
-        for (int u = 0; u < YK_SCANU; ++u) {  // atoms outside [a0, a1) belong to another workgroup's scan
-            if (jj[u] >= 0 && (unsigned int)(jj[u] - a0) >= (unsigned int)(a1 - a0)) jj[u] = -1;
-            pj[u] = jj[u] >= 0 ? pos[jj[u]] : 0u;
-        }
+        for (int u = 0; u < YK_SCANU; ++u) pj[u] = jj[u] >= 0 ? pos[jj[u]] : 0u;
 #pragma unroll
         for (int u = 0; u < YK_SCANU; ++u) {
             const int j = jj[u];
@@ -1273,12 +1293,12 @@ __device__ __forceinline__ void yk_rs_scan_n(const YdChain &c, const YkMol &m, c
 }
 __device__ __forceinline__ bool yk_rs_scan_cells_n(const YdChain &c, const YkMol &m, const unsigned int *pos,
                                                    unsigned int *bm, int *cl_cell, int *cl_start, int *ncl,
-                                                   int a0 = 0, int a1 = 0x7fffffff) {
+                                                   int split_g = 0, int split_G = 1) {
     switch (__builtin_amdgcn_readfirstlane(m.na)) {
-        case 1: return yk_rs_scan_cells<1>(c, m, pos, bm, cl_cell, cl_start, ncl, a0, a1);
-        case 2: return yk_rs_scan_cells<2>(c, m, pos, bm, cl_cell, cl_start, ncl, a0, a1);
-        case 3: return yk_rs_scan_cells<3>(c, m, pos, bm, cl_cell, cl_start, ncl, a0, a1);
-        default: return yk_rs_scan_cells<4>(c, m, pos, bm, cl_cell, cl_start, ncl, a0, a1);
+        case 1: return yk_rs_scan_cells<1>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
+        case 2: return yk_rs_scan_cells<2>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
+        case 3: return yk_rs_scan_cells<3>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
+        default: return yk_rs_scan_cells<4>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
     }
 }
 
@@ -2668,56 +2688,77 @@ YK_CHAIN_INST(true, 512)
 #define YG_PART 64                 // doubles per workgroup per phase in the partials buffer
 #define YG_SPIN (1u << 22)         // bound of a barrier wait, in polls (seconds)
 
+#define YG_RTMP 2048  // LDS staging of the partial sums for the reduction
+
 struct YgShared {
     YkShared s;
     double out2[YK_NRX + 1 + YK_NACC];              // the reduced real-space and new-sweep sums
+    double rtmp[YG_RTMP];                           // partial sums of a chunk of workgroups
     double lx[YD_MAXA], ly[YD_MAXA], lz[YD_MAXA];  // the last finalized molecule's positions
     int last_imol;                                  // -1: none in the previous move
     int gen, sync_ok;
 };
 
-// Grid-wide barrier of the G workgroups of a chain: bar[0] counts arrivals,
-// bar[1] is the generation.  Thread 0 releases the workgroup's writes at
-// agent scope (they reach the L2 of every XCD), arrives, waits for the
-// generation to move on and acquires.  The wait is bounded: past YG_SPIN
-// polls it returns false, and the caller stops the chain (every workgroup
-// then stops at the same barrier or the next one).
+// Grid-wide barrier of the G workgroups of a chain: bar (64 bits) counts
+// arrivals since the launch; barrier number n (gen) is complete when it
+// reaches G (n + 1).  Thread 0 releases the workgroup's writes at agent
+// scope (they reach the L2 of every XCD), arrives, polls the count and
+// acquires.  The wait is bounded: past YG_SPIN polls it returns false, and
+// the caller stops the chain (every workgroup then stops at the same barrier
+// or the next one).
 __device__ bool yg_sync(unsigned int *bar, unsigned int G, int &gen, int *ok_lds) {
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
-        const unsigned int g0 = (unsigned int)gen;
+        unsigned long long *cnt = reinterpret_cast<unsigned long long *>(bar);
+        const unsigned long long target = (unsigned long long)(gen + 1) * G;
         __threadfence();
-        const unsigned int a = atomicAdd(&bar[0], 1u);
-        if (a == G - 1) {
-            atomicExch(&bar[0], 0u);
-            __threadfence();
-            atomicAdd(&bar[1], 1u);
-        } else {
+        const unsigned long long a = atomicAdd(cnt, 1ull) + 1ull;
+        if (a < target) {
             unsigned int polls = 0;
-            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+            while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
                 if (++polls > YG_SPIN) { ok = 0; break; }
-                __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_s_sleep(1);
             }
         }
         __threadfence();
-        gen = (int)(g0 + 1);
+        gen = gen + 1;
         *ok_lds = ok;
     }
     __syncthreads();
     return *ok_lds != 0;
 }
 
-// Sums of the workgroups' partial vectors part[g * YG_PART + off + i],
-// i < n, in workgroup order into out[i] (threads i < n).
-__device__ __forceinline__ void yg_reduce(const double *part, int G, int off, int n, double *out) {
-    if ((int)threadIdx.x < n) {
-        double a = 0.0;
-        for (int g = 0; g < G; ++g)
-            a += __hip_atomic_load(part + (size_t)g * YG_PART + off + threadIdx.x, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        out[threadIdx.x] = a;
+// Sums of the workgroups' partial vectors part[g * YG_PART + i], i < n, in
+// workgroup order into out[i].  The partials of a chunk of workgroups are
+// loaded by all threads at once into LDS (one memory round trip per chunk),
+// then threads i < n add them in workgroup order.
+__device__ void yg_reduce(const double *part, int G, int n, double *tmp, double *out) {
+    const int tid = threadIdx.x, bs = blockDim.x, C = YG_RTMP / n;
+    double a = 0.0;
+    for (int g0 = 0; g0 < G; g0 += C) {
+        const int gc = G - g0 < C ? G - g0 : C;
+        for (int base = 0; base < gc * n; base += 8 * bs) {  // eight loads in flight per thread
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int idx = base + u * bs + tid, gg = idx / n, i = idx - gg * n;
+                v[u] = idx < gc * n ? __hip_atomic_load(part + (size_t)(g0 + gg) * YG_PART + i, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT)
+                                    : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int idx = base + u * bs + tid;
+                if (idx < gc * n) tmp[idx] = v[u];
+            }
+        }
+        __syncthreads();
+        if (tid < n)
+            for (int gg = 0; gg < gc; ++gg) a += tmp[gg * n + tid];
+        __syncthreads();
     }
+    if (tid < n) out[tid] = a;
     __syncthreads();
 }
 
@@ -2869,7 +2910,7 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
         if (!(ok = yg_sync(bar, G, gs.gen, &gs.sync_ok))) break;
         if (c.nc > 0 && !lead && tid == 0)
             sh.cell_ok = (int)__hip_atomic_load(part1 + YG_PART - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        yg_reduce(part1, G, 0, YK_NACC, sh.out);
+        yg_reduce(part1, G, YK_NACC, gs.rtmp, sh.out);
         if (tid < sh.m.na) {  // the per-site real-space arrays (written before B1 by their owner)
             const double4 u = reinterpret_cast<const double4 *>(c.uf)[sh.m.first + tid];
             sh.uu_old[tid] = u.x;
@@ -2920,16 +2961,20 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
             for (int i = 0; i < YK_NRX; ++i) acc[i] = 0.0;
             int ov = 0;
             yk_mol_tables(c, sh.m, true, tid);
-            const int wb = a0 >> 5, nwords = a1 > a0 ? ((a1 + 31) >> 5) - wb : 0;
-            if (nwords <= YK_BMWORDS && ((a1 + 31) >> 5) <= YK_BMWORDS) {
-                for (int i = tid; i < nwords; i += BS) sh.bm[wb + i] = 0u;
+            // Candidates of the workgroup: with usable cells, its share of
+            // the visited cells' atoms (at most YK_NBCAP, so the list cannot
+            // overflow); else the atoms of its own range.  Either way each
+            // atom is one workgroup's candidate at most.
+            const int nwords = (p.natom + 31) >> 5;
+            if (nwords <= YK_BMWORDS) {
+                for (int i = tid; i < nwords; i += BS) sh.bm[i] = 0u;
                 __syncthreads();
                 bool scanned = false;
                 if (c.nc > 0 && sh.cell_ok)
-                    scanned = yk_rs_scan_cells_n(c, sh.m, spos, sh.bm, sh.cl_cell, sh.cl_start, &sh.ncl, a0, a1);
+                    scanned = yk_rs_scan_cells_n(c, sh.m, spos, sh.bm, sh.cl_cell, sh.cl_start, &sh.ncl, g, G);
                 if (!scanned) yk_rs_scan_n(c, sh.m, spos, sh.bm, a0, a1);
                 __syncthreads();
-                yk_compact(sh.bm, nwords, sh.nblist, &sh.nbcount, sh.wsum, wb);
+                yk_compact(sh.bm, nwords, sh.nblist, &sh.nbcount, sh.wsum);
             } else if (tid == 0) {
                 sh.nbcount = YK_NBCAP + 1;
             }
@@ -2956,7 +3001,7 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
         }
         YK_TICK(5);
         if (!(ok = yg_sync(bar, G, gs.gen, &gs.sync_ok))) break;
-        yg_reduce(part2, G, 0, YK_NRX + 1 + YK_NACC, gs.out2);
+        yg_reduce(part2, G, YK_NRX + 1 + YK_NACC, gs.rtmp, gs.out2);
         if (tid == 0) {
             const double *o = gs.out2;
             sh.ov = o[YK_NRX] != 0.0;
