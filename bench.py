@@ -129,6 +129,8 @@ def main():
                     help="instead of the headline line: CYCLES run_mc cycles (33,968 moves each) of the "
                          "99,504-atom workload in FP64 and in FP32C (FP64 resync every 1000 moves) on one "
                          "RNG stream; rates, per-move dE error until the first differing decision, drift")
+    ap.add_argument("--large-groups", type=int, default=32,
+                    help="workgroups per chain of the --large-check runs (1 = one workgroup)")
     ap.add_argument("--replica-check", action="store_true",
                     help="instead of the headline line: 64 uneven FP32C replicas (128-2048 waters, 0-1 M NaCl, "
                          "280-360 K, seeded) advanced by one captured launch per sweep, launch shape autotuned")
@@ -162,7 +164,7 @@ def main():
 
     from yamc_amd import YAMC_FP32C, YAMC_FP64, config, system
     from yamc_amd.ewald import Batch, EwaldMC
-    from yamc_amd.tune import tune_batch, tune_single
+    from yamc_amd.tune import tune_batch, tune_groups, tune_single
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.forces_check:
@@ -195,6 +197,7 @@ def main():
         g.mc_init()
         g.set_rng(20261203, 0)
         shape1, _ = tune_single(g, shapes=(256, 512))  # two ordinary sweeps, one per shape
+        groups1, grates = tune_groups(g)  # one ordinary sweep per workgroup count
         for _ in range(args.warmup):
             g.run(1)
         sync()
@@ -209,7 +212,8 @@ def main():
         dt = time.perf_counter() - t0
         moves = args.steps * nmol
         res[form] = dict(moves_per_s=moves / dt, ms_per_step=1e3 * dt / args.steps, kernel_ms=kms,
-                         acceptance=nacc / moves, launch_shape=shape1)
+                         acceptance=nacc / moves, launch_shape=shape1, workgroups_per_chain=groups1,
+                         tuner_moves_per_s_by_workgroups={str(k): round(v, 1) for k, v in grates.items()})
         g.close()
 
     # ------------------------------------------------- fp32c error vs fp64
@@ -730,6 +734,7 @@ def large_check(args):
     final_pos = None
     for form, prec, rs in (("fp64", YAMC_FP64, 0), ("fp32c", YAMC_FP32C, 1000)):
         g = EwaldMC(s, precision=prec)
+        g.set_chain_groups(args.large_groups)
         g.mc_init()
         g.set_rng(20261207, 0)
         t0 = time.perf_counter()
@@ -739,7 +744,7 @@ def large_check(args):
         g.resync()
         fresh = g.energies()
         recs[form] = r
-        out[form] = dict(moves_per_s=len(r) / dt, seconds=dt, acceptance=n / len(r),
+        out[form] = dict(moves_per_s=len(r) / dt, seconds=dt, acceptance=n / len(r), workgroups_per_chain=args.large_groups,
                          kspace_drift_rel=max(abs(inc[k] - fresh[k]) / max(1.0, abs(fresh[k]))
                                               for k in ("E_kC", "E_kLJ")))
         print(f"large-check: {form} done, {len(r)} moves in {dt:.1f} s", file=sys.stderr, flush=True)

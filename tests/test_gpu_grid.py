@@ -154,3 +154,29 @@ def test_grid_large_systems_match_one_workgroup(lib, name, groups, ncyc):
     assert rel(a.sums(), b.sums(), np.abs(b.sums()).max()) < RTOL
     for G in chains:
         chains[G].close()
+
+
+def test_grid_fp32c_equals_one_workgroup(lib):
+    """FP32C: a k vector's S(k) update is the same arithmetic whichever
+    workgroup owns it, so a grid chain (with host resyncs between segments)
+    takes the decisions of the one-workgroup chain with dE within 1e-10."""
+    from yamc_amd import config
+    from yamc_amd.ewald import EwaldMC
+    s = config("c1", seed=23)
+    recs, state = {}, {}
+    for G in (1, 4):
+        g = EwaldMC(s, precision=1)
+        g.set_chain_groups(G)
+        g.mc_init()
+        g.set_rng(5, 5)
+        print(f"fp32c G={G}", flush=True)
+        _, recs[G] = g.run(2, resync_every=100, records=True)
+        state[G] = (g.energies(), g.positions(), g.sums())
+        g.close()
+    check_records(recs[4], recs[1], "fp32c")
+    (ea, pa, sa), (eb, pb, sb) = state[4], state[1]
+    for k in eb:
+        assert rel(ea[k], eb[k], max(1.0, abs(eb[k]))) < RTOL, k
+    for x, y in zip(pa, pb):
+        assert rel(x, y, 1.0) < 1e-12
+    assert rel(sa, sb, np.abs(sb).max()) < 1e-9

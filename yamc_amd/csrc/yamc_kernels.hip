@@ -31,7 +31,6 @@
 // deterministic ascending compaction into the list).
 #define YK_BMWORDS 4096  // 131072 atoms
 #define YK_MAXCL 125     // cells visited by one cell-list scan (5 per axis)
-static_assert(YK_MAXCL <= 128, "the bucket prefix runs on one wave, two cells per lane");
 // Positions each thread loads ahead in the real-space scan (independent
 // loads in flight per thread when the packed positions are read from HBM).
 #define YK_SCANU 8
@@ -768,9 +767,9 @@ __device__ __forceinline__ void yk_commit_load(const YdChain &c, const YkMol &m,
 // The loads of a thread's next k are issued before the current k's stores
 // (a different kk, so they never alias), which keeps one k of loads in
 // flight while the current one is updated.
-template <bool F32>
+template <bool F32, bool RANGE = false>
 __device__ void yk_commit(const YdChain &c, const YkMol &m, int rank, int size, int kend = -1) {
-    const int nk = c.p.nk, ke = kend >= 0 ? kend : nk;
+    const int nk = c.p.nk, ke = RANGE ? kend : nk;
     int kxyz_n = 0;
     double vn[1 + YK_MAXP][8];
     if (rank < ke) yk_commit_load<F32>(c, m, rank, kxyz_n, vn);
@@ -1027,7 +1026,7 @@ __device__ __forceinline__ bool yk_near(const float (&xof)[NA], const float (&yo
 // YK_NBCAP of them; when the cells hold more than G YK_NBCAP atoms it
 // returns false (the same in every workgroup) and the caller scans atom
 // ranges instead.
-template <int NA>
+template <int NA, bool SPLIT = false>
 __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int *cl_cell,
                                  int *cl_start, int *ncl_out, int split_g = 0, int split_G = 1) {
     const int nc = c.nc, tid = threadIdx.x;
@@ -1067,26 +1066,14 @@ __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigne
         cl_start[tid + 1] = c.cell_cnt[id];
     }
     __syncthreads();
-    if (tid < 64) {  // inclusive prefix of the bucket sizes on wave 0, two cells per lane (ncl <= 128)
-        const int i0 = 2 * tid, i1 = 2 * tid + 1;
-        const int v0 = i0 < ncl ? cl_start[i0 + 1] : 0, v1 = i1 < ncl ? cl_start[i1 + 1] : 0;
-        int incl = v0 + v1;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int t = __shfl_up(incl, off, 64);
-            if (tid >= off) incl += t;
-        }
-        const int excl = incl - v0 - v1;
-        if (i0 < ncl) cl_start[i0 + 1] = excl + v0;
-        if (i1 < ncl) cl_start[i1 + 1] = excl + v0 + v1;
-        if (tid == 0) {
-            cl_start[0] = 0;
-            *ncl_out = ncl;
-        }
+    if (tid == 0) {
+        cl_start[0] = 0;
+        for (int k = 0; k < ncl; ++k) cl_start[k + 1] += cl_start[k];
+        *ncl_out = ncl;
     }
     __syncthreads();
     int total = cl_start[ncl], tb = 0;
-    if (split_G > 1) {
+    if (SPLIT && split_G > 1) {
         if (total > split_G * YK_NBCAP) return false;
         tb = (int)((long long)total * split_g / split_G);
         total = (int)((long long)total * (split_g + 1) / split_G);
@@ -1131,10 +1118,11 @@ __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigne
 
 // pos: packed positions (yk_pack_pos); the molecule's own atoms, indices
 // first .. first + na - 1, are skipped.
-template <int NA>
+template <int NA, bool RANGE = false>
 __device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int a0 = 0,
                            int a1 = -1) {
-    const int N = a1 >= 0 ? a1 : c.p.natom, me = m.imol + 1;
+    if (!RANGE) a0 = 0;
+    const int N = RANGE ? a1 : c.p.natom, me = m.imol + 1;
     if (pos) {
         const float r2f = yk_scan_r2(c);
         const unsigned int f0 = (unsigned int)m.first, na = (unsigned int)m.na;
@@ -1282,23 +1270,25 @@ __device__ void yk_rs_commit_list(const YdChain &c, const int *list, int count, 
     }
 }
 
+template <bool RANGE = false>
 __device__ __forceinline__ void yk_rs_scan_n(const YdChain &c, const YkMol &m, const unsigned int *pos,
                                              unsigned int *bm, int a0 = 0, int a1 = -1) {
     switch (m.na) {
-        case 1: yk_rs_scan<1>(c, m, pos, bm, a0, a1); break;
-        case 2: yk_rs_scan<2>(c, m, pos, bm, a0, a1); break;
-        case 3: yk_rs_scan<3>(c, m, pos, bm, a0, a1); break;
-        default: yk_rs_scan<4>(c, m, pos, bm, a0, a1); break;
+        case 1: yk_rs_scan<1, RANGE>(c, m, pos, bm, a0, a1); break;
+        case 2: yk_rs_scan<2, RANGE>(c, m, pos, bm, a0, a1); break;
+        case 3: yk_rs_scan<3, RANGE>(c, m, pos, bm, a0, a1); break;
+        default: yk_rs_scan<4, RANGE>(c, m, pos, bm, a0, a1); break;
     }
 }
+template <bool SPLIT = false>
 __device__ __forceinline__ bool yk_rs_scan_cells_n(const YdChain &c, const YkMol &m, const unsigned int *pos,
                                                    unsigned int *bm, int *cl_cell, int *cl_start, int *ncl,
                                                    int split_g = 0, int split_G = 1) {
     switch (__builtin_amdgcn_readfirstlane(m.na)) {
-        case 1: return yk_rs_scan_cells<1>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
-        case 2: return yk_rs_scan_cells<2>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
-        case 3: return yk_rs_scan_cells<3>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
-        default: return yk_rs_scan_cells<4>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
+        case 1: return yk_rs_scan_cells<1, SPLIT>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
+        case 2: return yk_rs_scan_cells<2, SPLIT>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
+        case 3: return yk_rs_scan_cells<3, SPLIT>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
+        default: return yk_rs_scan_cells<4, SPLIT>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
     }
 }
 
@@ -1701,12 +1691,13 @@ __device__ int yk_choose(const YdChain &c, double rnd) {
 // DELTA adds dU_C (acc[0]) and evaluates the forces against S + dS.
 // The loads of a thread's next k (index, beta, S) are issued before the
 // current k's arithmetic.
-// kend (>= 0): the sweep stops there instead of at nk (a workgroup of a
-// grid chain sweeps its own range of k).
-template <bool F32, bool DELTA, int NA>
+// RANGE: the sweep stops at kend instead of nk (a workgroup of a grid chain
+// sweeps its own range of k); a separate instantiation, so that the
+// one-workgroup kernels keep code specialised to the whole range.
+template <bool F32, bool DELTA, int NA, bool RANGE = false>
 __device__ void yl_sweep_coul(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size,
                               int kend = -1) {
-    const int nk = kend >= 0 ? kend : c.p.nk;
+    const int nk = RANGE ? kend : c.p.nk;
     const double *tabs = DELTA ? m.tab_n : m.tab_o;
     int kxyz_n = 0;
     double bt_n = 0.0;
@@ -1765,10 +1756,10 @@ __device__ void yl_sweep_coul(const YdChain &c, const YkMol &m, double (&acc)[YK
 // LJ sets of a sweep: DELTA adds dU_LJ (acc[1]); forces of the LJ sites
 // against every LJ set (+ its delta).  The LJ q-weighted sin/cos of a
 // charged site is its Coulomb one times 1/q (RhoSquared.f90:173-176).
-template <bool F32, bool DELTA, int NA>
+template <bool F32, bool DELTA, int NA, bool RANGE = false>
 __device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank, int size,
                             int kend = -1) {
-    const int nk = c.p.nk, T = c.p.ntype, ke = kend >= 0 ? kend : nk;
+    const int nk = c.p.nk, T = c.p.ntype, ke = RANGE ? kend : nk;
     if (T == 0 || m.np == 0) return;
     const double *tabs = DELTA ? m.tab_n : m.tab_o;
     // the next k's index is loaded ahead; the S rows are loaded per set (a
@@ -1867,14 +1858,14 @@ __device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_N
     }
 }
 
-template <bool F32, bool DELTA>
+template <bool F32, bool DELTA, bool RANGE = false>
 __device__ __forceinline__ void yl_sweep_n(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC], int rank,
                                            int size, int kend = -1) {
     switch (__builtin_amdgcn_readfirstlane(m.na)) {
-        case 1: yl_sweep_coul<F32, DELTA, 1>(c, m, acc, rank, size, kend); yl_sweep_lj<F32, DELTA, 1>(c, m, acc, rank, size, kend); break;
-        case 2: yl_sweep_coul<F32, DELTA, 2>(c, m, acc, rank, size, kend); yl_sweep_lj<F32, DELTA, 2>(c, m, acc, rank, size, kend); break;
-        case 3: yl_sweep_coul<F32, DELTA, 3>(c, m, acc, rank, size, kend); yl_sweep_lj<F32, DELTA, 3>(c, m, acc, rank, size, kend); break;
-        default: yl_sweep_coul<F32, DELTA, 4>(c, m, acc, rank, size, kend); yl_sweep_lj<F32, DELTA, 4>(c, m, acc, rank, size, kend); break;
+        case 1: yl_sweep_coul<F32, DELTA, 1, RANGE>(c, m, acc, rank, size, kend); yl_sweep_lj<F32, DELTA, 1, RANGE>(c, m, acc, rank, size, kend); break;
+        case 2: yl_sweep_coul<F32, DELTA, 2, RANGE>(c, m, acc, rank, size, kend); yl_sweep_lj<F32, DELTA, 2, RANGE>(c, m, acc, rank, size, kend); break;
+        case 3: yl_sweep_coul<F32, DELTA, 3, RANGE>(c, m, acc, rank, size, kend); yl_sweep_lj<F32, DELTA, 3, RANGE>(c, m, acc, rank, size, kend); break;
+        default: yl_sweep_coul<F32, DELTA, 4, RANGE>(c, m, acc, rank, size, kend); yl_sweep_lj<F32, DELTA, 4, RANGE>(c, m, acc, rank, size, kend); break;
     }
 }
 
@@ -2903,7 +2894,7 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
             double acc[YK_NACC];
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-            yl_sweep_n<F32, false>(c, sh.m, acc, k0 + tid, BS, k1);
+            yl_sweep_n<F32, false, true>(c, sh.m, acc, k0 + tid, BS, k1);
             yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
             if (tid < YK_NACC) part1[(size_t)g * YG_PART + tid] = sh.out[tid];
         }
@@ -2971,8 +2962,8 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
                 __syncthreads();
                 bool scanned = false;
                 if (c.nc > 0 && sh.cell_ok)
-                    scanned = yk_rs_scan_cells_n(c, sh.m, spos, sh.bm, sh.cl_cell, sh.cl_start, &sh.ncl, g, G);
-                if (!scanned) yk_rs_scan_n(c, sh.m, spos, sh.bm, a0, a1);
+                    scanned = yk_rs_scan_cells_n<true>(c, sh.m, spos, sh.bm, sh.cl_cell, sh.cl_start, &sh.ncl, g, G);
+                if (!scanned) yk_rs_scan_n<true>(c, sh.m, spos, sh.bm, a0, a1);
                 __syncthreads();
                 yk_compact(sh.bm, nwords, sh.nblist, &sh.nbcount, sh.wsum);
             } else if (tid == 0) {
@@ -2995,7 +2986,7 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
             double acc[YK_NACC];
 #pragma unroll
             for (int i = 0; i < YK_NACC; ++i) acc[i] = 0.0;
-            yl_sweep_n<F32, true>(c, sh.m, acc, k0 + tid, BS, k1);
+            yl_sweep_n<F32, true, true>(c, sh.m, acc, k0 + tid, BS, k1);
             yd_block_sum<YK_NACC>(acc, sh.red, sh.out);
             if (tid < YK_NACC) part2[(size_t)g * YG_PART + YK_NRX + 1 + tid] = sh.out[tid];
         }
@@ -3038,7 +3029,7 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
         YK_TICK(6);
         if (!sh.accepted) continue;
         // ---- commit of the own k range and the own atoms' back terms ----
-        yk_commit<F32>(c, sh.m, k0 + tid, BS, k1);
+        yk_commit<F32, true>(c, sh.m, k0 + tid, BS, k1);
         if (sh.nbcount <= YK_NBCAP) {
             yk_rs_commit_list<true>(c, sh.nblist, sh.nbcount, tid, BS);
         } else {

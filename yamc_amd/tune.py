@@ -8,6 +8,13 @@ Which one is fastest depends on the replica count and size: one chain per
 workgroup spreads every sum over the workgroup (short moves, few chains per
 CU), one wave per replica trades a longer move for more chains in flight.
 
+A single chain can also spread over G workgroups of a cooperative launch
+(yamc_set_chain_groups): each takes a range of nk / G k vectors (nk / 4G per
+wavefront) and of the atoms, with two grid barriers per move.  More
+workgroups shorten the sweeps and the real-space pass but add barrier and
+reduction time, so the best G grows with nk and the atom count;
+tune_groups() times one cycle per candidate G and keeps the fastest.
+
 tune_batch() times one sweep of the batch itself with each shape and keeps
 the fastest.  The timed sweeps are ordinary Monte Carlo sweeps of the chains
 (every shape runs the same algorithm on the same RNG streams), so tuning
@@ -57,4 +64,31 @@ def tune_single(rep: EwaldMC, shapes: Sequence[int] = None, cycles: int = 1) -> 
         rates[bs] = rep.sys.nmol * cycles / (rep.last_kernel_ms() * 1e-3)
     best = max(rates, key=rates.get)
     rep.set_block_size(best)
+    return best, rates
+
+
+def groups_for(nk: int, natom: int) -> Tuple[int, ...]:
+    """Workgroup counts worth timing for one chain: 1, and powers of two
+    that leave at least 64 k vectors and 64 atoms per workgroup."""
+    out = [1]
+    g = 2
+    while g <= 64 and nk // g >= 64 and natom // g >= 64:
+        out.append(g)
+        g *= 2
+    return tuple(out)
+
+
+def tune_groups(rep: EwaldMC, groups: Sequence[int] = None, cycles: int = 1) -> Tuple[int, Dict[int, float]]:
+    """Workgroups per chain for yamc_mc_run: times `cycles` ordinary cycles
+    of the chain at each G (HIP-event kernel time), sets and returns the
+    fastest with {G: trial moves/s}."""
+    if groups is None:
+        groups = groups_for(rep.nk, rep.sys.natom)
+    rates = {}
+    for g in groups:
+        rep.set_chain_groups(g)
+        rep.run(cycles)
+        rates[g] = rep.sys.nmol * cycles / (rep.last_kernel_ms() * 1e-3)
+    best = max(rates, key=rates.get)
+    rep.set_chain_groups(best)
     return best, rates
