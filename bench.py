@@ -373,15 +373,16 @@ def main():
                     cpu_all = dict(value=ac["moves_per_s"], unit="trial moves/s", cores=ac["cores"], kind="reference",
                                    cpu_model=cpu_model(),
                                    sample=f"{ac['cores']} copies of the reference's Fortran driver at once, one per "
-                                          f"core, each {ref['moves']} movemc trial moves of the same 6224-atom "
-                                          f"workload (independent replicas), aggregate rate")
+                                                  f"core, each {ref['cycles']} run_mc cycles ({ref['moves']} "
+                                          f"force-biased trial moves) of the same 6224-atom workload with its own "
+                                          f"seed (independent replicas), aggregate rate")
                 cpu = dict(value=ref["moves_per_s"], unit="trial moves/s", cores=1, kind="reference",
                            cpu_model=cpu_model(),
-                           sample=f"{ref['moves']} movemc trial moves of the same 6224-atom workload by the "
-                                  f"reference's own Fortran (compiled from its sources with amdflang -O2, "
-                                  f"RND_CONSTANT uniforms, unbiased proposals, set-up time subtracted), "
-                                  f"1 thread; the plain-C "
-                                  f"port of the same path: {port['value']:.0f} trial moves/s")
+                           sample=f"{ref['cycles']} run_mc cycles ({ref['moves']} force-biased trial moves) "
+                                  f"of the same 6224-atom workload by the reference's own Fortran run_mc "
+                                  f"(compiled from its sources with amdflang -O2, its own random_number "
+                                  f"stream, set-up time subtracted), 1 thread; the plain-C port of the "
+                                  f"same path: {port['value']:.0f} trial moves/s")
             except Exception as e:  # the port stays the baseline
                 print(f"reference CPU timing failed ({e}); reporting the port", file=sys.stderr)
 
@@ -656,41 +657,34 @@ def npt_check(args):
 
 
 def time_reference(sysm, pyoracle, refcase, seconds=12.0):
-    """The reference's own movemc (oracle/_ref/ref_driver) on sysm's
-    configuration: driver runs with 0 and with M constant-uniform moves,
-    per-move time = difference / M.  The moves are unbiased (lambda_f =
-    lambda_c = 0): with one constant uniform per move the force-biased
-    rejection loops of strongly pushed molecules never end, and a move costs
-    the same either way (movemc computes the k-space forces of every move,
-    MC.f90:963-968; the bias only changes a few scalar expressions)."""
-    import dataclasses
+    """The reference's own run_mc (oracle/_ref/ref_driver, RUNMC section:
+    runmc.f90:178-334 with movemc's force-biased proposals and the
+    reference's own random_number stream) on sysm's configuration: driver
+    runs with 0 and with K cycles, per-move time = difference / (K nmol).
+    The same workload the device runs: force-biased single-molecule moves of
+    the same system, one cycle = nmol trial moves."""
     import tempfile
 
-    from yamc_amd import system as ys
+    def run_cfg(ncycles, seed):
+        return dict(ncycles=ncycles, seed=seed, psm=0.0, pressure=1.0e5, mvs=1.05, nstream=4, freq=[])
 
-    sysm = ys.build_system(dataclasses.replace(sysm.params, lambda_f=0.0, lambda_c=0.0), sysm.species, sysm.counts,
-                           sysm.mol_type, sysm.mol_pos, sysm.mol_angles)
-
-    def moves(nmoves, seed):
-        return refcase.safe_moves(sysm, nmoves, seed, pyoracle, (0.3, 0.45), True) if nmoves else ([], [])
-
-    def timed(imol, u):
+    def timed(ncycles, seed=20261203):
         with tempfile.TemporaryDirectory() as td:
-            refcase.write_case(td, sysm, imol, u)
-            _, dt = refcase.run(td, timeout=300)
+            refcase.write_case(td, sysm, [], [], runmc=run_cfg(ncycles, seed))
+            _, dt = refcase.run(td, timeout=600)
         return dt
 
-    t_setup = timed(*moves(0, 0))
-    t_probe = timed(*moves(100, 1))
-    per = max((t_probe - t_setup) / 100, 1e-6)
-    m = int(min(20000, max(200, seconds / per)))
-    imol, u = moves(m, 2)
-    t_m = timed(imol, u)
-    out = {"moves": m, "moves_per_s": m / max(t_m - t_setup, 1e-9), "setup_s": t_setup}
+    t_setup = timed(0)
+    t_probe = timed(1)
+    per_cycle = max(t_probe - t_setup, 1e-6)
+    K = int(max(1, min(200, round(seconds / per_cycle))))
+    t_k = timed(K)
+    m = K * sysm.nmol
+    out = {"moves": m, "cycles": K, "moves_per_s": m / max(t_k - t_setup, 1e-9), "setup_s": t_setup}
 
     # All cores: P copies of the same driver run at once, one per core (at
-    # most 16, a GPU host's share), each an independent replica of the same
-    # sample; aggregate rate = P m / (wall time - set-up time).
+    # most 16, a GPU host's share), each an independent replica (its own
+    # seed); aggregate rate = P m / (wall time - set-up time).
     import subprocess
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     P = max(1, min(16, ncpu))
@@ -699,12 +693,12 @@ def time_reference(sysm, pyoracle, refcase, seconds=12.0):
         for i in range(P):
             d = os.path.join(td, f"r{i}")
             os.makedirs(d)
-            refcase.write_case(d, sysm, imol, u)
+            refcase.write_case(d, sysm, [], [], runmc=run_cfg(K, 20261203 + i))
             dirs.append(d)
         t0 = time.perf_counter()
         procs = [subprocess.Popen([refcase.DRIVER, "case.in"], cwd=d, stdout=subprocess.DEVNULL,
                                   stderr=subprocess.PIPE) for d in dirs]
-        rcs = [pr.wait(timeout=600) for pr in procs]
+        rcs = [pr.wait(timeout=900) for pr in procs]
         wall = time.perf_counter() - t0
     if all(rc == 0 for rc in rcs):
         out["all_cores"] = {"cores": P, "moves": P * m, "moves_per_s": P * m / max(wall - t_setup, 1e-9)}

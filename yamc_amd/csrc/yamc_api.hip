@@ -472,7 +472,7 @@ extern "C" int yamc_kspace_init(yamc_ctx *c, double scale_beta, double *e_c, dou
         if (!c->beta_ready) return fail(YAMC_ERROR_INITIALIZATION, "scale_beta before beta is initialised");
         yk_scale_beta<<<(nk + 255) / 256, 256, 0, s>>>(c->ch, scale_beta);
     } else {
-        yk_beta<<<(nk + 63) / 64, 64, 0, s>>>(c->ch, c->d_xk2, c->d_beta6, c->d_beta12);
+        yk_beta<<<nk, 64, 0, s>>>(c->ch, c->d_xk2, c->d_beta6, c->d_beta12);
         if (T > 0) {
             long long n = (long long)T * T * nk;
             yk_beta_lj<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(c->ch, c->d_lj6, c->d_lj12, c->d_beta6, c->d_beta12);
@@ -734,7 +734,7 @@ extern "C" int yamc_rspace_partial(yamc_ctx *c, int32_t imol, const double *x, c
 
 extern "C" int yamc_tails(yamc_ctx *c, double *self_c, double *lj6, double *lj12) {
     if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
-    yk_tails<<<1, 64, 0, c->stream>>>(c->ch, c->d_lj6, c->d_lj12, c->d_counts);
+    yk_tails<<<1, 256, 0, c->stream>>>(c->ch, c->d_lj6, c->d_lj12, c->d_counts);
     HIPCHK(hipGetLastError());
     YdState st;
     HIPCHK(hipMemcpyAsync(&st, c->ch.st, sizeof st, hipMemcpyDeviceToHost, c->stream));

@@ -36,8 +36,14 @@
 #define YK_SCANU 8
 
 // ------------------------------------------------------------------------
-// Functions.f90:26-111 erfc_Luc_bet6_bet12
-__device__ void yk_erfc_bet6_bet12(double x, double x2, double e2, double *erfk, double *bet6, double *bet12) {
+// Functions.f90:26-111 erfc_Luc_bet6_bet12 on one wavefront: the series and
+// asymptotic branches on every lane (the same values), the trapezoid
+// branch's up to 7,500 steps spread over the lanes (lane l takes steps
+// 2 + l, 66 + l, ...), the lanes' partial sums added by a fixed butterfly;
+// the sums differ from the serial loop's by rounding only.  Every lane
+// returns the same values.
+__device__ void yk_erfc_bet6_bet12_wave(double x, double x2, double e2, double *erfk, double *bet6, double *bet12) {
+    const int lane = threadIdx.x & 63;
     double pi = 4.0 * atan(1.0);
     double pi12 = sqrt(pi);
     double pi15 = pow(pi, 1.5);
@@ -58,18 +64,24 @@ __device__ void yk_erfc_bet6_bet12(double x, double x2, double e2, double *erfk,
         dx = (xmax - x) / (n - 1);
         double x4 = (x * x) * (x * x), x3 = x * x * x, x5 = x4 * x, x8 = x4 * x4, x9 = x8 * x, x10 = x8 * x * x,
                x11 = x10 * x;
-        double som = dx * e2 / 2.0 - dx * dx / 12.0 * 2.0 * x * e2;
-        double som6 = dx * e2 / x4 / 2.0 - dx * dx / 12.0 * (2.0 / x3 + 4.0 / x5) * e2;
-        double som12 = dx * e2 / x10 / 2.0 - dx * dx / 12.0 * (2.0 / x9 + 10.0 / x11) * e2;
-        double y = x, e = e2;
-        for (int i = 2; i <= n; ++i) {
-            y = x + (double)(i - 1) * dx;
-            e = exp(-(y * y));
-            double y4 = (y * y) * (y * y), y10 = y4 * y4 * (y * y);
-            som = som + dx * e;
-            som6 = som6 + dx * e / y4;
-            som12 = som12 + dx * e / y10;
+        double p0 = 0.0, p6 = 0.0, p12 = 0.0;
+        for (int i = 2 + lane; i <= n; i += 64) {
+            const double y = x + (double)(i - 1) * dx, e = exp(-(y * y));
+            const double y4 = (y * y) * (y * y), y10 = y4 * y4 * (y * y);
+            p0 = p0 + dx * e;
+            p6 = p6 + dx * e / y4;
+            p12 = p12 + dx * e / y10;
         }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            p0 += __shfl_xor(p0, off, 64);
+            p6 += __shfl_xor(p6, off, 64);
+            p12 += __shfl_xor(p12, off, 64);
+        }
+        double som = dx * e2 / 2.0 - dx * dx / 12.0 * 2.0 * x * e2 + p0;
+        double som6 = dx * e2 / x4 / 2.0 - dx * dx / 12.0 * (2.0 / x3 + 4.0 / x5) * e2 + p6;
+        double som12 = dx * e2 / x10 / 2.0 - dx * dx / 12.0 * (2.0 / x9 + 10.0 / x11) * e2 + p12;
+        const double y = n >= 2 ? x + (double)(n - 1) * dx : x, e = n >= 2 ? exp(-(y * y)) : e2;
         double y2 = y * y, y3 = y2 * y, y4 = y2 * y2, y5 = y4 * y, y8 = y4 * y4, y9 = y8 * y, y10 = y8 * y2,
                y11 = y10 * y;
         som = som - dx * e / 2.0 + dx * dx / 12.0 * 2.0 * y * e;
@@ -84,20 +96,23 @@ __device__ void yk_erfc_bet6_bet12(double x, double x2, double e2, double *erfk,
     }
 }
 
-// EwaldSumKSpace.f90:263-350 initBeta: one thread per k.
-__global__ void yk_beta(YdChain c, const double *xk2, double *beta6, double *beta12) {
-    int kk = blockIdx.x * blockDim.x + threadIdx.x;
+// EwaldSumKSpace.f90:263-350 initBeta: one wavefront per k (blocks of 64
+// threads, one k each).
+__global__ void __launch_bounds__(64) yk_beta(YdChain c, const double *xk2, double *beta6, double *beta12) {
+    const int kk = blockIdx.x;
     if (kk >= c.p.nk) return;
     double poa = YD_PI / c.p.alpha, poa2 = poa * poa;
     double k2 = xk2[kk], b2 = poa2 * k2, e = exp(-b2);
     double bt = c.p.xclb * e / k2 / YD_TWO_PI;
     double b = poa * sqrt(k2), erfc_b, B6, B12;
-    yk_erfc_bet6_bet12(b, b2, e, &erfc_b, &B6, &B12);
+    yk_erfc_bet6_bet12_wave(b, b2, e, &erfc_b, &B6, &B12);
     double b6 = yd_p3(c.p.alpha) * B6, b12 = yd_p9(c.p.alpha) * B12;
     if ((c.kxyz[kk] & 255) > 0) { bt = 2.0 * bt; b6 = 2.0 * b6; b12 = 2.0 * b12; }
-    c.beta[kk] = bt;
-    beta6[kk] = b6;
-    beta12[kk] = b12;
+    if (threadIdx.x == 0) {
+        c.beta[kk] = bt;
+        beta6[kk] = b6;
+        beta12[kk] = b12;
+    }
 }
 
 // EwaldSumKSpace.f90:352-386 initBetaLJ, full symmetric rows.
@@ -270,10 +285,16 @@ __global__ void yk_mu(YdChain c, double *mu_out) {
 }
 
 // EwaldSumTails.f90:18-127 (constant terms), single thread.
-__global__ void yk_tails(YdChain c, const double *lj6, const double *lj12, const int *counts) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double S = 0.0;
-    for (int i = 0; i < c.p.natom; ++i) S = S + c.q[i] * c.q[i];
+// ewald_sum_coulomb_tail + ewald_sum_lj_tails (EwaldSumTails.f90:18-127):
+// the charge sum over one block of 256 threads (strided partial sums, block
+// reduction), the type sums on thread 0.
+__global__ void __launch_bounds__(256) yk_tails(YdChain c, const double *lj6, const double *lj12, const int *counts) {
+    __shared__ double red[4], out[1];
+    double v[1] = {0.0};
+    for (int i = threadIdx.x; i < c.p.natom; i += (int)blockDim.x) v[0] += c.q[i] * c.q[i];
+    yd_block_sum<1>(v, red, out);
+    if (threadIdx.x != 0) return;
+    const double S = out[0];
     c.st->tail_c = -c.p.xclb * c.p.alpha_over_sqrt_pi * S;
     int T = c.p.ntype;
     double a3 = yd_p3(c.p.alpha), a6 = a3 * a3, a9 = a3 * a6, a12 = a6 * a6, pspi = pow(YD_PI, 1.5);

@@ -256,7 +256,7 @@ extern "C" int yamc_op_beta(yamc_ws *w, double alpha, double xclb, double *beta,
     OTRY(wbuf(w, WB_OUT, 3 * n, &po));
     double *pb = (double *)po;
     c.beta = pb;
-    yk_beta<<<(w->nk + 63) / 64, 64, 0, w->s>>>(c, w->d_xk2, pb + w->nk, pb + 2 * w->nk);
+    yk_beta<<<w->nk, 64, 0, w->s>>>(c, w->d_xk2, pb + w->nk, pb + 2 * w->nk);
     OCHK(hipGetLastError());
     const char *h;
     OTRY(fetch(w, po, 3 * n, &h));
