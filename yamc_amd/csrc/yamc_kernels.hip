@@ -431,6 +431,13 @@ struct YkScalar {
     double U_old, frn[YD_MAXA][3], uun[YD_MAXA], cmp_new[3], cmp_old[3], duc, dulj, du_real, du_ext;
     double mu[3], mu_new[3], muo[3], com[3];
     long long acc_count, nmv, nov;
+    // the next move's pick, taken during this move's acceptance (workgroup
+    // chain kernel): its molecule and the RNG state after its draws
+    YdRng rng_next;
+    int prepicked, nimol;
+    // the state's carried energy sums (workgroup chain kernel), so that a
+    // commit does not wait for their loads
+    double e_kc, e_klj, uu_ew, uu_lj6, uu_lj12;
 };
 
 struct YkShared {
@@ -2017,22 +2024,26 @@ __device__ __forceinline__ void yk_block_sum_used(double (&v)[NV], double *red, 
 // which lets shi_sans_exp spread its trapezoid terms over the lanes.
 
 // Molecule pick (runmc.f90:206, 232, 258) and the molecule's scalar data.
-__device__ __forceinline__ void yk_sec_pick(const YdChain &c, YkMol &m, YdRng &rng, int mode, int mv) {
-    int imol;
+// The molecule of step mv (-1: an exchange step) and its draws.
+__device__ __forceinline__ int yk_pick_imol(const YdChain &c, YdRng &rng, int mode, int mv) {
     if (mode == 1) {
-        imol = c.cimol[mv];
         rng.cu = c.cu[mv];
-    } else {
-        if ((mv + c.mv_begin) % c.p.nmol == 0) (void)rng.next();  // pressure-step draw per cycle
-        if (rng.next() < c.p.pxch) {  // exchange step (runmc.f90:232-254): handed to the host
-            m.imol = -1;
-            return;
-        }
-        imol = yk_choose(c, rng.next());
+        return c.cimol[mv];
     }
+    if ((mv + c.mv_begin) % c.p.nmol == 0) (void)rng.next();  // pressure-step draw per cycle
+    if (rng.next() < c.p.pxch) return -1;  // exchange step (runmc.f90:232-254): handed to the host
+    return yk_choose(c, rng.next());
+}
+
+__device__ __forceinline__ void yk_sec_pick_as(const YdChain &c, YkMol &m, int imol) {
     m.imol = imol;
+    if (imol < 0) return;
     m.first = c.first[imol];
     m.na = c.last[imol] - m.first + 1;
+}
+
+__device__ __forceinline__ void yk_sec_pick(const YdChain &c, YkMol &m, YdRng &rng, int mode, int mv) {
+    yk_sec_pick_as(c, m, yk_pick_imol(c, rng, mode, mv));
 }
 
 // LJ types present in the molecule (after the per-site load).
@@ -2236,14 +2247,17 @@ __device__ __forceinline__ void yk_sec_reject_rec(const YdChain &c, const YkMol 
 
 // Commit of the moved molecule's own per-atom arrays, positions and the
 // energy components (MC.f90:1242-1318); single lane.
+// cached: the carried energy sums are t0's copies (kept current here)
+// instead of the state's.
 __device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m, YkScalar &t0, float4 *pos,
-                                                bool aos = false, unsigned int *posq = nullptr) {
+                                                bool aos = false, unsigned int *posq = nullptr, bool cached = false) {
     int f = m.first;
     YdState *st = c.st;
     // the state's sums are read before the per-atom stores, which the
     // compiler cannot tell apart from them
-    const double e_kc = st->e_kc, e_klj = st->e_klj, uu_ew = st->uu_ew, uu_lj6 = st->uu_lj6,
-                 uu_lj12 = st->uu_lj12;
+    const double e_kc = cached ? t0.e_kc : st->e_kc, e_klj = cached ? t0.e_klj : st->e_klj,
+                 uu_ew = cached ? t0.uu_ew : st->uu_ew, uu_lj6 = cached ? t0.uu_lj6 : st->uu_lj6,
+                 uu_lj12 = cached ? t0.uu_lj12 : st->uu_lj12;
     for (int a = 0; a < m.na; ++a) {
         c.uu[f + a] = t0.uun[a];
         c.Fx[f + a] = t0.frn[a][0];
@@ -2260,11 +2274,16 @@ __device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m
             r4[0] = make_double4(m.xn[a], m.yn[a], m.zn[a], m.q[a]);
         }
     }
-    st->e_kc = e_kc + t0.duc;
-    st->e_klj = e_klj + t0.dulj;
-    st->uu_ew = uu_ew + t0.cmp_new[0] - t0.cmp_old[0];
-    st->uu_lj6 = uu_lj6 + t0.cmp_new[1] - t0.cmp_old[1];
-    st->uu_lj12 = uu_lj12 + t0.cmp_new[2] - t0.cmp_old[2];
+    const double n_kc = e_kc + t0.duc, n_klj = e_klj + t0.dulj, n_ew = uu_ew + t0.cmp_new[0] - t0.cmp_old[0],
+                 n_lj6 = uu_lj6 + t0.cmp_new[1] - t0.cmp_old[1], n_lj12 = uu_lj12 + t0.cmp_new[2] - t0.cmp_old[2];
+    st->e_kc = n_kc;
+    st->e_klj = n_klj;
+    st->uu_ew = n_ew;
+    st->uu_lj6 = n_lj6;
+    st->uu_lj12 = n_lj12;
+    if (cached) {
+        t0.e_kc = n_kc; t0.e_klj = n_klj; t0.uu_ew = n_ew; t0.uu_lj6 = n_lj6; t0.uu_lj12 = n_lj12;
+    }
     for (int d = 0; d < 3; ++d) st->mu[d] = t0.mu_new[d];
 }
 
@@ -2453,7 +2472,9 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
     if (tid == 0) {
         for (int i = 0; i < 9; ++i) sh.ph[i] = 0;
         sh.t_last = wall_clock64();
-        t0.acc_count = 0; t0.nmv = 0; t0.nov = 0;
+        t0.acc_count = 0; t0.nmv = 0; t0.nov = 0; t0.prepicked = 0;
+        t0.e_kc = c.st->e_kc; t0.e_klj = c.st->e_klj;
+        t0.uu_ew = c.st->uu_ew; t0.uu_lj6 = c.st->uu_lj6; t0.uu_lj12 = c.st->uu_lj12;
         YdRng &rng = t0.rng;
         rng.key0 = c.st->key[0]; rng.key1 = c.st->key[1];
         rng.s0 = c.st->stream[0]; rng.s1 = c.st->stream[1];
@@ -2463,10 +2484,19 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
     for (int mv = 0; mv < total; ++mv) {
         // the periodic FP64 rebuild between move resync_every k - 1 and
         // resync_every k (workgroup-uniform)
-        if (resync_every > 0 && mv > 0 && mv % resync_every == 0) yk_chain_rebuild(c, sh, tid);
+        if (resync_every > 0 && mv > 0 && mv % resync_every == 0) {
+            yk_chain_rebuild(c, sh, tid);  // ends with a barrier
+            if (tid == 0) { t0.e_kc = c.st->e_kc; t0.e_klj = c.st->e_klj; }
+        }
         if (tid == 0) {
             __builtin_amdgcn_s_setprio(3);
-            yk_sec_pick(c, sh.m, t0.rng, mode, mv);
+            if (t0.prepicked) {  // taken during the last move's acceptance
+                yk_sec_pick_as(c, sh.m, t0.nimol);
+                t0.rng = t0.rng_next;
+                t0.prepicked = 0;
+            } else {
+                yk_sec_pick(c, sh.m, t0.rng, mode, mv);
+            }
             sh.ov = 0;
             sh.nbcount = 0;
             __builtin_amdgcn_s_setprio(0);
@@ -2612,7 +2642,10 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
             yk_block_sum_used(acc, sh.red, sh.out, [na](int i) { return i < 2 + 3 * na; });
         }
         YK_TICK(5);
-        // ---- acceptance on wave 0 (MC.f90:1176-1233) --------------------
+        // ---- acceptance on wave 0 (MC.f90:1176-1233); wave 1 meanwhile
+        // takes the next move's pick: the acceptance takes exactly one draw,
+        // so the pick's draws start one draw after t0.rng2 (the state the
+        // proposal left, which wave 0 does not write here) -------------------
         if (tid < 64) {
             __builtin_amdgcn_s_setprio(3);
             int accd = yk_sec_accept<true>(c, sh.m, t0, sh.out, mv);
@@ -2622,6 +2655,13 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
                 t0.acc_count += accd;
             }
             __builtin_amdgcn_s_setprio(0);
+        } else if (tid < 128 && mode == 0 && mv + 1 < total) {
+            // in LDS, the same values from every lane of the wave
+            t0.rng_next = t0.rng2;
+            (void)t0.rng_next.next();
+            const int nimol = yk_pick_imol(c, t0.rng_next, mode, mv + 1);
+            t0.nimol = nimol;
+            t0.prepicked = 1;
         }
         __syncthreads();
         YK_TICK(6);
@@ -2641,7 +2681,7 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         YK_TICK(7);
         if (tid == 0) {
             __builtin_amdgcn_s_setprio(3);
-            yk_sec_finalize(c, sh.m, t0, nullptr, true, spos);
+            yk_sec_finalize(c, sh.m, t0, nullptr, true, spos, true);
             if (c.nc > 0 && sh.cell_ok) yk_cells_move(c, sh.m, &sh.cell_ok);
             for (int d = 0; d < 3; ++d) t0.mu[d] = t0.mu_new[d];
             __builtin_amdgcn_s_setprio(0);
