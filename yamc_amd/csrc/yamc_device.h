@@ -340,6 +340,25 @@ __device__ __forceinline__ void yd_kr_s(const double *tab, int ks, double q, int
     o[6] = (ky > 0 && kz > 0) ? e + f : 0.0;
     o[7] = (ky > 0 && kz > 0) ? g - h : 0.0;
 }
+// yd_kr_s with q folded into the x table (tab[kx] = q cos, tab[ks + kx] =
+// q sin: the same products, since yd_kr_s's q cx cy is (q cx) cy) and the
+// slots of ky = 0 or kz = 0 left unzeroed (the caller zeroes them once).
+__device__ __forceinline__ void yd_kr_pre(const double *tab, int ks, int kx, int ky, int kz, double *o) {
+    double qcx = tab[kx], qsx = tab[ks + kx], cyk = tab[2 * ks + ky], syk = tab[3 * ks + ky];
+    double czk = tab[4 * ks + kz], szk = tab[5 * ks + kz];
+    double qcc = qcx * cyk, qss = qsx * syk, qsc = qsx * cyk, qcs = qcx * syk;
+    double qcpy = qcc - qss, qspy = qsc + qcs, qcmy = qcc + qss, qsmy = qsc - qcs;
+    double a = qcpy * czk, b = qspy * szk, c = qspy * czk, d = qcpy * szk;
+    o[0] = a - b;
+    o[1] = c + d;
+    o[2] = a + b;
+    o[3] = c - d;
+    double e = qcmy * czk, f = qsmy * szk, g = qsmy * czk, h = qcmy * szk;
+    o[4] = e - f;
+    o[5] = g + h;
+    o[6] = e + f;
+    o[7] = g - h;
+}
 __device__ __forceinline__ void yd_kr(const double *tab, double q, int kx, int ky, int kz, double *o) {
     yd_kr_s(tab, YD_KP, q, kx, ky, kz, o);
 }

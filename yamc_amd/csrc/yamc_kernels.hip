@@ -2304,8 +2304,6 @@ __device__ __forceinline__ void yk_chain_mu(const YdChain &c, YkShared &sh, int 
 
 static_assert(offsetof(YkShared, bm) % sizeof(double) == 0, "the rebuild's LDS tables live in bm");
 static_assert(sizeof(YkShared::red) >= sizeof(double) * 2 * (YK_KE_THREADS / 64), "red holds the energy sums");
-static_assert(sizeof(YkShared::red) >= 32 * sizeof(double) && sizeof(YkShared::out) >= 32 * sizeof(double),
-              "red and out hold a rebuild chunk's charges and inverse charges");
 
 // The periodic from-scratch rebuild of a chain inside its own launch
 // (calc_total_energy, EwaldSumKSpace.f90:225-260, plus mu): what a
@@ -2313,91 +2311,128 @@ static_assert(sizeof(YkShared::red) >= 32 * sizeof(double) && sizeof(YkShared::o
 // yk_kenergy) followed by a new launch computes, with the same additions in
 // the same order, so a chain rebuilt here equals the chain run in segments
 // with rebuilds between them bit for bit.
-// S(k): for each set and tile of blockDim consecutive k (one thread per k),
-// the set's atoms in list order, group by group (c.fs_ng equal groups, each
-// group's sum added to the total in group order), the axis tables of each
-// chunk of atoms built in LDS (the scan bitmap's space, free between moves).
+// S(k): for each set and pass over KT tiles of blockDim consecutive k (one
+// thread per k of each tile, the KT tiles' sums in registers), the set's
+// atoms in list order, group by group (c.fs_ng equal groups, each group's
+// sum added to the total in group order), the axis tables of each chunk of
+// atoms built in LDS once per pass (the scan bitmap's space, free between
+// moves) and read by every k of the pass.
 // E_kC, E_kLJ: yk_kenergy's per-thread k strides and block sum of
 // YK_KE_THREADS threads, each thread here standing in for
 // YK_KE_THREADS / blockDim of them.  Then mu and the cell buckets, as at
 // the start of a launch.
-__device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid) {
+template <int KT>
+__device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid, double *xtra, int xtra_bytes) {
     const int bs = (int)blockDim.x;
     const int nk = c.p.nk, T = c.p.ntype, kmax = c.p.kmax, kp = kmax + 1;
+    // per atom of a chunk: its six axis tables and its 1/q factor
+    const int stride = 6 * kp + 1;
     double *tb = reinterpret_cast<double *>(sh.bm);
-    double *qs = sh.red;   // chunk charges
-    double *iq = sh.out;   // chunk 1/q (charged atoms of an LJ set)
-    int C = (int)(sizeof(sh.bm) / (sizeof(double) * 6 * (size_t)kp));
-    if (C > 32) C = 32;
+    const int C1 = (int)(sizeof(sh.bm) / (sizeof(double) * (size_t)stride));
+    int C = C1 + (xtra ? xtra_bytes / (int)(sizeof(double) * stride) : 0);
+    if (3 * C > bs) C = bs / 3;  // one table item per thread
+    auto tabs = [&](int a) { return a < C1 ? tb + (size_t)a * stride : xtra + (size_t)(a - C1) * stride; };
     for (int set = 0; set <= T; ++set) {
         const int l0 = c.set_off[set], l1 = c.set_off[set + 1], ng = c.fs_ng;
         const int gsize = (l1 - l0 + ng - 1) / ng;
-        for (int k0 = 0; k0 < nk; k0 += bs) {
-            const int kk = k0 + tid;
-            const bool act = kk < nk;
-            int kx = 0, ky = 0, kz = 0;
-            if (act) {
-                const int kxyz = c.kxyz[kk];
-                kx = kxyz & 255; ky = (kxyz >> 8) & 255; kz = (kxyz >> 16) & 255;
+        for (int k0 = 0; k0 < nk; k0 += KT * bs) {
+            int kx[KT], ky[KT], kz[KT];
+            double tot[KT][8], sg[KT][8];
+#pragma unroll
+            for (int t = 0; t < KT; ++t) {
+                const int kk = k0 + t * bs + tid;
+                kx[t] = ky[t] = kz[t] = 0;
+                if (kk < nk) {
+                    const int kxyz = c.kxyz[kk];
+                    kx[t] = kxyz & 255; ky[t] = (kxyz >> 8) & 255; kz[t] = (kxyz >> 16) & 255;
+                }
+#pragma unroll
+                for (int m = 0; m < 8; ++m) { tot[t][m] = 0.0; sg[t][m] = 0.0; }
             }
+            // this thread's table item of a chunk: atom tid / 3, axis tid % 3;
+            // its coordinate and charge are loaded one chunk ahead
+            const int ia = tid / 3, id = tid % 3;
+            double r_n = 0.0, q_n = 0.0;
+            auto load_item = [&](int base) {
+                if (tid < 3 * C && base + ia < l1) {
+                    const int at = c.set_atoms[base + ia];
+                    r_n = id == 0 ? c.x[at] : id == 1 ? c.y[at] : c.z[at];
+                    q_n = c.q[at];
+                }
+            };
+            load_item(l0);
             int gend = l0 + gsize;  // end of the current group
-            double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0}, sg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             for (int base = l0; base < l1; base += C) {
                 const int nch = l1 - base < C ? l1 - base : C;
+                const double r = r_n, q = q_n;
                 __syncthreads();  // the previous chunk's tables are no longer read
                 if (tid < 3 * nch) {
-                    const int a = tid / 3, d = tid % 3;
-                    const int at = c.set_atoms[base + a];
-                    const double r = d == 0 ? c.x[at] : d == 1 ? c.y[at] : c.z[at];
-                    double *t = tb + (size_t)a * 6 * kp + 2 * d * kp;
+                    // yd_kr_atom's factors for this set: the Coulomb set holds
+                    // charged atoms only (q sin/cos); an LJ set takes (q
+                    // sin/cos) (1/q) of a charged atom, else sin/cos.  The x
+                    // table holds q cos, q sin: yd_kr_s's products q cx cy
+                    // are (q cx) cy, so the values are the same to the bit.
+                    double *ta = tabs(ia), *t = ta + 2 * id * kp;
                     yd_sincos_axis(r, kmax, t, t + kp);
-                    if (d == 0) {
-                        const double q = c.q[at];
-                        qs[a] = q;
-                        iq[a] = q != 0.0 ? 1.0 / q : 0.0;
+                    if (id == 0) {
+                        const double qk = set == 0 || q != 0.0 ? q : 1.0;
+                        for (int k = 0; k <= kmax; ++k) {
+                            t[k] = qk * t[k];
+                            t[kp + k] = qk * t[kp + k];
+                        }
+                        ta[6 * kp] = set > 0 && q != 0.0 ? 1.0 / q : 0.0;
                     }
                 }
+                load_item(base + C);
                 __syncthreads();
-                if (act) {
-                    for (int a = 0; a < nch; ++a) {
-                        if (base + a == gend) {  // group boundary: the group's sum into the total
+                for (int a = 0; a < nch; ++a) {
+                    if (base + a == gend) {  // group boundary: the group's sum into the total
+#pragma unroll
+                        for (int t = 0; t < KT; ++t)
 #pragma unroll
                             for (int m = 0; m < 8; ++m) {
-                                tot[m] = tot[m] + sg[m];
-                                sg[m] = 0.0;
+                                tot[t][m] = tot[t][m] + sg[t][m];
+                                sg[t][m] = 0.0;
                             }
-                            gend += gsize;
-                        }
-                        // yd_kr_atom's values for this set: the Coulomb set
-                        // holds charged atoms only (q sin/cos); an LJ set takes
-                        // (q sin/cos) (1/q) of a charged atom, else sin/cos
-                        const double q = qs[a];
+                        gend += gsize;
+                    }
+                    const double *ta = tabs(a);
+                    const double inv = ta[6 * kp];  // 0: no 1/q factor
+#pragma unroll
+                    for (int t = 0; t < KT; ++t) {
+                        if (k0 + t * bs + tid >= nk) continue;
                         double kr[8];
-                        yd_kr_s(tb + (size_t)a * 6 * kp, kp, set == 0 || q != 0.0 ? q : 1.0, kx, ky, kz, kr);
-                        if (set > 0 && q != 0.0) {
-                            const double inv = iq[a];
+                        yd_kr_pre(ta, kp, kx[t], ky[t], kz[t], kr);
+                        if (inv != 0.0) {
 #pragma unroll
                             for (int m = 0; m < 8; ++m) kr[m] = kr[m] * inv;
                         }
 #pragma unroll
-                        for (int m = 0; m < 8; ++m) sg[m] = sg[m] + kr[m];
+                        for (int m = 0; m < 8; ++m) sg[t][m] = sg[t][m] + kr[m];
                     }
                 }
             }
-            if (act) {
+#pragma unroll
+            for (int t = 0; t < KT; ++t) {
+                const int kk = k0 + t * bs + tid;
+                if (kk >= nk) continue;
                 if (l1 > l0) {
 #pragma unroll
-                    for (int m = 0; m < 8; ++m) tot[m] = tot[m] + sg[m];  // the last non-empty group
+                    for (int m = 0; m < 8; ++m) tot[t][m] = tot[t][m] + sg[t][m];  // the last non-empty group
                 }
+                // the slots yd_kr_s sets to zero at this k (ky = 0 or kz = 0)
+                // were summed as they came; the sum of those zeros is +0
+                if (kz[t] == 0) { tot[t][2] = 0.0; tot[t][3] = 0.0; tot[t][6] = 0.0; tot[t][7] = 0.0; }
+                if (ky[t] == 0) { tot[t][4] = 0.0; tot[t][5] = 0.0; tot[t][6] = 0.0; tot[t][7] = 0.0; }
 #pragma unroll
                 for (int m = 0; m < 8; ++m) {
                     const size_t idx = ((size_t)set * 8 + m) * nk + kk;
                     if (c.p.precision == YAMC_FP32C) {
-                        const float hi = (float)tot[m];
+                        const float hi = (float)tot[t][m];
                         c.Shi[idx] = hi;
-                        c.Slo[idx] = (float)(tot[m] - (double)hi);
+                        c.Slo[idx] = (float)(tot[t][m] - (double)hi);
                     } else {
-                        c.S[idx] = tot[m];
+                        c.S[idx] = tot[t][m];
                     }
                 }
             }
@@ -2443,7 +2478,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
 yk_chain(const YdChain *chains, const double *const *sig6s, const double *const *feps_s, int nsteps, int mode,
          int max_moves, int lds_atoms, int resync_every) {
     __shared__ YkShared sh;
-    extern __shared__ unsigned int lds_pos[];  // packed positions when the launch provides LDS for them
+    // packed positions when the launch provides LDS for them (8-byte aligned:
+    // the periodic rebuild also keeps FP64 tables there)
+    extern __shared__ __align__(16) unsigned int lds_pos[];
     const YdChain &c = chains[blockIdx.x];
     const double *sig6 = sig6s[blockIdx.x], *feps = feps_s[blockIdx.x];
     const int tid = threadIdx.x;
@@ -2485,7 +2522,14 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         // the periodic FP64 rebuild between move resync_every k - 1 and
         // resync_every k (workgroup-uniform)
         if (resync_every > 0 && mv > 0 && mv % resync_every == 0) {
-            yk_chain_rebuild(c, sh, tid);  // ends with a barrier
+            // the chunk tables also take the LDS of the packed positions,
+            // which are packed again afterwards (the positions are unchanged)
+            yk_chain_rebuild<2>(c, sh, tid, use_lds ? reinterpret_cast<double *>(lds_pos) : nullptr,
+                                use_lds ? 4 * p.natom : 0);  // ends with a barrier
+            if (use_lds) {
+                for (int i = tid; i < p.natom; i += (int)blockDim.x) spos[i] = yk_pack_pos(c.x[i], c.y[i], c.z[i]);
+                __syncthreads();
+            }
             if (tid == 0) { t0.e_kc = c.st->e_kc; t0.e_klj = c.st->e_klj; }
         }
         if (tid == 0) {
