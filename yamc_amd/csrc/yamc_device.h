@@ -106,6 +106,25 @@ struct YdChain {
     int *cell_cnt, *cell_atoms, *atom_cell, *atom_slot;
 };
 
+// Global-memory view of a device pointer.  The pointers of a chain
+// descriptor are read from memory, so the compiler only knows them as
+// generic pointers and emits flat loads and stores for them; a flat access
+// counts against the LDS wait counter as well as the vector-memory one, so
+// every wait for an LDS read would also wait for the flat loads in flight
+// (a load issued ahead of its use would be waited for at the next LDS
+// access).  Accesses through yd_g(p) are global_load / global_store.
+// (The host pass of hipcc also parses device code; there the qualifier is
+// left out, since the host target has no such address space.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define YD_GLOBAL __attribute__((address_space(1)))
+#else
+#define YD_GLOBAL
+#endif
+template <typename T>
+__device__ __forceinline__ YD_GLOBAL T *yd_g(T *p) {
+    return (YD_GLOBAL T *)p;
+}
+
 // Cell coordinate of one position component (box units).
 __device__ __forceinline__ int yd_cell1(double x, int nc) {
     int i = (int)floor((x + 0.5) * (double)nc) % nc;

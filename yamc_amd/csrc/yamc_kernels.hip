@@ -107,9 +107,9 @@ __global__ void __launch_bounds__(64) yk_beta(YdChain c, const double *xk2, doub
     double b = poa * sqrt(k2), erfc_b, B6, B12;
     yk_erfc_bet6_bet12_wave(b, b2, e, &erfc_b, &B6, &B12);
     double b6 = yd_p3(c.p.alpha) * B6, b12 = yd_p9(c.p.alpha) * B12;
-    if ((c.kxyz[kk] & 255) > 0) { bt = 2.0 * bt; b6 = 2.0 * b6; b12 = 2.0 * b12; }
+    if ((yd_g(c.kxyz)[kk] & 255) > 0) { bt = 2.0 * bt; b6 = 2.0 * b6; b12 = 2.0 * b12; }
     if (threadIdx.x == 0) {
-        c.beta[kk] = bt;
+        yd_g(c.beta)[kk] = bt;
         beta6[kk] = b6;
         beta12[kk] = b12;
     }
@@ -123,13 +123,13 @@ __global__ void yk_beta_lj(YdChain c, const double *lj6, const double *lj12, con
     if (i >= (long long)T * T * nk) return;
     int kk = (int)(i % nk), pr = (int)(i / nk), t1 = pr / T, t2 = pr % T;
     int a = t1 <= t2 ? t1 : t2, b = t1 <= t2 ? t2 : t1;
-    c.beta_lj[i] = lj12[a * T + b] * beta12[kk] - lj6[a * T + b] * beta6[kk];
+    yd_g(c.beta_lj)[i] = lj12[a * T + b] * beta12[kk] - lj6[a * T + b] * beta6[kk];
 }
 
 // EwaldSumKSpace.f90:210-211: beta(:) = beta(:) * scale_beta.
 __global__ void yk_scale_beta(YdChain c, double s) {
     int kk = blockIdx.x * blockDim.x + threadIdx.x;
-    if (kk < c.p.nk) c.beta[kk] = c.beta[kk] * s;
+    if (kk < c.p.nk) yd_g(c.beta)[kk] = yd_g(c.beta)[kk] * s;
 }
 
 // scale_box.f90:6-62: every molecule of more than one atom keeps its centre
@@ -138,19 +138,19 @@ __global__ void yk_scale_beta(YdChain c, double s) {
 __global__ void yk_scale_box(YdChain c, double sc) {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= c.p.nmol) return;
-    int f = c.first[m], l = c.last[m];
+    int f = yd_g(c.first)[m], l = yd_g(c.last)[m];
     if (l == f) return;
     double sx = 0.0, sy = 0.0, sz = 0.0, tm = 0.0;  // geometry.f90 center_of_mass
     for (int a = f; a <= l; ++a) {
-        double w = c.mass[a];
-        sx = sx + w * c.x[a]; sy = sy + w * c.y[a]; sz = sz + w * c.z[a];
+        double w = yd_g(c.mass)[a];
+        sx = sx + w * yd_g(c.x)[a]; sy = sy + w * yd_g(c.y)[a]; sz = sz + w * yd_g(c.z)[a];
         tm = tm + w;
     }
     double cx = sx / tm, cy = sy / tm, cz = sz / tm;
     for (int a = f; a <= l; ++a) {
-        c.x[a] = (c.x[a] - cx) * sc + cx;
-        c.y[a] = (c.y[a] - cy) * sc + cy;
-        c.z[a] = (c.z[a] - cz) * sc + cz;
+        yd_g(c.x)[a] = (yd_g(c.x)[a] - cx) * sc + cx;
+        yd_g(c.y)[a] = (yd_g(c.y)[a] - cy) * sc + cy;
+        yd_g(c.z)[a] = (yd_g(c.z)[a] - cz) * sc + cz;
     }
 }
 
@@ -182,15 +182,15 @@ __global__ void yk_full_sums(YdChain c, const double *tab, int ngroups, double *
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (1 + T) * nk) return;
     int set = i / nk, kk = i % nk;
-    int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
-    const int l0 = c.set_off[set], l1 = c.set_off[set + 1];
+    int kxyz = yd_g(c.kxyz)[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+    const int l0 = yd_g(c.set_off)[set], l1 = yd_g(c.set_off)[set + 1];
     const int chunk = (l1 - l0 + ngroups - 1) / ngroups;
     const int t0 = l0 + (int)blockIdx.y * chunk, t1 = t0 + chunk < l1 ? t0 + chunk : l1;
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int t = t0; t < t1; ++t) {
-        const int a = c.set_atoms[t];
+        const int a = yd_g(c.set_atoms)[t];
         double krc[8], krl[8];
-        yd_kr_atom(tab + (size_t)a * 6 * YD_KP, c.q[a], c.type[a], kx, ky, kz, krc, krl);
+        yd_kr_atom(tab + (size_t)a * 6 * YD_KP, yd_g(c.q)[a], yd_g(c.type)[a], kx, ky, kz, krc, krl);
         const double *kr = set == 0 ? krc : krl;
 #pragma unroll
         for (int m = 0; m < 8; ++m) s[m] = s[m] + kr[m];
@@ -211,30 +211,30 @@ __global__ void yk_full_sums_reduce(YdChain c, const double *part, int ngroups) 
     for (int g = 0; g < ngroups; ++g) s = s + part[(size_t)g * n + i];
     if (c.p.precision == YAMC_FP32C) {
         float hi = (float)s;
-        c.Shi[i] = hi;
-        c.Slo[i] = (float)(s - (double)hi);
+        yd_g(c.Shi)[i] = hi;
+        yd_g(c.Slo)[i] = (float)(s - (double)hi);
     } else {
-        c.S[i] = s;
+        yd_g(c.S)[i] = s;
     }
 }
 
 __device__ __forceinline__ double yk_ldS(const YdChain &c, size_t idx) {
-    if (c.p.precision == YAMC_FP32C) return (double)c.Shi[idx] + (double)c.Slo[idx];
-    return c.S[idx];
+    if (c.p.precision == YAMC_FP32C) return (double)yd_g(c.Shi)[idx] + (double)yd_g(c.Slo)[idx];
+    return yd_g(c.S)[idx];
 }
 
 // calc_coulomb_energy + calc_LJ_energy (EwaldSumKSpace.f90:389-526): the
 // terms of one k added into v[0] (Coulomb) and v[1] (LJ).
 __device__ __forceinline__ void yk_kenergy_k(const YdChain &c, int kk, double *v) {
     const int nk = c.p.nk, T = c.p.ntype;
-    int kxyz = c.kxyz[kk], ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+    int kxyz = yd_g(c.kxyz)[kk], ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
     double C[8];
     for (int m = 0; m < 8; ++m) C[m] = yk_ldS(c, (size_t)m * nk + kk);
     double r2 = C[0] * C[0] + C[1] * C[1];
     if (ky > 0) r2 = r2 + C[4] * C[4] + C[5] * C[5];
     if (kz > 0) r2 = r2 + C[2] * C[2] + C[3] * C[3];
     if (ky > 0 && kz > 0) r2 = r2 + C[6] * C[6] + C[7] * C[7];
-    v[0] += c.beta[kk] * r2;
+    v[0] += yd_g(c.beta)[kk] * r2;
     for (int t1 = 0; t1 < T; ++t1) {
         double A[8];
         for (int m = 0; m < 8; ++m) A[m] = yk_ldS(c, ((size_t)(1 + t1) * 8 + m) * nk + kk);
@@ -245,7 +245,7 @@ __device__ __forceinline__ void yk_kenergy_k(const YdChain &c, int kk, double *v
             if (ky > 0) loc = loc + A[4] * B[4] + A[5] * B[5];
             if (kz > 0) loc = loc + A[2] * B[2] + A[3] * B[3];
             if (ky > 0 && kz > 0) loc = loc + A[6] * B[6] + A[7] * B[7];
-            double term = c.beta_lj[((size_t)t1 * T + t2) * nk + kk] * loc;
+            double term = yd_g(c.beta_lj)[((size_t)t1 * T + t2) * nk + kk] * loc;
             v[1] += (t1 != t2) ? term + term : term;
         }
     }
@@ -261,8 +261,8 @@ __global__ void __launch_bounds__(YK_KE_THREADS) yk_kenergy(YdChain c) {
     for (int kk = threadIdx.x; kk < c.p.nk; kk += YK_KE_THREADS) yk_kenergy_k(c, kk, v);
     yd_block_sum<2>(v, red, out);
     if (threadIdx.x == 0) {
-        c.st->e_kc = out[0];
-        c.st->e_klj = out[1];
+        yd_g(c.st)->e_kc = out[0];
+        yd_g(c.st)->e_klj = out[1];
     }
 }
 
@@ -271,14 +271,14 @@ __global__ void yk_mu(YdChain c, double *mu_out) {
     __shared__ double red[16 * 3], out[3];
     double v[3] = {0, 0, 0};
     for (int i = threadIdx.x; i < c.p.natom; i += blockDim.x) {
-        double q = c.q[i];
-        v[0] += q * c.x[i];
-        v[1] += q * c.y[i];
-        v[2] += q * c.z[i];
+        double q = yd_g(c.q)[i];
+        v[0] += q * yd_g(c.x)[i];
+        v[1] += q * yd_g(c.y)[i];
+        v[2] += q * yd_g(c.z)[i];
     }
     yd_block_sum<3>(v, red, out);
     if (threadIdx.x == 0) {
-        for (int d = 0; d < 3; ++d) c.st->mu[d] = out[d];
+        for (int d = 0; d < 3; ++d) yd_g(c.st)->mu[d] = out[d];
         if (mu_out)
             for (int d = 0; d < 3; ++d) mu_out[d] = out[d];
     }
@@ -291,11 +291,11 @@ __global__ void yk_mu(YdChain c, double *mu_out) {
 __global__ void __launch_bounds__(256) yk_tails(YdChain c, const double *lj6, const double *lj12, const int *counts) {
     __shared__ double red[4], out[1];
     double v[1] = {0.0};
-    for (int i = threadIdx.x; i < c.p.natom; i += (int)blockDim.x) v[0] += c.q[i] * c.q[i];
+    for (int i = threadIdx.x; i < c.p.natom; i += (int)blockDim.x) v[0] += yd_g(c.q)[i] * yd_g(c.q)[i];
     yd_block_sum<1>(v, red, out);
     if (threadIdx.x != 0) return;
     const double S = out[0];
-    c.st->tail_c = -c.p.xclb * c.p.alpha_over_sqrt_pi * S;
+    yd_g(c.st)->tail_c = -c.p.xclb * c.p.alpha_over_sqrt_pi * S;
     int T = c.p.ntype;
     double a3 = yd_p3(c.p.alpha), a6 = a3 * a3, a9 = a3 * a6, a12 = a6 * a6, pspi = pow(YD_PI, 1.5);
     double sij6 = 0.0, sij12 = 0.0, sj6 = 0.0, sj12 = 0.0;
@@ -311,8 +311,8 @@ __global__ void __launch_bounds__(256) yk_tails(YdChain c, const double *lj6, co
         sj6 = sj6 + lj6[t * T + t] * (double)counts[t];
         sj12 = sj12 + lj12[t * T + t] * (double)counts[t];
     }
-    c.st->tail6 = pspi * a3 / 6.0 * sij6 - a6 / 12.0 * sj6;
-    c.st->tail12 = pspi * a9 / 1080.0 * sij12 - a12 / 1440.0 * sj12;
+    yd_g(c.st)->tail6 = pspi * a3 / 6.0 * sij6 - a6 / 12.0 * sj6;
+    yd_g(c.st)->tail12 = pspi * a9 / 1080.0 * sij12 - a12 / 1440.0 * sj12;
 }
 
 // Full real-space sum (EwaldSumRealSpace.f90:338-498, full mode): one thread
@@ -330,21 +330,21 @@ __global__ void __launch_bounds__(256) yk_rspace_full(YdChain c, const double *l
     const int jc = (N + ngroups - 1) / ngroups, j0 = (int)blockIdx.y * jc, j1 = j0 + jc < N ? j0 + jc : N;
     double v[6] = {0, 0, 0, 0, 0, 0};
     if (i < N) {
-        int im = c.molnum[i], it = c.type[i];
-        double qi = c.q[i], dhi = c.dh[i], xi = c.x[i], yi = c.y[i], zi = c.z[i];
+        int im = yd_g(c.molnum)[i], it = yd_g(c.type)[i];
+        double qi = yd_g(c.q)[i], dhi = yd_g(c.dh)[i], xi = yd_g(c.x)[i], yi = yd_g(c.y)[i], zi = yd_g(c.z)[i];
         double uu = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
         for (int j = j0; j < j1; ++j) {
             if (j == i) continue;
-            int jm = c.molnum[j], jt = c.type[j];
-            double xij = yd_wrap(xi - c.x[j]), yij = yd_wrap(yi - c.y[j]), zij = yd_wrap(zi - c.z[j]);
+            int jm = yd_g(c.molnum)[j], jt = yd_g(c.type)[j];
+            double xij = yd_wrap(xi - yd_g(c.x)[j]), yij = yd_wrap(yi - yd_g(c.y)[j]), zij = yd_wrap(zi - yd_g(c.z)[j]);
             double r2 = xij * xij + yij * yij + zij * zij;
-            double d12 = (dhi + c.dh[j]) / 2.0;
+            double d12 = (dhi + yd_g(c.dh)[j]) / 2.0;
             if (it > 0 && jt > 0 && im != jm && r2 < d12 * d12) *ov = 1;
             if (r2 >= c.p.rmax2) continue;
             bool lj = it != 0 && jt != 0;
             int off = lj ? (it - 1) * T + (jt - 1) : 0;
             YdPair pr;
-            yd_pair(c.p, r2, im == jm, qi, c.q[j], lj, lj_sig6[off], lj_4eps[off], pr);
+            yd_pair(c.p, r2, im == jm, qi, yd_g(c.q)[j], lj, lj_sig6[off], lj_4eps[off], pr);
             if (j > i) {
                 v[0] += pr.potew;
                 v[1] += pr.potlj6;
@@ -381,10 +381,10 @@ __global__ void yk_rspace_full_atoms(YdChain c, const double *pat, int ngroups) 
     double v[4] = {0, 0, 0, 0};
     for (int g = 0; g < ngroups; ++g)
         for (int q = 0; q < 4; ++q) v[q] = v[q] + pat[((size_t)g * 4 + q) * N + i];
-    c.uu[i] = v[0];
-    c.Fx[i] = v[1];
-    c.Fy[i] = v[2];
-    c.Fz[i] = v[3];
+    yd_g(c.uu)[i] = v[0];
+    yd_g(c.Fx)[i] = v[1];
+    yd_g(c.Fy)[i] = v[2];
+    yd_g(c.Fz)[i] = v[3];
 }
 
 __global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, double *comps_out) {
@@ -394,8 +394,8 @@ __global__ void yk_rspace_full_final(YdChain c, const double *part, int nblk, do
         for (int m = 0; m < 6; ++m) v[m] += part[b * 6 + m];
     yd_block_sum<6>(v, red, out);
     if (threadIdx.x == 0) {
-        c.st->uu_ew = out[0]; c.st->uu_lj6 = out[1]; c.st->uu_lj12 = out[2];
-        c.st->uu_ew_intra = out[3]; c.st->uu_lj6_intra = out[4]; c.st->uu_lj12_intra = out[5];
+        yd_g(c.st)->uu_ew = out[0]; yd_g(c.st)->uu_lj6 = out[1]; yd_g(c.st)->uu_lj12 = out[2];
+        yd_g(c.st)->uu_ew_intra = out[3]; yd_g(c.st)->uu_lj6_intra = out[4]; yd_g(c.st)->uu_lj12_intra = out[5];
         if (comps_out)
             for (int m = 0; m < 6; ++m) comps_out[m] = out[m];
     }
@@ -460,18 +460,18 @@ struct YkShared {
 // Loads molecule imol (old positions from the state) into LDS; thread 0.
 __device__ void yk_load_mol(const YdChain &c, YkMol &m, int imol) {
     m.imol = imol;
-    m.first = c.first[imol];
-    m.na = c.last[imol] - m.first + 1;
+    m.first = yd_g(c.first)[imol];
+    m.na = yd_g(c.last)[imol] - m.first + 1;
     m.np = 0;
     for (int a = 0; a < m.na; ++a) {
         int g = m.first + a;
-        m.q[a] = c.q[g];
-        m.typ[a] = c.type[g];
-        m.mass[a] = c.mass[g];
-        m.dh[a] = c.dh[g];
-        m.xo[a] = c.x[g];
-        m.yo[a] = c.y[g];
-        m.zo[a] = c.z[g];
+        m.q[a] = yd_g(c.q)[g];
+        m.typ[a] = yd_g(c.type)[g];
+        m.mass[a] = yd_g(c.mass)[g];
+        m.dh[a] = yd_g(c.dh)[g];
+        m.xo[a] = yd_g(c.x)[g];
+        m.yo[a] = yd_g(c.y)[g];
+        m.zo[a] = yd_g(c.z)[g];
         m.lslot[a] = -1;
         if (m.typ[a] > 0) {
             int s = -1;
@@ -539,13 +539,13 @@ __device__ __forceinline__ void yk_ld8(const YdChain &c, int set, int kk, double
     if (F32 && set == 0) {
 #pragma unroll
         for (int m = 0; m < 8; ++m)
-            C[m] = (double)c.Shi[base + (size_t)m * c.p.nk] + (double)c.Slo[base + (size_t)m * c.p.nk];
+            C[m] = (double)yd_g(c.Shi)[base + (size_t)m * c.p.nk] + (double)yd_g(c.Slo)[base + (size_t)m * c.p.nk];
         return;
     }
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        if (F32) C[m] = (double)c.Shi[base + (size_t)m * c.p.nk];
-        else C[m] = c.S[base + (size_t)m * c.p.nk];
+        if (F32) C[m] = (double)yd_g(c.Shi)[base + (size_t)m * c.p.nk];
+        else C[m] = yd_g(c.S)[base + (size_t)m * c.p.nk];
     }
 }
 
@@ -563,10 +563,10 @@ __device__ __forceinline__ void yk_raw8_load(const YdChain &c, int set, int kk, 
     for (int m = 0; m < 8; ++m) {
         const size_t idx = base + (size_t)m * c.p.nk;
         if constexpr (F32) {
-            r.h[m] = c.Shi[idx];
-            if (LO) r.l[m] = c.Slo[idx];
+            r.h[m] = yd_g(c.Shi)[idx];
+            if (LO) r.l[m] = yd_g(c.Slo)[idx];
         } else {
-            r.v[m] = c.S[idx];
+            r.v[m] = yd_g(c.S)[idx];
         }
     }
 }
@@ -615,8 +615,8 @@ __device__ void yk_sweep(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC
     const double *tabs = DELTA ? m.tab_n : m.tab_o;
     const bool pf = T <= YK_TPF;
     for (int kk = rank; kk < nk; kk += size) {
-        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
-        double bt = c.beta[kk];
+        int kxyz = yd_g(c.kxyz)[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        double bt = yd_g(c.beta)[kk];
         double C[8];
         double CP[YK_TPF][8], BL[YK_TPF][YK_MAXP];
         yk_ld8<F32>(c, 0, kk, C);
@@ -627,7 +627,7 @@ __device__ void yk_sweep(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC
                     yk_ld8<F32>(c, 1 + t1, kk, CP[t1]);
 #pragma unroll
                     for (int q = 0; q < YK_MAXP; ++q)
-                        BL[t1][q] = q < m.np ? c.beta_lj[((size_t)t1 * T + (m.ptype[q] - 1)) * nk + kk] : 0.0;
+                        BL[t1][q] = q < m.np ? yd_g(c.beta_lj)[((size_t)t1 * T + (m.ptype[q] - 1)) * nk + kk] : 0.0;
                 }
             }
         }
@@ -747,7 +747,7 @@ __device__ void yk_sweep(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC
 #pragma unroll
                 for (int q = 0; q < YK_MAXP; ++q)
                     if (q == m.lslot[a]) b = bl[q];
-                if (m.lslot[a] < 0) b = c.beta_lj[((size_t)(ta - 1) * T + t1) * nk + kk];
+                if (m.lslot[a] < 0) b = yd_g(c.beta_lj)[((size_t)(ta - 1) * T + t1) * nk + kk];
                 yd_force_k(C1, krl, kx, ky, kz, b, f);
                 acc[YK_FK(a, 0)] += f[0];
                 acc[YK_FK(a, 1)] += f[1];
@@ -764,7 +764,7 @@ __device__ void yk_sweep(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC
                 yk_ld8<F32>(c, 1 + t1, kk, C1);
 #pragma unroll
                 for (int q = 0; q < YK_MAXP; ++q)
-                    bl[q] = q < m.np ? c.beta_lj[((size_t)t1 * T + (m.ptype[q] - 1)) * nk + kk] : 0.0;
+                    bl[q] = q < m.np ? yd_g(c.beta_lj)[((size_t)t1 * T + (m.ptype[q] - 1)) * nk + kk] : 0.0;
                 lj_set(t1, C1, bl);
             }
         }
@@ -776,19 +776,12 @@ __device__ void yk_sweep(const YdChain &c, const YkMol &m, double (&acc)[YK_NACC
 // touched values of a k are loaded before the first store.
 template <bool F32>
 __device__ __forceinline__ void yk_commit_load(const YdChain &c, const YkMol &m, int kk, int &kxyz,
-                                               double (&v)[1 + YK_MAXP][8]) {
-    const int nk = c.p.nk;
-    kxyz = c.kxyz[kk];
+                                               YkRaw8<F32> (&v)[1 + YK_MAXP]) {
+    kxyz = yd_g(c.kxyz)[kk];
 #pragma unroll
     for (int set = 0; set <= YK_MAXP; ++set) {
         if (set > m.np) break;
-        int sidx = set == 0 ? 0 : m.ptype[set - 1];
-        size_t base = (size_t)sidx * 8 * nk + kk;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            size_t idx = base + (size_t)i * nk;
-            v[set][i] = F32 ? (double)c.Shi[idx] + (double)c.Slo[idx] : c.S[idx];
-        }
+        yk_raw8_load<F32, true>(c, set == 0 ? 0 : m.ptype[set - 1], kk, v[set]);
     }
 }
 
@@ -799,19 +792,19 @@ template <bool F32, bool RANGE = false>
 __device__ void yk_commit(const YdChain &c, const YkMol &m, int rank, int size, int kend = -1) {
     const int nk = c.p.nk, ke = RANGE ? kend : nk;
     int kxyz_n = 0;
-    double vn[1 + YK_MAXP][8];
+    YkRaw8<F32> vn[1 + YK_MAXP];  // the next k's values as loaded, converted where they are used
     if (rank < ke) yk_commit_load<F32>(c, m, rank, kxyz_n, vn);
     for (int kk = rank; kk < ke; kk += size) {
         const int kxyz = kxyz_n, kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
-        double v[1 + YK_MAXP][8];
+        YkRaw8<F32> vr[1 + YK_MAXP];
 #pragma unroll
-        for (int set = 0; set <= YK_MAXP; ++set)
-#pragma unroll
-            for (int i = 0; i < 8; ++i) v[set][i] = vn[set][i];
+        for (int set = 0; set <= YK_MAXP; ++set) vr[set] = vn[set];
         if (kk + size < ke) yk_commit_load<F32>(c, m, kk + size, kxyz_n, vn);
 #pragma unroll
         for (int set = 0; set <= YK_MAXP; ++set) {
             if (set > m.np) break;
+            double v[8];
+            yk_raw8_get<F32, true>(vr[set], v);
             double d[8];
             if (set == 0) yk_delta_coul(m, kx, ky, kz, d);
             else yk_delta_lj(m, set - 1, kx, ky, kz, d);
@@ -820,13 +813,13 @@ __device__ void yk_commit(const YdChain &c, const YkMol &m, int rank, int size, 
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 size_t idx = base + (size_t)i * nk;
-                double w = v[set][i] + d[i];
+                double w = v[i] + d[i];
                 if (F32) {
                     float hi = (float)w;
-                    c.Shi[idx] = hi;
-                    c.Slo[idx] = (float)(w - (double)hi);
+                    yd_g(c.Shi)[idx] = hi;
+                    yd_g(c.Slo)[idx] = (float)(w - (double)hi);
                 } else {
-                    c.S[idx] = w;
+                    yd_g(c.S)[idx] = w;
                 }
             }
         }
@@ -860,10 +853,10 @@ __device__ void yk_rs_new(const YdChain &c, const YkMol &m, const double *sig6, 
                           double (&acc)[YK_NRS], int &ov) {
     const int N = c.p.natom, T = c.p.ntype, me = m.imol + 1;
     for (int j = threadIdx.x; j < N; j += (int)blockDim.x) {
-        int jm = c.molnum[j];
+        int jm = yd_g(c.molnum)[j];
         if (jm == me) continue;
-        int jt = c.type[j];
-        double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+        int jt = yd_g(c.type)[j];
+        double qj = yd_g(c.q)[j], dhj = yd_g(c.dh)[j], xj = yd_g(c.x)[j], yj = yd_g(c.y)[j], zj = yd_g(c.z)[j];
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             int it = m.typ[a];
@@ -875,7 +868,7 @@ __device__ void yk_rs_new(const YdChain &c, const YkMol &m, const double *sig6, 
             bool lj = it != 0 && jt != 0;
             int off = lj ? (it - 1) * T + (jt - 1) : 0;
             YdPair pr;
-            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, yd_g(sig6)[off], yd_g(feps)[off], pr);
             yk_rs_acc(acc, a, pr, xij, yij, zij);
         }
     }
@@ -891,10 +884,10 @@ __device__ void yk_rs_move(const YdChain &c, const YkMol &m, const double *sig6,
                            double (&acc)[YK_NRS], int &ov, int *list, int *count, int rank, int size) {
     const int N = c.p.natom, T = c.p.ntype, me = m.imol + 1;
     for (int j = rank; j < N; j += size) {
-        int jm = c.molnum[j];
+        int jm = yd_g(c.molnum)[j];
         if (jm == me) continue;
-        int jt = c.type[j];
-        double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+        int jt = yd_g(c.type)[j];
+        double qj = yd_g(c.q)[j], dhj = yd_g(c.dh)[j], xj = yd_g(c.x)[j], yj = yd_g(c.y)[j], zj = yd_g(c.z)[j];
         bool cand = false;
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
@@ -910,7 +903,7 @@ __device__ void yk_rs_move(const YdChain &c, const YkMol &m, const double *sig6,
             bool lj = it != 0 && jt != 0;
             int off = lj ? (it - 1) * T + (jt - 1) : 0;
             YdPair pr;
-            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, yd_g(sig6)[off], yd_g(feps)[off], pr);
             yk_rs_acc(acc, a, pr, xij, yij, zij);
         }
         if (cand) {
@@ -928,8 +921,8 @@ template <bool AOS>
 __device__ __forceinline__ void yk_rs_apply(const YdChain &c, const YkMol &m, const double *sig6,
                                             const double *feps, int j, double (&acc)[3]) {
     const int T = c.p.ntype;
-    int jt = c.type[j];
-    double qj = c.q[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+    int jt = yd_g(c.type)[j];
+    double qj = yd_g(c.q)[j], xj = yd_g(c.x)[j], yj = yd_g(c.y)[j], zj = yd_g(c.z)[j];
     double nb[4] = {0, 0, 0, 0}, ob[4] = {0, 0, 0, 0};
     for (int pass = 0; pass < 2; ++pass) {
         for (int a = 0; a < m.na; ++a) {
@@ -942,7 +935,7 @@ __device__ __forceinline__ void yk_rs_apply(const YdChain &c, const YkMol &m, co
             bool lj = it != 0 && jt != 0;
             int off = lj ? (it - 1) * T + (jt - 1) : 0;
             YdPair pr;
-            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, yd_g(sig6)[off], yd_g(feps)[off], pr);
             double fx = xij * pr.FR, fy = yij * pr.FR, fz = zij * pr.FR;
             if (pass == 0) {
                 nb[0] = nb[0] + pr.pot; nb[1] = nb[1] - fx; nb[2] = nb[2] - fy; nb[3] = nb[3] - fz;
@@ -955,7 +948,7 @@ __device__ __forceinline__ void yk_rs_apply(const YdChain &c, const YkMol &m, co
         }
     }
     if (AOS) {  // the chain kernel's launch copy of the per-atom arrays
-        double4 *u = reinterpret_cast<double4 *>(c.uf) + j;
+        auto u = yd_g(reinterpret_cast<double4 *>(c.uf)) + j;
         double4 v = *u;
         v.x = v.x + nb[0] - ob[0];
         v.y = v.y + nb[1] - ob[1];
@@ -964,10 +957,10 @@ __device__ __forceinline__ void yk_rs_apply(const YdChain &c, const YkMol &m, co
         *u = v;
         return;
     }
-    c.uu[j] = c.uu[j] + nb[0] - ob[0];
-    c.Fx[j] = c.Fx[j] + nb[1] - ob[1];
-    c.Fy[j] = c.Fy[j] + nb[2] - ob[2];
-    c.Fz[j] = c.Fz[j] + nb[3] - ob[3];
+    yd_g(c.uu)[j] = yd_g(c.uu)[j] + nb[0] - ob[0];
+    yd_g(c.Fx)[j] = yd_g(c.Fx)[j] + nb[1] - ob[1];
+    yd_g(c.Fy)[j] = yd_g(c.Fy)[j] + nb[2] - ob[2];
+    yd_g(c.Fz)[j] = yd_g(c.Fz)[j] + nb[3] - ob[3];
 }
 
 // Commit of the real-space arrays over every other atom (the candidate list
@@ -977,7 +970,7 @@ __device__ void yk_rs_commit_all(const YdChain &c, const YkMol &m, const double 
                                  double (&acc)[3], int rank, int size) {
     const int me = m.imol + 1;
     for (int j = rank; j < c.p.natom; j += size)
-        if (c.molnum[j] != me) yk_rs_apply<AOS>(c, m, sig6, feps, j, acc);
+        if (yd_g(c.molnum)[j] != me) yk_rs_apply<AOS>(c, m, sig6, feps, j, acc);
 }
 
 // Real-space pass of a trial move in two stages.
@@ -1055,7 +1048,7 @@ __device__ __forceinline__ bool yk_near(const float (&xof)[NA], const float (&yo
 // returns false (the same in every workgroup) and the caller scans atom
 // ranges instead.
 template <int NA, bool SPLIT = false>
-__device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int *cl_cell,
+__device__ __forceinline__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int *cl_cell,
                                  int *cl_start, int *ncl_out, int split_g = 0, int split_G = 1) {
     const int nc = c.nc, tid = threadIdx.x;
     const unsigned int f0 = (unsigned int)m.first, na = (unsigned int)m.na;
@@ -1091,7 +1084,7 @@ __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigne
         if (cz < 0) cz += nc;
         int id = (cx * nc + cy) * nc + cz;
         cl_cell[tid] = id;
-        cl_start[tid + 1] = c.cell_cnt[id];
+        cl_start[tid + 1] = yd_g(c.cell_cnt)[id];
     }
     __syncthreads();
     if (tid == 0) {
@@ -1127,7 +1120,7 @@ __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigne
                     if (cl_start[mid] <= t) lo = mid;
                     else hi = mid - 1;
                 }
-                jj[u] = c.cell_atoms[(size_t)cl_cell[lo] * c.ccap + (t - cl_start[lo])];
+                jj[u] = yd_g(c.cell_atoms)[(size_t)cl_cell[lo] * c.ccap + (t - cl_start[lo])];
             }
         }
         unsigned int pj[YK_SCANU];
@@ -1147,7 +1140,7 @@ __device__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigne
 // pos: packed positions (yk_pack_pos); the molecule's own atoms, indices
 // first .. first + na - 1, are skipped.
 template <int NA, bool RANGE = false>
-__device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int a0 = 0,
+__device__ __forceinline__ void yk_rs_scan(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int a0 = 0,
                            int a1 = -1) {
     if (!RANGE) a0 = 0;
     const int N = RANGE ? a1 : c.p.natom, me = m.imol + 1;
@@ -1180,8 +1173,8 @@ __device__ void yk_rs_scan(const YdChain &c, const YkMol &m, const unsigned int 
         }
     } else {
         for (int j = a0 + (int)threadIdx.x; j < N; j += (int)blockDim.x) {
-            double xj = c.x[j], yj = c.y[j], zj = c.z[j];
-            int jm = c.molnum[j];
+            double xj = yd_g(c.x)[j], yj = yd_g(c.y)[j], zj = yd_g(c.z)[j];
+            int jm = yd_g(c.molnum)[j];
             bool near = false;
 #pragma unroll
             for (int a = 0; a < NA; ++a) {
@@ -1212,8 +1205,8 @@ __device__ void yk_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
     const int T = c.p.ntype;
     for (int i = rank; i < count; i += size) {
         int j = list[i];
-        int jt = c.type[j];
-        double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+        int jt = yd_g(c.type)[j];
+        double qj = yd_g(c.q)[j], dhj = yd_g(c.dh)[j], xj = yd_g(c.x)[j], yj = yd_g(c.y)[j], zj = yd_g(c.z)[j];
         double nb[4] = {0, 0, 0, 0}, ob[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
@@ -1226,7 +1219,7 @@ __device__ void yk_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
             bool lj = it != 0 && jt != 0;
             int off = lj ? (it - 1) * T + (jt - 1) : 0;
             YdPair pr;
-            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, yd_g(sig6)[off], yd_g(feps)[off], pr);
             yk_rs_acc(acc, a, pr, xij, yij, zij);
             nb[0] = nb[0] + pr.pot;
             nb[1] = nb[1] - xij * pr.FR;
@@ -1242,7 +1235,7 @@ __device__ void yk_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
             bool lj = it != 0 && jt != 0;
             int off = lj ? (it - 1) * T + (jt - 1) : 0;
             YdPair pr;
-            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, yd_g(sig6)[off], yd_g(feps)[off], pr);
             ob[0] = ob[0] + pr.pot;
             ob[1] = ob[1] - xij * pr.FR;
             ob[2] = ob[2] - yij * pr.FR;
@@ -1251,7 +1244,7 @@ __device__ void yk_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
             acc[YK_NRS + 1] += pr.potlj6;
             acc[YK_NRS + 2] += pr.potlj12;
         }
-        double *b = c.rsb + (size_t)i * 4;
+        auto b = yd_g(c.rsb) + (size_t)i * 4;
         b[0] = nb[0] - ob[0]; b[1] = nb[1] - ob[1]; b[2] = nb[2] - ob[2]; b[3] = nb[3] - ob[3];
     }
 }
@@ -1271,14 +1264,14 @@ __device__ void yk_rs_commit_list(const YdChain &c, const int *list, int count, 
             const int i = i0 + u * size;
             jj[u] = i < count ? list[i] : -1;
             if (jj[u] < 0) continue;
-            const double *bi = c.rsb + (size_t)i * 4;
+            auto bi = yd_g(c.rsb) + (size_t)i * 4;
             const int j = jj[u];
             b[u][0] = bi[0]; b[u][1] = bi[1]; b[u][2] = bi[2]; b[u][3] = bi[3];
             if (AOS) {
-                const double4 w = reinterpret_cast<const double4 *>(c.uf)[j];
+                const double4 w = yd_g(reinterpret_cast<const double4 *>(c.uf))[j];
                 v[u][0] = w.x; v[u][1] = w.y; v[u][2] = w.z; v[u][3] = w.w;
             } else {
-                v[u][0] = c.uu[j]; v[u][1] = c.Fx[j]; v[u][2] = c.Fy[j]; v[u][3] = c.Fz[j];
+                v[u][0] = yd_g(c.uu)[j]; v[u][1] = yd_g(c.Fx)[j]; v[u][2] = yd_g(c.Fy)[j]; v[u][3] = yd_g(c.Fz)[j];
             }
         }
 #pragma unroll
@@ -1286,14 +1279,14 @@ __device__ void yk_rs_commit_list(const YdChain &c, const int *list, int count, 
             const int j = jj[u];
             if (j < 0) continue;
             if (AOS) {
-                reinterpret_cast<double4 *>(c.uf)[j] =
+                yd_g(reinterpret_cast<double4 *>(c.uf))[j] =
                     make_double4(v[u][0] + b[u][0], v[u][1] + b[u][1], v[u][2] + b[u][2], v[u][3] + b[u][3]);
                 continue;
             }
-            c.uu[j] = v[u][0] + b[u][0];
-            c.Fx[j] = v[u][1] + b[u][1];
-            c.Fy[j] = v[u][2] + b[u][2];
-            c.Fz[j] = v[u][3] + b[u][3];
+            yd_g(c.uu)[j] = v[u][0] + b[u][0];
+            yd_g(c.Fx)[j] = v[u][1] + b[u][1];
+            yd_g(c.Fy)[j] = v[u][2] + b[u][2];
+            yd_g(c.Fz)[j] = v[u][3] + b[u][3];
         }
     }
 }
@@ -1324,15 +1317,15 @@ __device__ __forceinline__ bool yk_rs_scan_cells_n(const YdChain &c, const YkMol
 // workgroup); *ok = 0 if a bucket overflowed.
 __device__ void yk_cells_build(const YdChain &c, int *ok) {
     const int nc = c.nc, ncell = nc * nc * nc, N = c.p.natom;
-    for (int i = threadIdx.x; i < ncell; i += (int)blockDim.x) c.cell_cnt[i] = 0;
+    for (int i = threadIdx.x; i < ncell; i += (int)blockDim.x) yd_g(c.cell_cnt)[i] = 0;
     if (threadIdx.x == 0) *ok = 1;
     __syncthreads();
     for (int i = threadIdx.x; i < N; i += (int)blockDim.x) {
-        int id = (yd_cell1(c.x[i], nc) * nc + yd_cell1(c.y[i], nc)) * nc + yd_cell1(c.z[i], nc);
-        int slot = atomicAdd(&c.cell_cnt[id], 1);
-        c.atom_cell[i] = id;
-        c.atom_slot[i] = slot;
-        if (slot < c.ccap) c.cell_atoms[(size_t)id * c.ccap + slot] = i;
+        int id = (yd_cell1(yd_g(c.x)[i], nc) * nc + yd_cell1(yd_g(c.y)[i], nc)) * nc + yd_cell1(yd_g(c.z)[i], nc);
+        int slot = atomicAdd(&yd_g(c.cell_cnt)[id], 1);
+        yd_g(c.atom_cell)[i] = id;
+        yd_g(c.atom_slot)[i] = slot;
+        if (slot < c.ccap) yd_g(c.cell_atoms)[(size_t)id * c.ccap + slot] = i;
         else *ok = 0;
     }
     __syncthreads();
@@ -1346,19 +1339,19 @@ __device__ void yk_cells_move(const YdChain &c, const YkMol &m, int *ok) {
     for (int a = 0; a < m.na; ++a) {
         const int g = m.first + a;
         const int id = (yd_cell1(m.xn[a], nc) * nc + yd_cell1(m.yn[a], nc)) * nc + yd_cell1(m.zn[a], nc);
-        const int old = c.atom_cell[g];
+        const int old = yd_g(c.atom_cell)[g];
         if (id == old) continue;
-        const int s = c.atom_slot[g], last = c.cell_cnt[old] - 1;
-        const int moved = c.cell_atoms[(size_t)old * cap + last];
-        c.cell_atoms[(size_t)old * cap + s] = moved;
-        c.atom_slot[moved] = s;
-        c.cell_cnt[old] = last;
-        const int ns = c.cell_cnt[id];
+        const int s = yd_g(c.atom_slot)[g], last = yd_g(c.cell_cnt)[old] - 1;
+        const int moved = yd_g(c.cell_atoms)[(size_t)old * cap + last];
+        yd_g(c.cell_atoms)[(size_t)old * cap + s] = moved;
+        yd_g(c.atom_slot)[moved] = s;
+        yd_g(c.cell_cnt)[old] = last;
+        const int ns = yd_g(c.cell_cnt)[id];
         if (ns >= cap) { *ok = 0; return; }
-        c.cell_atoms[(size_t)id * cap + ns] = g;
-        c.atom_slot[g] = ns;
-        c.atom_cell[g] = id;
-        c.cell_cnt[id] = ns + 1;
+        yd_g(c.cell_atoms)[(size_t)id * cap + ns] = g;
+        yd_g(c.atom_slot)[g] = ns;
+        yd_g(c.atom_cell)[g] = id;
+        yd_g(c.cell_cnt)[id] = ns + 1;
     }
 }
 
@@ -1691,8 +1684,8 @@ __device__ int yk_choose(const YdChain &c, double rnd) {
 #pragma unroll
     for (int u = 0; u < 3; ++u) {
         const int i = i0 + u < i1 ? i0 + u : i1;
-        b[u] = c.freq_beg[i];
-        e[u] = c.freq_end[i];
+        b[u] = yd_g(c.freq_beg)[i];
+        e[u] = yd_g(c.freq_end)[i];
     }
 #pragma unroll
     for (int u = 0; u < 3; ++u)
@@ -1701,10 +1694,10 @@ __device__ int yk_choose(const YdChain &c, double rnd) {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (c.freq_beg[mid] < rnd) lo = mid;
+        if (yd_g(c.freq_beg)[mid] < rnd) lo = mid;
         else hi = mid - 1;
     }
-    if (rnd > c.freq_beg[lo] && rnd < c.freq_end[lo]) return lo;
+    if (rnd > yd_g(c.freq_beg)[lo] && rnd < yd_g(c.freq_end)[lo]) return lo;
     return 0;
 }
 
@@ -1731,8 +1724,8 @@ __device__ void yl_sweep_coul(const YdChain &c, const YkMol &m, double (&acc)[YK
     double bt_n = 0.0;
     YkRaw8<F32> r_n;
     if (rank < nk) {
-        kxyz_n = c.kxyz[rank];
-        bt_n = c.beta[rank];
+        kxyz_n = yd_g(c.kxyz)[rank];
+        bt_n = yd_g(c.beta)[rank];
         yk_raw8_load<F32, true>(c, 0, rank, r_n);
     }
     for (int kk = rank; kk < nk; kk += size) {
@@ -1740,8 +1733,8 @@ __device__ void yl_sweep_coul(const YdChain &c, const YkMol &m, double (&acc)[YK
         const double bt = bt_n;
         const YkRaw8<F32> r = r_n;
         if (kk + size < nk) {
-            kxyz_n = c.kxyz[kk + size];
-            bt_n = c.beta[kk + size];
+            kxyz_n = yd_g(c.kxyz)[kk + size];
+            bt_n = yd_g(c.beta)[kk + size];
             yk_raw8_load<F32, true>(c, 0, kk + size, r_n);
         }
         double C[8];
@@ -1793,10 +1786,10 @@ __device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_N
     // the next k's index is loaded ahead; the S rows are loaded per set (a
     // prefetch of the next set's row costs registers this loop does not have)
     int kxyz_n = 0;
-    if (rank < ke) kxyz_n = c.kxyz[rank];
+    if (rank < ke) kxyz_n = yd_g(c.kxyz)[rank];
     for (int kk = rank; kk < ke; kk += size) {
         const int kxyz = kxyz_n, kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
-        if (kk + size < ke) kxyz_n = c.kxyz[kk + size];
+        if (kk + size < ke) kxyz_n = yd_g(c.kxyz)[kk + size];
         double kl[NA][8];
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
@@ -1847,7 +1840,7 @@ __device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_N
             yk_ld8<F32>(c, 1 + t1, kk, C1);
 #pragma unroll
             for (int q = 0; q < YK_MAXP; ++q)
-                bl[q] = q < m.np ? c.beta_lj[((size_t)t1 * T + (m.ptype[q] - 1)) * nk + kk] : 0.0;
+                bl[q] = q < m.np ? yd_g(c.beta_lj)[((size_t)t1 * T + (m.ptype[q] - 1)) * nk + kk] : 0.0;
             if (DELTA) {
                 int s1 = -1;
 #pragma unroll
@@ -1875,7 +1868,7 @@ __device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_N
 #pragma unroll
                 for (int q = 0; q < YK_MAXP; ++q)
                     if (q == m.lslot[a]) b = bl[q];
-                if (m.lslot[a] < 0) b = c.beta_lj[((size_t)(ta - 1) * T + t1) * nk + kk];
+                if (m.lslot[a] < 0) b = yd_g(c.beta_lj)[((size_t)(ta - 1) * T + t1) * nk + kk];
                 double f[3] = {0, 0, 0};
                 yd_force_k(C1, kl[a], kx, ky, kz, b, f);
                 acc[YK_FK(a, 0)] += f[0];
@@ -1911,11 +1904,11 @@ __device__ void yl_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
     auto load_cand = [&](int i) {
         const int j = list[i];
         if (AOS) {  // one 64-byte record of the launch copy
-            const double4 *r4 = reinterpret_cast<const double4 *>(c.arec) + 2 * (size_t)j;
+            auto r4 = yd_g(reinterpret_cast<const double4 *>(c.arec)) + 2 * (size_t)j;
             const double4 a = r4[0], b = r4[1];
             x_n = a.x; y_n = a.y; z_n = a.z; q_n = a.w; dh_n = b.x; jt_n = (int)b.y;
         } else {
-            jt_n = c.type[j]; q_n = c.q[j]; dh_n = c.dh[j]; x_n = c.x[j]; y_n = c.y[j]; z_n = c.z[j];
+            jt_n = yd_g(c.type)[j]; q_n = yd_g(c.q)[j]; dh_n = yd_g(c.dh)[j]; x_n = yd_g(c.x)[j]; y_n = yd_g(c.y)[j]; z_n = yd_g(c.z)[j];
         }
     };
     if (rank < count) load_cand(rank);
@@ -1935,7 +1928,7 @@ __device__ void yl_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
             bool lj = it != 0 && jt != 0;
             int off = lj ? (it - 1) * T + (jt - 1) : 0;
             YdPair pr;
-            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, yd_g(sig6)[off], yd_g(feps)[off], pr);
             yk_rs_acc(acc, a, pr, xij, yij, zij);
             nb[0] = nb[0] + pr.pot;
             nb[1] = nb[1] - xij * pr.FR;
@@ -1951,7 +1944,7 @@ __device__ void yl_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
             bool lj = it != 0 && jt != 0;
             int off = lj ? (it - 1) * T + (jt - 1) : 0;
             YdPair pr;
-            yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+            yd_pair(c.p, r2, false, m.q[a], qj, lj, yd_g(sig6)[off], yd_g(feps)[off], pr);
             ob[0] = ob[0] + pr.pot;
             ob[1] = ob[1] - xij * pr.FR;
             ob[2] = ob[2] - yij * pr.FR;
@@ -1960,7 +1953,7 @@ __device__ void yl_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
             acc[YK_NRS + 1] += pr.potlj6;
             acc[YK_NRS + 2] += pr.potlj12;
         }
-        double *b = c.rsb + (size_t)i * 4;
+        auto b = yd_g(c.rsb) + (size_t)i * 4;
         b[0] = nb[0] - ob[0]; b[1] = nb[1] - ob[1]; b[2] = nb[2] - ob[2]; b[3] = nb[3] - ob[3];
     }
 }
@@ -2027,8 +2020,8 @@ __device__ __forceinline__ void yk_block_sum_used(double (&v)[NV], double *red, 
 // The molecule of step mv (-1: an exchange step) and its draws.
 __device__ __forceinline__ int yk_pick_imol(const YdChain &c, YdRng &rng, int mode, int mv) {
     if (mode == 1) {
-        rng.cu = c.cu[mv];
-        return c.cimol[mv];
+        rng.cu = yd_g(c.cu)[mv];
+        return yd_g(c.cimol)[mv];
     }
     if ((mv + c.mv_begin) % c.p.nmol == 0) (void)rng.next();  // pressure-step draw per cycle
     if (rng.next() < c.p.pxch) return -1;  // exchange step (runmc.f90:232-254): handed to the host
@@ -2038,8 +2031,8 @@ __device__ __forceinline__ int yk_pick_imol(const YdChain &c, YdRng &rng, int mo
 __device__ __forceinline__ void yk_sec_pick_as(const YdChain &c, YkMol &m, int imol) {
     m.imol = imol;
     if (imol < 0) return;
-    m.first = c.first[imol];
-    m.na = c.last[imol] - m.first + 1;
+    m.first = yd_g(c.first)[imol];
+    m.na = yd_g(c.last)[imol] - m.first + 1;
 }
 
 __device__ __forceinline__ void yk_sec_pick(const YdChain &c, YkMol &m, YdRng &rng, int mode, int mv) {
@@ -2160,7 +2153,7 @@ __device__ int yk_sec_propose(const YdChain &c, YkMol &m, YkScalar &t0, const do
 
 // Acceptance (MC.f90:1176-1233): new forces and torque, dU_ext, the
 // acceptance factor and the draw.  out holds the new k-space sweep's sums.
-// Returns 1 if accepted; the move record goes to c.rec[mv] when present.
+// Returns 1 if accepted; the move record goes to yd_g(c.rec)[mv] when present.
 template <bool WAVE>
 __device__ int yk_sec_accept(const YdChain &c, const YkMol &m, YkScalar &t0, const double *out, int mv) {
     const YdParams &p = c.p;
@@ -2226,7 +2219,7 @@ __device__ int yk_sec_accept(const YdChain &c, const YkMol &m, YkScalar &t0, con
             r.delta[d] = mcm.d[d];
         }
         r.angle = mcm.angle;
-        c.rec[mv] = r;
+        yd_g(c.rec)[mv] = r;
     }
     return accd;
 }
@@ -2242,7 +2235,7 @@ __device__ __forceinline__ void yk_sec_reject_rec(const YdChain &c, const YkMol 
         for (int d = 0; d < 3; ++d) { r.f_old[d] = mcm.fo[d]; r.t_old[d] = mcm.co[d]; r.delta[d] = mcm.d[d]; }
         r.angle = mcm.angle;
     }
-    c.rec[mv] = r;
+    yd_g(c.rec)[mv] = r;
 }
 
 // Commit of the moved molecule's own per-atom arrays, positions and the
@@ -2259,18 +2252,18 @@ __device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m
                  uu_ew = cached ? t0.uu_ew : st->uu_ew, uu_lj6 = cached ? t0.uu_lj6 : st->uu_lj6,
                  uu_lj12 = cached ? t0.uu_lj12 : st->uu_lj12;
     for (int a = 0; a < m.na; ++a) {
-        c.uu[f + a] = t0.uun[a];
-        c.Fx[f + a] = t0.frn[a][0];
-        c.Fy[f + a] = t0.frn[a][1];
-        c.Fz[f + a] = t0.frn[a][2];
-        c.x[f + a] = m.xn[a];
-        c.y[f + a] = m.yn[a];
-        c.z[f + a] = m.zn[a];
+        yd_g(c.uu)[f + a] = t0.uun[a];
+        yd_g(c.Fx)[f + a] = t0.frn[a][0];
+        yd_g(c.Fy)[f + a] = t0.frn[a][1];
+        yd_g(c.Fz)[f + a] = t0.frn[a][2];
+        yd_g(c.x)[f + a] = m.xn[a];
+        yd_g(c.y)[f + a] = m.yn[a];
+        yd_g(c.z)[f + a] = m.zn[a];
         if (pos) pos[f + a] = make_float4((float)m.xn[a], (float)m.yn[a], (float)m.zn[a], __int_as_float(m.imol + 1));
         if (posq) posq[f + a] = yk_pack_pos(m.xn[a], m.yn[a], m.zn[a]);
         if (aos) {  // the chain kernel's launch copies
-            reinterpret_cast<double4 *>(c.uf)[f + a] = make_double4(t0.uun[a], t0.frn[a][0], t0.frn[a][1], t0.frn[a][2]);
-            double4 *r4 = reinterpret_cast<double4 *>(c.arec) + 2 * (size_t)(f + a);
+            yd_g(reinterpret_cast<double4 *>(c.uf))[f + a] = make_double4(t0.uun[a], t0.frn[a][0], t0.frn[a][1], t0.frn[a][2]);
+            auto r4 = yd_g(reinterpret_cast<double4 *>(c.arec)) + 2 * (size_t)(f + a);
             r4[0] = make_double4(m.xn[a], m.yn[a], m.zn[a], m.q[a]);
         }
     }
@@ -2293,8 +2286,8 @@ __device__ __forceinline__ void yk_sec_finalize(const YdChain &c, const YkMol &m
 __device__ __forceinline__ void yk_chain_mu(const YdChain &c, YkShared &sh, int tid) {
     double v[3] = {0, 0, 0};
     for (int i = tid; i < c.p.natom; i += (int)blockDim.x) {
-        double q = c.q[i];
-        v[0] += q * c.x[i]; v[1] += q * c.y[i]; v[2] += q * c.z[i];
+        double q = yd_g(c.q)[i];
+        v[0] += q * yd_g(c.x)[i]; v[1] += q * yd_g(c.y)[i]; v[2] += q * yd_g(c.z)[i];
     }
     yd_block_sum<3>(v, sh.red, sh.out);
     if (tid == 0) {
@@ -2333,7 +2326,7 @@ __device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid, double
     if (3 * C > bs) C = bs / 3;  // one table item per thread
     auto tabs = [&](int a) { return a < C1 ? tb + (size_t)a * stride : xtra + (size_t)(a - C1) * stride; };
     for (int set = 0; set <= T; ++set) {
-        const int l0 = c.set_off[set], l1 = c.set_off[set + 1], ng = c.fs_ng;
+        const int l0 = yd_g(c.set_off)[set], l1 = yd_g(c.set_off)[set + 1], ng = c.fs_ng;
         const int gsize = (l1 - l0 + ng - 1) / ng;
         for (int k0 = 0; k0 < nk; k0 += KT * bs) {
             int kx[KT], ky[KT], kz[KT];
@@ -2343,7 +2336,7 @@ __device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid, double
                 const int kk = k0 + t * bs + tid;
                 kx[t] = ky[t] = kz[t] = 0;
                 if (kk < nk) {
-                    const int kxyz = c.kxyz[kk];
+                    const int kxyz = yd_g(c.kxyz)[kk];
                     kx[t] = kxyz & 255; ky[t] = (kxyz >> 8) & 255; kz[t] = (kxyz >> 16) & 255;
                 }
 #pragma unroll
@@ -2355,9 +2348,9 @@ __device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid, double
             double r_n = 0.0, q_n = 0.0;
             auto load_item = [&](int base) {
                 if (tid < 3 * C && base + ia < l1) {
-                    const int at = c.set_atoms[base + ia];
-                    r_n = id == 0 ? c.x[at] : id == 1 ? c.y[at] : c.z[at];
-                    q_n = c.q[at];
+                    const int at = yd_g(c.set_atoms)[base + ia];
+                    r_n = id == 0 ? yd_g(c.x)[at] : id == 1 ? yd_g(c.y)[at] : yd_g(c.z)[at];
+                    q_n = yd_g(c.q)[at];
                 }
             };
             load_item(l0);
@@ -2429,10 +2422,10 @@ __device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid, double
                     const size_t idx = ((size_t)set * 8 + m) * nk + kk;
                     if (c.p.precision == YAMC_FP32C) {
                         const float hi = (float)tot[t][m];
-                        c.Shi[idx] = hi;
-                        c.Slo[idx] = (float)(tot[t][m] - (double)hi);
+                        yd_g(c.Shi)[idx] = hi;
+                        yd_g(c.Slo)[idx] = (float)(tot[t][m] - (double)hi);
                     } else {
-                        c.S[idx] = tot[t][m];
+                        yd_g(c.S)[idx] = tot[t][m];
                     }
                 }
             }
@@ -2456,8 +2449,8 @@ __device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid, double
         if (tid < 2) {
             double a = 0.0;
             for (int vw = 0; vw < YK_KE_THREADS / 64; ++vw) a += sh.red[vw * 2 + tid];
-            if (tid == 0) c.st->e_kc = a;
-            else c.st->e_klj = a;
+            if (tid == 0) yd_g(c.st)->e_kc = a;
+            else yd_g(c.st)->e_klj = a;
         }
         __syncthreads();
     }
@@ -2492,13 +2485,13 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
     // LDS when the launch gives room for it, else in the context's pos4 buffer
     unsigned int *const spos = use_lds ? lds_pos : reinterpret_cast<unsigned int *>(c.pos4);
     for (int i = tid; i < p.natom; i += (int)blockDim.x) {
-        spos[i] = yk_pack_pos(c.x[i], c.y[i], c.z[i]);
+        spos[i] = yk_pack_pos(yd_g(c.x)[i], yd_g(c.y)[i], yd_g(c.z)[i]);
         // the launch copies: one record per atom for the exact pass, the
         // per-atom real-space arrays as (uu, F) quadruples
-        double4 *r4 = reinterpret_cast<double4 *>(c.arec) + 2 * (size_t)i;
-        r4[0] = make_double4(c.x[i], c.y[i], c.z[i], c.q[i]);
-        r4[1] = make_double4(c.dh[i], (double)c.type[i], 0.0, 0.0);
-        reinterpret_cast<double4 *>(c.uf)[i] = make_double4(c.uu[i], c.Fx[i], c.Fy[i], c.Fz[i]);
+        auto r4 = yd_g(reinterpret_cast<double4 *>(c.arec)) + 2 * (size_t)i;
+        r4[0] = make_double4(yd_g(c.x)[i], yd_g(c.y)[i], yd_g(c.z)[i], yd_g(c.q)[i]);
+        r4[1] = make_double4(yd_g(c.dh)[i], (double)yd_g(c.type)[i], 0.0, 0.0);
+        yd_g(reinterpret_cast<double4 *>(c.uf))[i] = make_double4(yd_g(c.uu)[i], yd_g(c.Fx)[i], yd_g(c.Fy)[i], yd_g(c.Fz)[i]);
     }
     if (c.nc > 0) yk_cells_build(c, &sh.cell_ok);  // workgroup-uniform; ends with a barrier
     YkScalar &t0 = sh.t0;
@@ -2510,13 +2503,13 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         for (int i = 0; i < 9; ++i) sh.ph[i] = 0;
         sh.t_last = wall_clock64();
         t0.acc_count = 0; t0.nmv = 0; t0.nov = 0; t0.prepicked = 0;
-        t0.e_kc = c.st->e_kc; t0.e_klj = c.st->e_klj;
-        t0.uu_ew = c.st->uu_ew; t0.uu_lj6 = c.st->uu_lj6; t0.uu_lj12 = c.st->uu_lj12;
+        t0.e_kc = yd_g(c.st)->e_kc; t0.e_klj = yd_g(c.st)->e_klj;
+        t0.uu_ew = yd_g(c.st)->uu_ew; t0.uu_lj6 = yd_g(c.st)->uu_lj6; t0.uu_lj12 = yd_g(c.st)->uu_lj12;
         YdRng &rng = t0.rng;
-        rng.key0 = c.st->key[0]; rng.key1 = c.st->key[1];
-        rng.s0 = c.st->stream[0]; rng.s1 = c.st->stream[1];
-        rng.ctr = c.st->ctr; rng.constant = mode == 1; rng.cu = 0.0;
-        rng.list = c.st->ulist; rng.nlist = c.st->nlist;
+        rng.key0 = yd_g(c.st)->key[0]; rng.key1 = yd_g(c.st)->key[1];
+        rng.s0 = yd_g(c.st)->stream[0]; rng.s1 = yd_g(c.st)->stream[1];
+        rng.ctr = yd_g(c.st)->ctr; rng.constant = mode == 1; rng.cu = 0.0;
+        rng.list = yd_g(c.st)->ulist; rng.nlist = yd_g(c.st)->nlist;
     }
     for (int mv = 0; mv < total; ++mv) {
         // the periodic FP64 rebuild between move resync_every k - 1 and
@@ -2527,10 +2520,10 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
             yk_chain_rebuild<2>(c, sh, tid, use_lds ? reinterpret_cast<double *>(lds_pos) : nullptr,
                                 use_lds ? 4 * p.natom : 0);  // ends with a barrier
             if (use_lds) {
-                for (int i = tid; i < p.natom; i += (int)blockDim.x) spos[i] = yk_pack_pos(c.x[i], c.y[i], c.z[i]);
+                for (int i = tid; i < p.natom; i += (int)blockDim.x) spos[i] = yk_pack_pos(yd_g(c.x)[i], yd_g(c.y)[i], yd_g(c.z)[i]);
                 __syncthreads();
             }
-            if (tid == 0) { t0.e_kc = c.st->e_kc; t0.e_klj = c.st->e_klj; }
+            if (tid == 0) { t0.e_kc = yd_g(c.st)->e_kc; t0.e_klj = yd_g(c.st)->e_klj; }
         }
         if (tid == 0) {
             __builtin_amdgcn_s_setprio(3);
@@ -2550,9 +2543,9 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         if (tid < sh.m.na) {  // one lane per site loads the molecule
             YkMol &m = sh.m;
             int a = tid, g = m.first + a;
-            m.q[a] = c.q[g]; m.typ[a] = c.type[g]; m.mass[a] = c.mass[g]; m.dh[a] = c.dh[g];
-            m.xo[a] = c.x[g]; m.yo[a] = c.y[g]; m.zo[a] = c.z[g];
-            const double4 u = reinterpret_cast<const double4 *>(c.uf)[g];
+            m.q[a] = yd_g(c.q)[g]; m.typ[a] = yd_g(c.type)[g]; m.mass[a] = yd_g(c.mass)[g]; m.dh[a] = yd_g(c.dh)[g];
+            m.xo[a] = yd_g(c.x)[g]; m.yo[a] = yd_g(c.y)[g]; m.zo[a] = yd_g(c.z)[g];
+            const double4 u = yd_g(reinterpret_cast<const double4 *>(c.uf))[g];
             sh.uu_old[a] = u.x;
             sh.fr_old[a][0] = u.y; sh.fr_old[a][1] = u.z; sh.fr_old[a][2] = u.w;
         }
@@ -2628,10 +2621,18 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
             if (nwords <= YK_BMWORDS) {
                 for (int i = tid; i < nwords; i += (int)blockDim.x) sh.bm[i] = 0u;
                 __syncthreads();
+                // two inlined copies, so that the LDS copy is read with LDS
+                // instructions (a pointer that may be either is a flat one)
                 bool scanned = false;
-                if (c.nc > 0 && sh.cell_ok)
-                    scanned = yk_rs_scan_cells_n(c, sh.m, spos, sh.bm, sh.cl_cell, sh.cl_start, &sh.ncl);
-                if (!scanned) yk_rs_scan_n(c, sh.m, spos, sh.bm);
+                if (use_lds) {
+                    if (c.nc > 0 && sh.cell_ok)
+                        scanned = yk_rs_scan_cells_n(c, sh.m, lds_pos, sh.bm, sh.cl_cell, sh.cl_start, &sh.ncl);
+                    if (!scanned) yk_rs_scan_n(c, sh.m, lds_pos, sh.bm);
+                } else {
+                    if (c.nc > 0 && sh.cell_ok)
+                        scanned = yk_rs_scan_cells_n(c, sh.m, spos, sh.bm, sh.cl_cell, sh.cl_start, &sh.ncl);
+                    if (!scanned) yk_rs_scan_n(c, sh.m, spos, sh.bm);
+                }
                 __syncthreads();
                 yk_compact(sh.bm, nwords, sh.nblist, &sh.nbcount, sh.wsum);
             } else if (tid == 0) {
@@ -2736,16 +2737,16 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
     // the launch copy of the per-atom real-space arrays back into the state
     __syncthreads();
     for (int i = tid; i < p.natom; i += (int)blockDim.x) {
-        const double4 u = reinterpret_cast<const double4 *>(c.uf)[i];
-        c.uu[i] = u.x; c.Fx[i] = u.y; c.Fy[i] = u.z; c.Fz[i] = u.w;
+        const double4 u = yd_g(reinterpret_cast<const double4 *>(c.uf))[i];
+        yd_g(c.uu)[i] = u.x; yd_g(c.Fx)[i] = u.y; yd_g(c.Fy)[i] = u.z; yd_g(c.Fz)[i] = u.w;
     }
     if (tid == 0) {
-        if (mode != 1) c.st->ctr = t0.rng.ctr;
-        c.st->n_moves += t0.nmv;
-        c.st->n_accepted += t0.acc_count;
-        c.st->n_overlap += t0.nov;
-        c.st->since_resync += t0.nmv;
-        for (int i = 0; i < 9; ++i) c.st->phase[i] += sh.ph[i];
+        if (mode != 1) yd_g(c.st)->ctr = t0.rng.ctr;
+        yd_g(c.st)->n_moves += t0.nmv;
+        yd_g(c.st)->n_accepted += t0.acc_count;
+        yd_g(c.st)->n_overlap += t0.nov;
+        yd_g(c.st)->since_resync += t0.nmv;
+        for (int i = 0; i < 9; ++i) yd_g(c.st)->phase[i] += sh.ph[i];
     }
 }
 
@@ -2867,9 +2868,9 @@ __device__ void yg_rs_full(const YdChain &c, const YkMol &m, const double *sig6,
                            int a1, double (&acc)[YK_NRX], int &ov) {
     const int T = c.p.ntype, me = m.imol + 1;
     for (int j = a0 + (int)threadIdx.x; j < a1; j += (int)blockDim.x) {
-        if (c.molnum[j] == me) continue;
-        const int jt = c.type[j];
-        const double qj = c.q[j], dhj = c.dh[j], xj = c.x[j], yj = c.y[j], zj = c.z[j];
+        if (yd_g(c.molnum)[j] == me) continue;
+        const int jt = yd_g(c.type)[j];
+        const double qj = yd_g(c.q)[j], dhj = yd_g(c.dh)[j], xj = yd_g(c.x)[j], yj = yd_g(c.y)[j], zj = yd_g(c.z)[j];
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             const int it = m.typ[a];
@@ -2881,7 +2882,7 @@ __device__ void yg_rs_full(const YdChain &c, const YkMol &m, const double *sig6,
                 const bool lj = it != 0 && jt != 0;
                 const int off = lj ? (it - 1) * T + (jt - 1) : 0;
                 YdPair pr;
-                yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+                yd_pair(c.p, r2, false, m.q[a], qj, lj, yd_g(sig6)[off], yd_g(feps)[off], pr);
                 yk_rs_acc(acc, a, pr, xij, yij, zij);
             }
             xij = yd_wrap(m.xo[a] - xj); yij = yd_wrap(m.yo[a] - yj); zij = yd_wrap(m.zo[a] - zj);
@@ -2890,7 +2891,7 @@ __device__ void yg_rs_full(const YdChain &c, const YkMol &m, const double *sig6,
                 const bool lj = it != 0 && jt != 0;
                 const int off = lj ? (it - 1) * T + (jt - 1) : 0;
                 YdPair pr;
-                yd_pair(c.p, r2, false, m.q[a], qj, lj, sig6[off], feps[off], pr);
+                yd_pair(c.p, r2, false, m.q[a], qj, lj, yd_g(sig6)[off], yd_g(feps)[off], pr);
                 acc[YK_NRS + 0] += pr.potew;
                 acc[YK_NRS + 1] += pr.potlj6;
                 acc[YK_NRS + 2] += pr.potlj12;
@@ -2943,15 +2944,15 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
     const bool use_lds = lds_atoms >= p.natom;
     unsigned int *const spos = use_lds ? lds_pos : reinterpret_cast<unsigned int *>(c.pos4);
     if (use_lds) {
-        for (int i = tid; i < p.natom; i += BS) spos[i] = yk_pack_pos(c.x[i], c.y[i], c.z[i]);
+        for (int i = tid; i < p.natom; i += BS) spos[i] = yk_pack_pos(yd_g(c.x)[i], yd_g(c.y)[i], yd_g(c.z)[i]);
     } else {
-        for (int i = a0 + tid; i < a1; i += BS) spos[i] = yk_pack_pos(c.x[i], c.y[i], c.z[i]);
+        for (int i = a0 + tid; i < a1; i += BS) spos[i] = yk_pack_pos(yd_g(c.x)[i], yd_g(c.y)[i], yd_g(c.z)[i]);
     }
     for (int i = a0 + tid; i < a1; i += BS) {  // the launch copies of the own atoms
-        double4 *r4 = reinterpret_cast<double4 *>(c.arec) + 2 * (size_t)i;
-        r4[0] = make_double4(c.x[i], c.y[i], c.z[i], c.q[i]);
-        r4[1] = make_double4(c.dh[i], (double)c.type[i], 0.0, 0.0);
-        reinterpret_cast<double4 *>(c.uf)[i] = make_double4(c.uu[i], c.Fx[i], c.Fy[i], c.Fz[i]);
+        auto r4 = yd_g(reinterpret_cast<double4 *>(c.arec)) + 2 * (size_t)i;
+        r4[0] = make_double4(yd_g(c.x)[i], yd_g(c.y)[i], yd_g(c.z)[i], yd_g(c.q)[i]);
+        r4[1] = make_double4(yd_g(c.dh)[i], (double)yd_g(c.type)[i], 0.0, 0.0);
+        yd_g(reinterpret_cast<double4 *>(c.uf))[i] = make_double4(yd_g(c.uu)[i], yd_g(c.Fx)[i], yd_g(c.Fy)[i], yd_g(c.Fz)[i]);
     }
     if (tid == 0) { gs.gen = 0; gs.last_imol = -1; }
     if (c.nc > 0 && lead) yk_cells_build(c, &sh.cell_ok);
@@ -2963,10 +2964,10 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
         sh.t_last = wall_clock64();
         t0.acc_count = 0; t0.nmv = 0; t0.nov = 0;
         YdRng &rng = t0.rng;
-        rng.key0 = c.st->key[0]; rng.key1 = c.st->key[1];
-        rng.s0 = c.st->stream[0]; rng.s1 = c.st->stream[1];
-        rng.ctr = c.st->ctr; rng.constant = mode == 1; rng.cu = 0.0;
-        rng.list = c.st->ulist; rng.nlist = c.st->nlist;
+        rng.key0 = yd_g(c.st)->key[0]; rng.key1 = yd_g(c.st)->key[1];
+        rng.s0 = yd_g(c.st)->stream[0]; rng.s1 = yd_g(c.st)->stream[1];
+        rng.ctr = yd_g(c.st)->ctr; rng.constant = mode == 1; rng.cu = 0.0;
+        rng.list = yd_g(c.st)->ulist; rng.nlist = yd_g(c.st)->nlist;
     }
     bool ok = yg_sync(bar, G, gs.gen, &gs.sync_ok);
     if (ok && c.nc > 0 && !lead && tid == 0)
@@ -2984,9 +2985,9 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
         if (tid < sh.m.na) {
             YkMol &m = sh.m;
             const int a = tid, gi = m.first + a;
-            m.q[a] = c.q[gi]; m.typ[a] = c.type[gi]; m.mass[a] = c.mass[gi]; m.dh[a] = c.dh[gi];
+            m.q[a] = yd_g(c.q)[gi]; m.typ[a] = yd_g(c.type)[gi]; m.mass[a] = yd_g(c.mass)[gi]; m.dh[a] = yd_g(c.dh)[gi];
             if (cached) { m.xo[a] = gs.lx[a]; m.yo[a] = gs.ly[a]; m.zo[a] = gs.lz[a]; }
-            else { m.xo[a] = c.x[gi]; m.yo[a] = c.y[gi]; m.zo[a] = c.z[gi]; }
+            else { m.xo[a] = yd_g(c.x)[gi]; m.yo[a] = yd_g(c.y)[gi]; m.zo[a] = yd_g(c.z)[gi]; }
         }
         __syncthreads();
         if (tid == 0) { yk_sec_slots(sh.m); gs.last_imol = -1; }
@@ -3008,7 +3009,7 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
             sh.cell_ok = (int)__hip_atomic_load(part1 + YG_PART - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         yg_reduce(part1, G, YK_NACC, gs.rtmp, sh.out);
         if (tid < sh.m.na) {  // the per-site real-space arrays (written before B1 by their owner)
-            const double4 u = reinterpret_cast<const double4 *>(c.uf)[sh.m.first + tid];
+            const double4 u = yd_g(reinterpret_cast<const double4 *>(c.uf))[sh.m.first + tid];
             sh.uu_old[tid] = u.x;
             sh.fr_old[tid][0] = u.y; sh.fr_old[tid][1] = u.z; sh.fr_old[tid][2] = u.w;
         }
@@ -3141,7 +3142,7 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
             double acc[3] = {0, 0, 0};
             const int me = sh.m.imol + 1;
             for (int j = a0 + tid; j < a1; j += BS)
-                if (c.molnum[j] != me) yk_rs_apply<true>(c, sh.m, sig6, feps, j, acc);
+                if (yd_g(c.molnum)[j] != me) yk_rs_apply<true>(c, sh.m, sig6, feps, j, acc);
         }
         __syncthreads();
         YK_TICK(7);
@@ -3168,18 +3169,18 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
     if (ok) ok = yg_sync(bar, G, gs.gen, &gs.sync_ok);
     if (ok) {
         for (int i = a0 + tid; i < a1; i += BS) {
-            const double4 u = reinterpret_cast<const double4 *>(c.uf)[i];
-            c.uu[i] = u.x; c.Fx[i] = u.y; c.Fy[i] = u.z; c.Fz[i] = u.w;
+            const double4 u = yd_g(reinterpret_cast<const double4 *>(c.uf))[i];
+            yd_g(c.uu)[i] = u.x; yd_g(c.Fx)[i] = u.y; yd_g(c.Fy)[i] = u.z; yd_g(c.Fz)[i] = u.w;
         }
     }
     if (tid == 0 && lead) {
-        if (mode != 1) c.st->ctr = t0.rng.ctr;
-        c.st->n_moves += t0.nmv;
-        c.st->n_accepted += t0.acc_count;
-        c.st->n_overlap += t0.nov;
-        c.st->since_resync += t0.nmv;
-        for (int i = 0; i < 9; ++i) c.st->phase[i] += sh.ph[i];
-        if (!ok) c.st->grid_fail = 1;
+        if (mode != 1) yd_g(c.st)->ctr = t0.rng.ctr;
+        yd_g(c.st)->n_moves += t0.nmv;
+        yd_g(c.st)->n_accepted += t0.acc_count;
+        yd_g(c.st)->n_overlap += t0.nov;
+        yd_g(c.st)->since_resync += t0.nmv;
+        for (int i = 0; i < 9; ++i) yd_g(c.st)->phase[i] += sh.ph[i];
+        if (!ok) yd_g(c.st)->grid_fail = 1;
     }
 }
 
@@ -3248,13 +3249,13 @@ __device__ int yw_scan(const YdChain &c, const YkMol &m, int lane) {
         int j = base + lane;
         bool near = false;
         if (j < N) {
-            float4 pj = c.pos4[j];
+            float4 pj = yd_g(c.pos4)[j];
             near = yk_near<NA>(xof, yof, zof, xnf, ynf, znf, pj, r2f) && __float_as_int(pj.w) != me;
         }
         unsigned long long mask = __ballot(near);
         if (near) {
             int pos = cnt + __popcll(mask & below);
-            if (pos < YK_NBCAP) c.nbl[pos] = j;
+            if (pos < YK_NBCAP) yd_g(c.nbl)[pos] = j;
         }
         cnt += __popcll(mask);
     }
@@ -3294,18 +3295,18 @@ yk_wchain(const YdChain *chains, int nchains,
     {
         double v[3] = {0, 0, 0};
         for (int i = lane; i < p.natom; i += 64) {
-            double x = c.x[i], y = c.y[i], z = c.z[i], q = c.q[i];
-            c.pos4[i] = make_float4((float)x, (float)y, (float)z, __int_as_float(c.molnum[i]));
+            double x = yd_g(c.x)[i], y = yd_g(c.y)[i], z = yd_g(c.z)[i], q = yd_g(c.q)[i];
+            yd_g(c.pos4)[i] = make_float4((float)x, (float)y, (float)z, __int_as_float(yd_g(c.molnum)[i]));
             v[0] += q * x; v[1] += q * y; v[2] += q * z;
         }
         yw_sum<3>(v);
         t0.mu[0] = v[0]; t0.mu[1] = v[1]; t0.mu[2] = v[2];
         t0.acc_count = 0; t0.nmv = 0; t0.nov = 0;
         YdRng &rng = t0.rng;
-        rng.key0 = c.st->key[0]; rng.key1 = c.st->key[1];
-        rng.s0 = c.st->stream[0]; rng.s1 = c.st->stream[1];
-        rng.ctr = c.st->ctr; rng.constant = 0; rng.cu = 0.0;
-        rng.list = c.st->ulist; rng.nlist = c.st->nlist;
+        rng.key0 = yd_g(c.st)->key[0]; rng.key1 = yd_g(c.st)->key[1];
+        rng.s0 = yd_g(c.st)->stream[0]; rng.s1 = yd_g(c.st)->stream[1];
+        rng.ctr = yd_g(c.st)->ctr; rng.constant = 0; rng.cu = 0.0;
+        rng.list = yd_g(c.st)->ulist; rng.nlist = yd_g(c.st)->nlist;
     }
     yw_sync();
     const int total = ncycles * p.nmol;
@@ -3315,10 +3316,10 @@ yk_wchain(const YdChain *chains, int nchains,
         if (m.imol < 0) break;  // exchange step: the host takes over (launched only without them)
         if (lane < m.na) {
             int a = lane, g = m.first + a;
-            m.q[a] = c.q[g]; m.typ[a] = c.type[g]; m.mass[a] = c.mass[g]; m.dh[a] = c.dh[g];
-            m.xo[a] = c.x[g]; m.yo[a] = c.y[g]; m.zo[a] = c.z[g];
-            sh.uu_old[a] = c.uu[g];
-            sh.fr_old[a][0] = c.Fx[g]; sh.fr_old[a][1] = c.Fy[g]; sh.fr_old[a][2] = c.Fz[g];
+            m.q[a] = yd_g(c.q)[g]; m.typ[a] = yd_g(c.type)[g]; m.mass[a] = yd_g(c.mass)[g]; m.dh[a] = yd_g(c.dh)[g];
+            m.xo[a] = yd_g(c.x)[g]; m.yo[a] = yd_g(c.y)[g]; m.zo[a] = yd_g(c.z)[g];
+            sh.uu_old[a] = yd_g(c.uu)[g];
+            sh.fr_old[a][0] = yd_g(c.Fx)[g]; sh.fr_old[a][1] = yd_g(c.Fy)[g]; sh.fr_old[a][2] = yd_g(c.Fz)[g];
         }
         yw_sync();
         yk_sec_slots(m);
@@ -3417,11 +3418,11 @@ yk_wchain(const YdChain *chains, int nchains,
         yw_sync();
     }
     if (lane == 0) {
-        c.st->ctr = t0.rng.ctr;
-        c.st->n_moves += t0.nmv;
-        c.st->n_accepted += t0.acc_count;
-        c.st->n_overlap += t0.nov;
-        c.st->since_resync += t0.nmv;
+        yd_g(c.st)->ctr = t0.rng.ctr;
+        yd_g(c.st)->n_moves += t0.nmv;
+        yd_g(c.st)->n_accepted += t0.acc_count;
+        yd_g(c.st)->n_overlap += t0.nov;
+        yd_g(c.st)->since_resync += t0.nmv;
     }
 }
 
@@ -3468,8 +3469,8 @@ __global__ void __launch_bounds__(YK_BS) yk_kspace_forces(YdChain c, int first, 
         YkMol &m = sh.m;
         m.imol = -1; m.first = first; m.na = n;
         for (int a = 0; a < n; ++a) {
-            m.q[a] = c.q[first + a];
-            m.typ[a] = c.type[first + a];
+            m.q[a] = yd_g(c.q)[first + a];
+            m.typ[a] = yd_g(c.type)[first + a];
             m.xo[a] = x[a]; m.yo[a] = y[a]; m.zo[a] = z[a];
         }
         yk_sec_slots(m);  // atoms beyond YK_MAXP distinct LJ types keep slot -1
@@ -3517,18 +3518,18 @@ __global__ void __launch_bounds__(YK_BS) yk_rspace_partial(YdChain c, int imol, 
         const YkMol &m = sh.m;
         for (int j = threadIdx.x; j < N; j += (int)blockDim.x) {
             double b[4] = {0, 0, 0, 0};
-            if (c.molnum[j] != me) {
-                int jt = c.type[j];
+            if (yd_g(c.molnum)[j] != me) {
+                int jt = yd_g(c.type)[j];
                 for (int a = 0; a < m.na; ++a) {
                     int it = m.typ[a];
-                    double xij = yd_wrap(m.xn[a] - c.x[j]), yij = yd_wrap(m.yn[a] - c.y[j]),
-                           zij = yd_wrap(m.zn[a] - c.z[j]);
+                    double xij = yd_wrap(m.xn[a] - yd_g(c.x)[j]), yij = yd_wrap(m.yn[a] - yd_g(c.y)[j]),
+                           zij = yd_wrap(m.zn[a] - yd_g(c.z)[j]);
                     double r2 = xij * xij + yij * yij + zij * zij;
                     if (r2 >= c.p.rmax2) continue;
                     bool lj = it != 0 && jt != 0;
                     int off = lj ? (it - 1) * T + (jt - 1) : 0;
                     YdPair pr;
-                    yd_pair(c.p, r2, false, m.q[a], c.q[j], lj, sig6[off], feps[off], pr);
+                    yd_pair(c.p, r2, false, m.q[a], yd_g(c.q)[j], lj, yd_g(sig6)[off], yd_g(feps)[off], pr);
                     b[0] = b[0] + pr.pot;
                     b[1] = b[1] - xij * pr.FR;
                     b[2] = b[2] - yij * pr.FR;
@@ -3557,7 +3558,7 @@ __global__ void __launch_bounds__(256) yk_xchg_delta(YdChain c, YxAtoms at, doub
     }
     __syncthreads();
     for (int kk = blockIdx.x * blockDim.x + tid; kk < nk; kk += gridDim.x * blockDim.x) {
-        int kxyz = c.kxyz[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        int kxyz = yd_g(c.kxyz)[kk], kx = kxyz & 255, ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
         for (int set = 0; set <= T; ++set) {
             if (set > 0 && !((at.pmask >> (set - 1)) & 1u)) continue;
             double ns[8] = {0, 0, 0, 0, 0, 0, 0, 0}, os[8] = {0, 0, 0, 0, 0, 0, 0, 0}, krc[8], krl[8];
@@ -3592,26 +3593,26 @@ __global__ void __launch_bounds__(256) yk_xchg_du(YdChain c, const double *D, un
     const int nk = c.p.nk, T = c.p.ntype;
     double acc[2] = {0.0, 0.0};
     for (int kk = threadIdx.x; kk < nk; kk += (int)blockDim.x) {
-        int kxyz = c.kxyz[kk], ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
+        int kxyz = yd_g(c.kxyz)[kk], ky = (kxyz >> 8) & 255, kz = (kxyz >> 16) & 255;
         double C[8], d[8], tw[8], d2[8];
         for (int set = 0; set <= T; ++set) {
             const bool pres = set == 0 || ((pmask >> (set - 1)) & 1u);
             size_t base = (size_t)set * 8 * nk + kk;
             for (int i = 0; i < 8; ++i) {
                 size_t idx = base + (size_t)i * nk;
-                C[i] = F32 ? (double)c.Shi[idx] + (double)c.Slo[idx] : c.S[idx];
+                C[i] = F32 ? (double)yd_g(c.Shi)[idx] + (double)yd_g(c.Slo)[idx] : yd_g(c.S)[idx];
                 d[i] = pres ? D[idx] : 0.0;
                 tw[i] = C[i] + C[i] + d[i];
             }
             if (set == 0) {
-                acc[0] += yx_slots(tw, d, ky, kz) * c.beta[kk];
+                acc[0] += yx_slots(tw, d, ky, kz) * yd_g(c.beta)[kk];
                 continue;
             }
             for (int t2 = 1; t2 <= T; ++t2) {
                 if (!((pmask >> (t2 - 1)) & 1u)) continue;
                 size_t b2 = (size_t)t2 * 8 * nk + kk;
                 for (int i = 0; i < 8; ++i) d2[i] = D[b2 + (size_t)i * nk];
-                acc[1] += yx_slots(tw, d2, ky, kz) * c.beta_lj[((size_t)(set - 1) * T + (t2 - 1)) * nk + kk];
+                acc[1] += yx_slots(tw, d2, ky, kz) * yd_g(c.beta_lj)[((size_t)(set - 1) * T + (t2 - 1)) * nk + kk];
             }
         }
     }
@@ -3631,12 +3632,12 @@ __global__ void yk_xchg_commit_sums(YdChain c, const double *D, unsigned pmask) 
         int set = (int)(i / ((size_t)8 * nk));
         if (set > 0 && !((pmask >> (set - 1)) & 1u)) continue;
         if (F32) {
-            double w = (double)c.Shi[i] + (double)c.Slo[i] + D[i];
+            double w = (double)yd_g(c.Shi)[i] + (double)yd_g(c.Slo)[i] + D[i];
             float hi = (float)w;
-            c.Shi[i] = hi;
-            c.Slo[i] = (float)(w - (double)hi);
+            yd_g(c.Shi)[i] = hi;
+            yd_g(c.Slo)[i] = (float)(w - (double)hi);
         } else {
-            c.S[i] = c.S[i] + D[i];
+            yd_g(c.S)[i] = yd_g(c.S)[i] + D[i];
         }
     }
 }
@@ -3650,15 +3651,15 @@ template __global__ void yk_xchg_commit_sums<true>(YdChain, const double *, unsi
 __global__ void yk_rspace_pair(YdChain c, int imol1, int imol2, const double *sig6, const double *feps,
                                double *out) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int T = c.p.ntype, f1 = c.first[imol1], l1 = c.last[imol1], f2 = c.first[imol2], l2 = c.last[imol2];
+    const int T = c.p.ntype, f1 = yd_g(c.first)[imol1], l1 = yd_g(c.last)[imol1], f2 = yd_g(c.first)[imol2], l2 = yd_g(c.last)[imol2];
     double e0 = 0.0, e6 = 0.0, e12 = 0.0;
     for (int i = f1; i <= l1; ++i) {
-        const int it = c.type[i];
+        const int it = yd_g(c.type)[i];
         for (int j = f2; j <= l2; ++j) {
-            const int jt = c.type[j];
-            double xij = yd_wrap(c.x[i] - c.x[j]), yij = yd_wrap(c.y[i] - c.y[j]), zij = yd_wrap(c.z[i] - c.z[j]);
+            const int jt = yd_g(c.type)[j];
+            double xij = yd_wrap(yd_g(c.x)[i] - yd_g(c.x)[j]), yij = yd_wrap(yd_g(c.y)[i] - yd_g(c.y)[j]), zij = yd_wrap(yd_g(c.z)[i] - yd_g(c.z)[j]);
             double r2 = xij * xij + yij * yij + zij * zij;
-            double d12 = (c.dh[i] + c.dh[j]) / 2.0;
+            double d12 = (yd_g(c.dh)[i] + yd_g(c.dh)[j]) / 2.0;
             if (it > 0 && jt > 0 && r2 < d12 * d12) {
                 out[0] = e0; out[1] = e6; out[2] = e12;
                 return;
@@ -3667,7 +3668,7 @@ __global__ void yk_rspace_pair(YdChain c, int imol1, int imol2, const double *si
             bool lj = it != 0 && jt != 0;
             int off = lj ? (it - 1) * T + (jt - 1) : 0;
             YdPair pr;
-            yd_pair(c.p, r2, false, c.q[i], c.q[j], lj, sig6[off], feps[off], pr);
+            yd_pair(c.p, r2, false, yd_g(c.q)[i], yd_g(c.q)[j], lj, yd_g(sig6)[off], yd_g(feps)[off], pr);
             e0 = e0 + pr.potew;
             if (lj) {
                 e6 = e6 + pr.potlj6;
@@ -3688,20 +3689,20 @@ __global__ void yk_xchg_commit_rs(YdChain c, const double *bn1, const double *bo
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
         if (j >= f1 && j < f1 + n1) {
             const double *v = mv + 4 * (j - f1);
-            c.uu[j] = v[0]; c.Fx[j] = v[1]; c.Fy[j] = v[2]; c.Fz[j] = v[3];
+            yd_g(c.uu)[j] = v[0]; yd_g(c.Fx)[j] = v[1]; yd_g(c.Fy)[j] = v[2]; yd_g(c.Fz)[j] = v[3];
         } else if (j >= f2 && j < f2 + n2) {
             const double *v = mv + 4 * (YD_MAXA + j - f2);
-            c.uu[j] = v[0]; c.Fx[j] = v[1]; c.Fy[j] = v[2]; c.Fz[j] = v[3];
+            yd_g(c.uu)[j] = v[0]; yd_g(c.Fx)[j] = v[1]; yd_g(c.Fy)[j] = v[2]; yd_g(c.Fz)[j] = v[3];
         } else if (n2 > 0) {
-            c.uu[j] = c.uu[j] + bn1[j] - bo1[j] + bn2[j] - bo2[j];
-            c.Fx[j] = c.Fx[j] + bn1[N + j] - bo1[N + j] + bn2[N + j] - bo2[N + j];
-            c.Fy[j] = c.Fy[j] + bn1[2 * N + j] - bo1[2 * N + j] + bn2[2 * N + j] - bo2[2 * N + j];
-            c.Fz[j] = c.Fz[j] + bn1[3 * N + j] - bo1[3 * N + j] + bn2[3 * N + j] - bo2[3 * N + j];
+            yd_g(c.uu)[j] = yd_g(c.uu)[j] + bn1[j] - bo1[j] + bn2[j] - bo2[j];
+            yd_g(c.Fx)[j] = yd_g(c.Fx)[j] + bn1[N + j] - bo1[N + j] + bn2[N + j] - bo2[N + j];
+            yd_g(c.Fy)[j] = yd_g(c.Fy)[j] + bn1[2 * N + j] - bo1[2 * N + j] + bn2[2 * N + j] - bo2[2 * N + j];
+            yd_g(c.Fz)[j] = yd_g(c.Fz)[j] + bn1[3 * N + j] - bo1[3 * N + j] + bn2[3 * N + j] - bo2[3 * N + j];
         } else {  // one molecule (movemc's commit, MC.f90:1257-1275)
-            c.uu[j] = c.uu[j] + bn1[j] - bo1[j];
-            c.Fx[j] = c.Fx[j] + bn1[N + j] - bo1[N + j];
-            c.Fy[j] = c.Fy[j] + bn1[2 * N + j] - bo1[2 * N + j];
-            c.Fz[j] = c.Fz[j] + bn1[3 * N + j] - bo1[3 * N + j];
+            yd_g(c.uu)[j] = yd_g(c.uu)[j] + bn1[j] - bo1[j];
+            yd_g(c.Fx)[j] = yd_g(c.Fx)[j] + bn1[N + j] - bo1[N + j];
+            yd_g(c.Fy)[j] = yd_g(c.Fy)[j] + bn1[2 * N + j] - bo1[2 * N + j];
+            yd_g(c.Fz)[j] = yd_g(c.Fz)[j] + bn1[3 * N + j] - bo1[3 * N + j];
         }
     }
 }
