@@ -423,6 +423,11 @@ module yamc_ewald
          type(c_ptr), value :: ctx
          integer(c_int32_t), value :: groups
       end function
+      integer(c_int) function yamc_set_cell_div(ctx, div) bind(C, name="yamc_set_cell_div")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value :: ctx
+         integer(c_int32_t), value :: div
+      end function
       integer(c_int) function yamc_last_kernel_ms(ctx, ms) bind(C, name="yamc_last_kernel_ms")
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: ctx

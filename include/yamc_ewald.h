@@ -428,6 +428,16 @@ int yamc_set_block_size(yamc_ctx *ctx, int32_t block);
  * summation order of the partial sums.  Batches always run one workgroup
  * per chain.  No reference counterpart (the reference is serial). */
 int yamc_set_chain_groups(yamc_ctx *ctx, int32_t groups);
+/* Cell list of the chain kernels' real-space scan: cubic cells of width at
+ * least rc / div box units (div = 1, 2, 3 or 4; nc = floor(div / rc) cells
+ * per axis), or none (div = 0: the scan visits every atom).  Fewer than 3
+ * cells per axis means none.  A move then tests only the atoms of the cells
+ * within the scan radius of its sites (a sphere-pruned stencil), which
+ * finds exactly the candidates of the full scan, so results do not depend
+ * on div.  Default at creation: 1 when that gives 4 or more cells per axis,
+ * else 0.  The reference has no cell list (EwaldSumRealSpace.f90:367 loops
+ * over all atoms); the autotuner (yamc_amd.tune) measures div. */
+int yamc_set_cell_div(yamc_ctx *ctx, int32_t div);
 
 /* ---- diagnostics ------------------------------------------------------- */
 /* Time of the last yamc_mc_run / yamc_batch_run kernel launch in ms (HIP

@@ -31,7 +31,7 @@ EXPORTS = [
     "yamc_mc_run", "yamc_mc_run_const", "yamc_resync", "yamc_mc_volume_move", "yamc_mc_run_npt", "yamc_box_length",
     "yamc_mc_set_frequencies", "yamc_mc_xchange", "yamc_set_box", "yamc_mc_commit_move", "yamc_batch_run", "yamc_batch_create", "yamc_batch_launch",
     "yamc_batch_destroy", "yamc_forces", "yamc_rdf",
-    "yamc_set_block_size", "yamc_set_chain_groups", "yamc_last_kernel_ms", "yamc_phase_times", "yamc_last_error",
+    "yamc_set_block_size", "yamc_set_chain_groups", "yamc_set_cell_div", "yamc_last_kernel_ms", "yamc_phase_times", "yamc_last_error",
     # per-procedure surface of the Fortran drop-in modules
     "yamc_ws_create", "yamc_ws_destroy", "yamc_ws_grid", "yamc_op_beta", "yamc_op_beta_lj", "yamc_op_sincos_xyz",
     "yamc_op_sincos_kr", "yamc_op_sincos_kr_atom", "yamc_op_sums", "yamc_op_rho_squared", "yamc_op_energy_coulomb",

@@ -108,6 +108,7 @@ struct yamc_ctx {
     const double **d_sig6s = nullptr, **d_feps = nullptr;
     bool beta_ready = false, state_ready = false;
     int chain_bs = 256;  // launch shape of the chain kernel (yamc_set_block_size)
+    int cell_div = 0;    // cells per cut-off of the scan's cell list (yamc_set_cell_div; 0: none)
     // grid chain (yamc_set_chain_groups): workgroups per chain and their
     // buffers (per-workgroup descriptors, back terms, partial sums, barrier)
     int groups = 1, gcap = 0;
@@ -146,6 +147,37 @@ static int dalloc(yamc_ctx *c, T **p, size_t n) {
         int r_ = (x);               \
         if (r_ != YAMC_OK) return r_; \
     } while (0)
+
+// Cell list of the chain kernels' scan: cells of width >= rc / div box units
+// (nc = floor(div / rc) per axis, at most 40), none when div = 0 or nc < 3.
+// Replaces the context's cell arrays; the buckets are built by each launch.
+static int setup_cells(yamc_ctx *c, int div) {
+    YdChain &h = c->ch;
+    for (void *p : {(void *)h.cell_cnt, (void *)h.cell_atoms, (void *)h.atom_cell, (void *)h.atom_slot}) {
+        if (!p) continue;
+        for (size_t i = 0; i < c->owned.size(); ++i)
+            if (c->owned[i] == p) { c->owned.erase(c->owned.begin() + (long)i); break; }
+        HIPCHK(hipFree(p));
+    }
+    h.cell_cnt = h.cell_atoms = h.atom_cell = h.atom_slot = nullptr;
+    h.nc = 0;
+    h.ccap = 0;
+    c->cell_div = 0;
+    const double rc = sqrt(h.p.rmax2);
+    int nc = div > 0 && rc > 0 ? (int)floor((double)div / (rc * (1.0 + 1e-9))) : 0;
+    if (nc > 40) nc = 40;
+    if (nc < 3) return YAMC_OK;
+    const int N = h.p.natom, ncell = nc * nc * nc;
+    const int cap = ((2 * N) / ncell + 64 + 63) / 64 * 64;
+    TRY(dalloc(c, &h.cell_cnt, (size_t)ncell));
+    TRY(dalloc(c, &h.cell_atoms, (size_t)ncell * cap));
+    TRY(dalloc(c, &h.atom_cell, (size_t)N));
+    TRY(dalloc(c, &h.atom_slot, (size_t)N));
+    h.nc = nc;
+    h.ccap = cap;
+    c->cell_div = div;
+    return YAMC_OK;
+}
 
 // Stream-ordered device scratch, freed (stream-ordered) on every exit path.
 struct StreamScratch {
@@ -303,22 +335,12 @@ extern "C" int yamc_ctx_create(const yamc_system_desc *d, yamc_ctx **out) {
         HIPCHK(hipStreamSynchronize(c->stream));
         TRY(dalloc(c, &h.freq_end, M)); TRY(dalloc(c, &h.freq_beg, M)); TRY(dalloc(c, &h.st, 1));
         // cell list of the real-space scan: cells of width >= rc (box units;
-        // rmax2 is constant in box units, so volume steps keep nc); only
-        // worth it from 4 cells per axis on
+        // rmax2 is constant in box units, so volume steps keep nc) when that
+        // gives 4 or more per axis (yamc_set_cell_div changes it)
         {
             const double rc = sqrt(d->rmax2);
-            int nc = rc > 0 ? (int)floor(1.0 / (rc * (1.0 + 1e-9))) : 0;
-            if (nc > 40) nc = 40;
-            if (nc >= 4) {
-                const int ncell = nc * nc * nc;
-                const int cap = ((2 * N) / ncell + 64 + 63) / 64 * 64;
-                h.nc = nc;
-                h.ccap = cap;
-                TRY(dalloc(c, &h.cell_cnt, (size_t)ncell));
-                TRY(dalloc(c, &h.cell_atoms, (size_t)ncell * cap));
-                TRY(dalloc(c, &h.atom_cell, (size_t)N));
-                TRY(dalloc(c, &h.atom_slot, (size_t)N));
-            }
+            const int nc = rc > 0 ? (int)floor(1.0 / (rc * (1.0 + 1e-9))) : 0;
+            if (nc >= 4) TRY(setup_cells(c, 1));
         }
         TRY(dalloc(c, &c->d_lj6, TT)); TRY(dalloc(c, &c->d_lj12, TT));
         TRY(dalloc(c, &c->d_sig6, TT)); TRY(dalloc(c, &c->d_4eps, TT));
@@ -978,6 +1000,15 @@ extern "C" int yamc_set_block_size(yamc_ctx *c, int32_t block) {
                     block);
     c->chain_bs = block;
     return YAMC_OK;
+}
+
+extern "C" int yamc_set_cell_div(yamc_ctx *c, int32_t div) {
+    if (!c) return fail(YAMC_ERROR_PARAMETER, "null context");
+    if (div < 0 || div > 4) return fail(YAMC_ERROR_PARAMETER, "%d cells per cut-off: must be 0..4", div);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    TRY(setup_cells(c, div));
+    HIPCHK(hipMemcpyAsync(c->d_chain, &c->ch, sizeof(YdChain), hipMemcpyHostToDevice, c->stream));
+    return sync(c);
 }
 
 extern "C" int yamc_set_chain_groups(yamc_ctx *c, int32_t groups) {

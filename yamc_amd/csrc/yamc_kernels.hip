@@ -30,7 +30,7 @@
 // Candidate bitmap over atom indices (order-independent marking, then a
 // deterministic ascending compaction into the list).
 #define YK_BMWORDS 4096  // 131072 atoms
-#define YK_MAXCL 125     // cells visited by one cell-list scan (5 per axis)
+#define YK_MAXCL 512     // cells visited by one cell-list scan
 // Positions each thread loads ahead in the real-space scan (independent
 // loads in flight per thread when the packed positions are read from HBM).
 #define YK_SCANU 8
@@ -1035,13 +1035,20 @@ __device__ __forceinline__ bool yk_near(const float (&xof)[NA], const float (&yo
     return near;
 }
 
-// Cell-list form of the scan: only the atoms in the cells around the old
-// and new sites are tested.  Cells are at least a cut-off wide, so every
-// atom within the cut-off of a site is in the site's cell or a neighbour of
-// it (per axis, modulo nc); the visited range per axis spans the sites'
-// cells +-1 (the whole axis when that reaches nc cells, or when the sites
-// are not within one cell of each other).  cl_* are the workgroup's LDS
-// arrays; all threads call this.
+// Cell-list form of the scan: only the atoms of the cells that can hold an
+// atom within the scan radius of an old or new site are tested.  Cells are
+// cubes of width 1/nc box units, any width (cells narrower than the cut-off
+// give a finer stencil).  The visited cells: per (x, y) column of the
+// range the sites' extent +- rv spans, the contiguous z run whose cells are
+// within rv of the sites' bounding box (rv = the scan radius plus the
+// packing error and rounding margins), so every atom the full scan would
+// mark is tested and the candidate bitmap is the full scan's.  An axis
+// whose range spans nc cells or more is visited whole (each cell once).
+// Wave 0 builds the list and its bucket offsets lane-parallel (cl_cell:
+// cell ids, cl_start: running atom offsets); then all threads scan the
+// cells' atoms as one flat range.  cl_* are the workgroup's LDS arrays; all
+// threads call this.  Returns false (the same in every thread) when the
+// list would exceed YK_MAXCL cells: the caller scans every atom instead.
 // split_G > 1 (a grid chain): workgroup split_g tests only its share
 // [total g / G, total (g + 1) / G) of the visited cells' atoms, at most
 // YK_NBCAP of them; when the cells hold more than G YK_NBCAP atoms it
@@ -1052,47 +1059,92 @@ __device__ __forceinline__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &
                                  int *cl_start, int *ncl_out, int split_g = 0, int split_G = 1) {
     const int nc = c.nc, tid = threadIdx.x;
     const unsigned int f0 = (unsigned int)m.first, na = (unsigned int)m.na;
-    int start[3], len[3];
+    const double w = 1.0 / (double)nc;
+    // the scan radius (yk_scan_r2) + the packing error of the tested
+    // position (< 8.5e-4) + FP32 and FP64 rounding, box units
+    const double rv = sqrt((double)yk_scan_r2(c)) + 1.2e-3, rv2 = rv * rv;
+    double lo[3], hi[3];
+    int st[3], ln[3];
+    bool whole[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const double *po = d == 0 ? m.xo : d == 1 ? m.yo : m.zo;
         const double *pn = d == 0 ? m.xn : d == 1 ? m.yn : m.zn;
-        const int base = yd_cell1(po[0], nc);
-        int lo = 0, hi = 0;
-        bool full = false;
-        for (int k = 0; k < 2 * NA; ++k) {
-            int ci = yd_cell1(k < NA ? po[k] : pn[k - NA], nc);
-            int off = ci - base;
-            if (off < 0) off += nc;
-            int dd = off == 0 ? 0 : off == 1 ? 1 : off == nc - 1 ? -1 : 2;
-            if (dd == 2) full = true;
-            lo = dd < lo ? dd : lo;
-            hi = dd > hi && dd < 2 ? dd : hi;
+        const double ref = po[0];
+        lo[d] = ref; hi[d] = ref;
+        for (int k = 1; k < 2 * NA; ++k) {  // the other sites' nearest images around site 0
+            const double v = ref + yd_wrap((k < NA ? po[k] : pn[k - NA]) - ref);
+            lo[d] = v < lo[d] ? v : lo[d];
+            hi[d] = v > hi[d] ? v : hi[d];
         }
-        lo -= 1;
-        hi += 1;
-        if (full || hi - lo + 1 >= nc) { start[d] = 0; len[d] = nc; }
-        else { start[d] = base + lo; len[d] = hi - lo + 1; }
+        const int i0 = (int)floor((lo[d] - rv + 0.5) * nc), i1 = (int)floor((hi[d] + rv + 0.5) * nc);
+        whole[d] = i1 - i0 + 1 >= nc;
+        st[d] = whole[d] ? 0 : i0;
+        ln[d] = whole[d] ? nc : i1 - i0 + 1;
     }
-    const int ncl = len[0] * len[1] * len[2];
-    if (ncl > YK_MAXCL) return false;  // uniform: the caller scans every atom instead
-    if (tid < ncl) {
-        int t = tid, iz = t % len[2], iy = (t / len[2]) % len[1], ix = t / (len[2] * len[1]);
-        int cx = (start[0] + ix) % nc, cy = (start[1] + iy) % nc, cz = (start[2] + iz) % nc;
-        if (cx < 0) cx += nc;
-        if (cy < 0) cy += nc;
-        if (cz < 0) cz += nc;
-        int id = (cx * nc + cy) * nc + cz;
-        cl_cell[tid] = id;
-        cl_start[tid + 1] = yd_g(c.cell_cnt)[id];
+    // distance from the sites' bounding box to unwrapped cell i on axis d
+    auto gap = [&](int d, int i) {
+        if (whole[d]) return 0.0;
+        const double a = (double)i * w - 0.5, b = a + w;
+        const double g1 = a - hi[d], g2 = lo[d] - b;
+        return g1 > 0.0 ? g1 : g2 > 0.0 ? g2 : 0.0;
+    };
+    auto wrapc = [&](int i) { i %= nc; return i < 0 ? i + nc : i; };
+    if (tid < 64) {
+        const int lane = tid, ncol = ln[0] * ln[1];
+        int total = 0, over = 0;
+        for (int col0 = 0; col0 < ncol; col0 += 64) {
+            const int col = col0 + lane;
+            int zs = 0, zl = 0, cx = 0, cy = 0;
+            if (col < ncol) {
+                const int ix = st[0] + col / ln[1], iy = st[1] + col % ln[1];
+                cx = wrapc(ix); cy = wrapc(iy);
+                const double gx = gap(0, ix), gy = gap(1, iy), rem = rv2 - gx * gx - gy * gy;
+                if (rem > 0.0) {
+                    if (whole[2]) {
+                        zs = 0; zl = nc;
+                    } else {
+                        const double rz = sqrt(rem);
+                        const int z0 = (int)floor((lo[2] - rz + 0.5) * nc), z1 = (int)floor((hi[2] + rz + 0.5) * nc);
+                        zs = z0; zl = z1 - z0 + 1;
+                        if (zl >= nc) { zs = 0; zl = nc; }
+                    }
+                }
+            }
+            int incl = zl;  // inclusive scan of the run lengths inside the wave
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int v = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += v;
+            }
+            const int sum = __shfl(incl, 63, 64);
+            if (total + sum > YK_MAXCL) { over = 1; break; }
+            const int e0 = total + incl - zl;
+            for (int k = 0; k < zl; ++k) cl_cell[e0 + k] = (cx * nc + cy) * nc + wrapc(zs + k);
+            total += sum;
+        }
+        if (!over) {
+            // the buckets' atom counts and their running offsets
+            int carry = 0;
+            if (lane == 0) cl_start[0] = 0;
+            for (int i0 = 0; i0 < total; i0 += 64) {
+                const int i = i0 + lane;
+                const int cnt = i < total ? yd_g(c.cell_cnt)[cl_cell[i]] : 0;
+                int incl = cnt;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int v = __shfl_up(incl, off, 64);
+                    if (lane >= off) incl += v;
+                }
+                if (i < total) cl_start[i + 1] = carry + incl;
+                carry += __shfl(incl, 63, 64);
+            }
+        }
+        if (lane == 0) *ncl_out = over ? -1 : total;
     }
     __syncthreads();
-    if (tid == 0) {
-        cl_start[0] = 0;
-        for (int k = 0; k < ncl; ++k) cl_start[k + 1] += cl_start[k];
-        *ncl_out = ncl;
-    }
-    __syncthreads();
+    const int ncl = *ncl_out;
+    if (ncl < 0) return false;  // uniform: the caller scans every atom instead
     int total = cl_start[ncl], tb = 0;
     if (SPLIT && split_G > 1) {
         if (total > split_G * YK_NBCAP) return false;

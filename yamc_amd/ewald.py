@@ -318,6 +318,11 @@ class EwaldMC:
         G > 1: one chain spread over G co-resident workgroups)."""
         check(self.lib.yamc_set_chain_groups(self.ctx, int(groups)))
 
+    def set_cell_div(self, div: int) -> None:
+        """Cells per cut-off of the real-space scan's cell list (0: none,
+        the scan visits every atom; 1..4: cells of width >= rc / div)."""
+        check(self.lib.yamc_set_cell_div(self.ctx, int(div)))
+
     def last_kernel_ms(self) -> float:
         v = C.c_double()
         check(self.lib.yamc_last_kernel_ms(self.ctx, C.byref(v)))

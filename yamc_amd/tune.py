@@ -19,6 +19,14 @@ tune_batch() times one sweep of the batch itself with each shape and keeps
 the fastest.  The timed sweeps are ordinary Monte Carlo sweeps of the chains
 (every shape runs the same algorithm on the same RNG streams), so tuning
 doubles as warm-up; the choice is applied to every context of the batch.
+
+The shape and the workgroup count fix how many k vectors each wavefront
+sweeps per move: nk / (G * shape / 64) (k_per_wave()).  The cell size of the
+real-space scan's cell list is the other tile: tune_cells_batch() and
+tune_cells() time the cell divisions (cells of width rc / div; div = 0: no
+cell list) that give distinct cell counts and keep the fastest.  The scan
+finds the same candidates at every cell size, so results do not depend on
+it.
 """
 from __future__ import annotations
 
@@ -91,4 +99,59 @@ def tune_groups(rep: EwaldMC, groups: Sequence[int] = None, cycles: int = 1) -> 
         rates[g] = rep.sys.nmol * cycles / (rep.last_kernel_ms() * 1e-3)
     best = max(rates, key=rates.get)
     rep.set_chain_groups(best)
+    return best, rates
+
+
+def k_per_wave(nk: int, shape: int, groups: int = 1) -> float:
+    """k vectors each wavefront sweeps per move for a launch shape and a
+    workgroup count (one wave per replica for the shapes 64 and 128)."""
+    waves = 1 if shape in (64, 128) else shape // 64
+    return nk / float(groups * waves)
+
+
+def cell_divs_for(rep: EwaldMC) -> Tuple[int, ...]:
+    """Cell divisions worth timing: 0 (no cell list) and the divisions 1..4
+    that give a distinct cell count of at least 3 per axis."""
+    rc = float(rep.sys.derived["rmax2"]) ** 0.5
+    out, seen = [0], set()
+    for div in (1, 2, 3, 4):
+        nc = min(40, int(div / (rc * (1.0 + 1e-9))))
+        if nc >= 3 and nc not in seen:
+            seen.add(nc)
+            out.append(div)
+    return tuple(out)
+
+
+def tune_cells_batch(reps: Sequence[EwaldMC], divs: Sequence[int] = None,
+                     cycles: int = 1) -> Tuple[int, Dict[int, float]]:
+    """Cell division of a batch (the same for every replica): times one
+    sweep per candidate with the replicas' current launch shape and keeps
+    the fastest.  Returns (best div, {div: trial moves/s})."""
+    if divs is None:
+        divs = cell_divs_for(reps[0])
+    moves = sum(r.sys.nmol for r in reps) * cycles
+    rates = {}
+    for div in divs:
+        for r in reps:
+            r.set_cell_div(div)
+        _, ms = batch_run(reps, cycles)
+        rates[div] = moves / (ms * 1e-3)
+    best = max(rates, key=rates.get)
+    for r in reps:
+        r.set_cell_div(best)
+    return best, rates
+
+
+def tune_cells(rep: EwaldMC, divs: Sequence[int] = None, cycles: int = 1) -> Tuple[int, Dict[int, float]]:
+    """The same for one chain (yamc_mc_run with its current shape and
+    workgroup count)."""
+    if divs is None:
+        divs = cell_divs_for(rep)
+    rates = {}
+    for div in divs:
+        rep.set_cell_div(div)
+        rep.run(cycles)
+        rates[div] = rep.sys.nmol * cycles / (rep.last_kernel_ms() * 1e-3)
+    best = max(rates, key=rates.get)
+    rep.set_cell_div(best)
     return best, rates
