@@ -164,7 +164,7 @@ def main():
 
     from yamc_amd import YAMC_FP32C, YAMC_FP64, config, system
     from yamc_amd.ewald import Batch, EwaldMC
-    from yamc_amd.tune import tune_batch, tune_groups, tune_single
+    from yamc_amd.tune import k_per_wave, tune_batch, tune_cells, tune_cells_batch, tune_groups, tune_single
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.forces_check:
@@ -198,6 +198,7 @@ def main():
         g.set_rng(20261203, 0)
         shape1, _ = tune_single(g, shapes=(256, 512))  # two ordinary sweeps, one per shape
         groups1, grates = tune_groups(g)  # one ordinary sweep per workgroup count
+        div1, crates = tune_cells(g)  # one ordinary sweep per cell size
         for _ in range(args.warmup):
             g.run(1)
         sync()
@@ -213,7 +214,10 @@ def main():
         moves = args.steps * nmol
         res[form] = dict(moves_per_s=moves / dt, ms_per_step=1e3 * dt / args.steps, kernel_ms=kms,
                          acceptance=nacc / moves, launch_shape=shape1, workgroups_per_chain=groups1,
-                         tuner_moves_per_s_by_workgroups={str(k): round(v, 1) for k, v in grates.items()})
+                         k_per_wave=round(k_per_wave(g.nk, 256 if groups1 > 1 else shape1, groups1), 1),
+                         cell_div=div1,
+                         tuner_moves_per_s_by_workgroups={str(k): round(v, 1) for k, v in grates.items()},
+                         tuner_moves_per_s_by_cell_div={str(k): round(v, 1) for k, v in crates.items()})
         g.close()
 
     # ------------------------------------------------- fp32c error vs fp64
@@ -259,6 +263,7 @@ def main():
             shape, tuned = args.block, None
         else:
             shape, tuned = tune_batch(reps)  # three ordinary sweeps, one per launch shape
+        div, ctuned = tune_cells_batch(reps)  # one ordinary sweep per cell size
         # one captured sweep per step; every replica's S(k) is rebuilt from
         # scratch in FP64 every RESYNC moves inside the sweep (batched)
         bat = Batch(reps, 1, resync_every=RESYNC)
@@ -277,7 +282,9 @@ def main():
         moves = args.steps * nmol * R
         res["batch"] = dict(replicas=R, moves_per_s=moves / dt, ms_per_step=1e3 * dt / args.steps, kernel_ms=kms,
                             acceptance=nacc / moves, launch_shape=shape, resync_every=RESYNC,
-                            tuner_moves_per_s={str(k): v for k, v in tuned.items()} if tuned else None)
+                            k_per_wave=round(k_per_wave(grid["kx"].size, shape), 1), cell_div=div,
+                            tuner_moves_per_s={str(k): v for k, v in tuned.items()} if tuned else None,
+                            tuner_moves_per_s_by_cell_div={str(k): round(v, 1) for k, v in ctuned.items()})
         bat.close()
         batch_err = batch_error(reps[:4], base, kT_kjmol)
         for r in reps:
@@ -535,7 +542,7 @@ def replica_check(args):
 
     from yamc_amd import YAMC_FP32C, replica_system
     from yamc_amd.ewald import Batch, EwaldMC
-    from yamc_amd.tune import tune_batch
+    from yamc_amd.tune import tune_batch, tune_cells_batch
 
     rng = np.random.default_rng(20261206)
     reps, sizes, kmaxes, temps = [], [], [], []
@@ -550,6 +557,7 @@ def replica_check(args):
         kmaxes.append(int(s.derived["kmax"]))
         temps.append(T)
     shape, tuned = tune_batch(reps)
+    div, ctuned = tune_cells_batch(reps, divs=(0, 1, 2, 3))
     bat = Batch(reps, 1, resync_every=RESYNC)
     bat.launch()
     torch_sync()
@@ -576,6 +584,7 @@ def replica_check(args):
         "replicas": 64, "atoms_min": min(sizes), "atoms_max": max(sizes), "atoms_total": sum(sizes),
         "kmax_min": min(kmaxes), "kmax_max": max(kmaxes), "T_min": min(temps), "T_max": max(temps),
         "launch_shape": shape, "tuner_moves_per_s": {str(k): v for k, v in tuned.items()},
+        "cell_div": div, "tuner_moves_per_s_by_cell_div": {str(k): round(v, 1) for k, v in ctuned.items()},
         "moves_per_sweep": moves_per_sweep, "steps": args.steps,
         "moves_per_s": moves_per_sweep * args.steps / dt, "ms_per_sweep": 1e3 * dt / args.steps,
         "kernel_ms_per_sweep": kms / args.steps, "acceptance": nacc / (moves_per_sweep * args.steps),
