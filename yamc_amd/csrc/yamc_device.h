@@ -230,6 +230,15 @@ __device__ double yd_shi_sans_exp(double x) {
     return exp(-x) * shi_f;
 }
 
+// Lane l's value of v for a wave-uniform l (two scalar reads, no LDS round
+// trip; the same bits a shuffle returns).
+__device__ __forceinline__ double yd_readlane(double v, int l) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)b, l);
+    const unsigned int hi = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // shi_sans_exp_f evaluated by a whole wave (every lane passes the same x and
 // gets the same result).  The trapezoid terms dx sinh(t_i)/t_i, i = 2..n
 // (n <= 27), are computed one per lane and then added in index order, which
@@ -253,9 +262,9 @@ __device__ double yd_shi_sans_exp_wave(double x) {
     shi_f = shi_f + dx * sinh(x1) / x1 / 2.0 + dx * dx / 12.0 * (cosh(x1) / x1 - sinh(x1) / (x1 * x1));
     const int lane = threadIdx.x & 63;
     double t = x1 + (double)(lane + 1) * dx, s = sinh(t), term = dx * s / t;  // i = lane + 2
-    for (int l = 0; l < n - 1; ++l) shi_f = shi_f + __shfl(term, l, 64);
-    t = __shfl(t, n - 2, 64);
-    s = __shfl(s, n - 2, 64);
+    for (int l = 0; l < n - 1; ++l) shi_f = shi_f + yd_readlane(term, l);
+    t = yd_readlane(t, n - 2);
+    s = yd_readlane(s, n - 2);
     shi_f = shi_f - dx * s / t / 2.0 - dx * dx / 12.0 * (cosh(t) / t - s / (t * t));
     return exp(-x) * shi_f;
 }
@@ -284,6 +293,13 @@ struct YdRng {
     double cu;
     const double *list;  // replay mode: recorded uniforms list[0 .. nlist)
     unsigned long long nlist;
+    // the last Philox block (the four words of counter block cblk - 1;
+    // cblk = 0: none): consecutive draws take the block's words in turn
+    // instead of evaluating the generator once per draw
+    unsigned long long cblk;
+    uint32_t blk[4];
+    // every constructor of a stream calls this after setting the fields above
+    __device__ __forceinline__ void reset_cache() { cblk = 0; }
     __device__ double next() {
         if (constant) return cu;
         if (list) {  // past the end: 0.5, and the counter shows the overrun to the host
@@ -292,9 +308,12 @@ struct YdRng {
         }
         for (;;) {
             unsigned long long n = ctr++;
-            uint32_t o[4];
-            yd_philox((uint32_t)(n >> 2), (uint32_t)(n >> 34), s0, s1, key0, key1, o);
-            uint32_t k = o[n & 3] >> 8;
+            const unsigned long long b = (n >> 2) + 1;
+            if (b != cblk) {
+                yd_philox((uint32_t)(n >> 2), (uint32_t)(n >> 34), s0, s1, key0, key1, blk);
+                cblk = b;
+            }
+            uint32_t k = blk[n & 3] >> 8;
             if (k != 0) return (double)k * (1.0 / 16777216.0);
         }
     }

@@ -1055,7 +1055,7 @@ __device__ __forceinline__ bool yk_near(const float (&xof)[NA], const float (&yo
 // returns false (the same in every workgroup) and the caller scans atom
 // ranges instead.
 template <int NA, bool SPLIT = false>
-__device__ __forceinline__ bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int *cl_cell,
+__device__ __attribute__((noinline)) bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int *cl_cell,
                                  int *cl_start, int *ncl_out, int split_g = 0, int split_G = 1) {
     const int nc = c.nc, tid = threadIdx.x;
     const unsigned int f0 = (unsigned int)m.first, na = (unsigned int)m.na;
@@ -2561,6 +2561,7 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         rng.key0 = yd_g(c.st)->key[0]; rng.key1 = yd_g(c.st)->key[1];
         rng.s0 = yd_g(c.st)->stream[0]; rng.s1 = yd_g(c.st)->stream[1];
         rng.ctr = yd_g(c.st)->ctr; rng.constant = mode == 1; rng.cu = 0.0;
+        rng.reset_cache();
         rng.list = yd_g(c.st)->ulist; rng.nlist = yd_g(c.st)->nlist;
     }
     for (int mv = 0; mv < total; ++mv) {
@@ -3019,6 +3020,7 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
         rng.key0 = yd_g(c.st)->key[0]; rng.key1 = yd_g(c.st)->key[1];
         rng.s0 = yd_g(c.st)->stream[0]; rng.s1 = yd_g(c.st)->stream[1];
         rng.ctr = yd_g(c.st)->ctr; rng.constant = mode == 1; rng.cu = 0.0;
+        rng.reset_cache();
         rng.list = yd_g(c.st)->ulist; rng.nlist = yd_g(c.st)->nlist;
     }
     bool ok = yg_sync(bar, G, gs.gen, &gs.sync_ok);
@@ -3358,6 +3360,7 @@ yk_wchain(const YdChain *chains, int nchains,
         rng.key0 = yd_g(c.st)->key[0]; rng.key1 = yd_g(c.st)->key[1];
         rng.s0 = yd_g(c.st)->stream[0]; rng.s1 = yd_g(c.st)->stream[1];
         rng.ctr = yd_g(c.st)->ctr; rng.constant = 0; rng.cu = 0.0;
+        rng.reset_cache();
         rng.list = yd_g(c.st)->ulist; rng.nlist = yd_g(c.st)->nlist;
     }
     yw_sync();
