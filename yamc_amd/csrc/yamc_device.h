@@ -397,6 +397,26 @@ __device__ __forceinline__ void yd_kr_pre(const double *tab, int ks, int kx, int
     o[6] = e + f;
     o[7] = g - h;
 }
+// yd_kr_s without the zeroing of the slots of ky = 0 or kz = 0: for callers
+// that read those slots only under the same ky > 0 / kz > 0 conditions
+// (the sweeps' dU and force terms), which then see the same values.
+__device__ __forceinline__ void yd_kr_nz(const double *tab, double q, int kx, int ky, int kz, double *o) {
+    const int ks = YD_KP;
+    double cxk = tab[kx], sxk = tab[ks + kx], cyk = tab[2 * ks + ky], syk = tab[3 * ks + ky];
+    double czk = tab[4 * ks + kz], szk = tab[5 * ks + kz];
+    double qcc = q * cxk * cyk, qss = q * sxk * syk, qsc = q * sxk * cyk, qcs = q * cxk * syk;
+    double qcpy = qcc - qss, qspy = qsc + qcs, qcmy = qcc + qss, qsmy = qsc - qcs;
+    double a = qcpy * czk, b = qspy * szk, c = qspy * czk, d = qcpy * szk;
+    o[0] = a - b;
+    o[1] = c + d;
+    o[2] = a + b;
+    o[3] = c - d;
+    double e = qcmy * czk, f = qsmy * szk, g = qsmy * czk, h = qcmy * szk;
+    o[4] = e - f;
+    o[5] = g + h;
+    o[6] = e + f;
+    o[7] = g - h;
+}
 __device__ __forceinline__ void yd_kr(const double *tab, double q, int kx, int ky, int kz, double *o) {
     yd_kr_s(tab, YD_KP, q, kx, ky, kz, o);
 }

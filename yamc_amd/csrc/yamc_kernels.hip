@@ -1794,7 +1794,7 @@ __device__ void yl_sweep_coul(const YdChain &c, const YkMol &m, double (&acc)[YK
         double kn[NA][8];
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
-            if (m.q[a] != 0.0) yd_kr(tabs + a * 6 * YD_KP, m.q[a], kx, ky, kz, kn[a]);
+            if (m.q[a] != 0.0) yd_kr_nz(tabs + a * 6 * YD_KP, m.q[a], kx, ky, kz, kn[a]);
         }
         if (DELTA) {
             double ns[8], os[8], ko[8];
@@ -1803,7 +1803,7 @@ __device__ void yl_sweep_coul(const YdChain &c, const YkMol &m, double (&acc)[YK
 #pragma unroll
             for (int a = 0; a < NA; ++a) {
                 if (m.q[a] == 0.0) continue;
-                yd_kr(m.tab_o + a * 6 * YD_KP, m.q[a], kx, ky, kz, ko);
+                yd_kr_nz(m.tab_o + a * 6 * YD_KP, m.q[a], kx, ky, kz, ko);
 #pragma unroll
                 for (int i = 0; i < 8; ++i) { ns[i] = ns[i] + kn[a][i]; os[i] = os[i] + ko[i]; }
             }
@@ -1848,12 +1848,12 @@ __device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_N
             if (m.typ[a] <= 0) continue;
             double qa = m.q[a];
             if (qa != 0.0) {
-                yd_kr(tabs + a * 6 * YD_KP, qa, kx, ky, kz, kl[a]);
+                yd_kr_nz(tabs + a * 6 * YD_KP, qa, kx, ky, kz, kl[a]);
                 double inv = 1.0 / qa;
 #pragma unroll
                 for (int i = 0; i < 8; ++i) kl[a][i] = kl[a][i] * inv;
             } else {
-                yd_kr(tabs + a * 6 * YD_KP, 1.0, kx, ky, kz, kl[a]);
+                yd_kr_nz(tabs + a * 6 * YD_KP, 1.0, kx, ky, kz, kl[a]);
             }
         }
         double dl[YK_MAXP][8];
@@ -1868,12 +1868,12 @@ __device__ void yl_sweep_lj(const YdChain &c, const YkMol &m, double (&acc)[YK_N
                 if (m.typ[a] <= 0) continue;
                 double qa = m.q[a], lo[8];
                 if (qa != 0.0) {
-                    yd_kr(m.tab_o + a * 6 * YD_KP, qa, kx, ky, kz, lo);
+                    yd_kr_nz(m.tab_o + a * 6 * YD_KP, qa, kx, ky, kz, lo);
                     double inv = 1.0 / qa;
 #pragma unroll
                     for (int i = 0; i < 8; ++i) lo[i] = lo[i] * inv;
                 } else {
-                    yd_kr(m.tab_o + a * 6 * YD_KP, 1.0, kx, ky, kz, lo);
+                    yd_kr_nz(m.tab_o + a * 6 * YD_KP, 1.0, kx, ky, kz, lo);
                 }
 #pragma unroll
                 for (int q = 0; q < YK_MAXP; ++q)
@@ -1987,7 +1987,7 @@ __device__ void yl_rs_exact(const YdChain &c, const YkMol &m, const double *sig6
             nb[2] = nb[2] - yij * pr.FR;
             nb[3] = nb[3] - zij * pr.FR;
         }
-#pragma unroll 1
+#pragma unroll
         for (int a = 0; a < NA; ++a) {
             int it = m.typ[a];
             double xij = yd_wrap(m.xo[a] - xj), yij = yd_wrap(m.yo[a] - yj), zij = yd_wrap(m.zo[a] - zj);
