@@ -497,36 +497,45 @@ __device__ void yk_mol_tables(const YdChain &c, YkMol &m, bool newpos, int t) {
 // Delta of the Coulomb sums at k: (sum of new) - (sum of old) over the
 // molecule's charged atoms in index order (RhoSquared_addAtom into nulified
 // sums, then RhoSquared_sub: MC.f90:946-958, 1096-1125).
+// The molecule's dS at one k (new - old, atoms in index order), with the
+// slots of ky = 0 or kz = 0 zero as SinCosKR_fill leaves them (set once
+// here: the per-atom terms of those slots are summed unzeroed and dropped).
+__device__ __forceinline__ void yk_zero_slots(double *d, int ky, int kz) {
+    if (kz == 0) { d[2] = 0.0; d[3] = 0.0; d[6] = 0.0; d[7] = 0.0; }
+    if (ky == 0) { d[4] = 0.0; d[5] = 0.0; d[6] = 0.0; d[7] = 0.0; }
+}
 __device__ __forceinline__ void yk_delta_coul(const YkMol &m, int kx, int ky, int kz, double *dc) {
     double ns[8] = {0, 0, 0, 0, 0, 0, 0, 0}, os[8] = {0, 0, 0, 0, 0, 0, 0, 0}, kr[8];
     for (int a = 0; a < m.na; ++a) {
         double q = m.q[a];
         if (q == 0.0) continue;
-        yd_kr(m.tab_n + a * 6 * YD_KP, q, kx, ky, kz, kr);
+        yd_kr_nz(m.tab_n + a * 6 * YD_KP, q, kx, ky, kz, kr);
 #pragma unroll
         for (int i = 0; i < 8; ++i) ns[i] = ns[i] + kr[i];
-        yd_kr(m.tab_o + a * 6 * YD_KP, q, kx, ky, kz, kr);
+        yd_kr_nz(m.tab_o + a * 6 * YD_KP, q, kx, ky, kz, kr);
 #pragma unroll
         for (int i = 0; i < 8; ++i) os[i] = os[i] + kr[i];
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) dc[i] = ns[i] - os[i];
+    yk_zero_slots(dc, ky, kz);
 }
-
-// Delta of the LJ sums of present slot p at k.
 __device__ __forceinline__ void yk_delta_lj(const YkMol &m, int p, int kx, int ky, int kz, double *dl) {
-    double ns[8] = {0, 0, 0, 0, 0, 0, 0, 0}, os[8] = {0, 0, 0, 0, 0, 0, 0, 0}, krc[8], krl[8];
+    double ns[8] = {0, 0, 0, 0, 0, 0, 0, 0}, os[8] = {0, 0, 0, 0, 0, 0, 0, 0}, kr[8];
     for (int a = 0; a < m.na; ++a) {
         if (m.lslot[a] != p) continue;
-        yd_kr_atom(m.tab_n + a * 6 * YD_KP, m.q[a], m.typ[a], kx, ky, kz, krc, krl);
+        // yd_kr_atom's LJ values: (q sin/cos) (1/q) of a charged site, else sin/cos
+        const double q = m.q[a], inv = q != 0.0 ? 1.0 / q : 1.0, qk = q != 0.0 ? q : 1.0;
+        yd_kr_nz(m.tab_n + a * 6 * YD_KP, qk, kx, ky, kz, kr);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) ns[i] = ns[i] + krl[i];
-        yd_kr_atom(m.tab_o + a * 6 * YD_KP, m.q[a], m.typ[a], kx, ky, kz, krc, krl);
+        for (int i = 0; i < 8; ++i) ns[i] = ns[i] + (q != 0.0 ? kr[i] * inv : kr[i]);
+        yd_kr_nz(m.tab_o + a * 6 * YD_KP, qk, kx, ky, kz, kr);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) os[i] = os[i] + krl[i];
+        for (int i = 0; i < 8; ++i) os[i] = os[i] + (q != 0.0 ? kr[i] * inv : kr[i]);
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) dl[i] = ns[i] - os[i];
+    yk_zero_slots(dl, ky, kz);
 }
 
 // Loads the 8 components of set at k.  FP32C reads the hi word of the LJ
