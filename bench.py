@@ -118,6 +118,8 @@ def main():
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--block", type=int, default=0, help="batch launch shape (64, 128, 256, 512); 0 = autotune")
+    ap.add_argument("--cell-div", type=int, default=-1,
+                    help="batch cell division of the real-space scan (0 = no cell list, 1..4); -1 = autotune")
     ap.add_argument("--cpu-moves", type=int, default=0, help="moves for the CPU baseline (0 = auto ~15 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32c,batch,batch_fp64")
@@ -263,7 +265,12 @@ def main():
             shape, tuned = args.block, None
         else:
             shape, tuned = tune_batch(reps)  # three ordinary sweeps, one per launch shape
-        div, ctuned = tune_cells_batch(reps)  # one ordinary sweep per cell size
+        if args.cell_div >= 0:
+            for r in reps:
+                r.set_cell_div(args.cell_div)
+            div, ctuned = args.cell_div, {}
+        else:
+            div, ctuned = tune_cells_batch(reps)  # one ordinary sweep per cell size
         # one captured sweep per step; every replica's S(k) is rebuilt from
         # scratch in FP64 every RESYNC moves inside the sweep (batched)
         bat = Batch(reps, 1, resync_every=RESYNC)
