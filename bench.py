@@ -39,13 +39,26 @@ RESYNC = 1000  # moves between from-scratch FP64 rebuilds of S(k) in the FP32C f
 
 
 def cpu_model() -> str:
+    """The host CPU as /proc/cpuinfo names it, with its family/model/stepping
+    numbers, the logical CPUs this process may use and the clock it reports."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (t.strip() for t in line.split(":", 1))
+                info.setdefault(k, v)
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        ncpu = os.cpu_count() or 0
+    extra = [f"family {info.get('cpu family', '?')} model {info.get('model', '?')} stepping {info.get('stepping', '?')}",
+             f"{ncpu} logical CPUs available"]
+    if "cpu MHz" in info:
+        extra.append(f"{float(info['cpu MHz']):.0f} MHz")
+    return f"{name} ({', '.join(extra)})"
 
 
 def batch_error(reps, base, kT_kjmol):
