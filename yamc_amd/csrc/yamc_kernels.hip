@@ -438,6 +438,7 @@ struct YkScalar {
     // the state's carried energy sums (workgroup chain kernel), so that a
     // commit does not wait for their loads
     double e_kc, e_klj, uu_ew, uu_lj6, uu_lj12;
+    double qold_pre;  // the rotation's Q_old, computed by wave 2 during the proposal
 };
 
 struct YkShared {
@@ -1625,7 +1626,16 @@ __device__ void yk_displacement_draws(const YdParams &p, YkMcm &m, YdRng &rng) {
 
 // MonteCarloMove.f90:269-444 chooseRotation (WAVE: called by a whole wave
 // with identical arguments in every lane).
-template <bool WAVE>
+// MonteCarloMove.f90's Q_old of the torque-biased rotation: shi_sans_exp(
+// vmax) / vmax, vmax = lambda_c |C_old| d_angle (1 when vmax <= 1e-2), as
+// yk_rotation computes it; by a whole wave.
+__device__ __forceinline__ double yk_qold(const YdParams &p, const YkMcm &m) {
+    const double vmax = p.lambda_c * m.couplold * p.d_angle;
+    return vmax > 1e-2 ? yd_shi_sans_exp_wave(vmax) / vmax : 1.0;
+}
+// QPRE: qold is computed elsewhere (another wave of the chain evaluates
+// yk_qold at the same time) and left unset here.
+template <bool WAVE, bool QPRE = false>
 __device__ void yk_rotation(const YdParams &p, YkMcm &m, YdRng &rng) {
     double cx = m.co[0], cy = m.co[1], cz = m.co[2], cabs = m.couplold;
     double vmax = p.lambda_c * cabs * p.d_angle, qold, angle, vx, vy, vz;
@@ -1648,7 +1658,7 @@ __device__ void yk_rotation(const YdParams &p, YkMcm &m, YdRng &rng) {
             xksi = rng.next();
         }
         double cosx = v / vmax;
-        qold = (WAVE ? yd_shi_sans_exp_wave(vmax) : yd_shi_sans_exp(vmax)) / vmax;
+        qold = QPRE ? 0.0 : (WAVE ? yd_shi_sans_exp_wave(vmax) : yd_shi_sans_exp(vmax)) / vmax;
         xksi = 1.0;
         for (int g = 0; xksi > (1.0 - 1e-10); ++g) {
             if (g >= YK_TRIES || m.stall) { m.stall = 1; break; }
@@ -1694,7 +1704,8 @@ __device__ void yk_rotation(const YdParams &p, YkMcm &m, YdRng &rng) {
     m.cphi = 1.0; m.sphi = 0.0;
     if (m.st > 0.0) { m.cphi = vx / m.st; m.sphi = vy / m.st; }
     m.v[0] = vx; m.v[1] = vy; m.v[2] = vz;
-    m.vmaxold = vmax; m.angle = angle; m.qold = qold;
+    m.vmaxold = vmax; m.angle = angle;
+    if (!QPRE) m.qold = qold;
 }
 
 // MonteCarloMove.f90:56-106 init_new
@@ -2647,7 +2658,10 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
             yk_displacement(p, t0.mcm, t0.rng);
         } else if (tid < 128) {
             yk_displacement_draws(p, t0.mcm2, t0.rng2);
-            yk_rotation<true>(p, t0.mcm2, t0.rng2);
+            yk_rotation<true, true>(p, t0.mcm2, t0.rng2);
+        } else if (tid < 192) {  // Q_old of the rotation, off the rotation wave's path
+            const double qo = yk_qold(p, t0.mcm);
+            if (tid == 128) t0.qold_pre = qo;
         }
         __syncthreads();
         if (tid < 64) {
@@ -2655,7 +2669,7 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
             const YkMcm &b = t0.mcm2;
             a.ca = b.ca; a.sa = b.sa; a.ct = b.ct; a.st = b.st; a.cphi = b.cphi; a.sphi = b.sphi;
             a.v[0] = b.v[0]; a.v[1] = b.v[1]; a.v[2] = b.v[2];
-            a.vmaxold = b.vmaxold; a.angle = b.angle; a.qold = b.qold;
+            a.vmaxold = b.vmaxold; a.angle = b.angle; a.qold = t0.qold_pre;
             a.stall = b.stall;  // the replay stalls exactly where the displacement does
             t0.rng = t0.rng2;
             int stall = yk_sec_propose_post(sh.m, t0);
@@ -3089,7 +3103,10 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
             yk_displacement(p, t0.mcm, t0.rng);
         } else if (tid < 128) {
             yk_displacement_draws(p, t0.mcm2, t0.rng2);
-            yk_rotation<true>(p, t0.mcm2, t0.rng2);
+            yk_rotation<true, true>(p, t0.mcm2, t0.rng2);
+        } else if (tid < 192) {  // Q_old of the rotation, off the rotation wave's path
+            const double qo = yk_qold(p, t0.mcm);
+            if (tid == 128) t0.qold_pre = qo;
         }
         __syncthreads();
         if (tid < 64) {
@@ -3097,7 +3114,7 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
             const YkMcm &b = t0.mcm2;
             a.ca = b.ca; a.sa = b.sa; a.ct = b.ct; a.st = b.st; a.cphi = b.cphi; a.sphi = b.sphi;
             a.v[0] = b.v[0]; a.v[1] = b.v[1]; a.v[2] = b.v[2];
-            a.vmaxold = b.vmaxold; a.angle = b.angle; a.qold = b.qold;
+            a.vmaxold = b.vmaxold; a.angle = b.angle; a.qold = t0.qold_pre;
             a.stall = b.stall;
             t0.rng = t0.rng2;
             int stall = yk_sec_propose_post(sh.m, t0);
