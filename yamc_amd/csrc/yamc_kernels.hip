@@ -439,6 +439,8 @@ struct YkScalar {
     // commit does not wait for their loads
     double e_kc, e_klj, uu_ew, uu_lj6, uu_lj12;
     double qold_pre;  // the rotation's Q_old, computed by wave 2 during the proposal
+    double qnew_pre;  // the acceptance's Q_new, computed by wave 2 during the acceptance
+    double du_total;  // the move's energy change, between the two parts of the acceptance
 };
 
 struct YkShared {
@@ -1398,10 +1400,16 @@ __device__ void yk_cells_build(const YdChain &c, int *ok) {
 // slot) and joins the new one.  *ok = 0 on overflow.
 __device__ void yk_cells_move(const YdChain &c, const YkMol &m, int *ok) {
     const int nc = c.nc, cap = c.ccap;
-    for (int a = 0; a < m.na; ++a) {
+    int oldc[YD_MAXA];  // the sites' cells, loaded together (a site's update
+                        // below writes no other site's cell)
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) oldc[a] = a < m.na ? yd_g(c.atom_cell)[m.first + a] : -1;
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) {
+        if (a >= m.na) break;
         const int g = m.first + a;
         const int id = (yd_cell1(m.xn[a], nc) * nc + yd_cell1(m.yn[a], nc)) * nc + yd_cell1(m.zn[a], nc);
-        const int old = yd_g(c.atom_cell)[g];
+        const int old = oldc[a];
         if (id == old) continue;
         const int s = yd_g(c.atom_slot)[g], last = yd_g(c.cell_cnt)[old] - 1;
         const int moved = yd_g(c.cell_atoms)[(size_t)old * cap + last];
@@ -1709,7 +1717,8 @@ __device__ void yk_rotation(const YdParams &p, YkMcm &m, YdRng &rng) {
 }
 
 // MonteCarloMove.f90:56-106 init_new
-template <bool WAVE>
+// QPRE: qnew is computed elsewhere (yk_qnew) and left unset here.
+template <bool WAVE, bool QPRE = false>
 __device__ void yk_init_new(const YdParams &p, YkMcm &m) {
     m.ftotnew = sqrt(m.fn[0] * m.fn[0] + m.fn[1] * m.fn[1] + m.fn[2] * m.fn[2]);
     m.couplnew = sqrt(m.cn[0] * m.cn[0] + m.cn[1] * m.cn[1] + m.cn[2] * m.cn[2]);
@@ -1722,10 +1731,14 @@ __device__ void yk_init_new(const YdParams &p, YkMcm &m) {
         umaxnew = 0.0; cnew = 1.0;
     }
     m.umaxnew = umaxnew; m.cnew = cnew;
-    double vmaxnew = p.lambda_c * m.couplnew * p.d_angle, qnew;
-    if (vmaxnew > 1e-2) qnew = (WAVE ? yd_shi_sans_exp_wave(vmaxnew) : yd_shi_sans_exp(vmaxnew)) / vmaxnew;
-    else { vmaxnew = 0.0; qnew = 1.0; }
-    m.vmaxnew = vmaxnew; m.qnew = qnew;
+    double vmaxnew = p.lambda_c * m.couplnew * p.d_angle, qnew = 1.0;
+    if (vmaxnew > 1e-2) {
+        if (!QPRE) qnew = (WAVE ? yd_shi_sans_exp_wave(vmaxnew) : yd_shi_sans_exp(vmaxnew)) / vmaxnew;
+    } else {
+        vmaxnew = 0.0;
+    }
+    m.vmaxnew = vmaxnew;
+    if (!QPRE) m.qnew = qnew;
 }
 
 // MonteCarloMove.f90:447-570 accept_or_decline
@@ -2226,15 +2239,12 @@ __device__ int yk_sec_propose(const YdChain &c, YkMol &m, YkScalar &t0, const do
 // Acceptance (MC.f90:1176-1233): new forces and torque, dU_ext, the
 // acceptance factor and the draw.  out holds the new k-space sweep's sums.
 // Returns 1 if accepted; the move record goes to yd_g(c.rec)[mv] when present.
-template <bool WAVE>
-__device__ int yk_sec_accept(const YdChain &c, const YkMol &m, YkScalar &t0, const double *out, int mv) {
-    const YdParams &p = c.p;
-    YkMcm &mcm = t0.mcm;
-    int na = m.na;
-    double U_new = 0.0;
-#pragma unroll
-    for (int a = 0; a < YD_MAXA; ++a) if (a < na) U_new = U_new + t0.uun[a];
-    double du_real = U_new - t0.U_old, duc = out[0], dulj = out[1];
+// The molecule's force and torque at its new position (MC.f90:1176-1212):
+// the real-space per-site forces, the k-space ones (4 pi out), the external
+// field's, and the torque about the new centre of mass.
+__device__ __forceinline__ void yk_new_ft(const YdParams &p, const YkMol &m, const YkScalar &t0, const double *out,
+                                          double (&Fn)[3], double (&Tn)[3], double (&mn)[3], double &du_ext) {
+    const int na = m.na;
     double fx[YD_MAXA], fy[YD_MAXA], fz[YD_MAXA];
 #pragma unroll
     for (int a = 0; a < YD_MAXA; ++a) if (a < na) {
@@ -2250,9 +2260,8 @@ __device__ int yk_sec_accept(const YdChain &c, const YkMol &m, YkScalar &t0, con
         mun[2] = mun[2] + m.q[a] * m.zn[a];
     }
     double dmu[3] = {mun[0] - t0.muo[0], mun[1] - t0.muo[1], mun[2] - t0.muo[2]};
-    double du_ext = p.kext * ((2 * t0.mu[0] + dmu[0]) * dmu[0] + (2 * t0.mu[1] + dmu[1]) * dmu[1] +
-                              (2 * t0.mu[2] + dmu[2]) * dmu[2]);
-    double mn[3];
+    du_ext = p.kext * ((2 * t0.mu[0] + dmu[0]) * dmu[0] + (2 * t0.mu[1] + dmu[1]) * dmu[1] +
+                       (2 * t0.mu[2] + dmu[2]) * dmu[2]);
     for (int d = 0; d < 3; ++d) mn[d] = t0.mu[d] + dmu[d];
 #pragma unroll
     for (int a = 0; a < YD_MAXA; ++a) if (a < na) {
@@ -2260,7 +2269,9 @@ __device__ int yk_sec_accept(const YdChain &c, const YkMol &m, YkScalar &t0, con
         fy[a] = fy[a] + p.m2kext * m.q[a] * mn[1];
         fz[a] = fz[a] + p.m2kext * m.q[a] * mn[2];
     }
-    double Fn[3] = {0, 0, 0}, Tn[3] = {0, 0, 0}, com[3] = {0, 0, 0}, tm = 0.0, tq[3];
+    double com[3] = {0, 0, 0}, tm = 0.0, tq[3];
+    Fn[0] = Fn[1] = Fn[2] = 0.0;
+    Tn[0] = Tn[1] = Tn[2] = 0.0;
 #pragma unroll
     for (int a = 0; a < YD_MAXA; ++a) if (a < na) { Fn[0] = Fn[0] + fx[a]; Fn[1] = Fn[1] + fy[a]; Fn[2] = Fn[2] + fz[a]; }
 #pragma unroll
@@ -2276,24 +2287,61 @@ __device__ int yk_sec_accept(const YdChain &c, const YkMol &m, YkScalar &t0, con
         yk_prod(m.xn[a] - com[0], m.yn[a] - com[1], m.zn[a] - com[2], fx[a], fy[a], fz[a], tq);
         Tn[0] = Tn[0] + tq[0]; Tn[1] = Tn[1] + tq[1]; Tn[2] = Tn[2] + tq[2];
     }
+}
+
+// MonteCarloMove.f90's Q_new of the acceptance factor from the new torque,
+// as yk_init_new computes it; by a whole wave.
+__device__ __forceinline__ double yk_qnew(const YdParams &p, const double (&Tn)[3]) {
+    const double couplnew = sqrt(Tn[0] * Tn[0] + Tn[1] * Tn[1] + Tn[2] * Tn[2]);
+    const double vmax = p.lambda_c * couplnew * p.d_angle;
+    return vmax > 1e-2 ? yd_shi_sans_exp_wave(vmax) / vmax : 1.0;
+}
+
+// MC.f90:1176-1233 up to the acceptance factor's inputs: the energy
+// change (returned) and the new-side quantities of the force bias in
+// t0.mcm.  QPRE: Q_new is computed elsewhere (another wave evaluates
+// yk_qnew at the same time) and left unset.
+template <bool WAVE, bool QPRE = false>
+__device__ double yk_sec_accept_pre(const YdChain &c, const YkMol &m, YkScalar &t0, const double *out) {
+    const YdParams &p = c.p;
+    YkMcm &mcm = t0.mcm;
+    double U_new = 0.0;
+#pragma unroll
+    for (int a = 0; a < YD_MAXA; ++a) if (a < m.na) U_new = U_new + t0.uun[a];
+    double du_real = U_new - t0.U_old, duc = out[0], dulj = out[1];
+    double Fn[3], Tn[3], mn[3], du_ext;
+    yk_new_ft(p, m, t0, out, Fn, Tn, mn, du_ext);
     for (int d = 0; d < 3; ++d) { mcm.fn[d] = Fn[d]; mcm.cn[d] = Tn[d]; }
     double du_total = du_real + (duc + dulj) + du_ext;
-    yk_init_new<WAVE>(p, mcm);
-    int accd = yk_accept(p, mcm, du_total, t0.rng);
+    yk_init_new<WAVE, QPRE>(p, mcm);
     t0.du_real = du_real; t0.duc = duc; t0.dulj = dulj; t0.du_ext = du_ext;
     for (int d = 0; d < 3; ++d) t0.mu_new[d] = mn[d];
+    return du_total;
+}
+
+// The acceptance test (one draw) and the move's record.
+__device__ int yk_sec_accept_post(const YdChain &c, const YkMol &m, YkScalar &t0, double du_total, int mv) {
+    const YkMcm &mcm = t0.mcm;
+    int accd = yk_accept(c.p, mcm, du_total, t0.rng);
     if (c.rec && (threadIdx.x & 63) == 0) {
         yamc_move_rec r = {};
         r.imol = m.imol; r.accepted = accd; r.overlap = 0; r.imol2 = -1;
-        r.du_real = du_real; r.du_kc = duc; r.du_klj = dulj; r.du_ext = du_ext; r.du_total = du_total;
+        r.du_real = t0.du_real; r.du_kc = t0.duc; r.du_klj = t0.dulj; r.du_ext = t0.du_ext; r.du_total = du_total;
         for (int d = 0; d < 3; ++d) {
-            r.f_old[d] = mcm.fo[d]; r.t_old[d] = mcm.co[d]; r.f_new[d] = Fn[d]; r.t_new[d] = Tn[d];
+            r.f_old[d] = mcm.fo[d]; r.t_old[d] = mcm.co[d]; r.f_new[d] = mcm.fn[d]; r.t_new[d] = mcm.cn[d];
             r.delta[d] = mcm.d[d];
         }
         r.angle = mcm.angle;
         yd_g(c.rec)[mv] = r;
     }
     return accd;
+}
+
+// MC.f90:1176-1233 on one wave.
+template <bool WAVE>
+__device__ int yk_sec_accept(const YdChain &c, const YkMol &m, YkScalar &t0, const double *out, int mv) {
+    const double du_total = yk_sec_accept_pre<WAVE>(c, m, t0, out);
+    return yk_sec_accept_post(c, m, t0, du_total, mv);
 }
 
 // Record of a move rejected before the acceptance test: overlap = 1 (hard
@@ -2767,22 +2815,41 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         // takes the next move's pick: the acceptance takes exactly one draw,
         // so the pick's draws start one draw after t0.rng2 (the state the
         // proposal left, which wave 0 does not write here) -------------------
+        // (wave 2 meanwhile computes Q_new from the new torque, off wave 0's
+        // path; wave 0 takes it after the barrier for the acceptance test)
         if (tid < 64) {
             __builtin_amdgcn_s_setprio(3);
-            int accd = yk_sec_accept<true>(c, sh.m, t0, sh.out, mv);
+            const double du = yk_sec_accept_pre<true, true>(c, sh.m, t0, sh.out);
+            if (tid == 0) t0.du_total = du;
+            __builtin_amdgcn_s_setprio(0);
+        } else if (tid < 128) {
+            if (mode == 0 && mv + 1 < total) {
+                // in LDS, the same values from every lane of the wave
+                t0.rng_next = t0.rng2;
+                (void)t0.rng_next.next();
+                const int nimol = yk_pick_imol(c, t0.rng_next, mode, mv + 1);
+                t0.nimol = nimol;
+                t0.prepicked = 1;
+            }
+        } else if (tid < 192) {
+            __builtin_amdgcn_s_setprio(3);
+            double Fn[3], Tn[3], mn[3], du_ext;
+            yk_new_ft(p, sh.m, t0, sh.out, Fn, Tn, mn, du_ext);
+            const double qn = yk_qnew(p, Tn);
+            if (tid == 128) t0.qnew_pre = qn;
+            __builtin_amdgcn_s_setprio(0);
+        }
+        __syncthreads();
+        if (tid < 64) {
+            __builtin_amdgcn_s_setprio(3);
+            t0.mcm.qnew = t0.qnew_pre;
+            int accd = yk_sec_accept_post(c, sh.m, t0, t0.du_total, mv);
             if (tid == 0) {
                 sh.accepted = accd;
                 ++t0.nmv;
                 t0.acc_count += accd;
             }
             __builtin_amdgcn_s_setprio(0);
-        } else if (tid < 128 && mode == 0 && mv + 1 < total) {
-            // in LDS, the same values from every lane of the wave
-            t0.rng_next = t0.rng2;
-            (void)t0.rng_next.next();
-            const int nimol = yk_pick_imol(c, t0.rng_next, mode, mv + 1);
-            t0.nimol = nimol;
-            t0.prepicked = 1;
         }
         __syncthreads();
         YK_TICK(6);
@@ -2861,12 +2928,9 @@ YK_CHAIN_INST(true, 512)
 #define YG_PART 64                 // doubles per workgroup per phase in the partials buffer
 #define YG_SPIN (1u << 22)         // bound of a barrier wait, in polls (seconds)
 
-#define YG_RTMP 2048  // LDS staging of the partial sums for the reduction
-
 struct YgShared {
     YkShared s;
     double out2[YK_NRX + 1 + YK_NACC];              // the reduced real-space and new-sweep sums
-    double rtmp[YG_RTMP];                           // partial sums of a chunk of workgroups
     double lx[YD_MAXA], ly[YD_MAXA], lz[YD_MAXA];  // the last finalized molecule's positions
     int last_imol;                                  // -1: none in the previous move
     int gen, sync_ok;
@@ -2902,38 +2966,42 @@ __device__ bool yg_sync(unsigned int *bar, unsigned int G, int &gen, int *ok_lds
     return *ok_lds != 0;
 }
 
-// Sums of the workgroups' partial vectors part[g * YG_PART + i], i < n, in
-// workgroup order into out[i].  The partials of a chunk of workgroups are
-// loaded by all threads at once into LDS (one memory round trip per chunk),
-// then threads i < n add them in workgroup order.
-__device__ void yg_reduce(const double *part, int G, int n, double *tmp, double *out) {
-    const int tid = threadIdx.x, bs = blockDim.x, C = YG_RTMP / n;
-    double a = 0.0;
-    for (int g0 = 0; g0 < G; g0 += C) {
-        const int gc = G - g0 < C ? G - g0 : C;
-        for (int base = 0; base < gc * n; base += 8 * bs) {  // eight loads in flight per thread
-            double v[8];
+// Sums of the workgroups' partial vectors part[g * YG_PART + i], i < n,
+// over g < G into out[i], by the first nw waves of the workgroup: wave w
+// takes the entries i = w, w + nw, ...  For each chunk of 64 workgroups lane
+// l holds workgroup g0 + l's value (all loads of the chunk are issued before
+// the first sum), the lanes are summed by an xor butterfly (every lane ends
+// with the same sum, in the same order on every run) and the chunks in
+// order.  n <= PER nw; the caller synchronizes before out[] is read.
+template <int PER>
+__device__ void yg_reduce(const double *part, int G, int n, int nw, double *out) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (w >= nw) return;
+    for (int g0 = 0; g0 < G; g0 += 64) {
+        const int g = g0 + lane;
+        double v[PER];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int idx = base + u * bs + tid, gg = idx / n, i = idx - gg * n;
-                v[u] = idx < gc * n ? __hip_atomic_load(part + (size_t)(g0 + gg) * YG_PART + i, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT)
-                                    : 0.0;
-            }
+        for (int u = 0; u < PER; ++u) {
+            const int i = w + u * nw;
+            v[u] = i < n && g < G ? __hip_atomic_load(part + (size_t)g * YG_PART + i, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                  : 0.0;
+        }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int idx = base + u * bs + tid;
-                if (idx < gc * n) tmp[idx] = v[u];
+        for (int u = 0; u < PER; ++u) {
+            const int i = w + u * nw;
+            if (i < n) {  // wave-uniform
+                double t = v[u];
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+                if (lane == 0) out[i] = g0 == 0 ? t : out[i] + t;
             }
         }
-        __syncthreads();
-        if (tid < n)
-            for (int gg = 0; gg < gc; ++gg) a += tmp[gg * n + tid];
-        __syncthreads();
     }
-    if (tid < n) out[tid] = a;
-    __syncthreads();
 }
+#define YG_N1 YK_NACC                    // entries of the first partials (old sweep)
+#define YG_N2 (YK_NRX + 1 + YK_NACC)     // of the second (real space, overlap flag, new sweep)
+static_assert(YG_N1 <= 5 * 3 && YG_N2 <= 10 * 4 && YG_N2 < YG_PART, "partials fit the reductions");
 
 // Full FP64 pass over atoms [a0, a1) when the workgroup's candidate list
 // overflowed: the new sites' partial sums and overlap test and the old
@@ -3004,6 +3072,7 @@ template <bool F32, int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
 yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nsteps, int mode, int max_moves,
           int lds_atoms, unsigned int *bar, double *part) {
+    static_assert(BS == 256, "the grid chain's reductions and proposal take four waves");
     __shared__ YgShared gs;
     extern __shared__ unsigned int lds_pos[];
     YkShared &sh = gs.s;
@@ -3038,7 +3107,9 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
     if (tid == 0) {
         for (int i = 0; i < 9; ++i) sh.ph[i] = 0;
         sh.t_last = wall_clock64();
-        t0.acc_count = 0; t0.nmv = 0; t0.nov = 0;
+        t0.acc_count = 0; t0.nmv = 0; t0.nov = 0; t0.prepicked = 0;
+        t0.e_kc = yd_g(c.st)->e_kc; t0.e_klj = yd_g(c.st)->e_klj;
+        t0.uu_ew = yd_g(c.st)->uu_ew; t0.uu_lj6 = yd_g(c.st)->uu_lj6; t0.uu_lj12 = yd_g(c.st)->uu_lj12;
         YdRng &rng = t0.rng;
         rng.key0 = yd_g(c.st)->key[0]; rng.key1 = yd_g(c.st)->key[1];
         rng.s0 = yd_g(c.st)->stream[0]; rng.s1 = yd_g(c.st)->stream[1];
@@ -3052,7 +3123,13 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
     __syncthreads();
     for (int mv = 0; ok && mv < total; ++mv) {
         if (tid == 0) {
-            yk_sec_pick(c, sh.m, t0.rng, mode, mv);
+            if (t0.prepicked) {  // taken during the last move's acceptance
+                yk_sec_pick_as(c, sh.m, t0.nimol);
+                t0.rng = t0.rng_next;
+                t0.prepicked = 0;
+            } else {
+                yk_sec_pick(c, sh.m, t0.rng, mode, mv);
+            }
             sh.ov = 0;
             sh.nbcount = 0;
         }
@@ -3084,11 +3161,14 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
         if (!(ok = yg_sync(bar, G, gs.gen, &gs.sync_ok))) break;
         if (c.nc > 0 && !lead && tid == 0)
             sh.cell_ok = (int)__hip_atomic_load(part1 + YG_PART - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        yg_reduce(part1, G, YK_NACC, gs.rtmp, sh.out);
-        if (tid < sh.m.na) {  // the per-site real-space arrays (written before B1 by their owner)
-            const double4 u = yd_g(reinterpret_cast<const double4 *>(c.uf))[sh.m.first + tid];
-            sh.uu_old[tid] = u.x;
-            sh.fr_old[tid][0] = u.y; sh.fr_old[tid][1] = u.z; sh.fr_old[tid][2] = u.w;
+        // waves 0-2 reduce; wave 3 meanwhile loads the per-site real-space
+        // arrays (written before B1 by their owner)
+        yg_reduce<5>(part1, G, YG_N1, 3, sh.out);
+        if (tid >= 192 && tid - 192 < sh.m.na) {
+            const int a = tid - 192;
+            const double4 u = yd_g(reinterpret_cast<const double4 *>(c.uf))[sh.m.first + a];
+            sh.uu_old[a] = u.x;
+            sh.fr_old[a][0] = u.y; sh.fr_old[a][1] = u.z; sh.fr_old[a][2] = u.w;
         }
         __syncthreads();
         YK_TICK(2);
@@ -3178,7 +3258,8 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
         }
         YK_TICK(5);
         if (!(ok = yg_sync(bar, G, gs.gen, &gs.sync_ok))) break;
-        yg_reduce(part2, G, YK_NRX + 1 + YK_NACC, gs.rtmp, gs.out2);
+        yg_reduce<10>(part2, G, YG_N2, 4, gs.out2);
+        __syncthreads();
         if (tid == 0) {
             const double *o = gs.out2;
             sh.ov = o[YK_NRX] != 0.0;
@@ -3202,9 +3283,32 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
             __syncthreads();
             continue;
         }
-        // ---- acceptance (every workgroup, the same decision) ------------
+        // ---- acceptance (every workgroup, the same decision); wave 1
+        // meanwhile takes the next move's pick, whose draws start one draw
+        // after t0.rng2 (the acceptance takes exactly one), and wave 2
+        // computes Q_new, which wave 0 takes after the barrier for the
+        // acceptance test ---------------------------------------------------
         if (tid < 64) {
-            int accd = yk_sec_accept<true>(c, sh.m, t0, gs.out2 + YK_NRX + 1, mv);
+            const double du = yk_sec_accept_pre<true, true>(c, sh.m, t0, gs.out2 + YK_NRX + 1);
+            if (tid == 0) t0.du_total = du;
+        } else if (tid < 128) {
+            if (mode == 0 && mv + 1 < total) {
+                t0.rng_next = t0.rng2;
+                (void)t0.rng_next.next();
+                const int nimol = yk_pick_imol(c, t0.rng_next, mode, mv + 1);
+                t0.nimol = nimol;
+                t0.prepicked = 1;
+            }
+        } else if (tid < 192) {
+            double Fn[3], Tn[3], mn[3], du_ext;
+            yk_new_ft(p, sh.m, t0, gs.out2 + YK_NRX + 1, Fn, Tn, mn, du_ext);
+            const double qn = yk_qnew(p, Tn);
+            if (tid == 128) t0.qnew_pre = qn;
+        }
+        __syncthreads();
+        if (tid < 64) {
+            t0.mcm.qnew = t0.qnew_pre;
+            int accd = yk_sec_accept_post(c, sh.m, t0, t0.du_total, mv);
             if (tid == 0) {
                 sh.accepted = accd;
                 ++t0.nmv;
@@ -3228,7 +3332,7 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
         YK_TICK(7);
         if (tid == 0) {
             if (lead) {
-                yk_sec_finalize(c, sh.m, t0, nullptr, true, use_lds ? nullptr : spos);
+                yk_sec_finalize(c, sh.m, t0, nullptr, true, use_lds ? nullptr : spos, true);
                 if (c.nc > 0 && sh.cell_ok) {
                     yk_cells_move(c, sh.m, &sh.cell_ok);
                     part1[YG_PART - 1] = sh.cell_ok;
