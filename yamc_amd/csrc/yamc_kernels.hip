@@ -1066,9 +1066,13 @@ __device__ __forceinline__ bool yk_near(const float (&xof)[NA], const float (&yo
 // YK_NBCAP of them; when the cells hold more than G YK_NBCAP atoms it
 // returns false (the same in every workgroup) and the caller scans atom
 // ranges instead.
-template <int NA, bool SPLIT = false>
+// LIST (grid chain): instead of marking bm, the candidates go straight to
+// list[0 .. *nlist) in the order of the visited atoms, with wsum
+// (YK_SCANU ints per wave) as the placement's LDS.
+template <int NA, bool SPLIT = false, bool LIST = false>
 __device__ __attribute__((noinline)) bool yk_rs_scan_cells(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int *cl_cell,
-                                 int *cl_start, int *ncl_out, int split_g = 0, int split_G = 1) {
+                                 int *cl_start, int *ncl_out, int split_g = 0, int split_G = 1,
+                                 int *list = nullptr, int *nlist = nullptr, int *wsum = nullptr) {
     const int nc = c.nc, tid = threadIdx.x;
     const unsigned int f0 = (unsigned int)m.first, na = (unsigned int)m.na;
     const double w = 1.0 / (double)nc;
@@ -1171,6 +1175,63 @@ __device__ __attribute__((noinline)) bool yk_rs_scan_cells(const YdChain &c, con
         xnf[a] = (float)m.xn[a]; ynf[a] = (float)m.yn[a]; znf[a] = (float)m.zn[a];
     }
     const int bs = (int)blockDim.x;
+    if (LIST) {
+        // thread tid tests items base + u bs + tid (one load round trip for
+        // all of them); wave ballots count each u's finds and a block pass
+        // over the counts (in wsum: YK_SCANU ints per wave) places them in
+        // ascending item order
+        const int lane = tid & 63, w = tid >> 6, nw = bs >> 6;
+        const unsigned long long below = (1ull << lane) - 1ull;
+        int run = 0;  // candidates listed so far (the same in every thread)
+        for (int base = tb; base < total; base += YK_SCANU * bs) {  // uniform trip count
+            int jj[YK_SCANU];
+#pragma unroll
+            for (int u = 0; u < YK_SCANU; ++u) {
+                const int t = base + u * bs + tid;
+                jj[u] = -1;
+                if (t < total) {
+                    int lo = 0, hi = ncl - 1;
+                    while (lo < hi) {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (cl_start[mid] <= t) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    jj[u] = yd_g(c.cell_atoms)[(size_t)cl_cell[lo] * c.ccap + (t - cl_start[lo])];
+                }
+            }
+            unsigned int pj[YK_SCANU];
+#pragma unroll
+            for (int u = 0; u < YK_SCANU; ++u) pj[u] = jj[u] >= 0 ? pos[jj[u]] : 0u;
+            int pre[YK_SCANU];
+#pragma unroll
+            for (int u = 0; u < YK_SCANU; ++u) {
+                const int j = jj[u];
+                const bool take = j >= 0 && (unsigned int)j - f0 >= na &&
+                                  yk_near<NA>(xof, yof, zof, xnf, ynf, znf, yk_unpack_pos(pj[u]), r2f);
+                if (!take) jj[u] = -1;
+                const unsigned long long bal = __ballot(take);
+                pre[u] = __popcll(bal & below);
+                if (lane == 0) wsum[w * YK_SCANU + u] = __popcll(bal);
+            }
+            __syncthreads();
+            int o = run;
+#pragma unroll
+            for (int u = 0; u < YK_SCANU; ++u) {
+                int all = 0, bw = 0;
+                for (int v = 0; v < nw; ++v) {
+                    const int x = wsum[v * YK_SCANU + u];
+                    all += x;
+                    bw += v < w ? x : 0;
+                }
+                if (jj[u] >= 0) list[o + bw + pre[u]] = jj[u];
+                o += all;
+            }
+            run = o;
+            __syncthreads();  // wsum is reused
+        }
+        if (tid == 0) *nlist = run;
+        return true;
+    }
     for (int t0 = tb + tid; t0 < total; t0 += YK_SCANU * bs) {
         int jj[YK_SCANU];  // atom indices first, then their positions, YK_SCANU loads at a time
 #pragma unroll
@@ -1203,10 +1264,11 @@ __device__ __attribute__((noinline)) bool yk_rs_scan_cells(const YdChain &c, con
 
 // pos: packed positions (yk_pack_pos); the molecule's own atoms, indices
 // first .. first + na - 1, are skipped.
+// RANGE: only atoms [a0, a1), marked in bm relative to word wb.
 template <int NA, bool RANGE = false>
 __device__ __forceinline__ void yk_rs_scan(const YdChain &c, const YkMol &m, const unsigned int *pos, unsigned int *bm, int a0 = 0,
-                           int a1 = -1) {
-    if (!RANGE) a0 = 0;
+                           int a1 = -1, int wb = 0) {
+    if (!RANGE) { a0 = 0; wb = 0; }
     const int N = RANGE ? a1 : c.p.natom, me = m.imol + 1;
     if (pos) {
         const float r2f = yk_scan_r2(c);
@@ -1232,7 +1294,7 @@ __device__ __forceinline__ void yk_rs_scan(const YdChain &c, const YkMol &m, con
                 const int j = j0 + u * bs;
                 if (j < N && (unsigned int)j - f0 >= na &&
                     yk_near<NA>(xof, yof, zof, xnf, ynf, znf, yk_unpack_pos(pj[u]), r2f))
-                    atomicOr(&bm[j >> 5], 1u << (j & 31));
+                    atomicOr(&bm[(j >> 5) - wb], 1u << (j & 31));
             }
         }
     } else {
@@ -1247,7 +1309,7 @@ __device__ __forceinline__ void yk_rs_scan(const YdChain &c, const YkMol &m, con
                 double xn = yd_wrap(m.xn[a] - xj), yn = yd_wrap(m.yn[a] - yj), zn = yd_wrap(m.zn[a] - zj);
                 near |= xn * xn + yn * yn + zn * zn < c.p.rmax2;
             }
-            if (near && jm != me) atomicOr(&bm[j >> 5], 1u << (j & 31));
+            if (near && jm != me) atomicOr(&bm[(j >> 5) - wb], 1u << (j & 31));
         }
     }
 }
@@ -1357,12 +1419,12 @@ __device__ void yk_rs_commit_list(const YdChain &c, const int *list, int count, 
 
 template <bool RANGE = false>
 __device__ __forceinline__ void yk_rs_scan_n(const YdChain &c, const YkMol &m, const unsigned int *pos,
-                                             unsigned int *bm, int a0 = 0, int a1 = -1) {
+                                             unsigned int *bm, int a0 = 0, int a1 = -1, int wb = 0) {
     switch (m.na) {
-        case 1: yk_rs_scan<1, RANGE>(c, m, pos, bm, a0, a1); break;
-        case 2: yk_rs_scan<2, RANGE>(c, m, pos, bm, a0, a1); break;
-        case 3: yk_rs_scan<3, RANGE>(c, m, pos, bm, a0, a1); break;
-        default: yk_rs_scan<4, RANGE>(c, m, pos, bm, a0, a1); break;
+        case 1: yk_rs_scan<1, RANGE>(c, m, pos, bm, a0, a1, wb); break;
+        case 2: yk_rs_scan<2, RANGE>(c, m, pos, bm, a0, a1, wb); break;
+        case 3: yk_rs_scan<3, RANGE>(c, m, pos, bm, a0, a1, wb); break;
+        default: yk_rs_scan<4, RANGE>(c, m, pos, bm, a0, a1, wb); break;
     }
 }
 template <bool SPLIT = false>
@@ -1374,6 +1436,18 @@ __device__ __forceinline__ bool yk_rs_scan_cells_n(const YdChain &c, const YkMol
         case 2: return yk_rs_scan_cells<2, SPLIT>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
         case 3: return yk_rs_scan_cells<3, SPLIT>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
         default: return yk_rs_scan_cells<4, SPLIT>(c, m, pos, bm, cl_cell, cl_start, ncl, split_g, split_G);
+    }
+}
+
+static_assert(YK_SCANU * YK_NW <= YK_BMWORDS, "the list scan's counts fit in the bitmap's LDS");
+__device__ __forceinline__ bool yk_rs_scan_cells_list_n(const YdChain &c, const YkMol &m, const unsigned int *pos,
+                                                        int *cl_cell, int *cl_start, int *ncl, int split_g, int split_G,
+                                                        int *list, int *nlist, int *wsum) {
+    switch (__builtin_amdgcn_readfirstlane(m.na)) {
+        case 1: return yk_rs_scan_cells<1, true, true>(c, m, pos, nullptr, cl_cell, cl_start, ncl, split_g, split_G, list, nlist, wsum);
+        case 2: return yk_rs_scan_cells<2, true, true>(c, m, pos, nullptr, cl_cell, cl_start, ncl, split_g, split_G, list, nlist, wsum);
+        case 3: return yk_rs_scan_cells<3, true, true>(c, m, pos, nullptr, cl_cell, cl_start, ncl, split_g, split_G, list, nlist, wsum);
+        default: return yk_rs_scan_cells<4, true, true>(c, m, pos, nullptr, cl_cell, cl_start, ncl, split_g, split_G, list, nlist, wsum);
     }
 }
 
@@ -1428,15 +1502,14 @@ __device__ void yk_cells_move(const YdChain &c, const YkMol &m, int *ok) {
 // Ascending compaction of the candidate bitmap into list[] (deterministic:
 // thread t owns a contiguous run of words, offsets from a block prefix sum).
 // *count gets the number of candidates (may exceed YK_NBCAP: list partial).
-// wb: the first word of the range compacted (a grid chain's workgroup
-// compacts the words of its own atoms).
+// bm[k] holds the bits of atoms 32 (wb + k) .. 32 (wb + k) + 31, k < nwords
+// (a grid chain's workgroup compacts the words of its own atom range).
 __device__ void yk_compact(const unsigned int *bm, int nwords, int *list, int *count, int *wsum, int wb = 0) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int bs = blockDim.x, per = (nwords + bs - 1) / bs, w0 = wb + tid * per;
-    nwords += wb;
+    const int bs = blockDim.x, per = (nwords + bs - 1) / bs, k0 = tid * per;
     int cnt = 0;
     for (int k = 0; k < per; ++k)
-        if (w0 + k < nwords) cnt += __popc(bm[w0 + k]);
+        if (k0 + k < nwords) cnt += __popc(bm[k0 + k]);
     int incl = cnt;  // inclusive scan inside the wave
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -1452,12 +1525,12 @@ __device__ void yk_compact(const unsigned int *bm, int nwords, int *list, int *c
     }
     int pos = base + incl - cnt;
     for (int k = 0; k < per; ++k) {
-        if (w0 + k >= nwords) break;
-        unsigned int b = bm[w0 + k];
+        if (k0 + k >= nwords) break;
+        unsigned int b = bm[k0 + k];
         while (b) {
             int bit = __ffs(b) - 1;
             b &= b - 1;
-            if (pos < YK_NBCAP) list[pos] = ((w0 + k) << 5) + bit;
+            if (pos < YK_NBCAP) list[pos] = ((wb + k0 + k) << 5) + bit;
             ++pos;
         }
     }
@@ -3222,18 +3295,24 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
             // the visited cells' atoms (at most YK_NBCAP, so the list cannot
             // overflow); else the atoms of its own range.  Either way each
             // atom is one workgroup's candidate at most.
-            const int nwords = (p.natom + 31) >> 5;
-            if (nwords <= YK_BMWORDS) {
-                for (int i = tid; i < nwords; i += BS) sh.bm[i] = 0u;
-                __syncthreads();
-                bool scanned = false;
-                if (c.nc > 0 && sh.cell_ok)
-                    scanned = yk_rs_scan_cells_n<true>(c, sh.m, spos, sh.bm, sh.cl_cell, sh.cl_start, &sh.ncl, g, G);
-                if (!scanned) yk_rs_scan_n<true>(c, sh.m, spos, sh.bm, a0, a1);
-                __syncthreads();
-                yk_compact(sh.bm, nwords, sh.nblist, &sh.nbcount, sh.wsum);
-            } else if (tid == 0) {
-                sh.nbcount = YK_NBCAP + 1;
+            // With cells the scan writes the list itself, in the order of
+            // the visited atoms; over the atom range it marks the bitmap's
+            // words of the range (only they are cleared and compacted).
+            bool listed = false;
+            if (c.nc > 0 && sh.cell_ok)
+                listed = yk_rs_scan_cells_list_n(c, sh.m, spos, sh.cl_cell, sh.cl_start, &sh.ncl, g, G, sh.nblist,
+                                                 &sh.nbcount, reinterpret_cast<int *>(sh.bm));
+            if (!listed) {
+                const int w0 = a0 >> 5, w1 = (a1 + 31) >> 5;
+                if (w1 - w0 <= YK_BMWORDS) {
+                    for (int i = tid; i < w1 - w0; i += BS) sh.bm[i] = 0u;
+                    __syncthreads();
+                    yk_rs_scan_n<true>(c, sh.m, spos, sh.bm, a0, a1, w0);
+                    __syncthreads();
+                    yk_compact(sh.bm, w1 - w0, sh.nblist, &sh.nbcount, sh.wsum, w0);
+                } else if (tid == 0) {
+                    sh.nbcount = YK_NBCAP + 1;
+                }
             }
             __syncthreads();
             if (sh.nbcount <= YK_NBCAP)
