@@ -755,9 +755,22 @@ def large_check(args):
            "cycles": args.large_check, "moves": args.large_check * s.nmol}
     recs = {}
     final_pos = None
+    # the cell size of the real-space scan, tuned on a scratch chain (one
+    # cycle per size) so that both forms start from the same state
+    from yamc_amd.tune import tune_cells
+    t = EwaldMC(s, precision=YAMC_FP64)
+    t.set_chain_groups(args.large_groups)
+    t.mc_init()
+    t.set_rng(20261209, 0)
+    cell_div, by_div = tune_cells(t)
+    t.close()
+    out["cell_div"] = cell_div
+    out["tuner_moves_per_s_by_cell_div"] = {str(k): round(v, 1) for k, v in by_div.items()}
+    print(f"large-check: cell div {cell_div} ({out['tuner_moves_per_s_by_cell_div']})", file=sys.stderr, flush=True)
     for form, prec, rs in (("fp64", YAMC_FP64, 0), ("fp32c", YAMC_FP32C, 1000)):
         g = EwaldMC(s, precision=prec)
         g.set_chain_groups(args.large_groups)
+        g.set_cell_div(cell_div)
         g.mc_init()
         g.set_rng(20261207, 0)
         t0 = time.perf_counter()
