@@ -279,6 +279,12 @@ int yamc_mc_commit_move(yamc_ctx *ctx, int32_t imol, const double *x, const doub
  * resync_every moves with a from-scratch FP64 rebuild of its S(k) between
  * them (all replicas' rebuilds in one batched pass), exactly the segments
  * and rebuilds of yamc_mc_run(ctx, ncycles, resync_every) alone; 0 = none.
+ * Members set to G > 1 workgroups per chain (yamc_set_chain_groups) make a
+ * batch of grid chains: all members' workgroups (one chain over G of them
+ * each, G = 1 allowed for the others) in one cooperative launch, which
+ * must fit on the device at once (else YAMC_ERROR_LIMITS); such a batch
+ * takes resync_every = 0 only (else YAMC_ERROR_PARAMETER), and a member
+ * with G > 1 advances exactly as yamc_mc_run of that context alone.
  * n_accepted[n] (may be NULL) receives each replica's accepted moves. */
 int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int32_t resync_every, int64_t *n_accepted);
 /* The same sweep captured once in a HIP graph: yamc_batch_create stages the
@@ -286,8 +292,9 @@ int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int32_t re
  * rebuilds included), yamc_batch_launch replays it (each replay continues
  * every replica's chain and RNG stream, and first re-reads every member's
  * current box parameters; a member whose state is not ready is refused),
- * yamc_batch_destroy frees the graph.  The contexts must outlive the batch;
- * no reference counterpart (the reference runs one system serially). */
+ * yamc_batch_destroy frees the graph (a batch of grid chains is staged the
+ * same way and launched without a graph).  The contexts must outlive the
+ * batch; no reference counterpart (the reference runs one system serially). */
 typedef struct yamc_batch yamc_batch;
 int yamc_batch_create(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int32_t resync_every, yamc_batch **out);
 int yamc_batch_launch(yamc_batch *batch, int64_t *n_accepted);
@@ -425,8 +432,9 @@ int yamc_set_block_size(yamc_ctx *ctx, int32_t block);
  * and the real-space pass by atom range, with two grid barriers per move
  * (the loops of EwaldSumKSpace.f90:528-753 and EwaldSumRealSpace.f90:
  * 367-496 split across CUs).  Same decisions and results as G = 1 up to the
- * summation order of the partial sums.  Batches always run one workgroup
- * per chain.  No reference counterpart (the reference is serial). */
+ * summation order of the partial sums.  In a batch, the member's chain
+ * takes its G workgroups too (yamc_batch_run).  No reference counterpart
+ * (the reference is serial). */
 int yamc_set_chain_groups(yamc_ctx *ctx, int32_t groups);
 /* Cell list of the chain kernels' real-space scan: cubic cells of width at
  * least rc / div box units (div = 1, 2, 3 or 4; nc = floor(div / rc) cells

@@ -35,9 +35,11 @@ template <bool F32, int BS>
 __global__ void yk_chain(const YdChain *chains, const double *const *sig6s, const double *const *feps_s, int nsteps,
                          int mode, int max_moves, int lds_atoms, int resync_every);
 template <bool F32, int BS>
-__global__ void yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nsteps, int mode,
-                          int max_moves, int lds_atoms, unsigned int *bar, double *part);
-__global__ void yk_gchain_prep(const YdChain *src, YdChain *dst, int G, double *rsb, unsigned int *bar);
+__global__ void yk_gchain(const YdChain *chains, const YgGroup *groups, int nsteps, int mode, int max_moves,
+                          int lds_atoms);
+__global__ void yk_gchain_prep(const YdChain *src, YdChain *dst, YgGroup *grp, int G, double *rsb, unsigned int *bar,
+                               double *part, const double *sig6, const double *feps);
+__global__ void yk_gchain_bprep(const YdChain *src, YdChain *dst, const YgGroup *grp, int nwg, double *rsb);
 #define YG_PART 64  // yk_gchain: doubles per workgroup per phase of the partials buffer
 template <bool F32, int OCC>
 __global__ void yk_wchain(const YdChain *chains, int nchains, const double *const *sig6s, const double *const *feps_s,
@@ -113,6 +115,7 @@ struct yamc_ctx {
     // buffers (per-workgroup descriptors, back terms, partial sums, barrier)
     int groups = 1, gcap = 0;
     YdChain *d_gch = nullptr;
+    YgGroup *d_ggrp = nullptr;
     double *d_grsb = nullptr, *d_gpart = nullptr;
     unsigned int *d_gbar = nullptr;
     // host copies of the box-dependent inputs, scaled to the current box
@@ -950,14 +953,13 @@ static int launch_gchain(yamc_ctx *c, const YdChain *d_chain, int nsteps, int mo
     if (per_cu < 1 || G > per_cu * ncu)
         return fail(YAMC_ERROR_LIMITS, "%d workgroups per chain do not fit on the device at once (%d per CU, %d CUs)",
                     G, per_cu, ncu);
-    yk_gchain_prep<<<(G + 63) / 64, 64, 0, c->stream>>>(d_chain, c->d_gch, G, c->d_grsb, c->d_gbar);
+    yk_gchain_prep<<<(G + 63) / 64, 64, 0, c->stream>>>(d_chain, c->d_gch, c->d_ggrp, G, c->d_grsb, c->d_gbar,
+                                                         c->d_gpart, c->d_sig6, c->d_4eps);
     HIPCHK(hipGetLastError());
     const YdChain *chains = c->d_gch;
-    const double *sig6 = c->d_sig6, *feps = c->d_4eps;
-    unsigned int *bar = c->d_gbar;
-    double *part = c->d_gpart;
-    void *args[] = {(void *)&chains, (void *)&sig6, (void *)&feps, (void *)&nsteps, (void *)&mode,
-                    (void *)&max_moves, (void *)&lds_atoms, (void *)&bar, (void *)&part};
+    const YgGroup *grp = c->d_ggrp;
+    void *args[] = {(void *)&chains, (void *)&grp, (void *)&nsteps, (void *)&mode, (void *)&max_moves,
+                    (void *)&lds_atoms};
     HIPCHK(hipLaunchCooperativeKernel(fn, dim3(G), dim3(256), args, (unsigned int)dyn, c->stream));
     return YAMC_OK;
 }
@@ -1020,6 +1022,7 @@ extern "C" int yamc_set_chain_groups(yamc_ctx *c, int32_t groups) {
                     c->natom);
     if (groups > c->gcap) {
         TRY(dalloc(c, &c->d_gch, (size_t)groups));
+        TRY(dalloc(c, &c->d_ggrp, (size_t)groups));
         TRY(dalloc(c, &c->d_grsb, (size_t)groups * YD_NBCAP * 4));
         TRY(dalloc(c, &c->d_gpart, (size_t)2 * groups * YG_PART));
         TRY(dalloc(c, &c->d_gbar, 2));
@@ -1153,6 +1156,17 @@ struct yamc_batch {
     long long *d_cnt = nullptr;  // accepted-move counters before and after a launch (2 x n)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // grid chains (a member set to more than one workgroup per chain): the
+    // nwg workgroups of all members in one cooperative launch, with their
+    // descriptors and group records, back terms, partial sums, barrier
+    // words and the members' stop flags; launched without a graph
+    bool grid = false;
+    int nwg = 0;
+    YdChain *d_gch = nullptr;
+    YgGroup *d_grp = nullptr;
+    double *d_grsb = nullptr, *d_gpart = nullptr;
+    unsigned int *d_gbar = nullptr;
+    int *d_fail = nullptr;
 };
 
 // Checks the members and writes their current descriptors (box parameters,
@@ -1201,6 +1215,40 @@ static int batch_stage(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int32_
     HIPCHK(hipMemcpy(b->d_ch, b->h_ch.data(), sizeof(YdChain) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_s6, s6.data(), sizeof(void *) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->d_fe, fe.data(), sizeof(void *) * n, hipMemcpyHostToDevice));
+    for (int i = 0; i < n; ++i) b->grid = b->grid || ctxs[i]->groups > 1;
+    if (b->grid) {
+        if (resync_every > 0)
+            return fail(YAMC_ERROR_PARAMETER,
+                        "a batch with grid chains (a member with more than one workgroup per chain) runs without "
+                        "the in-launch rebuild: resync_every must be 0");
+        int nwg = 0;
+        size_t npart = 0;
+        for (int i = 0; i < n; ++i) {
+            nwg += ctxs[i]->groups;
+            npart += (size_t)2 * ctxs[i]->groups * YG_PART;
+        }
+        b->nwg = nwg;
+        HIPCHK(hipMalloc((void **)&b->d_gch, sizeof(YdChain) * nwg));
+        HIPCHK(hipMalloc((void **)&b->d_grp, sizeof(YgGroup) * nwg));
+        HIPCHK(hipMalloc((void **)&b->d_grsb, sizeof(double) * (size_t)nwg * YD_NBCAP * 4));
+        HIPCHK(hipMalloc((void **)&b->d_gpart, sizeof(double) * npart));
+        HIPCHK(hipMalloc((void **)&b->d_gbar, sizeof(unsigned int) * 2 * n));
+        HIPCHK(hipMalloc((void **)&b->d_fail, sizeof(int) * n));
+        std::vector<YgGroup> h(nwg);
+        size_t off = 0;
+        for (int i = 0, w = 0; i < n; ++i) {
+            const int G = ctxs[i]->groups;
+            for (int g = 0; g < G; ++g, ++w) {
+                YgGroup &r = h[w];
+                r.g = g; r.G = G; r.rep = i;
+                r.bar = b->d_gbar + 2 * i;
+                r.part = b->d_gpart + off;
+                r.sig6 = s6[i]; r.feps = fe[i];
+            }
+            off += (size_t)2 * G * YG_PART;
+        }
+        HIPCHK(hipMemcpy(b->d_grp, h.data(), sizeof(YgGroup) * nwg, hipMemcpyHostToDevice));
+    }
     return YAMC_OK;
 }
 
@@ -1211,25 +1259,88 @@ static void batch_free(yamc_batch *b) {
     if (b->d_s6) (void)hipFree(b->d_s6);
     if (b->d_fe) (void)hipFree(b->d_fe);
     if (b->d_cnt) (void)hipFree(b->d_cnt);
+    for (void *q : {(void *)b->d_gch, (void *)b->d_grp, (void *)b->d_grsb, (void *)b->d_gpart, (void *)b->d_gbar,
+                    (void *)b->d_fail})
+        if (q) (void)hipFree(q);
     delete b;
 }
 
+// One sweep of a batch of grid chains on stream s: the per-workgroup
+// descriptors from the members' current ones, then one cooperative launch
+// of all nwg workgroups (which must fit on the device at once).
+template <bool F32>
+static int launch_gbatch(yamc_batch *b, hipStream_t s) {
+    yamc_ctx *c0 = b->c0;
+    const void *fn = (const void *)yk_gchain<F32, 256>;
+    int lds_max = 0, ncu = 0, coop = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c0->device));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c0->device));
+    HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c0->device));
+    if (!coop) return fail(YAMC_ERROR_DEVICE, "the device has no cooperative launch: use one workgroup per chain");
+    hipFuncAttributes fa;
+    HIPCHK(hipFuncGetAttributes(&fa, fn));
+    const size_t want = (size_t)b->max_natom * 4;
+    const size_t room = b->nwg > ncu ? (size_t)lds_max / 2 : (size_t)lds_max;
+    int lds_atoms = 0;
+    size_t dyn = 0;
+    if (want + fa.sharedSizeBytes + 1024 <= room) {
+        lds_atoms = b->max_natom;
+        dyn = want;
+        HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    }
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, dyn));
+    if (per_cu < 1 || b->nwg > per_cu * ncu)
+        return fail(YAMC_ERROR_LIMITS, "%d workgroups of grid chains do not fit on the device at once (%d per CU, %d CUs)",
+                    b->nwg, per_cu, ncu);
+    yk_gchain_bprep<<<(b->nwg + 63) / 64, 64, 0, s>>>(b->d_ch, b->d_gch, b->d_grp, b->nwg, b->d_grsb);
+    HIPCHK(hipGetLastError());
+    const YdChain *chains = b->d_gch;
+    const YgGroup *grp = b->d_grp;
+    int nsteps = b->ncycles, mode = 0, max_moves = 0;
+    void *args[] = {(void *)&chains, (void *)&grp, (void *)&nsteps, (void *)&mode, (void *)&max_moves,
+                    (void *)&lds_atoms};
+    HIPCHK(hipLaunchCooperativeKernel(fn, dim3(b->nwg), dim3(256), args, (unsigned int)dyn, s));
+    return YAMC_OK;
+}
+
 // Enqueues one sweep of the batch on stream s: one kernel launch (so it can
-// be captured).
-static int batch_sweep(yamc_batch *b, hipStream_t) {
+// be captured), or for grid chains the descriptors' set-up and one
+// cooperative launch.
+static int batch_sweep(yamc_batch *b, hipStream_t s) {
+    if (b->grid)
+        return b->c0->precision == YAMC_FP32C ? launch_gbatch<true>(b, s) : launch_gbatch<false>(b, s);
     return launch_chain(b->c0, b->d_ch, b->d_s6, b->d_fe, b->n, b->ncycles, 0, b->max_natom, 0, b->resync_every);
 }
 
 // Copies every member's accepted-move counter into out[0..n) on the stream
 // (one small kernel instead of a device-to-host copy per member).
-__global__ void yk_batch_counts(const YdChain *chains, int n, long long *out) {
+// (fail: the members' grid-chain stop flags, or null)
+__global__ void yk_batch_counts(const YdChain *chains, int n, long long *out, int *fail) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = chains[i].st->n_accepted;
+    if (i < n) {
+        out[i] = chains[i].st->n_accepted;
+        if (fail) fail[i] = chains[i].st->grid_fail;
+    }
 }
 
 static int batch_counts_gather(const yamc_batch *b, int slot, hipStream_t s) {
-    yk_batch_counts<<<(b->n + 255) / 256, 256, 0, s>>>(b->d_ch, b->n, b->d_cnt + (size_t)slot * b->n);
+    yk_batch_counts<<<(b->n + 255) / 256, 256, 0, s>>>(b->d_ch, b->n, b->d_cnt + (size_t)slot * b->n,
+                                                       slot == 1 ? b->d_fail : nullptr);
     HIPCHK(hipGetLastError());
+    return YAMC_OK;
+}
+
+// A member of a grid batch whose barrier wait ran past its bound.
+static int batch_check_grid(const yamc_batch *b, hipStream_t s) {
+    if (!b->grid) return YAMC_OK;
+    yk_batch_counts<<<(b->n + 255) / 256, 256, 0, s>>>(b->d_ch, b->n, b->d_cnt + b->n, b->d_fail);
+    HIPCHK(hipGetLastError());
+    std::vector<int> h(b->n);
+    HIPCHK(hipMemcpyAsync(h.data(), b->d_fail, sizeof(int) * b->n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int i = 0; i < b->n; ++i)
+        if (h[i]) return fail(YAMC_ERROR_DEVICE, "batch member %d: grid chain stopped, a barrier wait ran past its bound", i);
     return YAMC_OK;
 }
 
@@ -1259,13 +1370,14 @@ static int batch_go(yamc_batch *b, int64_t *n_accepted, bool graph) {
     TRY(batch_refresh(b, s));
     if (n_accepted) TRY(batch_counts_gather(b, 0, s));
     HIPCHK(hipEventRecord(c0->ev0, s));
-    if (graph) HIPCHK(hipGraphLaunch(b->exec, s));
+    if (graph && b->exec) HIPCHK(hipGraphLaunch(b->exec, s));
     else TRY(batch_sweep(b, s));
     HIPCHK(hipEventRecord(c0->ev1, s));
     if (n_accepted) TRY(batch_counts_gather(b, 1, s));
     TRY(sync(c0));
     HIPCHK(hipEventElapsedTime(&c0->last_ms, c0->ev0, c0->ev1));
     if (n_accepted) TRY(batch_counts_diff(b, s, n_accepted));
+    TRY(batch_check_grid(b, s));
     return YAMC_OK;
 }
 
@@ -1285,7 +1397,7 @@ extern "C" int yamc_batch_create(yamc_ctx *const *ctxs, int32_t n, int32_t ncycl
     if (ncycles < 1) return fail(YAMC_ERROR_PARAMETER, "ncycles must be positive");
     yamc_batch *b = new yamc_batch;
     int rc = batch_stage(ctxs, n, ncycles, resync_every, b);
-    if (rc == YAMC_OK) {
+    if (rc == YAMC_OK && !b->grid) {  // (a batch of grid chains is launched directly)
         yamc_ctx *c0 = b->c0;
         auto go = [&]() -> int {
             // The capture records kernel nodes only (no memset or copy nodes).

@@ -106,6 +106,16 @@ struct YdChain {
     int *cell_cnt, *cell_atoms, *atom_cell, *atom_slot;
 };
 
+// One workgroup of a grid chain (one chain over G co-resident workgroups):
+// its index g among the G, the chain's index in a batch, the chain's barrier
+// words and partial-sum buffer, and the chain's LJ tables.
+struct YgGroup {
+    int g, G, rep;
+    unsigned int *bar;
+    double *part;
+    const double *sig6, *feps;
+};
+
 // Global-memory view of a device pointer.  The pointers of a chain
 // descriptor are read from memory, so the compiler only knows them as
 // generic pointers and emits flat loads and stores for them; a flat access

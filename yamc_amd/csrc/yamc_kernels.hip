@@ -2988,7 +2988,8 @@ YK_CHAIN_INST(true, 512)
 // cell buckets).  Two grid barriers per move:
 //   B1 after the old sweep: the per-workgroup sums of the old k-space forces;
 //   B2 after the real-space pass and the new sweep: their sums.
-// The partial sums of the workgroups are added in workgroup order.
+// The partial sums of the workgroups are added in a fixed order (a lane
+// butterfly over the workgroups).
 //
 // Ordering of the shared data: a workgroup's writes of move m (its S(k)
 // range, its atoms' back terms, workgroup 0's finalize) are released before
@@ -3130,27 +3131,49 @@ __device__ __forceinline__ void yg_rs_full_n(const YdChain &c, const YkMol &m, c
 // Per-workgroup chain descriptors of a grid chain from the context's one:
 // workgroup g gets its own back-term buffer, workgroup 0 alone the move
 // records; the barrier words start at zero.
-__global__ void yk_gchain_prep(const YdChain *src, YdChain *dst, int G, double *rsb, unsigned int *bar) {
+__global__ void yk_gchain_prep(const YdChain *src, YdChain *dst, YgGroup *grp, int G, double *rsb, unsigned int *bar,
+                               double *part, const double *sig6, const double *feps) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < G) {
         YdChain ch = *src;
         ch.rsb = rsb + (size_t)g * YD_NBCAP * 4;
         if (g > 0) ch.rec = nullptr;
         dst[g] = ch;
+        YgGroup r;
+        r.g = g; r.G = G; r.rep = 0; r.bar = bar; r.part = part; r.sig6 = sig6; r.feps = feps;
+        grp[g] = r;
     }
     if (g < 2) bar[g] = 0u;
 }
 
+// The same for a batch of grid chains: workgroup w runs workgroup grp[w].g
+// of chain grp[w].rep (src holds the chains' current descriptors); the first
+// workgroup of each chain clears its barrier words.
+__global__ void yk_gchain_bprep(const YdChain *src, YdChain *dst, const YgGroup *grp, int nwg, double *rsb) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nwg) {
+        const YgGroup r = grp[w];
+        YdChain ch = src[r.rep];
+        ch.rsb = rsb + (size_t)w * YD_NBCAP * 4;
+        if (r.g > 0) ch.rec = nullptr;
+        dst[w] = ch;
+        if (r.g == 0) { r.bar[0] = 0u; r.bar[1] = 0u; }
+    }
+}
+
 template <bool F32, int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
-yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nsteps, int mode, int max_moves,
-          int lds_atoms, unsigned int *bar, double *part) {
+yk_gchain(const YdChain *chains, const YgGroup *groups, int nsteps, int mode, int max_moves, int lds_atoms) {
     static_assert(BS == 256, "the grid chain's reductions and proposal take four waves");
     __shared__ YgShared gs;
     extern __shared__ unsigned int lds_pos[];
     YkShared &sh = gs.s;
-    const int G = (int)gridDim.x, g = (int)blockIdx.x, tid = threadIdx.x;
-    const YdChain &c = chains[g];
+    const YgGroup gr = groups[blockIdx.x];
+    const int G = gr.G, g = gr.g, tid = threadIdx.x;
+    unsigned int *const bar = gr.bar;
+    double *const part = gr.part;
+    const double *const sig6 = gr.sig6, *const feps = gr.feps;
+    const YdChain &c = chains[blockIdx.x];
     const YdParams &p = c.p;
     const bool lead = g == 0;
     int total = mode == 1 ? nsteps : nsteps * p.nmol;
@@ -3447,9 +3470,8 @@ yk_gchain(const YdChain *chains, const double *sig6, const double *feps, int nst
     }
 }
 
-#define YK_GCHAIN_INST(F, B)                                                                                   \
-    template __global__ void yk_gchain<F, B>(const YdChain *, const double *, const double *, int, int, int, int, \
-                                             unsigned int *, double *);
+#define YK_GCHAIN_INST(F, B) \
+    template __global__ void yk_gchain<F, B>(const YdChain *, const YgGroup *, int, int, int, int);
 YK_GCHAIN_INST(false, 256)
 YK_GCHAIN_INST(true, 256)
 
