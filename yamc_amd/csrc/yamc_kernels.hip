@@ -2131,6 +2131,19 @@ __device__ __forceinline__ void yl_rs_exact_n(const YdChain &c, const YkMol &m, 
 // yd_block_sum over the accumulators a move uses: entries with !used(i)
 // (uniform) are zero in every thread and are written as zero without the
 // shuffles; the used ones are summed in yd_block_sum's order.
+// Workgroup barrier that orders LDS accesses only: the waves' global stores
+// still in flight are not waited for.  Used where no other wave reads those
+// stores before the next full barrier (__syncthreads waits for every wave's
+// stores), so a phase's last stores overlap the next phase.
+__device__ __forceinline__ void yk_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// (its barriers are yk_lds_sync: the one-workgroup chain kernel calls it
+// after its sweeps, which store nothing, and after the exact real-space
+// pass, whose back terms are read back by the thread that stored them)
 template <int NV, typename Used>
 __device__ __forceinline__ void yk_block_sum_used(double (&v)[NV], double *red, double *out, Used used) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -2146,13 +2159,13 @@ __device__ __forceinline__ void yk_block_sum_used(double (&v)[NV], double *red, 
 #pragma unroll
         for (int i = 0; i < NV; ++i) red[w * NV + i] = used(i) ? v[i] : 0.0;
     }
-    __syncthreads();
+    yk_lds_sync();
     if (threadIdx.x < NV) {
         double a = 0.0;
         for (int j = 0; j < nw; ++j) a += red[j * NV + threadIdx.x];
         out[threadIdx.x] = a;
     }
-    __syncthreads();
+    yk_lds_sync();
 }
 
 // Phase timers of the chain kernel: thread 0 adds wall-clock ticks between
@@ -2709,6 +2722,7 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
         // the periodic FP64 rebuild between move resync_every k - 1 and
         // resync_every k (workgroup-uniform)
         if (resync_every > 0 && mv > 0 && mv % resync_every == 0) {
+            __syncthreads();  // the last commit's and finalize's stores
             // the chunk tables also take the LDS of the packed positions,
             // which are packed again afterwards (the positions are unchanged)
             yk_chain_rebuild<2>(c, sh, tid, use_lds ? reinterpret_cast<double *>(lds_pos) : nullptr,
@@ -2938,7 +2952,10 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
             if (tid == 0)
                 for (int i = 0; i < 3; ++i) t0.cmp_old[i] = sh.out[i];
         }
-        __syncthreads();
+        // The commit's and the finalize's global stores are not waited for
+        // here: the next move reads them only after its pick barrier (a full
+        // one), and a rebuild starts with one.
+        yk_lds_sync();
         YK_TICK(7);
         if (tid == 0) {
             __builtin_amdgcn_s_setprio(3);
@@ -2947,7 +2964,7 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
             for (int d = 0; d < 3; ++d) t0.mu[d] = t0.mu_new[d];
             __builtin_amdgcn_s_setprio(0);
         }
-        __syncthreads();
+        yk_lds_sync();
         YK_TICK(8);
     }
     // the launch copy of the per-atom real-space arrays back into the state
