@@ -314,9 +314,11 @@ class EwaldMC:
         check(self.lib.yamc_set_block_size(self.ctx, int(block)))
 
     def set_chain_groups(self, groups: int) -> None:
-        """Workgroups per chain of the single-chain runs (1: one workgroup;
-        G > 1: one chain spread over G co-resident workgroups)."""
+        """Workgroups per chain of the single-chain runs and of this chain in
+        a batch (1: one workgroup; G > 1: one chain spread over G co-resident
+        workgroups)."""
         check(self.lib.yamc_set_chain_groups(self.ctx, int(groups)))
+        self.groups = int(groups)
 
     def set_cell_div(self, div: int) -> None:
         """Cells per cut-off of the real-space scan's cell list (0: none,

@@ -13,7 +13,10 @@ A single chain can also spread over G workgroups of a cooperative launch
 wavefront) and of the atoms, with two grid barriers per move.  More
 workgroups shorten the sweeps and the real-space pass but add barrier and
 reduction time, so the best G grows with nk and the atom count;
-tune_groups() times one cycle per candidate G and keeps the fastest.
+tune_groups() times one cycle per candidate G and keeps the fastest.  In a
+batch the members can be grid chains too (one cooperative launch of all
+their workgroups); tune_groups_batch() times the batch with every member at
+natom // A workgroups for a few A and keeps the fastest.
 
 tune_batch() times one sweep of the batch itself with each shape and keeps
 the fastest.  The timed sweeps are ordinary Monte Carlo sweeps of the chains
@@ -99,6 +102,28 @@ def tune_groups(rep: EwaldMC, groups: Sequence[int] = None, cycles: int = 1) -> 
         rates[g] = rep.sys.nmol * cycles / (rep.last_kernel_ms() * 1e-3)
     best = max(rates, key=rates.get)
     rep.set_chain_groups(best)
+    return best, rates
+
+
+def tune_groups_batch(reps: Sequence[EwaldMC], atoms_per_group: Sequence[int] = (0, 600, 1000, 1500),
+                      gmax: int = 8, cycles: int = 1) -> Tuple[int, Dict[int, float]]:
+    """Workgroups per chain inside a batch (a batch of grid chains, one
+    cooperative launch): for each A in atoms_per_group, every replica takes
+    G = natom // A workgroups (1 to gmax; A = 0: one workgroup each, the
+    batch's launch shape), and `cycles` cycles of the whole batch are timed
+    (HIP-event kernel time, no rebuilds).  Sets and returns the fastest A
+    with {A: trial moves/s}."""
+    def apply(a: int) -> None:
+        for r in reps:
+            r.set_chain_groups(1 if a == 0 else max(1, min(gmax, r.sys.natom // a)))
+    moves = sum(r.sys.nmol for r in reps) * cycles
+    rates = {}
+    for a in atoms_per_group:
+        apply(a)
+        batch_run(reps, cycles)
+        rates[a] = moves / (reps[0].last_kernel_ms() * 1e-3)
+    best = max(rates, key=rates.get)
+    apply(best)
     return best, rates
 
 
