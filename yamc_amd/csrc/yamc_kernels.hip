@@ -3359,11 +3359,13 @@ yk_gchain(const YdChain *chains, const YgGroup *groups, int nsteps, int mode, in
                 yl_rs_exact_n<true>(c, sh.m, sig6, feps, sh.nblist, sh.nbcount, acc, ov, tid, BS);
             else
                 yg_rs_full_n(c, sh.m, sig6, feps, a0, a1, acc, ov);
-            yd_block_sum<YK_NRX>(acc, sh.red, sh.out);
+            // (LDS-only barriers: the back terms are read back by the thread
+            // that stored them; the partials are released at the barrier)
+            yk_block_sum_used(acc, sh.red, sh.out, [](int) { return true; });
             if (tid < YK_NRX) part2[(size_t)g * YG_PART + tid] = sh.out[tid];
             // overlap flag of the workgroup (any thread)
             if (ov) sh.ov = 1;
-            __syncthreads();
+            yk_lds_sync();
             if (tid == 0) part2[(size_t)g * YG_PART + YK_NRX] = (double)sh.ov;
         }
         YK_TICK(4);
@@ -3447,7 +3449,11 @@ yk_gchain(const YdChain *chains, const YgGroup *groups, int nsteps, int mode, in
             for (int j = a0 + tid; j < a1; j += BS)
                 if (yd_g(c.molnum)[j] != me) yk_rs_apply<true>(c, sh.m, sig6, feps, j, acc);
         }
-        __syncthreads();
+        // The commit's and the finalize's global stores are not waited for
+        // here: the next move reads them after its pick barrier (within the
+        // workgroup) or after the first grid barrier, which starts with a
+        // full workgroup barrier and the agent-scope release.
+        yk_lds_sync();
         YK_TICK(7);
         if (tid == 0) {
             if (lead) {
@@ -3464,7 +3470,7 @@ yk_gchain(const YdChain *chains, const YgGroup *groups, int nsteps, int mode, in
             gs.last_imol = sh.m.imol;
             for (int d = 0; d < 3; ++d) t0.mu[d] = t0.mu_new[d];
         }
-        __syncthreads();
+        yk_lds_sync();
         YK_TICK(8);
     }
     // the launch copy of the per-atom arrays back into the state, own atoms
