@@ -598,35 +598,44 @@ def replica_check(args):
         r.resync()
         fresh = r.energies()
         drift = max(drift, max(abs(inc[k] - fresh[k]) / max(1.0, abs(fresh[k])) for k in ("E_kC", "E_kLJ")))
-    # The same replicas in FP64 (no rebuild needed), with the workgroups per
-    # chain tuned over the batch: one 512-thread workgroup per replica, or
-    # grid chains of natom // A workgroups (1 to 8), all in one launch.
+    # The same replicas with the workgroups per chain tuned over the batch:
+    # one 512-thread workgroup per replica, or grid chains of natom // A
+    # workgroups (1 to 8), all in one launch; in FP64 (no rebuild needed)
+    # and in FP32C with the rebuild every RESYNC moves (inside the launch,
+    # each member's workgroups over their own k ranges).
     from yamc_amd.tune import tune_groups_batch
-    rng = np.random.default_rng(20261206)
-    reps64 = []
-    for i in range(64):
-        nw, conc, T = int(rng.integers(128, 2049)), float(rng.uniform(0.0, 1.0)), float(rng.uniform(280.0, 360.0))
-        s = replica_system(nw, conc, T, seed=1000 + i)
-        r = EwaldMC(s, precision=YAMC_FP64)
-        r.set_block_size(512)
-        r.mc_init()
-        r.set_rng(20261206, i)
-        reps64.append(r)
-    apg, apg_rates = tune_groups_batch(reps64)
-    b64 = Batch(reps64, 1)
-    b64.launch()
-    torch_sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        b64.launch()
-    torch_sync()
-    dt64 = time.perf_counter() - t0
-    b64.close()
-    fp64 = {"atoms_per_workgroup": apg, "workgroups": sum(getattr(r, "groups", 1) for r in reps64),
-            "tuner_moves_per_s_by_atoms_per_workgroup": {str(k): round(v, 1) for k, v in apg_rates.items()},
-            "moves_per_s": moves_per_sweep * args.steps / dt64, "ms_per_sweep": 1e3 * dt64 / args.steps}
-    for r in reps64:
-        r.close()
+
+    def grid_leg(precision, resync):
+        rng = np.random.default_rng(20261206)
+        rs = []
+        for i in range(64):
+            nw, conc, T = int(rng.integers(128, 2049)), float(rng.uniform(0.0, 1.0)), float(rng.uniform(280.0, 360.0))
+            s = replica_system(nw, conc, T, seed=1000 + i)
+            r = EwaldMC(s, precision=precision)
+            r.set_block_size(512)
+            r.mc_init()
+            r.set_rng(20261206, i)
+            rs.append(r)
+        apg, apg_rates = tune_groups_batch(rs)
+        bg = Batch(rs, 1, resync_every=resync)
+        bg.launch()
+        torch_sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            bg.launch()
+        torch_sync()
+        dtg = time.perf_counter() - t0
+        bg.close()
+        res = {"atoms_per_workgroup": apg, "workgroups": sum(getattr(r, "groups", 1) for r in rs),
+               "tuner_moves_per_s_by_atoms_per_workgroup": {str(k): round(v, 1) for k, v in apg_rates.items()},
+               "moves_per_s": moves_per_sweep * args.steps / dtg, "ms_per_sweep": 1e3 * dtg / args.steps,
+               "resync_every": resync}
+        for r in rs:
+            r.close()
+        return res
+
+    fp64 = grid_leg(YAMC_FP64, 0)
+    fp32c_grid = grid_leg(YAMC_FP32C, RESYNC)
     out = {
         "metric": "batched uneven replicas: trial moves/s, FP32C, one captured launch per sweep",
         "workload": "64 replicas, seeded N_w 128-2048, NaCl 0-1 M, T 280-360 K, rho 0.0334/A^3, rc 12 A",
@@ -639,6 +648,7 @@ def replica_check(args):
         "kernel_ms_per_sweep": kms / args.steps, "acceptance": nacc / (moves_per_sweep * args.steps),
         "kspace_drift_rel_after_run": drift,
         "fp64_batch": fp64,
+        "fp32c_grid_batch": fp32c_grid,
     }
     print(json.dumps(out))
 

@@ -282,10 +282,11 @@ int yamc_mc_commit_move(yamc_ctx *ctx, int32_t imol, const double *x, const doub
  * Members set to G > 1 workgroups per chain (yamc_set_chain_groups) make a
  * batch of grid chains: all members' workgroups (one chain over G of them
  * each, G = 1 allowed for the others) in one cooperative launch, which
- * must fit on the device at once (else YAMC_ERROR_LIMITS); such a batch
- * takes resync_every = 0 only (else YAMC_ERROR_PARAMETER), and a member
- * with G > 1 advances exactly as yamc_mc_run of that context alone (the
- * members' workgroup counts when the batch is staged fix its launch).
+ * must fit on the device at once (else YAMC_ERROR_LIMITS); there each
+ * member's workgroups rebuild its S(k) together inside the launch (each
+ * its own k range), and a member with G > 1 advances exactly as
+ * yamc_mc_run of that context alone (the members' workgroup counts when
+ * the batch is staged fix its launch).
  * n_accepted[n] (may be NULL) receives each replica's accepted moves. */
 int yamc_batch_run(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int32_t resync_every, int64_t *n_accepted);
 /* The same sweep captured once in a HIP graph: yamc_batch_create stages the

@@ -74,13 +74,23 @@ def test_grid_batch_member_on_one_workgroup(lib):
         assert a[i].rng_counter() == b[i].rng_counter(), i
 
 
-def test_grid_batch_refuses_rebuilds(lib):
-    """A batch with grid chains takes no in-launch rebuild: resync_every > 0
-    is a parameter error, before anything runs."""
-    from yamc_amd import YamcError
+@pytest.mark.parametrize("precision", [1, 0])
+def test_grid_batch_with_rebuilds_equals_grid_chains_alone(lib, precision):
+    """resync_every = 37 moves with members at 2, 3, 2, 4 and 2 workgroups
+    per chain: each member's workgroups rebuild its S(k) inside the launch,
+    each over its own k range; the batch equals every replica run alone as
+    a grid chain by yamc_mc_run with the same interval (segments of 37 moves
+    with the launch-level rebuild between them), bit for bit, over two
+    sweeps."""
     from yamc_amd.ewald import batch_run
-    a, _ = pairs(0, seed0=160)
-    a[1].set_chain_groups(2)
-    with pytest.raises(YamcError) as ei:
-        batch_run(a, 1, resync_every=50)
-    assert ei.value.code == 3
+    a, b = pairs(precision, seed0=160)
+    for G, x, y in zip((2, 3, 2, 4, 2), a, b):
+        x.set_chain_groups(G)
+        y.set_chain_groups(G)
+    print(f"grid batch with rebuilds, precision={precision}", flush=True)
+    for sweep in range(2):
+        nacc, _ = batch_run(a, 2, resync_every=37)
+        for i, y in enumerate(b):
+            n, _ = y.run(2, resync_every=37)
+            assert n == nacc[i], (sweep, i)
+            same_state(a[i], y, (sweep, i))

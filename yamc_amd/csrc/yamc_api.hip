@@ -36,7 +36,7 @@ __global__ void yk_chain(const YdChain *chains, const double *const *sig6s, cons
                          int mode, int max_moves, int lds_atoms, int resync_every);
 template <bool F32, int BS>
 __global__ void yk_gchain(const YdChain *chains, const YgGroup *groups, int nsteps, int mode, int max_moves,
-                          int lds_atoms);
+                          int lds_atoms, int resync_every);
 __global__ void yk_gchain_prep(const YdChain *src, YdChain *dst, YgGroup *grp, int G, double *rsb, unsigned int *bar,
                                double *part, const double *sig6, const double *feps);
 __global__ void yk_gchain_bprep(const YdChain *src, YdChain *dst, const YgGroup *grp, int nwg, double *rsb);
@@ -958,8 +958,9 @@ static int launch_gchain(yamc_ctx *c, const YdChain *d_chain, int nsteps, int mo
     HIPCHK(hipGetLastError());
     const YdChain *chains = c->d_gch;
     const YgGroup *grp = c->d_ggrp;
+    int resync_every = 0;  // (a single grid chain is rebuilt between launches, over the whole GPU)
     void *args[] = {(void *)&chains, (void *)&grp, (void *)&nsteps, (void *)&mode, (void *)&max_moves,
-                    (void *)&lds_atoms};
+                    (void *)&lds_atoms, (void *)&resync_every};
     HIPCHK(hipLaunchCooperativeKernel(fn, dim3(G), dim3(256), args, (unsigned int)dyn, c->stream));
     return YAMC_OK;
 }
@@ -1217,10 +1218,6 @@ static int batch_stage(yamc_ctx *const *ctxs, int32_t n, int32_t ncycles, int32_
     HIPCHK(hipMemcpy(b->d_fe, fe.data(), sizeof(void *) * n, hipMemcpyHostToDevice));
     for (int i = 0; i < n; ++i) b->grid = b->grid || ctxs[i]->groups > 1;
     if (b->grid) {
-        if (resync_every > 0)
-            return fail(YAMC_ERROR_PARAMETER,
-                        "a batch with grid chains (a member with more than one workgroup per chain) runs without "
-                        "the in-launch rebuild: resync_every must be 0");
         int nwg = 0;
         size_t npart = 0;
         for (int i = 0; i < n; ++i) {
@@ -1297,9 +1294,9 @@ static int launch_gbatch(yamc_batch *b, hipStream_t s) {
     HIPCHK(hipGetLastError());
     const YdChain *chains = b->d_gch;
     const YgGroup *grp = b->d_grp;
-    int nsteps = b->ncycles, mode = 0, max_moves = 0;
+    int nsteps = b->ncycles, mode = 0, max_moves = 0, resync_every = b->resync_every;
     void *args[] = {(void *)&chains, (void *)&grp, (void *)&nsteps, (void *)&mode, (void *)&max_moves,
-                    (void *)&lds_atoms};
+                    (void *)&lds_atoms, (void *)&resync_every};
     HIPCHK(hipLaunchCooperativeKernel(fn, dim3(b->nwg), dim3(256), args, (unsigned int)dyn, s));
     return YAMC_OK;
 }

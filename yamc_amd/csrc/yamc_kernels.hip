@@ -2516,14 +2516,14 @@ static_assert(sizeof(YkShared::red) >= sizeof(double) * 2 * (YK_KE_THREADS / 64)
 // sum added to the total in group order), the axis tables of each chunk of
 // atoms built in LDS once per pass (the scan bitmap's space, free between
 // moves) and read by every k of the pass.
-// E_kC, E_kLJ: yk_kenergy's per-thread k strides and block sum of
-// YK_KE_THREADS threads, each thread here standing in for
-// YK_KE_THREADS / blockDim of them.  Then mu and the cell buckets, as at
-// the start of a launch.
+// Here over the k vectors [kb, ke) (a grid chain's workgroup rebuilds its
+// own range; the per-k arithmetic does not depend on the range).
 template <int KT>
-__device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid, double *xtra, int xtra_bytes) {
+__device__ void yk_rebuild_sums(const YdChain &c, YkShared &sh, int tid, int kb, int ke, double *xtra,
+                                int xtra_bytes) {
     const int bs = (int)blockDim.x;
-    const int nk = c.p.nk, T = c.p.ntype, kmax = c.p.kmax, kp = kmax + 1;
+    const int T = c.p.ntype, kmax = c.p.kmax, kp = kmax + 1, nkr = c.p.nk;  // nkr: the row length of S
+    const int nk = ke;  // (the end of the range)
     // per atom of a chunk: its six axis tables and its 1/q factor
     const int stride = 6 * kp + 1;
     double *tb = reinterpret_cast<double *>(sh.bm);
@@ -2534,7 +2534,7 @@ __device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid, double
     for (int set = 0; set <= T; ++set) {
         const int l0 = yd_g(c.set_off)[set], l1 = yd_g(c.set_off)[set + 1], ng = c.fs_ng;
         const int gsize = (l1 - l0 + ng - 1) / ng;
-        for (int k0 = 0; k0 < nk; k0 += KT * bs) {
+        for (int k0 = kb; k0 < nk; k0 += KT * bs) {
             int kx[KT], ky[KT], kz[KT];
             double tot[KT][8], sg[KT][8];
 #pragma unroll
@@ -2625,7 +2625,7 @@ __device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid, double
                 if (ky[t] == 0) { tot[t][4] = 0.0; tot[t][5] = 0.0; tot[t][6] = 0.0; tot[t][7] = 0.0; }
 #pragma unroll
                 for (int m = 0; m < 8; ++m) {
-                    const size_t idx = ((size_t)set * 8 + m) * nk + kk;
+                    const size_t idx = ((size_t)set * 8 + m) * nkr + kk;
                     if (c.p.precision == YAMC_FP32C) {
                         const float hi = (float)tot[t][m];
                         yd_g(c.Shi)[idx] = hi;
@@ -2637,7 +2637,14 @@ __device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid, double
             }
         }
     }
-    __syncthreads();  // S(k) complete (workgroup-scope visibility of the stores)
+}
+
+// E_kC, E_kLJ from the rebuilt S(k): yk_kenergy's per-thread k strides and
+// block sum of YK_KE_THREADS threads, each thread here standing in for
+// YK_KE_THREADS / blockDim of them; into e[0..1] (LDS) and, with store, the
+// chain state.
+__device__ void yk_rebuild_energies(const YdChain &c, YkShared &sh, int tid, bool store, double *e) {
+    const int bs = (int)blockDim.x, nk = c.p.nk;
     {
         const int J = YK_KE_THREADS / bs, nw = bs >> 6, lane = tid & 63, w = tid >> 6;
         for (int j = 0; j < J; ++j) {
@@ -2655,11 +2662,23 @@ __device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid, double
         if (tid < 2) {
             double a = 0.0;
             for (int vw = 0; vw < YK_KE_THREADS / 64; ++vw) a += sh.red[vw * 2 + tid];
-            if (tid == 0) yd_g(c.st)->e_kc = a;
-            else yd_g(c.st)->e_klj = a;
+            e[tid] = a;
+            if (store) {
+                if (tid == 0) yd_g(c.st)->e_kc = a;
+                else yd_g(c.st)->e_klj = a;
+            }
         }
         __syncthreads();
     }
+}
+
+// The whole rebuild of a one-workgroup chain: S(k) over all k, the
+// energies, then mu and the cell buckets, as at the start of a launch.
+template <int KT>
+__device__ void yk_chain_rebuild(const YdChain &c, YkShared &sh, int tid, double *xtra, int xtra_bytes, double *e) {
+    yk_rebuild_sums<KT>(c, sh, tid, 0, c.p.nk, xtra, xtra_bytes);
+    __syncthreads();  // S(k) complete (workgroup-scope visibility of the stores)
+    yk_rebuild_energies(c, sh, tid, true, e);
     yk_chain_mu(c, sh, tid);
     if (c.nc > 0) yk_cells_build(c, &sh.cell_ok);
     __syncthreads();
@@ -2726,7 +2745,7 @@ yk_chain(const YdChain *chains, const double *const *sig6s, const double *const 
             // the chunk tables also take the LDS of the packed positions,
             // which are packed again afterwards (the positions are unchanged)
             yk_chain_rebuild<2>(c, sh, tid, use_lds ? reinterpret_cast<double *>(lds_pos) : nullptr,
-                                use_lds ? 4 * p.natom : 0);  // ends with a barrier
+                                use_lds ? 4 * p.natom : 0, sh.out);  // ends with a barrier
             if (use_lds) {
                 for (int i = tid; i < p.natom; i += (int)blockDim.x) spos[i] = yk_pack_pos(yd_g(c.x)[i], yd_g(c.y)[i], yd_g(c.z)[i]);
                 __syncthreads();
@@ -3180,7 +3199,8 @@ __global__ void yk_gchain_bprep(const YdChain *src, YdChain *dst, const YgGroup 
 
 template <bool F32, int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
-yk_gchain(const YdChain *chains, const YgGroup *groups, int nsteps, int mode, int max_moves, int lds_atoms) {
+yk_gchain(const YdChain *chains, const YgGroup *groups, int nsteps, int mode, int max_moves, int lds_atoms,
+          int resync_every) {
     static_assert(BS == 256, "the grid chain's reductions and proposal take four waves");
     __shared__ YgShared gs;
     extern __shared__ unsigned int lds_pos[];
@@ -3235,6 +3255,32 @@ yk_gchain(const YdChain *chains, const YgGroup *groups, int nsteps, int mode, in
         sh.cell_ok = (int)__hip_atomic_load(part1 + YG_PART - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     for (int mv = 0; ok && mv < total; ++mv) {
+        // The periodic FP64 rebuild between move resync_every k - 1 and
+        // resync_every k (uniform over the chain's workgroups): after a grid
+        // barrier (the last move's S(k) commits and the lead's finalize of
+        // the positions), each workgroup rebuilds S(k) over its own k range
+        // with the one-workgroup rebuild's per-k arithmetic, and the lead
+        // the cell buckets; after a second barrier every workgroup sums the
+        // energies over all k (the same sum in each, stored by the lead) and
+        // recomputes mu, as a new launch would.
+        if (resync_every > 0 && mv > 0 && mv % resync_every == 0) {
+            if (!(ok = yg_sync(bar, G, gs.gen, &gs.sync_ok))) break;
+            yk_rebuild_sums<1>(c, sh, tid, k0, k1, use_lds ? reinterpret_cast<double *>(lds_pos) : nullptr,
+                               use_lds ? 4 * p.natom : 0);
+            __syncthreads();
+            if (use_lds) {  // the chunk tables took the packed positions' LDS
+                for (int i = tid; i < p.natom; i += BS) spos[i] = yk_pack_pos(yd_g(c.x)[i], yd_g(c.y)[i], yd_g(c.z)[i]);
+            }
+            if (c.nc > 0 && lead) yk_cells_build(c, &sh.cell_ok);
+            if (c.nc > 0 && lead && tid == 0) part1[YG_PART - 1] = sh.cell_ok;
+            if (!(ok = yg_sync(bar, G, gs.gen, &gs.sync_ok))) break;
+            if (c.nc > 0 && !lead && tid == 0)
+                sh.cell_ok = (int)__hip_atomic_load(part1 + YG_PART - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            yk_rebuild_energies(c, sh, tid, lead, gs.out2);
+            yk_chain_mu(c, sh, tid);
+            if (tid == 0) { t0.e_kc = gs.out2[0]; t0.e_klj = gs.out2[1]; }
+            __syncthreads();
+        }
         if (tid == 0) {
             if (t0.prepicked) {  // taken during the last move's acceptance
                 yk_sec_pick_as(c, sh.m, t0.nimol);
@@ -3494,7 +3540,7 @@ yk_gchain(const YdChain *chains, const YgGroup *groups, int nsteps, int mode, in
 }
 
 #define YK_GCHAIN_INST(F, B) \
-    template __global__ void yk_gchain<F, B>(const YdChain *, const YgGroup *, int, int, int, int);
+    template __global__ void yk_gchain<F, B>(const YdChain *, const YgGroup *, int, int, int, int, int);
 YK_GCHAIN_INST(false, 256)
 YK_GCHAIN_INST(true, 256)
 
