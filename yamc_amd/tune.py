@@ -105,7 +105,7 @@ def tune_groups(rep: EwaldMC, groups: Sequence[int] = None, cycles: int = 1) -> 
     return best, rates
 
 
-def tune_groups_batch(reps: Sequence[EwaldMC], atoms_per_group: Sequence[int] = (0, 600, 1000, 1500),
+def tune_groups_batch(reps: Sequence[EwaldMC], atoms_per_group: Sequence[int] = (0, 600, 800, 1000, 1200, 1500),
                       gmax: int = 8, cycles: int = 1) -> Tuple[int, Dict[int, float]]:
     """Workgroups per chain inside a batch (a batch of grid chains, one
     cooperative launch): for each A in atoms_per_group, every replica takes
